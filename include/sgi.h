@@ -1,0 +1,271 @@
+/*
+ * sgi.h — C ABI of the MI355X deferred VPL global-illumination pass.
+ *
+ * This library replaces, for one GL frame, the reference renderer's VPL GI
+ * pass (StratusGFX Source/Engine/StratusRendererBackend.cpp:1954-2080):
+ * global VPL sun-visibility culling, per-tile VPL culling, the per-pixel VPL
+ * gather, deferred direct sun lighting with cascaded-shadow PCF, and the
+ * 4-pass a-trous / temporal denoiser.  Every call is stream ordered on the
+ * HIP stream passed in `stream` (NULL = the default stream); every pointer
+ * named "device" is HIP device memory owned by the caller.
+ *
+ * Texture conventions (restating the reference's GL state):
+ *  - 2D planes are row-major [height][width][channels] FP32, row 0 is the
+ *    bottom image row (GL window origin); pixel (x, y) is sampled at
+ *    uv = ((x + 0.5) / W, (y + 0.5) / H).  Every G-buffer / GI texture in the
+ *    reference is CLAMP_TO_EDGE (StratusRendererBackend.cpp:384-589) and is
+ *    only ever sampled at texel centres, so a tap is texel (clamp(x), clamp(y)).
+ *  - Cube map atlases follow GL_TEXTURE_CUBE_MAP_ARRAY: texel (i, j) of face f
+ *    of layer l is at ((l * 6 + f) * res + j) * res + i; faces are
+ *    +X, -X, +Y, -Y, +Z, -Z and (s, t) come from the GL major-axis table.
+ *    Texels hold distance / farPlane (Source/Shaders/shadowVpl.fs:17-35).
+ *  - Matrices are 4x4 column-major (glm / GL uniform layout).
+ *
+ * Errors: every entry point returns SGI_OK (0) or a negative SGI_E_* code and
+ * records a message readable with sgi_last_error() (thread local).  No C++
+ * exception crosses this boundary.  The reference throws std::runtime_error or
+ * returns silently (StratusRendererBackend.cpp:1582,1673,1680,1817); the
+ * silent "no VPLs" returns are kept as SGI_OK with untouched outputs where the
+ * reference leaves its FBOs untouched.
+ */
+#ifndef SGI_H
+#define SGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants shared with the reference shaders (vpl_common.glsl:7-9,
+ * pbr.glsl:70-72, StratusGpuCommon.h:15-36). */
+#define SGI_MAX_TOTAL_VPLS_BEFORE_CULLING 10000
+#define SGI_MAX_TOTAL_SHADOW_ATLASES 14
+#define SGI_MAX_TOTAL_SHADOWS_PER_ATLAS 320
+#define SGI_MAX_TOTAL_VPLS_PER_FRAME (SGI_MAX_TOTAL_SHADOW_ATLASES * SGI_MAX_TOTAL_SHADOWS_PER_ATLAS)
+#define SGI_REFERENCE_MAX_VPLS_PER_TILE 12
+#define SGI_MAX_VPLS_PER_TILE_LIMIT 64
+#define SGI_NUM_CASCADES 4
+
+enum {
+    SGI_OK = 0,
+    SGI_E_INVALID = -1,   /* bad argument (NULL, size out of range, ...) */
+    SGI_E_HIP = -2,       /* a HIP runtime call failed */
+    SGI_E_STATE = -3      /* a required sgi_set_* call is missing */
+};
+
+/* GpuVplData (StratusGpuCommon.h:217-232, vpl_common.glsl:48-56), std430, 64 B. */
+typedef struct sgi_vpl {
+    float position[4];
+    float color[4];
+    float specularPosition[4];
+    float radius;
+    float farPlane;
+    float intensity;
+    float _pad;
+} sgi_vpl;
+
+/* GpuAtlasEntry (StratusGpuCommon.h:276-279, pbr.glsl:75-78), 8 B. */
+typedef struct sgi_atlas_entry {
+    int32_t index; /* atlas (samplerCubeArray) index, < SGI_MAX_TOTAL_SHADOW_ATLASES */
+    int32_t layer; /* cube layer inside the atlas */
+} sgi_atlas_entry;
+
+/* GpuVplStage1PerTileOutputs (StratusGpuCommon.h:192-195), 32 B. */
+typedef struct sgi_tile_stage1 {
+    float averageLocalPosition[4];
+    float averageLocalNormal[4];
+} sgi_tile_stage1;
+
+/* Per-tile VPL list = GpuVplStage2PerTileOutputs (StratusGpuCommon.h:203-209)
+ * with the cap K a run-time value: record t is int32[1 + K] at t * (1 + K):
+ * { numVisible, indices[K] }.  Entries past numVisible are unspecified. */
+
+/* G-buffer planes (StratusRendererBackend.h:327-338), FP32. */
+typedef struct sgi_gbuffer {
+    int32_t width;
+    int32_t height;
+    const float* depth;                    /* [H][W]    window-space depth in [0,1]   */
+    const float* normal;                   /* [H][W][3] normal * 0.5 + 0.5            */
+    const float* albedo;                   /* [H][W][4] rgb albedo, a = emissive.r    */
+    const float* baseReflectivity;         /* [H][W][2] r = reflectance, g = emissive.g */
+    const float* roughnessMetallicAmbient; /* [H][W][3] roughness, metallic, emissive.b */
+    const float* velocity;                 /* [H][W][2] uv motion (cur - prev); NULL = 0 */
+    const float* id;                       /* [H][W]    mesh id; NULL = 0               */
+    const float* ssao;                     /* [H][W]    SSAO; NULL = 1.0                */
+} sgi_gbuffer;
+
+enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1 };
+
+/* VPL distance cube-map atlases (StratusRendererBackend.cpp:2397-2449). */
+typedef struct sgi_cube_atlases {
+    int32_t resolution;       /* face edge in texels */
+    int32_t format;           /* SGI_TEXEL_F32 or SGI_TEXEL_F16 */
+    int32_t layers_per_atlas; /* cube layers in each atlas */
+    int32_t num_atlases;      /* <= SGI_MAX_TOTAL_SHADOW_ATLASES */
+    const void* atlas[SGI_MAX_TOTAL_SHADOW_ATLASES]; /* device [layers][6][res][res] */
+} sgi_cube_atlases;
+
+/* Cascaded sun shadow map (StratusRendererBackend.cpp:341-356, 2452-2474). */
+typedef struct sgi_csm {
+    int32_t resolution;                      /* square cascade edge in texels */
+    const float* depth;                      /* device [4][res][res], depth in [0,1] */
+    float cascadeProjViews[SGI_NUM_CASCADES][16];
+    float cascadePlanes[SGI_NUM_CASCADES - 1][4];
+    float znear;      /* infiniteLightZnear */
+    float zfar;       /* infiniteLightZfar  */
+    float depthBias;  /* infiniteLightDepthBias */
+} sgi_csm;
+
+/* Per-frame uniforms (RendererFrame, StratusRendererBackend.h:297-323, and the
+ * renderer settings StratusRendererBackend.h:186-294). */
+typedef struct sgi_frame {
+    float invProjectionView[16];
+    float prevInvProjectionView[16];
+    float viewPosition[3];
+    float infiniteLightDirection[3]; /* unit vector pointing toward the sun */
+    float infiniteLightColor[3];     /* sun colour * intensity (StratusLight.h:62) */
+    float time;                      /* RNG seed z (vpl_pbr_gi.fs:132) */
+    float framesPerSecond;           /* history cap (vpl_pbr_gi_denoise.fs:422) */
+    int32_t frameCount;
+    float minRoughness;              /* 0.08 */
+    int32_t usePerceptualRoughness;  /* 1 */
+    float minGiOcclusionFactor;      /* 0.95 */
+    float fogDensity;                /* 0 */
+    float fogColor[3];
+    float emissionStrength;          /* 0 */
+} sgi_frame;
+
+enum { SGI_PRECISION_FP32 = 0, SGI_PRECISION_FP16 = 1 };
+
+typedef struct sgi_config {
+    int32_t width;
+    int32_t height;
+    int32_t tile_size;          /* tile edge in pixels: 8, 16 or 32 */
+    int32_t max_vpls_per_tile;  /* K: 1 .. SGI_MAX_VPLS_PER_TILE_LIMIT */
+} sgi_config;
+
+typedef struct sgi_ctx sgi_ctx;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Creates a context for one viewport size (RendererBackend::UpdateWindowDimensions_,
+ * StratusRendererBackend.cpp:436-589): validates the config and allocates the
+ * denoiser's ping-pong workspace on the current device. */
+int sgi_create(const sgi_config* config, sgi_ctx** out);
+void sgi_destroy(sgi_ctx* ctx);
+const char* sgi_last_error(void);
+/* Number of tiles along x / y for the context's config (ceil division). */
+int sgi_tile_counts(const sgi_ctx* ctx, int32_t* tiles_x, int32_t* tiles_y);
+
+/* Per-frame state (the uniform uploads of InitCoreCSMData_ / InitLights_,
+ * StratusRendererBackend.cpp:2452-2604).  The csm depth pointer is kept, not copied. */
+int sgi_set_frame(sgi_ctx* ctx, const sgi_frame* frame);
+int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* csm);
+
+/* ---- kernels ------------------------------------------------------------- */
+
+/* viscull_vpls.cs:47-120 driven by PerformVirtualPointLightCullingStage1_
+ * (StratusRendererBackend.cpp:1578-1661).  For i < n, VPL i is sun-visible when
+ * 1 - calculateInfiniteShadowValue(position, blends, sunDir, false) < 1.
+ * Visible records are compacted into vplUpdatedData in ascending input order
+ * and their input indices into visibleIndices[1..]; visibleIndices[0] receives
+ * min(count, SGI_MAX_TOTAL_VPLS_PER_FRAME).  (The reference compacts with
+ * atomics, so its order is nondeterministic; ascending order is one of the
+ * orders it can produce.)  Needs sgi_set_frame and sgi_set_csm.
+ * vplData: device [n]; vplUpdatedData: device [min(n, 4480)];
+ * visibleIndices: device [1 + min(n, 4480)].  0 <= n <= 10000. */
+int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n,
+                    sgi_vpl* vplUpdatedData, int32_t* visibleIndices, void* stream);
+
+/* vpl_tiled_deferred_culling_stage1.cs:50-90 + _stage2.cs:61-129 (the dispatches
+ * commented out at StratusRendererBackend.cpp:1733-1783).
+ * Stage 1: per tile, the mean world position and the normalised mean decoded
+ * normal of its in-viewport pixels, summed sequentially in row-major local order.
+ * Stage 2: per tile, over slots s = 0 .. count-1 of the visible list, the VPLs
+ * vpls[s] with dot(n, L - P) >= 0 and |L - P| / radius <= 1, kept as the K
+ * smallest ratios, ties in slot order (the reference's strict-< insertion sort).
+ * count = visibleIndices[0].  List entries are slots s when remap == 0, or
+ * visibleIndices[1 + s] (the input VPL index, as the reference stage 2 stores)
+ * when remap != 0.  stage1 (device [tiles]) may be NULL.
+ * tiles: device int32 [tiles_x * tiles_y][1 + K]. */
+int sgi_cull_tiles(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
+                   const int32_t* visibleIndices, int32_t remap,
+                   sgi_tile_stage1* stage1, int32_t* tiles, void* stream);
+
+/* vpl_pbr_gi.fs:81-216 (ComputeVirtualPointLightGlobalIllumination_,
+ * StratusRendererBackend.cpp:1816-1877).  vpls = vplUpdatedData (compacted),
+ * shadowIndices[slot] = that VPL's cube atlas entry, count = visibleIndices[0].
+ * tiles == NULL: the reference's live estimator (hashed-random VPL slots from the
+ * whole visible list, <= 5 rejections).  tiles != NULL: every VPL of the
+ * pixel's tile list is evaluated once, with slot entries (remap == 0).
+ * color: device [H][W][3] = albedo + 1e-5; reservoir: device [H][W][4] =
+ * (min(sum C, 65504), max(sum w, 1)).  precision: SGI_PRECISION_FP32 (the
+ * restatement) or SGI_PRECISION_FP16 (packed FP16 shading, FP32 accumulation). */
+int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
+               const int32_t* visibleIndices, const sgi_atlas_entry* shadowIndices,
+               const sgi_cube_atlases* cubes, const int32_t* tiles, int32_t precision,
+               float* color, float* reservoir, void* stream);
+
+/* pbr.fs:82-145 with INFINITE_LIGHTING_ENABLED (StratusRendererBackend.cpp:2010-2038),
+ * zero point lights: sun lighting (BRDF_Burley) times the 3x3 PCF cascade
+ * shadow, plus emissive * emissionStrength, bounded to 65504.
+ * lit: device [H][W][3].  Needs sgi_set_frame and sgi_set_csm. */
+int sgi_direct_light(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, float* lit, void* stream);
+
+/* vpl_pbr_gi_denoise.fs:142-435 run as the 4-pass loop of
+ * StratusRendererBackend.cpp:1912-1951: pass 0 = reservoir merge (centre only)
+ * and normalisation, passes 1-3 = 5x5 edge-stopping taps at stride 2^pass,
+ * last pass = temporal blend with history and the combine.
+ * screen = direct light [H][W][3]; color / reservoir = sgi_gather outputs;
+ * prev = previous frame's G-buffer (normal, id, depth are read; may equal cur);
+ * prevGi [H][W][3] and prevHistory [H][W] are the previous frame's giOut and
+ * historyOut (NULL = zero history).  Outputs: combined [H][W][3] (the lit
+ * buffer), giOut [H][W][3], reservoirOut [H][W][4], historyOut [H][W].
+ * numVisible = visibleIndices[0] (pass 0's probabilistic weight). */
+int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
+                const int32_t* visibleIndices, const float* screen,
+                const float* color, const float* reservoir,
+                const float* prevGi, const float* prevHistory,
+                float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                void* stream);
+
+/* ---- one frame ------------------------------------------------------------ */
+
+/* Caller-owned buffers of one GI frame, in the order RenderScene uses them. */
+typedef struct sgi_frame_io {
+    const sgi_gbuffer* gbuffer;
+    const sgi_gbuffer* prevGbuffer;        /* NULL = gbuffer */
+    const sgi_vpl* vplData;                /* [numVpls] frustum-culled VPLs */
+    int32_t numVpls;
+    const sgi_atlas_entry* shadowIndices;  /* [numVpls] by input index */
+    const sgi_cube_atlases* cubes;
+    int32_t useTiles;                      /* 0 = reference random gather, 1 = tile lists */
+    int32_t precision;                     /* SGI_PRECISION_* of the gather */
+    /* scratch / outputs */
+    sgi_vpl* vplUpdatedData;               /* [min(numVpls,4480)] */
+    int32_t* visibleIndices;               /* [1 + min(numVpls,4480)] */
+    sgi_atlas_entry* visibleShadowIndices; /* [min(numVpls,4480)] */
+    int32_t* tiles;                        /* [tiles][1 + K] (useTiles) */
+    float* direct;                         /* [H][W][3] */
+    float* color;                          /* [H][W][3] */
+    float* reservoir;                      /* [H][W][4] */
+    const float* prevGi;                   /* history in (NULL = none) */
+    const float* prevHistory;
+    float* combined;                       /* [H][W][3] lit output */
+    float* giOut;                          /* [H][W][3] history out */
+    float* reservoirOut;                   /* [H][W][4] */
+    float* historyOut;                     /* [H][W]    */
+} sgi_frame_io;
+
+/* RenderScene's GI steps (StratusRendererBackend.cpp:1982, 2010-2045) without
+ * host round trips: cull_global -> gather the visible atlas entries ->
+ * direct_light -> (cull_tiles) -> gather -> denoise. */
+int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGI_H */

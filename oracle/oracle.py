@@ -1,0 +1,154 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so, the CPU restatement (FP32 and FP64) of
+StratusGFX's deferred VPL GI shaders.  Only tests/, __graft_entry__.smoke()
+and bench.py's CPU baseline may import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+
+import numpy as np
+
+from stratusgfx_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+GATHER_RANDOM, GATHER_TILES, GATHER_CONVERGED = 0, 1, 2
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        sig = {
+            "orc_hash1": (c_uint32, [c_uint32]),
+            "orc_hash3": (c_uint32, [c_uint32, c_uint32, c_uint32]),
+            "orc_random3": (c_float, [c_float, c_float, c_float]),
+            "orc_ndf_ggx": (c_double, [c_double, c_double]),
+            "orc_ndf_ggx_f": (c_float, [c_float, c_float]),
+            "orc_vis_smith": (c_double, [c_double, c_double, c_double]),
+            "orc_fresnel": (c_double, [c_double, c_double]),
+            "orc_fd_burley": (c_double, [c_double, c_double, c_double, c_double]),
+            "orc_vpl_atten": (c_double, [c_double, c_double]),
+            "orc_brdf_burley": (None, [c_void_p] * 4 + [c_double] * 4 + [c_int, c_void_p]),
+            "orc_vpl_lighting": (None, [c_void_p] * 7 + [c_double] * 5 + [c_int, c_void_p]),
+            "orc_cube_fetch": (c_double, [POINTER(_abi.CubeAtlases), c_int, c_int, c_double, c_double, c_double]),
+            "orc_shadow1": (c_float, [POINTER(_abi.CubeAtlases), c_int, c_int, c_float, c_void_p, c_void_p,
+                                      c_float, c_float]),
+            "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
+            "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
+            "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+            "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
+                                        c_void_p]),
+            "orc_cull_tiles": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_int, c_int, c_void_p, c_void_p,
+                                      c_int, c_int, c_int, c_void_p, c_void_p]),
+            "orc_gather": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_void_p, c_int, c_void_p,
+                                  POINTER(_abi.CubeAtlases), c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                  c_void_p, c_void_p]),
+            "orc_direct_light": (None, [POINTER(_abi.Frame), POINTER(_abi.Csm), POINTER(_abi.Gbuffer), c_int, c_int,
+                                        c_void_p]),
+            "orc_denoise": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), POINTER(_abi.Gbuffer), c_int]
+                            + [c_void_p] * 9),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a) -> int:
+    return 0 if a is None else a.ctypes.data
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def cull_global(frame: _abi.Frame, csm: _abi.Csm, vpls: np.ndarray):
+    """viscull_vpls.cs: returns (vplUpdatedData [count,16], visibleIndices [1+count])."""
+    vpls = _f32(vpls)
+    n = vpls.shape[0]
+    cap = min(n, _abi.MAX_TOTAL_VPLS_PER_FRAME)
+    out = np.zeros((max(cap, 1), 16), np.float32)
+    idx = np.zeros(1 + max(cap, 1), np.int32)
+    count = lib().orc_cull_global(ctypes.byref(frame), ctypes.byref(csm), _p(vpls), n, _p(out), _p(idx))
+    return out[:count], idx[:1 + count]
+
+
+def cull_tiles(frame, gbuf, tile_size, K, vpls, visible, remap=0, t0=0, t1=None, tiles_x=None, tiles_y=None):
+    """Stage 1 + stage 2 for tiles [t0, t1): returns (stage1 [T,8] f32, tiles [T,1+K] i32)."""
+    W, H = gbuf.width, gbuf.height
+    tx = (W + tile_size - 1) // tile_size
+    ty = (H + tile_size - 1) // tile_size
+    T = tx * ty
+    t1 = T if t1 is None else t1
+    vpls = _f32(vpls)
+    visible = np.ascontiguousarray(visible, np.int32)
+    stage1 = np.zeros((T, 8), np.float32)
+    tiles = np.zeros((T, 1 + K), np.int32)
+    lib().orc_cull_tiles(ctypes.byref(frame), ctypes.byref(gbuf), tile_size, K, _p(vpls), _p(visible), remap, t0, t1,
+                         _p(stage1), _p(tiles))
+    return stage1, tiles
+
+
+def gather(frame, gbuf, vpls, num_visible, entries, cubes, tiles=None, tile_size=16, K=12, mode=GATHER_RANDOM,
+           precision=0, y0=0, y1=None):
+    """vpl_pbr_gi.fs over rows [y0, y1): returns (color [rows,W,3], reservoir [rows,W,4])."""
+    W, H = gbuf.width, gbuf.height
+    y1 = H if y1 is None else y1
+    dt = np.float32 if precision == 0 else np.float64
+    color = np.zeros((y1 - y0, W, 3), dt)
+    res = np.zeros((y1 - y0, W, 4), dt)
+    vpls = _f32(vpls)
+    entries = np.ascontiguousarray(entries, np.int32)
+    t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
+    lib().orc_gather(ctypes.byref(frame), ctypes.byref(gbuf), _p(vpls), num_visible, _p(entries), ctypes.byref(cubes),
+                     _p(t), tile_size, K, mode, precision, y0, y1, _p(color), _p(res))
+    return color, res
+
+
+def direct_light(frame, csm, gbuf, y0=0, y1=None):
+    W, H = gbuf.width, gbuf.height
+    y1 = H if y1 is None else y1
+    lit = np.zeros((y1 - y0, W, 3), np.float32)
+    lib().orc_direct_light(ctypes.byref(frame), ctypes.byref(csm), ctypes.byref(gbuf), y0, y1, _p(lit))
+    return lit
+
+
+def denoise(frame, cur, prev, num_visible, screen, color, reservoir, prev_gi=None, prev_history=None):
+    W, H = cur.width, cur.height
+    combined = np.zeros((H, W, 3), np.float32)
+    gi = np.zeros((H, W, 3), np.float32)
+    res = np.zeros((H, W, 4), np.float32)
+    hist = np.zeros((H, W), np.float32)
+    screen, color, reservoir = _f32(screen), _f32(color), _f32(reservoir)
+    pg = None if prev_gi is None else _f32(prev_gi)
+    ph = None if prev_history is None else _f32(prev_history)
+    lib().orc_denoise(ctypes.byref(frame), ctypes.byref(cur), ctypes.byref(prev) if prev is not None else None,
+                      num_visible, _p(screen), _p(color), _p(reservoir), _p(pg), _p(ph), _p(combined), _p(gi),
+                      _p(res), _p(hist))
+    return combined, gi, res, hist
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))

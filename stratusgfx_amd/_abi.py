@@ -1,0 +1,190 @@
+"""ctypes mirror of include/sgi.h and include/sgi_scene.h.
+
+The structures below are byte-for-byte the C structures (natural alignment),
+so a caller can fill them from torch device pointers or numpy host pointers.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from ctypes import POINTER, c_float, c_int32, c_uint64, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(_HERE)
+INCLUDE_DIR = os.path.join(REPO, "include")
+LIB_PATH = os.path.join(_HERE, "libsgi.so")
+HOST_LIB_PATH = os.path.join(_HERE, "libsgi_host.so")
+
+SGI_OK = 0
+SGI_E_INVALID = -1
+SGI_E_HIP = -2
+SGI_E_STATE = -3
+MAX_TOTAL_VPLS_BEFORE_CULLING = 10000
+MAX_TOTAL_SHADOW_ATLASES = 14
+MAX_TOTAL_SHADOWS_PER_ATLAS = 320
+MAX_TOTAL_VPLS_PER_FRAME = MAX_TOTAL_SHADOW_ATLASES * MAX_TOTAL_SHADOWS_PER_ATLAS
+MAX_VPLS_PER_TILE_LIMIT = 64
+NUM_CASCADES = 4
+TEXEL_F32 = 0
+TEXEL_F16 = 1
+PRECISION_FP32 = 0
+PRECISION_FP16 = 1
+SCENE_CORNELL = 0
+SCENE_CITY = 1
+
+VPL_FLOATS = 16  # sgi_vpl is 16 floats (64 B)
+
+
+class Gbuffer(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int32), ("height", c_int32),
+        ("depth", c_void_p), ("normal", c_void_p), ("albedo", c_void_p),
+        ("baseReflectivity", c_void_p), ("roughnessMetallicAmbient", c_void_p),
+        ("velocity", c_void_p), ("id", c_void_p), ("ssao", c_void_p),
+    ]
+
+
+class CubeAtlases(ctypes.Structure):
+    _fields_ = [
+        ("resolution", c_int32), ("format", c_int32), ("layers_per_atlas", c_int32),
+        ("num_atlases", c_int32), ("atlas", c_void_p * MAX_TOTAL_SHADOW_ATLASES),
+    ]
+
+
+class Csm(ctypes.Structure):
+    _fields_ = [
+        ("resolution", c_int32), ("depth", c_void_p),
+        ("cascadeProjViews", (c_float * 16) * NUM_CASCADES),
+        ("cascadePlanes", (c_float * 4) * (NUM_CASCADES - 1)),
+        ("znear", c_float), ("zfar", c_float), ("depthBias", c_float),
+    ]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [
+        ("invProjectionView", c_float * 16), ("prevInvProjectionView", c_float * 16),
+        ("viewPosition", c_float * 3), ("infiniteLightDirection", c_float * 3),
+        ("infiniteLightColor", c_float * 3), ("time", c_float), ("framesPerSecond", c_float),
+        ("frameCount", c_int32), ("minRoughness", c_float), ("usePerceptualRoughness", c_int32),
+        ("minGiOcclusionFactor", c_float), ("fogDensity", c_float), ("fogColor", c_float * 3),
+        ("emissionStrength", c_float),
+    ]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32), ("tile_size", c_int32),
+                ("max_vpls_per_tile", c_int32)]
+
+
+class FrameIO(ctypes.Structure):
+    _fields_ = [
+        ("gbuffer", POINTER(Gbuffer)), ("prevGbuffer", POINTER(Gbuffer)),
+        ("vplData", c_void_p), ("numVpls", c_int32), ("shadowIndices", c_void_p),
+        ("cubes", POINTER(CubeAtlases)), ("useTiles", c_int32), ("precision", c_int32),
+        ("vplUpdatedData", c_void_p), ("visibleIndices", c_void_p), ("visibleShadowIndices", c_void_p),
+        ("tiles", c_void_p), ("direct", c_void_p), ("color", c_void_p), ("reservoir", c_void_p),
+        ("prevGi", c_void_p), ("prevHistory", c_void_p), ("combined", c_void_p), ("giOut", c_void_p),
+        ("reservoirOut", c_void_p), ("historyOut", c_void_p),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("seed", c_uint64), ("num_vpls", c_int32),
+                ("num_city_boxes", c_int32), ("city_half_extent", c_float),
+                ("vpl_color_multiplier", c_float)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32), ("eye", c_float * 3), ("yaw_deg", c_float),
+                ("pitch_deg", c_float), ("fovy_deg", c_float), ("znear", c_float), ("zfar", c_float),
+                ("csm_resolution", c_int32)]
+
+
+_P = c_void_p
+_I = c_int32
+
+# name -> (restype, argtypes) for every entry point of include/sgi.h and include/sgi_scene.h
+SGI_SIGNATURES = {
+    "sgi_create": (c_int32, [POINTER(Config), POINTER(_P)]),
+    "sgi_destroy": (None, [_P]),
+    "sgi_last_error": (ctypes.c_char_p, []),
+    "sgi_tile_counts": (c_int32, [_P, POINTER(c_int32), POINTER(c_int32)]),
+    "sgi_set_frame": (c_int32, [_P, POINTER(Frame)]),
+    "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
+    "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
+    "sgi_cull_tiles": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _I, _P, _P, _P]),
+    "sgi_gather": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, _I, _P, _P, _P]),
+    "sgi_direct_light": (c_int32, [_P, POINTER(Gbuffer), _P, _P]),
+    "sgi_denoise": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                              _P]),
+    "sgi_render_frame": (c_int32, [_P, POINTER(FrameIO), _P]),
+}
+SCENE_SIGNATURES = {
+    "sgi_scene_create": (c_int32, [POINTER(SceneDesc), POINTER(_P)]),
+    "sgi_scene_destroy": (None, [_P]),
+    "sgi_scene_num_boxes": (c_int32, [_P]),
+    "sgi_scene_default_view": (c_int32, [_P, _I, _I, POINTER(View)]),
+    "sgi_scene_set_view": (c_int32, [_P, POINTER(View), POINTER(View), POINTER(Frame), POINTER(Csm)]),
+    "sgi_scene_vpls": (c_int32, [_P, _I, _P, _P]),
+    "sgi_scene_gbuffer_host": (c_int32, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "sgi_scene_gbuffer_device": (c_int32, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sgi_scene_cubes_host": (c_int32, [_P, POINTER(CubeAtlases), _I, _I]),
+    "sgi_scene_cubes_device": (c_int32, [_P, POINTER(CubeAtlases), _I, _I, _P]),
+    "sgi_scene_csm_host": (c_int32, [_P, _P]),
+    "sgi_scene_csm_device": (c_int32, [_P, _P, _P]),
+}
+HOST_LIB_SYMBOLS = [n for n in SCENE_SIGNATURES if not n.endswith("_device")] + ["sgi_last_error"]
+
+
+def header_functions(path: str) -> list[str]:
+    """Names of the functions a public header declares (`int name(` / `void name(` / `const char* name(`)."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:int|void|const char\*)\s+(sgi_\w+)\s*\(", text, flags=re.M)
+
+
+class SgiError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with {code}: {msg}")
+        self.code = code
+
+
+_libs: dict[str, ctypes.CDLL] = {}
+
+
+def _bind(lib: ctypes.CDLL, sigs: dict) -> None:
+    for name, (res, args) in sigs.items():
+        if not hasattr(lib, name):
+            continue
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads the HIP library.  Raises if it has not been built: there is no fallback."""
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    _bind(lib, SGI_SIGNATURES)
+    _bind(lib, SCENE_SIGNATURES)
+    _libs[path] = lib
+    return lib
+
+
+def load_host() -> ctypes.CDLL:
+    """Loads the host-only scene generator library (no HIP runtime needed)."""
+    return load(HOST_LIB_PATH)
+
+
+def check(lib: ctypes.CDLL, fn: str, code: int) -> None:
+    if code != SGI_OK:
+        msg = lib.sgi_last_error()
+        raise SgiError(fn, code, msg.decode() if msg else "")
+
+
+def call(lib: ctypes.CDLL, fn: str, *args) -> None:
+    check(lib, fn, getattr(lib, fn)(*args))

@@ -1,0 +1,249 @@
+// C ABI of the GI pass: context lifetime, argument validation, and the
+// per-frame orchestration that mirrors RendererBackend's GI steps
+// (StratusRendererBackend.cpp:1982, 2010-2045).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <new>
+#include <string>
+
+#include "../../include/sgi.h"
+#include "launch.h"
+
+void sgi_set_error(const std::string& msg);
+
+struct sgi_ctx {
+    sgi_config cfg;
+    int tilesX = 0, tilesY = 0;
+    bool hasFrame = false, hasCsm = false;
+    sgi_frame frame{};
+    sgi_dev::FrameArgs frameArgs{};
+    sgi_dev::CsmArgs csmArgs{};
+    sgi_launch::DenoiseBuffers ws{};
+    int device = 0;
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+    sgi_set_error(msg);
+    return code;
+}
+
+int hipFail(hipError_t e, const char* what) {
+    return fail(SGI_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int checkGbuffer(const sgi_ctx* c, const sgi_gbuffer* g, const char* who, bool needMaterial) {
+    if (!g) return fail(SGI_E_INVALID, std::string(who) + ": NULL gbuffer");
+    if (g->width != c->cfg.width || g->height != c->cfg.height)
+        return fail(SGI_E_INVALID, std::string(who) + ": gbuffer size differs from the context's viewport");
+    if (!g->depth || !g->normal) return fail(SGI_E_INVALID, std::string(who) + ": gbuffer depth / normal is NULL");
+    if (needMaterial && (!g->albedo || !g->baseReflectivity || !g->roughnessMetallicAmbient))
+        return fail(SGI_E_INVALID, std::string(who) + ": gbuffer albedo / reflectivity / roughness plane is NULL");
+    return SGI_OK;
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int sgi_create(const sgi_config* config, sgi_ctx** out) {
+    if (!config || !out) return fail(SGI_E_INVALID, "sgi_create: NULL argument");
+    *out = nullptr;
+    const sgi_config& c = *config;
+    if (c.width <= 0 || c.height <= 0 || c.width > 16384 || c.height > 16384)
+        return fail(SGI_E_INVALID, "sgi_create: viewport must be within 1..16384 on each side");
+    if (c.tile_size != 8 && c.tile_size != 16 && c.tile_size != 32)
+        return fail(SGI_E_INVALID, "sgi_create: tile_size must be 8, 16 or 32");
+    if (c.max_vpls_per_tile < 1 || c.max_vpls_per_tile > SGI_MAX_VPLS_PER_TILE_LIMIT)
+        return fail(SGI_E_INVALID, "sgi_create: max_vpls_per_tile must be within 1..64");
+    sgi_ctx* ctx = new (std::nothrow) sgi_ctx();
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_create: out of host memory");
+    ctx->cfg = c;
+    ctx->tilesX = (c.width + c.tile_size - 1) / c.tile_size;
+    ctx->tilesY = (c.height + c.tile_size - 1) / c.tile_size;
+    hipError_t e = hipGetDevice(&ctx->device);
+    const size_t plane = (size_t)c.width * c.height * sizeof(float4);
+    if (e == hipSuccess) e = hipMalloc(&ctx->ws.resA, plane);
+    if (e == hipSuccess) e = hipMalloc(&ctx->ws.resB, plane);
+    if (e == hipSuccess) e = hipMalloc(&ctx->ws.normals, plane);
+    if (e != hipSuccess) {
+        sgi_destroy(ctx);
+        return hipFail(e, "sgi_create: workspace allocation");
+    }
+    *out = ctx;
+    return SGI_OK;
+}
+
+void sgi_destroy(sgi_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->ws.resA) (void)hipFree(ctx->ws.resA);
+    if (ctx->ws.resB) (void)hipFree(ctx->ws.resB);
+    if (ctx->ws.normals) (void)hipFree(ctx->ws.normals);
+    delete ctx;
+}
+
+int sgi_tile_counts(const sgi_ctx* ctx, int32_t* tx, int32_t* ty) {
+    if (!ctx || !tx || !ty) return fail(SGI_E_INVALID, "sgi_tile_counts: NULL argument");
+    *tx = ctx->tilesX;
+    *ty = ctx->tilesY;
+    return SGI_OK;
+}
+
+int sgi_set_frame(sgi_ctx* ctx, const sgi_frame* f) {
+    if (!ctx || !f) return fail(SGI_E_INVALID, "sgi_set_frame: NULL argument");
+    if (!(f->framesPerSecond > 0.0f)) return fail(SGI_E_INVALID, "sgi_set_frame: framesPerSecond must be > 0");
+    ctx->frame = *f;
+    ctx->frameArgs = sgi_dev::frameArgs(*f);
+    ctx->hasFrame = true;
+    return SGI_OK;
+}
+
+int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* c) {
+    if (!ctx || !c) return fail(SGI_E_INVALID, "sgi_set_csm: NULL argument");
+    if (c->resolution <= 0 || c->resolution > 16384 || !c->depth)
+        return fail(SGI_E_INVALID, "sgi_set_csm: resolution must be within 1..16384 and depth non-NULL");
+    sgi_dev::CsmArgs& a = ctx->csmArgs;
+    a.depth = c->depth;
+    a.res = c->resolution;
+    for (int i = 0; i < SGI_NUM_CASCADES; ++i)
+        for (int k = 0; k < 16; ++k) a.projViews[i][k] = c->cascadeProjViews[i][k];
+    for (int i = 0; i < SGI_NUM_CASCADES - 1; ++i)
+        for (int k = 0; k < 4; ++k) a.planes[i][k] = c->cascadePlanes[i][k];
+    // pbr.glsl:116: infiniteLightDepthBias / (infiniteLightZfar - infiniteLightZnear)
+    a.bias = c->depthBias / (c->zfar - c->znear);
+    ctx->hasCsm = true;
+    return SGI_OK;
+}
+
+int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n, sgi_vpl* vplUpdatedData, int32_t* visibleIndices,
+                    void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_cull_global: NULL context");
+    if (!ctx->hasFrame || !ctx->hasCsm) return fail(SGI_E_STATE, "sgi_cull_global: sgi_set_frame and sgi_set_csm first");
+    if (n < 0 || n > SGI_MAX_TOTAL_VPLS_BEFORE_CULLING)
+        return fail(SGI_E_INVALID, "sgi_cull_global: n must be within 0..10000");
+    if (!visibleIndices || (n > 0 && (!vplData || !vplUpdatedData)))
+        return fail(SGI_E_INVALID, "sgi_cull_global: NULL buffer");
+    const hipError_t e = sgi_launch::cullGlobal(ctx->csmArgs, ctx->frameArgs, vplData, n, vplUpdatedData,
+                                                visibleIndices, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_global");
+}
+
+int sgi_cull_tiles(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
+                   int32_t remap, sgi_tile_stage1* stage1, int32_t* tiles, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_cull_tiles: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_cull_tiles: sgi_set_frame first");
+    int r = checkGbuffer(ctx, g, "sgi_cull_tiles", false);
+    if (r != SGI_OK) return r;
+    if (!vpls || !visibleIndices || !tiles) return fail(SGI_E_INVALID, "sgi_cull_tiles: NULL buffer");
+    const hipError_t e = sgi_launch::cullTiles(sgi_dev::gbufArgs(*g), ctx->frameArgs, ctx->cfg.tile_size, ctx->tilesX,
+                                               ctx->tilesY, ctx->cfg.max_vpls_per_tile, vpls, visibleIndices, remap,
+                                               stage1, tiles, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_tiles");
+}
+
+int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
+               const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles,
+               int32_t precision, float* color, float* reservoir, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_gather: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_gather: sgi_set_frame first");
+    int r = checkGbuffer(ctx, g, "sgi_gather", true);
+    if (r != SGI_OK) return r;
+    if (!vpls || !visibleIndices || !shadowIndices || !cubes || !color || !reservoir)
+        return fail(SGI_E_INVALID, "sgi_gather: NULL buffer");
+    if (!aligned16(reservoir)) return fail(SGI_E_INVALID, "sgi_gather: reservoir must be 16-byte aligned");
+    if (precision != SGI_PRECISION_FP32 && precision != SGI_PRECISION_FP16)
+        return fail(SGI_E_INVALID, "sgi_gather: unknown precision");
+    if (cubes->resolution <= 0 || cubes->resolution > 4096 ||
+        (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16) || cubes->layers_per_atlas <= 0 ||
+        cubes->layers_per_atlas > SGI_MAX_TOTAL_SHADOWS_PER_ATLAS || cubes->num_atlases <= 0 ||
+        cubes->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES)
+        return fail(SGI_E_INVALID, "sgi_gather: bad cube atlas description");
+    sgi_dev::CubeArgs ca{};
+    for (int a = 0; a < cubes->num_atlases; ++a) {
+        if (!cubes->atlas[a]) return fail(SGI_E_INVALID, "sgi_gather: NULL cube atlas");
+        ca.atlas[a] = cubes->atlas[a];
+    }
+    for (int a = cubes->num_atlases; a < SGI_MAX_TOTAL_SHADOW_ATLASES; ++a) ca.atlas[a] = cubes->atlas[0];
+    ca.numAtlases = cubes->num_atlases;
+    ca.res = cubes->resolution;
+    ca.layers = cubes->layers_per_atlas;
+    ca.layerStride = 6u * (uint32_t)cubes->resolution * (uint32_t)cubes->resolution;
+    const hipError_t e = sgi_launch::gather(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format,
+                                            ctx->cfg.tile_size, ctx->tilesX, ctx->tilesY, ctx->cfg.max_vpls_per_tile,
+                                            vpls, visibleIndices, shadowIndices, tiles, precision, color, reservoir,
+                                            (hipStream_t)stream);
+    if (e == hipErrorNotSupported) return fail(SGI_E_INVALID, "sgi_gather: this precision is not built");
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather");
+}
+
+int sgi_direct_light(sgi_ctx* ctx, const sgi_gbuffer* g, float* lit, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_direct_light: NULL context");
+    if (!ctx->hasFrame || !ctx->hasCsm) return fail(SGI_E_STATE, "sgi_direct_light: sgi_set_frame and sgi_set_csm first");
+    int r = checkGbuffer(ctx, g, "sgi_direct_light", true);
+    if (r != SGI_OK) return r;
+    if (!lit) return fail(SGI_E_INVALID, "sgi_direct_light: NULL output");
+    const hipError_t e =
+        sgi_launch::directLight(sgi_dev::gbufArgs(*g), ctx->frameArgs, ctx->csmArgs, lit, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_direct_light");
+}
+
+int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev, const int32_t* visibleIndices,
+                const float* screen, const float* color, const float* reservoir, const float* prevGi,
+                const float* prevHistory, float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_denoise: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_denoise: sgi_set_frame first");
+    int r = checkGbuffer(ctx, cur, "sgi_denoise", false);
+    if (r != SGI_OK) return r;
+    if (prev && (r = checkGbuffer(ctx, prev, "sgi_denoise(prev)", false)) != SGI_OK) return r;
+    if (!visibleIndices || !screen || !color || !reservoir || !combined || !giOut || !reservoirOut || !historyOut)
+        return fail(SGI_E_INVALID, "sgi_denoise: NULL buffer");
+    if (!aligned16(reservoir) || !aligned16(reservoirOut))
+        return fail(SGI_E_INVALID, "sgi_denoise: reservoirs must be 16-byte aligned");
+    const sgi_dev::GbufArgs c = sgi_dev::gbufArgs(*cur);
+    const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
+    const hipError_t e = sgi_launch::denoise(c, p, ctx->frameArgs, visibleIndices, screen, color, reservoir, prevGi,
+                                             prevHistory, combined, giOut, reservoirOut, historyOut, ctx->ws,
+                                             (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise");
+}
+
+// RenderScene's GI steps without the reference's host readbacks: the visible
+// list stays on the device, and the per-slot atlas entries are gathered on the
+// device instead of GetOrAllocateShadowMapForLight_ on the host.
+int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream) {
+    if (!ctx || !io) return fail(SGI_E_INVALID, "sgi_render_frame: NULL argument");
+    if (io->numVpls < 0 || io->numVpls > SGI_MAX_TOTAL_VPLS_BEFORE_CULLING)
+        return fail(SGI_E_INVALID, "sgi_render_frame: numVpls must be within 0..10000");
+    if (!io->shadowIndices || !io->visibleShadowIndices)
+        return fail(SGI_E_INVALID, "sgi_render_frame: NULL atlas entry buffer");
+    if (io->useTiles && !io->tiles) return fail(SGI_E_INVALID, "sgi_render_frame: tile lists requested, tiles NULL");
+    const hipStream_t s = (hipStream_t)stream;
+    // UpdatePointLights_ -> PerformVirtualPointLightCullingStage1_
+    int r = sgi_cull_global(ctx, io->vplData, io->numVpls, io->vplUpdatedData, io->visibleIndices, stream);
+    if (r != SGI_OK) return r;
+    // PerformVirtualPointLightCullingStage2_: atlas entries of the visible slots
+    const int maxCount = io->numVpls < SGI_MAX_TOTAL_VPLS_PER_FRAME ? io->numVpls : SGI_MAX_TOTAL_VPLS_PER_FRAME;
+    hipError_t e = sgi_launch::gatherVisibleEntries(io->shadowIndices, io->visibleIndices, maxCount,
+                                                    io->visibleShadowIndices, s);
+    if (e != hipSuccess) return hipFail(e, "sgi_render_frame: atlas entries");
+    // deferred lighting pass (pbr.fs)
+    if ((r = sgi_direct_light(ctx, io->gbuffer, io->direct, stream)) != SGI_OK) return r;
+    // tile culling (the commented-out dispatches, StratusRendererBackend.cpp:1733-1783)
+    if (io->useTiles &&
+        (r = sgi_cull_tiles(ctx, io->gbuffer, io->vplUpdatedData, io->visibleIndices, 0, nullptr, io->tiles, stream)) !=
+            SGI_OK)
+        return r;
+    // ComputeVirtualPointLightGlobalIllumination_
+    if ((r = sgi_gather(ctx, io->gbuffer, io->vplUpdatedData, io->visibleIndices, io->visibleShadowIndices, io->cubes,
+                        io->useTiles ? io->tiles : nullptr, io->precision, io->color, io->reservoir, stream)) != SGI_OK)
+        return r;
+    return sgi_denoise(ctx, io->gbuffer, io->prevGbuffer, io->visibleIndices, io->direct, io->color, io->reservoir,
+                       io->prevGi, io->prevHistory, io->combined, io->giOut, io->reservoirOut, io->historyOut, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// VPL culling kernels.
+//
+// cullGlobalKernel — viscull_vpls.cs:47-120: per VPL, the sun-visibility test
+// through the cascades, then a stable compaction (wave ballot + block prefix)
+// of the visible records in ascending input order.
+//
+// cullTilesKernel — vpl_tiled_deferred_culling_stage1.cs:50-90 and
+// _stage2.cs:61-129 fused: one wave per tile.  Stage 1 reconstructs the
+// tile's pixels in parallel into LDS and then sums each of the six components
+// serially in row-major local order, which is the reference's summation order
+// (thread 0's loop), so the tile mean is bit-identical.  Stage 2 evaluates 64
+// VPLs per step across the lanes and keeps the sorted top-K list in lane
+// registers (lane k holds entry k); candidates below the current K-th ratio
+// are inserted one at a time in slot order with a ballot for the position and
+// a lane shift, which reproduces the reference's strict-< insertion sort.
+#include "launch.h"
+
+namespace {
+
+using namespace sgi_dev;
+
+constexpr int kCullThreads = 1024;
+
+__global__ __launch_bounds__(kCullThreads) void cullGlobalKernel(CsmArgs csm, f3 sun, const sgi_vpl* __restrict__ in,
+                                                                 int n, sgi_vpl* __restrict__ out,
+                                                                 int32_t* __restrict__ visible) {
+    __shared__ int waveCount[kCullThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int running = 0;
+    for (int c0 = 0; c0 < n; c0 += kCullThreads) {
+        const int i = c0 + threadIdx.x;
+        bool vis = false;
+        if (i < n) {
+            const f3 p = ld3(in[i].position);
+            const f3 blends = cascadeBlends(csm, p);
+            const float shadowFactor = 1.0f - infiniteShadowValue(csm, sun, p, blends, sun, 0.0f);
+            vis = shadowFactor < 1.0f;
+        }
+        const uint64_t m = __ballot(vis);
+        const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+        const int lanePrefix = __popcll(below);
+        if (lane == 0) waveCount[wave] = __popcll(m);
+        __syncthreads();
+        int wavePrefix = 0, total = 0;
+        for (int w = 0; w < kCullThreads / 64; ++w) {
+            const int c = waveCount[w];
+            if (w < wave) wavePrefix += c;
+            total += c;
+        }
+        const int pos = running + wavePrefix + lanePrefix;
+        if (vis && pos < SGI_MAX_TOTAL_VPLS_PER_FRAME) {
+            out[pos] = in[i];
+            visible[1 + pos] = i;
+        }
+        running += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) visible[0] = running < SGI_MAX_TOTAL_VPLS_PER_FRAME ? running : SGI_MAX_TOTAL_VPLS_PER_FRAME;
+}
+
+__global__ void gatherEntriesKernel(const sgi_atlas_entry* __restrict__ all, const int32_t* __restrict__ visible,
+                                    int maxCount, sgi_atlas_entry* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int count = visible[0];
+    if (s < count && s < maxCount) out[s] = all[visible[1 + s]];
+}
+
+template <int TS>
+__global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, int tilesX, int nTiles, int K,
+                                                      const sgi_vpl* __restrict__ vpls,
+                                                      const int32_t* __restrict__ visible, int remap,
+                                                      sgi_tile_stage1* __restrict__ stage1,
+                                                      int32_t* __restrict__ tiles) {
+    constexpr int NPX = TS * TS;
+    __shared__ float comp[6][NPX];
+    __shared__ float sums[6];
+    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
+    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+    const int lane = threadIdx.x;
+    const int x0 = tx * TS, y0 = ty * TS;
+    const int vw = min(TS, g.W - x0), vh = min(TS, g.H - y0);
+
+    // Stage 1: per-pixel world position and decoded normal.
+    for (int l = lane; l < NPX; l += 64) {
+        const int lx = l % TS, ly = l / TS;
+        if (lx < vw && ly < vh) {
+            const int x = x0 + lx, y = y0 + ly;
+            const size_t p = (size_t)y * g.W + x;
+            const f3 fp = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV);
+            const f3 nn = decodeNormal(g.normal + 3 * p);
+            comp[0][l] = fp.x; comp[1][l] = fp.y; comp[2][l] = fp.z;
+            comp[3][l] = nn.x; comp[4][l] = nn.y; comp[5][l] = nn.z;
+        }
+    }
+    __syncthreads();
+    if (lane < 6) {  // the reference's serial loop, one component per lane
+        float s = 0.0f;
+        for (int ly = 0; ly < vh; ++ly)
+            for (int lx = 0; lx < vw; ++lx) s += comp[lane][ly * TS + lx];
+        sums[lane] = s;
+    }
+    __syncthreads();
+    const float cnt = (float)(vw * vh);
+    const f3 P = f3{sums[0], sums[1], sums[2]} / cnt;
+    const f3 Nrm = normalize(f3{sums[3], sums[4], sums[5]} / cnt);
+    if (stage1 && lane == 0) {
+        sgi_tile_stage1 o;
+        o.averageLocalPosition[0] = P.x; o.averageLocalPosition[1] = P.y; o.averageLocalPosition[2] = P.z;
+        o.averageLocalPosition[3] = 1.0f;
+        o.averageLocalNormal[0] = Nrm.x; o.averageLocalNormal[1] = Nrm.y; o.averageLocalNormal[2] = Nrm.z;
+        o.averageLocalNormal[3] = 1.0f;
+        stage1[t] = o;
+    }
+
+    // Stage 2: sorted top-K in lane registers.
+    float myDist = kFLOAT_MAX;
+    int myIdx = 0;
+    int count = 0;
+    const int numVisible = visible[0];
+    for (int b = 0; b < numVisible; b += 64) {
+        const int s = b + lane;
+        bool cand = false;
+        float ratio = 0.0f;
+        if (s < numVisible) {
+            const f3 lmf = ld3(vpls[s].position) - P;
+            const float side = dot(Nrm, lmf);
+            if (!(side < 0.0f)) {
+                const float distance = length(lmf);
+                ratio = distance / vpls[s].radius;
+                cand = !(ratio > 1.0f) && ratio == ratio;
+            }
+        }
+        const float thr = __shfl(myDist, K - 1);
+        uint64_t m = __ballot(cand && ratio < thr);
+        while (m) {
+            const int c = __builtin_ctzll(m);
+            m &= m - 1;
+            const float r = __shfl(ratio, c);
+            const float thrNow = __shfl(myDist, K - 1);
+            if (!(r < thrNow)) continue;
+            const int p = __popcll(__ballot(lane < K && myDist <= r));
+            const float upD = __shfl_up(myDist, 1);
+            const int upI = __shfl_up(myIdx, 1);
+            if (lane < K) {
+                if (lane > p) {
+                    myDist = upD;
+                    myIdx = upI;
+                } else if (lane == p) {
+                    myDist = r;
+                    myIdx = remap ? visible[1 + b + c] : b + c;
+                }
+            }
+            count = count + 1 < K ? count + 1 : K;
+        }
+    }
+    int32_t* rec = tiles + (size_t)t * (1 + K);
+    if (lane == 0) rec[0] = count;
+    if (lane < count) rec[1 + lane] = myIdx;
+}
+
+}  // namespace
+
+namespace sgi_launch {
+
+hipError_t cullGlobal(const CsmArgs& csm, const FrameArgs& fr, const sgi_vpl* in, int n, sgi_vpl* out,
+                      int32_t* visible, hipStream_t s) {
+    hipLaunchKernelGGL(cullGlobalKernel, dim3(1), dim3(kCullThreads), 0, s, csm, ld3(fr.sunDir), in, n, out, visible);
+    return hipGetLastError();
+}
+
+hipError_t gatherVisibleEntries(const sgi_atlas_entry* all, const int32_t* visible, int maxCount, sgi_atlas_entry* out,
+                                hipStream_t s) {
+    if (maxCount <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gatherEntriesKernel, dim3((maxCount + 255) / 256), dim3(256), 0, s, all, visible, maxCount, out);
+    return hipGetLastError();
+}
+
+hipError_t cullTiles(const GbufArgs& g, const FrameArgs& fr, int tileSize, int tilesX, int tilesY, int K,
+                     const sgi_vpl* vpls, const int32_t* visible, int remap, sgi_tile_stage1* stage1, int32_t* tiles,
+                     hipStream_t s) {
+    const int nTiles = tilesX * tilesY;
+    switch (tileSize) {
+        case 8:
+            hipLaunchKernelGGL(cullTilesKernel<8>, dim3(nTiles), dim3(64), 0, s, g, fr, tilesX, nTiles, K, vpls,
+                               visible, remap, stage1, tiles);
+            break;
+        case 16:
+            hipLaunchKernelGGL(cullTilesKernel<16>, dim3(nTiles), dim3(64), 0, s, g, fr, tilesX, nTiles, K, vpls,
+                               visible, remap, stage1, tiles);
+            break;
+        case 32:
+            hipLaunchKernelGGL(cullTilesKernel<32>, dim3(nTiles), dim3(64), 0, s, g, fr, tilesX, nTiles, K, vpls,
+                               visible, remap, stage1, tiles);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sgi_launch

@@ -1,0 +1,179 @@
+// A-trous spatiotemporal denoiser — vpl_pbr_gi_denoise.fs:142-435 run as the
+// reference's 4-pass loop (StratusRendererBackend.cpp:1912-1951).
+//
+// Only the reservoir changes between passes (the colour / GI attachment passes
+// through until the final pass), so the passes exchange one float4 plane:
+//   pass 0  denoiseMergeKernel: centre-only reservoir merge (computeMergedReservoir
+//           with minmaxNearest = 0) and normalisation; it also stores the decoded
+//           normals once, since every later tap re-reads them;
+//   pass 1,2 denoiseFilterKernel<false>: 5x5 taps at stride 2^pass with weight
+//           pow(max(0, n.n'), 128) * exp(-|z - z'|) (centre weight 1);
+//   pass 3  denoiseFilterKernel<true>: the same filter, then the temporal blend
+//           with the reprojected history and the combine screen + albedo * gi.
+#include "launch.h"
+
+namespace {
+
+using namespace sgi_dev;
+
+__device__ __forceinline__ int clampx(int x, int W) { return x < 0 ? 0 : (x >= W ? W - 1 : x); }
+
+// GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv.
+__device__ __forceinline__ float bilinear(const float* t, int W, int H, int C, float u, float v, int c) {
+    const float fu = u * (float)W - 0.5f, fv = v * (float)H - 0.5f;
+    const float i0f = floorf(fu), j0f = floorf(fv);
+    const float a = fu - i0f, b = fv - j0f;
+    const int ia = texelIndex(i0f, W), ib = texelIndex(i0f + 1.0f, W);
+    const int ja = texelIndex(j0f, H), jb = texelIndex(j0f + 1.0f, H);
+    const float t00 = t[((size_t)ja * W + ia) * C + c], t10 = t[((size_t)ja * W + ib) * C + c];
+    const float t01 = t[((size_t)jb * W + ia) * C + c], t11 = t[((size_t)jb * W + ib) * C + c];
+    return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 + a * b * t11;
+}
+
+__device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C, float u, float v, int c) {
+    const int i = texelIndex(floorf(u * (float)W), W), j = texelIndex(floorf(v * (float)H), H);
+    return t[((size_t)j * W + i) * C + c];
+}
+
+__global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
+                                                          const float4* __restrict__ resIn, float4* __restrict__ resOut,
+                                                          float4* __restrict__ normals) {
+    const int x = blockIdx.x * 64 + (int)threadIdx.x % 64, y = blockIdx.y * 4 + (int)threadIdx.x / 64;
+    if (x >= g.W || y >= g.H) return;
+    const size_t p = (size_t)y * g.W + x;
+    const f3 centerNormal = decodeNormal(g.normal + 3 * p);
+    normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, 0.0f);
+    const float centerDepth = g.depth[p];
+    const int numVisible = visible[0];
+    float cr0 = 0.0f, cr1 = 0.0f, cr2 = 0.0f, cr3 = 0.0f;
+    const float depthCutoff = 0.1f * centerDepth;
+    float runningSum = 0.0f;
+    const float probabilisticWeight = 1.0f / (float)numVisible;
+    bool skip = dot(centerNormal, centerNormal) < 0.906f;  // the centre tap's normal test
+    if (!skip) skip = fabsf(centerDepth - centerDepth) > depthCutoff;
+    if (!skip) {
+        const float4 rr = resIn[p];
+        const float randUpdate = random3((float)x + 0.5f, (float)y + 0.5f, fr.time);
+        const float probability = rr.w * probabilisticWeight;
+        cr3 += probability;
+        if (randUpdate < probability / cr3) {
+            cr0 += rr.x;
+            cr1 += rr.y;
+            cr2 += rr.z;
+            runningSum += rr.w;
+        }
+    }
+    cr3 = runningSum;
+    const float den = gmax(kEPS, cr3);
+    resOut[p] = make_float4(cr0 / den, cr1 / den, cr2 / den, 1.0f);
+}
+
+template <bool FINAL>
+__global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, GbufArgs prev, FrameArgs fr, int stride,
+                                                           const float4* __restrict__ resIn,
+                                                           const float4* __restrict__ normals,
+                                                           float4* __restrict__ resOut, const float* __restrict__ screen,
+                                                           const float* __restrict__ color,
+                                                           const float* __restrict__ prevGi,
+                                                           const float* __restrict__ prevHistory,
+                                                           float* __restrict__ combined, float* __restrict__ giOut,
+                                                           float* __restrict__ historyOut) {
+    const int x = blockIdx.x * 64 + (int)threadIdx.x % 64, y = blockIdx.y * 4 + (int)threadIdx.x / 64;
+    if (x >= g.W || y >= g.H) return;
+    const int W = g.W, H = g.H;
+    const size_t p = (size_t)y * W + x;
+    const float4 cn4 = normals[p];
+    const f3 centerNormal = f3{cn4.x, cn4.y, cn4.z};
+    const float centerDepth = g.depth[p];
+    float rf0 = 0.0f, rf1 = 0.0f, rf2 = 0.0f, rf3 = 0.0f;
+    for (int dx = -2; dx <= 2; ++dx) {
+        const int tx = clampx(x + dx * stride, W);
+        for (int dy = -2; dy <= 2; ++dy) {
+            const int ty = clampx(y + dy * stride, H);
+            const size_t q = (size_t)ty * W + tx;
+            const float4 rr = resIn[q];
+            float filtered;
+            if (dx == 0 && dy == 0) {
+                filtered = 1.0f;
+            } else {  // filterInput, vpl_pbr_gi_denoise.fs:142-190
+                const float currDepth = g.depth[q];
+                const float wz = expf(-fabsf(centerDepth - currDepth)) / 1.0f;
+                const float4 n4 = normals[q];
+                float wn = gmax(0.0f, dot(centerNormal, f3{n4.x, n4.y, n4.z}));
+                wn = pow128(wn);
+                filtered = wn * wz * 1.0f;
+            }
+            rf0 += filtered * rr.x;
+            rf1 += filtered * rr.y;
+            rf2 += filtered * rr.z;
+            rf3 += filtered * rr.w;
+        }
+    }
+    const float den = gmax(kEPS, rf3);
+    const f3 shadowFactor = f3{rf0 / den, rf1 / den, rf2 / den};
+    if (!FINAL) {
+        resOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
+        return;
+    }
+    const f3 gi = ld3(color + 3 * p);
+    float historyAccum = prevHistory ? prevHistory[p] : 0.0f;
+    float accumMultiplier = 1.0f;
+    const f3 currGi = shadowFactor;
+    const float u = texelCentre(x, W), v = texelCentre(y, H);
+    const float vx = g.velocity ? g.velocity[2 * p] : 0.0f, vy = g.velocity ? g.velocity[2 * p + 1] : 0.0f;
+    const float pu = u - vx, pv = v - vy;
+    const float prevCenterDepth = bilinear(prev.depth, W, H, 1, pu, pv, 0);
+    const f3 prevCenterNormal = normalize(f3{bilinear(prev.normal, W, H, 3, pu, pv, 0),
+                                             bilinear(prev.normal, W, H, 3, pu, pv, 1),
+                                             bilinear(prev.normal, W, H, 3, pu, pv, 2)} * 2.0f - splat(1.0f));
+    f3 prevGiV = splat(0.0f);
+    if (prevGi)
+        prevGiV = f3{bilinear(prevGi, W, H, 3, pu, pv, 0), bilinear(prevGi, W, H, 3, pu, pv, 1),
+                     bilinear(prevGi, W, H, 3, pu, pv, 2)};
+    const float currId = g.id ? g.id[p] : 0.0f;
+    const float prevId = prev.id ? nearestTap(prev.id, W, H, 1, pu, pv, 0) : 0.0f;
+    const float wn = gmax(0.0f, dot(centerNormal, prevCenterNormal));
+    const float wz = expf(-fabsf(centerDepth - prevCenterDepth));
+    const float wid = currId != prevId ? 0.0f : 1.0f;
+    const float similarity = wn * wz * wid;
+    if (similarity < 0.99f) accumMultiplier = 0.0f;
+    historyAccum = gmin(1.0f + historyAccum * accumMultiplier, fr.fps);
+    const float maxAccumulationFactor = 1.0f / historyAccum;
+    const f3 illumAvg = mixv(prevGiV, currGi, maxAccumulationFactor);
+    const f3 sc = ld3(screen + 3 * p);
+    const f3 comb = sc + gi * illumAvg;
+    combined[3 * p + 0] = comb.x;
+    combined[3 * p + 1] = comb.y;
+    combined[3 * p + 2] = comb.z;
+    giOut[3 * p + 0] = illumAvg.x;
+    giOut[3 * p + 1] = illumAvg.y;
+    giOut[3 * p + 2] = illumAvg.z;
+    resOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
+    historyOut[p] = historyAccum;
+}
+
+}  // namespace
+
+namespace sgi_launch {
+
+hipError_t denoise(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const int32_t* visible,
+                   const float* screen, const float* color, const float* reservoir, const float* prevGi,
+                   const float* prevHistory, float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                   const DenoiseBuffers& ws, hipStream_t s) {
+    const dim3 grid((cur.W + 63) / 64, (cur.H + 3) / 4), block(256);
+    hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
+                       ws.normals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel<false>, grid, block, 0, s, cur, prev, fr, 2, ws.resA, ws.normals, ws.resB,
+                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel<false>, grid, block, 0, s, cur, prev, fr, 4, ws.resB, ws.normals, ws.resA,
+                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel<true>, grid, block, 0, s, cur, prev, fr, 8, ws.resA, ws.normals,
+                       (float4*)reservoirOut, screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    return hipGetLastError();
+}
+
+}  // namespace sgi_launch

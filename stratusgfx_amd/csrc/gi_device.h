@@ -1,0 +1,314 @@
+// Device helpers shared by the GI kernels: FP32 vector arithmetic in the
+// reference shaders' evaluation order, GL texture addressing, the hash RNG and
+// the cube / cascade lookups.
+//
+// The FP32 kernels are built with -ffp-contract=off and rely on HIP's default
+// correctly rounded FP32 division and sqrt, so every decision the reference
+// makes (side tests, radius tests, cube texel choice, sample counts, top-K
+// order) is taken on the same bits as the FP32 restatement of the shaders.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sgi.h"
+
+namespace sgi_dev {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 splat(float s) { return f3{s, s, s}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator+(f3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
+__device__ __forceinline__ f3 operator-(f3 a, float s) { return {a.x - s, a.y - s, a.z - s}; }
+__device__ __forceinline__ f3 operator/(float s, f3 a) { return {s / a.x, s / a.y, s / a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(f3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a / length(a); }
+// GLSL max / min as written in the restatement: a > b ? a : b.
+__device__ __forceinline__ float gmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float gmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float clampv(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__device__ __forceinline__ float saturate(float x) { return clampv(x, 0.0f, 1.0f); }
+__device__ __forceinline__ float mixv(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ f3 mixv(f3 x, f3 y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ float pow5(float x) { const float x2 = x * x; return x2 * x2 * x; }
+__device__ __forceinline__ float pow128(float x) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) x = x * x;
+    return x;
+}
+__host__ __device__ __forceinline__ f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+
+constexpr float kPI = 3.14159265359f;     // common.glsl:3
+constexpr float kEPS = 0.00001f;          // common.glsl:4 PREVENT_DIV_BY_ZERO
+constexpr float kFLOAT_MAX = 3.402823466e+38f;
+
+// ---- RNG (common.glsl:183-214) ----
+__device__ __forceinline__ uint32_t hash1(uint32_t x) {
+    x += (x << 10u);
+    x ^= (x >> 6u);
+    x += (x << 3u);
+    x ^= (x >> 11u);
+    x += (x << 15u);
+    return x;
+}
+__device__ __forceinline__ float random3(float x, float y, float z) {
+    uint32_t m = hash1(__float_as_uint(x) ^ hash1(__float_as_uint(y)) ^ hash1(__float_as_uint(z)));
+    m &= 0x007FFFFFu;
+    m |= 0x3F800000u;
+    return __uint_as_float(m) - 1.0f;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// floor(coordinate) -> texel index clamped to [0, n-1]; the float is clamped
+// first so that NaN / huge coordinates convert without overflow (NaN -> 0).
+__device__ __forceinline__ int texelIndex(float f, int n) {
+    const float c = f > -1.0f ? (f < (float)n ? f : (float)n) : -1.0f;
+    return clampi((int)c, 0, n - 1);
+}
+
+// Column-major mat4 * (x, y, z, w), summed column by column as GLSL does.
+__device__ __forceinline__ void mulMat4(const float* m, float x, float y, float z, float w, float out[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[r] = m[r] * x + m[4 + r] * y + m[8 + r] * z + m[12 + r] * w;
+}
+
+// common.glsl:143-155
+__device__ __forceinline__ f3 worldPositionFromDepth(float u, float v, float depth, const float* invPV) {
+    const float z = depth * 2.0f - 1.0f;
+    float w[4];
+    mulMat4(invPV, u * 2.0f - 1.0f, v * 2.0f - 1.0f, z, 1.0f, w);
+    return f3{w[0] / w[3], w[1] / w[3], w[2] / w[3]};
+}
+
+__device__ __forceinline__ f3 decodeNormal(const float* p) {
+    return normalize(f3{p[0], p[1], p[2]} * 2.0f - splat(1.0f));
+}
+
+// Texel-centre uv of pixel x in a viewport of size W.
+__device__ __forceinline__ float texelCentre(int x, int W) { return ((float)x + 0.5f) / (float)W; }
+
+// samplerCubeArray NEAREST (GL major-axis table; ties x, then y, then z).
+// Returns the texel offset of direction d inside a layer's 6 faces.
+__device__ __forceinline__ uint32_t cubeTexelOffset(f3 d, int res) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    const float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    const int i = texelIndex(floorf(s * (float)res), res);
+    const int j = texelIndex(floorf(t * (float)res), res);
+    return ((uint32_t)face * res + j) * res + i;
+}
+
+// Kernel-argument view of the cube atlases.
+struct CubeArgs {
+    const void* atlas[SGI_MAX_TOTAL_SHADOW_ATLASES];
+    int numAtlases;
+    int res;
+    int layers;
+    uint32_t layerStride;  // 6 * res * res texels
+};
+
+// Base address of cube `layer` in atlas `a`; the unrolled select keeps the
+// pointer table in kernel-argument registers instead of private memory.
+__device__ __forceinline__ const void* cubeLayerBase(const CubeArgs& c, int a, int layer, int texelBytes) {
+    const void* r = c.atlas[0];
+    a = clampi(a, 0, c.numAtlases - 1);  // out-of-range entries read atlas 0
+#pragma unroll
+    for (int i = 1; i < SGI_MAX_TOTAL_SHADOW_ATLASES; ++i)
+        if (a == i) r = c.atlas[i];
+    const int l = clampi(layer, 0, c.layers - 1);
+    return (const char*)r + (size_t)l * c.layerStride * texelBytes;
+}
+
+template <int FMT>
+__device__ __forceinline__ float cubeFetch(const void* layerBase, int res, f3 d) {
+    const uint32_t off = cubeTexelOffset(d, res);
+    if (FMT == SGI_TEXEL_F16) return __half2float(((const __half*)layerBase)[off]);
+    return ((const float*)layerBase)[off];
+}
+
+// Kernel-argument view of the cascades and sun.
+struct CsmArgs {
+    const float* depth;
+    int res;
+    float projViews[SGI_NUM_CASCADES][16];
+    float planes[SGI_NUM_CASCADES - 1][4];
+    float bias;  // depthBias / (zfar - znear)
+};
+
+// sampler2DArrayShadow LINEAR + LEQUAL compare, CLAMP_TO_EDGE.
+__device__ __forceinline__ float shadowCompare(const CsmArgs& c, float s, float t, int layer, float ref) {
+    const float* base = c.depth + (size_t)layer * c.res * c.res;
+    const float fu = s * (float)c.res - 0.5f, fv = t * (float)c.res - 0.5f;
+    const float i0f = floorf(fu), j0f = floorf(fv);
+    const float a = fu - i0f, b = fv - j0f;
+    const int ia = texelIndex(i0f, c.res), ib = texelIndex(i0f + 1.0f, c.res);
+    const int ja = texelIndex(j0f, c.res), jb = texelIndex(j0f + 1.0f, c.res);
+    const float c00 = ref <= base[(size_t)ja * c.res + ia] ? 1.0f : 0.0f;
+    const float c10 = ref <= base[(size_t)ja * c.res + ib] ? 1.0f : 0.0f;
+    const float c01 = ref <= base[(size_t)jb * c.res + ia] ? 1.0f : 0.0f;
+    const float c11 = ref <= base[(size_t)jb * c.res + ib] ? 1.0f : 0.0f;
+    return (1.0f - a) * (1.0f - b) * c00 + a * (1.0f - b) * c10 + (1.0f - a) * b * c01 + a * b * c11;
+}
+
+__device__ __forceinline__ f3 cascadeBlends(const CsmArgs& c, f3 p) {
+    float b[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) b[i] = c.planes[i][0] * p.x + c.planes[i][1] * p.y + c.planes[i][2] * p.z + c.planes[i][3] * 1.0f;
+    return f3{b[0], b[1], b[2]};
+}
+
+// pbr.glsl:158-227 calculateInfiniteShadowValue; `bias` already zero when unused.
+__device__ __forceinline__ float infiniteShadowValue(const CsmArgs& c, f3 sunDir, f3 fragPos, f3 blends, f3 normal,
+                                                     float bias) {
+    const f3 position = fragPos + normal * (1.0f - saturate(dot(normal, sunDir))) * 1.0f;
+    const bool beyond2 = blends.y >= 0.0f, beyond3 = blends.z >= 0.0f;
+    const int index1 = beyond2 ? 2 : 0, index2 = beyond3 ? 3 : 1;
+    float c1[4], c2[4];
+    mulMat4(c.projViews[index1], position.x, position.y, position.z, 1.0f, c1);
+    mulMat4(c.projViews[index2], position.x, position.y, position.z, 1.0f, c2);
+    const float s1 = (c1[0] / c1[3]) * 0.5f + 0.5f, t1 = (c1[1] / c1[3]) * 0.5f + 0.5f;
+    const float depth1 = (c1[2] / c1[3]) * 0.5f + 0.5f;
+    const float s2 = (c2[0] / c2[3]) * 0.5f + 0.5f, t2 = (c2[1] / c2[3]) * 0.5f + 0.5f;
+    float depth2 = (c2[2] / c2[3]) * 0.5f + 0.5f;
+    depth2 = beyond3 ? saturate(depth2) : depth2;
+    const float weight = beyond2 ? saturate(blends.y) - saturate(blends.z) : 1.0f - saturate(blends.x);
+    const float wh = 1.0f / (float)c.res;
+    float light1 = 0.0f, light2 = 0.0f, samples = 0.0f;
+    for (float y = -1.0f; y <= 1.0f; y += 1.0f) {
+        for (float x = -1.0f; x <= 1.0f; x += 1.0f) {
+            light1 += shadowCompare(c, s1 + x * wh, t1 + y * wh, index1, depth1 - bias);
+            light2 += shadowCompare(c, s2 + x * wh, t2 + y * wh, index2, depth2 - bias);
+            samples += 1.0f;
+        }
+    }
+    return mixv(light2, light1, weight) * (1.0f / samples);
+}
+
+// Kernel-argument view of a G-buffer.
+struct GbufArgs {
+    int W, H;
+    const float* depth;
+    const float* normal;
+    const float* albedo;
+    const float* refl;
+    const float* rma;
+    const float* velocity;
+    const float* id;
+    const float* ssao;
+};
+
+__host__ inline GbufArgs gbufArgs(const sgi_gbuffer& g) {
+    GbufArgs a;
+    a.W = g.width;
+    a.H = g.height;
+    a.depth = g.depth;
+    a.normal = g.normal;
+    a.albedo = g.albedo;
+    a.refl = g.baseReflectivity;
+    a.rma = g.roughnessMetallicAmbient;
+    a.velocity = g.velocity;
+    a.id = g.id;
+    a.ssao = g.ssao;
+    return a;
+}
+
+// Per-frame uniforms the kernels read.
+struct FrameArgs {
+    float invPV[16];
+    float viewPos[3];
+    float sunDir[3];
+    float sunColor[3];
+    float time;
+    float fps;
+    float minRoughness;
+    int perceptual;
+    float minGiOcc;
+    float fogDensity;
+    float fogColor[3];
+    float emissionStrength;
+};
+
+__host__ inline FrameArgs frameArgs(const sgi_frame& f) {
+    FrameArgs a;
+    for (int i = 0; i < 16; ++i) a.invPV[i] = f.invProjectionView[i];
+    for (int k = 0; k < 3; ++k) {
+        a.viewPos[k] = f.viewPosition[k];
+        a.sunDir[k] = f.infiniteLightDirection[k];
+        a.sunColor[k] = f.infiniteLightColor[k];
+        a.fogColor[k] = f.fogColor[k];
+    }
+    a.time = f.time;
+    a.fps = f.framesPerSecond;
+    a.minRoughness = f.minRoughness;
+    a.perceptual = f.usePerceptualRoughness;
+    a.minGiOcc = f.minGiOcclusionFactor;
+    a.fogDensity = f.fogDensity;
+    a.emissionStrength = f.emissionStrength;
+    return a;
+}
+
+// pbr2.glsl:139-144
+__device__ __forceinline__ float remapRoughness(float r, const FrameArgs& f) {
+    if (f.perceptual) r = r * r;
+    return gmax(gmax(0.0f, f.minRoughness), r);
+}
+// pbr2.glsl:51-55
+__device__ __forceinline__ float ndfGGX(float NdotH, float r) {
+    const float r2 = r * r;
+    const float f = (NdotH * r2 - NdotH) * NdotH + 1.0f;
+    return r2 / gmax(kPI * f * f, kEPS);
+}
+// pbr2.glsl:59-64
+__device__ __forceinline__ float visSmithGGX(float NdotV, float NdotL, float r) {
+    const float a2 = r * r;
+    const float ggxv = NdotL * __builtin_sqrtf((NdotV - a2 * NdotV) * NdotV + a2);
+    const float ggxl = NdotV * __builtin_sqrtf((NdotL - a2 * NdotL) * NdotL + a2);
+    return 0.5f / gmax(ggxv + ggxl, kEPS);
+}
+// pbr2.glsl:73-76
+__device__ __forceinline__ f3 fresnelSchlick(float u, f3 f0) {
+    const float f = pow5(1.0f - u);
+    return splat(f) + f0 * (1.0f - f);
+}
+// pbr.glsl:260-267 with the length already known
+__device__ __forceinline__ float vplAttenuation(float lightDist, float radius) {
+    const float ratio = gmin(lightDist / radius, 1.0f);
+    const float minDist = mixv(10.0f, 1.0f, expf(-3.0f * ratio)) * radius;
+    return 1.0f / (minDist + 1.0f * lightDist * lightDist);
+}
+
+// XCD-aware block -> work mapping: hardware dispatches consecutive blocks round
+// robin over the 8 XCDs, so hand each XCD a contiguous run of work items so
+// that neighbouring tiles (which read the same VPL cube texels) share an L2.
+__device__ __forceinline__ uint32_t xcdSwizzle(uint32_t b, uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = n / kXcd, rem = n % kXcd;
+    const uint32_t x = b % kXcd, k = b / kXcd;
+    // XCD x owns items [x*per + min(x, rem), ...) of length per + (x < rem)
+    const uint32_t start = x * per + (x < rem ? x : rem);
+    return start + k;
+}
+
+}  // namespace sgi_dev

@@ -1,0 +1,145 @@
+"""Python handle on the GI pass's C ABI (include/sgi.h) over torch device memory.
+
+torch supplies device memory and the stream; every computation runs in the
+HIP kernels of libsgi.so.  A missing library raises — there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+from . import _abi
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def _stream(stream) -> int:
+    if stream is not None:
+        return stream
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+class GiPass:
+    """One viewport's GI pass context (sgi_create / sgi_destroy)."""
+
+    def __init__(self, width: int, height: int, tile_size: int = 16, max_vpls_per_tile: int = 12, lib=None):
+        self.lib = lib if lib is not None else _abi.load()
+        self.width, self.height = width, height
+        self.tile_size, self.K = tile_size, max_vpls_per_tile
+        cfg = _abi.Config(width, height, tile_size, max_vpls_per_tile)
+        h = ctypes.c_void_p()
+        _abi.call(self.lib, "sgi_create", ctypes.byref(cfg), ctypes.byref(h))
+        self.ctx = h
+        tx, ty = ctypes.c_int32(), ctypes.c_int32()
+        _abi.call(self.lib, "sgi_tile_counts", self.ctx, ctypes.byref(tx), ctypes.byref(ty))
+        self.tiles_x, self.tiles_y = tx.value, ty.value
+
+    @property
+    def num_tiles(self) -> int:
+        return self.tiles_x * self.tiles_y
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.sgi_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_frame(self, frame: _abi.Frame) -> None:
+        _abi.call(self.lib, "sgi_set_frame", self.ctx, ctypes.byref(frame))
+
+    def set_csm(self, csm: _abi.Csm) -> None:
+        _abi.call(self.lib, "sgi_set_csm", self.ctx, ctypes.byref(csm))
+
+    def cull_global(self, vpls, n, out, visible, stream=None) -> None:
+        _abi.call(self.lib, "sgi_cull_global", self.ctx, _ptr(vpls), n, _ptr(out), _ptr(visible), _stream(stream))
+
+    def cull_tiles(self, gbuf: _abi.Gbuffer, vpls, visible, tiles, remap=0, stage1=None, stream=None) -> None:
+        _abi.call(self.lib, "sgi_cull_tiles", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), remap,
+                  _ptr(stage1), _ptr(tiles), _stream(stream))
+
+    def gather(self, gbuf: _abi.Gbuffer, vpls, visible, entries, cubes: _abi.CubeAtlases, tiles, color, reservoir,
+               precision=_abi.PRECISION_FP32, stream=None) -> None:
+        _abi.call(self.lib, "sgi_gather", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
+                  ctypes.byref(cubes), _ptr(tiles), precision, _ptr(color), _ptr(reservoir), _stream(stream))
+
+    def direct_light(self, gbuf: _abi.Gbuffer, lit, stream=None) -> None:
+        _abi.call(self.lib, "sgi_direct_light", self.ctx, ctypes.byref(gbuf), _ptr(lit), _stream(stream))
+
+    def denoise(self, cur: _abi.Gbuffer, prev: _abi.Gbuffer | None, visible, screen, color, reservoir, prev_gi,
+                prev_history, combined, gi_out, reservoir_out, history_out, stream=None) -> None:
+        _abi.call(self.lib, "sgi_denoise", self.ctx, ctypes.byref(cur), ctypes.byref(prev) if prev else None,
+                  _ptr(visible), _ptr(screen), _ptr(color), _ptr(reservoir), _ptr(prev_gi), _ptr(prev_history),
+                  _ptr(combined), _ptr(gi_out), _ptr(reservoir_out), _ptr(history_out), _stream(stream))
+
+    def render_frame(self, io: _abi.FrameIO, stream=None) -> None:
+        _abi.call(self.lib, "sgi_render_frame", self.ctx, ctypes.byref(io), _stream(stream))
+
+
+@dataclass
+class FrameBuffers:
+    """Device buffers of one GI frame (scratch and outputs of sgi_render_frame)."""
+    vpl_updated: object
+    visible: object
+    visible_entries: object
+    tiles: object
+    direct: object
+    color: object
+    reservoir: object
+    combined: object
+    gi: object
+    reservoir_out: object
+    history: object
+
+    @staticmethod
+    def allocate(width: int, height: int, num_vpls: int, num_tiles: int, K: int, device="cuda") -> "FrameBuffers":
+        import torch
+        cap = max(1, min(num_vpls, _abi.MAX_TOTAL_VPLS_PER_FRAME))
+        f = dict(dtype=torch.float32, device=device)
+        i = dict(dtype=torch.int32, device=device)
+        return FrameBuffers(
+            vpl_updated=torch.zeros((cap, _abi.VPL_FLOATS), **f),
+            visible=torch.zeros((1 + cap,), **i),
+            visible_entries=torch.zeros((cap, 2), **i),
+            tiles=torch.zeros((num_tiles, 1 + K), **i),
+            direct=torch.zeros((height, width, 3), **f),
+            color=torch.zeros((height, width, 3), **f),
+            reservoir=torch.zeros((height, width, 4), **f),
+            combined=torch.zeros((height, width, 3), **f),
+            gi=torch.zeros((height, width, 3), **f),
+            reservoir_out=torch.zeros((height, width, 4), **f),
+            history=torch.zeros((height, width), **f),
+        )
+
+    def frame_io(self, gbuf_ref, prev_gbuf_ref, vpls, num_vpls, entries, cubes_ref, use_tiles: bool,
+                 precision=_abi.PRECISION_FP32, prev_gi=None, prev_history=None) -> _abi.FrameIO:
+        io = _abi.FrameIO()
+        io.gbuffer = ctypes.pointer(gbuf_ref)
+        io.prevGbuffer = ctypes.pointer(prev_gbuf_ref) if prev_gbuf_ref is not None else None
+        io.vplData = _ptr(vpls)
+        io.numVpls = num_vpls
+        io.shadowIndices = _ptr(entries)
+        io.cubes = ctypes.pointer(cubes_ref)
+        io.useTiles = 1 if use_tiles else 0
+        io.precision = precision
+        io.vplUpdatedData = _ptr(self.vpl_updated)
+        io.visibleIndices = _ptr(self.visible)
+        io.visibleShadowIndices = _ptr(self.visible_entries)
+        io.tiles = _ptr(self.tiles)
+        io.direct = _ptr(self.direct)
+        io.color = _ptr(self.color)
+        io.reservoir = _ptr(self.reservoir)
+        io.prevGi = _ptr(prev_gi)
+        io.prevHistory = _ptr(prev_history)
+        io.combined = _ptr(self.combined)
+        io.giOut = _ptr(self.gi)
+        io.reservoirOut = _ptr(self.reservoir_out)
+        io.historyOut = _ptr(self.history)
+        return io
