@@ -138,7 +138,7 @@ __global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameAr
     const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
     const int tx = (int)t % tilesX, ty = (int)t / tilesX;
     const int32_t* rec = tiles + (size_t)t * (1 + K);
-    if (threadIdx.x == 0) ln = rec[0];
+    if (threadIdx.x == 0) ln = min(max(rec[0], 0), K);
     if ((int)threadIdx.x < K) {
         const int n = rec[0];
         if ((int)threadIdx.x < n) {
