@@ -179,6 +179,15 @@ int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* csm);
 int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n,
                     sgi_vpl* vplUpdatedData, int32_t* visibleIndices, void* stream);
 
+/* PerformVirtualPointLightCullingStage2_'s host loop (StratusRendererBackend.cpp:
+ * 1675-1701), which reads the visible list back and uploads one atlas entry per
+ * visible slot, done on the device: out[s] = shadowIndices[visibleIndices[1 + s]]
+ * for s < min(visibleIndices[0], maxCount).  shadowIndices: device [n] by input
+ * index; out: device [maxCount]. */
+int sgi_visible_atlas_entries(sgi_ctx* ctx, const sgi_atlas_entry* shadowIndices,
+                              const int32_t* visibleIndices, int32_t maxCount,
+                              sgi_atlas_entry* out, void* stream);
+
 /* vpl_tiled_deferred_culling_stage1.cs:50-90 + _stage2.cs:61-129 (the dispatches
  * commented out at StratusRendererBackend.cpp:1733-1783).
  * Stage 1: per tile, the mean world position and the normalised mean decoded

@@ -48,6 +48,10 @@ int sgi_scene_create(const sgi_scene_desc* desc, sgi_scene** out);
 void sgi_scene_destroy(sgi_scene* scene);
 int sgi_scene_num_boxes(const sgi_scene* scene);
 
+/* The scene's boxes (host memory, 16 floats each): centre xyz, cos(yaw),
+ * half extents xyz, sin(yaw), albedo rgb, roughness, metallic, mesh id, 0, 0. */
+int sgi_scene_boxes(const sgi_scene* scene, float* out);
+
 /* A view of the scene that frames it (camera at the Cornell box mouth, or over
  * the city block). */
 int sgi_scene_default_view(const sgi_scene* scene, int32_t width, int32_t height, sgi_view* out);

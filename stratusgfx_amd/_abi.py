@@ -113,6 +113,7 @@ SGI_SIGNATURES = {
     "sgi_set_frame": (c_int32, [_P, POINTER(Frame)]),
     "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
+    "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
     "sgi_cull_tiles": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _I, _P, _P, _P]),
     "sgi_gather": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, _I, _P, _P, _P]),
     "sgi_direct_light": (c_int32, [_P, POINTER(Gbuffer), _P, _P]),
@@ -124,6 +125,7 @@ SCENE_SIGNATURES = {
     "sgi_scene_create": (c_int32, [POINTER(SceneDesc), POINTER(_P)]),
     "sgi_scene_destroy": (None, [_P]),
     "sgi_scene_num_boxes": (c_int32, [_P]),
+    "sgi_scene_boxes": (c_int32, [_P, _P]),
     "sgi_scene_default_view": (c_int32, [_P, _I, _I, POINTER(View)]),
     "sgi_scene_set_view": (c_int32, [_P, POINTER(View), POINTER(View), POINTER(Frame), POINTER(Csm)]),
     "sgi_scene_vpls": (c_int32, [_P, _I, _P, _P]),

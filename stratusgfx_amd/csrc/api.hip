@@ -132,6 +132,18 @@ int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n, sgi_vpl* vp
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_global");
 }
 
+int sgi_visible_atlas_entries(sgi_ctx* ctx, const sgi_atlas_entry* shadowIndices, const int32_t* visibleIndices,
+                              int32_t maxCount, sgi_atlas_entry* out, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_visible_atlas_entries: NULL context");
+    if (maxCount < 0 || maxCount > SGI_MAX_TOTAL_VPLS_PER_FRAME)
+        return fail(SGI_E_INVALID, "sgi_visible_atlas_entries: maxCount must be within 0..4480");
+    if (!shadowIndices || !visibleIndices || (maxCount > 0 && !out))
+        return fail(SGI_E_INVALID, "sgi_visible_atlas_entries: NULL buffer");
+    const hipError_t e =
+        sgi_launch::gatherVisibleEntries(shadowIndices, visibleIndices, maxCount, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_visible_atlas_entries");
+}
+
 int sgi_cull_tiles(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
                    int32_t remap, sgi_tile_stage1* stage1, int32_t* tiles, void* stream) {
     if (!ctx) return fail(SGI_E_INVALID, "sgi_cull_tiles: NULL context");
@@ -222,15 +234,14 @@ int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream) {
     if (!io->shadowIndices || !io->visibleShadowIndices)
         return fail(SGI_E_INVALID, "sgi_render_frame: NULL atlas entry buffer");
     if (io->useTiles && !io->tiles) return fail(SGI_E_INVALID, "sgi_render_frame: tile lists requested, tiles NULL");
-    const hipStream_t s = (hipStream_t)stream;
     // UpdatePointLights_ -> PerformVirtualPointLightCullingStage1_
     int r = sgi_cull_global(ctx, io->vplData, io->numVpls, io->vplUpdatedData, io->visibleIndices, stream);
     if (r != SGI_OK) return r;
     // PerformVirtualPointLightCullingStage2_: atlas entries of the visible slots
     const int maxCount = io->numVpls < SGI_MAX_TOTAL_VPLS_PER_FRAME ? io->numVpls : SGI_MAX_TOTAL_VPLS_PER_FRAME;
-    hipError_t e = sgi_launch::gatherVisibleEntries(io->shadowIndices, io->visibleIndices, maxCount,
-                                                    io->visibleShadowIndices, s);
-    if (e != hipSuccess) return hipFail(e, "sgi_render_frame: atlas entries");
+    if ((r = sgi_visible_atlas_entries(ctx, io->shadowIndices, io->visibleIndices, maxCount,
+                                       io->visibleShadowIndices, stream)) != SGI_OK)
+        return r;
     // deferred lighting pass (pbr.fs)
     if ((r = sgi_direct_light(ctx, io->gbuffer, io->direct, stream)) != SGI_OK) return r;
     // tile culling (the commented-out dispatches, StratusRendererBackend.cpp:1733-1783)

@@ -8,9 +8,12 @@
 // its own VPL slots and reads the records through the cache hierarchy.
 //
 // Per-pixel quantities of the BRDF (fragment position, view vector, demodulated
-// diffuse colour, f0, remapped roughness, N.V) are computed once per pixel;
-// the per-VPL body keeps the FP32 restatement's operation order, so outputs
-// agree with the oracle to the last few ulps (expf is the only libm call).
+// diffuse colour, f0, remapped roughness, N.V) are computed once per pixel.
+// Everything a discrete decision of the reference depends on — the side and
+// radius tests, the cube texel choice, the shadow compare, the sample count —
+// is computed with correctly rounded division and sqrt in the restatement's
+// operation order, so every branch matches the oracle bit for bit; the
+// continuous BRDF terms use hardware rcp / rsq / exp2 (~1 ulp each).
 #include "launch.h"
 
 namespace {
@@ -24,7 +27,7 @@ struct Pixel {
 
 struct Vpl {  // the fields of sgi_vpl the gather reads, plus its cube's base address
     f3 pos, spec, color;
-    float radius, farPlane;
+    float radius, invRadius, farPlane;
     const void* cube;
 };
 
@@ -35,10 +38,50 @@ __device__ __forceinline__ Vpl loadVpl(const sgi_vpl* v, const sgi_atlas_entry* 
     o.spec = ld3(v->specularPosition);
     o.color = ld3(v->color);
     o.radius = v->radius;
+    o.invRadius = 1.0f / v->radius;
     o.farPlane = v->farPlane;
     const sgi_atlas_entry en = *e;
     o.cube = cubeLayerBase(cubes, en.index, en.layer, FMT == SGI_TEXEL_F16 ? 2 : 4);
     return o;
+}
+
+// vplDiffuseAttenuation (pbr.glsl:260-267) with the light distance known.
+__device__ __forceinline__ float vplAttenuationFast(float lightDist, float radius, float invRadius) {
+    const float ratio = gmin(lightDist * invRadius, 1.0f);
+    const float minDist = mixv(10.0f, 1.0f, fexp(-3.0f * ratio)) * radius;
+    return frcp(minDist + 1.0f * lightDist * lightDist);
+}
+
+// The continuous part of one VPL's lighting: calculateVirtualPointLighting2 ->
+// calculateLighting_Lambert_SeparateDiffuseSpecular (pbr2.glsl:541-574, 613-657),
+// with FMA contraction allowed (no decision depends on these values).
+__device__ __forceinline__ f3 vplLighting(const Pixel& px, const Vpl& L, f3 lmf, float dist) {
+#pragma clang fp contract(fast)
+    const f3 specDir = L.spec - px.P;
+    const float s2 = dot(specDir, specDir);
+    const float sInv = frsq(s2);
+    const float sLen = s2 * sInv;
+    const f3 Ls = specDir * sInv;
+    const f3 h = px.V + Ls;
+    const f3 Hs = h * frsq(dot(h, h));
+    const float NdotL = saturate(dot(px.N, Ls));
+    const float NdotH = saturate(dot(px.N, Hs));
+    const float LdotH = saturate(dot(Ls, Hs));
+    const float r2 = px.a * px.a;  // NDF_GGX / Visibility_G_SmithGGX (pbr2.glsl:51-64)
+    const float f = (NdotH * r2 - NdotH) * NdotH + 1.0f;
+    const float ndf = r2 * frcp(gmax(kPI * f * f, kEPS));
+    const float ggxv = NdotL * fsqrt((px.NdotV - r2 * px.NdotV) * px.NdotV + r2);
+    const float ggxl = px.NdotV * fsqrt((NdotL - r2 * NdotL) * NdotL + r2);
+    const float vis = 0.5f * frcp(gmax(ggxv + ggxl, kEPS));
+    const f3 Fr = (ndf * vis) * fresnelSchlick(LdotH, px.f0) * NdotL;
+    const float invDist = frcp(dist);
+    const float NdotLd = saturate(dot(px.N, lmf) * invDist);
+    const f3 Fd = px.diffColor * (1.0f / kPI) * NdotLd;
+    const f3 brdf = vplAttenuationFast(sLen, L.radius, L.invRadius) * Fr +
+                    vplAttenuationFast(dist, L.radius, L.invRadius) * Fd;
+    // the ambient term is brdf * ao * colour * 0 and the fog intensity is 0, so
+    // the lighting is (brdf * colour) * exp(-fogDensity * viewDist)
+    return (brdf * L.color) * px.fog;
 }
 
 // One VPL's contribution (the body of the sample loop, vpl_pbr_gi.fs:149-199).
@@ -47,55 +90,36 @@ template <int FMT>
 __device__ __forceinline__ bool evalVpl(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
                                         bool allowReject, f3& vplColor, float& validSamples) {
     const f3 lmf = L.pos - px.P;
-    const float dist = length(lmf);
-    const f3 Ld = lmf / dist;  // normalize(lightMinusFrag)
-    if (allowReject) {
-        const float sideCheck = dot(px.N, Ld);
-        if (sideCheck < 0.0f || dist > L.radius) return false;
-    }
-    const float distanceRatio = clampv((2.0f * dist) / L.radius, 0.0f, 1.0f);
+    const float dist = length(lmf);  // exact: radius test and shadow compare
+    if (allowReject && (normalizedDotNegative(px.N, lmf, dist) || dist > L.radius)) return false;
+    const float distanceRatio = clampv((2.0f * dist) * L.invRadius, 0.0f, 1.0f);
     float shadowFactor = 0.0f;
     if (px.distToCamera < 500.0f) {  // calculateShadowValue1Sample, pbr.glsl:116-136
         const f3 fragDir = px.P - L.pos;
-        const float currentDepth = length(fragDir);
+        const float currentDepth = dist;  // |P - L| == |L - P| bit for bit
         const float bias = currentDepth * gmax(0.1f, saturate(dot(lmf, px.N)) * 0.03f);
         const float depth = cubeFetch<FMT>(L.cube, cubes.res, fragDir) * L.farPlane;
         shadowFactor = (currentDepth - bias) > depth ? 1.0f : 0.0f;
     }
+    // min(shadow, mix(minGiOcc, 1, ratio)) > 0 exactly when the shadow test fired
     shadowFactor = gmin(shadowFactor, mixv(fr.minGiOcc, 1.0f, distanceRatio));
     float reweightingFactor = 1.0f;
     if (shadowFactor > 0.0f) reweightingFactor = (1.0f - distanceRatio) * fr.minGiOcc + distanceRatio;
     validSamples += reweightingFactor;
-
-    // calculateVirtualPointLighting2 -> calculateLighting_Lambert_SeparateDiffuseSpecular
-    const f3 specDir = L.spec - px.P;
-    const float sLen = length(specDir);
-    const f3 Ls = specDir / sLen;
-    const f3 Hs = normalize(px.V + Ls);
-    const float NdotL = saturate(dot(px.N, Ls));
-    const float NdotH = saturate(dot(px.N, Hs));
-    const float LdotH = saturate(dot(Ls, Hs));
-    const f3 Fr = (ndfGGX(NdotH, px.a) * visSmithGGX(px.NdotV, NdotL, px.a)) * fresnelSchlick(LdotH, px.f0) * NdotL;
-    const float NdotLd = saturate(dot(px.N, Ld));
-    const f3 Fd = px.diffColor * (1.0f / kPI) * NdotLd;
-    const f3 brdf = vplAttenuation(sLen, L.radius) * Fr + vplAttenuation(dist, L.radius) * Fd;
-    // ambient term is brdf * ao * colour * 0 and the fog intensity is 0, so the
-    // lighting is (brdf * colour) * exp(-fogDensity * viewDist)
-    const f3 lit = (brdf * L.color) * px.fog;
-    const f3 tmpColor = px.ao * lit;
-    vplColor = vplColor + (1.0f - shadowFactor) * tmpColor;
+    vplColor = vplColor + (1.0f - shadowFactor) * (px.ao * vplLighting(px, L, lmf, dist));
     return true;
 }
 
 __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& fr, int x, int y, size_t p) {
     Pixel px;
     px.P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV);
-    const f3 viewPos = ld3(fr.viewPos);
-    px.V = normalize(viewPos - px.P);
-    px.distToCamera = length(viewPos - px.P);
+    const f3 vp = ld3(fr.viewPos) - px.P;
+    const float vp2 = dot(vp, vp);
+    px.distToCamera = __builtin_sqrtf(vp2);  // exact: shadow on / off and the sample count
+    px.V = vp * frsq(vp2);
     const float* al = g.albedo + 4 * p;
     const f3 baseColor = f3{gmax(al[0], 0.01f), gmax(al[1], 0.01f), gmax(al[2], 0.01f)};
-    px.N = decodeNormal(g.normal + 3 * p);
+    px.N = decodeNormal(g.normal + 3 * p);  // exact: side test
     const float* rma = g.rma + 3 * p;
     const float roughness = gmax(0.5f, rma[0]);
     const float metallic = rma[1];
@@ -103,11 +127,11 @@ __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& 
     const f3 baseRefl = splat(g.refl[2 * p]);
     // BRDF_Base with diffuseDivisor = baseColor, specularMultiplier = 1 / (baseColor + eps)
     px.a = remapRoughness(roughness, fr);
-    px.diffColor = ((1.0f - metallic) * baseColor) / (baseColor + kEPS);
-    const f3 specMul = 1.0f / (baseColor + kEPS);
+    const f3 specMul = f3{frcp(baseColor.x + kEPS), frcp(baseColor.y + kEPS), frcp(baseColor.z + kEPS)};
+    px.diffColor = ((1.0f - metallic) * baseColor) * specMul;
     px.f0 = (0.16f * baseRefl * baseRefl * (1.0f - metallic) + baseColor * metallic) * specMul;
     px.NdotV = saturate(dot(px.N, px.V));
-    px.fog = expf(-fr.fogDensity * px.distToCamera);
+    px.fog = fexp(-fr.fogDensity * px.distToCamera);
     return px;
 }
 

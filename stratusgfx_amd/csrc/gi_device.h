@@ -49,6 +49,15 @@ __device__ __forceinline__ float pow128(float x) {
 }
 __host__ __device__ __forceinline__ f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 
+// Hardware reciprocal, reciprocal square root and exp2 (about 1 ulp).  Used
+// only for continuous shading terms, never for a value a discrete decision of
+// the reference depends on, so outputs stay within a few ulps of the exact
+// restatement while every branch is taken on exactly rounded values.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(1.44269504088896f * x); }
+
 constexpr float kPI = 3.14159265359f;     // common.glsl:3
 constexpr float kEPS = 0.00001f;          // common.glsl:4 PREVENT_DIV_BY_ZERO
 constexpr float kFLOAT_MAX = 3.402823466e+38f;
@@ -141,9 +150,54 @@ __device__ __forceinline__ const void* cubeLayerBase(const CubeArgs& c, int a, i
     return (const char*)r + (size_t)l * c.layerStride * texelBytes;
 }
 
+// floor(0.5 * (sc / ma + 1) * res) as the correctly rounded formula gives it,
+// clamped to [0, res-1].  The estimate with a hardware reciprocal is within
+// 5 * 2^-24 * res of the exact value, so when it lies more than res * 2^-19
+// from an integer its floor is the exact floor; otherwise the exact formula
+// runs (about 0.1 % of lookups at res 256).
+__device__ __forceinline__ int cubeTexelCoord(float sc, float ma, int res) {
+    const float fres = (float)res;
+    const float x = 0.5f * (sc * frcp(ma) + 1.0f) * fres;
+    const float fl = floorf(x);
+    const float margin = fres * 0x1p-19f;
+    if (x - fl > margin && (fl + 1.0f) - x > margin) return texelIndex(fl, res);
+    return texelIndex(floorf(0.5f * (sc / ma + 1.0f) * fres), res);
+}
+
+// cubeTexelOffset with the exact floor decided from the estimate when safe.
+__device__ __forceinline__ uint32_t cubeTexelOffsetFast(f3 d, int res) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    const int i = cubeTexelCoord(sc, ma, res);
+    const int j = cubeTexelCoord(tc, ma, res);
+    return ((uint32_t)face * res + j) * res + i;
+}
+
+// dot(N, v / len) < 0 as the reference rounds it.  dot(N, v) carries at most
+// 3 * 2^-24 * sum|N_i v_i| of rounding error and the reference's value at most
+// 4 * 2^-24 * sum|N_i v_i| / len, so a margin of 2^-20 * sum|N_i v_i| decides
+// the sign; inside the margin the reference's formula runs.
+__device__ __forceinline__ bool normalizedDotNegative(f3 N, f3 v, float len) {
+    const float a = dot(N, v);
+    const float mag = fabsf(N.x * v.x) + fabsf(N.y * v.y) + fabsf(N.z * v.z);
+    if (mag > 1e-30f && fabsf(a) > mag * 0x1p-20f) return a < 0.0f;
+    return dot(N, v / len) < 0.0f;
+}
+
 template <int FMT>
 __device__ __forceinline__ float cubeFetch(const void* layerBase, int res, f3 d) {
-    const uint32_t off = cubeTexelOffset(d, res);
+    const uint32_t off = cubeTexelOffsetFast(d, res);
     if (FMT == SGI_TEXEL_F16) return __half2float(((const __half*)layerBase)[off]);
     return ((const float*)layerBase)[off];
 }

@@ -511,6 +511,13 @@ void sgi_scene_destroy(sgi_scene* s) {
 
 int sgi_scene_num_boxes(const sgi_scene* s) { return s ? (int)s->boxes.size() : 0; }
 
+int sgi_scene_boxes(const sgi_scene* s, float* out) {
+    if (!s || !out) { sgi_set_error("sgi_scene_boxes: NULL argument"); return SGI_E_INVALID; }
+    static_assert(sizeof(sgi_rt::Box) == 16 * sizeof(float), "box record is 16 floats");
+    std::memcpy(out, s->boxes.data(), s->boxes.size() * sizeof(sgi_rt::Box));
+    return SGI_OK;
+}
+
 int sgi_scene_default_view(const sgi_scene* s, int32_t width, int32_t height, sgi_view* v) {
     if (!s || !v || width <= 0 || height <= 0) { sgi_set_error("sgi_scene_default_view: bad argument"); return SGI_E_INVALID; }
     std::memset(v, 0, sizeof(*v));

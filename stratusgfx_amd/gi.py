@@ -61,6 +61,10 @@ class GiPass:
     def cull_global(self, vpls, n, out, visible, stream=None) -> None:
         _abi.call(self.lib, "sgi_cull_global", self.ctx, _ptr(vpls), n, _ptr(out), _ptr(visible), _stream(stream))
 
+    def visible_atlas_entries(self, entries, visible, max_count, out, stream=None) -> None:
+        _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
+                  _stream(stream))
+
     def cull_tiles(self, gbuf: _abi.Gbuffer, vpls, visible, tiles, remap=0, stage1=None, stream=None) -> None:
         _abi.call(self.lib, "sgi_cull_tiles", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), remap,
                   _ptr(stage1), _ptr(tiles), _stream(stream))
