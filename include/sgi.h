@@ -217,6 +217,32 @@ int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
                const sgi_cube_atlases* cubes, const int32_t* tiles, int32_t precision,
                float* color, float* reservoir, void* stream);
 
+/* Gather modes of sgi_gather_ex. */
+enum {
+    SGI_GATHER_REFERENCE = 0, /* the reference's estimator (as sgi_gather with tiles == NULL) */
+    SGI_GATHER_TILES = 1,     /* every VPL of the pixel's tile list (as sgi_gather with tiles) */
+    SGI_GATHER_ALL = 2,       /* every visible VPL: the converged estimate of the pass */
+    SGI_GATHER_SAMPLES = 3    /* `samples` hashed-random slots over all visible VPLs per pixel */
+};
+
+typedef struct sgi_gather_opts {
+    int32_t mode;             /* SGI_GATHER_* */
+    int32_t samples;          /* SGI_GATHER_SAMPLES: draws per pixel (1..4096) */
+    int32_t precision;        /* SGI_PRECISION_* of the shading */
+    int32_t accumulate_fp64;  /* SGI_GATHER_ALL: sum C and w in FP64 */
+} sgi_gather_opts;
+
+/* The gather's further modes, for measurement and for the learned denoiser's
+ * data (no counterpart in the reference, which only has the estimator of
+ * vpl_pbr_gi.fs:145-200).  SGI_GATHER_ALL evaluates every visible VPL with the
+ * per-VPL body of vpl_pbr_gi.fs:153-199, skipping the ones the side / radius
+ * test rejects; SGI_GATHER_SAMPLES draws slot int(count * rand) with the
+ * reference's hash and <= 5 rejections.  Outputs as sgi_gather. */
+int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
+                  const int32_t* visibleIndices, const sgi_atlas_entry* shadowIndices,
+                  const sgi_cube_atlases* cubes, const int32_t* tiles,
+                  const sgi_gather_opts* opts, float* color, float* reservoir, void* stream);
+
 /* pbr.fs:82-145 with INFINITE_LIGHTING_ENABLED (StratusRendererBackend.cpp:2010-2038),
  * zero point lights: sun lighting (BRDF_Burley) times the 3x3 PCF cascade
  * shadow, plus emissive * emissionStrength, bounded to 65504.

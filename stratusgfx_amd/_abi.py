@@ -89,6 +89,13 @@ class FrameIO(ctypes.Structure):
     ]
 
 
+class GatherOpts(ctypes.Structure):
+    _fields_ = [("mode", c_int32), ("samples", c_int32), ("precision", c_int32), ("accumulate_fp64", c_int32)]
+
+
+GATHER_REFERENCE, GATHER_TILES, GATHER_ALL, GATHER_SAMPLES = 0, 1, 2, 3
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("seed", c_uint64), ("num_vpls", c_int32),
                 ("num_city_boxes", c_int32), ("city_half_extent", c_float),
@@ -116,6 +123,8 @@ SGI_SIGNATURES = {
     "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
     "sgi_cull_tiles": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _I, _P, _P, _P]),
     "sgi_gather": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, _I, _P, _P, _P]),
+    "sgi_gather_ex": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, POINTER(GatherOpts), _P,
+                                _P, _P]),
     "sgi_direct_light": (c_int32, [_P, POINTER(Gbuffer), _P, _P]),
     "sgi_denoise": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               _P]),

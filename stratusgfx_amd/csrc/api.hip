@@ -20,6 +20,7 @@ struct sgi_ctx {
     sgi_dev::FrameArgs frameArgs{};
     sgi_dev::CsmArgs csmArgs{};
     sgi_launch::DenoiseBuffers ws{};
+    uint8_t* visFlags = nullptr;  // [SGI_MAX_TOTAL_VPLS_BEFORE_CULLING] per-VPL sun visibility
     int device = 0;
 };
 
@@ -70,6 +71,7 @@ int sgi_create(const sgi_config* config, sgi_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->ws.resA, plane);
     if (e == hipSuccess) e = hipMalloc(&ctx->ws.resB, plane);
     if (e == hipSuccess) e = hipMalloc(&ctx->ws.normals, plane);
+    if (e == hipSuccess) e = hipMalloc(&ctx->visFlags, SGI_MAX_TOTAL_VPLS_BEFORE_CULLING);
     if (e != hipSuccess) {
         sgi_destroy(ctx);
         return hipFail(e, "sgi_create: workspace allocation");
@@ -83,6 +85,7 @@ void sgi_destroy(sgi_ctx* ctx) {
     if (ctx->ws.resA) (void)hipFree(ctx->ws.resA);
     if (ctx->ws.resB) (void)hipFree(ctx->ws.resB);
     if (ctx->ws.normals) (void)hipFree(ctx->ws.normals);
+    if (ctx->visFlags) (void)hipFree(ctx->visFlags);
     delete ctx;
 }
 
@@ -128,7 +131,7 @@ int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n, sgi_vpl* vp
     if (!visibleIndices || (n > 0 && (!vplData || !vplUpdatedData)))
         return fail(SGI_E_INVALID, "sgi_cull_global: NULL buffer");
     const hipError_t e = sgi_launch::cullGlobal(ctx->csmArgs, ctx->frameArgs, vplData, n, vplUpdatedData,
-                                                visibleIndices, (hipStream_t)stream);
+                                                visibleIndices, ctx->visFlags, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_global");
 }
 
@@ -157,26 +160,17 @@ int sgi_cull_tiles(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, cons
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_tiles");
 }
 
-int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
-               const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles,
-               int32_t precision, float* color, float* reservoir, void* stream) {
-    if (!ctx) return fail(SGI_E_INVALID, "sgi_gather: NULL context");
-    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_gather: sgi_set_frame first");
-    int r = checkGbuffer(ctx, g, "sgi_gather", true);
-    if (r != SGI_OK) return r;
-    if (!vpls || !visibleIndices || !shadowIndices || !cubes || !color || !reservoir)
-        return fail(SGI_E_INVALID, "sgi_gather: NULL buffer");
-    if (!aligned16(reservoir)) return fail(SGI_E_INVALID, "sgi_gather: reservoir must be 16-byte aligned");
-    if (precision != SGI_PRECISION_FP32 && precision != SGI_PRECISION_FP16)
-        return fail(SGI_E_INVALID, "sgi_gather: unknown precision");
+namespace {
+
+int cubeArgs(const sgi_cube_atlases* cubes, const char* who, sgi_dev::CubeArgs& ca) {
     if (cubes->resolution <= 0 || cubes->resolution > 4096 ||
         (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16) || cubes->layers_per_atlas <= 0 ||
         cubes->layers_per_atlas > SGI_MAX_TOTAL_SHADOWS_PER_ATLAS || cubes->num_atlases <= 0 ||
         cubes->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES)
-        return fail(SGI_E_INVALID, "sgi_gather: bad cube atlas description");
-    sgi_dev::CubeArgs ca{};
+        return fail(SGI_E_INVALID, std::string(who) + ": bad cube atlas description");
+    ca = sgi_dev::CubeArgs{};
     for (int a = 0; a < cubes->num_atlases; ++a) {
-        if (!cubes->atlas[a]) return fail(SGI_E_INVALID, "sgi_gather: NULL cube atlas");
+        if (!cubes->atlas[a]) return fail(SGI_E_INVALID, std::string(who) + ": NULL cube atlas");
         ca.atlas[a] = cubes->atlas[a];
     }
     for (int a = cubes->num_atlases; a < SGI_MAX_TOTAL_SHADOW_ATLASES; ++a) ca.atlas[a] = cubes->atlas[0];
@@ -184,12 +178,64 @@ int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const in
     ca.res = cubes->resolution;
     ca.layers = cubes->layers_per_atlas;
     ca.layerStride = 6u * (uint32_t)cubes->resolution * (uint32_t)cubes->resolution;
+    return SGI_OK;
+}
+
+int checkGatherArgs(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
+                    const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, float* color,
+                    float* reservoir, const char* who) {
+    if (!ctx) return fail(SGI_E_INVALID, std::string(who) + ": NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, std::string(who) + ": sgi_set_frame first");
+    int r = checkGbuffer(ctx, g, who, true);
+    if (r != SGI_OK) return r;
+    if (!vpls || !visibleIndices || !shadowIndices || !cubes || !color || !reservoir)
+        return fail(SGI_E_INVALID, std::string(who) + ": NULL buffer");
+    if (!aligned16(reservoir)) return fail(SGI_E_INVALID, std::string(who) + ": reservoir must be 16-byte aligned");
+    return SGI_OK;
+}
+
+}  // namespace
+
+int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
+               const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles,
+               int32_t precision, float* color, float* reservoir, void* stream) {
+    int r = checkGatherArgs(ctx, g, vpls, visibleIndices, shadowIndices, cubes, color, reservoir, "sgi_gather");
+    if (r != SGI_OK) return r;
+    if (precision != SGI_PRECISION_FP32 && precision != SGI_PRECISION_FP16)
+        return fail(SGI_E_INVALID, "sgi_gather: unknown precision");
+    sgi_dev::CubeArgs ca;
+    if ((r = cubeArgs(cubes, "sgi_gather", ca)) != SGI_OK) return r;
     const hipError_t e = sgi_launch::gather(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format,
                                             ctx->cfg.tile_size, ctx->tilesX, ctx->tilesY, ctx->cfg.max_vpls_per_tile,
                                             vpls, visibleIndices, shadowIndices, tiles, precision, color, reservoir,
                                             (hipStream_t)stream);
     if (e == hipErrorNotSupported) return fail(SGI_E_INVALID, "sgi_gather: this precision is not built");
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather");
+}
+
+int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
+                  const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles,
+                  const sgi_gather_opts* opts, float* color, float* reservoir, void* stream) {
+    int r = checkGatherArgs(ctx, g, vpls, visibleIndices, shadowIndices, cubes, color, reservoir, "sgi_gather_ex");
+    if (r != SGI_OK) return r;
+    if (!opts) return fail(SGI_E_INVALID, "sgi_gather_ex: NULL options");
+    if (opts->mode == SGI_GATHER_REFERENCE || opts->mode == SGI_GATHER_TILES) {
+        if (opts->mode == SGI_GATHER_TILES && !tiles) return fail(SGI_E_INVALID, "sgi_gather_ex: tile mode without tiles");
+        return sgi_gather(ctx, g, vpls, visibleIndices, shadowIndices, cubes,
+                          opts->mode == SGI_GATHER_TILES ? tiles : nullptr, opts->precision, color, reservoir, stream);
+    }
+    if (opts->mode != SGI_GATHER_ALL && opts->mode != SGI_GATHER_SAMPLES)
+        return fail(SGI_E_INVALID, "sgi_gather_ex: unknown mode");
+    if (opts->mode == SGI_GATHER_SAMPLES && (opts->samples < 1 || opts->samples > 4096))
+        return fail(SGI_E_INVALID, "sgi_gather_ex: samples must be within 1..4096");
+    if (opts->precision != SGI_PRECISION_FP32)
+        return fail(SGI_E_INVALID, "sgi_gather_ex: these modes shade in FP32");
+    sgi_dev::CubeArgs ca;
+    if ((r = cubeArgs(cubes, "sgi_gather_ex", ca)) != SGI_OK) return r;
+    const hipError_t e = sgi_launch::gatherEx(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format, opts->mode,
+                                              opts->samples, opts->accumulate_fp64, vpls, visibleIndices,
+                                              shadowIndices, color, reservoir, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather_ex");
 }
 
 int sgi_direct_light(sgi_ctx* ctx, const sgi_gbuffer* g, float* lit, void* stream) {

@@ -1,8 +1,8 @@
 // VPL culling kernels.
 //
-// cullGlobalKernel — viscull_vpls.cs:47-120: per VPL, the sun-visibility test
-// through the cascades, then a stable compaction (wave ballot + block prefix)
-// of the visible records in ascending input order.
+// cullVisibilityKernel + cullCompactKernel — viscull_vpls.cs:47-120: per VPL,
+// the sun-visibility test through the cascades, then a stable compaction (wave
+// ballot + block prefix) of the visible records in ascending input order.
 //
 // cullTilesKernel — vpl_tiled_deferred_culling_stage1.cs:50-90 and
 // _stage2.cs:61-129 fused: one wave per tile.  Stage 1 reconstructs the
@@ -21,21 +21,30 @@ using namespace sgi_dev;
 
 constexpr int kCullThreads = 1024;
 
-__global__ __launch_bounds__(kCullThreads) void cullGlobalKernel(CsmArgs csm, f3 sun, const sgi_vpl* __restrict__ in,
-                                                                 int n, sgi_vpl* __restrict__ out,
-                                                                 int32_t* __restrict__ visible) {
+// Per-VPL sun visibility, one thread per VPL over many blocks (each VPL's 36
+// cascade compares read scattered texels; spreading them over CUs keeps the
+// loads in flight on many L1s).
+__global__ __launch_bounds__(64) void cullVisibilityKernel(CsmArgs csm, f3 sun, const sgi_vpl* __restrict__ in, int n,
+                                                          uint8_t* __restrict__ flags) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const f3 p = ld3(in[i].position);
+    const f3 blends = cascadeBlends(csm, p);
+    const float shadowFactor = 1.0f - infiniteShadowValue(csm, sun, p, blends, sun, 0.0f);
+    flags[i] = shadowFactor < 1.0f ? 1 : 0;
+}
+
+// Stable compaction of the flags: wave ballots and a block prefix, in input order.
+__global__ __launch_bounds__(kCullThreads) void cullCompactKernel(const sgi_vpl* __restrict__ in, int n,
+                                                                  const uint8_t* __restrict__ flags,
+                                                                  sgi_vpl* __restrict__ out,
+                                                                  int32_t* __restrict__ visible) {
     __shared__ int waveCount[kCullThreads / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int running = 0;
     for (int c0 = 0; c0 < n; c0 += kCullThreads) {
         const int i = c0 + threadIdx.x;
-        bool vis = false;
-        if (i < n) {
-            const f3 p = ld3(in[i].position);
-            const f3 blends = cascadeBlends(csm, p);
-            const float shadowFactor = 1.0f - infiniteShadowValue(csm, sun, p, blends, sun, 0.0f);
-            vis = shadowFactor < 1.0f;
-        }
+        const bool vis = i < n && flags[i] != 0;
         const uint64_t m = __ballot(vis);
         const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
         const int lanePrefix = __popcll(below);
@@ -130,17 +139,20 @@ __global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, 
                 cand = !(ratio > 1.0f) && ratio == ratio;
             }
         }
-        const float thr = __shfl(myDist, K - 1);
+        // K-1 and c are wave-uniform, so the broadcasts are v_readlane
+        const float thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myDist), K - 1));
         uint64_t m = __ballot(cand && ratio < thr);
         while (m) {
             const int c = __builtin_ctzll(m);
             m &= m - 1;
-            const float r = __shfl(ratio, c);
-            const float thrNow = __shfl(myDist, K - 1);
+            const float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ratio), c));
+            const float thrNow = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myDist), K - 1));
             if (!(r < thrNow)) continue;
             const int p = __popcll(__ballot(lane < K && myDist <= r));
-            const float upD = __shfl_up(myDist, 1);
-            const int upI = __shfl_up(myIdx, 1);
+            // lane i takes lane i-1's entry: a DPP wave shift, no LDS round trip
+            const float upD = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(myDist), __float_as_int(myDist),
+                                                                         0x138, 0xF, 0xF, false));
+            const int upI = __builtin_amdgcn_update_dpp(myIdx, myIdx, 0x138, 0xF, 0xF, false);
             if (lane < K) {
                 if (lane > p) {
                     myDist = upD;
@@ -163,8 +175,13 @@ __global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, 
 namespace sgi_launch {
 
 hipError_t cullGlobal(const CsmArgs& csm, const FrameArgs& fr, const sgi_vpl* in, int n, sgi_vpl* out,
-                      int32_t* visible, hipStream_t s) {
-    hipLaunchKernelGGL(cullGlobalKernel, dim3(1), dim3(kCullThreads), 0, s, csm, ld3(fr.sunDir), in, n, out, visible);
+                      int32_t* visible, uint8_t* flags, hipStream_t s) {
+    if (n > 0) {
+        hipLaunchKernelGGL(cullVisibilityKernel, dim3((n + 63) / 64), dim3(64), 0, s, csm, ld3(fr.sunDir), in, n, flags);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(cullCompactKernel, dim3(1), dim3(kCullThreads), 0, s, in, n, flags, out, visible);
     return hipGetLastError();
 }
 

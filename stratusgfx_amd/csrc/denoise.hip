@@ -4,12 +4,20 @@
 // Only the reservoir changes between passes (the colour / GI attachment passes
 // through until the final pass), so the passes exchange one float4 plane:
 //   pass 0  denoiseMergeKernel: centre-only reservoir merge (computeMergedReservoir
-//           with minmaxNearest = 0) and normalisation; it also stores the decoded
-//           normals once, since every later tap re-reads them;
-//   pass 1,2 denoiseFilterKernel<false>: 5x5 taps at stride 2^pass with weight
+//           with minmaxNearest = 0) and normalisation; it also stores each pixel's
+//           decoded normal and depth as one float4, the 16 B every later tap reads;
+//   pass 1,2 denoiseFilterKernel: 5x5 taps at stride 2^pass with weight
 //           pow(max(0, n.n'), 128) * exp(-|z - z'|) (centre weight 1);
-//   pass 3  denoiseFilterKernel<true>: the same filter, then the temporal blend
-//           with the reprojected history and the combine screen + albedo * gi.
+//   pass 3  denoiseFilterKernel at stride 8 into the reservoir output, then
+//           denoiseTemporalKernel: the temporal blend with the reprojected
+//           history and the combine screen + albedo * gi.
+//
+// Blocks cover 64x4 pixels and are numbered so that each XCD walks a
+// contiguous band of rows: the 5x5 taps at stride 8 reach 16 rows up and down,
+// and that window stays in the XCD's L2 instead of being fetched once per XCD.
+// The edge-stopping weights and the normalisation are continuous, so they use
+// hardware exp2 / rcp; the merge test and the temporal similarity test are
+// decisions and keep correctly rounded arithmetic.
 #include "launch.h"
 
 namespace {
@@ -17,6 +25,14 @@ namespace {
 using namespace sgi_dev;
 
 __device__ __forceinline__ int clampx(int x, int W) { return x < 0 ? 0 : (x >= W ? W - 1 : x); }
+
+// 64x4-pixel block of a row-band (XCD-contiguous) numbering.
+__device__ __forceinline__ void blockPixel(int W, int H, int& x, int& y) {
+    const int bx = (W + 63) / 64, by = (H + 3) / 4;
+    const uint32_t b = xcdSwizzle(blockIdx.x, (uint32_t)(bx * by));
+    x = (int)(b % bx) * 64 + (int)threadIdx.x % 64;
+    y = (int)(b / bx) * 4 + (int)threadIdx.x / 64;
+}
 
 // GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv.
 __device__ __forceinline__ float bilinear(const float* t, int W, int H, int C, float u, float v, int c) {
@@ -38,12 +54,13 @@ __device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C,
 __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
                                                           const float4* __restrict__ resIn, float4* __restrict__ resOut,
                                                           float4* __restrict__ normals) {
-    const int x = blockIdx.x * 64 + (int)threadIdx.x % 64, y = blockIdx.y * 4 + (int)threadIdx.x / 64;
+    int x, y;
+    blockPixel(g.W, g.H, x, y);
     if (x >= g.W || y >= g.H) return;
     const size_t p = (size_t)y * g.W + x;
     const f3 centerNormal = decodeNormal(g.normal + 3 * p);
-    normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, 0.0f);
     const float centerDepth = g.depth[p];
+    normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, centerDepth);
     const int numVisible = visible[0];
     float cr0 = 0.0f, cr1 = 0.0f, cr2 = 0.0f, cr3 = 0.0f;
     const float depthCutoff = 0.1f * centerDepth;
@@ -64,30 +81,26 @@ __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs 
         }
     }
     cr3 = runningSum;
-    const float den = gmax(kEPS, cr3);
-    resOut[p] = make_float4(cr0 / den, cr1 / den, cr2 / den, 1.0f);
+    const float inv = frcp(gmax(kEPS, cr3));
+    resOut[p] = make_float4(cr0 * inv, cr1 * inv, cr2 * inv, 1.0f);
 }
 
-template <bool FINAL>
-__global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, GbufArgs prev, FrameArgs fr, int stride,
-                                                           const float4* __restrict__ resIn,
+__global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
                                                            const float4* __restrict__ normals,
-                                                           float4* __restrict__ resOut, const float* __restrict__ screen,
-                                                           const float* __restrict__ color,
-                                                           const float* __restrict__ prevGi,
-                                                           const float* __restrict__ prevHistory,
-                                                           float* __restrict__ combined, float* __restrict__ giOut,
-                                                           float* __restrict__ historyOut) {
-    const int x = blockIdx.x * 64 + (int)threadIdx.x % 64, y = blockIdx.y * 4 + (int)threadIdx.x / 64;
+                                                           float4* __restrict__ resOut) {
+    int x, y;
+    blockPixel(g.W, g.H, x, y);
     if (x >= g.W || y >= g.H) return;
     const int W = g.W, H = g.H;
     const size_t p = (size_t)y * W + x;
     const float4 cn4 = normals[p];
     const f3 centerNormal = f3{cn4.x, cn4.y, cn4.z};
-    const float centerDepth = g.depth[p];
+    const float centerDepth = cn4.w;
     float rf0 = 0.0f, rf1 = 0.0f, rf2 = 0.0f, rf3 = 0.0f;
+#pragma unroll
     for (int dx = -2; dx <= 2; ++dx) {
         const int tx = clampx(x + dx * stride, W);
+#pragma unroll
         for (int dy = -2; dy <= 2; ++dy) {
             const int ty = clampx(y + dy * stride, H);
             const size_t q = (size_t)ty * W + tx;
@@ -96,25 +109,43 @@ __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, GbufArgs 
             if (dx == 0 && dy == 0) {
                 filtered = 1.0f;
             } else {  // filterInput, vpl_pbr_gi_denoise.fs:142-190
-                const float currDepth = g.depth[q];
-                const float wz = expf(-fabsf(centerDepth - currDepth)) / 1.0f;
                 const float4 n4 = normals[q];
-                float wn = gmax(0.0f, dot(centerNormal, f3{n4.x, n4.y, n4.z}));
+                const float wz = fexp(-fabsf(centerDepth - n4.w));
+                float wn = gmax(0.0f, dotF(centerNormal, f3{n4.x, n4.y, n4.z}));
                 wn = pow128(wn);
-                filtered = wn * wz * 1.0f;
+                filtered = wn * wz;
             }
-            rf0 += filtered * rr.x;
-            rf1 += filtered * rr.y;
-            rf2 += filtered * rr.z;
-            rf3 += filtered * rr.w;
+            rf0 = __builtin_fmaf(filtered, rr.x, rf0);
+            rf1 = __builtin_fmaf(filtered, rr.y, rf1);
+            rf2 = __builtin_fmaf(filtered, rr.z, rf2);
+            rf3 = __builtin_fmaf(filtered, rr.w, rf3);
         }
     }
-    const float den = gmax(kEPS, rf3);
-    const f3 shadowFactor = f3{rf0 / den, rf1 / den, rf2 / den};
-    if (!FINAL) {
-        resOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
-        return;
-    }
+    const float inv = frcp(gmax(kEPS, rf3));
+    resOut[p] = make_float4(rf0 * inv, rf1 * inv, rf2 * inv, 1.0f);
+}
+
+// The final pass's temporal blend and combine (vpl_pbr_gi_denoise.fs:336-434) on
+// the filtered reservoir the last filter pass wrote.
+__global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArgs prev, FrameArgs fr,
+                                                             const float4* __restrict__ filtered,
+                                                             const float4* __restrict__ normals,
+                                                             const float* __restrict__ screen,
+                                                             const float* __restrict__ color,
+                                                             const float* __restrict__ prevGi,
+                                                             const float* __restrict__ prevHistory,
+                                                             float* __restrict__ combined, float* __restrict__ giOut,
+                                                             float* __restrict__ historyOut) {
+    int x, y;
+    blockPixel(g.W, g.H, x, y);
+    if (x >= g.W || y >= g.H) return;
+    const int W = g.W, H = g.H;
+    const size_t p = (size_t)y * W + x;
+    const float4 cn4 = normals[p];
+    const f3 centerNormal = f3{cn4.x, cn4.y, cn4.z};
+    const float centerDepth = cn4.w;
+    const float4 sf4 = filtered[p];
+    const f3 shadowFactor = f3{sf4.x, sf4.y, sf4.z};
     const f3 gi = ld3(color + 3 * p);
     float historyAccum = prevHistory ? prevHistory[p] : 0.0f;
     float accumMultiplier = 1.0f;
@@ -138,7 +169,7 @@ __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, GbufArgs 
     const float similarity = wn * wz * wid;
     if (similarity < 0.99f) accumMultiplier = 0.0f;
     historyAccum = gmin(1.0f + historyAccum * accumMultiplier, fr.fps);
-    const float maxAccumulationFactor = 1.0f / historyAccum;
+    const float maxAccumulationFactor = frcp(historyAccum);
     const f3 illumAvg = mixv(prevGiV, currGi, maxAccumulationFactor);
     const f3 sc = ld3(screen + 3 * p);
     const f3 comb = sc + gi * illumAvg;
@@ -148,7 +179,6 @@ __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, GbufArgs 
     giOut[3 * p + 0] = illumAvg.x;
     giOut[3 * p + 1] = illumAvg.y;
     giOut[3 * p + 2] = illumAvg.z;
-    resOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
     historyOut[p] = historyAccum;
 }
 
@@ -160,19 +190,19 @@ hipError_t denoise(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& f
                    const float* screen, const float* color, const float* reservoir, const float* prevGi,
                    const float* prevHistory, float* combined, float* giOut, float* reservoirOut, float* historyOut,
                    const DenoiseBuffers& ws, hipStream_t s) {
-    const dim3 grid((cur.W + 63) / 64, (cur.H + 3) / 4), block(256);
+    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
     hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
                        ws.normals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel<false>, grid, block, 0, s, cur, prev, fr, 2, ws.resA, ws.normals, ws.resB,
-                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2, ws.resA, ws.normals, ws.resB);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel<false>, grid, block, 0, s, cur, prev, fr, 4, ws.resB, ws.normals, ws.resA,
-                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 4, ws.resB, ws.normals, ws.resA);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel<true>, grid, block, 0, s, cur, prev, fr, 8, ws.resA, ws.normals,
-                       (float4*)reservoirOut, screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 8, ws.resA, ws.normals, (float4*)reservoirOut);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseTemporalKernel, grid, block, 0, s, cur, prev, fr, (const float4*)reservoirOut, ws.normals,
+                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void directKernel(GbufArgs g, FrameArgs fr, Cs
 
     const f3 sun = ld3(fr.sunDir);
     const f3 blends = cascadeBlends(csm, fragPos);
-    const float shadowFactor = infiniteShadowValue(csm, sun, fragPos, blends, N, csm.bias);
+    const float shadowFactor = infiniteShadowValueBlock(csm, sun, fragPos, blends, N, csm.bias);
     // calculateDirectionalLighting passes 1 - shadowFactor, which the Burley path inverts again
     const float sf = 1.0f - (1.0f - shadowFactor);
 

@@ -23,6 +23,9 @@ using namespace sgi_dev;
 struct Pixel {
     f3 P, V, N, f0, diffColor;
     float distToCamera, a, NdotV, ao, fog;
+    float nvRaw;    // dot(N, V) before saturation
+    float r2, r2m1; // remapped roughness^2 and roughness^2 - 1
+    float gv;       // sqrt((NdotV - r2 NdotV) NdotV + r2), the view half of the Smith term
 };
 
 struct Vpl {  // the fields of sgi_vpl the gather reads, plus its cube's base address
@@ -45,68 +48,103 @@ __device__ __forceinline__ Vpl loadVpl(const sgi_vpl* v, const sgi_atlas_entry* 
     return o;
 }
 
-// vplDiffuseAttenuation (pbr.glsl:260-267) with the light distance known.
+// vplDiffuseAttenuation (pbr.glsl:260-267) with the light distance known:
+// 1 / (mix(10, 1, exp(-3 min(d / R, 1))) R + d^2).
 __device__ __forceinline__ float vplAttenuationFast(float lightDist, float radius, float invRadius) {
     const float ratio = gmin(lightDist * invRadius, 1.0f);
-    const float minDist = mixv(10.0f, 1.0f, fexp(-3.0f * ratio)) * radius;
-    return frcp(minDist + 1.0f * lightDist * lightDist);
+    const float minDist = __builtin_fmaf(-9.0f, fexp(-3.0f * ratio), 10.0f) * radius;
+    return frcp(__builtin_fmaf(lightDist, lightDist, minDist));
 }
 
-// The continuous part of one VPL's lighting: calculateVirtualPointLighting2 ->
+// The continuous part of one VPL's lighting, calculateVirtualPointLighting2 ->
 // calculateLighting_Lambert_SeparateDiffuseSpecular (pbr2.glsl:541-574, 613-657),
-// with FMA contraction allowed (no decision depends on these values).
-__device__ __forceinline__ f3 vplLighting(const Pixel& px, const Vpl& L, f3 lmf, float dist) {
-#pragma clang fp contract(fast)
+// fused.  With unit V and Ls, H = (V + Ls) / |V + Ls| gives |V + Ls|^2 = 2 + 2 V.Ls,
+// N.H = (N.V + N.Ls) / |V + Ls| and Ls.H = (1 + V.Ls) / |V + Ls|.
+// nl = dot(N, L - P) (already computed for the side test).
+__device__ __forceinline__ f3 vplLighting(const Pixel& px, const Vpl& L, float dist, float nl) {
     const f3 specDir = L.spec - px.P;
-    const float s2 = dot(specDir, specDir);
+    const float s2 = dotF(specDir, specDir);
     const float sInv = frsq(s2);
     const float sLen = s2 * sInv;
     const f3 Ls = specDir * sInv;
-    const f3 h = px.V + Ls;
-    const f3 Hs = h * frsq(dot(h, h));
-    const float NdotL = saturate(dot(px.N, Ls));
-    const float NdotH = saturate(dot(px.N, Hs));
-    const float LdotH = saturate(dot(Ls, Hs));
-    const float r2 = px.a * px.a;  // NDF_GGX / Visibility_G_SmithGGX (pbr2.glsl:51-64)
-    const float f = (NdotH * r2 - NdotH) * NdotH + 1.0f;
-    const float ndf = r2 * frcp(gmax(kPI * f * f, kEPS));
-    const float ggxv = NdotL * fsqrt((px.NdotV - r2 * px.NdotV) * px.NdotV + r2);
-    const float ggxl = px.NdotV * fsqrt((NdotL - r2 * NdotL) * NdotL + r2);
-    const float vis = 0.5f * frcp(gmax(ggxv + ggxl, kEPS));
-    const f3 Fr = (ndf * vis) * fresnelSchlick(LdotH, px.f0) * NdotL;
-    const float invDist = frcp(dist);
-    const float NdotLd = saturate(dot(px.N, lmf) * invDist);
-    const f3 Fd = px.diffColor * (1.0f / kPI) * NdotLd;
-    const f3 brdf = vplAttenuationFast(sLen, L.radius, L.invRadius) * Fr +
-                    vplAttenuationFast(dist, L.radius, L.invRadius) * Fd;
+    const float nlRaw = dotF(px.N, Ls);
+    const float lv = dotF(Ls, px.V);
+    const float hInv = frsq(__builtin_fmaf(2.0f, lv, 2.0f));
+    const float NdotL = saturate(nlRaw);
+    const float NdotH = saturate((px.nvRaw + nlRaw) * hInv);
+    const float LdotH = saturate((1.0f + lv) * hInv);
+    // NDF_GGX and Visibility_G_SmithGGX (pbr2.glsl:51-64)
+    const float f = __builtin_fmaf(NdotH * px.r2m1, NdotH, 1.0f);
+    const float ndf = px.r2 * frcp(gmax(kPI * f * f, kEPS));
+    const float ggxl = px.NdotV * fsqrt(__builtin_fmaf(NdotL * -px.r2m1, NdotL, px.r2));
+    const float vis = 0.5f * frcp(gmax(__builtin_fmaf(NdotL, px.gv, ggxl), kEPS));
+    // Fr_Fresnel_Schlick (pbr2.glsl:73-76)
+    const float fr5 = pow5(1.0f - LdotH);
+    const f3 F = fmaF(px.f0, 1.0f - fr5, splat(fr5));
+    const float spec = ndf * vis * NdotL * vplAttenuationFast(sLen, L.radius, L.invRadius);
+    const float diff = saturate(nl * frcp(dist)) * (1.0f / kPI) * vplAttenuationFast(dist, L.radius, L.invRadius);
+    const f3 brdf = fmaF(px.diffColor, diff, F * spec);
     // the ambient term is brdf * ao * colour * 0 and the fog intensity is 0, so
     // the lighting is (brdf * colour) * exp(-fogDensity * viewDist)
-    return (brdf * L.color) * px.fog;
+    return brdf * L.color;
+}
+
+// Exact side + radius test and exact shadow compare with the correctly rounded
+// |L - P| (out of line: taken by a small fraction of lanes).
+__device__ __attribute__((noinline)) bool rejectExact(f3 N, f3 lmf, float d2, float radius) {
+    const float dist = __builtin_sqrtf(d2);
+    return dot(N, lmf / dist) < 0.0f || dist > radius;
+}
+
+__device__ __attribute__((noinline)) bool shadowedExact(float d2, float m, float depth) {
+    const float currentDepth = __builtin_sqrtf(d2);
+    return (currentDepth - currentDepth * m) > depth;
 }
 
 // One VPL's contribution (the body of the sample loop, vpl_pbr_gi.fs:149-199).
 // Returns false when the side / radius test rejects it.
+//
+// |L - P| comes from the hardware sqrt (within a few ulps); each comparison
+// that uses it is taken from that estimate only when it clears a margin of
+// 2^-18 relative, far above the estimate's error, and otherwise from the
+// correctly rounded length, so every branch matches the restatement.
 template <int FMT>
 __device__ __forceinline__ bool evalVpl(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
                                         bool allowReject, f3& vplColor, float& validSamples) {
     const f3 lmf = L.pos - px.P;
-    const float dist = length(lmf);  // exact: radius test and shadow compare
-    if (allowReject && (normalizedDotNegative(px.N, lmf, dist) || dist > L.radius)) return false;
+    const float d2 = dot(lmf, lmf);
+    const float dist = fsqrt(d2);
+    const float nl = dot(px.N, lmf);  // exact order: side test and shadow bias
+    if (allowReject) {
+        bool reject;
+        if (__builtin_expect(signIsSafe(px.N, lmf, nl) && fabsf(dist - L.radius) > L.radius * 0x1p-18f, 1))
+            reject = nl < 0.0f || dist > L.radius;
+        else
+            reject = rejectExact(px.N, lmf, d2, L.radius);
+        if (reject) return false;
+    }
     const float distanceRatio = clampv((2.0f * dist) * L.invRadius, 0.0f, 1.0f);
     float shadowFactor = 0.0f;
     if (px.distToCamera < 500.0f) {  // calculateShadowValue1Sample, pbr.glsl:116-136
         const f3 fragDir = px.P - L.pos;
-        const float currentDepth = dist;  // |P - L| == |L - P| bit for bit
-        const float bias = currentDepth * gmax(0.1f, saturate(dot(lmf, px.N)) * 0.03f);
+        // bias = |P - L| * max(minBias, saturate(dot(L - P, N)) * 0.03); dot(L - P, N)
+        // has the same products in the same order as dot(N, L - P)
+        const float m = gmax(0.1f, saturate(nl) * 0.03f);
         const float depth = cubeFetch<FMT>(L.cube, cubes.res, fragDir) * L.farPlane;
-        shadowFactor = (currentDepth - bias) > depth ? 1.0f : 0.0f;
+        const float lhs = dist - dist * m;
+        bool shadowed;
+        if (__builtin_expect(fabsf(lhs - depth) > dist * 0x1p-18f, 1))
+            shadowed = lhs > depth;
+        else
+            shadowed = shadowedExact(d2, m, depth);
+        shadowFactor = shadowed ? 1.0f : 0.0f;
     }
     // min(shadow, mix(minGiOcc, 1, ratio)) > 0 exactly when the shadow test fired
     shadowFactor = gmin(shadowFactor, mixv(fr.minGiOcc, 1.0f, distanceRatio));
     float reweightingFactor = 1.0f;
     if (shadowFactor > 0.0f) reweightingFactor = (1.0f - distanceRatio) * fr.minGiOcc + distanceRatio;
     validSamples += reweightingFactor;
-    vplColor = vplColor + (1.0f - shadowFactor) * (px.ao * vplLighting(px, L, lmf, dist));
+    vplColor = fmaF(vplLighting(px, L, dist, nl), (1.0f - shadowFactor) * px.ao * px.fog, vplColor);
     return true;
 }
 
@@ -130,7 +168,11 @@ __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& 
     const f3 specMul = f3{frcp(baseColor.x + kEPS), frcp(baseColor.y + kEPS), frcp(baseColor.z + kEPS)};
     px.diffColor = ((1.0f - metallic) * baseColor) * specMul;
     px.f0 = (0.16f * baseRefl * baseRefl * (1.0f - metallic) + baseColor * metallic) * specMul;
-    px.NdotV = saturate(dot(px.N, px.V));
+    px.nvRaw = dot(px.N, px.V);
+    px.NdotV = saturate(px.nvRaw);
+    px.r2 = px.a * px.a;
+    px.r2m1 = px.r2 - 1.0f;
+    px.gv = fsqrt((px.NdotV - px.r2 * px.NdotV) * px.NdotV + px.r2);
     px.fog = fexp(-fr.fogDensity * px.distToCamera);
     return px;
 }
@@ -223,6 +265,110 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
     writeOut(g, p, vplColor, validSamples, color, reservoir);
 }
 
+// Every visible VPL per pixel (the converged estimate), optionally summed in
+// FP64.  The VPLs are staged through LDS 256 at a time and every lane of the
+// 16x16 tile walks the chunk.
+template <int FMT, bool ACC64>
+__global__ __launch_bounds__(256) void gatherAllKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
+                                                      int nTiles, const sgi_vpl* __restrict__ vpls,
+                                                      const int32_t* __restrict__ visible,
+                                                      const sgi_atlas_entry* __restrict__ entries,
+                                                      float* __restrict__ color, float* __restrict__ reservoir) {
+    __shared__ Vpl lv[256];
+    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
+    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+    const int x = tx * 16 + (int)threadIdx.x % 16, y = ty * 16 + (int)threadIdx.x / 16;
+    const bool inside = x < g.W && y < g.H;
+    const size_t p = inside ? (size_t)y * g.W + x : 0;
+    Pixel px;
+    if (inside) px = setupPixel(g, fr, x, y, p);
+    const int count = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
+    double c64[4] = {0.0, 0.0, 0.0, 0.0};
+    f3 vplColor = splat(0.0f);
+    float validSamples = 0.0f;
+    for (int base = 0; base < count; base += 256) {
+        const int n = min(256, count - base);
+        __syncthreads();
+        if ((int)threadIdx.x < n) lv[threadIdx.x] = loadVpl<FMT>(vpls + base + threadIdx.x, entries + base + threadIdx.x, cubes);
+        __syncthreads();
+        if (!inside) continue;
+        for (int k = 0; k < n; ++k) {
+            if (ACC64) {
+                f3 c = splat(0.0f);
+                float w = 0.0f;
+                if (evalVpl<FMT>(px, fr, cubes, lv[k], true, c, w)) {
+                    c64[0] += (double)c.x;
+                    c64[1] += (double)c.y;
+                    c64[2] += (double)c.z;
+                    c64[3] += (double)w;
+                }
+            } else {
+                evalVpl<FMT>(px, fr, cubes, lv[k], true, vplColor, validSamples);
+            }
+        }
+    }
+    if (!inside) return;
+    if (ACC64) {
+        vplColor = f3{(float)c64[0], (float)c64[1], (float)c64[2]};
+        validSamples = (float)c64[3];
+    }
+    writeOut(g, p, vplColor, validSamples, color, reservoir);
+}
+
+// `samples` hashed-random slots per pixel over all visible VPLs, with the
+// reference's hash, seed stepping and <= 5 rejections.
+template <int FMT>
+__global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
+                                                          int nTiles, int samples, const sgi_vpl* __restrict__ vpls,
+                                                          const int32_t* __restrict__ visible,
+                                                          const sgi_atlas_entry* __restrict__ entries,
+                                                          float* __restrict__ color, float* __restrict__ reservoir) {
+    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
+    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+    const int x = tx * 16 + (int)threadIdx.x % 16, y = ty * 16 + (int)threadIdx.x / 16;
+    if (x >= g.W || y >= g.H) return;
+    const size_t p = (size_t)y * g.W + x;
+    const Pixel px = setupPixel(g, fr, x, y, p);
+    f3 vplColor = splat(0.0f);
+    float validSamples = 0.0f;
+    const int count = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
+    if (count > 0) {
+        const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
+        float seedZ = fr.time;
+        for (int i = 0, resamples = 0; i < samples; i += 1) {
+            const float rnd = random3(seedX, seedY, seedZ);
+            seedZ += 10000.0f;
+            const int lightIndex = min((int)((float)count * rnd), count - 1);
+            const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
+            if (!evalVpl<FMT>(px, fr, cubes, L, resamples < 5, vplColor, validSamples)) {
+                ++resamples;
+                --i;
+                continue;
+            }
+        }
+    }
+    writeOut(g, p, vplColor, validSamples, color, reservoir);
+}
+
+template <int FMT>
+hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int mode, int samples,
+                    int accumulate64, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
+                    float* color, float* reservoir, hipStream_t s) {
+    const int rx = (g.W + 15) / 16, ry = (g.H + 15) / 16;
+    if (mode == SGI_GATHER_ALL) {
+        if (accumulate64)
+            hipLaunchKernelGGL((gatherAllKernel<FMT, true>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry,
+                               vpls, visible, entries, color, reservoir);
+        else
+            hipLaunchKernelGGL((gatherAllKernel<FMT, false>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
+                               rx * ry, vpls, visible, entries, color, reservoir);
+    } else {
+        hipLaunchKernelGGL(gatherSamplesKernel<FMT>, dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry, samples,
+                           vpls, visible, entries, color, reservoir);
+    }
+    return hipGetLastError();
+}
+
 template <int FMT>
 hipError_t launchFmt(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int tileSize, int tilesX,
                      int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
@@ -266,6 +412,16 @@ hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes,
                                         color, reservoir, s);
     return launchFmt<SGI_TEXEL_F32>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
                                     reservoir, s);
+}
+
+hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int mode, int samples,
+                    int accumulate64, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
+                    float* color, float* reservoir, hipStream_t s) {
+    if (format == SGI_TEXEL_F16)
+        return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, accumulate64, vpls, visible, entries, color,
+                                       reservoir, s);
+    return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, accumulate64, vpls, visible, entries, color,
+                                   reservoir, s);
 }
 
 }  // namespace sgi_launch

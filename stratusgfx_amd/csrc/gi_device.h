@@ -155,51 +155,59 @@ __device__ __forceinline__ const void* cubeLayerBase(const CubeArgs& c, int a, i
 // 5 * 2^-24 * res of the exact value, so when it lies more than res * 2^-19
 // from an integer its floor is the exact floor; otherwise the exact formula
 // runs (about 0.1 % of lookups at res 256).
+// The rare exact paths are kept out of line so that the compiler does not
+// if-convert them into every lane's common path.
+__device__ __attribute__((noinline)) int cubeTexelCoordExact(float sc, float ma, int res) {
+    return texelIndex(floorf(0.5f * (sc / ma + 1.0f) * (float)res), res);
+}
+
 __device__ __forceinline__ int cubeTexelCoord(float sc, float ma, int res) {
     const float fres = (float)res;
     const float x = 0.5f * (sc * frcp(ma) + 1.0f) * fres;
     const float fl = floorf(x);
     const float margin = fres * 0x1p-19f;
-    if (x - fl > margin && (fl + 1.0f) - x > margin) return texelIndex(fl, res);
-    return texelIndex(floorf(0.5f * (sc / ma + 1.0f) * fres), res);
+    if (__builtin_expect(x - fl > margin && (fl + 1.0f) - x > margin, 1)) return texelIndex(fl, res);
+    return cubeTexelCoordExact(sc, ma, res);
 }
 
-// cubeTexelOffset with the exact floor decided from the estimate when safe.
+// cubeTexelOffset with the exact floor decided from the estimate when safe;
+// the face is chosen with selects (same table, same tie order).
 __device__ __forceinline__ uint32_t cubeTexelOffsetFast(f3 d, int res) {
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int face;
-    float sc, tc, ma;
-    if (ax >= ay && ax >= az) {
-        ma = ax;
-        if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
-    } else if (ay >= az) {
-        ma = ay;
-        if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
-    } else {
-        ma = az;
-        if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
-    }
+    const bool xMajor = ax >= ay && ax >= az;
+    const bool yMajor = !xMajor && ay >= az;
+    const bool px = d.x >= 0.0f, py = d.y >= 0.0f, pz = d.z >= 0.0f;
+    const float ma = xMajor ? ax : (yMajor ? ay : az);
+    const int face = xMajor ? (px ? 0 : 1) : (yMajor ? (py ? 2 : 3) : (pz ? 4 : 5));
+    const float sc = xMajor ? (px ? -d.z : d.z) : (yMajor ? d.x : (pz ? d.x : -d.x));
+    const float tc = xMajor ? -d.y : (yMajor ? (py ? d.z : -d.z) : -d.y);
     const int i = cubeTexelCoord(sc, ma, res);
     const int j = cubeTexelCoord(tc, ma, res);
     return ((uint32_t)face * res + j) * res + i;
 }
 
-// dot(N, v / len) < 0 as the reference rounds it.  dot(N, v) carries at most
-// 3 * 2^-24 * sum|N_i v_i| of rounding error and the reference's value at most
-// 4 * 2^-24 * sum|N_i v_i| / len, so a margin of 2^-20 * sum|N_i v_i| decides
-// the sign; inside the margin the reference's formula runs.
-__device__ __forceinline__ bool normalizedDotNegative(f3 N, f3 v, float len) {
-    const float a = dot(N, v);
+// True when dot(N, v) = a is far enough from zero, relative to the rounding
+// error of both a and the reference's dot(N, v / len), for its sign to be the
+// sign of the reference's value: a carries at most 3 * 2^-24 * sum|N_i v_i|,
+// the reference at most 4 * 2^-24 * sum|N_i v_i| / len; the margin is 2^-20.
+__device__ __forceinline__ bool signIsSafe(f3 N, f3 v, float a) {
     const float mag = fabsf(N.x * v.x) + fabsf(N.y * v.y) + fabsf(N.z * v.z);
-    if (mag > 1e-30f && fabsf(a) > mag * 0x1p-20f) return a < 0.0f;
-    return dot(N, v / len) < 0.0f;
+    return mag > 1e-30f && fabsf(a) > mag * 0x1p-20f;
+}
+
+// Fused helpers for continuous terms only.
+__device__ __forceinline__ float dotF(f3 a, f3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ f3 fmaF(f3 a, float s, f3 c) {
+    return f3{__builtin_fmaf(a.x, s, c.x), __builtin_fmaf(a.y, s, c.y), __builtin_fmaf(a.z, s, c.z)};
 }
 
 template <int FMT>
 __device__ __forceinline__ float cubeFetch(const void* layerBase, int res, f3 d) {
+    typedef const __attribute__((address_space(1))) unsigned short* GHalf;
+    typedef const __attribute__((address_space(1))) float* GFloat;
     const uint32_t off = cubeTexelOffsetFast(d, res);
-    if (FMT == SGI_TEXEL_F16) return __half2float(((const __half*)layerBase)[off]);
-    return ((const float*)layerBase)[off];
+    if (FMT == SGI_TEXEL_F16) return __half2float(__ushort_as_half(((GHalf)layerBase)[off]));
+    return ((GFloat)layerBase)[off];
 }
 
 // Kernel-argument view of the cascades and sun.
@@ -250,14 +258,90 @@ __device__ __forceinline__ float infiniteShadowValue(const CsmArgs& c, f3 sunDir
     const float weight = beyond2 ? saturate(blends.y) - saturate(blends.z) : 1.0f - saturate(blends.x);
     const float wh = 1.0f / (float)c.res;
     float light1 = 0.0f, light2 = 0.0f, samples = 0.0f;
-    for (float y = -1.0f; y <= 1.0f; y += 1.0f) {
-        for (float x = -1.0f; x <= 1.0f; x += 1.0f) {
+    // the reference loops a float from -1 to 1; integer counters give the same
+    // offsets and let the 36 compares' loads issue together
+#pragma unroll
+    for (int iy = -1; iy <= 1; ++iy) {
+#pragma unroll
+        for (int ix = -1; ix <= 1; ++ix) {
+            const float x = (float)ix, y = (float)iy;
             light1 += shadowCompare(c, s1 + x * wh, t1 + y * wh, index1, depth1 - bias);
             light2 += shadowCompare(c, s2 + x * wh, t2 + y * wh, index2, depth2 - bias);
             samples += 1.0f;
         }
     }
     return mixv(light2, light1, weight) * (1.0f / samples);
+}
+
+// One cascade's 9 PCF taps (pbr.glsl:166-174), each the bilinear blend of 4
+// LEQUAL compares, summed in the reference's (y, x) order.  The taps sit one
+// texel apart, so together they cover a 4x4 texel block: the block is read and
+// compared once, and each tap picks its 4 compares from it.  A tap whose
+// rounded floor lands outside the block (its offset rounds across a texel
+// edge) reads its own texels, so the result equals 9 shadowCompare calls.
+__device__ __forceinline__ float pcf9Block(const CsmArgs& c, float s, float t, int layer, float ref) {
+    const float* base = c.depth + (size_t)layer * c.res * c.res;
+    const float fres = (float)c.res;
+    const float wh = 1.0f / fres;
+    const float I0 = floorf((s + -1.0f * wh) * fres - 0.5f);
+    const float J0 = floorf((t + -1.0f * wh) * fres - 0.5f);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const size_t row = (size_t)texelIndex(J0 + (float)r, c.res) * c.res;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ref <= base[row + texelIndex(I0 + (float)q, c.res)]) mask |= 1u << (r * 4 + q);
+    }
+    float light = 0.0f;
+#pragma unroll
+    for (int iy = -1; iy <= 1; ++iy) {
+#pragma unroll
+        for (int ix = -1; ix <= 1; ++ix) {
+            const float x = (float)ix, y = (float)iy;
+            const float fu = (s + x * wh) * fres - 0.5f, fv = (t + y * wh) * fres - 0.5f;
+            const float i0f = floorf(fu), j0f = floorf(fv);
+            const float a = fu - i0f, b = fv - j0f;
+            const float dqf = i0f - I0, drf = j0f - J0;
+            float c00, c10, c01, c11;
+            if (dqf >= 0.0f && dqf <= 2.0f && drf >= 0.0f && drf <= 2.0f) {
+                const int bit = (int)drf * 4 + (int)dqf;
+                c00 = (mask >> bit) & 1u ? 1.0f : 0.0f;
+                c10 = (mask >> (bit + 1)) & 1u ? 1.0f : 0.0f;
+                c01 = (mask >> (bit + 4)) & 1u ? 1.0f : 0.0f;
+                c11 = (mask >> (bit + 5)) & 1u ? 1.0f : 0.0f;
+            } else {
+                const int ia = texelIndex(i0f, c.res), ib = texelIndex(i0f + 1.0f, c.res);
+                const int ja = texelIndex(j0f, c.res), jb = texelIndex(j0f + 1.0f, c.res);
+                c00 = ref <= base[(size_t)ja * c.res + ia] ? 1.0f : 0.0f;
+                c10 = ref <= base[(size_t)ja * c.res + ib] ? 1.0f : 0.0f;
+                c01 = ref <= base[(size_t)jb * c.res + ia] ? 1.0f : 0.0f;
+                c11 = ref <= base[(size_t)jb * c.res + ib] ? 1.0f : 0.0f;
+            }
+            light += (1.0f - a) * (1.0f - b) * c00 + a * (1.0f - b) * c10 + (1.0f - a) * b * c01 + a * b * c11;
+        }
+    }
+    return light;
+}
+
+// calculateInfiniteShadowValue with the block-read PCF (same result).
+__device__ __forceinline__ float infiniteShadowValueBlock(const CsmArgs& c, f3 sunDir, f3 fragPos, f3 blends, f3 normal,
+                                                          float bias) {
+    const f3 position = fragPos + normal * (1.0f - saturate(dot(normal, sunDir))) * 1.0f;
+    const bool beyond2 = blends.y >= 0.0f, beyond3 = blends.z >= 0.0f;
+    const int index1 = beyond2 ? 2 : 0, index2 = beyond3 ? 3 : 1;
+    float c1[4], c2[4];
+    mulMat4(c.projViews[index1], position.x, position.y, position.z, 1.0f, c1);
+    mulMat4(c.projViews[index2], position.x, position.y, position.z, 1.0f, c2);
+    const float s1 = (c1[0] / c1[3]) * 0.5f + 0.5f, t1 = (c1[1] / c1[3]) * 0.5f + 0.5f;
+    const float depth1 = (c1[2] / c1[3]) * 0.5f + 0.5f;
+    const float s2 = (c2[0] / c2[3]) * 0.5f + 0.5f, t2 = (c2[1] / c2[3]) * 0.5f + 0.5f;
+    float depth2 = (c2[2] / c2[3]) * 0.5f + 0.5f;
+    depth2 = beyond3 ? saturate(depth2) : depth2;
+    const float weight = beyond2 ? saturate(blends.y) - saturate(blends.z) : 1.0f - saturate(blends.x);
+    const float light1 = pcf9Block(c, s1, t1, index1, depth1 - bias);
+    const float light2 = pcf9Block(c, s2, t2, index2, depth2 - bias);
+    return mixv(light2, light1, weight) * (1.0f / 9.0f);
 }
 
 // Kernel-argument view of a G-buffer.

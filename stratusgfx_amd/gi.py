@@ -74,6 +74,14 @@ class GiPass:
         _abi.call(self.lib, "sgi_gather", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
                   ctypes.byref(cubes), _ptr(tiles), precision, _ptr(color), _ptr(reservoir), _stream(stream))
 
+    def gather_ex(self, gbuf: _abi.Gbuffer, vpls, visible, entries, cubes: _abi.CubeAtlases, tiles, color, reservoir,
+                  mode=_abi.GATHER_ALL, samples=8, accumulate_fp64=False, precision=_abi.PRECISION_FP32,
+                  stream=None) -> None:
+        opts = _abi.GatherOpts(mode, samples, precision, 1 if accumulate_fp64 else 0)
+        _abi.call(self.lib, "sgi_gather_ex", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
+                  ctypes.byref(cubes), _ptr(tiles), ctypes.byref(opts), _ptr(color), _ptr(reservoir),
+                  _stream(stream))
+
     def direct_light(self, gbuf: _abi.Gbuffer, lit, stream=None) -> None:
         _abi.call(self.lib, "sgi_direct_light", self.ctx, ctypes.byref(gbuf), _ptr(lit), _stream(stream))
 
