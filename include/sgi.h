@@ -211,7 +211,9 @@ int sgi_cull_tiles(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls
  * pixel's tile list is evaluated once, with slot entries (remap == 0).
  * color: device [H][W][3] = albedo + 1e-5; reservoir: device [H][W][4] =
  * (min(sum C, 65504), max(sum w, 1)).  precision: SGI_PRECISION_FP32 (the
- * restatement) or SGI_PRECISION_FP16 (packed FP16 shading, FP32 accumulation). */
+ * restatement) or SGI_PRECISION_FP16: the same accept / shadow decisions and
+ * the same sum w (both FP32), the BRDF and attenuation of two VPLs at a time in
+ * packed FP16, accumulated into FP32 colour; it assumes VPL radii <= 16384. */
 int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
                const int32_t* visibleIndices, const sgi_atlas_entry* shadowIndices,
                const sgi_cube_atlases* cubes, const int32_t* tiles, int32_t precision,

@@ -1,0 +1,160 @@
+/*
+ * sgi_kpcn.h — the learned denoiser of the VPL GI pass: a kernel-predicting
+ * convolutional network that replaces the a-trous filter of
+ * vpl_pbr_gi_denoise.fs for low-sample gathers, with a per-tile fallback to
+ * the a-trous filter where the network's own error estimate is high.
+ *
+ * The reference has no learned denoiser; the network stands beside
+ * sgi_denoise (vpl_pbr_gi_denoise.fs:142-435, StratusRendererBackend.cpp:
+ * 1879-1951) with the same inputs (G-buffer, gather colour / reservoir, lit
+ * screen) and the same outputs (combined = screen + albedo * gi, gi,
+ * reservoir, history).
+ *
+ * Network: a 5-level encoder-decoder of 3x3 convolutions with 32 channels
+ * (levels at 1, 1/2, ..., 1/16 resolution; 2x2 mean pooling down, nearest
+ * upsampling plus skip addition up; ReLU after every convolution but the
+ * last).  The last convolution produces per pixel 25 logits of a 5x5 filter
+ * (softmax-normalised, applied to the noisy radiance) and one logit of the
+ * predicted log relative squared error.
+ *
+ *   layer  0: features -> c0          (level 0)      layer 1: c0 -> e0
+ *   layer 2: pool(e0) -> c2           (level 1)      layer 3: c2 -> e1
+ *   layer 4: pool(e1) -> c4           (level 2)      layer 5: c4 -> e2
+ *   layer 6: pool(e2) -> c6           (level 3)      layer 7: c6 -> e3
+ *   layer 8: pool(e3) -> c8           (level 4)      layer 9: c8 -> e4
+ *   layer 10: e3 + up(e4) -> d3       layer 11: e2 + up(d3) -> d2
+ *   layer 12: e1 + up(d2) -> d1       layer 13: e0 + up(d1) -> d0
+ *   layer 14: d0 -> logits (no ReLU, FP32)
+ *
+ * Tensors: activations and their gradients are BF16 [h][w][32] (channels
+ * innermost, 64 B per pixel); level l has ceil(h / 2^l) x ceil(w / 2^l)
+ * pixels.  Weights of one layer are BF16 [9 taps][32 out][32 in] (tap =
+ * 3 * dy + dx over the 3x3 window, dy / dx = 0, 1, 2 for offsets -1, 0, +1;
+ * zero padding at the borders); the matching FP32 parameters of a layer are
+ * [9][32][32] weights followed by 32 biases (SGI_KPCN_LAYER_PARAMS floats).
+ * Matrix work runs on MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
+ *
+ * Errors and streams as sgi.h.
+ */
+#ifndef SGI_KPCN_H
+#define SGI_KPCN_H
+
+#include "sgi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGI_KPCN_CHANNELS 32
+#define SGI_KPCN_LEVELS 5
+#define SGI_KPCN_LAYERS 15
+#define SGI_KPCN_LAYER_WEIGHTS (9 * 32 * 32)
+#define SGI_KPCN_LAYER_PARAMS (SGI_KPCN_LAYER_WEIGHTS + 32)
+#define SGI_KPCN_PARAMS (SGI_KPCN_LAYERS * SGI_KPCN_LAYER_PARAMS)
+#define SGI_KPCN_TAPS 25          /* 5x5 predicted filter */
+#define SGI_KPCN_FEATURES 10      /* log radiance 3, albedo 3, normal 3, depth 1 */
+#define SGI_KPCN_TILE 16          /* fallback decision granularity (pixels) */
+
+/* A trained network in inference form. */
+typedef struct sgi_kpcn_net {
+    const uint16_t* weights;  /* device BF16 [SGI_KPCN_LAYERS][9][32][32] */
+    const float* biases;      /* device FP32 [SGI_KPCN_LAYERS][32] */
+} sgi_kpcn_net;
+
+/* ---- building blocks (training and tests) -------------------------------- */
+
+/* out = act(conv3x3(in (x) [mask > 0]) + bias): in BF16 [h][w][32]; mask NULL
+ * or BF16 [h][w][32] (the input is zeroed where mask <= 0: the ReLU gradient
+ * gate); weights BF16 [9][32][32]; bias FP32 [32] or NULL; relu 0/1; out BF16
+ * [h][w][32] (out_f32 = 0) or FP32 [h][w][32] (out_f32 = 1). */
+int sgi_kpcn_conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* weights, const float* bias,
+                     int32_t relu, int32_t height, int32_t width, void* out, int32_t out_f32, void* stream);
+
+/* Weight gradient of one layer: grad[t][o][i] = sum_p dz[p][o] * x[p + off(t)][i]
+ * and grad[9216 + o] = sum_p dz[p][o], with dz = dy (x) [mask > 0] (mask NULL:
+ * dz = dy).  x, dy, mask BF16 [h][w][32]; grad FP32 [SGI_KPCN_LAYER_PARAMS]
+ * (overwritten).  partials: device FP32 workspace of
+ * sgi_kpcn_wgrad_partials(h, w) floats. */
+size_t sgi_kpcn_wgrad_partials(int32_t height, int32_t width);
+int sgi_kpcn_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int32_t height,
+                           int32_t width, float* partials, float* grad, void* stream);
+
+/* 2x2 mean pooling: out[y][x] = mean of the in-range in[2y + i][2x + j].
+ * Backward: din[child] (+)= dout[parent] / (number of in-range children);
+ * accumulate 0 overwrites din, 1 adds to it. Sizes are the finer level's. */
+int sgi_kpcn_pool(const uint16_t* in, int32_t height, int32_t width, uint16_t* out, void* stream);
+int sgi_kpcn_pool_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* din,
+                           int32_t accumulate, void* stream);
+
+/* out = skip + up(low) (nearest): skip, out BF16 at (height, width), low at
+ * the coarser level.  Backward of the low branch: dlow[Y][X] = sum of the
+ * in-range dout[2Y + i][2X + j] (the skip branch's gradient is dout itself). */
+int sgi_kpcn_upsample_add(const uint16_t* skip, const uint16_t* low, int32_t height, int32_t width,
+                          uint16_t* out, void* stream);
+int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* dlow,
+                               void* stream);
+
+/* Network input from a gather: radiance = reservoir.rgb / max(reservoir.a,
+ * 1e-5) (FP32 [h][w][3], the a-trous filter's pass-0 normalisation,
+ * vpl_pbr_gi_denoise.fs:329) and BF16 features [h][w][32]: log(1 +
+ * radiance), albedo, decoded normal, log(1 + view distance) / 8, zeros.
+ * Needs sgi_set_frame. */
+int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const float* reservoir, uint16_t* features,
+                      float* radiance, void* stream);
+
+/* Filter application: out = sum_j softmax(logits[0..24])_j * radiance(p + o_j)
+ * over the 5x5 window (clamped to the image), err = logits[25].  logits FP32
+ * [h][w][32]; out FP32 [h][w][3]; err FP32 [h][w] or NULL. */
+int sgi_kpcn_apply(const float* logits, const float* radiance, int32_t height, int32_t width, float* out,
+                   float* err, void* stream);
+
+/* Training loss and its gradient.  With out / err as sgi_kpcn_apply,
+ * loss = mean_{p,c} (out - t)^2 / (t^2 + 0.01)
+ *      + err_weight * mean_p (err - log(r_p + 1e-4))^2,
+ * r_p = mean_c (out - t)^2 / (t^2 + 0.01) (held constant).  target FP32
+ * [h][w][3]; dlogits BF16 [h][w][32] (channels 26..31 zero); loss_sums FP32
+ * [2] accumulates the two sums (before the means).  scale multiplies the
+ * gradient (1 / batch). */
+int sgi_kpcn_loss(const float* logits, const float* radiance, const float* target, int32_t height,
+                  int32_t width, float err_weight, float scale, uint16_t* dlogits, float* loss_sums,
+                  void* stream);
+
+/* One Adam step over all SGI_KPCN_PARAMS parameters (FP32 params, grad, m, v)
+ * followed by the BF16 copies: weights [15][9][32][32], weights_t (the
+ * transposed, tap-mirrored weights of the input gradient: weights_t[l][t][i][o]
+ * = W[l][8 - t][o][i]) and biases FP32 [15][32]. */
+int sgi_kpcn_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                  float eps, int32_t step, uint16_t* weights, uint16_t* weights_t, float* biases, void* stream);
+
+/* The BF16 / FP32 copies sgi_kpcn_adam writes after its step, from params
+ * alone (initial weights, loaded weights). */
+int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weights_t, float* biases, void* stream);
+
+/* ---- inference ------------------------------------------------------------ */
+
+/* Device workspace of sgi_denoise_learned at a viewport (bytes). */
+size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width);
+
+/* The learned denoise of one frame: features -> network -> 5x5 filter of the
+ * noisy radiance; then per 16x16 tile, if the mean predicted relative squared
+ * error exp(err) of its pixels exceeds `threshold`, the tile takes the a-trous
+ * result instead (the a-trous passes run only over the flagged tiles and a
+ * 28-pixel margin, and not at all when no tile is flagged).  Outputs as
+ * sgi_denoise (single frame: history = 1, reservoirOut = (gi, 1)); prevGi /
+ * prevHistory as sgi_denoise.  fallbackTiles (device int32 [1 + tiles], may be
+ * NULL) receives the number of fallback tiles and one flag per tile.
+ * threshold <= 0 disables the network's output (every tile falls back);
+ * a threshold of +inf disables the fallback.  workspace: device memory of
+ * sgi_kpcn_workspace_bytes(); net: trained weights. */
+int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
+                        const int32_t* visibleIndices, const float* screen, const float* color,
+                        const float* reservoir, const float* prevGi, const float* prevHistory,
+                        const sgi_kpcn_net* net, float threshold, void* workspace, size_t workspaceBytes,
+                        float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                        int32_t* fallbackTiles, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGI_KPCN_H */

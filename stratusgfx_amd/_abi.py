@@ -1,4 +1,4 @@
-"""ctypes mirror of include/sgi.h and include/sgi_scene.h.
+"""ctypes mirror of include/sgi.h, include/sgi_kpcn.h and include/sgi_scene.h.
 
 The structures below are byte-for-byte the C structures (natural alignment),
 so a caller can fill them from torch device pointers or numpy host pointers.
@@ -34,6 +34,14 @@ SCENE_CORNELL = 0
 SCENE_CITY = 1
 
 VPL_FLOATS = 16  # sgi_vpl is 16 floats (64 B)
+KPCN_CHANNELS = 32
+KPCN_LEVELS = 5
+KPCN_LAYERS = 15
+KPCN_LAYER_WEIGHTS = 9 * 32 * 32
+KPCN_LAYER_PARAMS = KPCN_LAYER_WEIGHTS + 32
+KPCN_PARAMS = KPCN_LAYERS * KPCN_LAYER_PARAMS
+KPCN_TAPS = 25
+KPCN_FEATURES = 10
 
 
 class Gbuffer(ctypes.Structure):
@@ -96,6 +104,10 @@ class GatherOpts(ctypes.Structure):
 GATHER_REFERENCE, GATHER_TILES, GATHER_ALL, GATHER_SAMPLES = 0, 1, 2, 3
 
 
+class KpcnNet(ctypes.Structure):
+    _fields_ = [("weights", c_void_p), ("biases", c_void_p)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("seed", c_uint64), ("num_vpls", c_int32),
                 ("num_city_boxes", c_int32), ("city_half_extent", c_float),
@@ -129,6 +141,22 @@ SGI_SIGNATURES = {
     "sgi_denoise": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               _P]),
     "sgi_render_frame": (c_int32, [_P, POINTER(FrameIO), _P]),
+    # include/sgi_kpcn.h
+    "sgi_kpcn_conv3x3": (c_int32, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
+    "sgi_kpcn_wgrad_partials": (ctypes.c_size_t, [_I, _I]),
+    "sgi_kpcn_conv3x3_wgrad": (c_int32, [_P, _P, _P, _I, _I, _P, _P, _P]),
+    "sgi_kpcn_pool": (c_int32, [_P, _I, _I, _P, _P]),
+    "sgi_kpcn_pool_backward": (c_int32, [_P, _I, _I, _P, _I, _P]),
+    "sgi_kpcn_upsample_add": (c_int32, [_P, _P, _I, _I, _P, _P]),
+    "sgi_kpcn_upsample_backward": (c_int32, [_P, _I, _I, _P, _P]),
+    "sgi_kpcn_features": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, _P]),
+    "sgi_kpcn_apply": (c_int32, [_P, _P, _I, _I, _P, _P, _P]),
+    "sgi_kpcn_loss": (c_int32, [_P, _P, _P, _I, _I, c_float, c_float, _P, _P, _P]),
+    "sgi_kpcn_adam": (c_int32, [_P, _P, _P, _P, c_float, c_float, c_float, c_float, _I, _P, _P, _P, _P]),
+    "sgi_kpcn_pack_weights": (c_int32, [_P, _P, _P, _P, _P]),
+    "sgi_kpcn_workspace_bytes": (ctypes.c_size_t, [_I, _I]),
+    "sgi_denoise_learned": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P,
+                                      POINTER(KpcnNet), c_float, _P, ctypes.c_size_t, _P, _P, _P, _P, _P, _P]),
 }
 SCENE_SIGNATURES = {
     "sgi_scene_create": (c_int32, [POINTER(SceneDesc), POINTER(_P)]),
@@ -149,10 +177,11 @@ HOST_LIB_SYMBOLS = [n for n in SCENE_SIGNATURES if not n.endswith("_device")] + 
 
 
 def header_functions(path: str) -> list[str]:
-    """Names of the functions a public header declares (`int name(` / `void name(` / `const char* name(`)."""
+    """Names of the functions a public header declares (`int name(`, `void name(`, `size_t name(`,
+    `const char* name(`)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return re.findall(r"^\s*(?:int|void|const char\*)\s+(sgi_\w+)\s*\(", text, flags=re.M)
+    return re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(sgi_\w+)\s*\(", text, flags=re.M)
 
 
 class SgiError(RuntimeError):

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/sgi.h"
+#include "../../include/sgi_kpcn.h"
 #include "launch.h"
 
 void sgi_set_error(const std::string& msg);
@@ -301,6 +302,251 @@ int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream) {
         return r;
     return sgi_denoise(ctx, io->gbuffer, io->prevGbuffer, io->visibleIndices, io->direct, io->color, io->reservoir,
                        io->prevGi, io->prevHistory, io->combined, io->giOut, io->reservoirOut, io->historyOut, stream);
+}
+
+/* ---- learned denoiser ------------------------------------------------------ */
+
+static int dims(int32_t h, int32_t w, const char* who) {
+    if (h <= 0 || w <= 0 || h > 16384 || w > 16384)
+        return fail(SGI_E_INVALID, std::string(who) + ": size must be within 1..16384 on each side");
+    return SGI_OK;
+}
+
+int sgi_kpcn_conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* weights, const float* bias,
+                     int32_t relu, int32_t height, int32_t width, void* out, int32_t out_f32, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_conv3x3");
+    if (r != SGI_OK) return r;
+    if (!in || !weights || !out) return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3: NULL buffer");
+    if (!aligned16(in) || !aligned16(weights) || !aligned16(out) || (mask && !aligned16(mask)))
+        return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3: buffers must be 16-byte aligned");
+    const hipError_t e = sgi_kpcn::conv3x3(in, mask, weights, bias, relu ? 1 : 0, height, width, out, out_f32 ? 1 : 0,
+                                           (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_conv3x3");
+}
+
+size_t sgi_kpcn_wgrad_partials(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0) return 0;
+    return (size_t)sgi_kpcn::wgradBlocks(height, width) * SGI_KPCN_LAYER_PARAMS;
+}
+
+int sgi_kpcn_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int32_t height,
+                           int32_t width, float* partials, float* grad, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_conv3x3_wgrad");
+    if (r != SGI_OK) return r;
+    if (!x || !dy || !partials || !grad) return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3_wgrad: NULL buffer");
+    if (!aligned16(x) || !aligned16(dy) || (mask && !aligned16(mask)))
+        return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3_wgrad: buffers must be 16-byte aligned");
+    const hipError_t e = sgi_kpcn::wgrad(x, dy, mask, height, width, partials, grad, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_conv3x3_wgrad");
+}
+
+int sgi_kpcn_pool(const uint16_t* in, int32_t height, int32_t width, uint16_t* out, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_pool");
+    if (r != SGI_OK) return r;
+    if (!in || !out || !aligned16(in) || !aligned16(out))
+        return fail(SGI_E_INVALID, "sgi_kpcn_pool: NULL or unaligned buffer");
+    const hipError_t e = sgi_kpcn::pool(in, height, width, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_pool");
+}
+
+int sgi_kpcn_pool_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* din, int32_t accumulate,
+                           void* stream) {
+    int r = dims(height, width, "sgi_kpcn_pool_backward");
+    if (r != SGI_OK) return r;
+    if (!dout || !din || !aligned16(dout) || !aligned16(din))
+        return fail(SGI_E_INVALID, "sgi_kpcn_pool_backward: NULL or unaligned buffer");
+    const hipError_t e = sgi_kpcn::poolBackward(dout, height, width, din, accumulate ? 1 : 0, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_pool_backward");
+}
+
+int sgi_kpcn_upsample_add(const uint16_t* skip, const uint16_t* low, int32_t height, int32_t width, uint16_t* out,
+                          void* stream) {
+    int r = dims(height, width, "sgi_kpcn_upsample_add");
+    if (r != SGI_OK) return r;
+    if (!skip || !low || !out || !aligned16(skip) || !aligned16(low) || !aligned16(out))
+        return fail(SGI_E_INVALID, "sgi_kpcn_upsample_add: NULL or unaligned buffer");
+    const hipError_t e = sgi_kpcn::upAdd(skip, low, height, width, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_upsample_add");
+}
+
+int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* dlow, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_upsample_backward");
+    if (r != SGI_OK) return r;
+    if (!dout || !dlow || !aligned16(dout) || !aligned16(dlow))
+        return fail(SGI_E_INVALID, "sgi_kpcn_upsample_backward: NULL or unaligned buffer");
+    const hipError_t e = sgi_kpcn::upBackward(dout, height, width, dlow, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_upsample_backward");
+}
+
+int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* g, const float* reservoir, uint16_t* features, float* radiance,
+                      void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_kpcn_features: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_kpcn_features: sgi_set_frame first");
+    int r = checkGbuffer(ctx, g, "sgi_kpcn_features", true);
+    if (r != SGI_OK) return r;
+    if (!reservoir || !features || !radiance || !aligned16(reservoir) || !aligned16(features))
+        return fail(SGI_E_INVALID, "sgi_kpcn_features: NULL or unaligned buffer");
+    const hipError_t e =
+        sgi_kpcn::features(sgi_dev::gbufArgs(*g), ctx->frameArgs, reservoir, features, radiance, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_features");
+}
+
+int sgi_kpcn_apply(const float* logits, const float* radiance, int32_t height, int32_t width, float* out, float* err,
+                   void* stream) {
+    int r = dims(height, width, "sgi_kpcn_apply");
+    if (r != SGI_OK) return r;
+    if (!logits || !radiance || !out || !aligned16(logits)) return fail(SGI_E_INVALID, "sgi_kpcn_apply: bad buffer");
+    const hipError_t e = sgi_kpcn::apply(logits, radiance, height, width, out, err, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_apply");
+}
+
+int sgi_kpcn_loss(const float* logits, const float* radiance, const float* target, int32_t height, int32_t width,
+                  float err_weight, float scale, uint16_t* dlogits, float* loss_sums, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_loss");
+    if (r != SGI_OK) return r;
+    if (!logits || !radiance || !target || !dlogits || !loss_sums || !aligned16(logits) || !aligned16(dlogits))
+        return fail(SGI_E_INVALID, "sgi_kpcn_loss: bad buffer");
+    const hipError_t e = sgi_kpcn::loss(logits, radiance, target, height, width, err_weight, scale, dlogits,
+                                        loss_sums, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_loss");
+}
+
+int sgi_kpcn_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2, float eps,
+                  int32_t step, uint16_t* weights, uint16_t* weights_t, float* biases, void* stream) {
+    if (!params || !grad || !m || !v) return fail(SGI_E_INVALID, "sgi_kpcn_adam: NULL buffer");
+    if (step < 1) return fail(SGI_E_INVALID, "sgi_kpcn_adam: step counts from 1");
+    const hipError_t e = sgi_kpcn::adam(params, grad, m, v, lr, beta1, beta2, eps, step, weights, weights_t, biases,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_adam");
+}
+
+int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weights_t, float* biases, void* stream) {
+    if (!params) return fail(SGI_E_INVALID, "sgi_kpcn_pack_weights: NULL parameters");
+    const hipError_t e = sgi_kpcn::packWeights(params, weights, weights_t, biases, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_pack_weights");
+}
+
+}  // extern "C"
+
+namespace {
+
+// The inference workspace: per level l three activation planes (e_l, and two
+// temporaries), then the level-0 features, radiance, logits, filtered output,
+// error and the tile flags.
+struct KpcnWs {
+    uint16_t* act[SGI_KPCN_LEVELS][3];
+    uint16_t* feat;
+    float *radiance, *logits, *out, *err;
+    uint8_t *flags, *dilated;
+    int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
+};
+
+size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    int h = H, w = W;
+    for (int l = 0; l < SGI_KPCN_LEVELS; ++l) {
+        if (ws) ws->h[l] = h, ws->w[l] = w;
+        for (int k = 0; k < 3; ++k) {
+            char* p = take((size_t)h * w * SGI_KPCN_CHANNELS * 2);
+            if (ws) ws->act[l][k] = (uint16_t*)p;
+        }
+        h = (h + 1) / 2;
+        w = (w + 1) / 2;
+    }
+    const size_t P = (size_t)H * W;
+    const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+    char* feat = take(P * SGI_KPCN_CHANNELS * 2);
+    char* rad = take(P * 3 * 4);
+    char* logits = take(P * SGI_KPCN_CHANNELS * 4);
+    char* out = take(P * 3 * 4);
+    char* err = take(P * 4);
+    char* flags = take(tiles);
+    char* dil = take(tiles);
+    if (ws) {
+        ws->feat = (uint16_t*)feat;
+        ws->radiance = (float*)rad;
+        ws->logits = (float*)logits;
+        ws->out = (float*)out;
+        ws->err = (float*)err;
+        ws->flags = (uint8_t*)flags;
+        ws->dilated = (uint8_t*)dil;
+    }
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0 || height > 16384 || width > 16384) return 0;
+    return kpcnLayout(height, width, nullptr, nullptr) + 256;
+}
+
+int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev, const int32_t* visibleIndices,
+                        const float* screen, const float* color, const float* reservoir, const float* prevGi,
+                        const float* prevHistory, const sgi_kpcn_net* net, float threshold, void* workspace,
+                        size_t workspaceBytes, float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                        int32_t* fallbackTiles, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_denoise_learned: sgi_set_frame first");
+    int r = checkGbuffer(ctx, cur, "sgi_denoise_learned", true);
+    if (r != SGI_OK) return r;
+    if (prev && (r = checkGbuffer(ctx, prev, "sgi_denoise_learned(prev)", false)) != SGI_OK) return r;
+    if (!visibleIndices || !screen || !color || !reservoir || !combined || !giOut || !reservoirOut || !historyOut)
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL buffer");
+    if (!net || !net->weights || !net->biases || !aligned16(net->weights))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: network weights missing or unaligned");
+    if (!aligned16(reservoir) || !aligned16(reservoirOut))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: reservoirs must be 16-byte aligned");
+    const int H = cur->height, W = cur->width;
+    if (!workspace || workspaceBytes < sgi_kpcn_workspace_bytes(H, W))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: workspace smaller than sgi_kpcn_workspace_bytes()");
+    char* base = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    KpcnWs ws;
+    kpcnLayout(H, W, base, &ws);
+    hipStream_t s = (hipStream_t)stream;
+    const sgi_dev::GbufArgs c = sgi_dev::gbufArgs(*cur);
+    const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
+    auto Wl = [&](int l) { return net->weights + (size_t)l * SGI_KPCN_LAYER_WEIGHTS; };
+    auto Bl = [&](int l) { return net->biases + (size_t)l * SGI_KPCN_CHANNELS; };
+    hipError_t e = sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.radiance, s);
+#define KPCN_STEP(call) \
+    if (e == hipSuccess) e = (call)
+    // encoder: level 0 from the features, then pool -> conv -> conv per level
+    KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
+    KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][1], nullptr, Wl(1), Bl(1), 1, ws.h[0], ws.w[0], ws.act[0][0], 0, s));
+    for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
+        KPCN_STEP(sgi_kpcn::pool(ws.act[l - 1][0], ws.h[l - 1], ws.w[l - 1], ws.act[l][1], s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][2], 0,
+                                    s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
+                                    ws.act[l][0], 0, s));
+    }
+    // decoder: d_l = conv(e_l + up(d_{l+1})), d_4 = e_4; d_l lands in act[l][2]
+    const uint16_t* low = ws.act[SGI_KPCN_LEVELS - 1][0];
+    for (int l = SGI_KPCN_LEVELS - 2, layer = 10; l >= 0; --l, ++layer) {
+        KPCN_STEP(sgi_kpcn::upAdd(ws.act[l][0], low, ws.h[l], ws.w[l], ws.act[l][1], s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(layer), Bl(layer), 1, ws.h[l], ws.w[l], ws.act[l][2], 0,
+                                    s));
+        low = ws.act[l][2];
+    }
+    KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][2], nullptr, Wl(14), Bl(14), 0, H, W, ws.logits, 1, s));
+    KPCN_STEP(sgi_kpcn::apply(ws.logits, ws.radiance, H, W, ws.out, ws.err, s));
+    KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
+    // a-trous over the flagged tiles (and their margin), the network elsewhere
+    KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
+                                         ws.dilated, s));
+    KPCN_STEP(sgi_kpcn::select(ws.out, ws.flags, H, W, reservoirOut, s));
+    KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
+                                          combined, giOut, historyOut, ctx->ws, s));
+#undef KPCN_STEP
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise_learned");
 }
 
 }  // extern "C"

@@ -51,9 +51,16 @@ __device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C,
     return t[((size_t)j * W + i) * C + c];
 }
 
+// mask (per 16x16 tile, may be NULL): pixels of unmarked tiles are skipped
+// (the decoded normals are still written: the temporal pass reads them all).
+__device__ __forceinline__ bool maskedOut(const uint8_t* mask, int W, int x, int y) {
+    return mask && !mask[(y >> 4) * ((W + 15) >> 4) + (x >> 4)];
+}
+
 __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
                                                           const float4* __restrict__ resIn, float4* __restrict__ resOut,
-                                                          float4* __restrict__ normals) {
+                                                          float4* __restrict__ normals,
+                                                          const uint8_t* __restrict__ mask) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
     if (x >= g.W || y >= g.H) return;
@@ -61,6 +68,7 @@ __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs 
     const f3 centerNormal = decodeNormal(g.normal + 3 * p);
     const float centerDepth = g.depth[p];
     normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, centerDepth);
+    if (maskedOut(mask, g.W, x, y)) return;
     const int numVisible = visible[0];
     float cr0 = 0.0f, cr1 = 0.0f, cr2 = 0.0f, cr3 = 0.0f;
     const float depthCutoff = 0.1f * centerDepth;
@@ -87,10 +95,11 @@ __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs 
 
 __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
                                                            const float4* __restrict__ normals,
-                                                           float4* __restrict__ resOut) {
+                                                           float4* __restrict__ resOut, const uint8_t* __restrict__ mask) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
     if (x >= g.W || y >= g.H) return;
+    if (maskedOut(mask, g.W, x, y)) return;
     const int W = g.W, H = g.H;
     const size_t p = (size_t)y * W + x;
     const float4 cn4 = normals[p];
@@ -186,24 +195,39 @@ __global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArg
 
 namespace sgi_launch {
 
+hipError_t denoiseSpatial(const GbufArgs& cur, const FrameArgs& fr, const int32_t* visible, const float* reservoir,
+                          float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask, hipStream_t s) {
+    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
+    hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
+                       ws.normals, mask);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2, ws.resA, ws.normals, ws.resB, mask);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 4, ws.resB, ws.normals, ws.resA, mask);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 8, ws.resA, ws.normals, (float4*)reservoirOut,
+                       mask);
+    return hipGetLastError();
+}
+
+hipError_t denoiseTemporal(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const float* screen,
+                           const float* color, const float* filtered, const float* prevGi, const float* prevHistory,
+                           float* combined, float* giOut, float* historyOut, const DenoiseBuffers& ws, hipStream_t s) {
+    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
+    hipLaunchKernelGGL(denoiseTemporalKernel, grid, block, 0, s, cur, prev, fr, (const float4*)filtered, ws.normals,
+                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    return hipGetLastError();
+}
+
 hipError_t denoise(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const int32_t* visible,
                    const float* screen, const float* color, const float* reservoir, const float* prevGi,
                    const float* prevHistory, float* combined, float* giOut, float* reservoirOut, float* historyOut,
                    const DenoiseBuffers& ws, hipStream_t s) {
-    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
-    hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
-                       ws.normals);
-    hipError_t e = hipGetLastError();
+    const hipError_t e = denoiseSpatial(cur, fr, visible, reservoir, reservoirOut, ws, nullptr, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2, ws.resA, ws.normals, ws.resB);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 4, ws.resB, ws.normals, ws.resA);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 8, ws.resA, ws.normals, (float4*)reservoirOut);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseTemporalKernel, grid, block, 0, s, cur, prev, fr, (const float4*)reservoirOut, ws.normals,
-                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
-    return hipGetLastError();
+    return denoiseTemporal(cur, prev, fr, screen, color, reservoirOut, prevGi, prevHistory, combined, giOut,
+                           historyOut, ws, s);
 }
 
 }  // namespace sgi_launch

@@ -101,16 +101,23 @@ __device__ __attribute__((noinline)) bool shadowedExact(float d2, float m, float
     return (currentDepth - currentDepth * m) > depth;
 }
 
-// One VPL's contribution (the body of the sample loop, vpl_pbr_gi.fs:149-199).
-// Returns false when the side / radius test rejects it.
+// The visibility half of one VPL sample (vpl_pbr_gi.fs:149-199 up to the
+// lighting call): the side and radius tests, the shadow-cube compare and the
+// reweighting.  Returns false when the side / radius test rejects the VPL;
+// otherwise adds the reweighting factor to validSamples and returns |L - P|,
+// dot(N, L - P) and the lighting's scale (1 - shadow) * ssao * fog.
 //
 // |L - P| comes from the hardware sqrt (within a few ulps); each comparison
 // that uses it is taken from that estimate only when it clears a margin of
 // 2^-18 relative, far above the estimate's error, and otherwise from the
 // correctly rounded length, so every branch matches the restatement.
+struct Hit {
+    float dist, nl, factor;
+};
+
 template <int FMT>
-__device__ __forceinline__ bool evalVpl(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
-                                        bool allowReject, f3& vplColor, float& validSamples) {
+__device__ __forceinline__ bool visibility(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
+                                           bool allowReject, float& validSamples, Hit& h) {
     const f3 lmf = L.pos - px.P;
     const float d2 = dot(lmf, lmf);
     const float dist = fsqrt(d2);
@@ -144,8 +151,134 @@ __device__ __forceinline__ bool evalVpl(const Pixel& px, const FrameArgs& fr, co
     float reweightingFactor = 1.0f;
     if (shadowFactor > 0.0f) reweightingFactor = (1.0f - distanceRatio) * fr.minGiOcc + distanceRatio;
     validSamples += reweightingFactor;
-    vplColor = fmaF(vplLighting(px, L, dist, nl), (1.0f - shadowFactor) * px.ao * px.fog, vplColor);
+    h.dist = dist;
+    h.nl = nl;
+    h.factor = (1.0f - shadowFactor) * px.ao * px.fog;
     return true;
+}
+
+// One VPL's whole contribution in FP32.  Returns false when rejected.
+template <int FMT>
+__device__ __forceinline__ bool evalVpl(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
+                                        bool allowReject, f3& vplColor, float& validSamples) {
+    Hit h;
+    if (!visibility<FMT>(px, fr, cubes, L, allowReject, validSamples, h)) return false;
+    vplColor = fmaF(vplLighting(px, L, h.dist, h.nl), h.factor, vplColor);
+    return true;
+}
+
+// ---- packed-FP16 shading (SGI_PRECISION_FP16) ------------------------------
+//
+// The visibility half stays in FP32 with the exact decisions above, so the
+// accepted samples, the shadow results and validSamples are those of the FP32
+// path.  The lighting of two VPLs at a time runs in packed FP16 (one VPL per
+// half of a register: v_pk_fma_f16 / v_pk_mul_f16 / v_pk_max_f16), and each
+// product is accumulated into the FP32 colour with mixed-precision FMAs.
+// Distances do not fit FP16 (|L - P|^2 reaches 2.5e5 and the attenuation falls
+// to ~1e-6), so the attenuation is evaluated in a scaled form:
+//   vplDiffuseAttenuation(d, R) = (64 / R^2) / (m(u) * 64 / R + u^2),  u = 8 d / R,
+// with m = mix(10, 1, exp(-3 min(u / 8, 1))); the FP16 factor is at most R / 64
+// and the 64 / R^2 goes into the FP32 colour scale.  The FP16 form therefore
+// assumes VPL radii up to 16384 (the products stay below the FP16 maximum).
+typedef _Float16 h1;
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2 hpair(float a, float b) { return h2{(h1)a, (h1)b}; }
+__device__ __forceinline__ h2 hs(float a) { return h2{(h1)a, (h1)a}; }
+__device__ __forceinline__ h2 hfma(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
+__device__ __forceinline__ h2 hmax(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ h2 hmin(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ h2 hsat(h2 a) { return hmin(hmax(a, hs(0.0f)), hs(1.0f)); }
+__device__ __forceinline__ h2 hrcp(h2 a) { return h2{(h1)__builtin_amdgcn_rcph(a.x), (h1)__builtin_amdgcn_rcph(a.y)}; }
+__device__ __forceinline__ h2 hrsq(h2 a) { return h2{(h1)__builtin_amdgcn_rsqh(a.x), (h1)__builtin_amdgcn_rsqh(a.y)}; }
+__device__ __forceinline__ h2 hsqrt(h2 a) { return __builtin_elementwise_sqrt(a); }
+__device__ __forceinline__ h2 hexp2(h2 a) { return __builtin_elementwise_exp2(a); }
+
+struct PixelH {  // the pixel's shading inputs, both halves equal
+    h2 nx, ny, nz, vx, vy, vz;
+    h2 nv, ndotv, r2, r2m1, gv;
+    h2 f0r, f0g, f0b, dr, dg, db;
+};
+
+struct LightH {  // one VPL's FP32 inputs to the packed stage
+    float lx, ly, lz;  // unit specular direction
+    float uD, uS;      // 8 |L - P| / R and 8 |S - P| / R
+    float cosD;        // dot(N, L - P) / |L - P|
+    float invR64;      // 64 / R
+    f3 col;            // colour * (1 - shadow) * ssao * fog * 64 / R^2 (0 for no light)
+};
+
+__device__ __forceinline__ LightH noLightH() {
+    LightH o;
+    o.lx = 0.0f; o.ly = 0.0f; o.lz = 1.0f;
+    o.uD = 0.0f; o.uS = 0.0f; o.cosD = 0.0f; o.invR64 = 1.0f;
+    o.col = splat(0.0f);
+    return o;
+}
+
+__device__ __forceinline__ PixelH toHalf(const Pixel& px) {
+    PixelH q;
+    q.nx = hs(px.N.x); q.ny = hs(px.N.y); q.nz = hs(px.N.z);
+    q.vx = hs(px.V.x); q.vy = hs(px.V.y); q.vz = hs(px.V.z);
+    q.nv = hs(px.nvRaw); q.ndotv = hs(px.NdotV); q.r2 = hs(px.r2); q.r2m1 = hs(px.r2m1); q.gv = hs(px.gv);
+    q.f0r = hs(px.f0.x); q.f0g = hs(px.f0.y); q.f0b = hs(px.f0.z);
+    q.dr = hs(px.diffColor.x); q.dg = hs(px.diffColor.y); q.db = hs(px.diffColor.z);
+    return q;
+}
+
+__device__ __forceinline__ LightH prepHalf(const Pixel& px, const Vpl& L, const Hit& h) {
+    const f3 specDir = L.spec - px.P;
+    const float s2 = dotF(specDir, specDir);
+    const float sInv = frsq(s2);
+    LightH o;
+    o.lx = specDir.x * sInv; o.ly = specDir.y * sInv; o.lz = specDir.z * sInv;
+    const float invR8 = 8.0f * L.invRadius;
+    o.uD = h.dist * invR8;
+    o.uS = gmin((s2 * sInv) * invR8, 240.0f);
+    o.cosD = h.nl * frcp(h.dist);
+    o.invR64 = 64.0f * L.invRadius;
+    o.col = L.color * (h.factor * L.invRadius * o.invR64);
+    return o;
+}
+
+// The lighting of two VPLs (pbr2.glsl:51-76, 541-574, 613-657 and pbr.glsl:260-267,
+// rearranged as in vplLighting), accumulated into acc.
+__device__ __forceinline__ void shadePairH(const PixelH& q, const LightH& a, const LightH& b, f3& acc) {
+    const h2 lx = hpair(a.lx, b.lx), ly = hpair(a.ly, b.ly), lz = hpair(a.lz, b.lz);
+    const h2 nlRaw = hfma(q.nz, lz, hfma(q.ny, ly, q.nx * lx));
+    const h2 lv = hfma(q.vz, lz, hfma(q.vy, ly, q.vx * lx));
+    // |V + Ls|^2 = 2 + 2 V.Ls, kept >= 2^-14 so that H stays finite
+    const h2 hInv = hrsq(hmax(hfma(hs(2.0f), lv, hs(2.0f)), hs(0x1p-14f)));
+    const h2 NdotL = hsat(nlRaw);
+    const h2 NdotH = hsat((q.nv + nlRaw) * hInv);
+    const h2 LdotH = hsat((hs(1.0f) + lv) * hInv);
+    const h2 f = hfma(NdotH * q.r2m1, NdotH, hs(1.0f));
+    const h2 ndf = q.r2 * hrcp(hmax(hs(kPI) * f * f, hs(kEPS)));
+    const h2 ggxl = q.ndotv * hsqrt(hfma(NdotL * -q.r2m1, NdotL, q.r2));
+    // Vis * N.L <= 2: the product is formed before it meets the NDF
+    const h2 visNl = (hs(0.5f) * NdotL) * hrcp(hmax(hfma(NdotL, q.gv, ggxl), hs(kEPS)));
+    const h2 specT = ndf * visNl;
+    const h2 t = hs(1.0f) - LdotH;
+    const h2 t2 = t * t;
+    const h2 fr5 = t2 * t2 * t;
+    const h2 omf = hs(1.0f) - fr5;
+    const h2 uD = hpair(a.uD, b.uD), uS = hpair(a.uS, b.uS), iR = hpair(a.invR64, b.invR64);
+    const h2 kE = hs(-3.0f * 1.44269504088896f / 8.0f);
+    const h2 mD = hfma(hs(-9.0f), hexp2(hmin(uD, hs(8.0f)) * kE), hs(10.0f));
+    const h2 mS = hfma(hs(-9.0f), hexp2(hmin(uS, hs(8.0f)) * kE), hs(10.0f));
+    const h2 AD = hrcp(hfma(mD, iR, uD * uD));
+    const h2 AS = hrcp(hfma(mS, iR, uS * uS));
+    const h2 diff = hsat(hpair(a.cosD, b.cosD)) * hs(1.0f / kPI) * AD;
+    const h2 spec = specT * AS;
+    const h2 wr = hfma(q.dr, diff, hfma(q.f0r, omf, fr5) * spec);
+    const h2 wg = hfma(q.dg, diff, hfma(q.f0g, omf, fr5) * spec);
+    const h2 wb = hfma(q.db, diff, hfma(q.f0b, omf, fr5) * spec);
+    acc.x = __builtin_fmaf((float)wr.x, a.col.x, acc.x);
+    acc.y = __builtin_fmaf((float)wg.x, a.col.y, acc.y);
+    acc.z = __builtin_fmaf((float)wb.x, a.col.z, acc.z);
+    acc.x = __builtin_fmaf((float)wr.y, b.col.x, acc.x);
+    acc.y = __builtin_fmaf((float)wg.y, b.col.y, acc.y);
+    acc.z = __builtin_fmaf((float)wb.y, b.col.z, acc.z);
 }
 
 __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& fr, int x, int y, size_t p) {
@@ -193,7 +326,7 @@ __device__ __forceinline__ void writeOut(const GbufArgs& g, size_t p, f3 vplColo
 
 constexpr int kMaxK = SGI_MAX_VPLS_PER_TILE_LIMIT;
 
-template <int FMT, int TS>
+template <int FMT, int TS, bool HALF>
 __global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
                                                             int nTiles, int K, const sgi_vpl* __restrict__ vpls,
                                                             const sgi_atlas_entry* __restrict__ entries,
@@ -220,11 +353,24 @@ __global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameAr
     f3 vplColor = splat(0.0f);
     float validSamples = 0.0f;
     const int n = ln;
-    for (int k = 0; k < n; ++k) evalVpl<FMT>(px, fr, cubes, lv[k], true, vplColor, validSamples);
+    if constexpr (HALF) {
+        // list positions k and k + 1 share one packed evaluation
+        const PixelH q = toHalf(px);
+        for (int k = 0; k < n; k += 2) {
+            LightH a = noLightH(), b = noLightH();
+            Hit h;
+            if (visibility<FMT>(px, fr, cubes, lv[k], true, validSamples, h)) a = prepHalf(px, lv[k], h);
+            if (k + 1 < n && visibility<FMT>(px, fr, cubes, lv[k + 1], true, validSamples, h))
+                b = prepHalf(px, lv[k + 1], h);
+            shadePairH(q, a, b, vplColor);
+        }
+    } else {
+        for (int k = 0; k < n; ++k) evalVpl<FMT>(px, fr, cubes, lv[k], true, vplColor, validSamples);
+    }
     writeOut(g, p, vplColor, validSamples, color, reservoir);
 }
 
-template <int FMT>
+template <int FMT, bool HALF>
 __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
                                                          int nTiles, const sgi_vpl* __restrict__ vpls,
                                                          const int32_t* __restrict__ visible,
@@ -250,15 +396,39 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
         const int maxRandomIndex = numVisible - 1;
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
         float seedZ = fr.time;
-        for (int i = 0, resamples = 0; i < samplesMax; i += 1) {
-            const float rnd = random3(seedX, seedY, seedZ);
-            seedZ += 10000.0f;
-            const int lightIndex = (int)((float)maxRandomIndex * rnd);
-            const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
-            if (!evalVpl<FMT>(px, fr, cubes, L, resamples < 5, vplColor, validSamples)) {
-                ++resamples;
-                --i;
-                continue;
+        if constexpr (HALF) {
+            // accepted samples are paired in the order drawn
+            const PixelH q = toHalf(px);
+            LightH pending = noLightH();
+            bool hasPending = false;
+            for (int i = 0, resamples = 0; i < samplesMax; i += 1) {
+                const float rnd = random3(seedX, seedY, seedZ);
+                seedZ += 10000.0f;
+                const int lightIndex = (int)((float)maxRandomIndex * rnd);
+                const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
+                Hit h;
+                if (!visibility<FMT>(px, fr, cubes, L, resamples < 5, validSamples, h)) {
+                    ++resamples;
+                    --i;
+                    continue;
+                }
+                const LightH cur = prepHalf(px, L, h);
+                if (hasPending) shadePairH(q, pending, cur, vplColor);
+                else pending = cur;
+                hasPending = !hasPending;
+            }
+            if (hasPending) shadePairH(q, pending, noLightH(), vplColor);
+        } else {
+            for (int i = 0, resamples = 0; i < samplesMax; i += 1) {
+                const float rnd = random3(seedX, seedY, seedZ);
+                seedZ += 10000.0f;
+                const int lightIndex = (int)((float)maxRandomIndex * rnd);
+                const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
+                if (!evalVpl<FMT>(px, fr, cubes, L, resamples < 5, vplColor, validSamples)) {
+                    ++resamples;
+                    --i;
+                    continue;
+                }
             }
         }
     }
@@ -369,34 +539,46 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
     return hipGetLastError();
 }
 
-template <int FMT>
+template <int FMT, bool HALF>
 hipError_t launchFmt(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int tileSize, int tilesX,
                      int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
                      const int32_t* tiles, float* color, float* reservoir, hipStream_t s) {
     if (!tiles) {
         const int rx = (g.W + 15) / 16, ry = (g.H + 15) / 16;
-        hipLaunchKernelGGL(gatherRandomKernel<FMT>, dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry, vpls,
-                           visible, entries, color, reservoir);
+        hipLaunchKernelGGL((gatherRandomKernel<FMT, HALF>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry,
+                           vpls, visible, entries, color, reservoir);
         return hipGetLastError();
     }
     const int nTiles = tilesX * tilesY;
     switch (tileSize) {
         case 8:
-            hipLaunchKernelGGL((gatherTilesKernel<FMT, 8>), dim3(nTiles), dim3(64), 0, s, g, fr, cubes, tilesX, nTiles,
-                               K, vpls, entries, tiles, color, reservoir);
+            hipLaunchKernelGGL((gatherTilesKernel<FMT, 8, HALF>), dim3(nTiles), dim3(64), 0, s, g, fr, cubes, tilesX,
+                               nTiles, K, vpls, entries, tiles, color, reservoir);
             break;
         case 16:
-            hipLaunchKernelGGL((gatherTilesKernel<FMT, 16>), dim3(nTiles), dim3(256), 0, s, g, fr, cubes, tilesX,
+            hipLaunchKernelGGL((gatherTilesKernel<FMT, 16, HALF>), dim3(nTiles), dim3(256), 0, s, g, fr, cubes, tilesX,
                                nTiles, K, vpls, entries, tiles, color, reservoir);
             break;
         case 32:
-            hipLaunchKernelGGL((gatherTilesKernel<FMT, 32>), dim3(nTiles), dim3(1024), 0, s, g, fr, cubes, tilesX,
-                               nTiles, K, vpls, entries, tiles, color, reservoir);
+            hipLaunchKernelGGL((gatherTilesKernel<FMT, 32, HALF>), dim3(nTiles), dim3(1024), 0, s, g, fr, cubes,
+                               tilesX, nTiles, K, vpls, entries, tiles, color, reservoir);
             break;
         default:
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int FMT>
+hipError_t launchPrecision(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int tileSize, int tilesX,
+                           int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible,
+                           const sgi_atlas_entry* entries, const int32_t* tiles, int precision, float* color,
+                           float* reservoir, hipStream_t s) {
+    if (precision == SGI_PRECISION_FP16)
+        return launchFmt<FMT, true>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
+                                    reservoir, s);
+    return launchFmt<FMT, false>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
+                                 reservoir, s);
 }
 
 }  // namespace
@@ -406,12 +588,12 @@ namespace sgi_launch {
 hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int tileSize, int tilesX,
                   int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
                   const int32_t* tiles, int precision, float* color, float* reservoir, hipStream_t s) {
-    if (precision != SGI_PRECISION_FP32) return hipErrorNotSupported;
+    if (precision != SGI_PRECISION_FP32 && precision != SGI_PRECISION_FP16) return hipErrorNotSupported;
     if (format == SGI_TEXEL_F16)
-        return launchFmt<SGI_TEXEL_F16>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles,
-                                        color, reservoir, s);
-    return launchFmt<SGI_TEXEL_F32>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
-                                    reservoir, s);
+        return launchPrecision<SGI_TEXEL_F16>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries,
+                                              tiles, precision, color, reservoir, s);
+    return launchPrecision<SGI_TEXEL_F32>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles,
+                                          precision, color, reservoir, s);
 }
 
 hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int mode, int samples,

@@ -1,0 +1,731 @@
+// Learned denoiser: the kernel-predicting encoder-decoder network's kernels.
+//
+// The convolutions are implicit GEMMs on MFMA (v_mfma_f32_16x16x32_bf16,
+// BF16 operands, FP32 accumulation).  One MFMA multiplies a 16 x 32 block of
+// weights (16 output channels x 32 input channels of one tap) by a 32 x 16
+// block of the input (32 channels x 16 pixels of one image row), so the
+// channels are the reduction dimension and each lane reads 8 consecutive
+// channels (16 B) of one pixel.  The result lane layout puts 4 consecutive
+// output channels of one pixel in each lane, which the epilogue stores as 8 B
+// (BF16) or 16 B (FP32).
+//
+//   conv3x3Kernel  a 256-thread block computes 16x16 output pixels x 32
+//                  channels: the 18x18 input halo is staged once in LDS, the
+//                  wave's 4 rows x 2 channel blocks (8 accumulators) run the
+//                  9 taps; the layer's weights (9 taps x 2 channel blocks,
+//                  72 VGPRs) stay in registers while the block walks its
+//                  contiguous run of tiles.
+//   wgradKernel    weight gradient, reduction over pixels: the gradient tile
+//                  and three column-shifted copies of the input halo are
+//                  staged transposed (channel-major, 8 pixels per 16 B) so the
+//                  pixel dimension becomes the MFMA reduction; per-block
+//                  partial sums are reduced in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include "launch.h"
+#include "../../include/sgi_kpcn.h"
+
+namespace {
+
+using namespace sgi_dev;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kC = SGI_KPCN_CHANNELS;
+constexpr int kTS = 16;       // output tile edge
+constexpr int kHS = kTS + 2;  // halo tile edge
+constexpr int kPxU4 = 5;      // LDS uint4 slots per staged pixel: 4 used, 1 pad (bank spread)
+constexpr int kW = SGI_KPCN_LAYER_WEIGHTS;
+constexpr int kP = SGI_KPCN_LAYER_PARAMS;
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Zero the BF16 lanes of v whose mask lane is not > 0.
+__device__ __forceinline__ uint4 gateBf16x8(uint4 v, uint4 m) {
+    uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+    const uint32_t mm[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t lo = (int16_t)(mm[i] & 0xffffu) > 0 ? 0x0000ffffu : 0u;
+        const uint32_t hi = (int16_t)(mm[i] >> 16) > 0 ? 0xffff0000u : 0u;
+        vv[i] &= lo | hi;
+    }
+    return make_uint4(vv[0], vv[1], vv[2], vv[3]);
+}
+
+// The contiguous run of tiles of this block; blocks of one XCD get adjacent runs.
+__device__ __forceinline__ void blockTiles(int nTiles, int& t0, int& t1) {
+    const int L = (int)xcdSwizzle(blockIdx.x, gridDim.x);
+    const int per = (nTiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    t0 = min(nTiles, L * per);
+    t1 = min(nTiles, t0 + per);
+}
+
+__device__ __forceinline__ bf16x8 asBf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <bool OUTF32>
+__global__ __launch_bounds__(256) void conv3x3Kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ mask,
+                                                     const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                     int relu, int H, int W, int tilesX, int nTiles,
+                                                     void* __restrict__ out) {
+    __shared__ uint4 halo[kHS * kHS * kPxU4];  // 16-B aligned by type
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lg = lane >> 4;
+    // A operands: W[t][mb*16 + l16][8 lg .. 8 lg + 7]
+    bf16x8 wa[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+            wa[t][mb] = asBf16x8(*reinterpret_cast<const uint4*>(w + (t * kC + mb * 16 + l16) * kC + 8 * lg));
+    float bv[2][4];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[mb][r] = bias ? bias[mb * 16 + 4 * lg + r] : 0.0f;
+
+    int t0, t1;
+    blockTiles(nTiles, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int tx = tile % tilesX, ty = tile / tilesX;
+        const int x0 = tx * kTS - 1, y0 = ty * kTS - 1;
+        __syncthreads();
+        for (int i = threadIdx.x; i < kHS * kHS * 4; i += 256) {
+            const int px = i >> 2, c = i & 3;
+            const int gy = y0 + px / kHS, gx = x0 + px % kHS;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
+                v = *reinterpret_cast<const uint4*>(in + g);
+                if (mask) v = gateBf16x8(v, *reinterpret_cast<const uint4*>(mask + g));
+            }
+            halo[px * kPxU4 + c] = v;
+        }
+        __syncthreads();
+        f32x4 acc[2][4];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int row = wave * 4 + nb;
+                const bf16x8 b = asBf16x8(halo[((row + dy) * kHS + l16 + dx) * kPxU4 + lg]);
+                acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], b, acc[0][nb], 0, 0, 0);
+                acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], b, acc[1][nb], 0, 0, 0);
+            }
+        }
+        // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
+        const int x = tx * kTS + l16;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int y = ty * kTS + wave * 4 + nb;
+            if (x >= W || y >= H) continue;
+            const size_t p = (size_t)y * W + x;
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    v[r] = acc[mb][nb][r] + bv[mb][r];
+                    if (relu) v[r] = gmax(v[r], 0.0f);
+                }
+                const int co = mb * 16 + 4 * lg;
+                if (OUTF32) {
+                    *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
+                        make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+                    const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                               (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) = o;
+                }
+            }
+        }
+    }
+}
+
+// Weight gradient.  LDS: xs[s][ci][row][chunk] = 8 pixels x[y0 - 1 + row][x0 + 8 chunk + s - 1 .. +7][ci]
+// (s = dx shift 0..2, row 0..17) and dz[co][row][chunk] = 8 pixels of the gated gradient.
+__global__ __launch_bounds__(256) void wgradKernel(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                                                   const uint16_t* __restrict__ mask, int H, int W, int tilesX,
+                                                   int nTiles, float* __restrict__ partials) {
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3][kC][kHS][kTS];
+    __shared__ __attribute__((aligned(16))) uint16_t dz[kC][kTS][kTS];
+    __shared__ float dbPart[4][kC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lg = lane >> 4;
+    f32x4 acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float dbAcc = 0.0f;
+    int t0, t1;
+    blockTiles(nTiles, t0, t1);
+    for (int tile = t0; tile < t1; ++tile) {
+        const int tx = tile % tilesX, ty = tile / tilesX;
+        const int bx = tx * kTS, by = ty * kTS;
+        __syncthreads();
+        // gated gradient tile, transposed
+        for (int i = threadIdx.x; i < kTS * kTS * 4; i += 256) {
+            const int px = i >> 2, c = i & 3;
+            const int gy = by + px / kTS, gx = bx + px % kTS;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx < W && gy < H) {
+                const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
+                v = *reinterpret_cast<const uint4*>(dy + g);
+                if (mask) v = gateBf16x8(v, *reinterpret_cast<const uint4*>(mask + g));
+            }
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                dz[c * 8 + 2 * j][px / kTS][px % kTS] = (uint16_t)(vv[j] & 0xffffu);
+                dz[c * 8 + 2 * j + 1][px / kTS][px % kTS] = (uint16_t)(vv[j] >> 16);
+            }
+        }
+        // input halo, transposed, in three column shifts
+        for (int i = threadIdx.x; i < kHS * kHS * 4; i += 256) {
+            const int px = i >> 2, c = i & 3;
+            const int hy = px / kHS, hx = px % kHS;
+            const int gy = by - 1 + hy, gx = bx - 1 + hx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H)
+                v = *reinterpret_cast<const uint4*>(x + ((size_t)gy * W + gx) * kC + c * 8);
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int col = hx - s;
+                if (col < 0 || col >= kTS) continue;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    xs[s][c * 8 + 2 * j][hy][col] = (uint16_t)(vv[j] & 0xffffu);
+                    xs[s][c * 8 + 2 * j + 1][hy][col] = (uint16_t)(vv[j] >> 16);
+                }
+            }
+        }
+        __syncthreads();
+        // bias gradient: lane pairs per channel
+        if (threadIdx.x < 2 * kC) {
+            const int co = threadIdx.x >> 1, half = threadIdx.x & 1;
+            float s = 0.0f;
+            for (int r = half * 8; r < half * 8 + 8; ++r)
+                for (int c = 0; c < kTS; ++c) s += bf2f(dz[co][r][c]);
+            dbAcc += s;
+        }
+        // 8 reduction steps of 32 pixels (two tile rows); combo q = wave * 9 + i:
+        // tap = q / 4, mb = (q / 2) % 2, nb = q % 2
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int prow = 2 * k + (lg >> 1), chunk = lg & 1;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const int q = wave * 9 + i;
+                const int tap = q >> 2, mb = (q >> 1) & 1, nb = q & 1;
+                const int tdy = tap / 3, tdx = tap % 3;
+                const bf16x8 a = asBf16x8(*reinterpret_cast<const uint4*>(&dz[mb * 16 + l16][prow][chunk * 8]));
+                const bf16x8 b =
+                    asBf16x8(*reinterpret_cast<const uint4*>(&xs[tdx][nb * 16 + l16][prow + tdy][chunk * 8]));
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+            }
+        }
+    }
+    // partial sums of this block: [tap][co][ci] then 32 biases
+    float* part = partials + (size_t)blockIdx.x * kP;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int q = wave * 9 + i;
+        const int tap = q >> 2, mb = (q >> 1) & 1, nb = q & 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = mb * 16 + 4 * lg + r, ci = nb * 16 + l16;
+            part[(tap * kC + co) * kC + ci] = acc[i][r];
+        }
+    }
+    if (threadIdx.x < 2 * kC) dbPart[threadIdx.x & 1][threadIdx.x >> 1] = dbAcc;
+    __syncthreads();
+    if (threadIdx.x < kC) part[kW + threadIdx.x] = dbPart[0][threadIdx.x] + dbPart[1][threadIdx.x];
+}
+
+__global__ void wgradReduceKernel(const float* __restrict__ partials, int blocks, float* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kP) return;
+    float s = 0.0f;
+    for (int b = 0; b < blocks; ++b) s += partials[(size_t)b * kP + i];
+    grad[i] = s;
+}
+
+// 2x2 mean pooling over the in-range children; one thread per (coarse pixel, 8-channel chunk).
+__global__ void poolKernel(const uint16_t* __restrict__ in, int H, int W, uint16_t* __restrict__ out) {
+    const int h = (H + 1) / 2, w = (W + 1) / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h * w * 4) return;
+    const int c = i & 3, p = i >> 2;
+    const int X = p % w, Y = p / w;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int n = 0;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const int yy = 2 * Y + dy, xx = 2 * X + dx;
+            if (yy >= H || xx >= W) continue;
+            ++n;
+            const uint4 v = *reinterpret_cast<const uint4*>(in + ((size_t)yy * W + xx) * kC + c * 8);
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s[2 * j] += bf2f((uint16_t)(vv[j] & 0xffffu));
+                s[2 * j + 1] += bf2f((uint16_t)(vv[j] >> 16));
+            }
+        }
+    const float inv = 1.0f / (float)n;
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(s[2 * j] * inv) | ((uint32_t)f2bf(s[2 * j + 1] * inv) << 16);
+    *reinterpret_cast<uint4*>(out + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// din[child] (+)= dout[parent] / n(parent); one thread per (fine pixel, chunk).
+__global__ void poolBackwardKernel(const uint16_t* __restrict__ dout, int H, int W, uint16_t* __restrict__ din,
+                                   int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W * 4) return;
+    const int c = i & 3, p = i >> 2;
+    const int x = p % W, y = p / W;
+    const int w = (W + 1) / 2;
+    const int X = x / 2, Y = y / 2;
+    const int n = (min(2 * Y + 1, H - 1) - 2 * Y + 1) * (min(2 * X + 1, W - 1) - 2 * X + 1);
+    const float inv = 1.0f / (float)n;
+    const uint4 g = *reinterpret_cast<const uint4*>(dout + ((size_t)Y * w + X) * kC + c * 8);
+    uint4 prev = make_uint4(0u, 0u, 0u, 0u);
+    uint16_t* dst = din + (size_t)p * kC + c * 8;
+    if (accumulate) prev = *reinterpret_cast<const uint4*>(dst);
+    const uint32_t gg[4] = {g.x, g.y, g.z, g.w}, pp[4] = {prev.x, prev.y, prev.z, prev.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float lo = bf2f((uint16_t)(pp[j] & 0xffffu)) + bf2f((uint16_t)(gg[j] & 0xffffu)) * inv;
+        const float hi = bf2f((uint16_t)(pp[j] >> 16)) + bf2f((uint16_t)(gg[j] >> 16)) * inv;
+        o[j] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+    }
+    *reinterpret_cast<uint4*>(dst) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// out = skip + low[y / 2][x / 2]
+__global__ void upAddKernel(const uint16_t* __restrict__ skip, const uint16_t* __restrict__ low, int H, int W,
+                            uint16_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W * 4) return;
+    const int c = i & 3, p = i >> 2;
+    const int x = p % W, y = p / W;
+    const int w = (W + 1) / 2;
+    const uint4 a = *reinterpret_cast<const uint4*>(skip + (size_t)p * kC + c * 8);
+    const uint4 b = *reinterpret_cast<const uint4*>(low + ((size_t)(y / 2) * w + x / 2) * kC + c * 8);
+    const uint32_t aa[4] = {a.x, a.y, a.z, a.w}, bb[4] = {b.x, b.y, b.z, b.w};
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float lo = bf2f((uint16_t)(aa[j] & 0xffffu)) + bf2f((uint16_t)(bb[j] & 0xffffu));
+        const float hi = bf2f((uint16_t)(aa[j] >> 16)) + bf2f((uint16_t)(bb[j] >> 16));
+        o[j] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+    }
+    *reinterpret_cast<uint4*>(out + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// dlow[Y][X] = sum of the in-range dout children
+__global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W, uint16_t* __restrict__ dlow) {
+    const int h = (H + 1) / 2, w = (W + 1) / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h * w * 4) return;
+    const int c = i & 3, p = i >> 2;
+    const int X = p % w, Y = p / w;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const int yy = 2 * Y + dy, xx = 2 * X + dx;
+            if (yy >= H || xx >= W) continue;
+            const uint4 v = *reinterpret_cast<const uint4*>(dout + ((size_t)yy * W + xx) * kC + c * 8);
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s[2 * j] += bf2f((uint16_t)(vv[j] & 0xffffu));
+                s[2 * j + 1] += bf2f((uint16_t)(vv[j] >> 16));
+            }
+        }
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(s[2 * j]) | ((uint32_t)f2bf(s[2 * j + 1]) << 16);
+    *reinterpret_cast<uint4*>(dlow + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// Network input features and the noisy radiance.
+__global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restrict__ reservoir,
+                               uint16_t* __restrict__ feat, float* __restrict__ radiance) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.W * g.H) return;
+    const int x = i % g.W, y = i / g.W;
+    const float4 r = reservoir[i];
+    const float inv = 1.0f / gmax(kEPS, r.w);
+    const float L[3] = {r.x * inv, r.y * inv, r.z * inv};
+    radiance[3 * i + 0] = L[0];
+    radiance[3 * i + 1] = L[1];
+    radiance[3 * i + 2] = L[2];
+    const f3 P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[i], fr.invPV);
+    const f3 d = ld3(fr.viewPos) - P;
+    const f3 n = decodeNormal(g.normal + 3 * i);
+    float f[SGI_KPCN_FEATURES];
+    f[0] = log1pf(gmax(L[0], 0.0f));
+    f[1] = log1pf(gmax(L[1], 0.0f));
+    f[2] = log1pf(gmax(L[2], 0.0f));
+    f[3] = g.albedo[4 * i + 0];
+    f[4] = g.albedo[4 * i + 1];
+    f[5] = g.albedo[4 * i + 2];
+    f[6] = n.x;
+    f[7] = n.y;
+    f[8] = n.z;
+    f[9] = log1pf(sqrtf(dot(d, d))) * 0.125f;
+    uint32_t o[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int a = 2 * j, b = 2 * j + 1;
+        const uint32_t lo = a < SGI_KPCN_FEATURES ? f2bf(f[a]) : 0u;
+        const uint32_t hi = b < SGI_KPCN_FEATURES ? f2bf(f[b]) : 0u;
+        o[j] = lo | (hi << 16);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(feat + (size_t)i * kC);
+    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    dst[2] = make_uint4(o[8], o[9], o[10], o[11]);
+    dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
+}
+
+// Softmax weights of the 25 filter logits (max-subtracted).
+__device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
+    float m = lg[0];
+#pragma unroll
+    for (int j = 1; j < SGI_KPCN_TAPS; ++j) m = gmax(m, lg[j]);
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        wgt[j] = expf(lg[j] - m);
+        s += wgt[j];
+    }
+    const float inv = 1.0f / s;
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) wgt[j] *= inv;
+}
+
+__device__ __forceinline__ void loadLogits(const float* logits, size_t p, float* lg) {
+    const float4* src = reinterpret_cast<const float4*>(logits + p * kC);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const float4 v = src[j];
+        lg[4 * j] = v.x;
+        lg[4 * j + 1] = v.y;
+        lg[4 * j + 2] = v.z;
+        lg[4 * j + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int x, int y, const float* wgt) {
+    f3 o = splat(0.0f);
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
+        o = o + ld3(radiance + 3 * ((size_t)yy * W + xx)) * wgt[j];
+    }
+    return o;
+}
+
+__global__ void applyKernel(const float* __restrict__ logits, const float* __restrict__ radiance, int H, int W,
+                            float* __restrict__ out, float* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int x = i % W, y = i / W;
+    float lg[28], wgt[SGI_KPCN_TAPS];
+    loadLogits(logits, i, lg);
+    softmax25(lg, wgt);
+    const f3 o = filtered(radiance, W, H, x, y, wgt);
+    out[3 * i + 0] = o.x;
+    out[3 * i + 1] = o.y;
+    out[3 * i + 2] = o.z;
+    if (err) err[i] = lg[SGI_KPCN_TAPS];
+}
+
+__global__ __launch_bounds__(256) void lossKernel(const float* __restrict__ logits, const float* __restrict__ radiance,
+                                                  const float* __restrict__ target, int H, int W, float errWeight,
+                                                  float scale, uint16_t* __restrict__ dlogits,
+                                                  float* __restrict__ lossSums) {
+    __shared__ float red[2][256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l0 = 0.0f, l1 = 0.0f;
+    if (i < W * H) {
+        const int x = i % W, y = i / W;
+        const float invN = 1.0f / (3.0f * (float)W * (float)H), invP = 1.0f / ((float)W * (float)H);
+        float lg[28], wgt[SGI_KPCN_TAPS];
+        loadLogits(logits, i, lg);
+        softmax25(lg, wgt);
+        const f3 o = filtered(radiance, W, H, x, y, wgt);
+        const f3 t = ld3(target + 3 * i);
+        const f3 den = f3{t.x * t.x + 0.01f, t.y * t.y + 0.01f, t.z * t.z + 0.01f};
+        const f3 e = o - t;
+        const float rx = e.x * e.x / den.x, ry = e.y * e.y / den.y, rz = e.z * e.z / den.z;
+        l0 = rx + ry + rz;
+        const float rp = l0 * (1.0f / 3.0f);
+        const float errT = logf(rp + 1e-4f);
+        const float de = lg[SGI_KPCN_TAPS] - errT;
+        l1 = de * de;
+        // d loss / d out_c, then through the weights and the softmax
+        const f3 dOut = f3{2.0f * e.x / den.x, 2.0f * e.y / den.y, 2.0f * e.z / den.z} * (invN * scale);
+        float dw[SGI_KPCN_TAPS];
+        float sdw = 0.0f;
+#pragma unroll
+        for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+            const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
+            dw[j] = dot(dOut, ld3(radiance + 3 * ((size_t)yy * W + xx)));
+            sdw += wgt[j] * dw[j];
+        }
+        float dl[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) dl[j] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < SGI_KPCN_TAPS; ++j) dl[j] = wgt[j] * (dw[j] - sdw);
+        dl[SGI_KPCN_TAPS] = 2.0f * errWeight * de * invP * scale;
+        uint4* dst = reinterpret_cast<uint4*>(dlogits + (size_t)i * kC);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t o4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                o4[j] = (uint32_t)f2bf(dl[q * 8 + 2 * j]) | ((uint32_t)f2bf(dl[q * 8 + 2 * j + 1]) << 16);
+            dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+        }
+    }
+    red[0][threadIdx.x] = l0;
+    red[1][threadIdx.x] = l1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&lossSums[0], red[0][0]);
+        atomicAdd(&lossSums[1], red[1][0]);
+    }
+}
+
+__global__ void adamKernel(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
+                           float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SGI_KPCN_PARAMS) return;
+    const float g = grad[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    params[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// BF16 forward weights, transposed / mirrored input-gradient weights and FP32 biases.
+__global__ void weightCopiesKernel(const float* __restrict__ params, uint16_t* __restrict__ w,
+                                   uint16_t* __restrict__ wt, float* __restrict__ biases) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SGI_KPCN_LAYERS * kP) return;
+    const int l = i / kP, r = i % kP;
+    const float val = params[i];
+    if (r >= kW) {
+        if (biases) biases[l * kC + (r - kW)] = val;
+        return;
+    }
+    const int t = r / (kC * kC), o = (r / kC) % kC, in = r % kC;
+    if (w) w[(size_t)l * kW + r] = f2bf(val);
+    if (wt) wt[(size_t)l * kW + ((8 - t) * kC + in) * kC + o] = f2bf(val);
+}
+
+// Per 16x16 tile: flag = mean exp(err) > threshold (every tile when threshold <= 0).
+__global__ __launch_bounds__(256) void tileFlagKernel(const float* __restrict__ err, int H, int W, int tilesX,
+                                                      float threshold, uint8_t* __restrict__ flags) {
+    __shared__ float red[256];
+    const int t = blockIdx.x;
+    const int tx = t % tilesX, ty = t / tilesX;
+    const int x = tx * 16 + (int)threadIdx.x % 16, y = ty * 16 + (int)threadIdx.x / 16;
+    const bool in = x < W && y < H;
+    red[threadIdx.x] = in ? expf(gmin(err[(size_t)y * W + x], 80.0f)) : 0.0f;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int n = (min(W, tx * 16 + 16) - tx * 16) * (min(H, ty * 16 + 16) - ty * 16);
+        flags[t] = (!(threshold > 0.0f) || red[0] / (float)n > threshold) ? 1 : 0;
+    }
+}
+
+// counts[0] = number of flagged tiles, counts[1 + t] = flag of tile t (one block).
+__global__ __launch_bounds__(1024) void countFlagsKernel(const uint8_t* __restrict__ flags, int n,
+                                                         int32_t* __restrict__ counts) {
+    __shared__ int red[1024];
+    int c = 0;
+    for (int t = threadIdx.x; t < n; t += 1024) {
+        c += flags[t];
+        counts[1 + t] = flags[t];
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[0] = red[0];
+}
+
+__global__ void dilateKernel(const uint8_t* __restrict__ flags, int tilesX, int tilesY, uint8_t* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tilesX * tilesY) return;
+    const int tx = t % tilesX, ty = t / tilesX;
+    uint8_t f = 0;
+    for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int xx = tx + dx, yy = ty + dy;
+            if (xx >= 0 && xx < tilesX && yy >= 0 && yy < tilesY) f |= flags[yy * tilesX + xx];
+        }
+    out[t] = f;
+}
+
+// reservoirOut = flagged tile ? a-trous result (already there) : (network output, 1)
+__global__ void selectKernel(const float* __restrict__ net, const uint8_t* __restrict__ flags, int H, int W,
+                             int tilesX, float4* __restrict__ res) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int x = i % W, y = i / W;
+    if (flags[(y / 16) * tilesX + x / 16]) return;
+    res[i] = make_float4(net[3 * i], net[3 * i + 1], net[3 * i + 2], 1.0f);
+}
+
+int gridFor(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+namespace sgi_kpcn {
+
+hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
+                   int W, void* out, int outF32, hipStream_t s) {
+    const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
+    const int grid = nTiles < 2048 ? nTiles : 2048;
+    if (outF32)
+        hipLaunchKernelGGL(conv3x3Kernel<true>, dim3(grid), dim3(256), 0, s, in, mask, w, bias, relu, H, W, tilesX,
+                           nTiles, out);
+    else
+        hipLaunchKernelGGL(conv3x3Kernel<false>, dim3(grid), dim3(256), 0, s, in, mask, w, bias, relu, H, W, tilesX,
+                           nTiles, out);
+    return hipGetLastError();
+}
+
+int wgradBlocks(int H, int W) {
+    const int nTiles = ((W + kTS - 1) / kTS) * ((H + kTS - 1) / kTS);
+    return nTiles < 512 ? nTiles : 512;
+}
+
+hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,
+                 float* grad, hipStream_t s) {
+    const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
+    const int blocks = wgradBlocks(H, W);
+    hipLaunchKernelGGL(wgradKernel, dim3(blocks), dim3(256), 0, s, x, dy, mask, H, W, tilesX, nTiles, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(wgradReduceKernel, dim3(gridFor(kP, 256)), dim3(256), 0, s, partials, blocks, grad);
+    return hipGetLastError();
+}
+
+hipError_t pool(const uint16_t* in, int H, int W, uint16_t* out, hipStream_t s) {
+    const int n = ((H + 1) / 2) * ((W + 1) / 2) * 4;
+    hipLaunchKernelGGL(poolKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, in, H, W, out);
+    return hipGetLastError();
+}
+
+hipError_t poolBackward(const uint16_t* dout, int H, int W, uint16_t* din, int accumulate, hipStream_t s) {
+    const int n = H * W * 4;
+    hipLaunchKernelGGL(poolBackwardKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, dout, H, W, din, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t upAdd(const uint16_t* skip, const uint16_t* low, int H, int W, uint16_t* out, hipStream_t s) {
+    const int n = H * W * 4;
+    hipLaunchKernelGGL(upAddKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, skip, low, H, W, out);
+    return hipGetLastError();
+}
+
+hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, hipStream_t s) {
+    const int n = ((H + 1) / 2) * ((W + 1) / 2) * 4;
+    hipLaunchKernelGGL(upBackwardKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, dout, H, W, dlow);
+    return hipGetLastError();
+}
+
+hipError_t features(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, uint16_t* feat, float* radiance,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(featuresKernel, dim3(gridFor(g.W * g.H, 256)), dim3(256), 0, s, g, fr,
+                       (const float4*)reservoir, feat, radiance);
+    return hipGetLastError();
+}
+
+hipError_t apply(const float* logits, const float* radiance, int H, int W, float* out, float* err, hipStream_t s) {
+    hipLaunchKernelGGL(applyKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, H, W, out, err);
+    return hipGetLastError();
+}
+
+hipError_t loss(const float* logits, const float* radiance, const float* target, int H, int W, float errWeight,
+                float scale, uint16_t* dlogits, float* lossSums, hipStream_t s) {
+    hipLaunchKernelGGL(lossKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, target, H, W,
+                       errWeight, scale, dlogits, lossSums);
+    return hipGetLastError();
+}
+
+hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
+                int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s) {
+    const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
+    hipLaunchKernelGGL(adamKernel, dim3(gridFor(SGI_KPCN_PARAMS, 256)), dim3(256), 0, s, params, grad, m, v, lr, b1,
+                       b2, eps, c1, c2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(weightCopiesKernel, dim3(gridFor(SGI_KPCN_PARAMS, 256)), dim3(256), 0, s, params, w, wt,
+                       biases);
+    return hipGetLastError();
+}
+
+hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s) {
+    hipLaunchKernelGGL(weightCopiesKernel, dim3(gridFor(SGI_KPCN_PARAMS, 256)), dim3(256), 0, s, params, w, wt,
+                       biases);
+    return hipGetLastError();
+}
+
+hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, uint8_t* dilated,
+                     int32_t* counts, hipStream_t s) {
+    const int tilesX = (W + 15) / 16, tilesY = (H + 15) / 16;
+    hipLaunchKernelGGL(tileFlagKernel, dim3(tilesX * tilesY), dim3(256), 0, s, err, H, W, tilesX, threshold, flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dilateKernel, dim3(gridFor(tilesX * tilesY, 256)), dim3(256), 0, s, flags, tilesX, tilesY,
+                       dilated);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (counts) {
+        hipLaunchKernelGGL(countFlagsKernel, dim3(1), dim3(1024), 0, s, flags, tilesX * tilesY, counts);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s) {
+    hipLaunchKernelGGL(selectKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, net, flags, H, W, (W + 15) / 16,
+                       (float4*)res);
+    return hipGetLastError();
+}
+
+}  // namespace sgi_kpcn

@@ -42,4 +42,42 @@ hipError_t denoise(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, 
                    const float* prevGi, const float* prevHistory, float* combined, float* giOut,
                    float* reservoirOut, float* historyOut, const DenoiseBuffers& ws, hipStream_t s);
 
+// The merge and the three filter passes into reservoirOut; mask (per 16x16
+// tile, NULL = all) limits them to the marked tiles.
+hipError_t denoiseSpatial(const sgi_dev::GbufArgs& cur, const sgi_dev::FrameArgs& fr, const int32_t* visible,
+                          const float* reservoir, float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask,
+                          hipStream_t s);
+
+// The temporal blend and combine on a filtered reservoir (normals from denoiseSpatial).
+hipError_t denoiseTemporal(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, const sgi_dev::FrameArgs& fr,
+                           const float* screen, const float* color, const float* filtered, const float* prevGi,
+                           const float* prevHistory, float* combined, float* giOut, float* historyOut,
+                           const DenoiseBuffers& ws, hipStream_t s);
+
 }  // namespace sgi_launch
+
+// Learned-denoiser launchers (kpcn.hip).
+namespace sgi_kpcn {
+
+hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
+                   int W, void* out, int outF32, hipStream_t s);
+int wgradBlocks(int H, int W);
+hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,
+                 float* grad, hipStream_t s);
+hipError_t pool(const uint16_t* in, int H, int W, uint16_t* out, hipStream_t s);
+hipError_t poolBackward(const uint16_t* dout, int H, int W, uint16_t* din, int accumulate, hipStream_t s);
+hipError_t upAdd(const uint16_t* skip, const uint16_t* low, int H, int W, uint16_t* out, hipStream_t s);
+hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, hipStream_t s);
+hipError_t features(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir, uint16_t* feat,
+                    float* radiance, hipStream_t s);
+hipError_t apply(const float* logits, const float* radiance, int H, int W, float* out, float* err, hipStream_t s);
+hipError_t loss(const float* logits, const float* radiance, const float* target, int H, int W, float errWeight,
+                float scale, uint16_t* dlogits, float* lossSums, hipStream_t s);
+hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
+                int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
+hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
+hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, uint8_t* dilated,
+                     int32_t* counts, hipStream_t s);
+hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s);
+
+}  // namespace sgi_kpcn

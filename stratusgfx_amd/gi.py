@@ -91,6 +91,24 @@ class GiPass:
                   _ptr(visible), _ptr(screen), _ptr(color), _ptr(reservoir), _ptr(prev_gi), _ptr(prev_history),
                   _ptr(combined), _ptr(gi_out), _ptr(reservoir_out), _ptr(history_out), _stream(stream))
 
+    def kpcn_features(self, gbuf: _abi.Gbuffer, reservoir, features, radiance, stream=None) -> None:
+        _abi.call(self.lib, "sgi_kpcn_features", self.ctx, ctypes.byref(gbuf), _ptr(reservoir), _ptr(features),
+                  _ptr(radiance), _stream(stream))
+
+    def denoise_learned(self, cur: _abi.Gbuffer, prev: _abi.Gbuffer | None, visible, screen, color, reservoir,
+                        prev_gi, prev_history, net: _abi.KpcnNet, threshold: float, workspace, combined, gi_out,
+                        reservoir_out, history_out, fallback_tiles=None, stream=None) -> None:
+        _abi.call(self.lib, "sgi_denoise_learned", self.ctx, ctypes.byref(cur), ctypes.byref(prev) if prev else None,
+                  _ptr(visible), _ptr(screen), _ptr(color), _ptr(reservoir), _ptr(prev_gi), _ptr(prev_history),
+                  ctypes.byref(net), ctypes.c_float(threshold), _ptr(workspace),
+                  workspace.numel() * workspace.element_size(), _ptr(combined), _ptr(gi_out), _ptr(reservoir_out),
+                  _ptr(history_out), _ptr(fallback_tiles), _stream(stream))
+
+    def kpcn_workspace(self, device="cuda"):
+        import torch
+        n = int(self.lib.sgi_kpcn_workspace_bytes(self.height, self.width))
+        return torch.empty(n, dtype=torch.uint8, device=device)
+
     def render_frame(self, io: _abi.FrameIO, stream=None) -> None:
         _abi.call(self.lib, "sgi_render_frame", self.ctx, ctypes.byref(io), _stream(stream))
 

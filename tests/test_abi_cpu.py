@@ -11,7 +11,7 @@ from stratusgfx_amd import _abi
 
 
 def _headers():
-    return [os.path.join(_abi.INCLUDE_DIR, h) for h in ("sgi.h", "sgi_scene.h")]
+    return [os.path.join(_abi.INCLUDE_DIR, h) for h in ("sgi.h", "sgi_kpcn.h", "sgi_scene.h")]
 
 
 def test_library_exports_every_declared_function():
@@ -35,12 +35,12 @@ def test_host_library_exports_host_generator():
 STRUCTS = {"sgi_vpl": None, "sgi_atlas_entry": None, "sgi_tile_stage1": None, "sgi_gbuffer": _abi.Gbuffer,
            "sgi_cube_atlases": _abi.CubeAtlases, "sgi_csm": _abi.Csm, "sgi_frame": _abi.Frame,
            "sgi_config": _abi.Config, "sgi_frame_io": _abi.FrameIO, "sgi_scene_desc": _abi.SceneDesc,
-           "sgi_view": _abi.View}
+           "sgi_view": _abi.View, "sgi_kpcn_net": _abi.KpcnNet}
 
 
 def test_struct_layouts_match_c():
     """sizeof of every public struct as gcc lays it out vs the ctypes mirror and the reference sizes."""
-    src = '#include <stdio.h>\n#include "sgi_scene.h"\nint main(void){\n'
+    src = '#include <stdio.h>\n#include "sgi_scene.h"\n#include "sgi_kpcn.h"\nint main(void){\n'
     for name in STRUCTS:
         src += f'printf("{name} %zu\\n", sizeof({name}));\n'
     src += "return 0;}\n"

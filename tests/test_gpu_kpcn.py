@@ -1,0 +1,283 @@
+"""The learned denoiser's HIP kernels (MFMA convolutions, weight gradients,
+pooling, filter, loss, Adam) and the whole network's gradients against the
+CPU oracle (oracle/kpcn_oracle.py, pinned by finite differences).
+
+Bounds: BF16-stored outputs within one BF16 rounding of the oracle evaluated
+on the same BF16 operands (|d| <= 2^-7 |ref| + 1e-6); FP32 weight gradients
+within rel-L2 1e-5 (exact BF16 products, FP32 sums in another order); the
+whole network's gradients, which pass through 15 layers of BF16 activations,
+within rel-L2 3e-2 per layer and cosine >= 0.999 of the BF16-emulating oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import kpcn_oracle as ko
+from gpu_util import require_gpu
+from stratusgfx_amd import _abi, kpcn
+
+pytestmark = pytest.mark.gpu
+
+H, W = 37, 45  # partial 16x16 tiles on both axes; levels 37x45 .. 3x3
+
+
+def _bf(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda().to(torch.bfloat16)
+
+
+def _np(t):
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+def _close_bf16(got, want, what):
+    tol = 2.0 ** -7 * np.abs(want) + 1e-6
+    bad = np.abs(got - want) > tol
+    assert not bad.any(), f"{what}: {bad.sum()} of {bad.size} off, worst {np.abs(got - want).max():.3e}"
+
+
+@pytest.fixture(scope="module")
+def lib():
+    require_gpu()
+    return _abi.load()
+
+
+def _rand_layer(seed):
+    rng = np.random.default_rng(seed)
+    Wt = ko.to_bf16(rng.normal(0, 0.1, (9, 32, 32)))
+    b = rng.normal(0, 0.1, 32).astype(np.float32)
+    x = ko.to_bf16(rng.normal(0, 1, (H, W, 32)))
+    return rng, Wt, b, x
+
+
+def test_conv3x3_forward(lib):
+    torch = require_gpu()
+    _, Wt, b, x = _rand_layer(0)
+    for relu, f32 in ((1, 0), (0, 1)):
+        out = torch.empty((H, W, 32), dtype=torch.float32 if f32 else torch.bfloat16, device="cuda")
+        _abi.call(lib, "sgi_kpcn_conv3x3", _bf(torch, x).data_ptr(), 0, _bf(torch, Wt).data_ptr(),
+                  torch.from_numpy(b).cuda().data_ptr(), relu, H, W, out.data_ptr(), f32, 0)
+        torch.cuda.synchronize()
+        want = ko.conv3x3(x, Wt, b.astype(np.float64))
+        if relu:
+            want = np.maximum(want, 0)
+        got = _np(out)
+        if f32:
+            assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+        else:
+            _close_bf16(got, ko.to_bf16(want), "conv bf16")
+
+
+def test_conv3x3_input_gradient(lib):
+    """dgrad = conv with the mirrored, transposed weights and the ReLU gate."""
+    torch = require_gpu()
+    rng, Wt, _, dy = _rand_layer(1)
+    y = ko.to_bf16(np.maximum(rng.normal(0, 1, (H, W, 32)), 0))
+    params = np.zeros(ko.PARAMS, np.float32)
+    params[:ko.LAYER_WEIGHTS] = Wt.reshape(-1)
+    wts = kpcn.Weights(params)
+    out = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    _abi.call(lib, "sgi_kpcn_conv3x3", _bf(torch, dy).data_ptr(), _bf(torch, y).data_ptr(),
+              wts.layer_wt(0).data_ptr(), 0, 0, H, W, out.data_ptr(), 0, 0)
+    torch.cuda.synchronize()
+    _, _, dx = ko.conv3x3_backward(np.zeros((H, W, 32)), dy * (y > 0), Wt)
+    _close_bf16(_np(out), ko.to_bf16(dx), "dgrad")
+
+
+def test_conv3x3_weight_gradient(lib):
+    torch = require_gpu()
+    rng, _, _, x = _rand_layer(2)
+    dy = ko.to_bf16(rng.normal(0, 1, (H, W, 32)))
+    y = ko.to_bf16(np.maximum(rng.normal(0, 1, (H, W, 32)), 0))
+    parts = torch.empty(int(lib.sgi_kpcn_wgrad_partials(H, W)), dtype=torch.float32, device="cuda")
+    grad = torch.empty(ko.LAYER_PARAMS, dtype=torch.float32, device="cuda")
+    for mask in (None, y):
+        mptr = _bf(torch, mask).data_ptr() if mask is not None else 0
+        _abi.call(lib, "sgi_kpcn_conv3x3_wgrad", _bf(torch, x).data_ptr(), _bf(torch, dy).data_ptr(), mptr, H, W,
+                  parts.data_ptr(), grad.data_ptr(), 0)
+        torch.cuda.synchronize()
+        dz = dy if mask is None else dy * (mask > 0)
+        dW, db, _ = ko.conv3x3_backward(x, dz, np.zeros((9, 32, 32)))
+        g = grad.cpu().numpy().astype(np.float64)
+        want = np.concatenate([dW.reshape(-1), db])
+        e = np.linalg.norm(g - want) / np.linalg.norm(want)
+        print(f"wgrad mask={mask is not None} rel-L2 {e:.2e}")
+        assert e <= 1e-5
+
+
+def test_pool_and_upsample(lib):
+    torch = require_gpu()
+    rng = np.random.default_rng(3)
+    x = ko.to_bf16(rng.normal(0, 1, (H, W, 32)))
+    h, w = (H + 1) // 2, (W + 1) // 2
+    low = ko.to_bf16(rng.normal(0, 1, (h, w, 32)))
+    prev = ko.to_bf16(rng.normal(0, 1, (H, W, 32)))
+    out = torch.empty((h, w, 32), dtype=torch.bfloat16, device="cuda")
+    _abi.call(lib, "sgi_kpcn_pool", _bf(torch, x).data_ptr(), H, W, out.data_ptr(), 0)
+    up = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    _abi.call(lib, "sgi_kpcn_upsample_add", _bf(torch, x).data_ptr(), _bf(torch, low).data_ptr(), H, W,
+              up.data_ptr(), 0)
+    dlow = torch.empty((h, w, 32), dtype=torch.bfloat16, device="cuda")
+    _abi.call(lib, "sgi_kpcn_upsample_backward", _bf(torch, x).data_ptr(), H, W, dlow.data_ptr(), 0)
+    din = _bf(torch, prev)
+    _abi.call(lib, "sgi_kpcn_pool_backward", _bf(torch, low).data_ptr(), H, W, din.data_ptr(), 1, 0)
+    din0 = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    _abi.call(lib, "sgi_kpcn_pool_backward", _bf(torch, low).data_ptr(), H, W, din0.data_ptr(), 0, 0)
+    torch.cuda.synchronize()
+    _close_bf16(_np(out), ko.to_bf16(ko.pool(x)), "pool")
+    _close_bf16(_np(up), ko.to_bf16(x + ko.upsample(low, H, W)), "upsample_add")
+    _close_bf16(_np(dlow), ko.to_bf16(ko.upsample_backward(x)), "upsample_backward")
+    _close_bf16(_np(din), ko.to_bf16(prev + ko.pool_backward(low, H, W)), "pool_backward accumulate")
+    _close_bf16(_np(din0), ko.to_bf16(ko.pool_backward(low, H, W)), "pool_backward")
+
+
+def test_filter_apply_and_loss(lib):
+    torch = require_gpu()
+    rng = np.random.default_rng(4)
+    logits = rng.normal(0, 2, (H, W, 32)).astype(np.float32)
+    rad = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    target = rng.uniform(0.1, 1.5, (H, W, 3)).astype(np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    err = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    _abi.call(lib, "sgi_kpcn_apply", t(logits).data_ptr(), t(rad).data_ptr(), H, W, out.data_ptr(), err.data_ptr(), 0)
+    dl = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    sums = torch.zeros(2, dtype=torch.float32, device="cuda")
+    _abi.call(lib, "sgi_kpcn_loss", t(logits).data_ptr(), t(rad).data_ptr(), t(target).data_ptr(), H, W,
+              ctypes.c_float(0.25), ctypes.c_float(0.5), dl.data_ptr(), sums.data_ptr(), 0)
+    torch.cuda.synchronize()
+    o, e = ko.apply(logits.astype(np.float64), rad.astype(np.float64))
+    assert np.abs(out.cpu().numpy() - o).max() <= 1e-5 * np.abs(o).max()
+    assert np.array_equal(err.cpu().numpy(), logits[..., ko.TAPS])
+    L, dlo = ko.loss(logits.astype(np.float64), rad.astype(np.float64), target.astype(np.float64), 0.25)
+    s = sums.cpu().numpy().astype(np.float64)
+    P = H * W
+    assert abs(s[0] / (3 * P) + 0.25 * s[1] / P - L) <= 1e-4 * abs(L)
+    _close_bf16(_np(dl), ko.to_bf16(0.5 * dlo), "dlogits")
+
+
+def test_adam_step(lib):
+    torch = require_gpu()
+    rng = np.random.default_rng(5)
+    p0 = rng.normal(0, 0.1, ko.PARAMS).astype(np.float32)
+    g = rng.normal(0, 1e-3, ko.PARAMS).astype(np.float32)
+    wts = kpcn.Weights(p0)
+    tr_m = torch.zeros(ko.PARAMS, dtype=torch.float32, device="cuda")
+    tr_v = torch.zeros(ko.PARAMS, dtype=torch.float32, device="cuda")
+    gd = torch.from_numpy(g).cuda()
+    for step in (1, 2):
+        _abi.call(lib, "sgi_kpcn_adam", wts.params.data_ptr(), gd.data_ptr(), tr_m.data_ptr(), tr_v.data_ptr(),
+                  ctypes.c_float(1e-3), ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8), step,
+                  wts.w.data_ptr(), wts.wt.data_ptr(), wts.biases.data_ptr(), 0)
+    torch.cuda.synchronize()
+    m = np.zeros_like(p0, np.float64)
+    v = np.zeros_like(p0, np.float64)
+    p = p0.astype(np.float64)
+    for step in (1, 2):
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g.astype(np.float64) ** 2
+        p -= 1e-3 * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
+    got = wts.params.cpu().numpy()
+    assert np.abs(got - p).max() <= 1e-6
+    # the BF16 copies follow the new parameters, the transposed copy mirrors the taps
+    Wl = got[:ko.LAYER_WEIGHTS].reshape(9, 32, 32)
+    assert np.array_equal(_np(wts.layer_w(0)).reshape(9, 32, 32), ko.to_bf16(Wl))
+    assert np.array_equal(_np(wts.layer_wt(0)).reshape(9, 32, 32), ko.to_bf16(Wl[::-1].transpose(0, 2, 1)))
+    assert np.array_equal(wts.biases[:32].cpu().numpy(), got[ko.LAYER_WEIGHTS:ko.LAYER_PARAMS])
+
+
+def test_network_gradients_match_oracle(lib):
+    """One training forward / backward of the whole network on the GPU against the BF16-emulating oracle."""
+    torch = require_gpu()
+    rng = np.random.default_rng(6)
+    params = kpcn.init_params(7)
+    for l in range(ko.LAYERS):
+        params[l * ko.LAYER_PARAMS + ko.LAYER_WEIGHTS:(l + 1) * ko.LAYER_PARAMS] = rng.normal(0, 0.05, 32)
+    feat = np.zeros((H, W, 32))
+    feat[..., :10] = rng.normal(0, 1, (H, W, 10))
+    feat = ko.to_bf16(feat)
+    rad = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    target = rng.uniform(0.1, 1.5, (H, W, 3)).astype(np.float32)
+    wts = kpcn.Weights(params)
+    tr = kpcn.Trainer(wts, H, W)
+    tr.zero_grad()
+    logits = tr.forward(_bf(torch, feat))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    tr.loss_backward(t(rad), t(target), err_weight=0.1)
+    torch.cuda.synchronize()
+    net = ko.Net(params.astype(np.float64), bf16=True)
+    want_logits = net.forward(feat)
+    got_logits = logits.cpu().numpy().astype(np.float64)
+    e_log = np.linalg.norm(got_logits - want_logits) / np.linalg.norm(want_logits)
+    print(f"logits rel-L2 {e_log:.2e}")
+    assert e_log <= 1e-2
+    _, dl = ko.loss(want_logits, rad.astype(np.float64), target.astype(np.float64), 0.1)
+    want = net.backward(dl)
+    got = tr.grad.cpu().numpy().astype(np.float64).reshape(ko.LAYERS, ko.LAYER_PARAMS)
+    for l in range(ko.LAYERS):
+        a, b = got[l], want[l]
+        e = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
+        print(f"layer {l:2d} grad rel-L2 {e:.2e} cos {cos:.6f}")
+        assert e <= 3e-2 and cos >= 0.999, l
+
+
+def test_denoise_learned_threshold_semantics(lib):
+    """threshold <= 0: every tile falls back, so the outputs are sgi_denoise's bit for bit;
+    threshold = inf: no tile falls back and gi is the network's filtered radiance."""
+    from gpu_util import Case
+    from stratusgfx_amd import scene as sc
+    from stratusgfx_amd.gi import GiPass
+
+    torch = require_gpu()
+    case = Case(sc.CORNELL_640x360)
+    w = case.w
+    Hh, Ww = w.height, w.width
+    gi = GiPass(Ww, Hh, w.tile_size, w.max_vpls_per_tile)
+    gi.set_frame(case.dev.frame)
+    gi.set_csm(case.dev.csm_struct())
+    n = w.num_vpls
+    g = case.dev.gbuffer()
+    upd = torch.zeros((n, 16), dtype=torch.float32, device="cuda")
+    vis = torch.zeros((1 + n,), dtype=torch.int32, device="cuda")
+    gi.cull_global(case.dev.vpls, n, upd, vis)
+    ent = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    gi.visible_atlas_entries(case.dev.entries, vis, n, ent)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    col, res = torch.zeros((Hh, Ww, 3), **f32), torch.zeros((Hh, Ww, 4), **f32)
+    gi.gather_ex(g, upd, vis, ent, case.dev.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=8)
+    lit = torch.zeros((Hh, Ww, 3), **f32)
+    gi.direct_light(g, lit)
+    wts = kpcn.Weights(kpcn.init_params(0))
+    ws = gi.kpcn_workspace()
+    n_tiles = ((Ww + 15) // 16) * ((Hh + 15) // 16)
+
+    def run_learned(thr):
+        outs = [torch.full((Hh, Ww, c), -1.0, **f32) for c in (3, 3, 4)]
+        hist = torch.full((Hh, Ww), -1.0, **f32)
+        fb = torch.full((1 + n_tiles,), -1, dtype=torch.int32, device="cuda")
+        gi.denoise_learned(g, None, vis, lit, col, res, None, None, wts.net(), thr, ws, outs[0], outs[1], outs[2],
+                           hist, fb)
+        return outs, hist, fb
+
+    outs, hist, fb = run_learned(-1.0)
+    ref = [torch.zeros((Hh, Ww, c), **f32) for c in (3, 3, 4)]
+    rh = torch.zeros((Hh, Ww), **f32)
+    gi.denoise(g, None, vis, lit, col, res, None, None, ref[0], ref[1], ref[2], rh)
+    torch.cuda.synchronize()
+    assert int(fb[0]) == n_tiles and bool((fb[1:] == 1).all())
+    for a, b in zip(outs + [hist], ref + [rh]):
+        assert torch.equal(a, b)
+
+    outs, hist, fb = run_learned(float("inf"))
+    feat = torch.empty((Hh, Ww, 32), dtype=torch.bfloat16, device="cuda")
+    rad = torch.empty((Hh, Ww, 3), **f32)
+    gi.kpcn_features(g, res, feat, rad)
+    tr = kpcn.Trainer(wts, Hh, Ww)
+    logits = tr.forward(feat)
+    out = torch.empty((Hh, Ww, 3), **f32)
+    _abi.call(lib, "sgi_kpcn_apply", logits.data_ptr(), rad.data_ptr(), Hh, Ww, out.data_ptr(), 0, 0)
+    torch.cuda.synchronize()
+    assert int(fb[0]) == 0
+    assert torch.equal(outs[1], out)
+    assert torch.all(hist == 1.0)
+    assert torch.equal(outs[0], lit + col * out)
