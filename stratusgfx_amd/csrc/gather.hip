@@ -253,10 +253,12 @@ __device__ __forceinline__ void shadePairH(const PixelH& q, const LightH& a, con
     const h2 NdotH = hsat((q.nv + nlRaw) * hInv);
     const h2 LdotH = hsat((hs(1.0f) + lv) * hInv);
     const h2 f = hfma(NdotH * q.r2m1, NdotH, hs(1.0f));
-    const h2 ndf = q.r2 * hrcp(hmax(hs(kPI) * f * f, hs(kEPS)));
+    // the FP32 path's 1e-5 floors are below FP16's normal range (subnormals flush), so
+    // the floors here are 2^-14, the smallest normal FP16 value
+    const h2 ndf = q.r2 * hrcp(hmax(hs(kPI) * f * f, hs(0x1p-14f)));
     const h2 ggxl = q.ndotv * hsqrt(hfma(NdotL * -q.r2m1, NdotL, q.r2));
     // Vis * N.L <= 2: the product is formed before it meets the NDF
-    const h2 visNl = (hs(0.5f) * NdotL) * hrcp(hmax(hfma(NdotL, q.gv, ggxl), hs(kEPS)));
+    const h2 visNl = (hs(0.5f) * NdotL) * hrcp(hmax(hfma(NdotL, q.gv, ggxl), hs(0x1p-14f)));
     const h2 specT = ndf * visNl;
     const h2 t = hs(1.0f) - LdotH;
     const h2 t2 = t * t;

@@ -1,0 +1,221 @@
+"""Trains the learned denoiser on pairs the gather itself generates, on the GPU.
+
+    python -m stratusgfx_amd.train_kpcn --scenes 48 --views 2 --size 512 --steps 1500
+
+Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
+the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES) of
+4096 VPLs, target = the converged gather over every visible VPL with FP64
+sums (sgi_gather_ex ALL), both normalised as the denoiser's pass 0 does.
+Training: Adam on the relative-MSE loss plus the error head's regression
+(sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
+held-out seeds reports the relative MSE of the noisy input, the a-trous
+filter (sgi_denoise), the network alone, and the network with its a-trous
+fallback, and picks the fallback threshold.  Writes the FP32 parameters
+(.npy) and a JSON summary beside them.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _abi, kpcn
+from . import scene as sc
+from .gi import GiPass
+
+
+def rel_mse(x, ref) -> float:
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), in FP64."""
+    x = x.double()
+    r = ref.double()
+    return float(((x - r) ** 2 / (r * r + 1e-2)).mean())
+
+
+def _inside_box(boxes: np.ndarray, p, margin=0.75) -> bool:
+    c, h = boxes[:, 0:3], boxes[:, 4:7]
+    d = np.abs(np.asarray(p, np.float32)[None, :] - c)
+    return bool(np.any(np.all(d <= h + margin, axis=1)))
+
+
+def random_view(scene: sc.Scene, rng, size: int) -> _abi.View:
+    """A pose uniform over the block at 1.5-30 m, random yaw / pitch, outside every box."""
+    n = scene.lib.sgi_scene_num_boxes(scene.handle)
+    boxes = np.zeros((n, 16), np.float32)
+    _abi.call(scene.lib, "sgi_scene_boxes", scene.handle, boxes.ctypes.data)
+    v = _abi.View()
+    _abi.call(scene.lib, "sgi_scene_default_view", scene.handle, size, size, ctypes.byref(v))
+    e = scene.w.city_half_extent * 0.9
+    for _ in range(100):
+        eye = (rng.uniform(-e, e), rng.uniform(1.5, 30.0), rng.uniform(-e, e))
+        if not _inside_box(boxes, eye):
+            break
+    v.eye[0], v.eye[1], v.eye[2] = eye
+    v.yaw_deg = float(rng.uniform(0.0, 360.0))
+    v.pitch_deg = float(rng.uniform(-35.0, 5.0))
+    return v
+
+
+class Sample:
+    """One view's network inputs and targets on the device."""
+
+    def __init__(self, feat, rad, target, extra=None):
+        self.feat, self.rad, self.target, self.extra = feat, rad, target, extra
+
+
+def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: int, rng, keep_frame=False):
+    import torch
+
+    w = sc.Workload(f"city-train-{seed}", _abi.SCENE_CITY, seed, size, size, num_vpls, 64, _abi.TEXEL_F16, 1024, 16,
+                    64)
+    scene = sc.Scene(w)
+    out = []
+    inp = None
+    for _ in range(views):
+        scene.set_view(random_view(scene, rng, size))
+        inp = sc.generate_device(scene)
+        gi = GiPass(size, size, 16, 64)
+        gi.set_frame(inp.frame)
+        gi.set_csm(inp.csm_struct())
+        f32 = dict(dtype=torch.float32, device="cuda")
+        n = num_vpls
+        upd = torch.zeros((n, 16), **f32)
+        vis = torch.zeros((1 + n,), dtype=torch.int32, device="cuda")
+        gi.cull_global(inp.vpls, n, upd, vis)
+        ent = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+        gi.visible_atlas_entries(inp.entries, vis, n, ent)
+        g = inp.gbuffer()
+        col, res = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
+        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=samples_pp)
+        colT, resT = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
+        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, colT, resT, mode=_abi.GATHER_ALL, accumulate_fp64=True)
+        feat = torch.empty((size, size, 32), dtype=torch.bfloat16, device="cuda")
+        rad = torch.empty((size, size, 3), **f32)
+        gi.kpcn_features(g, res, feat, rad)
+        target = resT[..., :3] / resT[..., 3:4].clamp_min(1e-5)
+        extra = None
+        if keep_frame:
+            lit = torch.zeros((size, size, 3), **f32)
+            gi.direct_light(g, lit)
+            extra = dict(gi=gi, inp=inp, vis=vis, col=col, res=res, lit=lit)
+        out.append(Sample(feat, rad, target.contiguous(), extra))
+    torch.cuda.synchronize()
+    return out
+
+
+def evaluate(wts: kpcn.Weights, val, thresholds):
+    """relMSE of noisy / a-trous / network / network + fallback at each threshold, averaged over views."""
+    import torch
+
+    acc = {"noisy": 0.0, "atrous": 0.0, "network": 0.0}
+    fb = {t: [0.0, 0.0] for t in thresholds}
+    for s in val:
+        e = s.extra
+        gi, inp = e["gi"], e["inp"]
+        g = inp.gbuffer()
+        H = W = s.rad.shape[0]
+        f32 = dict(dtype=torch.float32, device="cuda")
+        outs = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)]
+        hist = torch.zeros((H, W), **f32)
+        gi.denoise(g, None, e["vis"], e["lit"], e["col"], e["res"], None, None, outs[0], outs[1], outs[2], hist)
+        acc["atrous"] += rel_mse(outs[1], s.target)
+        acc["noisy"] += rel_mse(s.rad, s.target)
+        ws = gi.kpcn_workspace()
+        n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+        fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
+        gi.denoise_learned(g, None, e["vis"], e["lit"], e["col"], e["res"], None, None, wts.net(), float("inf"), ws,
+                           outs[0], outs[1], outs[2], hist, fbt)
+        acc["network"] += rel_mse(outs[1], s.target)
+        for t in thresholds:
+            gi.denoise_learned(g, None, e["vis"], e["lit"], e["col"], e["res"], None, None, wts.net(), t, ws,
+                               outs[0], outs[1], outs[2], hist, fbt)
+            fb[t][0] += rel_mse(outs[1], s.target)
+            fb[t][1] += int(fbt[0]) / n_tiles
+    n = len(val)
+    res = {k: v / n for k, v in acc.items()}
+    res["fallback"] = {f"{t:g}": {"relmse": v[0] / n, "tiles": v[1] / n} for t, v in fb.items()}
+    return res
+
+
+def main(argv=None) -> None:
+    import torch
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--views", type=int, default=2)
+    ap.add_argument("--val-scenes", type=int, default=4)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--vpls", type=int, default=4096)
+    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--err-weight", type=float, default=0.02)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=kpcn.DEFAULT_WEIGHTS)
+    ap.add_argument("--log", default=None)
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(a.seed)
+    t0 = time.time()
+    train = []
+    for i in range(a.scenes):
+        train += make_samples(1000 + i, a.views, a.size, a.vpls, a.spp, rng)
+        if i % 8 == 7:
+            print(f"[data] {len(train)} training views, {time.time() - t0:.0f} s", flush=True)
+    val = []
+    for i in range(a.val_scenes):
+        val += make_samples(5000 + i, 1, a.size, a.vpls, a.spp, rng, keep_frame=True)
+    tmed = float(torch.stack([s.target.median() for s in train]).median())
+    print(f"[data] {len(train)} train / {len(val)} val views in {time.time() - t0:.0f} s; median target {tmed:.4g}",
+          flush=True)
+
+    wts = kpcn.Weights(kpcn.init_params(a.seed))
+    tr = kpcn.Trainer(wts, a.size, a.size)
+    base = evaluate(wts, val, [])
+    print(f"[val] before training {json.dumps(base)}", flush=True)
+    hist = []
+    t1 = time.time()
+    for step in range(1, a.steps + 1):
+        lr = a.lr * 0.5 * (1.0 + math.cos(math.pi * (step - 1) / a.steps))
+        tr.zero_grad()
+        for j in rng.choice(len(train), a.batch, replace=False):
+            s = train[j]
+            tr.forward(s.feat)
+            tr.loss_backward(s.rad, s.target, err_weight=a.err_weight, scale=1.0 / a.batch)
+        tr.step(lr)
+        if step % 50 == 0 or step == 1:
+            ls = tr.loss_sums.cpu().numpy() / (a.batch * a.size * a.size)
+            hist.append({"step": step, "relmse": float(ls[0] / 3.0), "err_loss": float(ls[1]), "lr": lr})
+            print(f"[train] step {step} relMSE {ls[0] / 3.0:.5f} err-head {ls[1]:.4f} lr {lr:.2e} "
+                  f"{time.time() - t1:.0f} s", flush=True)
+    torch.cuda.synchronize()
+    train_s = time.time() - t1
+    # thresholds on the predicted mean relative squared error of a tile
+    thresholds = [0.01, 0.02, 0.05, 0.1, 0.2, 0.5]
+    after = evaluate(wts, val, thresholds)
+    print(f"[val] after training {json.dumps(after)}", flush=True)
+    # the fallback threshold: the one with the lowest validation relMSE (inf = none)
+    best_t, best = float("inf"), after["network"]
+    for t in thresholds:
+        v = after["fallback"][f"{t:g}"]["relmse"]
+        if v < best:
+            best_t, best = t, v
+    kpcn.save_params(wts.params.cpu().numpy(), a.out)
+    summary = {"train_views": len(train), "val_views": len(val), "size": a.size, "vpls": a.vpls, "spp": a.spp,
+               "steps": a.steps, "batch": a.batch, "lr": a.lr, "err_weight": a.err_weight,
+               "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
+               "before": base, "after": after, "threshold": best_t, "history": hist}
+    with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    if a.log:
+        with open(a.log, "w") as f:
+            json.dump(summary, f, indent=1)
+    print(f"[done] threshold {best_t} relMSE {best:.5f}; weights {a.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
