@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
-"""GI frames/s of the deferred VPL GI pass on one MI355X.
+"""GI frames/s of the deferred VPL GI pass on one MI355X: the learned form
+against the plain form at their relative MSE to the converged render.
 
-A step is one GI frame of the pass on resident inputs, in RenderScene's order
-(StratusRendererBackend.cpp:1982, 2010-2045): global VPL sun-visibility cull
-and compaction, the visible slots' atlas entries, deferred direct sun light
-with cascaded PCF, per-tile VPL culling, the per-pixel VPL gather and the
-4-pass a-trous / temporal denoiser with the combine.  The workload is the
-procedural city block at 1920x1080 with 1024 VPLs (6x256x256 FP16 distance
-cubes, 16x16 tiles, up to 64 VPLs per tile).  Inputs are generated on the GPU
-before the timed region; the denoiser's history alternates between two buffer
-sets as the reference swaps its FBOs.
+A step is one GI frame on resident inputs, in RenderScene's order
+(StratusRendererBackend.cpp:1982, 2010-2045).  Both forms run on the
+procedural city block at 1920x1080 (6x256x256 FP16 distance cubes):
 
-Prints one JSON line (rank 0).  With the default flags it also times the CPU
-oracle on a bounded sample of the same frame as the CPU baseline.
+  plain    4096 VPLs: global VPL sun-visibility cull and compaction, the
+           visible slots' atlas entries, deferred direct sun light with
+           cascaded PCF, per-tile VPL culling (16x16 tiles, up to 64 VPLs),
+           the tile-list gather and the 4-pass a-trous / temporal denoiser;
+  learned  1024 VPLs: the same culling and direct light, an 8-sample-per-pixel
+           gather over the visible VPLs and the learned denoiser (network +
+           5x5 predicted filter, a-trous on the tiles whose predicted error
+           exceeds the trained threshold) with the same temporal pass.
+
+relMSE (mean of (x - r)^2 / (r^2 + 1e-2), FP64) of each form's GI output
+after one frame against the converged render: every visible VPL of the
+4096-VPL scene with FP64 sums.  `value` is the learned form's frames/s;
+the plain form's figures are reported beside it.  Inputs are generated on
+the GPU before the timed region; the denoiser history alternates between two
+buffer sets as the reference swaps its FBOs.
+
+Prints one JSON line.  With the default flags it also times the CPU oracle
+on a bounded sample of the plain frame as the CPU baseline.
 """
 import argparse
 import json
@@ -32,7 +43,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--gather", choices=["tiles", "random"], default="tiles")
+    ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
+    ap.add_argument("--spp", type=int, default=8, help="gather samples per pixel of the learned form")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--cpu-rows", type=int, default=24, help="image rows of the CPU baseline sample")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -128,6 +140,88 @@ def cpu_baseline(host, w, rows, visible_count):
     }
 
 
+def rel_mse(x, ref) -> float:
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), in FP64."""
+    x, r = x.double(), ref.double()
+    return float(((x - r) ** 2 / (r * r + 1e-2)).mean())
+
+
+def timed_loop(step, steps, warmup, nstages):
+    """Runs warmup + steps frames; returns (elapsed seconds of the timed steps, mean ms per stage)."""
+    import torch
+    for k in range(warmup):
+        step(k, None)
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(nstages + 1)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k, events[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    per = [sum(events[k][i].elapsed_time(events[k][i + 1]) for k in range(steps)) / steps for i in range(nstages)]
+    return elapsed, per
+
+
+class FormState:
+    """One form's scene, pass context and frame buffers."""
+
+    def __init__(self, w):
+        import torch
+
+        from stratusgfx_amd import scene as sc
+        from stratusgfx_amd.gi import FrameBuffers, GiPass
+
+        print(f"[bench] generating {w.name} on the GPU", file=sys.stderr, flush=True)
+        self.w = w
+        self.scene = sc.Scene(w)
+        self.dev = sc.generate_device(self.scene)
+        self.gi = GiPass(w.width, w.height, w.tile_size, w.max_vpls_per_tile)
+        self.gi.set_frame(self.dev.frame)
+        self.gi.set_csm(self.dev.csm_struct())
+        self.g, self.cubes = self.dev.gbuffer(), self.dev.cubes()
+        self.n = w.num_vpls
+        self.fb = FrameBuffers.allocate(w.width, w.height, self.n, self.gi.num_tiles, w.max_vpls_per_tile)
+        self.hist = [(torch.zeros_like(self.fb.gi), torch.zeros_like(self.fb.history)) for _ in range(2)]
+
+    def cull(self):
+        from stratusgfx_amd import _abi
+        fb = self.fb
+        self.gi.cull_global(self.dev.vpls, self.n, fb.vpl_updated, fb.visible)
+        self.gi.visible_atlas_entries(self.dev.entries, fb.visible, min(self.n, _abi.MAX_TOTAL_VPLS_PER_FRAME),
+                                      fb.visible_entries)
+
+
+def conv_roofline(net_w, H, W, reps=20):
+    """HIP-event timing of the level-0 32 -> 32 convolution of the learned denoiser (layer 1) at the frame size."""
+    import torch
+
+    from stratusgfx_amd import _abi, kpcn
+    lib = _abi.load()
+    x = torch.randn((H, W, 32), device="cuda").clamp_min(0).to(torch.bfloat16)
+    y = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    args = (x.data_ptr(), 0, net_w.layer_w(1).data_ptr(), net_w.layer_b(1).data_ptr(), 1, H, W, y.data_ptr(), 0, s)
+    for _ in range(3):
+        _abi.call(lib, "sgi_kpcn_conv3x3", *args)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record()
+    for i in range(reps):
+        _abi.call(lib, "sgi_kpcn_conv3x3", *args)
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    ms = sum(ev[i].elapsed_time(ev[i + 1]) for i in range(reps)) / reps
+    P = H * W
+    alg_bytes = P * 32 * 2 * 2 + kpcn.LAYER_WEIGHTS * 2 + 32 * 4
+    flops = 2 * 9 * 32 * 32 * P
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"kernel": "conv3x3Kernel<bf16 out> (level 0, 32->32, 1920x1080)", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("conv_city1080p_traffic.json"),
+            "algorithmic_bytes": alg_bytes, "flops": flops, "avg_launch_ms": round(ms, 4),
+            "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -135,103 +229,181 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    import dataclasses
+
     import torch
 
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
         return
 
-    from stratusgfx_amd import _abi
+    from stratusgfx_amd import _abi, kpcn
     from stratusgfx_amd import scene as sc
-    from stratusgfx_amd.gi import FrameBuffers, GiPass
 
-    w = sc.CITY_1080P
-    print(f"[bench] generating {w.name} on the GPU", file=sys.stderr, flush=True)
-    scene = sc.Scene(w)
-    dev = sc.generate_device(scene)
-    gi = GiPass(w.width, w.height, w.tile_size, w.max_vpls_per_tile)
-    gi.set_frame(dev.frame)
-    gi.set_csm(dev.csm_struct())
-    g, cubes = dev.gbuffer(), dev.cubes()
-    n = w.num_vpls
-    cap = min(n, _abi.MAX_TOTAL_VPLS_PER_FRAME)
-    fb = FrameBuffers.allocate(w.width, w.height, n, gi.num_tiles, w.max_vpls_per_tile)
-    hist = [(torch.zeros_like(fb.gi), torch.zeros_like(fb.history)) for _ in range(2)]
+    w_learned = sc.CITY_1080P
+    w_plain = dataclasses.replace(sc.CITY_1080P, name="city-1920x1080-4096vpl", num_vpls=4096)
+    plain = FormState(w_plain)
+    learned = FormState(w_learned)
+    H, W = w_plain.height, w_plain.width
+    f32 = dict(dtype=torch.float32, device="cuda")
+
+    # the converged render: every visible VPL of the 4096-VPL scene, FP64 sums
+    plain.cull()
+    conv_col, conv_res = torch.zeros((H, W, 3), **f32), torch.zeros((H, W, 4), **f32)
+    plain.gi.gather_ex(plain.g, plain.fb.vpl_updated, plain.fb.visible, plain.fb.visible_entries, plain.cubes, None,
+                       conv_col, conv_res, mode=_abi.GATHER_ALL, accumulate_fp64=True)
+    target = (conv_res[..., :3] / conv_res[..., 3:4].clamp_min(1e-5)).contiguous()
+    del conv_col, conv_res
+
+    # ---- plain form ----------------------------------------------------------
     use_tiles = args.gather == "tiles"
-    stages = ["cull_global", "direct_light", "cull_tiles", "gather", "denoise"]
 
-    def step(k, ev=None):
-        prev_gi, prev_h = hist[k % 2]
-        out_gi, out_h = hist[(k + 1) % 2]
+    def plain_step(k, ev, hist=True):
+        p, fb = plain, plain.fb
+        prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
+        out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
-        gi.cull_global(dev.vpls, n, fb.vpl_updated, fb.visible)
-        gi.visible_atlas_entries(dev.entries, fb.visible, cap, fb.visible_entries)
+        p.cull()
         rec(1)
-        gi.direct_light(g, fb.direct)
+        p.gi.direct_light(p.g, fb.direct)
         rec(2)
         if use_tiles:
-            gi.cull_tiles(g, fb.vpl_updated, fb.visible, fb.tiles)
+            p.gi.cull_tiles(p.g, fb.vpl_updated, fb.visible, fb.tiles)
         rec(3)
-        gi.gather(g, fb.vpl_updated, fb.visible, fb.visible_entries, cubes, fb.tiles if use_tiles else None,
-                  fb.color, fb.reservoir)
+        p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, fb.tiles if use_tiles else None,
+                    fb.color, fb.reservoir)
         rec(4)
-        gi.denoise(g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined, out_gi,
-                   fb.reservoir_out, out_h)
+        p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined, out_gi,
+                     fb.reservoir_out, out_h)
         rec(5)
+        return out_gi
 
-    for k in range(args.warmup):
-        step(k)
+    gi_plain = plain_step(0, None, hist=False)
     torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(args.warmup + k, events[k])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    per_stage = {s: sum(events[k][i].elapsed_time(events[k][i + 1]) for k in range(args.steps)) / args.steps
-                 for i, s in enumerate(stages)}
-    visible = int(fb.visible[0].item())
-    tiles_np = fb.tiles.cpu().numpy()
-    texel_bytes = 2 if w.cube_format == _abi.TEXEL_F16 else 4
-    gather_ms = per_stage["gather"]
-    roofline = None
+    plain_rel = rel_mse(gi_plain, target)
+    plain_noisy = rel_mse(plain.fb.reservoir[..., :3] / plain.fb.reservoir[..., 3:4].clamp_min(1e-5), target)
+    print(f"[bench] timing {args.steps} plain frames", file=sys.stderr, flush=True)
+    el_plain, per_plain = timed_loop(plain_step, args.steps, args.warmup, 5)
+    tiles_np = plain.fb.tiles.cpu().numpy()
+    visible_plain = int(plain.fb.visible[0].item())
+
+    # ---- learned form ----------------------------------------------------------
+    weights_path = kpcn.DEFAULT_WEIGHTS
+    learned_res = None
+    el_learned = per_learned = None
+    if os.path.exists(weights_path):
+        meta_path = os.path.splitext(weights_path)[0] + ".json"
+        threshold = float("inf")
+        if os.path.exists(meta_path):
+            with open(meta_path) as f:
+                threshold = float(json.load(f).get("threshold", float("inf")))
+        wts = kpcn.Weights(kpcn.load_params(weights_path))
+        net = wts.net()
+        ws = learned.gi.kpcn_workspace()
+        n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+        fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
+
+        def learned_step(k, ev, hist=True, thr=threshold):
+            p, fb = learned, learned.fb
+            prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
+            out_gi, out_h = p.hist[(k + 1) % 2]
+            rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
+            rec(0)
+            p.cull()
+            rec(1)
+            p.gi.direct_light(p.g, fb.direct)
+            rec(2)
+            p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
+                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp)
+            rec(3)
+            p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net, thr,
+                                 ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
+            rec(4)
+            return out_gi
+
+        gi_l = learned_step(0, None, hist=False)
+        torch.cuda.synchronize()
+        learned_rel = rel_mse(gi_l, target)
+        fallback_tiles = int(fbt[0].item())
+        learned_noisy = rel_mse(learned.fb.reservoir[..., :3] / learned.fb.reservoir[..., 3:4].clamp_min(1e-5),
+                                target)
+        gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
+        torch.cuda.synchronize()
+        learned_rel_nofallback = rel_mse(gi_nf, target)
+        # a-trous alone on the same 8-sample input
+        fb = learned.fb
+        tmp = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)] + [torch.zeros((H, W), **f32)]
+        learned.gi.denoise(learned.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, None, None, *tmp)
+        torch.cuda.synchronize()
+        atrous_8spp = rel_mse(tmp[1], target)
+        del tmp
+        print(f"[bench] timing {args.steps} learned frames", file=sys.stderr, flush=True)
+        el_learned, per_learned = timed_loop(learned_step, args.steps, args.warmup, 4)
+        learned_res = {
+            "frames_per_s": round(args.steps / el_learned, 2), "ms_per_step": round(el_learned * 1e3 / args.steps, 4),
+            "relmse": round(learned_rel, 6), "relmse_without_fallback": round(learned_rel_nofallback, 6),
+            "relmse_noisy_input": round(learned_noisy, 6), "relmse_atrous_on_same_input": round(atrous_8spp, 6),
+            "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
+            "kernels_ms": {k: round(v, 4) for k, v in
+                           zip(["cull_global", "direct_light", "gather_8spp", "denoise_learned"], per_learned)},
+            "vpls": w_learned.num_vpls, "visible_vpls": int(learned.fb.visible[0].item()),
+            "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
+        }
+
+    texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
+    gather_roof = None
     if use_tiles:
-        alg_bytes, pairs = gather_algorithmic_bytes(tiles_np, w.tile_size, w.width, w.height, w.max_vpls_per_tile,
+        alg_bytes, pairs = gather_algorithmic_bytes(tiles_np, w_plain.tile_size, W, H, w_plain.max_vpls_per_tile,
                                                     texel_bytes)
-        achieved = alg_bytes / (gather_ms * 1e-3) / 1e9
-        roofline = {"kernel": "gatherTilesKernel", "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": load_traffic("gather_city1080p_traffic.json"),
-                    "algorithmic_bytes": alg_bytes, "pixel_vpl_pairs": pairs}
+        achieved = alg_bytes / (per_plain[3] * 1e-3) / 1e9
+        gather_roof = {"kernel": "gatherTilesKernel<F16,16> (plain form)", "bound": "hbm",
+                       "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4),
+                       "traffic": load_traffic("gather_city1080p_traffic.json"),
+                       "algorithmic_bytes": alg_bytes, "pixel_vpl_pairs": pairs,
+                       "avg_launch_ms": round(per_plain[3], 4)}
+    plain_res = {
+        "frames_per_s": round(args.steps / el_plain, 2), "ms_per_step": round(el_plain * 1e3 / args.steps, 4),
+        "relmse": round(plain_rel, 6), "relmse_undenoised": round(plain_noisy, 6),
+        "kernels_ms": {k: round(v, 4) for k, v in
+                       zip(["cull_global", "direct_light", "cull_tiles", "gather", "denoise"], per_plain)},
+        "vpls": w_plain.num_vpls, "visible_vpls": visible_plain, "gather": args.gather,
+    }
+    if learned_res is not None:
+        value, el, form = learned_res["frames_per_s"], el_learned, "learned"
+        learned_res["speedup_vs_plain"] = round(learned_res["frames_per_s"] / plain_res["frames_per_s"], 3)
+        learned_res["relmse_at_most_plain"] = learned_res["relmse"] <= plain_res["relmse"]
+        roofline = conv_roofline(wts, H, W)
+    else:
+        value, el, form = plain_res["frames_per_s"], el_plain, "plain (no trained weights found)"
+        roofline = gather_roof
     result = {
         "metric": METRIC,
-        "value": round(args.steps / elapsed, 2),
+        "value": value,
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1000.0 / args.steps, 4),
+        "ms_per_step": round(el * 1000.0 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32 shading, bf16 network (fp32 accumulation)" if learned_res else "fp32",
         "data": "synthetic: seeded procedural city block, ray-cast G-buffer / VPL cubes / cascades",
-        "config": {"workload": w.name + ("-tiles16-K64" if use_tiles else "-random"), "width": w.width,
-                   "height": w.height, "vpls": n, "visible_vpls": visible, "cube": f"6x{w.cube_res}x{w.cube_res} fp16",
-                   "tile_size": w.tile_size, "max_vpls_per_tile": w.max_vpls_per_tile, "gather": args.gather,
-                   "form": "plain FP32 restatement + a-trous (learned form not built)"},
-        "kernels_ms": {k: round(v, 4) for k, v in per_stage.items()},
+        "config": {"workload": "city-1920x1080 learned 1024 VPLs 8 spp vs plain 4096 VPLs tiles16-K64",
+                   "width": W, "height": H, "cube": f"6x{w_plain.cube_res}x{w_plain.cube_res} fp16",
+                   "form": form, "converged": "all visible VPLs of the 4096-VPL scene, FP64 sums"},
+        "learned": learned_res,
+        "plain": plain_res,
         "roofline": roofline,
+        "roofline_plain_gather": gather_roof,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
-        host = sc.to_host(dev)
-        result["cpu_baseline"] = cpu_baseline(host, w, args.cpu_rows, visible)
+        host = sc.to_host(plain.dev)
+        result["cpu_baseline"] = cpu_baseline(host, w_plain, args.cpu_rows, visible_plain)
     print(json.dumps(result), flush=True)
 
 

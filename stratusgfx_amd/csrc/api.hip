@@ -430,13 +430,12 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 
 namespace {
 
-// The inference workspace: per level l three activation planes (e_l, and two
-// temporaries), then the level-0 features, radiance, logits, filtered output,
-// error and the tile flags.
+// The inference workspace: per level l two activation planes (e_l and a
+// temporary for c_l / d_l), then the radiance, the filtered output, the error
+// logit and the tile flags.
 struct KpcnWs {
-    uint16_t* act[SGI_KPCN_LEVELS][3];
-    uint16_t* feat;
-    float *radiance, *logits, *out, *err;
+    uint16_t* act[SGI_KPCN_LEVELS][2];
+    float *radiance, *out, *err;
     uint8_t *flags, *dilated;
     int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
 };
@@ -451,7 +450,7 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     int h = H, w = W;
     for (int l = 0; l < SGI_KPCN_LEVELS; ++l) {
         if (ws) ws->h[l] = h, ws->w[l] = w;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 2; ++k) {
             char* p = take((size_t)h * w * SGI_KPCN_CHANNELS * 2);
             if (ws) ws->act[l][k] = (uint16_t*)p;
         }
@@ -460,17 +459,13 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     }
     const size_t P = (size_t)H * W;
     const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
-    char* feat = take(P * SGI_KPCN_CHANNELS * 2);
     char* rad = take(P * 3 * 4);
-    char* logits = take(P * SGI_KPCN_CHANNELS * 4);
     char* out = take(P * 3 * 4);
     char* err = take(P * 4);
     char* flags = take(tiles);
     char* dil = take(tiles);
     if (ws) {
-        ws->feat = (uint16_t*)feat;
         ws->radiance = (float*)rad;
-        ws->logits = (float*)logits;
         ws->out = (float*)out;
         ws->err = (float*)err;
         ws->flags = (uint8_t*)flags;
@@ -515,29 +510,27 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
     auto Wl = [&](int l) { return net->weights + (size_t)l * SGI_KPCN_LAYER_WEIGHTS; };
     auto Bl = [&](int l) { return net->biases + (size_t)l * SGI_KPCN_CHANNELS; };
-    hipError_t e = sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.radiance, s);
+    hipError_t e = hipSuccess;
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
-    // encoder: level 0 from the features, then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
+    // encoder: level 0 from the G-buffer features (computed in the first
+    // convolution's staging), then pool -> conv -> conv per level
+    KPCN_STEP(sgi_kpcn::convFeatures(c, ctx->frameArgs, reservoir, ws.radiance, Wl(0), Bl(0), ws.act[0][1], s));
     KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][1], nullptr, Wl(1), Bl(1), 1, ws.h[0], ws.w[0], ws.act[0][0], 0, s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
-        KPCN_STEP(sgi_kpcn::pool(ws.act[l - 1][0], ws.h[l - 1], ws.w[l - 1], ws.act[l][1], s));
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][2], 0,
-                                    s));
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
+        KPCN_STEP(sgi_kpcn::convPool(ws.act[l - 1][0], ws.h[l - 1], ws.w[l - 1], Wl(2 * l), Bl(2 * l), ws.act[l][1],
+                                     s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
                                     ws.act[l][0], 0, s));
     }
-    // decoder: d_l = conv(e_l + up(d_{l+1})), d_4 = e_4; d_l lands in act[l][2]
+    // decoder: d_l = conv(e_l + up(d_{l+1})) with d_4 = e_4; d_l lands in act[l][1]
     const uint16_t* low = ws.act[SGI_KPCN_LEVELS - 1][0];
     for (int l = SGI_KPCN_LEVELS - 2, layer = 10; l >= 0; --l, ++layer) {
-        KPCN_STEP(sgi_kpcn::upAdd(ws.act[l][0], low, ws.h[l], ws.w[l], ws.act[l][1], s));
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(layer), Bl(layer), 1, ws.h[l], ws.w[l], ws.act[l][2], 0,
-                                    s));
-        low = ws.act[l][2];
+        KPCN_STEP(sgi_kpcn::convUpAdd(ws.act[l][0], low, ws.h[l], ws.w[l], Wl(layer), Bl(layer), ws.act[l][1], s));
+        low = ws.act[l][1];
     }
-    KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][2], nullptr, Wl(14), Bl(14), 0, H, W, ws.logits, 1, s));
-    KPCN_STEP(sgi_kpcn::apply(ws.logits, ws.radiance, H, W, ws.out, ws.err, s));
+    // the last convolution and the 5x5 filter of the radiance in one kernel
+    KPCN_STEP(sgi_kpcn::convApply(ws.act[0][1], H, W, Wl(14), Bl(14), ws.radiance, ws.out, ws.err, s));
     KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
     // a-trous over the flagged tiles (and their margin), the network elsewhere
     KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,

@@ -196,8 +196,9 @@ __device__ __forceinline__ h2 hexp2(h2 a) { return __builtin_elementwise_exp2(a)
 
 struct PixelH {  // the pixel's shading inputs, both halves equal
     h2 nx, ny, nz, vx, vy, vz;
-    h2 nv, ndotv, r2, r2m1, gv;
+    h2 nv, ndotv, r2, r2m1, gv, halfInvGv;
     h2 f0r, f0g, f0b, dr, dg, db;
+    bool nvZero;  // N.V <= 0 (saturated to 0)
 };
 
 struct LightH {  // one VPL's FP32 inputs to the packed stage
@@ -221,6 +222,8 @@ __device__ __forceinline__ PixelH toHalf(const Pixel& px) {
     q.nx = hs(px.N.x); q.ny = hs(px.N.y); q.nz = hs(px.N.z);
     q.vx = hs(px.V.x); q.vy = hs(px.V.y); q.vz = hs(px.V.z);
     q.nv = hs(px.nvRaw); q.ndotv = hs(px.NdotV); q.r2 = hs(px.r2); q.r2m1 = hs(px.r2m1); q.gv = hs(px.gv);
+    q.halfInvGv = hs(0.5f / px.gv);
+    q.nvZero = px.NdotV == 0.0f;
     q.f0r = hs(px.f0.x); q.f0g = hs(px.f0.y); q.f0b = hs(px.f0.z);
     q.dr = hs(px.diffColor.x); q.dg = hs(px.diffColor.y); q.db = hs(px.diffColor.z);
     return q;
@@ -257,8 +260,11 @@ __device__ __forceinline__ void shadePairH(const PixelH& q, const LightH& a, con
     // the floors here are 2^-14, the smallest normal FP16 value
     const h2 ndf = q.r2 * hrcp(hmax(hs(kPI) * f * f, hs(0x1p-14f)));
     const h2 ggxl = q.ndotv * hsqrt(hfma(NdotL * -q.r2m1, NdotL, q.r2));
-    // Vis * N.L <= 2: the product is formed before it meets the NDF
-    const h2 visNl = (hs(0.5f) * NdotL) * hrcp(hmax(hfma(NdotL, q.gv, ggxl), hs(0x1p-14f)));
+    // Vis * N.L <= 2: the product is formed before it meets the NDF.  At N.V = 0
+    // it is 0.5 / gv for any N.L > 0 (and 0 at N.L = 0), which FP16 cannot reach
+    // through the quotient once N.L * gv falls below the normal range.
+    const h2 visNl = q.nvZero ? hmin(NdotL * hs(16384.0f), hs(1.0f)) * q.halfInvGv
+                              : (hs(0.5f) * NdotL) * hrcp(hmax(hfma(NdotL, q.gv, ggxl), hs(0x1p-14f)));
     const h2 specT = ndf * visNl;
     const h2 t = hs(1.0f) - LdotH;
     const h2 t2 = t * t;

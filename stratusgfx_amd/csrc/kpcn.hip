@@ -68,12 +68,176 @@ __device__ __forceinline__ void blockTiles(int nTiles, int& t0, int& t1) {
 
 __device__ __forceinline__ bf16x8 asBf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-template <bool OUTF32>
-__global__ __launch_bounds__(256) void conv3x3Kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ mask,
-                                                     const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                                     int relu, int H, int W, int tilesX, int nTiles,
-                                                     void* __restrict__ out) {
-    __shared__ uint4 halo[kHS * kHS * kPxU4];  // 16-B aligned by type
+// Network input features of pixel i (the 16 packed BF16 pairs of its 32
+// channels) and its noisy radiance reservoir.rgb / max(reservoir.a, 1e-5).
+__device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs& fr,
+                                              const float4* __restrict__ reservoir, int x, int y, uint32_t* o,
+                                              float* L) {
+    const size_t i = (size_t)y * g.W + x;
+    const float4 r = reservoir[i];
+    const float inv = 1.0f / gmax(kEPS, r.w);
+    L[0] = r.x * inv;
+    L[1] = r.y * inv;
+    L[2] = r.z * inv;
+    const f3 P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[i], fr.invPV);
+    const f3 d = ld3(fr.viewPos) - P;
+    const f3 n = decodeNormal(g.normal + 3 * i);
+    float f[SGI_KPCN_FEATURES];
+    f[0] = log1pf(gmax(L[0], 0.0f));
+    f[1] = log1pf(gmax(L[1], 0.0f));
+    f[2] = log1pf(gmax(L[2], 0.0f));
+    f[3] = g.albedo[4 * i + 0];
+    f[4] = g.albedo[4 * i + 1];
+    f[5] = g.albedo[4 * i + 2];
+    f[6] = n.x;
+    f[7] = n.y;
+    f[8] = n.z;
+    f[9] = log1pf(sqrtf(dot(d, d))) * 0.125f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int a = 2 * j, b = 2 * j + 1;
+        const uint32_t lo = a < SGI_KPCN_FEATURES ? f2bf(f[a]) : 0u;
+        const uint32_t hi = b < SGI_KPCN_FEATURES ? f2bf(f[b]) : 0u;
+        o[j] = lo | (hi << 16);
+    }
+}
+
+// Network input features and the noisy radiance (the training path's separate pass).
+__global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restrict__ reservoir,
+                               uint16_t* __restrict__ feat, float* __restrict__ radiance) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.W * g.H) return;
+    uint32_t o[16];
+    float L[3];
+    pixelFeatures(g, fr, reservoir, i % g.W, i / g.W, o, L);
+    radiance[3 * i + 0] = L[0];
+    radiance[3 * i + 1] = L[1];
+    radiance[3 * i + 2] = L[2];
+    uint4* dst = reinterpret_cast<uint4*>(feat + (size_t)i * kC);
+    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    dst[2] = make_uint4(o[8], o[9], o[10], o[11]);
+    dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
+}
+
+// Softmax weights of the 25 filter logits (max-subtracted).
+__device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
+    float m = lg[0];
+#pragma unroll
+    for (int j = 1; j < SGI_KPCN_TAPS; ++j) m = gmax(m, lg[j]);
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        wgt[j] = expf(lg[j] - m);
+        s += wgt[j];
+    }
+    const float inv = 1.0f / s;
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) wgt[j] *= inv;
+}
+
+__device__ __forceinline__ void loadLogits(const float* logits, size_t p, float* lg) {
+    const float4* src = reinterpret_cast<const float4*>(logits + p * kC);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const float4 v = src[j];
+        lg[4 * j] = v.x;
+        lg[4 * j + 1] = v.y;
+        lg[4 * j + 2] = v.z;
+        lg[4 * j + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int x, int y, const float* wgt) {
+    f3 o = splat(0.0f);
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
+        o = o + ld3(radiance + 3 * ((size_t)yy * W + xx)) * wgt[j];
+    }
+    return o;
+}
+
+// Where the convolution's input comes from (staged into the LDS halo):
+//   IN_PLAIN     a [h][w][32] (gated by mask when given);
+//   IN_UPADD     a + up(b): the skip tensor plus the coarser level (nearest);
+//   IN_POOL      2x2 mean of a at the finer level (fineH x fineW);
+//   IN_FEATURES  the network input features computed from the G-buffer and the
+//                reservoir, also writing the radiance of the tile's own pixels.
+// and where the result goes:
+//   OUT_BF16 / OUT_F32  [h][w][32] with bias (and ReLU for BF16 when relu != 0);
+//   OUT_APPLY           the 26 logits stay in LDS and the epilogue applies the
+//                       5x5 softmax filter to the radiance (out [h][w][3]) and
+//                       writes the error logit (err [h][w]).
+// Each fused form computes exactly what the separate kernels compute.
+enum { IN_PLAIN = 0, IN_UPADD = 1, IN_POOL = 2, IN_FEATURES = 3 };
+enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2 };
+
+struct ConvSrc {
+    const uint16_t* a;
+    const uint16_t* b;
+    const uint16_t* mask;
+    int fineH, fineW;
+    GbufArgs g;
+    FrameArgs fr;
+    const float4* reservoir;
+    float* radiance;
+};
+
+__device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
+    const float lo = bf2f((uint16_t)(a & 0xffffu)) + bf2f((uint16_t)(b & 0xffffu));
+    const float hi = bf2f((uint16_t)(a >> 16)) + bf2f((uint16_t)(b >> 16));
+    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+template <int IN>
+__device__ __forceinline__ uint4 stageChunk(const ConvSrc& src, int H, int W, int gx, int gy, int c) {
+    if constexpr (IN == IN_PLAIN) {
+        const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
+        uint4 v = *reinterpret_cast<const uint4*>(src.a + g);
+        if (src.mask) v = gateBf16x8(v, *reinterpret_cast<const uint4*>(src.mask + g));
+        return v;
+    } else if constexpr (IN == IN_UPADD) {
+        const int w2 = (W + 1) / 2;
+        const uint4 a = *reinterpret_cast<const uint4*>(src.a + ((size_t)gy * W + gx) * kC + c * 8);
+        const uint4 b = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+        return make_uint4(addBf16x2(a.x, b.x), addBf16x2(a.y, b.y), addBf16x2(a.z, b.z), addBf16x2(a.w, b.w));
+    } else {  // IN_POOL
+        float s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int n = 0;
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int yy = 2 * gy + dy, xx = 2 * gx + dx;
+                if (yy >= src.fineH || xx >= src.fineW) continue;
+                ++n;
+                const uint4 v = *reinterpret_cast<const uint4*>(src.a + ((size_t)yy * src.fineW + xx) * kC + c * 8);
+                const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    s8[2 * j] += bf2f((uint16_t)(vv[j] & 0xffffu));
+                    s8[2 * j + 1] += bf2f((uint16_t)(vv[j] >> 16));
+                }
+            }
+        const float inv = 1.0f / (float)n;
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = (uint32_t)f2bf(s8[2 * j] * inv) | ((uint32_t)f2bf(s8[2 * j + 1] * inv) << 16);
+        return make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (26 used)
+constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
+                                                                             : kTS * kTS * kLogitStride / 4 + 1;
+
+template <int IN, int OUT>
+__global__ __launch_bounds__(256) void conv3x3Kernel(ConvSrc src, const uint16_t* __restrict__ w,
+                                                     const float* __restrict__ bias, int relu, int H, int W,
+                                                     int tilesX, int nTiles, void* __restrict__ out,
+                                                     float* __restrict__ err) {
+    __shared__ uint4 smem[kSmemU4];  // the input halo, then (OUT_APPLY) the tile's logits
+    uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
     // A operands: W[t][mb*16 + l16][8 lg .. 8 lg + 7]
@@ -95,16 +259,35 @@ __global__ __launch_bounds__(256) void conv3x3Kernel(const uint16_t* __restrict_
         const int tx = tile % tilesX, ty = tile / tilesX;
         const int x0 = tx * kTS - 1, y0 = ty * kTS - 1;
         __syncthreads();
-        for (int i = threadIdx.x; i < kHS * kHS * 4; i += 256) {
-            const int px = i >> 2, c = i & 3;
-            const int gy = y0 + px / kHS, gx = x0 + px % kHS;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-                const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
-                v = *reinterpret_cast<const uint4*>(in + g);
-                if (mask) v = gateBf16x8(v, *reinterpret_cast<const uint4*>(mask + g));
+        if constexpr (IN == IN_FEATURES) {
+            for (int px = threadIdx.x; px < kHS * kHS; px += 256) {
+                const int hy = px / kHS, hx = px % kHS;
+                const int gy = y0 + hy, gx = x0 + hx;
+                uint32_t o[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) o[j] = 0u;
+                if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                    float L[3];
+                    pixelFeatures(src.g, src.fr, src.reservoir, gx, gy, o, L);
+                    if (hy >= 1 && hy <= kTS && hx >= 1 && hx <= kTS) {
+                        float* rd = src.radiance + 3 * ((size_t)gy * W + gx);
+                        rd[0] = L[0];
+                        rd[1] = L[1];
+                        rd[2] = L[2];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    halo[px * kPxU4 + c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
             }
-            halo[px * kPxU4 + c] = v;
+        } else {
+            for (int i = threadIdx.x; i < kHS * kHS * 4; i += 256) {
+                const int px = i >> 2, c = i & 3;
+                const int gy = y0 + px / kHS, gx = x0 + px % kHS;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (gx >= 0 && gx < W && gy >= 0 && gy < H) v = stageChunk<IN>(src, H, W, gx, gy, c);
+                halo[px * kPxU4 + c] = v;
+            }
         }
         __syncthreads();
         f32x4 acc[2][4];
@@ -124,28 +307,58 @@ __global__ __launch_bounds__(256) void conv3x3Kernel(const uint16_t* __restrict_
             }
         }
         // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
-        const int x = tx * kTS + l16;
+        if constexpr (OUT == OUT_APPLY) {
+            float* logits = reinterpret_cast<float*>(smem);
+            __syncthreads();  // every wave has read the halo the logits overwrite
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int y = ty * kTS + wave * 4 + nb;
-            if (x >= W || y >= H) continue;
-            const size_t p = (size_t)y * W + x;
+            for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb) {
-                float v[4];
+                for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    v[r] = acc[mb][nb][r] + bv[mb][r];
-                    if (relu) v[r] = gmax(v[r], 0.0f);
-                }
-                const int co = mb * 16 + 4 * lg;
-                if (OUTF32) {
-                    *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
-                        make_float4(v[0], v[1], v[2], v[3]);
-                } else {
-                    const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
-                                               (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
-                    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) = o;
+                    for (int r = 0; r < 4; ++r) {
+                        const int co = mb * 16 + 4 * lg + r;
+                        if (co <= SGI_KPCN_TAPS)
+                            logits[((wave * 4 + nb) * kTS + l16) * kLogitStride + co] = acc[mb][nb][r] + bv[mb][r];
+                    }
+            __syncthreads();
+            const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
+            if (x < W && y < H) {
+                float lgv[SGI_KPCN_TAPS + 1], wgt[SGI_KPCN_TAPS];
+#pragma unroll
+                for (int j = 0; j <= SGI_KPCN_TAPS; ++j) lgv[j] = logits[threadIdx.x * kLogitStride + j];
+                softmax25(lgv, wgt);
+                const f3 o = filtered(src.radiance, W, H, x, y, wgt);
+                const size_t p = (size_t)y * W + x;
+                float* dst = static_cast<float*>(out) + 3 * p;
+                dst[0] = o.x;
+                dst[1] = o.y;
+                dst[2] = o.z;
+                if (err) err[p] = lgv[SGI_KPCN_TAPS];
+            }
+        } else {
+            const int x = tx * kTS + l16;
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int y = ty * kTS + wave * 4 + nb;
+                if (x >= W || y >= H) continue;
+                const size_t p = (size_t)y * W + x;
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        v[r] = acc[mb][nb][r] + bv[mb][r];
+                        if (OUT == OUT_BF16 && relu) v[r] = gmax(v[r], 0.0f);
+                    }
+                    const int co = mb * 16 + 4 * lg;
+                    if constexpr (OUT == OUT_F32) {
+                        *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
+                            make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+                        const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                                   (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                        *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) = o;
+                    }
                 }
             }
         }
@@ -362,85 +575,6 @@ __global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W
     *reinterpret_cast<uint4*>(dlow + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// Network input features and the noisy radiance.
-__global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restrict__ reservoir,
-                               uint16_t* __restrict__ feat, float* __restrict__ radiance) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.W * g.H) return;
-    const int x = i % g.W, y = i / g.W;
-    const float4 r = reservoir[i];
-    const float inv = 1.0f / gmax(kEPS, r.w);
-    const float L[3] = {r.x * inv, r.y * inv, r.z * inv};
-    radiance[3 * i + 0] = L[0];
-    radiance[3 * i + 1] = L[1];
-    radiance[3 * i + 2] = L[2];
-    const f3 P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[i], fr.invPV);
-    const f3 d = ld3(fr.viewPos) - P;
-    const f3 n = decodeNormal(g.normal + 3 * i);
-    float f[SGI_KPCN_FEATURES];
-    f[0] = log1pf(gmax(L[0], 0.0f));
-    f[1] = log1pf(gmax(L[1], 0.0f));
-    f[2] = log1pf(gmax(L[2], 0.0f));
-    f[3] = g.albedo[4 * i + 0];
-    f[4] = g.albedo[4 * i + 1];
-    f[5] = g.albedo[4 * i + 2];
-    f[6] = n.x;
-    f[7] = n.y;
-    f[8] = n.z;
-    f[9] = log1pf(sqrtf(dot(d, d))) * 0.125f;
-    uint32_t o[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int a = 2 * j, b = 2 * j + 1;
-        const uint32_t lo = a < SGI_KPCN_FEATURES ? f2bf(f[a]) : 0u;
-        const uint32_t hi = b < SGI_KPCN_FEATURES ? f2bf(f[b]) : 0u;
-        o[j] = lo | (hi << 16);
-    }
-    uint4* dst = reinterpret_cast<uint4*>(feat + (size_t)i * kC);
-    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
-    dst[2] = make_uint4(o[8], o[9], o[10], o[11]);
-    dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
-}
-
-// Softmax weights of the 25 filter logits (max-subtracted).
-__device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
-    float m = lg[0];
-#pragma unroll
-    for (int j = 1; j < SGI_KPCN_TAPS; ++j) m = gmax(m, lg[j]);
-    float s = 0.0f;
-#pragma unroll
-    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
-        wgt[j] = expf(lg[j] - m);
-        s += wgt[j];
-    }
-    const float inv = 1.0f / s;
-#pragma unroll
-    for (int j = 0; j < SGI_KPCN_TAPS; ++j) wgt[j] *= inv;
-}
-
-__device__ __forceinline__ void loadLogits(const float* logits, size_t p, float* lg) {
-    const float4* src = reinterpret_cast<const float4*>(logits + p * kC);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const float4 v = src[j];
-        lg[4 * j] = v.x;
-        lg[4 * j + 1] = v.y;
-        lg[4 * j + 2] = v.z;
-        lg[4 * j + 3] = v.w;
-    }
-}
-
-__device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int x, int y, const float* wgt) {
-    f3 o = splat(0.0f);
-#pragma unroll
-    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
-        const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
-        o = o + ld3(radiance + 3 * ((size_t)yy * W + xx)) * wgt[j];
-    }
-    return o;
-}
-
 __global__ void applyKernel(const float* __restrict__ logits, const float* __restrict__ radiance, int H, int W,
                             float* __restrict__ out, float* __restrict__ err) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,17 +750,58 @@ int gridFor(int n, int b) { return (n + b - 1) / b; }
 
 namespace sgi_kpcn {
 
-hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
-                   int W, void* out, int outF32, hipStream_t s) {
+template <int IN, int OUT>
+hipError_t launchConv(const ConvSrc& src, const uint16_t* w, const float* bias, int relu, int H, int W, void* out,
+                      float* err, hipStream_t s) {
     const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
     const int grid = nTiles < 2048 ? nTiles : 2048;
-    if (outF32)
-        hipLaunchKernelGGL(conv3x3Kernel<true>, dim3(grid), dim3(256), 0, s, in, mask, w, bias, relu, H, W, tilesX,
-                           nTiles, out);
-    else
-        hipLaunchKernelGGL(conv3x3Kernel<false>, dim3(grid), dim3(256), 0, s, in, mask, w, bias, relu, H, W, tilesX,
-                           nTiles, out);
+    hipLaunchKernelGGL((conv3x3Kernel<IN, OUT>), dim3(grid), dim3(256), 0, s, src, w, bias, relu, H, W, tilesX,
+                       nTiles, out, err);
     return hipGetLastError();
+}
+
+hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
+                   int W, void* out, int outF32, hipStream_t s) {
+    ConvSrc src{};
+    src.a = in;
+    src.mask = mask;
+    if (outF32) return launchConv<IN_PLAIN, OUT_F32>(src, w, bias, relu, H, W, out, nullptr, s);
+    return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
+}
+
+hipError_t convFeatures(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, float* radiance,
+                        const uint16_t* w, const float* bias, uint16_t* out, hipStream_t s) {
+    ConvSrc src{};
+    src.g = g;
+    src.fr = fr;
+    src.reservoir = (const float4*)reservoir;
+    src.radiance = radiance;
+    return launchConv<IN_FEATURES, OUT_BF16>(src, w, bias, 1, g.H, g.W, out, nullptr, s);
+}
+
+hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
+                    hipStream_t s) {
+    ConvSrc src{};
+    src.a = fine;
+    src.fineH = fineH;
+    src.fineW = fineW;
+    return launchConv<IN_POOL, OUT_BF16>(src, w, bias, 1, (fineH + 1) / 2, (fineW + 1) / 2, out, nullptr, s);
+}
+
+hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
+                     uint16_t* out, hipStream_t s) {
+    ConvSrc src{};
+    src.a = skip;
+    src.b = low;
+    return launchConv<IN_UPADD, OUT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
+}
+
+hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, float* radiance,
+                     float* out, float* err, hipStream_t s) {
+    ConvSrc src{};
+    src.a = in;
+    src.radiance = radiance;
+    return launchConv<IN_PLAIN, OUT_APPLY>(src, w, bias, 0, H, W, out, err, s);
 }
 
 int wgradBlocks(int H, int W) {

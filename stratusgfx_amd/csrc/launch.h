@@ -61,6 +61,16 @@ namespace sgi_kpcn {
 
 hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
                    int W, void* out, int outF32, hipStream_t s);
+// Inference forms of the convolution with the neighbouring pass fused into the
+// input staging or the epilogue (same results as the separate kernels).
+hipError_t convFeatures(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir,
+                        float* radiance, const uint16_t* w, const float* bias, uint16_t* out, hipStream_t s);
+hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
+                    hipStream_t s);
+hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
+                     uint16_t* out, hipStream_t s);
+hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, float* radiance,
+                     float* out, float* err, hipStream_t s);
 int wgradBlocks(int H, int W);
 hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,
                  float* grad, hipStream_t s);
