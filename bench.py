@@ -1,29 +1,34 @@
 #!/usr/bin/env python3
 """GI frames/s of the deferred VPL GI pass on one MI355X: the learned form
-against the plain form at their relative MSE to the converged render.
+against the a-trous form at equal relative MSE to the converged render.
 
 A step is one GI frame on resident inputs, in RenderScene's order
-(StratusRendererBackend.cpp:1982, 2010-2045).  Both forms run on the
-procedural city block at 1920x1080 (6x256x256 FP16 distance cubes):
+(StratusRendererBackend.cpp:1982, 2010-2045), on the procedural city block at
+1920x1080 (6x256x256 FP16 distance cubes):
 
-  plain    4096 VPLs: global VPL sun-visibility cull and compaction, the
-           visible slots' atlas entries, deferred direct sun light with
-           cascaded PCF, per-tile VPL culling (16x16 tiles, up to 64 VPLs),
-           the tile-list gather and the 4-pass a-trous / temporal denoiser;
-  learned  1024 VPLs: the same culling and direct light, an 8-sample-per-pixel
-           gather over the visible VPLs and the learned denoiser (network +
-           5x5 predicted filter, a-trous on the tiles whose predicted error
-           exceeds the trained threshold) with the same temporal pass.
+  learned   1024 VPLs: global VPL sun-visibility cull and compaction, the
+            visible slots' atlas entries, deferred direct sun light with
+            cascaded PCF, an 8-sample-per-pixel gather over the visible VPLs
+            (the reference's hashed estimator, vpl_pbr_gi.fs:145-200, at a
+            fixed sample count) and the learned denoiser (network + 5x5
+            predicted filter, a-trous on the tiles whose predicted error
+            exceeds the trained threshold, the temporal pass and combine);
+  a-trous   4096 VPLs: the same steps with S samples per pixel and the 4-pass
+            a-trous / temporal denoiser, S swept over --atrous-spp; the
+            equal-relMSE point is the smallest S whose relMSE is at most the
+            learned form's;
+  tile list 4096 VPLs with per-tile VPL culling (16x16 tiles, up to 64 VPLs)
+            and the tile-list gather + a-trous: the parity-tested plain
+            pipeline, timed for its own throughput and roofline.
 
-relMSE (mean of (x - r)^2 / (r^2 + 1e-2), FP64) of each form's GI output
-after one frame against the converged render: every visible VPL of the
-4096-VPL scene with FP64 sums.  `value` is the learned form's frames/s;
-the plain form's figures are reported beside it.  Inputs are generated on
-the GPU before the timed region; the denoiser history alternates between two
-buffer sets as the reference swaps its FBOs.
+relMSE = mean of (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after
+one frame, against the converged render (every visible VPL of the 4096-VPL
+scene, FP64 sums).  `value` is the learned form's frames/s.  Inputs are
+generated on the GPU before the timed regions; the denoiser history alternates
+between two buffer sets as the reference swaps its FBOs.
 
-Prints one JSON line.  With the default flags it also times the CPU oracle
-on a bounded sample of the plain frame as the CPU baseline.
+Prints one JSON line.  With the default flags it also times the CPU oracle on
+a bounded sample of the tile-list frame as the CPU baseline.
 """
 import argparse
 import json
@@ -45,6 +50,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
     ap.add_argument("--spp", type=int, default=8, help="gather samples per pixel of the learned form")
+    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[4, 8, 16, 32, 64, 128],
+                    help="samples per pixel of the a-trous form's sweep")
+    ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
+    ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
+                    help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--cpu-rows", type=int, default=24, help="image rows of the CPU baseline sample")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -253,9 +263,15 @@ def main():
     plain.gi.gather_ex(plain.g, plain.fb.vpl_updated, plain.fb.visible, plain.fb.visible_entries, plain.cubes, None,
                        conv_col, conv_res, mode=_abi.GATHER_ALL, accumulate_fp64=True)
     target = (conv_res[..., :3] / conv_res[..., 3:4].clamp_min(1e-5)).contiguous()
+    # how far the 1024-VPL scene's own converged render is from it (the learned form's floor)
+    learned.cull()
+    learned.gi.gather_ex(learned.g, learned.fb.vpl_updated, learned.fb.visible, learned.fb.visible_entries,
+                         learned.cubes, None, conv_col, conv_res, mode=_abi.GATHER_ALL, accumulate_fp64=True)
+    floor_rel = rel_mse(conv_res[..., :3] / conv_res[..., 3:4].clamp_min(1e-5), target)
     del conv_col, conv_res
 
-    # ---- plain form ----------------------------------------------------------
+    # ---- plain forms ---------------------------------------------------------
+    # (1) the pipeline of the reference with tile lists (the parity-tested path): timed, roofline
     use_tiles = args.gather == "tiles"
 
     def plain_step(k, ev, hist=True):
@@ -282,30 +298,18 @@ def main():
     gi_plain = plain_step(0, None, hist=False)
     torch.cuda.synchronize()
     plain_rel = rel_mse(gi_plain, target)
-    plain_noisy = rel_mse(plain.fb.reservoir[..., :3] / plain.fb.reservoir[..., 3:4].clamp_min(1e-5), target)
-    print(f"[bench] timing {args.steps} plain frames", file=sys.stderr, flush=True)
-    el_plain, per_plain = timed_loop(plain_step, args.steps, args.warmup, 5)
+    el_plain, per_plain = None, None
+    if args.forms in ("all", "tiles"):
+        print(f"[bench] timing {args.steps} plain frames", file=sys.stderr, flush=True)
+        el_plain, per_plain = timed_loop(plain_step, args.steps, args.warmup, 5)
     tiles_np = plain.fb.tiles.cpu().numpy()
     visible_plain = int(plain.fb.visible[0].item())
 
-    # ---- learned form ----------------------------------------------------------
-    weights_path = kpcn.DEFAULT_WEIGHTS
-    learned_res = None
-    el_learned = per_learned = None
-    if os.path.exists(weights_path):
-        meta_path = os.path.splitext(weights_path)[0] + ".json"
-        threshold = float("inf")
-        if os.path.exists(meta_path):
-            with open(meta_path) as f:
-                threshold = float(json.load(f).get("threshold", float("inf")))
-        wts = kpcn.Weights(kpcn.load_params(weights_path))
-        net = wts.net()
-        ws = learned.gi.kpcn_workspace()
-        n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
-        fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
-
-        def learned_step(k, ev, hist=True, thr=threshold):
-            p, fb = learned, learned.fb
+    # (2) the a-trous form at equal relMSE: the reference's estimator (hashed slots over
+    # every visible VPL, <= 5 rejections) at S samples per pixel + the a-trous denoiser
+    def atrous_step(S):
+        def step(k, ev, hist=True):
+            p, fb = plain, plain.fb
             prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
             out_gi, out_h = p.hist[(k + 1) % 2]
             rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
@@ -315,45 +319,30 @@ def main():
             p.gi.direct_light(p.g, fb.direct)
             rec(2)
             p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
-                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp)
+                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S)
             rec(3)
-            p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net, thr,
-                                 ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
+            p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
+                         out_gi, fb.reservoir_out, out_h)
             rec(4)
             return out_gi
+        return step
 
-        gi_l = learned_step(0, None, hist=False)
+    sweep = []
+    for S in (args.atrous_spp if args.forms == "all" else []):
+        st = atrous_step(S)
+        g_s = st(0, None, hist=False)
         torch.cuda.synchronize()
-        learned_rel = rel_mse(gi_l, target)
-        fallback_tiles = int(fbt[0].item())
-        learned_noisy = rel_mse(learned.fb.reservoir[..., :3] / learned.fb.reservoir[..., 3:4].clamp_min(1e-5),
-                                target)
-        gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
-        torch.cuda.synchronize()
-        learned_rel_nofallback = rel_mse(gi_nf, target)
-        # a-trous alone on the same 8-sample input
-        fb = learned.fb
-        tmp = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)] + [torch.zeros((H, W), **f32)]
-        learned.gi.denoise(learned.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, None, None, *tmp)
-        torch.cuda.synchronize()
-        atrous_8spp = rel_mse(tmp[1], target)
-        del tmp
-        print(f"[bench] timing {args.steps} learned frames", file=sys.stderr, flush=True)
-        el_learned, per_learned = timed_loop(learned_step, args.steps, args.warmup, 4)
-        learned_res = {
-            "frames_per_s": round(args.steps / el_learned, 2), "ms_per_step": round(el_learned * 1e3 / args.steps, 4),
-            "relmse": round(learned_rel, 6), "relmse_without_fallback": round(learned_rel_nofallback, 6),
-            "relmse_noisy_input": round(learned_noisy, 6), "relmse_atrous_on_same_input": round(atrous_8spp, 6),
-            "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
-            "kernels_ms": {k: round(v, 4) for k, v in
-                           zip(["cull_global", "direct_light", "gather_8spp", "denoise_learned"], per_learned)},
-            "vpls": w_learned.num_vpls, "visible_vpls": int(learned.fb.visible[0].item()),
-            "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
-        }
+        rm = rel_mse(g_s, target)
+        el_s, per_s = timed_loop(st, args.sweep_steps, 2, 4)
+        sweep.append({"spp": S, "relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_s, 2),
+                      "kernels_ms": {k: round(v, 4) for k, v in
+                                     zip(["cull_global", "direct_light", "gather", "denoise"], per_s)}})
+        print(f"[bench] a-trous at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
+              file=sys.stderr, flush=True)
 
     texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
     gather_roof = None
-    if use_tiles:
+    if use_tiles and per_plain is not None:
         alg_bytes, pairs = gather_algorithmic_bytes(tiles_np, w_plain.tile_size, W, H, w_plain.max_vpls_per_tile,
                                                     texel_bytes)
         achieved = alg_bytes / (per_plain[3] * 1e-3) / 1e9
@@ -363,21 +352,32 @@ def main():
                        "traffic": load_traffic("gather_city1080p_traffic.json"),
                        "algorithmic_bytes": alg_bytes, "pixel_vpl_pairs": pairs,
                        "avg_launch_ms": round(per_plain[3], 4)}
-    plain_res = {
-        "frames_per_s": round(args.steps / el_plain, 2), "ms_per_step": round(el_plain * 1e3 / args.steps, 4),
-        "relmse": round(plain_rel, 6), "relmse_undenoised": round(plain_noisy, 6),
-        "kernels_ms": {k: round(v, 4) for k, v in
-                       zip(["cull_global", "direct_light", "cull_tiles", "gather", "denoise"], per_plain)},
-        "vpls": w_plain.num_vpls, "visible_vpls": visible_plain, "gather": args.gather,
-    }
+    plain_res = None
+    if per_plain is not None:
+        plain_res = {
+            "frames_per_s": round(args.steps / el_plain, 2), "ms_per_step": round(el_plain * 1e3 / args.steps, 4),
+            "relmse": round(plain_rel, 6),
+            "kernels_ms": {k: round(v, 4) for k, v in
+                           zip(["cull_global", "direct_light", "cull_tiles", "gather", "denoise"], per_plain)},
+            "vpls": w_plain.num_vpls, "visible_vpls": visible_plain, "gather": args.gather,
+        }
     if learned_res is not None:
         value, el, form = learned_res["frames_per_s"], el_learned, "learned"
-        learned_res["speedup_vs_plain"] = round(learned_res["frames_per_s"] / plain_res["frames_per_s"], 3)
-        learned_res["relmse_at_most_plain"] = learned_res["relmse"] <= plain_res["relmse"]
+        if sweep:
+            ok = [r for r in sweep if r["relmse"] <= learned_res["relmse"]]
+            eq = min(ok, key=lambda r: r["spp"]) if ok else max(sweep, key=lambda r: r["spp"])
+            learned_res["atrous_at_equal_relmse"] = {**eq, "reached": bool(ok)}
+            learned_res["speedup_vs_atrous_at_equal_relmse"] = round(
+                learned_res["frames_per_s"] / eq["frames_per_s"], 3)
+        if plain_res is not None:
+            learned_res["speedup_vs_tile_list_plain"] = round(
+                learned_res["frames_per_s"] / plain_res["frames_per_s"], 3)
         roofline = conv_roofline(wts, H, W)
-    else:
-        value, el, form = plain_res["frames_per_s"], el_plain, "plain (no trained weights found)"
+    elif plain_res is not None:
+        value, el, form = plain_res["frames_per_s"], el_plain, "tile-list plain form only"
         roofline = gather_roof
+    else:
+        raise SystemExit("bench.py: nothing was timed (no trained weights for the learned form?)")
     result = {
         "metric": METRIC,
         "value": value,
@@ -391,16 +391,19 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 shading, bf16 network (fp32 accumulation)" if learned_res else "fp32",
         "data": "synthetic: seeded procedural city block, ray-cast G-buffer / VPL cubes / cascades",
-        "config": {"workload": "city-1920x1080 learned 1024 VPLs 8 spp vs plain 4096 VPLs tiles16-K64",
+        "config": {"workload": "city-1920x1080: learned 1024 VPLs 8 spp + network vs a-trous 4096 VPLs at "
+                               "equal relMSE",
                    "width": W, "height": H, "cube": f"6x{w_plain.cube_res}x{w_plain.cube_res} fp16",
-                   "form": form, "converged": "all visible VPLs of the 4096-VPL scene, FP64 sums"},
+                   "form": form, "converged": "all visible VPLs of the 4096-VPL scene, FP64 sums",
+                   "converged_1024_vs_4096_relmse": round(floor_rel, 6)},
         "learned": learned_res,
-        "plain": plain_res,
+        "atrous_sweep": sweep,
+        "plain_tile_lists": plain_res,
         "roofline": roofline,
         "roofline_plain_gather": gather_roof,
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)
         result["cpu_baseline"] = cpu_baseline(host, w_plain, args.cpu_rows, visible_plain)

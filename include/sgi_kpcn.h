@@ -54,6 +54,7 @@ extern "C" {
 #define SGI_KPCN_TAPS 25          /* 5x5 predicted filter */
 #define SGI_KPCN_FEATURES 10      /* log radiance 3, albedo 3, normal 3, depth 1 */
 #define SGI_KPCN_TILE 16          /* fallback decision granularity (pixels) */
+#define SGI_KPCN_RADIANCE_SCALE 64.0f  /* radiance feature = log(1 + 64 radiance) */
 
 /* A trained network in inference form. */
 typedef struct sgi_kpcn_net {
@@ -97,7 +98,8 @@ int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t wid
 /* Network input from a gather: radiance = reservoir.rgb / max(reservoir.a,
  * 1e-5) (FP32 [h][w][3], the a-trous filter's pass-0 normalisation,
  * vpl_pbr_gi_denoise.fs:329) and BF16 features [h][w][32]: log(1 +
- * radiance), albedo, decoded normal, log(1 + view distance) / 8, zeros.
+ * SGI_KPCN_RADIANCE_SCALE * radiance), albedo, decoded normal, log(1 + view
+ * distance) / 8, zeros.
  * Needs sgi_set_frame. */
 int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const float* reservoir, uint16_t* features,
                       float* radiance, void* stream);

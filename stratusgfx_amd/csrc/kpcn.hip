@@ -38,6 +38,7 @@ constexpr int kHS = kTS + 2;  // halo tile edge
 constexpr int kPxU4 = 5;      // LDS uint4 slots per staged pixel: 4 used, 1 pad (bank spread)
 constexpr int kW = SGI_KPCN_LAYER_WEIGHTS;
 constexpr int kP = SGI_KPCN_LAYER_PARAMS;
+constexpr float kRadianceScale = SGI_KPCN_RADIANCE_SCALE;
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
@@ -83,9 +84,9 @@ __device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs
     const f3 d = ld3(fr.viewPos) - P;
     const f3 n = decodeNormal(g.normal + 3 * i);
     float f[SGI_KPCN_FEATURES];
-    f[0] = log1pf(gmax(L[0], 0.0f));
-    f[1] = log1pf(gmax(L[1], 0.0f));
-    f[2] = log1pf(gmax(L[2], 0.0f));
+    f[0] = log1pf(gmax(L[0], 0.0f) * kRadianceScale);
+    f[1] = log1pf(gmax(L[1], 0.0f) * kRadianceScale);
+    f[2] = log1pf(gmax(L[2], 0.0f) * kRadianceScale);
     f[3] = g.albedo[4 * i + 0];
     f[4] = g.albedo[4 * i + 1];
     f[5] = g.albedo[4 * i + 2];

@@ -1,6 +1,6 @@
 """Trains the learned denoiser on pairs the gather itself generates, on the GPU.
 
-    python -m stratusgfx_amd.train_kpcn --scenes 48 --views 2 --size 512 --steps 1500
+    python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --size 512 --steps 4000
 
 Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
 the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES) of
@@ -145,16 +145,19 @@ def main(argv=None) -> None:
     import torch
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--views", type=int, default=2)
     ap.add_argument("--val-scenes", type=int, default=4)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--vpls", type=int, default=4096)
     ap.add_argument("--spp", type=int, default=8)
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--err-weight", type=float, default=0.02)
+    ap.add_argument("--err-weight", type=float, default=1e-4,
+                    help="weight of the error head's loss; its gradient otherwise swamps the filter's")
+    ap.add_argument("--loss-scale", type=float, default=1000.0,
+                    help="gradient scale (the relative-MSE gradients are ~1e-9 per logit at these radiances)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=kpcn.DEFAULT_WEIGHTS)
     ap.add_argument("--log", default=None)
@@ -185,7 +188,7 @@ def main(argv=None) -> None:
         for j in rng.choice(len(train), a.batch, replace=False):
             s = train[j]
             tr.forward(s.feat)
-            tr.loss_backward(s.rad, s.target, err_weight=a.err_weight, scale=1.0 / a.batch)
+            tr.loss_backward(s.rad, s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch)
         tr.step(lr)
         if step % 50 == 0 or step == 1:
             ls = tr.loss_sums.cpu().numpy() / (a.batch * a.size * a.size)
@@ -207,6 +210,7 @@ def main(argv=None) -> None:
     kpcn.save_params(wts.params.cpu().numpy(), a.out)
     summary = {"train_views": len(train), "val_views": len(val), "size": a.size, "vpls": a.vpls, "spp": a.spp,
                "steps": a.steps, "batch": a.batch, "lr": a.lr, "err_weight": a.err_weight,
+               "loss_scale": a.loss_scale,
                "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
                "before": base, "after": after, "threshold": best_t, "history": hist}
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
