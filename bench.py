@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--atrous-spp", type=int, nargs="+", default=[4, 8, 16, 32, 64, 128],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
+    ap.add_argument("--rejection-limit", type=int, default=-1,
+                    help="sampled gathers: rejections before draws are taken unconditionally (5 = the reference's "
+                         "rule; -1 = never, so the estimate converges to the every-VPL render)")
     ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
                     help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
@@ -305,8 +308,8 @@ def main():
     tiles_np = plain.fb.tiles.cpu().numpy()
     visible_plain = int(plain.fb.visible[0].item())
 
-    # (2) the a-trous form at equal relMSE: the reference's estimator (hashed slots over
-    # every visible VPL, <= 5 rejections) at S samples per pixel + the a-trous denoiser
+    # (2) the a-trous form at equal relMSE: the reference's hashed estimator over every
+    # visible VPL at S samples per pixel + the a-trous denoiser
     def atrous_step(S):
         def step(k, ev, hist=True):
             p, fb = plain, plain.fb
@@ -319,7 +322,7 @@ def main():
             p.gi.direct_light(p.g, fb.direct)
             rec(2)
             p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
-                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S)
+                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S, rejection_limit=args.rejection_limit)
             rec(3)
             p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                          out_gi, fb.reservoir_out, out_h)
@@ -338,6 +341,72 @@ def main():
                       "kernels_ms": {k: round(v, 4) for k, v in
                                      zip(["cull_global", "direct_light", "gather", "denoise"], per_s)}})
         print(f"[bench] a-trous at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
+              file=sys.stderr, flush=True)
+
+    # ---- learned form ----------------------------------------------------------
+    weights_path = kpcn.DEFAULT_WEIGHTS
+    learned_res = None
+    el_learned = None
+    wts = None
+    if os.path.exists(weights_path) and args.forms in ("all", "learned"):
+        meta_path = os.path.splitext(weights_path)[0] + ".json"
+        threshold = float("inf")
+        if os.path.exists(meta_path):
+            with open(meta_path) as f:
+                threshold = float(json.load(f).get("threshold", float("inf")))
+        wts = kpcn.Weights(kpcn.load_params(weights_path))
+        net = wts.net()
+        ws = learned.gi.kpcn_workspace()
+        n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+        fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
+
+        def learned_step(k, ev, hist=True, thr=threshold):
+            p, fb = learned, learned.fb
+            prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
+            out_gi, out_h = p.hist[(k + 1) % 2]
+            rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
+            rec(0)
+            p.cull()
+            rec(1)
+            p.gi.direct_light(p.g, fb.direct)
+            rec(2)
+            p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
+                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp,
+                           rejection_limit=args.rejection_limit)
+            rec(3)
+            p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net, thr,
+                                 ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
+            rec(4)
+            return out_gi
+
+        gi_l = learned_step(0, None, hist=False)
+        torch.cuda.synchronize()
+        learned_rel = rel_mse(gi_l, target)
+        fallback_tiles = int(fbt[0].item())
+        fb = learned.fb
+        learned_noisy = rel_mse(fb.reservoir[..., :3] / fb.reservoir[..., 3:4].clamp_min(1e-5), target)
+        gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
+        torch.cuda.synchronize()
+        learned_rel_nofallback = rel_mse(gi_nf, target)
+        # the a-trous filter alone on the same 8-sample input
+        tmp = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)] + [torch.zeros((H, W), **f32)]
+        learned.gi.denoise(learned.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, None, None, *tmp)
+        torch.cuda.synchronize()
+        atrous_same_input = rel_mse(tmp[1], target)
+        del tmp
+        print(f"[bench] timing {args.steps} learned frames", file=sys.stderr, flush=True)
+        el_learned, per_learned = timed_loop(learned_step, args.steps, args.warmup, 4)
+        learned_res = {
+            "frames_per_s": round(args.steps / el_learned, 2), "ms_per_step": round(el_learned * 1e3 / args.steps, 4),
+            "relmse": round(learned_rel, 6), "relmse_without_fallback": round(learned_rel_nofallback, 6),
+            "relmse_noisy_input": round(learned_noisy, 6), "relmse_atrous_on_same_input": round(atrous_same_input, 6),
+            "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
+            "kernels_ms": {k: round(v, 4) for k, v in
+                           zip(["cull_global", "direct_light", "gather_8spp", "denoise_learned"], per_learned)},
+            "vpls": w_learned.num_vpls, "visible_vpls": int(fb.visible[0].item()),
+            "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
+        }
+        print(f"[bench] learned: relMSE {learned_rel:.5f}, {learned_res['frames_per_s']} frames/s",
               file=sys.stderr, flush=True)
 
     texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4

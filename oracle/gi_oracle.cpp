@@ -464,7 +464,7 @@ enum GatherMode { RANDOM = 0, TILES = 1, CONVERGED = 2, SAMPLES = 3 };
 template <class R>
 void gatherPixel(const sgi_frame& fr, const GB& g, int x, int y, const sgi_vpl* vpls, int numVisible,
                  const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases& cubes, const int32_t* tileList,
-                 int mode, int samples, R outColor[3], R outRes[4]) {
+                 int mode, int samples, int rejectionLimit, R outColor[3], R outRes[4]) {
     const Settings<R> st = settingsOf<R>(fr);
     const int W = g.depth.W, H = g.depth.H;
     const R u = R((float)((float)x + 0.5f) / (float)W), v = R((float)((float)y + 0.5f) / (float)H);
@@ -545,16 +545,22 @@ void gatherPixel(const sgi_frame& fr, const GB& g, int x, int y, const sgi_vpl* 
         const int n = tileList[0];
         for (int k = 0; k < n; ++k) evalVpl(tileList[1 + k], true);
     } else if (mode == SAMPLES) {
-        // the fixed-count estimator of the learned form: slot int(count * rand) over
-        // all visible slots, the reference's hash, seed step and <= 5 rejections
+        // the fixed-count estimator: slot int(count * rand) over all visible slots,
+        // the reference's hash and seed step; after `rejectionLimit` rejections the
+        // draws are taken unconditionally (the reference's rule is 5), and with a
+        // negative limit every failing draw is rejected, with at most 32 draws per
+        // requested sample
         float seedZ = fr.time;
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
-        for (int i = 0, resamples = 0; numVisible > 0 && i < samples; i += 1) {
+        const int maxDraws = 32 * samples;
+        for (int i = 0, resamples = 0, draws = 0; numVisible > 0 && i < samples; i += 1) {
+            if (rejectionLimit < 0 && draws >= maxDraws) break;
+            ++draws;
             const float rnd = random3(seedX, seedY, seedZ);
             seedZ += 10000.0f;
             int lightIndex = (int)((float)numVisible * rnd);
             if (lightIndex > numVisible - 1) lightIndex = numVisible - 1;
-            if (!evalVpl(lightIndex, resamples < 5)) {
+            if (!evalVpl(lightIndex, rejectionLimit < 0 || resamples < rejectionLimit)) {
                 ++resamples;
                 --i;
                 continue;
@@ -920,11 +926,12 @@ void orc_cull_tiles(const sgi_frame* fr, const sgi_gbuffer* g, int ts, int K, co
 }
 
 // Gather over pixel rows [y0, y1).  mode: 0 random (reference), 1 tile lists, 2 every visible VPL,
-// 3 `samples` hashed draws over every visible VPL.
+// 3 `samples` hashed draws over every visible VPL (rejectionLimit as gatherPixel).
 // precision 0 = FP32 restatement (float outputs), 1 = FP64 twin (double outputs).
 void orc_gather(const sgi_frame* fr, const sgi_gbuffer* g, const sgi_vpl* vpls, int numVisible,
                 const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles, int ts,
-                int K, int mode, int samples, int precision, int y0, int y1, void* color, void* reservoir) {
+                int K, int mode, int samples, int rejectionLimit, int precision, int y0, int y1, void* color,
+                void* reservoir) {
     const GB gb = wrap(*g);
     const int W = g->width, tx = (W + ts - 1) / ts;
     for (int y = y0; y < y1; ++y)
@@ -934,12 +941,14 @@ void orc_gather(const sgi_frame* fr, const sgi_gbuffer* g, const sgi_vpl* vpls, 
             const size_t p = (size_t)(y - y0) * W + x;
             if (precision == 0) {
                 float c[3], r[4];
-                gatherPixel<float>(*fr, gb, x, y, vpls, numVisible, shadowIndices, *cubes, list, mode, samples, c, r);
+                gatherPixel<float>(*fr, gb, x, y, vpls, numVisible, shadowIndices, *cubes, list, mode, samples,
+                                   rejectionLimit, c, r);
                 if (color) for (int k = 0; k < 3; ++k) ((float*)color)[p * 3 + k] = c[k];
                 for (int k = 0; k < 4; ++k) ((float*)reservoir)[p * 4 + k] = r[k];
             } else {
                 double c[3], r[4];
-                gatherPixel<double>(*fr, gb, x, y, vpls, numVisible, shadowIndices, *cubes, list, mode, samples, c, r);
+                gatherPixel<double>(*fr, gb, x, y, vpls, numVisible, shadowIndices, *cubes, list, mode, samples,
+                                    rejectionLimit, c, r);
                 if (color) for (int k = 0; k < 3; ++k) ((double*)color)[p * 3 + k] = c[k];
                 for (int k = 0; k < 4; ++k) ((double*)reservoir)[p * 4 + k] = r[k];
             }

@@ -59,7 +59,7 @@ def lib() -> ctypes.CDLL:
                                       c_int, c_int, c_int, c_void_p, c_void_p]),
             "orc_gather": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_void_p, c_int, c_void_p,
                                   POINTER(_abi.CubeAtlases), c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
-                                  c_int, c_void_p, c_void_p]),
+                                  c_int, c_int, c_void_p, c_void_p]),
             "orc_direct_light": (None, [POINTER(_abi.Frame), POINTER(_abi.Csm), POINTER(_abi.Gbuffer), c_int, c_int,
                                         c_void_p]),
             "orc_denoise": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), POINTER(_abi.Gbuffer), c_int]
@@ -109,7 +109,7 @@ def cull_tiles(frame, gbuf, tile_size, K, vpls, visible, remap=0, t0=0, t1=None,
 
 
 def gather(frame, gbuf, vpls, num_visible, entries, cubes, tiles=None, tile_size=16, K=12, mode=GATHER_RANDOM,
-           precision=0, y0=0, y1=None, samples=8):
+           precision=0, y0=0, y1=None, samples=8, rejection_limit=5):
     """vpl_pbr_gi.fs over rows [y0, y1): returns (color [rows,W,3], reservoir [rows,W,4])."""
     W, H = gbuf.width, gbuf.height
     y1 = H if y1 is None else y1
@@ -120,7 +120,7 @@ def gather(frame, gbuf, vpls, num_visible, entries, cubes, tiles=None, tile_size
     entries = np.ascontiguousarray(entries, np.int32)
     t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
     lib().orc_gather(ctypes.byref(frame), ctypes.byref(gbuf), _p(vpls), num_visible, _p(entries), ctypes.byref(cubes),
-                     _p(t), tile_size, K, mode, samples, precision, y0, y1, _p(color), _p(res))
+                     _p(t), tile_size, K, mode, samples, rejection_limit, precision, y0, y1, _p(color), _p(res))
     return color, res
 
 

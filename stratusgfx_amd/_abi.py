@@ -98,7 +98,8 @@ class FrameIO(ctypes.Structure):
 
 
 class GatherOpts(ctypes.Structure):
-    _fields_ = [("mode", c_int32), ("samples", c_int32), ("precision", c_int32), ("accumulate_fp64", c_int32)]
+    _fields_ = [("mode", c_int32), ("samples", c_int32), ("precision", c_int32), ("accumulate_fp64", c_int32),
+                ("rejection_limit", c_int32)]
 
 
 GATHER_REFERENCE, GATHER_TILES, GATHER_ALL, GATHER_SAMPLES = 0, 1, 2, 3

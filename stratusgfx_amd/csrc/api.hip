@@ -22,6 +22,7 @@ struct sgi_ctx {
     sgi_dev::CsmArgs csmArgs{};
     sgi_launch::DenoiseBuffers ws{};
     uint8_t* visFlags = nullptr;  // [SGI_MAX_TOTAL_VPLS_BEFORE_CULLING] per-VPL sun visibility
+    void* vplPrep = nullptr;      // [SGI_MAX_TOTAL_VPLS_PER_FRAME] packed records of the sampled gathers
     int device = 0;
 };
 
@@ -73,6 +74,7 @@ int sgi_create(const sgi_config* config, sgi_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->ws.resB, plane);
     if (e == hipSuccess) e = hipMalloc(&ctx->ws.normals, plane);
     if (e == hipSuccess) e = hipMalloc(&ctx->visFlags, SGI_MAX_TOTAL_VPLS_BEFORE_CULLING);
+    if (e == hipSuccess) e = hipMalloc(&ctx->vplPrep, SGI_MAX_TOTAL_VPLS_PER_FRAME * sgi_launch::packedVplBytes());
     if (e != hipSuccess) {
         sgi_destroy(ctx);
         return hipFail(e, "sgi_create: workspace allocation");
@@ -87,6 +89,7 @@ void sgi_destroy(sgi_ctx* ctx) {
     if (ctx->ws.resB) (void)hipFree(ctx->ws.resB);
     if (ctx->ws.normals) (void)hipFree(ctx->ws.normals);
     if (ctx->visFlags) (void)hipFree(ctx->visFlags);
+    if (ctx->vplPrep) (void)hipFree(ctx->vplPrep);
     delete ctx;
 }
 
@@ -208,8 +211,8 @@ int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const in
     if ((r = cubeArgs(cubes, "sgi_gather", ca)) != SGI_OK) return r;
     const hipError_t e = sgi_launch::gather(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format,
                                             ctx->cfg.tile_size, ctx->tilesX, ctx->tilesY, ctx->cfg.max_vpls_per_tile,
-                                            vpls, visibleIndices, shadowIndices, tiles, precision, color, reservoir,
-                                            (hipStream_t)stream);
+                                            vpls, visibleIndices, shadowIndices, tiles, precision, ctx->vplPrep, color,
+                                            reservoir, (hipStream_t)stream);
     if (e == hipErrorNotSupported) return fail(SGI_E_INVALID, "sgi_gather: this precision is not built");
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather");
 }
@@ -234,8 +237,9 @@ int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const
     sgi_dev::CubeArgs ca;
     if ((r = cubeArgs(cubes, "sgi_gather_ex", ca)) != SGI_OK) return r;
     const hipError_t e = sgi_launch::gatherEx(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format, opts->mode,
-                                              opts->samples, opts->accumulate_fp64, vpls, visibleIndices,
-                                              shadowIndices, color, reservoir, (hipStream_t)stream);
+                                              opts->samples, opts->rejection_limit, opts->accumulate_fp64, vpls,
+                                              visibleIndices,
+                                              shadowIndices, ctx->vplPrep, color, reservoir, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather_ex");
 }
 

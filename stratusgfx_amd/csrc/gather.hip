@@ -48,6 +48,49 @@ __device__ __forceinline__ Vpl loadVpl(const sgi_vpl* v, const sgi_atlas_entry* 
     return o;
 }
 
+// A visible VPL's fields as the gather reads them, prepared once per frame for
+// the sampled estimators (which fetch a random record per sample): 64 B, four
+// 16-B loads, with the cube's base address already resolved.
+struct PackedVpl {
+    float4 posRadius;   // position, radius
+    float4 specInvR;    // specular position, 1 / radius
+    float4 colorFar;    // colour, farPlane
+    const void* cube;
+    uint64_t pad;
+};
+
+__device__ __forceinline__ PackedVpl packVpl(const Vpl& v) {
+    PackedVpl o;
+    o.posRadius = make_float4(v.pos.x, v.pos.y, v.pos.z, v.radius);
+    o.specInvR = make_float4(v.spec.x, v.spec.y, v.spec.z, v.invRadius);
+    o.colorFar = make_float4(v.color.x, v.color.y, v.color.z, v.farPlane);
+    o.cube = v.cube;
+    o.pad = 0;
+    return o;
+}
+
+__device__ __forceinline__ Vpl unpackVpl(const PackedVpl& p) {
+    Vpl v;
+    v.pos = f3{p.posRadius.x, p.posRadius.y, p.posRadius.z};
+    v.radius = p.posRadius.w;
+    v.spec = f3{p.specInvR.x, p.specInvR.y, p.specInvR.z};
+    v.invRadius = p.specInvR.w;
+    v.color = f3{p.colorFar.x, p.colorFar.y, p.colorFar.z};
+    v.farPlane = p.colorFar.w;
+    v.cube = p.cube;
+    return v;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void prepVplsKernel(const sgi_vpl* __restrict__ vpls,
+                                                     const sgi_atlas_entry* __restrict__ entries,
+                                                     const int32_t* __restrict__ visible, CubeArgs cubes,
+                                                     PackedVpl* __restrict__ out) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int count = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
+    if (s < count) out[s] = packVpl(loadVpl<FMT>(vpls + s, entries + s, cubes));
+}
+
 // vplDiffuseAttenuation (pbr.glsl:260-267) with the light distance known:
 // 1 / (mix(10, 1, exp(-3 min(d / R, 1))) R + d^2).
 __device__ __forceinline__ float vplAttenuationFast(float lightDist, float radius, float invRadius) {
@@ -378,11 +421,28 @@ __global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameAr
     writeOut(g, p, vplColor, validSamples, color, reservoir);
 }
 
+// vpl_pbr_gi.fs:158-161: slot int(maxRandomIndex * random(fragCoord, seedZ)),
+// then the seed moves on by 10000.
+__device__ __forceinline__ int drawSlot(float seedX, float seedY, float& seedZ, int maxRandomIndex) {
+    const float rnd = random3(seedX, seedY, seedZ);
+    seedZ += 10000.0f;
+    return (int)((float)maxRandomIndex * rnd);
+}
+
+// The same draw over every visible slot (the last one included).
+__device__ __forceinline__ int drawSlotAll(float seedX, float seedY, float& seedZ, int count) {
+    const float rnd = random3(seedX, seedY, seedZ);
+    seedZ += 10000.0f;
+    return min((int)((float)count * rnd), count - 1);
+}
+
+// The sampled estimators draw one hashed slot per loop iteration whether the
+// sample is accepted or rejected, so the slot of iteration k + 1 is known before
+// iteration k is shaded: its record is fetched one iteration ahead.
 template <int FMT, bool HALF>
 __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
-                                                         int nTiles, const sgi_vpl* __restrict__ vpls,
+                                                         int nTiles, const PackedVpl* __restrict__ prep,
                                                          const int32_t* __restrict__ visible,
-                                                         const sgi_atlas_entry* __restrict__ entries,
                                                          float* __restrict__ color, float* __restrict__ reservoir) {
     const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
     const int tx = (int)t % tilesX, ty = (int)t / tilesX;
@@ -409,11 +469,10 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
             const PixelH q = toHalf(px);
             LightH pending = noLightH();
             bool hasPending = false;
+            PackedVpl next = prep[drawSlot(seedX, seedY, seedZ, maxRandomIndex)];
             for (int i = 0, resamples = 0; i < samplesMax; i += 1) {
-                const float rnd = random3(seedX, seedY, seedZ);
-                seedZ += 10000.0f;
-                const int lightIndex = (int)((float)maxRandomIndex * rnd);
-                const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
+                const Vpl L = unpackVpl(next);
+                next = prep[drawSlot(seedX, seedY, seedZ, maxRandomIndex)];
                 Hit h;
                 if (!visibility<FMT>(px, fr, cubes, L, resamples < 5, validSamples, h)) {
                     ++resamples;
@@ -427,11 +486,10 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
             }
             if (hasPending) shadePairH(q, pending, noLightH(), vplColor);
         } else {
+            PackedVpl next = prep[drawSlot(seedX, seedY, seedZ, maxRandomIndex)];
             for (int i = 0, resamples = 0; i < samplesMax; i += 1) {
-                const float rnd = random3(seedX, seedY, seedZ);
-                seedZ += 10000.0f;
-                const int lightIndex = (int)((float)maxRandomIndex * rnd);
-                const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
+                const Vpl L = unpackVpl(next);
+                next = prep[drawSlot(seedX, seedY, seedZ, maxRandomIndex)];
                 if (!evalVpl<FMT>(px, fr, cubes, L, resamples < 5, vplColor, validSamples)) {
                     ++resamples;
                     --i;
@@ -494,12 +552,12 @@ __global__ __launch_bounds__(256) void gatherAllKernel(GbufArgs g, FrameArgs fr,
 }
 
 // `samples` hashed-random slots per pixel over all visible VPLs, with the
-// reference's hash, seed stepping and <= 5 rejections.
+// reference's hash and seed stepping; the rejection rule as sgi_gather_opts.
 template <int FMT>
 __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
-                                                          int nTiles, int samples, const sgi_vpl* __restrict__ vpls,
+                                                          int nTiles, int samples, int rejectionLimit,
+                                                          const PackedVpl* __restrict__ prep,
                                                           const int32_t* __restrict__ visible,
-                                                          const sgi_atlas_entry* __restrict__ entries,
                                                           float* __restrict__ color, float* __restrict__ reservoir) {
     const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
     const int tx = (int)t % tilesX, ty = (int)t / tilesX;
@@ -513,12 +571,15 @@ __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs
     if (count > 0) {
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
         float seedZ = fr.time;
-        for (int i = 0, resamples = 0; i < samples; i += 1) {
-            const float rnd = random3(seedX, seedY, seedZ);
-            seedZ += 10000.0f;
-            const int lightIndex = min((int)((float)count * rnd), count - 1);
-            const Vpl L = loadVpl<FMT>(vpls + lightIndex, entries + lightIndex, cubes);
-            if (!evalVpl<FMT>(px, fr, cubes, L, resamples < 5, vplColor, validSamples)) {
+        const int maxDraws = 32 * samples;
+        PackedVpl next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+        for (int i = 0, resamples = 0, draws = 0; i < samples; i += 1) {
+            if (rejectionLimit < 0 && draws >= maxDraws) break;
+            ++draws;
+            const Vpl L = unpackVpl(next);
+            next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+            const bool allowReject = rejectionLimit < 0 || resamples < rejectionLimit;
+            if (!evalVpl<FMT>(px, fr, cubes, L, allowReject, vplColor, validSamples)) {
                 ++resamples;
                 --i;
                 continue;
@@ -529,9 +590,17 @@ __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs
 }
 
 template <int FMT>
+hipError_t prepare(const CubeArgs& cubes, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
+                   PackedVpl* prep, hipStream_t s) {
+    hipLaunchKernelGGL(prepVplsKernel<FMT>, dim3((SGI_MAX_TOTAL_VPLS_PER_FRAME + 255) / 256), dim3(256), 0, s, vpls,
+                       entries, visible, cubes, prep);
+    return hipGetLastError();
+}
+
+template <int FMT>
 hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int mode, int samples,
-                    int accumulate64, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
-                    float* color, float* reservoir, hipStream_t s) {
+                    int rejectionLimit, int accumulate64, const sgi_vpl* vpls, const int32_t* visible,
+                    const sgi_atlas_entry* entries, PackedVpl* prep, float* color, float* reservoir, hipStream_t s) {
     const int rx = (g.W + 15) / 16, ry = (g.H + 15) / 16;
     if (mode == SGI_GATHER_ALL) {
         if (accumulate64)
@@ -541,8 +610,10 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
             hipLaunchKernelGGL((gatherAllKernel<FMT, false>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
                                rx * ry, vpls, visible, entries, color, reservoir);
     } else {
+        const hipError_t e = prepare<FMT>(cubes, vpls, visible, entries, prep, s);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(gatherSamplesKernel<FMT>, dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry, samples,
-                           vpls, visible, entries, color, reservoir);
+                           rejectionLimit, prep, visible, color, reservoir);
     }
     return hipGetLastError();
 }
@@ -550,11 +621,13 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
 template <int FMT, bool HALF>
 hipError_t launchFmt(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int tileSize, int tilesX,
                      int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
-                     const int32_t* tiles, float* color, float* reservoir, hipStream_t s) {
+                     const int32_t* tiles, PackedVpl* prep, float* color, float* reservoir, hipStream_t s) {
     if (!tiles) {
         const int rx = (g.W + 15) / 16, ry = (g.H + 15) / 16;
+        const hipError_t e = prepare<FMT>(cubes, vpls, visible, entries, prep, s);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL((gatherRandomKernel<FMT, HALF>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry,
-                           vpls, visible, entries, color, reservoir);
+                           prep, visible, color, reservoir);
         return hipGetLastError();
     }
     const int nTiles = tilesX * tilesY;
@@ -580,38 +653,42 @@ hipError_t launchFmt(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cub
 template <int FMT>
 hipError_t launchPrecision(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int tileSize, int tilesX,
                            int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible,
-                           const sgi_atlas_entry* entries, const int32_t* tiles, int precision, float* color,
-                           float* reservoir, hipStream_t s) {
+                           const sgi_atlas_entry* entries, const int32_t* tiles, int precision, PackedVpl* prep,
+                           float* color, float* reservoir, hipStream_t s) {
     if (precision == SGI_PRECISION_FP16)
-        return launchFmt<FMT, true>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
-                                    reservoir, s);
-    return launchFmt<FMT, false>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, color,
-                                 reservoir, s);
+        return launchFmt<FMT, true>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, prep,
+                                    color, reservoir, s);
+    return launchFmt<FMT, false>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles, prep,
+                                 color, reservoir, s);
 }
 
 }  // namespace
 
 namespace sgi_launch {
 
+size_t packedVplBytes() { return sizeof(PackedVpl); }
+
 hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int tileSize, int tilesX,
                   int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
-                  const int32_t* tiles, int precision, float* color, float* reservoir, hipStream_t s) {
+                  const int32_t* tiles, int precision, void* prep, float* color, float* reservoir, hipStream_t s) {
     if (precision != SGI_PRECISION_FP32 && precision != SGI_PRECISION_FP16) return hipErrorNotSupported;
+    PackedVpl* pv = static_cast<PackedVpl*>(prep);
     if (format == SGI_TEXEL_F16)
         return launchPrecision<SGI_TEXEL_F16>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries,
-                                              tiles, precision, color, reservoir, s);
+                                              tiles, precision, pv, color, reservoir, s);
     return launchPrecision<SGI_TEXEL_F32>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles,
-                                          precision, color, reservoir, s);
+                                          precision, pv, color, reservoir, s);
 }
 
 hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int mode, int samples,
-                    int accumulate64, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
-                    float* color, float* reservoir, hipStream_t s) {
+                    int rejectionLimit, int accumulate64, const sgi_vpl* vpls, const int32_t* visible,
+                    const sgi_atlas_entry* entries, void* prep, float* color, float* reservoir, hipStream_t s) {
+    PackedVpl* pv = static_cast<PackedVpl*>(prep);
     if (format == SGI_TEXEL_F16)
-        return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, accumulate64, vpls, visible, entries, color,
-                                       reservoir, s);
-    return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, accumulate64, vpls, visible, entries, color,
-                                   reservoir, s);
+        return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, vpls, visible,
+                                       entries, pv, color, reservoir, s);
+    return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, vpls, visible, entries,
+                                   pv, color, reservoir, s);
 }
 
 }  // namespace sgi_launch

@@ -228,12 +228,28 @@ __device__ __forceinline__ uint4 stageChunk(const ConvSrc& src, int H, int W, in
     }
 }
 
+// The halo chunks one thread stages (every load issued before any is used).
+constexpr int kStageSlots = (kHS * kHS * 4 + 255) / 256;
+
+template <int IN>
+__device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v) {
+    const int x0 = (tile % tilesX) * kTS - 1, y0 = (tile / tilesX) * kTS - 1;
+#pragma unroll
+    for (int k = 0; k < kStageSlots; ++k) {
+        const int i = (int)threadIdx.x + k * 256;
+        const int px = i >> 2, c = i & 3;
+        const int gy = y0 + px / kHS, gx = x0 + px % kHS;
+        v[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) v[k] = stageChunk<IN>(src, H, W, gx, gy, c);
+    }
+}
+
 constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (26 used)
 constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
                                                                              : kTS * kTS * kLogitStride / 4 + 1;
 
 template <int IN, int OUT>
-__global__ __launch_bounds__(256) void conv3x3Kernel(ConvSrc src, const uint16_t* __restrict__ w,
+__global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint16_t* __restrict__ w,
                                                      const float* __restrict__ bias, int relu, int H, int W,
                                                      int tilesX, int nTiles, void* __restrict__ out,
                                                      float* __restrict__ err) {
@@ -256,6 +272,9 @@ __global__ __launch_bounds__(256) void conv3x3Kernel(ConvSrc src, const uint16_t
 
     int t0, t1;
     blockTiles(nTiles, t0, t1);
+    uint4 pre[kStageSlots];
+    if constexpr (IN != IN_FEATURES)
+        if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre);
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
         const int x0 = tx * kTS - 1, y0 = ty * kTS - 1;
@@ -281,16 +300,17 @@ __global__ __launch_bounds__(256) void conv3x3Kernel(ConvSrc src, const uint16_t
                 for (int c = 0; c < 4; ++c)
                     halo[px * kPxU4 + c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
             }
+            __syncthreads();
         } else {
-            for (int i = threadIdx.x; i < kHS * kHS * 4; i += 256) {
-                const int px = i >> 2, c = i & 3;
-                const int gy = y0 + px / kHS, gx = x0 + px % kHS;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (gx >= 0 && gx < W && gy >= 0 && gy < H) v = stageChunk<IN>(src, H, W, gx, gy, c);
-                halo[px * kPxU4 + c] = v;
+            // this tile's chunks were loaded into `pre` before the previous tile's MFMAs
+#pragma unroll
+            for (int k = 0; k < kStageSlots; ++k) {
+                const int i = (int)threadIdx.x + k * 256;
+                if (i < kHS * kHS * 4) halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
             }
+            __syncthreads();
+            if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre);
         }
-        __syncthreads();
         f32x4 acc[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)

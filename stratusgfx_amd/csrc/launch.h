@@ -18,15 +18,20 @@ hipError_t cullTiles(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, i
                      int K, const sgi_vpl* vpls, const int32_t* visible, int remap, sgi_tile_stage1* stage1,
                      int32_t* tiles, hipStream_t s);
 
+// prep: device scratch of SGI_MAX_TOTAL_VPLS_PER_FRAME * packedVplBytes() (the
+// sampled estimators' per-frame VPL records)
+size_t packedVplBytes();
+
 hipError_t gather(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const sgi_dev::CubeArgs& cubes,
                   int format, int tileSize, int tilesX, int tilesY, int K, const sgi_vpl* vpls,
                   const int32_t* visible, const sgi_atlas_entry* entries, const int32_t* tiles, int precision,
-                  float* color, float* reservoir, hipStream_t s);
+                  void* prep, float* color, float* reservoir, hipStream_t s);
 
 // modes / samples / fp64 as sgi_gather_opts
 hipError_t gatherEx(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const sgi_dev::CubeArgs& cubes,
-                    int format, int mode, int samples, int accumulate64, const sgi_vpl* vpls, const int32_t* visible,
-                    const sgi_atlas_entry* entries, float* color, float* reservoir, hipStream_t s);
+                    int format, int mode, int samples, int rejectionLimit, int accumulate64, const sgi_vpl* vpls,
+                    const int32_t* visible, const sgi_atlas_entry* entries, void* prep, float* color,
+                    float* reservoir, hipStream_t s);
 
 hipError_t directLight(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const sgi_dev::CsmArgs& csm,
                        float* lit, hipStream_t s);

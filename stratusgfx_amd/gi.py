@@ -76,8 +76,8 @@ class GiPass:
 
     def gather_ex(self, gbuf: _abi.Gbuffer, vpls, visible, entries, cubes: _abi.CubeAtlases, tiles, color, reservoir,
                   mode=_abi.GATHER_ALL, samples=8, accumulate_fp64=False, precision=_abi.PRECISION_FP32,
-                  stream=None) -> None:
-        opts = _abi.GatherOpts(mode, samples, precision, 1 if accumulate_fp64 else 0)
+                  rejection_limit=5, stream=None) -> None:
+        opts = _abi.GatherOpts(mode, samples, precision, 1 if accumulate_fp64 else 0, rejection_limit)
         _abi.call(self.lib, "sgi_gather_ex", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
                   ctypes.byref(cubes), _ptr(tiles), ctypes.byref(opts), _ptr(color), _ptr(reservoir),
                   _stream(stream))

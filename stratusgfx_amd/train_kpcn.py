@@ -3,9 +3,10 @@
     python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --size 512 --steps 4000
 
 Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
-the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES) of
-4096 VPLs, target = the converged gather over every visible VPL with FP64
-sums (sgi_gather_ex ALL), both normalised as the denoiser's pass 0 does.
+the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES, every
+failing draw rejected, so that more samples converge to the target) of 4096
+VPLs, target = the converged gather over every visible VPL with FP64 sums
+(sgi_gather_ex ALL), both normalised as the denoiser's pass 0 does.
 Training: Adam on the relative-MSE loss plus the error head's regression
 (sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
 held-out seeds reports the relative MSE of the noisy input, the a-trous
@@ -67,7 +68,8 @@ class Sample:
         self.feat, self.rad, self.target, self.extra = feat, rad, target, extra
 
 
-def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: int, rng, keep_frame=False):
+def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: int, rng, keep_frame=False,
+                 rejection_limit=-1):
     import torch
 
     w = sc.Workload(f"city-train-{seed}", _abi.SCENE_CITY, seed, size, size, num_vpls, 64, _abi.TEXEL_F16, 1024, 16,
@@ -90,7 +92,8 @@ def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: in
         gi.visible_atlas_entries(inp.entries, vis, n, ent)
         g = inp.gbuffer()
         col, res = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
-        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=samples_pp)
+        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=samples_pp,
+                     rejection_limit=rejection_limit)
         colT, resT = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
         gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, colT, resT, mode=_abi.GATHER_ALL, accumulate_fp64=True)
         feat = torch.empty((size, size, 32), dtype=torch.bfloat16, device="cuda")

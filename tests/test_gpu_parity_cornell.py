@@ -161,10 +161,12 @@ def test_render_frame_chain(case, ora, gi, use_tiles):
     assert e <= REL_L2 and e_gi <= REL_L2
 
 
-@pytest.mark.parametrize("mode,acc64", [("all", False), ("all", True), ("samples", False)])
-def test_gather_ex_modes(case, ora, gi, mode, acc64):
+@pytest.mark.parametrize("mode,acc64,rej", [("all", False, 5), ("all", True, 5), ("samples", False, 5),
+                                            ("samples", False, -1)])
+def test_gather_ex_modes(case, ora, gi, mode, acc64, rej):
     """sgi_gather_ex: every visible VPL (FP32 or FP64 sums of the FP32 per-VPL terms, both against the
-    FP32 restatement: the decisions are the same, the sums differ by rounding) and 8 hashed draws per pixel."""
+    FP32 restatement: the decisions are the same, the sums differ by rounding) and 8 hashed draws per pixel
+    with the reference's 5-rejection rule and with every failing draw rejected."""
     from stratusgfx_amd import _abi
     torch = require_gpu()
     upd, vis, ent = _device_visible(case, ora)
@@ -173,13 +175,13 @@ def test_gather_ex_modes(case, ora, gi, mode, acc64):
     res = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     m = _abi.GATHER_ALL if mode == "all" else _abi.GATHER_SAMPLES
     gi.gather_ex(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), None, color, res, mode=m, samples=8,
-                 accumulate_fp64=acc64)
+                 accumulate_fp64=acc64, rejection_limit=rej)
     torch.cuda.synchronize()
     h = case.host
     omode = orc.GATHER_CONVERGED if mode == "all" else orc.GATHER_SAMPLES
     oc, orr = orc.gather(h.frame, case.hg, ora["upd"], int(ora["vis"][0]), ora["ent"], case.hcubes, mode=omode,
-                         precision=0, samples=8)
+                         precision=0, samples=8, rejection_limit=rej)
     e = orc.rel_l2(res.cpu().numpy(), orr)
-    print(f"gather_ex[{mode}, fp64={acc64}] rel-L2 {e:.3e}")
+    print(f"gather_ex[{mode}, fp64={acc64}, rejection limit {rej}] rel-L2 {e:.3e}")
     assert e <= REL_L2
     assert orc.rel_l2(color.cpu().numpy(), oc) <= REL_L2

@@ -315,3 +315,33 @@ def test_fp64_twin_agrees_with_fp32(cornell):
     assert np.quantile(d, 0.99) < 1e-4
     assert (d > 1e-3).mean() < 1e-3
     assert r64[..., :3].sum() > 0
+
+
+def test_sampled_gather_converges_to_every_vpl_render(cornell):
+    """With every failing draw rejected, the hashed sampler is a ratio estimator over the accepted VPLs:
+    its normalised output approaches the every-VPL render as the sample count grows, while the
+    reference's 5-rejection rule converges elsewhere (accepted failing draws add weight, no light)."""
+    w, inp = cornell
+    csm = inp.csm_struct()
+    upd, vis = orc.cull_global(inp.frame, csm, inp.vpls)
+    ent = inp.entries[vis[1:]]
+    g, cubes = inp.gbuffer(), inp.cubes()
+    y0, y1 = 60, 72
+
+    def norm(res):
+        return res[..., :3] / np.maximum(res[..., 3:4], 1e-5)
+
+    _, ref = orc.gather(inp.frame, g, upd, int(vis[0]), ent, cubes, mode=orc.GATHER_CONVERGED, y0=y0, y1=y1)
+    target = norm(ref).astype(np.float64)
+
+    def err(samples, rej):
+        _, r = orc.gather(inp.frame, g, upd, int(vis[0]), ent, cubes, mode=orc.GATHER_SAMPLES, samples=samples,
+                          rejection_limit=rej, y0=y0, y1=y1)
+        x = norm(r).astype(np.float64)
+        return float(((x - target) ** 2 / (target ** 2 + 1e-2)).mean())
+
+    e4, e64, e1024 = err(4, -1), err(64, -1), err(1024, -1)
+    assert e1024 < e64 < e4
+    assert e1024 < 0.05 * e4
+    # the reference's rule: capped rejections leave a bias that more samples do not remove
+    assert err(1024, 5) > 5 * e1024
