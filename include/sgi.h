@@ -305,6 +305,64 @@ typedef struct sgi_frame_io {
  * direct_light -> (cull_tiles) -> gather -> denoise. */
 int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream);
 
+/* ---- batched views ---------------------------------------------------------
+ *
+ * Many viewpoints of one scene (shared VPLs and cubes), each with its own
+ * viewport size, G-buffer, uniforms and cascades, rendered as sgi_render_frame
+ * with tile lists would render each view, but with one launch per stage over
+ * the work of every view (the stages' blocks are numbered over all views'
+ * tiles / pixel blocks), and optionally replayed from a captured graph.  The
+ * reference renders one view per frame (RenderScene); this has no counterpart
+ * there and serves callers that render many views per frame (probes,
+ * reflections, multi-camera capture). */
+typedef struct sgi_views sgi_views;
+
+typedef struct sgi_views_config {
+    int32_t max_views;          /* 1..64 */
+    int64_t max_pixels;         /* total pixels of a batch (workspace size) */
+    int32_t tile_size;          /* 8, 16 or 32 */
+    int32_t max_vpls_per_tile;  /* 1..64 */
+} sgi_views_config;
+
+typedef struct sgi_view_io {
+    sgi_gbuffer gbuffer;        /* the view's device planes and viewport size */
+    sgi_frame frame;            /* the view's uniforms (as sgi_set_frame) */
+    sgi_csm csm;                /* the view's cascades (as sgi_set_csm) */
+    int32_t* tiles;             /* device [tiles][1 + K] at the batch's tile size */
+    float* direct;              /* device [H][W][3] */
+    float* color;               /* device [H][W][3] */
+    float* reservoir;           /* device [H][W][4], 16-byte aligned */
+    const float* prevGi;        /* history in (NULL = none) */
+    const float* prevHistory;
+    float* combined;            /* device [H][W][3] */
+    float* giOut;               /* device [H][W][3] */
+    float* reservoirOut;        /* device [H][W][4], 16-byte aligned */
+    float* historyOut;          /* device [H][W] */
+} sgi_view_io;
+
+int sgi_views_create(const sgi_views_config* config, sgi_views** out);
+void sgi_views_destroy(sgi_views* views);
+
+/* Describes the batch: numViews views, the shared VPLs (input order, as
+ * sgi_cull_global), their atlas entries by input index and the cube atlases.
+ * The per-view descriptors are copied to the device on `stream`. */
+int sgi_views_set(sgi_views* views, int32_t numViews, const sgi_view_io* io, const sgi_vpl* vpls,
+                  int32_t numVpls, const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes,
+                  void* stream);
+
+/* One frame of every view: VPL culling, atlas entries, direct light, tile
+ * culling, tile-list gather (FP32) and the a-trous / temporal denoiser, each
+ * stage one launch over all views. */
+int sgi_views_run(sgi_views* views, void* stream);
+
+/* Captures sgi_views_run's launches on `stream` (not the NULL stream) into a
+ * graph owned by the context; sgi_views_replay launches it.  The graph keeps
+ * the batch's shape: after sgi_views_set with other sizes, capture again
+ * (sgi_views_replay fails until then); new buffer addresses or frame data of
+ * the same shape are picked up from the device descriptors. */
+int sgi_views_capture(sgi_views* views, void* stream);
+int sgi_views_replay(sgi_views* views, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

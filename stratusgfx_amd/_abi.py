@@ -97,6 +97,17 @@ class FrameIO(ctypes.Structure):
     ]
 
 
+class ViewsConfig(ctypes.Structure):
+    _fields_ = [("max_views", c_int32), ("max_pixels", ctypes.c_int64), ("tile_size", c_int32),
+                ("max_vpls_per_tile", c_int32)]
+
+
+class ViewIO(ctypes.Structure):
+    _fields_ = [("gbuffer", Gbuffer), ("frame", Frame), ("csm", Csm), ("tiles", c_void_p), ("direct", c_void_p),
+                ("color", c_void_p), ("reservoir", c_void_p), ("prevGi", c_void_p), ("prevHistory", c_void_p),
+                ("combined", c_void_p), ("giOut", c_void_p), ("reservoirOut", c_void_p), ("historyOut", c_void_p)]
+
+
 class GatherOpts(ctypes.Structure):
     _fields_ = [("mode", c_int32), ("samples", c_int32), ("precision", c_int32), ("accumulate_fp64", c_int32),
                 ("rejection_limit", c_int32)]
@@ -142,6 +153,12 @@ SGI_SIGNATURES = {
     "sgi_denoise": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               _P]),
     "sgi_render_frame": (c_int32, [_P, POINTER(FrameIO), _P]),
+    "sgi_views_create": (c_int32, [POINTER(ViewsConfig), POINTER(_P)]),
+    "sgi_views_destroy": (None, [_P]),
+    "sgi_views_set": (c_int32, [_P, _I, POINTER(ViewIO), _P, _I, _P, POINTER(CubeAtlases), _P]),
+    "sgi_views_run": (c_int32, [_P, _P]),
+    "sgi_views_capture": (c_int32, [_P, _P]),
+    "sgi_views_replay": (c_int32, [_P, _P]),
     # include/sgi_kpcn.h
     "sgi_kpcn_conv3x3": (c_int32, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
     "sgi_kpcn_wgrad_partials": (ctypes.c_size_t, [_I, _I]),

@@ -14,6 +14,7 @@
 // are inserted one at a time in slot order with a ballot for the position and
 // a lane shift, which reproduces the reference's strict-< insertion sort.
 #include "launch.h"
+#include "views.h"
 
 namespace {
 
@@ -24,9 +25,8 @@ constexpr int kCullThreads = 1024;
 // Per-VPL sun visibility, one thread per VPL over many blocks (each VPL's 36
 // cascade compares read scattered texels; spreading them over CUs keeps the
 // loads in flight on many L1s).
-__global__ __launch_bounds__(64) void cullVisibilityKernel(CsmArgs csm, f3 sun, const sgi_vpl* __restrict__ in, int n,
-                                                          uint8_t* __restrict__ flags) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void cullVisibilityBody(const CsmArgs& csm, f3 sun, const sgi_vpl* __restrict__ in, int n,
+                                                   uint8_t* __restrict__ flags, int i) {
     if (i >= n) return;
     const f3 p = ld3(in[i].position);
     const f3 blends = cascadeBlends(csm, p);
@@ -34,12 +34,22 @@ __global__ __launch_bounds__(64) void cullVisibilityKernel(CsmArgs csm, f3 sun, 
     flags[i] = shadowFactor < 1.0f ? 1 : 0;
 }
 
+__global__ __launch_bounds__(64) void cullVisibilityKernel(CsmArgs csm, f3 sun, const sgi_vpl* __restrict__ in, int n,
+                                                          uint8_t* __restrict__ flags) {
+    cullVisibilityBody(csm, sun, in, n, flags, blockIdx.x * 64 + threadIdx.x);
+}
+
+// Batched: blocks [v * bpv, (v + 1) * bpv) test the VPLs against view v's cascades.
+__global__ __launch_bounds__(64) void cullVisibilityViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                               const sgi_vpl* __restrict__ in, int n, int bpv) {
+    const sgi_batch::ViewDesc& d = views[blockIdx.x / bpv];
+    cullVisibilityBody(d.csm, ld3(d.fr.sunDir), in, n, d.visFlags, (blockIdx.x % bpv) * 64 + threadIdx.x);
+}
+
 // Stable compaction of the flags: wave ballots and a block prefix, in input order.
-__global__ __launch_bounds__(kCullThreads) void cullCompactKernel(const sgi_vpl* __restrict__ in, int n,
-                                                                  const uint8_t* __restrict__ flags,
-                                                                  sgi_vpl* __restrict__ out,
-                                                                  int32_t* __restrict__ visible) {
-    __shared__ int waveCount[kCullThreads / 64];
+__device__ __forceinline__ void cullCompactBody(const sgi_vpl* __restrict__ in, int n, const uint8_t* __restrict__ flags,
+                                                sgi_vpl* __restrict__ out, int32_t* __restrict__ visible,
+                                                int* waveCount) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int running = 0;
     for (int c0 = 0; c0 < n; c0 += kCullThreads) {
@@ -67,6 +77,22 @@ __global__ __launch_bounds__(kCullThreads) void cullCompactKernel(const sgi_vpl*
     if (threadIdx.x == 0) visible[0] = running < SGI_MAX_TOTAL_VPLS_PER_FRAME ? running : SGI_MAX_TOTAL_VPLS_PER_FRAME;
 }
 
+__global__ __launch_bounds__(kCullThreads) void cullCompactKernel(const sgi_vpl* __restrict__ in, int n,
+                                                                  const uint8_t* __restrict__ flags,
+                                                                  sgi_vpl* __restrict__ out,
+                                                                  int32_t* __restrict__ visible) {
+    __shared__ int waveCount[kCullThreads / 64];
+    cullCompactBody(in, n, flags, out, visible, waveCount);
+}
+
+// Batched: one block per view.
+__global__ __launch_bounds__(kCullThreads) void cullCompactViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                                       const sgi_vpl* __restrict__ in, int n) {
+    __shared__ int waveCount[kCullThreads / 64];
+    const sgi_batch::ViewDesc& d = views[blockIdx.x];
+    cullCompactBody(in, n, d.visFlags, d.vplUpdated, d.visible, waveCount);
+}
+
 __global__ void gatherEntriesKernel(const sgi_atlas_entry* __restrict__ all, const int32_t* __restrict__ visible,
                                     int maxCount, sgi_atlas_entry* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,17 +100,22 @@ __global__ void gatherEntriesKernel(const sgi_atlas_entry* __restrict__ all, con
     if (s < count && s < maxCount) out[s] = all[visible[1 + s]];
 }
 
+// Batched: blocks [v * bpv, (v + 1) * bpv) gather view v's visible slots.
+__global__ void gatherEntriesViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                         const sgi_atlas_entry* __restrict__ all, int maxCount, int bpv) {
+    const sgi_batch::ViewDesc& d = views[blockIdx.x / bpv];
+    const int s = (blockIdx.x % bpv) * blockDim.x + threadIdx.x;
+    const int count = d.visible[0];
+    if (s < count && s < maxCount) d.entries[s] = all[d.visible[1 + s]];
+}
+
 template <int TS>
-__global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, int tilesX, int nTiles, int K,
-                                                      const sgi_vpl* __restrict__ vpls,
-                                                      const int32_t* __restrict__ visible, int remap,
-                                                      sgi_tile_stage1* __restrict__ stage1,
-                                                      int32_t* __restrict__ tiles) {
+__device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs& fr, int t, int tilesX, int K,
+                                             const sgi_vpl* __restrict__ vpls, const int32_t* __restrict__ visible,
+                                             int remap, sgi_tile_stage1* __restrict__ stage1,
+                                             int32_t* __restrict__ tiles, float (*comp)[TS * TS], float* sums) {
     constexpr int NPX = TS * TS;
-    __shared__ float comp[6][NPX];
-    __shared__ float sums[6];
-    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
-    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+    const int tx = t % tilesX, ty = t / tilesX;
     const int lane = threadIdx.x;
     const int x0 = tx * TS, y0 = ty * TS;
     const int vw = min(TS, g.W - x0), vh = min(TS, g.H - y0);
@@ -170,6 +201,30 @@ __global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, 
     if (lane < count) rec[1 + lane] = myIdx;
 }
 
+template <int TS>
+__global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, int tilesX, int nTiles, int K,
+                                                      const sgi_vpl* __restrict__ vpls,
+                                                      const int32_t* __restrict__ visible, int remap,
+                                                      sgi_tile_stage1* __restrict__ stage1,
+                                                      int32_t* __restrict__ tiles) {
+    __shared__ float comp[6][TS * TS];
+    __shared__ float sums[6];
+    cullTileBody<TS>(g, fr, (int)xcdSwizzle(blockIdx.x, (uint32_t)nTiles), tilesX, K, vpls, visible, remap, stage1,
+                     tiles, comp, sums);
+}
+
+// Batched: the tiles of every view, numbered view after view.
+template <int TS>
+__global__ __launch_bounds__(64) void cullTilesViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                           int numViews, int totalTiles, int K) {
+    __shared__ float comp[6][TS * TS];
+    __shared__ float sums[6];
+    const int gt = (int)xcdSwizzle(blockIdx.x, (uint32_t)totalTiles);
+    const sgi_batch::ViewDesc& d = views[sgi_batch::findView<sgi_batch::kTiles>(views, numViews, gt)];
+    cullTileBody<TS>(d.g, d.fr, gt - d.tileStart, d.tilesX, K, d.vplUpdated, d.visible, 0, nullptr, d.tiles, comp,
+                     sums);
+}
+
 }  // namespace
 
 namespace sgi_launch {
@@ -208,6 +263,45 @@ hipError_t cullTiles(const GbufArgs& g, const FrameArgs& fr, int tileSize, int t
         case 32:
             hipLaunchKernelGGL(cullTilesKernel<32>, dim3(nTiles), dim3(64), 0, s, g, fr, tilesX, nTiles, K, vpls,
                                visible, remap, stage1, tiles);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t cullGlobalViews(const sgi_batch::ViewDesc* views, int numViews, const sgi_vpl* in, int n, hipStream_t s) {
+    if (n > 0) {
+        const int bpv = (n + 63) / 64;
+        hipLaunchKernelGGL(cullVisibilityViewsKernel, dim3(numViews * bpv), dim3(64), 0, s, views, in, n, bpv);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(cullCompactViewsKernel, dim3(numViews), dim3(kCullThreads), 0, s, views, in, n);
+    return hipGetLastError();
+}
+
+hipError_t gatherVisibleEntriesViews(const sgi_batch::ViewDesc* views, int numViews, const sgi_atlas_entry* all,
+                                     int maxCount, hipStream_t s) {
+    if (maxCount <= 0) return hipSuccess;
+    const int bpv = (maxCount + 255) / 256;
+    hipLaunchKernelGGL(gatherEntriesViewsKernel, dim3(numViews * bpv), dim3(256), 0, s, views, all, maxCount, bpv);
+    return hipGetLastError();
+}
+
+hipError_t cullTilesViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles, int tileSize, int K,
+                          hipStream_t s) {
+    switch (tileSize) {
+        case 8:
+            hipLaunchKernelGGL(cullTilesViewsKernel<8>, dim3(totalTiles), dim3(64), 0, s, views, numViews, totalTiles, K);
+            break;
+        case 16:
+            hipLaunchKernelGGL(cullTilesViewsKernel<16>, dim3(totalTiles), dim3(64), 0, s, views, numViews, totalTiles,
+                               K);
+            break;
+        case 32:
+            hipLaunchKernelGGL(cullTilesViewsKernel<32>, dim3(totalTiles), dim3(64), 0, s, views, numViews, totalTiles,
+                               K);
             break;
         default:
             return hipErrorInvalidValue;

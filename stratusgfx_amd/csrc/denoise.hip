@@ -19,6 +19,7 @@
 // hardware exp2 / rcp; the merge test and the temporal similarity test are
 // decisions and keep correctly rounded arithmetic.
 #include "launch.h"
+#include "views.h"
 
 namespace {
 
@@ -26,12 +27,26 @@ using namespace sgi_dev;
 
 __device__ __forceinline__ int clampx(int x, int W) { return x < 0 ? 0 : (x >= W ? W - 1 : x); }
 
-// 64x4-pixel block of a row-band (XCD-contiguous) numbering.
+// 64x4-pixel block b of a row-band numbering.
+__device__ __forceinline__ void blockXY(int W, int b, int& x, int& y) {
+    const int bx = (W + 63) / 64;
+    x = (b % bx) * 64 + (int)threadIdx.x % 64;
+    y = (b / bx) * 4 + (int)threadIdx.x / 64;
+}
+
+// The block of this launch in an XCD-contiguous numbering: each XCD walks a band of rows.
 __device__ __forceinline__ void blockPixel(int W, int H, int& x, int& y) {
     const int bx = (W + 63) / 64, by = (H + 3) / 4;
-    const uint32_t b = xcdSwizzle(blockIdx.x, (uint32_t)(bx * by));
-    x = (int)(b % bx) * 64 + (int)threadIdx.x % 64;
-    y = (int)(b / bx) * 4 + (int)threadIdx.x / 64;
+    blockXY(W, (int)xcdSwizzle(blockIdx.x, (uint32_t)(bx * by)), x, y);
+}
+
+// Batched launches: the view of this block and the block's pixel in it.
+__device__ __forceinline__ const sgi_batch::ViewDesc& viewBlock(const sgi_batch::ViewDesc* views, int numViews,
+                                                               int total, int& x, int& y) {
+    const int gb = (int)xcdSwizzle(blockIdx.x, (uint32_t)total);
+    const sgi_batch::ViewDesc& d = views[sgi_batch::findView<sgi_batch::kDenoiseBlocks>(views, numViews, gb)];
+    blockXY(d.g.W, gb - d.dblockStart, x, y);
+    return d;
 }
 
 // GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv.
@@ -57,12 +72,10 @@ __device__ __forceinline__ bool maskedOut(const uint8_t* mask, int W, int x, int
     return mask && !mask[(y >> 4) * ((W + 15) >> 4) + (x >> 4)];
 }
 
-__global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
-                                                          const float4* __restrict__ resIn, float4* __restrict__ resOut,
-                                                          float4* __restrict__ normals,
-                                                          const uint8_t* __restrict__ mask) {
-    int x, y;
-    blockPixel(g.W, g.H, x, y);
+__device__ __forceinline__ void mergeBody(const GbufArgs& g, const FrameArgs& fr, const int32_t* __restrict__ visible,
+                                          const float4* __restrict__ resIn, float4* __restrict__ resOut,
+                                          float4* __restrict__ normals, const uint8_t* __restrict__ mask, int x,
+                                          int y) {
     if (x >= g.W || y >= g.H) return;
     const size_t p = (size_t)y * g.W + x;
     const f3 centerNormal = decodeNormal(g.normal + 3 * p);
@@ -93,11 +106,25 @@ __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs 
     resOut[p] = make_float4(cr0 * inv, cr1 * inv, cr2 * inv, 1.0f);
 }
 
-__global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
-                                                           const float4* __restrict__ normals,
-                                                           float4* __restrict__ resOut, const uint8_t* __restrict__ mask) {
+__global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
+                                                          const float4* __restrict__ resIn, float4* __restrict__ resOut,
+                                                          float4* __restrict__ normals,
+                                                          const uint8_t* __restrict__ mask) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
+    mergeBody(g, fr, visible, resIn, resOut, normals, mask, x, y);
+}
+
+__global__ __launch_bounds__(256) void denoiseMergeViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                               int numViews, int total) {
+    int x, y;
+    const sgi_batch::ViewDesc& d = viewBlock(views, numViews, total, x, y);
+    mergeBody(d.g, d.fr, d.visible, (const float4*)d.reservoir, d.resA, d.normals, nullptr, x, y);
+}
+
+__device__ __forceinline__ void filterBody(const GbufArgs& g, int stride, const float4* __restrict__ resIn,
+                                           const float4* __restrict__ normals, float4* __restrict__ resOut,
+                                           const uint8_t* __restrict__ mask, int x, int y) {
     if (x >= g.W || y >= g.H) return;
     if (maskedOut(mask, g.W, x, y)) return;
     const int W = g.W, H = g.H;
@@ -134,19 +161,32 @@ __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int strid
     resOut[p] = make_float4(rf0 * inv, rf1 * inv, rf2 * inv, 1.0f);
 }
 
-// The final pass's temporal blend and combine (vpl_pbr_gi_denoise.fs:336-434) on
-// the filtered reservoir the last filter pass wrote.
-__global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArgs prev, FrameArgs fr,
-                                                             const float4* __restrict__ filtered,
-                                                             const float4* __restrict__ normals,
-                                                             const float* __restrict__ screen,
-                                                             const float* __restrict__ color,
-                                                             const float* __restrict__ prevGi,
-                                                             const float* __restrict__ prevHistory,
-                                                             float* __restrict__ combined, float* __restrict__ giOut,
-                                                             float* __restrict__ historyOut) {
+__global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
+                                                           const float4* __restrict__ normals,
+                                                           float4* __restrict__ resOut, const uint8_t* __restrict__ mask) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
+    filterBody(g, stride, resIn, normals, resOut, mask, x, y);
+}
+
+// pass 1: resA -> resB, pass 2: resB -> resA, pass 3: resA -> reservoirOut
+__global__ __launch_bounds__(256) void denoiseFilterViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                                int numViews, int total, int pass) {
+    int x, y;
+    const sgi_batch::ViewDesc& d = viewBlock(views, numViews, total, x, y);
+    const float4* in = pass == 2 ? d.resB : d.resA;
+    float4* out = pass == 1 ? d.resB : (pass == 2 ? d.resA : (float4*)d.reservoirOut);
+    filterBody(d.g, 1 << pass, in, d.normals, out, nullptr, x, y);
+}
+
+// The final pass's temporal blend and combine (vpl_pbr_gi_denoise.fs:336-434) on
+// the filtered reservoir the last filter pass wrote.
+__device__ __forceinline__ void temporalBody(const GbufArgs& g, const GbufArgs& prev, const FrameArgs& fr,
+                                             const float4* __restrict__ filtered, const float4* __restrict__ normals,
+                                             const float* __restrict__ screen, const float* __restrict__ color,
+                                             const float* __restrict__ prevGi, const float* __restrict__ prevHistory,
+                                             float* __restrict__ combined, float* __restrict__ giOut,
+                                             float* __restrict__ historyOut, int x, int y) {
     if (x >= g.W || y >= g.H) return;
     const int W = g.W, H = g.H;
     const size_t p = (size_t)y * W + x;
@@ -191,6 +231,29 @@ __global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArg
     historyOut[p] = historyAccum;
 }
 
+__global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArgs prev, FrameArgs fr,
+                                                             const float4* __restrict__ filtered,
+                                                             const float4* __restrict__ normals,
+                                                             const float* __restrict__ screen,
+                                                             const float* __restrict__ color,
+                                                             const float* __restrict__ prevGi,
+                                                             const float* __restrict__ prevHistory,
+                                                             float* __restrict__ combined, float* __restrict__ giOut,
+                                                             float* __restrict__ historyOut) {
+    int x, y;
+    blockPixel(g.W, g.H, x, y);
+    temporalBody(g, prev, fr, filtered, normals, screen, color, prevGi, prevHistory, combined, giOut, historyOut, x, y);
+}
+
+// Batched views: the previous frame's G-buffer is the current one (a single frame per view).
+__global__ __launch_bounds__(256) void denoiseTemporalViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                                  int numViews, int total) {
+    int x, y;
+    const sgi_batch::ViewDesc& d = viewBlock(views, numViews, total, x, y);
+    temporalBody(d.g, d.g, d.fr, (const float4*)d.reservoirOut, d.normals, d.direct, d.color, d.prevGi,
+                 d.prevHistory, d.combined, d.giOut, d.historyOut, x, y);
+}
+
 }  // namespace
 
 namespace sgi_launch {
@@ -228,6 +291,19 @@ hipError_t denoise(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& f
     if (e != hipSuccess) return e;
     return denoiseTemporal(cur, prev, fr, screen, color, reservoirOut, prevGi, prevHistory, combined, giOut,
                            historyOut, ws, s);
+}
+
+hipError_t denoiseViews(const sgi_batch::ViewDesc* views, int numViews, int totalBlocks, hipStream_t s) {
+    hipLaunchKernelGGL(denoiseMergeViewsKernel, dim3(totalBlocks), dim3(256), 0, s, views, numViews, totalBlocks);
+    hipError_t e = hipGetLastError();
+    for (int pass = 1; pass <= 3 && e == hipSuccess; ++pass) {
+        hipLaunchKernelGGL(denoiseFilterViewsKernel, dim3(totalBlocks), dim3(256), 0, s, views, numViews, totalBlocks,
+                           pass);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(denoiseTemporalViewsKernel, dim3(totalBlocks), dim3(256), 0, s, views, numViews, totalBlocks);
+    return hipGetLastError();
 }
 
 }  // namespace sgi_launch

@@ -4,15 +4,15 @@
 // (pbr2.glsl:194-250) through calculateLighting_Burley (pbr2.glsl:454-478).
 // One thread per pixel; blocks cover 16x16 tiles, XCD-contiguous.
 #include "launch.h"
+#include "views.h"
 
 namespace {
 
 using namespace sgi_dev;
 
-__global__ __launch_bounds__(256) void directKernel(GbufArgs g, FrameArgs fr, CsmArgs csm, int tilesX, int nTiles,
-                                                    float* __restrict__ lit) {
-    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
-    const int x = ((int)t % tilesX) * 16 + (int)threadIdx.x % 16, y = ((int)t / tilesX) * 16 + (int)threadIdx.x / 16;
+__device__ __forceinline__ void directBody(const GbufArgs& g, const FrameArgs& fr, const CsmArgs& csm, int t,
+                                           int tilesX, float* __restrict__ lit) {
+    const int x = (t % tilesX) * 16 + (int)threadIdx.x % 16, y = (t / tilesX) * 16 + (int)threadIdx.x / 16;
     if (x >= g.W || y >= g.H) return;
     const size_t p = (size_t)y * g.W + x;
     const f3 fragPos = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV);
@@ -64,6 +64,19 @@ __global__ __launch_bounds__(256) void directKernel(GbufArgs g, FrameArgs fr, Cs
     lit[3 * p + 2] = gmin(outc.z, 65504.0f);
 }
 
+__global__ __launch_bounds__(256) void directKernel(GbufArgs g, FrameArgs fr, CsmArgs csm, int tilesX, int nTiles,
+                                                    float* __restrict__ lit) {
+    directBody(g, fr, csm, (int)xcdSwizzle(blockIdx.x, (uint32_t)nTiles), tilesX, lit);
+}
+
+// Batched: the 16x16 tiles of every view, numbered view after view.
+__global__ __launch_bounds__(256) void directViewsKernel(const sgi_batch::ViewDesc* __restrict__ views, int numViews,
+                                                         int total) {
+    const int gt = (int)xcdSwizzle(blockIdx.x, (uint32_t)total);
+    const sgi_batch::ViewDesc& d = views[sgi_batch::findView<sgi_batch::kDirectTiles>(views, numViews, gt)];
+    directBody(d.g, d.fr, d.csm, gt - d.dtileStart, (d.g.W + 15) / 16, d.direct);
+}
+
 }  // namespace
 
 namespace sgi_launch {
@@ -71,6 +84,11 @@ namespace sgi_launch {
 hipError_t directLight(const GbufArgs& g, const FrameArgs& fr, const CsmArgs& csm, float* lit, hipStream_t s) {
     const int tx = (g.W + 15) / 16, ty = (g.H + 15) / 16;
     hipLaunchKernelGGL(directKernel, dim3(tx * ty), dim3(256), 0, s, g, fr, csm, tx, tx * ty, lit);
+    return hipGetLastError();
+}
+
+hipError_t directLightViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles16, hipStream_t s) {
+    hipLaunchKernelGGL(directViewsKernel, dim3(totalTiles16), dim3(256), 0, s, views, numViews, totalTiles16);
     return hipGetLastError();
 }
 

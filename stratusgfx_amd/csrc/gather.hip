@@ -15,6 +15,7 @@
 // operation order, so every branch matches the oracle bit for bit; the
 // continuous BRDF terms use hardware rcp / rsq / exp2 (~1 ulp each).
 #include "launch.h"
+#include "views.h"
 
 namespace {
 
@@ -378,15 +379,12 @@ __device__ __forceinline__ void writeOut(const GbufArgs& g, size_t p, f3 vplColo
 constexpr int kMaxK = SGI_MAX_VPLS_PER_TILE_LIMIT;
 
 template <int FMT, int TS, bool HALF>
-__global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
-                                                            int nTiles, int K, const sgi_vpl* __restrict__ vpls,
-                                                            const sgi_atlas_entry* __restrict__ entries,
-                                                            const int32_t* __restrict__ tiles,
-                                                            float* __restrict__ color, float* __restrict__ reservoir) {
-    __shared__ Vpl lv[kMaxK];
-    __shared__ int ln;
-    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
-    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+__device__ __forceinline__ void gatherTileBody(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int t,
+                                               int tilesX, int K, const sgi_vpl* __restrict__ vpls,
+                                               const sgi_atlas_entry* __restrict__ entries,
+                                               const int32_t* __restrict__ tiles, float* __restrict__ color,
+                                               float* __restrict__ reservoir, Vpl* lv, int& ln) {
+    const int tx = t % tilesX, ty = t / tilesX;
     const int32_t* rec = tiles + (size_t)t * (1 + K);
     if (threadIdx.x == 0) ln = min(max(rec[0], 0), K);
     if ((int)threadIdx.x < K) {
@@ -419,6 +417,30 @@ __global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameAr
         for (int k = 0; k < n; ++k) evalVpl<FMT>(px, fr, cubes, lv[k], true, vplColor, validSamples);
     }
     writeOut(g, p, vplColor, validSamples, color, reservoir);
+}
+
+template <int FMT, int TS, bool HALF>
+__global__ __launch_bounds__(TS * TS) void gatherTilesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
+                                                            int nTiles, int K, const sgi_vpl* __restrict__ vpls,
+                                                            const sgi_atlas_entry* __restrict__ entries,
+                                                            const int32_t* __restrict__ tiles,
+                                                            float* __restrict__ color, float* __restrict__ reservoir) {
+    __shared__ Vpl lv[kMaxK];
+    __shared__ int ln;
+    gatherTileBody<FMT, TS, HALF>(g, fr, cubes, (int)xcdSwizzle(blockIdx.x, (uint32_t)nTiles), tilesX, K, vpls,
+                                  entries, tiles, color, reservoir, lv, ln);
+}
+
+// Batched: the tiles of every view, numbered view after view (FP32 shading).
+template <int FMT, int TS>
+__global__ __launch_bounds__(TS * TS) void gatherTilesViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
+                                                                 int numViews, int totalTiles, int K, CubeArgs cubes) {
+    __shared__ Vpl lv[kMaxK];
+    __shared__ int ln;
+    const int gt = (int)xcdSwizzle(blockIdx.x, (uint32_t)totalTiles);
+    const sgi_batch::ViewDesc& d = views[sgi_batch::findView<sgi_batch::kTiles>(views, numViews, gt)];
+    gatherTileBody<FMT, TS, false>(d.g, d.fr, cubes, gt - d.tileStart, d.tilesX, K, d.vplUpdated, d.entries, d.tiles,
+                                   d.color, d.reservoir, lv, ln);
 }
 
 // vpl_pbr_gi.fs:158-161: slot int(maxRandomIndex * random(fragCoord, seedZ)),
@@ -689,6 +711,27 @@ hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
                                        entries, pv, color, reservoir, s);
     return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, vpls, visible, entries,
                                    pv, color, reservoir, s);
+}
+
+hipError_t gatherViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles, int tileSize, int K,
+                       const CubeArgs& cubes, int format, hipStream_t s) {
+    const int threads = tileSize * tileSize;
+#define SGI_GATHER_VIEWS(F, T)                                                                                      \
+    hipLaunchKernelGGL((gatherTilesViewsKernel<F, T>), dim3(totalTiles), dim3(threads), 0, s, views, numViews,     \
+                       totalTiles, K, cubes)
+    if (format == SGI_TEXEL_F16) {
+        if (tileSize == 8) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 8);
+        else if (tileSize == 16) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 16);
+        else if (tileSize == 32) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 32);
+        else return hipErrorInvalidValue;
+    } else {
+        if (tileSize == 8) SGI_GATHER_VIEWS(SGI_TEXEL_F32, 8);
+        else if (tileSize == 16) SGI_GATHER_VIEWS(SGI_TEXEL_F32, 16);
+        else if (tileSize == 32) SGI_GATHER_VIEWS(SGI_TEXEL_F32, 32);
+        else return hipErrorInvalidValue;
+    }
+#undef SGI_GATHER_VIEWS
+    return hipGetLastError();
 }
 
 }  // namespace sgi_launch

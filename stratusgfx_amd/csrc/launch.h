@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gi_device.h"
+#include "views.h"
 
 namespace sgi_launch {
 
@@ -46,6 +47,17 @@ hipError_t denoise(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, 
                    const int32_t* visible, const float* screen, const float* color, const float* reservoir,
                    const float* prevGi, const float* prevHistory, float* combined, float* giOut,
                    float* reservoirOut, float* historyOut, const DenoiseBuffers& ws, hipStream_t s);
+
+// Batched views (views: device array of ViewDesc, numbered in item order).
+hipError_t cullGlobalViews(const sgi_batch::ViewDesc* views, int numViews, const sgi_vpl* in, int n, hipStream_t s);
+hipError_t gatherVisibleEntriesViews(const sgi_batch::ViewDesc* views, int numViews, const sgi_atlas_entry* all,
+                                     int maxCount, hipStream_t s);
+hipError_t cullTilesViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles, int tileSize, int K,
+                          hipStream_t s);
+hipError_t gatherViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles, int tileSize, int K,
+                       const sgi_dev::CubeArgs& cubes, int format, hipStream_t s);
+hipError_t directLightViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles16, hipStream_t s);
+hipError_t denoiseViews(const sgi_batch::ViewDesc* views, int numViews, int totalBlocks, hipStream_t s);
 
 // The merge and the three filter passes into reservoirOut; mask (per 16x16
 // tile, NULL = all) limits them to the marked tiles.

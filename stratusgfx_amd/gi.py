@@ -173,3 +173,53 @@ class FrameBuffers:
         io.reservoirOut = _ptr(self.reservoir_out)
         io.historyOut = _ptr(self.history)
         return io
+
+
+def view_io(inp, tiles, direct, color, reservoir, combined, gi, reservoir_out, history, prev_gi=None,
+            prev_history=None) -> _abi.ViewIO:
+    """One view of a batch (sgi_view_io) from a scene view's device inputs and its buffers."""
+    io = _abi.ViewIO()
+    io.gbuffer = inp.gbuffer()
+    ctypes.memmove(ctypes.byref(io.frame), ctypes.byref(inp.frame), ctypes.sizeof(_abi.Frame))
+    io.csm = inp.csm_struct()
+    io.tiles, io.direct, io.color, io.reservoir = _ptr(tiles), _ptr(direct), _ptr(color), _ptr(reservoir)
+    io.prevGi, io.prevHistory = _ptr(prev_gi), _ptr(prev_history)
+    io.combined, io.giOut, io.reservoirOut, io.historyOut = _ptr(combined), _ptr(gi), _ptr(reservoir_out), _ptr(history)
+    return io
+
+
+class Views:
+    """A batch of views rendered with one launch per stage (sgi_views_*)."""
+
+    def __init__(self, max_views: int, max_pixels: int, tile_size: int = 16, max_vpls_per_tile: int = 64, lib=None):
+        self.lib = lib if lib is not None else _abi.load()
+        cfg = _abi.ViewsConfig(max_views, max_pixels, tile_size, max_vpls_per_tile)
+        h = ctypes.c_void_p()
+        _abi.call(self.lib, "sgi_views_create", ctypes.byref(cfg), ctypes.byref(h))
+        self.handle = h
+        self.tile_size, self.K = tile_size, max_vpls_per_tile
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sgi_views_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, ios: list, vpls, num_vpls: int, entries, cubes: _abi.CubeAtlases, stream=None) -> None:
+        arr = (_abi.ViewIO * len(ios))(*ios)
+        _abi.call(self.lib, "sgi_views_set", self.handle, len(ios), arr, _ptr(vpls), num_vpls, _ptr(entries),
+                  ctypes.byref(cubes), _stream(stream))
+
+    def run(self, stream=None) -> None:
+        _abi.call(self.lib, "sgi_views_run", self.handle, _stream(stream))
+
+    def capture(self, stream) -> None:
+        _abi.call(self.lib, "sgi_views_capture", self.handle, stream)
+
+    def replay(self, stream=None) -> None:
+        _abi.call(self.lib, "sgi_views_replay", self.handle, _stream(stream))

@@ -219,3 +219,30 @@ def to_host(inp: Inputs) -> Inputs:
     return Inputs(inp.width, inp.height, inp.frame, inp.csm, h(inp.depth), h(inp.normal), h(inp.albedo),
                   h(inp.refl), h(inp.rma), h(inp.velocity), h(inp.ids), h(inp.vpls), h(inp.entries),
                   [h(a) for a in inp.atlases], inp.cube_res, inp.cube_format, inp.layers, h(inp.csm_depth))
+
+
+def generate_view_device(scene: Scene, shared: Inputs, device="cuda", stream=None) -> Inputs:
+    """The G-buffer and cascades of the scene's current view (sized by the view), with the VPL
+    records and cube atlases of `shared` (another view of the same scene: they do not depend on
+    the camera)."""
+    import torch
+
+    def clone(st):
+        c = type(st)()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(st), ctypes.sizeof(st))
+        return c
+
+    lib, h = scene.lib, scene.handle
+    W, H = scene.view.width, scene.view.height
+    f32 = dict(dtype=torch.float32, device=device)
+    planes = [torch.empty((H, W), **f32), torch.empty((H, W, 3), **f32), torch.empty((H, W, 4), **f32),
+              torch.empty((H, W, 2), **f32), torch.empty((H, W, 3), **f32), torch.empty((H, W, 2), **f32),
+              torch.empty((H, W), **f32)]
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _abi.call(lib, "sgi_scene_gbuffer_device", h, *[p.data_ptr() for p in planes], s)
+    inp = Inputs(W, H, clone(scene.frame), clone(scene.csm), *planes, shared.vpls, shared.entries,
+                 shared.atlases, shared.cube_res, shared.cube_format, shared.layers)
+    inp.csm_depth = torch.empty((4, scene.view.csm_resolution, scene.view.csm_resolution), **f32)
+    _abi.call(lib, "sgi_scene_csm_device", h, inp.csm_depth.data_ptr(), s)
+    torch.cuda.synchronize()
+    return inp

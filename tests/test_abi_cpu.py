@@ -35,7 +35,8 @@ def test_host_library_exports_host_generator():
 STRUCTS = {"sgi_vpl": None, "sgi_atlas_entry": None, "sgi_tile_stage1": None, "sgi_gbuffer": _abi.Gbuffer,
            "sgi_cube_atlases": _abi.CubeAtlases, "sgi_csm": _abi.Csm, "sgi_frame": _abi.Frame,
            "sgi_config": _abi.Config, "sgi_frame_io": _abi.FrameIO, "sgi_scene_desc": _abi.SceneDesc,
-           "sgi_view": _abi.View, "sgi_kpcn_net": _abi.KpcnNet, "sgi_gather_opts": _abi.GatherOpts}
+           "sgi_view": _abi.View, "sgi_kpcn_net": _abi.KpcnNet, "sgi_gather_opts": _abi.GatherOpts,
+           "sgi_views_config": _abi.ViewsConfig, "sgi_view_io": _abi.ViewIO}
 
 
 def test_struct_layouts_match_c():
