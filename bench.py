@@ -4,23 +4,24 @@ against the a-trous form at equal relative MSE to the converged render.
 
 A step is one GI frame on resident inputs, in RenderScene's order
 (StratusRendererBackend.cpp:1982, 2010-2045), on the procedural city block at
-1920x1080 (6x256x256 FP16 distance cubes):
+1920x1080 with 4096 VPLs (6x256x256 FP16 distance cubes):
 
-  learned   1024 VPLs: global VPL sun-visibility cull and compaction, the
-            visible slots' atlas entries, deferred direct sun light with
-            cascaded PCF, an 8-sample-per-pixel gather over the visible VPLs
-            (the reference's hashed estimator, vpl_pbr_gi.fs:145-200, at a
-            fixed sample count) and the learned denoiser (network + 5x5
+  learned   global VPL sun-visibility cull and compaction, the visible slots'
+            atlas entries, deferred direct sun light with cascaded PCF, an
+            8-sample-per-pixel gather over the visible VPLs (the reference's
+            hashed estimator, vpl_pbr_gi.fs:145-200, at a fixed draw count
+            with failing draws skipped) and the learned denoiser (network + 5x5
             predicted filter, a-trous on the tiles whose predicted error
             exceeds the trained threshold, the temporal pass and combine);
-  a-trous   4096 VPLs: the same steps with S samples per pixel and the 4-pass
-            a-trous / temporal denoiser, S swept over --atrous-spp; the
-            equal-relMSE point is the smallest S whose relMSE is at most the
-            learned form's;
-  tile list 4096 VPLs with per-tile VPL culling (16x16 tiles, up to 64 VPLs)
-            and the tile-list gather + a-trous: the parity-tested plain
-            pipeline, timed for its own throughput and roofline.
+  a-trous   the same steps with S samples per pixel and the 4-pass a-trous /
+            temporal denoiser, S swept over --atrous-spp; the equal-relMSE
+            point is the smallest S whose relMSE is at most the learned form's;
+  tile list per-tile VPL culling (16x16 tiles, up to 64 VPLs) and the
+            tile-list gather + a-trous: the parity-tested plain pipeline, timed
+            for its own throughput and roofline.
 
+The learned form is also run on the 1024-VPL scene (same geometry), whose own
+converged render lies `converged_1024_vs_4096_relmse` from the target.
 relMSE = mean of (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after
 one frame, against the converged render (every visible VPL of the 4096-VPL
 scene, FP64 sums).  `value` is the learned form's frames/s.  Inputs are
@@ -50,12 +51,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
     ap.add_argument("--spp", type=int, default=8, help="gather samples per pixel of the learned form")
-    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[4, 8, 16, 32, 64, 128],
+    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[2, 4, 8, 16, 32, 64, 128],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
     ap.add_argument("--rejection-limit", type=int, default=-1,
                     help="sampled gathers: rejections before draws are taken unconditionally (5 = the reference's "
-                         "rule; -1 = never, so the estimate converges to the every-VPL render)")
+                         "rule); -1 = fixed draw count with failing draws skipped, which converges to the "
+                         "every-VPL render")
+    ap.add_argument("--views", type=int, default=32,
+                    help="batched views of the city block at uneven resolutions (0 = skip)")
     ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
                     help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
@@ -235,6 +239,40 @@ def conv_roofline(net_w, H, W, reps=20):
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
 
 
+def batched_views(args):
+    """Config C2: many views of the city block at uneven resolutions, one launch per stage over all views
+    in a captured graph, launch shape from the autotuner."""
+    import numpy as np
+    import torch
+
+    from stratusgfx_amd import autotune
+    from stratusgfx_amd import scene as sc
+    from stratusgfx_amd.train_kpcn import random_view
+
+    rng = np.random.default_rng(2)
+    sizes = [(1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+    scene = sc.Scene(sc.CITY_1080P)
+    shared = sc.generate_device(scene)
+    inputs = []
+    for i in range(args.views):
+        W, H = sizes[int(rng.integers(len(sizes)))]
+        v = random_view(scene, rng, W, H)
+        v.csm_resolution = sc.CITY_1080P.csm_res
+        scene.set_view(v)
+        inputs.append(sc.generate_view_device(scene, shared))
+    mp = sum(i.width * i.height for i in inputs) / 1e6
+    print(f"[bench] batched views: {len(inputs)} views, {mp:.1f} MP; autotuning", file=sys.stderr, flush=True)
+    tuned = autotune.tune(inputs, shared, sc.CITY_1080P.num_vpls, reps=5)
+    c = tuned["choice"]
+    ms, _ = autotune.measure(inputs, shared, sc.CITY_1080P.num_vpls, c["tile_size"], c["max_vpls_per_tile"],
+                             reps=args.steps)
+    return {"views": len(inputs), "megapixels": round(mp, 2), "tile_size": c["tile_size"],
+            "max_vpls_per_tile": c["max_vpls_per_tile"], "ms_per_batch": round(ms, 3),
+            "views_per_s": round(len(inputs) / (ms * 1e-3), 1), "megapixels_per_s": round(mp / (ms * 1e-3), 1),
+            "reference_shape": tuned["reference"], "autotune": tuned["table"],
+            "note": "one launch per stage over every view, replayed from a captured graph"}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -344,8 +382,11 @@ def main():
               file=sys.stderr, flush=True)
 
     # ---- learned form ----------------------------------------------------------
+    # on the a-trous form's 4096-VPL scene (same VPLs, same sampler: only the sample count and the
+    # denoiser differ), and on the 1024-VPL scene (whose own converged render sits
+    # `converged_1024_vs_4096_relmse` from the target)
     weights_path = kpcn.DEFAULT_WEIGHTS
-    learned_res = None
+    learned_res = learned_1024 = None
     el_learned = None
     wts = None
     if os.path.exists(weights_path) and args.forms in ("all", "learned"):
@@ -356,58 +397,64 @@ def main():
                 threshold = float(json.load(f).get("threshold", float("inf")))
         wts = kpcn.Weights(kpcn.load_params(weights_path))
         net = wts.net()
-        ws = learned.gi.kpcn_workspace()
         n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
         fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
 
-        def learned_step(k, ev, hist=True, thr=threshold):
-            p, fb = learned, learned.fb
-            prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
-            out_gi, out_h = p.hist[(k + 1) % 2]
-            rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
-            rec(0)
-            p.cull()
-            rec(1)
-            p.gi.direct_light(p.g, fb.direct)
-            rec(2)
-            p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
-                           fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp,
-                           rejection_limit=args.rejection_limit)
-            rec(3)
-            p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net, thr,
-                                 ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
-            rec(4)
-            return out_gi
+        def learned_form(state, label):
+            ws = state.gi.kpcn_workspace()
 
-        gi_l = learned_step(0, None, hist=False)
-        torch.cuda.synchronize()
-        learned_rel = rel_mse(gi_l, target)
-        fallback_tiles = int(fbt[0].item())
-        fb = learned.fb
-        learned_noisy = rel_mse(fb.reservoir[..., :3] / fb.reservoir[..., 3:4].clamp_min(1e-5), target)
-        gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
-        torch.cuda.synchronize()
-        learned_rel_nofallback = rel_mse(gi_nf, target)
-        # the a-trous filter alone on the same 8-sample input
-        tmp = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)] + [torch.zeros((H, W), **f32)]
-        learned.gi.denoise(learned.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, None, None, *tmp)
-        torch.cuda.synchronize()
-        atrous_same_input = rel_mse(tmp[1], target)
-        del tmp
-        print(f"[bench] timing {args.steps} learned frames", file=sys.stderr, flush=True)
-        el_learned, per_learned = timed_loop(learned_step, args.steps, args.warmup, 4)
-        learned_res = {
-            "frames_per_s": round(args.steps / el_learned, 2), "ms_per_step": round(el_learned * 1e3 / args.steps, 4),
-            "relmse": round(learned_rel, 6), "relmse_without_fallback": round(learned_rel_nofallback, 6),
-            "relmse_noisy_input": round(learned_noisy, 6), "relmse_atrous_on_same_input": round(atrous_same_input, 6),
-            "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
-            "kernels_ms": {k: round(v, 4) for k, v in
-                           zip(["cull_global", "direct_light", "gather_8spp", "denoise_learned"], per_learned)},
-            "vpls": w_learned.num_vpls, "visible_vpls": int(fb.visible[0].item()),
-            "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
-        }
-        print(f"[bench] learned: relMSE {learned_rel:.5f}, {learned_res['frames_per_s']} frames/s",
-              file=sys.stderr, flush=True)
+            def learned_step(k, ev, hist=True, thr=threshold):
+                p, fb = state, state.fb
+                prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
+                out_gi, out_h = p.hist[(k + 1) % 2]
+                rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
+                rec(0)
+                p.cull()
+                rec(1)
+                p.gi.direct_light(p.g, fb.direct)
+                rec(2)
+                p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
+                               fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp,
+                               rejection_limit=args.rejection_limit)
+                rec(3)
+                p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net,
+                                     thr, ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
+                rec(4)
+                return out_gi
+
+            gi_l = learned_step(0, None, hist=False)
+            torch.cuda.synchronize()
+            rel = rel_mse(gi_l, target)
+            fallback_tiles = int(fbt[0].item())
+            fb = state.fb
+            noisy = rel_mse(fb.reservoir[..., :3] / fb.reservoir[..., 3:4].clamp_min(1e-5), target)
+            gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
+            torch.cuda.synchronize()
+            rel_nf = rel_mse(gi_nf, target)
+            # the a-trous filter alone on the same input
+            tmp = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)] + [torch.zeros((H, W), **f32)]
+            state.gi.denoise(state.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, None, None, *tmp)
+            torch.cuda.synchronize()
+            atrous_same = rel_mse(tmp[1], target)
+            del tmp
+            print(f"[bench] timing {args.steps} learned frames ({label})", file=sys.stderr, flush=True)
+            el, per = timed_loop(learned_step, args.steps, args.warmup, 4)
+            res = {
+                "frames_per_s": round(args.steps / el, 2), "ms_per_step": round(el * 1e3 / args.steps, 4),
+                "relmse": round(rel, 6), "relmse_without_fallback": round(rel_nf, 6),
+                "relmse_noisy_input": round(noisy, 6), "relmse_atrous_on_same_input": round(atrous_same, 6),
+                "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
+                "kernels_ms": {k: round(v, 4) for k, v in
+                               zip(["cull_global", "direct_light", "gather", "denoise_learned"], per)},
+                "vpls": state.w.num_vpls, "visible_vpls": int(fb.visible[0].item()),
+                "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
+            }
+            print(f"[bench] learned ({label}): relMSE {rel:.5f}, {res['frames_per_s']} frames/s", file=sys.stderr,
+                  flush=True)
+            return res, el
+
+        learned_res, el_learned = learned_form(plain, "4096 VPLs")
+        learned_1024, _ = learned_form(learned, "1024 VPLs")
 
     texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
     gather_roof = None
@@ -460,22 +507,29 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 shading, bf16 network (fp32 accumulation)" if learned_res else "fp32",
         "data": "synthetic: seeded procedural city block, ray-cast G-buffer / VPL cubes / cascades",
-        "config": {"workload": "city-1920x1080: learned 1024 VPLs 8 spp + network vs a-trous 4096 VPLs at "
+        "config": {"workload": "city-1920x1080, 4096 VPLs: learned (8 spp + network) vs a-trous (S spp) at "
                                "equal relMSE",
                    "width": W, "height": H, "cube": f"6x{w_plain.cube_res}x{w_plain.cube_res} fp16",
                    "form": form, "converged": "all visible VPLs of the 4096-VPL scene, FP64 sums",
                    "converged_1024_vs_4096_relmse": round(floor_rel, 6)},
         "learned": learned_res,
+        "learned_1024_vpls": learned_1024,
         "atrous_sweep": sweep,
         "plain_tile_lists": plain_res,
         "roofline": roofline,
         "roofline_plain_gather": gather_roof,
+        "batched_views": None,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)
         result["cpu_baseline"] = cpu_baseline(host, w_plain, args.cpu_rows, visible_plain)
+        del host
+    if args.views > 0 and args.forms == "all":
+        del plain, learned
+        torch.cuda.empty_cache()
+        result["batched_views"] = batched_views(args)
     print(json.dumps(result), flush=True)
 
 

@@ -232,10 +232,11 @@ typedef struct sgi_gather_opts {
     int32_t samples;          /* SGI_GATHER_SAMPLES: draws per pixel (1..4096) */
     int32_t precision;        /* SGI_PRECISION_* of the shading */
     int32_t accumulate_fp64;  /* SGI_GATHER_ALL: sum C and w in FP64 */
-    int32_t rejection_limit;  /* SGI_GATHER_SAMPLES: rejections after which draws are taken
-                                 unconditionally (the reference's rule is 5); negative: every
-                                 failing draw is rejected, with at most 32 draws per requested
-                                 sample (the estimate then converges to SGI_GATHER_ALL's) */
+    int32_t rejection_limit;  /* SGI_GATHER_SAMPLES, >= 0: a draw failing the side / radius
+                                 test is drawn again, and after this many rejections draws are
+                                 taken unconditionally (the reference's rule is 5); negative:
+                                 exactly `samples` draws, failing ones skipped (no light, no
+                                 weight), so the estimate converges to SGI_GATHER_ALL's */
 } sgi_gather_opts;
 
 /* The gather's further modes, for measurement and for the learned denoiser's

@@ -546,21 +546,22 @@ void gatherPixel(const sgi_frame& fr, const GB& g, int x, int y, const sgi_vpl* 
         for (int k = 0; k < n; ++k) evalVpl(tileList[1 + k], true);
     } else if (mode == SAMPLES) {
         // the fixed-count estimator: slot int(count * rand) over all visible slots,
-        // the reference's hash and seed step; after `rejectionLimit` rejections the
-        // draws are taken unconditionally (the reference's rule is 5), and with a
-        // negative limit every failing draw is rejected, with at most 32 draws per
-        // requested sample
+        // the reference's hash and seed step.  rejectionLimit >= 0: a failing draw
+        // is drawn again, and after that many rejections draws are taken
+        // unconditionally (the reference's rule is 5); rejectionLimit < 0:
+        // exactly `samples` draws, failing ones skipped (no light, no weight)
         float seedZ = fr.time;
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
-        const int maxDraws = 32 * samples;
-        for (int i = 0, resamples = 0, draws = 0; numVisible > 0 && i < samples; i += 1) {
-            if (rejectionLimit < 0 && draws >= maxDraws) break;
-            ++draws;
+        for (int i = 0, resamples = 0; numVisible > 0 && i < samples; i += 1) {
             const float rnd = random3(seedX, seedY, seedZ);
             seedZ += 10000.0f;
             int lightIndex = (int)((float)numVisible * rnd);
             if (lightIndex > numVisible - 1) lightIndex = numVisible - 1;
-            if (!evalVpl(lightIndex, rejectionLimit < 0 || resamples < rejectionLimit)) {
+            if (rejectionLimit < 0) {
+                evalVpl(lightIndex, true);
+                continue;
+            }
+            if (!evalVpl(lightIndex, resamples < rejectionLimit)) {
                 ++resamples;
                 --i;
                 continue;

@@ -439,6 +439,7 @@ namespace {
 // logit and the tile flags.
 struct KpcnWs {
     uint16_t* act[SGI_KPCN_LEVELS][2];
+    uint16_t* feat;
     float *radiance, *out, *err;
     uint8_t *flags, *dilated;
     int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
@@ -463,12 +464,14 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     }
     const size_t P = (size_t)H * W;
     const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+    char* feat = take(P * SGI_KPCN_CHANNELS * 2);
     char* rad = take(P * 3 * 4);
     char* out = take(P * 3 * 4);
     char* err = take(P * 4);
     char* flags = take(tiles);
     char* dil = take(tiles);
     if (ws) {
+        ws->feat = (uint16_t*)feat;
         ws->radiance = (float*)rad;
         ws->out = (float*)out;
         ws->err = (float*)err;
@@ -517,9 +520,9 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     hipError_t e = hipSuccess;
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
-    // encoder: level 0 from the G-buffer features (computed in the first
-    // convolution's staging), then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::convFeatures(c, ctx->frameArgs, reservoir, ws.radiance, Wl(0), Bl(0), ws.act[0][1], s));
+    // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
+    KPCN_STEP(sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.radiance, s));
+    KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
     KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][1], nullptr, Wl(1), Bl(1), 1, ws.h[0], ws.w[0], ws.act[0][0], 0, s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
         KPCN_STEP(sgi_kpcn::convPool(ws.act[l - 1][0], ws.h[l - 1], ws.w[l - 1], Wl(2 * l), Bl(2 * l), ws.act[l][1],
@@ -536,9 +539,10 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     // the last convolution and the 5x5 filter of the radiance in one kernel
     KPCN_STEP(sgi_kpcn::convApply(ws.act[0][1], H, W, Wl(14), Bl(14), ws.radiance, ws.out, ws.err, s));
     KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
-    // a-trous over the flagged tiles (and their margin), the network elsewhere
+    // a-trous over the flagged tiles (and their margin), the network elsewhere; without a
+    // threshold no tile can be flagged, and only the merge pass runs (for its normals)
     KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
-                                         ws.dilated, s));
+                                         ws.dilated, std::isinf(threshold) && threshold > 0.0f, s));
     KPCN_STEP(sgi_kpcn::select(ws.out, ws.flags, H, W, reservoirOut, s));
     KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
                                           combined, giOut, historyOut, ctx->ws, s));

@@ -259,12 +259,13 @@ __global__ __launch_bounds__(256) void denoiseTemporalViewsKernel(const sgi_batc
 namespace sgi_launch {
 
 hipError_t denoiseSpatial(const GbufArgs& cur, const FrameArgs& fr, const int32_t* visible, const float* reservoir,
-                          float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask, hipStream_t s) {
+                          float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask, bool mergeOnly,
+                          hipStream_t s) {
     const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
     hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
                        ws.normals, mask);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || mergeOnly) return e;
     hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2, ws.resA, ws.normals, ws.resB, mask);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 4, ws.resB, ws.normals, ws.resA, mask);
@@ -287,7 +288,7 @@ hipError_t denoise(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& f
                    const float* screen, const float* color, const float* reservoir, const float* prevGi,
                    const float* prevHistory, float* combined, float* giOut, float* reservoirOut, float* historyOut,
                    const DenoiseBuffers& ws, hipStream_t s) {
-    const hipError_t e = denoiseSpatial(cur, fr, visible, reservoir, reservoirOut, ws, nullptr, s);
+    const hipError_t e = denoiseSpatial(cur, fr, visible, reservoir, reservoirOut, ws, nullptr, false, s);
     if (e != hipSuccess) return e;
     return denoiseTemporal(cur, prev, fr, screen, color, reservoirOut, prevGi, prevHistory, combined, giOut,
                            historyOut, ws, s);

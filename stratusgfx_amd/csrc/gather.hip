@@ -593,15 +593,15 @@ __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs
     if (count > 0) {
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
         float seedZ = fr.time;
-        const int maxDraws = 32 * samples;
         PackedVpl next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
-        for (int i = 0, resamples = 0, draws = 0; i < samples; i += 1) {
-            if (rejectionLimit < 0 && draws >= maxDraws) break;
-            ++draws;
+        for (int i = 0, resamples = 0; i < samples; i += 1) {
             const Vpl L = unpackVpl(next);
             next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
-            const bool allowReject = rejectionLimit < 0 || resamples < rejectionLimit;
-            if (!evalVpl<FMT>(px, fr, cubes, L, allowReject, vplColor, validSamples)) {
+            if (rejectionLimit < 0) {  // fixed draw count: a failing draw is skipped
+                evalVpl<FMT>(px, fr, cubes, L, true, vplColor, validSamples);
+                continue;
+            }
+            if (!evalVpl<FMT>(px, fr, cubes, L, resamples < rejectionLimit, vplColor, validSamples)) {
                 ++resamples;
                 --i;
                 continue;

@@ -162,16 +162,14 @@ __device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int 
 // Where the convolution's input comes from (staged into the LDS halo):
 //   IN_PLAIN     a [h][w][32] (gated by mask when given);
 //   IN_UPADD     a + up(b): the skip tensor plus the coarser level (nearest);
-//   IN_POOL      2x2 mean of a at the finer level (fineH x fineW);
-//   IN_FEATURES  the network input features computed from the G-buffer and the
-//                reservoir, also writing the radiance of the tile's own pixels.
+//   IN_POOL      2x2 mean of a at the finer level (fineH x fineW).
 // and where the result goes:
 //   OUT_BF16 / OUT_F32  [h][w][32] with bias (and ReLU for BF16 when relu != 0);
 //   OUT_APPLY           the 26 logits stay in LDS and the epilogue applies the
 //                       5x5 softmax filter to the radiance (out [h][w][3]) and
 //                       writes the error logit (err [h][w]).
 // Each fused form computes exactly what the separate kernels compute.
-enum { IN_PLAIN = 0, IN_UPADD = 1, IN_POOL = 2, IN_FEATURES = 3 };
+enum { IN_PLAIN = 0, IN_UPADD = 1, IN_POOL = 2 };
 enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2 };
 
 struct ConvSrc {
@@ -179,10 +177,7 @@ struct ConvSrc {
     const uint16_t* b;
     const uint16_t* mask;
     int fineH, fineW;
-    GbufArgs g;
-    FrameArgs fr;
-    const float4* reservoir;
-    float* radiance;
+    const float* radiance;  // OUT_APPLY
 };
 
 __device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
@@ -273,44 +268,18 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
     int t0, t1;
     blockTiles(nTiles, t0, t1);
     uint4 pre[kStageSlots];
-    if constexpr (IN != IN_FEATURES)
-        if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre);
+    if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre);
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
-        const int x0 = tx * kTS - 1, y0 = ty * kTS - 1;
         __syncthreads();
-        if constexpr (IN == IN_FEATURES) {
-            for (int px = threadIdx.x; px < kHS * kHS; px += 256) {
-                const int hy = px / kHS, hx = px % kHS;
-                const int gy = y0 + hy, gx = x0 + hx;
-                uint32_t o[16];
+        // this tile's chunks were loaded into `pre` before the previous tile's MFMAs
 #pragma unroll
-                for (int j = 0; j < 16; ++j) o[j] = 0u;
-                if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-                    float L[3];
-                    pixelFeatures(src.g, src.fr, src.reservoir, gx, gy, o, L);
-                    if (hy >= 1 && hy <= kTS && hx >= 1 && hx <= kTS) {
-                        float* rd = src.radiance + 3 * ((size_t)gy * W + gx);
-                        rd[0] = L[0];
-                        rd[1] = L[1];
-                        rd[2] = L[2];
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    halo[px * kPxU4 + c] = make_uint4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
-            }
-            __syncthreads();
-        } else {
-            // this tile's chunks were loaded into `pre` before the previous tile's MFMAs
-#pragma unroll
-            for (int k = 0; k < kStageSlots; ++k) {
-                const int i = (int)threadIdx.x + k * 256;
-                if (i < kHS * kHS * 4) halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
-            }
-            __syncthreads();
-            if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre);
+        for (int k = 0; k < kStageSlots; ++k) {
+            const int i = (int)threadIdx.x + k * 256;
+            if (i < kHS * kHS * 4) halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
         }
+        __syncthreads();
+        if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre);
         f32x4 acc[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -790,16 +759,6 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
     return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
 }
 
-hipError_t convFeatures(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, float* radiance,
-                        const uint16_t* w, const float* bias, uint16_t* out, hipStream_t s) {
-    ConvSrc src{};
-    src.g = g;
-    src.fr = fr;
-    src.reservoir = (const float4*)reservoir;
-    src.radiance = radiance;
-    return launchConv<IN_FEATURES, OUT_BF16>(src, w, bias, 1, g.H, g.W, out, nullptr, s);
-}
-
 hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
                     hipStream_t s) {
     ConvSrc src{};
@@ -817,7 +776,7 @@ hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, co
     return launchConv<IN_UPADD, OUT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
 }
 
-hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, float* radiance,
+hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
                      float* out, float* err, hipStream_t s) {
     ConvSrc src{};
     src.a = in;

@@ -60,10 +60,11 @@ hipError_t directLightViews(const sgi_batch::ViewDesc* views, int numViews, int 
 hipError_t denoiseViews(const sgi_batch::ViewDesc* views, int numViews, int totalBlocks, hipStream_t s);
 
 // The merge and the three filter passes into reservoirOut; mask (per 16x16
-// tile, NULL = all) limits them to the marked tiles.
+// tile, NULL = all) limits them to the marked tiles; mergeOnly stops after the
+// merge (whose decoded normals the temporal pass reads).
 hipError_t denoiseSpatial(const sgi_dev::GbufArgs& cur, const sgi_dev::FrameArgs& fr, const int32_t* visible,
                           const float* reservoir, float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask,
-                          hipStream_t s);
+                          bool mergeOnly, hipStream_t s);
 
 // The temporal blend and combine on a filtered reservoir (normals from denoiseSpatial).
 hipError_t denoiseTemporal(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, const sgi_dev::FrameArgs& fr,
@@ -79,14 +80,13 @@ namespace sgi_kpcn {
 hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
                    int W, void* out, int outF32, hipStream_t s);
 // Inference forms of the convolution with the neighbouring pass fused into the
-// input staging or the epilogue (same results as the separate kernels).
-hipError_t convFeatures(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir,
-                        float* radiance, const uint16_t* w, const float* bias, uint16_t* out, hipStream_t s);
+// input staging (pooling, upsample + skip) or the epilogue (the 5x5 filter);
+// same results as the separate kernels.
 hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
                     hipStream_t s);
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
                      uint16_t* out, hipStream_t s);
-hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, float* radiance,
+hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
                      float* out, float* err, hipStream_t s);
 int wgradBlocks(int H, int W);
 hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,

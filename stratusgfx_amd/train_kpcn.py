@@ -3,9 +3,8 @@
     python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --size 512 --steps 4000
 
 Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
-the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES, every
-failing draw rejected, so that more samples converge to the target) of 4096
-VPLs, target = the converged gather over every visible VPL with FP64 sums
+the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES, failing
+draws skipped, so that more samples converge to the target) of 4096 VPLs, target = the converged gather over every visible VPL with FP64 sums
 (sgi_gather_ex ALL), both normalised as the denoiser's pass 0 does.
 Training: Adam on the relative-MSE loss plus the error head's regression
 (sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
@@ -43,13 +42,14 @@ def _inside_box(boxes: np.ndarray, p, margin=0.75) -> bool:
     return bool(np.any(np.all(d <= h + margin, axis=1)))
 
 
-def random_view(scene: sc.Scene, rng, size: int) -> _abi.View:
+def random_view(scene: sc.Scene, rng, width: int, height: int | None = None) -> _abi.View:
     """A pose uniform over the block at 1.5-30 m, random yaw / pitch, outside every box."""
     n = scene.lib.sgi_scene_num_boxes(scene.handle)
     boxes = np.zeros((n, 16), np.float32)
     _abi.call(scene.lib, "sgi_scene_boxes", scene.handle, boxes.ctypes.data)
     v = _abi.View()
-    _abi.call(scene.lib, "sgi_scene_default_view", scene.handle, size, size, ctypes.byref(v))
+    _abi.call(scene.lib, "sgi_scene_default_view", scene.handle, width, height if height else width,
+              ctypes.byref(v))
     e = scene.w.city_half_extent * 0.9
     for _ in range(100):
         eye = (rng.uniform(-e, e), rng.uniform(1.5, 30.0), rng.uniform(-e, e))

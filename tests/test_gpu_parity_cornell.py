@@ -166,7 +166,7 @@ def test_render_frame_chain(case, ora, gi, use_tiles):
 def test_gather_ex_modes(case, ora, gi, mode, acc64, rej):
     """sgi_gather_ex: every visible VPL (FP32 or FP64 sums of the FP32 per-VPL terms, both against the
     FP32 restatement: the decisions are the same, the sums differ by rounding) and 8 hashed draws per pixel
-    with the reference's 5-rejection rule and with every failing draw rejected."""
+    with the reference's 5-rejection rule and with failing draws skipped."""
     from stratusgfx_amd import _abi
     torch = require_gpu()
     upd, vis, ent = _device_visible(case, ora)

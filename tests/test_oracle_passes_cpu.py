@@ -318,9 +318,10 @@ def test_fp64_twin_agrees_with_fp32(cornell):
 
 
 def test_sampled_gather_converges_to_every_vpl_render(cornell):
-    """With every failing draw rejected, the hashed sampler is a ratio estimator over the accepted VPLs:
-    its normalised output approaches the every-VPL render as the sample count grows, while the
-    reference's 5-rejection rule converges elsewhere (accepted failing draws add weight, no light)."""
+    """With failing draws skipped, the hashed sampler is a ratio estimator over the VPLs that pass the
+    side / radius test: its normalised output approaches the every-VPL render as the sample count grows,
+    while the reference's 5-rejection rule converges elsewhere (accepted failing draws add weight, no
+    light)."""
     w, inp = cornell
     csm = inp.csm_struct()
     upd, vis = orc.cull_global(inp.frame, csm, inp.vpls)
