@@ -239,14 +239,19 @@ __device__ __forceinline__ h2 hsqrt(h2 a) { return __builtin_elementwise_sqrt(a)
 __device__ __forceinline__ h2 hexp2(h2 a) { return __builtin_elementwise_exp2(a); }
 
 struct PixelH {  // the pixel's shading inputs, both halves equal
-    h2 nx, ny, nz, vx, vy, vz;
-    h2 nv, ndotv, r2, r2m1, gv, halfInvGv;
+    h2 ndotv, r2, r2m1, gv, halfInvGv;
     h2 f0r, f0g, f0b, dr, dg, db;
     bool nvZero;  // N.V <= 0 (saturated to 0)
 };
 
+// The three dot products of the unit specular direction Ls are formed in FP32:
+// rounded to FP16 first, 1 + V.Ls cancels when the light lies opposite the
+// viewer (V.Ls near -1, a grazing view) and the half vector's length and
+// direction lose every significant bit.
 struct LightH {  // one VPL's FP32 inputs to the packed stage
-    float lx, ly, lz;  // unit specular direction
+    float nl;          // N.Ls
+    float opl;         // 1 + V.Ls = |V + Ls|^2 / 2
+    float nvl;         // N.V + N.Ls = N.(V + Ls)
     float uD, uS;      // 8 |L - P| / R and 8 |S - P| / R
     float cosD;        // dot(N, L - P) / |L - P|
     float invR64;      // 64 / R
@@ -255,7 +260,7 @@ struct LightH {  // one VPL's FP32 inputs to the packed stage
 
 __device__ __forceinline__ LightH noLightH() {
     LightH o;
-    o.lx = 0.0f; o.ly = 0.0f; o.lz = 1.0f;
+    o.nl = 0.0f; o.opl = 1.0f; o.nvl = 0.0f;
     o.uD = 0.0f; o.uS = 0.0f; o.cosD = 0.0f; o.invR64 = 1.0f;
     o.col = splat(0.0f);
     return o;
@@ -263,9 +268,7 @@ __device__ __forceinline__ LightH noLightH() {
 
 __device__ __forceinline__ PixelH toHalf(const Pixel& px) {
     PixelH q;
-    q.nx = hs(px.N.x); q.ny = hs(px.N.y); q.nz = hs(px.N.z);
-    q.vx = hs(px.V.x); q.vy = hs(px.V.y); q.vz = hs(px.V.z);
-    q.nv = hs(px.nvRaw); q.ndotv = hs(px.NdotV); q.r2 = hs(px.r2); q.r2m1 = hs(px.r2m1); q.gv = hs(px.gv);
+    q.ndotv = hs(px.NdotV); q.r2 = hs(px.r2); q.r2m1 = hs(px.r2m1); q.gv = hs(px.gv);
     q.halfInvGv = hs(0.5f / px.gv);
     q.nvZero = px.NdotV == 0.0f;
     q.f0r = hs(px.f0.x); q.f0g = hs(px.f0.y); q.f0b = hs(px.f0.z);
@@ -278,7 +281,9 @@ __device__ __forceinline__ LightH prepHalf(const Pixel& px, const Vpl& L, const 
     const float s2 = dotF(specDir, specDir);
     const float sInv = frsq(s2);
     LightH o;
-    o.lx = specDir.x * sInv; o.ly = specDir.y * sInv; o.lz = specDir.z * sInv;
+    o.nl = dotF(px.N, specDir) * sInv;
+    o.opl = 1.0f + dotF(px.V, specDir) * sInv;
+    o.nvl = px.nvRaw + o.nl;
     const float invR8 = 8.0f * L.invRadius;
     o.uD = h.dist * invR8;
     o.uS = gmin((s2 * sInv) * invR8, 240.0f);
@@ -291,14 +296,12 @@ __device__ __forceinline__ LightH prepHalf(const Pixel& px, const Vpl& L, const 
 // The lighting of two VPLs (pbr2.glsl:51-76, 541-574, 613-657 and pbr.glsl:260-267,
 // rearranged as in vplLighting), accumulated into acc.
 __device__ __forceinline__ void shadePairH(const PixelH& q, const LightH& a, const LightH& b, f3& acc) {
-    const h2 lx = hpair(a.lx, b.lx), ly = hpair(a.ly, b.ly), lz = hpair(a.lz, b.lz);
-    const h2 nlRaw = hfma(q.nz, lz, hfma(q.ny, ly, q.nx * lx));
-    const h2 lv = hfma(q.vz, lz, hfma(q.vy, ly, q.vx * lx));
-    // |V + Ls|^2 = 2 + 2 V.Ls, kept >= 2^-14 so that H stays finite
-    const h2 hInv = hrsq(hmax(hfma(hs(2.0f), lv, hs(2.0f)), hs(0x1p-14f)));
+    const h2 nlRaw = hpair(a.nl, b.nl), opl = hpair(a.opl, b.opl), nvl = hpair(a.nvl, b.nvl);
+    // |V + Ls|^2 = 2 (1 + V.Ls), kept >= 2^-14 so that H stays finite
+    const h2 hInv = hrsq(hmax(hs(2.0f) * opl, hs(0x1p-14f)));
     const h2 NdotL = hsat(nlRaw);
-    const h2 NdotH = hsat((q.nv + nlRaw) * hInv);
-    const h2 LdotH = hsat((hs(1.0f) + lv) * hInv);
+    const h2 NdotH = hsat(nvl * hInv);
+    const h2 LdotH = hsat(opl * hInv);
     const h2 f = hfma(NdotH * q.r2m1, NdotH, hs(1.0f));
     // the FP32 path's 1e-5 floors are below FP16's normal range (subnormals flush), so
     // the floors here are 2^-14, the smallest normal FP16 value

@@ -7,9 +7,11 @@ the shaded radiance (reservoir.rgb) carries FP16 rounding.  Its error is stated
 per output channel: rel-L2 against the FP64 oracle, rel-L2 against the FP32 GPU
 path (the FP16 arithmetic alone) and the 99th percentile of the per-pixel
 relative error.  Bounds: rel-L2 <= 1e-2 per channel, p99 <= 1e-2 (FP16 has an
-11-bit significand, ~5e-4 relative per rounding, the shading chains about
-twenty roundings per VPL, and near-grazing half vectors amplify the rounding
-of N.H for a few pixels)."""
+11-bit significand, ~5e-4 relative per rounding, and the shading chains about
+twenty roundings per VPL).  The random mode sums one to six VPLs per pixel, so
+a single badly conditioned VPL is not averaged away there: its case is a light
+opposite a grazing viewer (V.Ls near -1), where 1 + V.Ls cancels; the FP16
+form takes N.Ls, 1 + V.Ls and N.V + N.Ls from FP32 for that reason."""
 import numpy as np
 import pytest
 
