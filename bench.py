@@ -240,7 +240,7 @@ def conv_roofline(net_w, H, W, reps=20):
 
 
 def batched_views(args):
-    """Config C2: many views of the city block at uneven resolutions, one launch per stage over all views
+    """Many views of the city block at uneven resolutions, one launch per stage over all views
     in a captured graph, launch shape from the autotuner."""
     import numpy as np
     import torch

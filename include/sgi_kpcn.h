@@ -13,9 +13,9 @@
  * Network: a 5-level encoder-decoder of 3x3 convolutions with 32 channels
  * (levels at 1, 1/2, ..., 1/16 resolution; 2x2 mean pooling down, nearest
  * upsampling plus skip addition up; ReLU after every convolution but the
- * last).  The last convolution produces per pixel 25 logits of a 5x5 filter
- * (softmax-normalised, applied to the noisy radiance) and one logit of the
- * predicted log relative squared error.
+ * filter heads).  A head per level predicts per pixel the 25 logits of a 5x5
+ * filter (softmax-normalised) of that level's radiance, and one blend logit;
+ * the level-0 head also predicts the log relative squared error.
  *
  *   layer  0: features -> c0          (level 0)      layer 1: c0 -> e0
  *   layer 2: pool(e0) -> c2           (level 1)      layer 3: c2 -> e1
@@ -24,7 +24,19 @@
  *   layer 8: pool(e3) -> c8           (level 4)      layer 9: c8 -> e4
  *   layer 10: e3 + up(e4) -> d3       layer 11: e2 + up(d3) -> d2
  *   layer 12: e1 + up(d2) -> d1       layer 13: e0 + up(d1) -> d0
- *   layer 14: d0 -> logits (no ReLU, FP32)
+ *   heads (no ReLU, FP32 logits): layer 14: d0 (level 0), 15: d1, 16: d2,
+ *   17: d3, 18: e4 (level 4)
+ *
+ * Logit channels: 0..24 filter taps (tap j at offset (j % 5 - 2, j / 5 - 2),
+ * clamped to the image), SGI_KPCN_ERR_CHANNEL the error estimate (level 0),
+ * SGI_KPCN_BLEND_CHANNEL the blend (levels 0..3); the others are unused.
+ *
+ * Reconstruction, coarse to fine: r_0 = the noisy radiance, r_l = the 2x2
+ * mean of r_(l-1) (the radiance pyramid); F_l = the level's filter applied
+ * to r_l; o_4 = F_4 and o_l = F_l + sigmoid(blend_l) (up(o_(l+1)) - F_l),
+ * where up is bilinear: fine pixel x samples the coarser grid at
+ * (x + 0.5) / 2 - 0.5 (weights 1/4, 3/4), indices clamped to the grid.  The
+ * output is o_0.
  *
  * Tensors: activations and their gradients are BF16 [h][w][32] (channels
  * innermost, 64 B per pixel); level l has ceil(h / 2^l) x ceil(w / 2^l)
@@ -47,11 +59,13 @@ extern "C" {
 
 #define SGI_KPCN_CHANNELS 32
 #define SGI_KPCN_LEVELS 5
-#define SGI_KPCN_LAYERS 15
+#define SGI_KPCN_LAYERS 19
 #define SGI_KPCN_LAYER_WEIGHTS (9 * 32 * 32)
 #define SGI_KPCN_LAYER_PARAMS (SGI_KPCN_LAYER_WEIGHTS + 32)
 #define SGI_KPCN_PARAMS (SGI_KPCN_LAYERS * SGI_KPCN_LAYER_PARAMS)
 #define SGI_KPCN_TAPS 25          /* 5x5 predicted filter */
+#define SGI_KPCN_ERR_CHANNEL 25   /* level-0 logit of the log relative squared error */
+#define SGI_KPCN_BLEND_CHANNEL 26 /* logit of the blend with the coarser level */
 #define SGI_KPCN_FEATURES 10      /* log radiance 3, albedo 3, normal 3, depth 1 */
 #define SGI_KPCN_TILE 16          /* fallback decision granularity (pixels) */
 #define SGI_KPCN_RADIANCE_SCALE 64.0f  /* radiance feature = log(1 + 64 radiance) */
@@ -88,12 +102,21 @@ int sgi_kpcn_pool_backward(const uint16_t* dout, int32_t height, int32_t width, 
                            int32_t accumulate, void* stream);
 
 /* out = skip + up(low) (nearest): skip, out BF16 at (height, width), low at
- * the coarser level.  Backward of the low branch: dlow[Y][X] = sum of the
- * in-range dout[2Y + i][2X + j] (the skip branch's gradient is dout itself). */
+ * the coarser level.  Backward of the low branch: dlow[Y][X] (+)= sum of the
+ * in-range dout[2Y + i][2X + j] (the skip branch's gradient is dout itself);
+ * accumulate 0 overwrites dlow, 1 adds to it. */
 int sgi_kpcn_upsample_add(const uint16_t* skip, const uint16_t* low, int32_t height, int32_t width,
                           uint16_t* out, void* stream);
 int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* dlow,
-                               void* stream);
+                               int32_t accumulate, void* stream);
+
+/* The radiance pyramid: levels 1..4 of r_l = 2x2 mean (in-range children) of
+ * r_(l-1), r_0 = radiance FP32 [h][w][3].  pyramid: FP32, the levels one
+ * after the other ([h_1][w_1][3] first, h_l = ceil(h_(l-1) / 2)), of
+ * sgi_kpcn_pyramid_floats(h, w) floats. */
+size_t sgi_kpcn_pyramid_floats(int32_t height, int32_t width);
+int sgi_kpcn_radiance_pyramid(const float* radiance, int32_t height, int32_t width, float* pyramid,
+                              void* stream);
 
 /* Network input from a gather: radiance = reservoir.rgb / max(reservoir.a,
  * 1e-5) (FP32 [h][w][3], the a-trous filter's pass-0 normalisation,
@@ -104,27 +127,45 @@ int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t wid
 int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const float* reservoir, uint16_t* features,
                       float* radiance, void* stream);
 
-/* Filter application: out = sum_j softmax(logits[0..24])_j * radiance(p + o_j)
- * over the 5x5 window (clamped to the image), err = logits[25].  logits FP32
- * [h][w][32]; out FP32 [h][w][3]; err FP32 [h][w] or NULL. */
-int sgi_kpcn_apply(const float* logits, const float* radiance, int32_t height, int32_t width, float* out,
-                   float* err, void* stream);
+/* One level of the reconstruction: F = sum_j softmax(logits[0..24])_j *
+ * radiance(p + o_j) over the 5x5 window (clamped to the image); out = F when
+ * coarse is NULL, else F + sigmoid(logits[SGI_KPCN_BLEND_CHANNEL]) *
+ * (up(coarse) - F); err = logits[SGI_KPCN_ERR_CHANNEL].  logits FP32
+ * [h][w][32]; radiance FP32 [h][w][3] (this level's); coarse FP32
+ * [ceil(h/2)][ceil(w/2)][3] or NULL; out FP32 [h][w][3]; err FP32 [h][w] or
+ * NULL. */
+int sgi_kpcn_apply(const float* logits, const float* radiance, const float* coarse, int32_t height,
+                   int32_t width, float* out, float* err, void* stream);
 
-/* Training loss and its gradient.  With out / err as sgi_kpcn_apply,
+/* Backward of sgi_kpcn_apply for the output gradient dout (FP32 [h][w][3]):
+ * dlogits BF16 [h][w][32] (filter and blend channels; the error channel
+ * takes derr (FP32 [h][w]) or 0 when derr is NULL; the rest 0) and, when
+ * coarse is given, dup FP32 [h][w][3] = the gradient of up(coarse). */
+int sgi_kpcn_apply_backward(const float* logits, const float* radiance, const float* coarse, const float* dout,
+                            const float* derr, int32_t height, int32_t width, uint16_t* dlogits, float* dup,
+                            void* stream);
+
+/* The transpose of the bilinear up(): dcoarse FP32 [ceil(h/2)][ceil(w/2)][3]
+ * = sum over fine pixels of their bilinear weight times dup (FP32 [h][w][3]);
+ * overwrites dcoarse. */
+int sgi_kpcn_upsample_bilinear_backward(const float* dup, int32_t height, int32_t width, float* dcoarse,
+                                        void* stream);
+
+/* Training loss and its gradient on the reconstruction out (FP32 [h][w][3])
+ * and the level-0 logits' error channel err:
  * loss = mean_{p,c} (out - t)^2 / (t^2 + 0.01)
  *      + err_weight * mean_p (err - log(r_p + 1e-4))^2,
  * r_p = mean_c (out - t)^2 / (t^2 + 0.01) (held constant).  target FP32
- * [h][w][3]; dlogits BF16 [h][w][32] (channels 26..31 zero); loss_sums FP32
- * [2] accumulates the two sums (before the means).  scale multiplies the
- * gradient (1 / batch). */
-int sgi_kpcn_loss(const float* logits, const float* radiance, const float* target, int32_t height,
-                  int32_t width, float err_weight, float scale, uint16_t* dlogits, float* loss_sums,
-                  void* stream);
+ * [h][w][3]; dout FP32 [h][w][3] and derr FP32 [h][w] receive the gradients
+ * times scale (1 / batch); loss_sums FP32 [2] accumulates the two sums
+ * (before the means). */
+int sgi_kpcn_loss(const float* out, const float* logits, const float* target, int32_t height, int32_t width,
+                  float err_weight, float scale, float* dout, float* derr, float* loss_sums, void* stream);
 
 /* One Adam step over all SGI_KPCN_PARAMS parameters (FP32 params, grad, m, v)
- * followed by the BF16 copies: weights [15][9][32][32], weights_t (the
+ * followed by the BF16 copies: weights [layers][9][32][32], weights_t (the
  * transposed, tap-mirrored weights of the input gradient: weights_t[l][t][i][o]
- * = W[l][8 - t][o][i]) and biases FP32 [15][32]. */
+ * = W[l][8 - t][o][i]) and biases FP32 [layers][32]. */
 int sgi_kpcn_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
                   float eps, int32_t step, uint16_t* weights, uint16_t* weights_t, float* biases, void* stream);
 
@@ -137,8 +178,8 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 /* Device workspace of sgi_denoise_learned at a viewport (bytes). */
 size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width);
 
-/* The learned denoise of one frame: features -> network -> 5x5 filter of the
- * noisy radiance; then per 16x16 tile, if the mean predicted relative squared
+/* The learned denoise of one frame: features -> network -> multi-scale
+ * reconstruction of the noisy radiance; then per 16x16 tile, if the mean predicted relative squared
  * error exp(err) of its pixels exceeds `threshold`, the tile takes the a-trous
  * result instead (the a-trous passes run only over the flagged tiles and a
  * 28-pixel margin, and not at all when no tile is flagged).  Outputs as

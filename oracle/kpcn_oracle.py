@@ -2,11 +2,12 @@
 
 numpy float64 restatement of the learned denoiser (the kernel-predicting
 encoder-decoder whose interface include/sgi_kpcn.h declares): forward pass,
-loss and the full backward pass.  The reference has no learned denoiser, so
+multi-scale filter reconstruction, loss and the full backward pass.  The reference has no learned denoiser, so
 this oracle follows the interface's own definitions; it is pinned by central
-finite differences of its loss (tests/test_kpcn_oracle_cpu.py) and by small
-closed forms (a one-hot filter copies its tap, pooling of a constant is the
-constant, upsampling commutes with pooling's adjoint).
+finite differences of its loss (tests/test_kpcn_oracle_cpu.py), by adjoint
+identities of every linear piece (bilinear upsampling against its transpose
+included) and by small closed forms (a one-hot filter copies its tap, pooling
+of a constant is the constant, bilinear upsampling reproduces linear ramps).
 
 `bf16=True` rounds every stored tensor to BF16 (round to nearest even) at the
 points where the GPU stores BF16 (weights, layer outputs, gradients between
@@ -19,8 +20,11 @@ import numpy as np
 
 C = 32
 LEVELS = 5
-LAYERS = 15
+LAYERS = 19
 TAPS = 25
+ERR = TAPS       # level-0 logit channel of the predicted log relative squared error
+BLEND = TAPS + 1  # logit channel of the blend with the coarser level's output
+HEADS = (14, 15, 16, 17, 18)  # the filter-predicting layer of levels 0..4
 LAYER_WEIGHTS = 9 * C * C
 LAYER_PARAMS = LAYER_WEIGHTS + C
 PARAMS = LAYERS * LAYER_PARAMS
@@ -127,33 +131,128 @@ def softmax25(logits):
     return e / e.sum(-1, keepdims=True)
 
 
-def apply(logits, radiance):
-    """(out [H][W][3], err [H][W]) as sgi_kpcn_apply."""
-    w = softmax25(logits)
-    out = np.einsum("hwj,jhwc->hwc", w, _taps(radiance))
-    return out, logits[..., TAPS]
+def filter5(logits, radiance):
+    """The 5x5 softmax filter of the radiance alone (no blend)."""
+    return np.einsum("hwj,jhwc->hwc", softmax25(logits), _taps(radiance))
 
 
-def loss(logits, radiance, target, err_weight):
-    """(loss, dlogits [H][W][32]) as sgi_kpcn_loss with scale 1."""
-    H, W = radiance.shape[:2]
+def _bilinear_axis(n_fine, n_coarse):
+    """Source indices and weights of one axis of 2x bilinear upsampling: fine
+    pixel x samples the coarse grid at (x + 0.5) / 2 - 0.5, indices clamped."""
+    x = np.arange(n_fine)
+    k = x // 2
+    odd = (x % 2) == 1
+    i0 = np.where(odd, k, k - 1)
+    w0 = np.where(odd, 0.75, 0.25)
+    return np.clip(i0, 0, n_coarse - 1), np.clip(i0 + 1, 0, n_coarse - 1), w0, 1.0 - w0
+
+
+def upsample_bilinear(low, H, W):
+    """[H][W][c] from the coarser level low ([ceil(H/2)][ceil(W/2)][c])."""
+    h, w = low.shape[:2]
+    y0, y1, wy0, wy1 = _bilinear_axis(H, h)
+    x0, x1, wx0, wx1 = _bilinear_axis(W, w)
+    wx0, wx1 = wx0[None, :, None], wx1[None, :, None]
+    top = wx0 * low[y0][:, x0] + wx1 * low[y0][:, x1]
+    bot = wx0 * low[y1][:, x0] + wx1 * low[y1][:, x1]
+    return wy0[:, None, None] * top + wy1[:, None, None] * bot
+
+
+def upsample_bilinear_backward(d):
+    """The transpose of upsample_bilinear: the coarser level's gradient."""
+    H, W = d.shape[:2]
+    h, w = (H + 1) // 2, (W + 1) // 2
+    y0, y1, wy0, wy1 = _bilinear_axis(H, h)
+    x0, x1, wx0, wx1 = _bilinear_axis(W, w)
+    out = np.zeros((h, w) + d.shape[2:])
+    for ys, wy in ((y0, wy0), (y1, wy1)):
+        for xs, wx in ((x0, wx0), (x1, wx1)):
+            np.add.at(out, (ys[:, None], xs[None, :]), wy[:, None, None] * wx[None, :, None] * d)
+    return out
+
+
+def sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def apply(logits, radiance, coarse=None):
+    """(out [H][W][3], err [H][W]) as sgi_kpcn_apply: the 5x5 softmax filter f
+    of the radiance, blended with the coarser level's output when given:
+    out = f + sigmoid(logits[BLEND]) (up(coarse) - f)."""
+    f = filter5(logits, radiance)
+    if coarse is not None:
+        b = sigmoid(logits[..., BLEND])[..., None]
+        f = f + b * (upsample_bilinear(coarse, *radiance.shape[:2]) - f)
+    return f, logits[..., ERR]
+
+
+def apply_backward(logits, radiance, coarse, dout, derr=None):
+    """(dlogits [H][W][32], dup [H][W][3] or None) as sgi_kpcn_apply_backward:
+    dup is the gradient of up(coarse), dlogits[ERR] = derr."""
     taps = _taps(radiance)
     w = softmax25(logits)
-    out = np.einsum("hwj,jhwc->hwc", w, taps)
+    dl = np.zeros(logits.shape)
+    dup = None
+    df = dout
+    if coarse is not None:
+        f = np.einsum("hwj,jhwc->hwc", w, taps)
+        u = upsample_bilinear(coarse, *radiance.shape[:2])
+        b = sigmoid(logits[..., BLEND])
+        df = dout * (1.0 - b)[..., None]
+        dup = dout * b[..., None]
+        dl[..., BLEND] = (dout * (u - f)).sum(-1) * b * (1.0 - b)
+    dw = np.einsum("hwc,jhwc->hwj", df, taps)
+    dl[..., :TAPS] = w * (dw - (w * dw).sum(-1, keepdims=True))
+    if derr is not None:
+        dl[..., ERR] = derr
+    return dl, dup
+
+
+def loss(out, err_logit, target, err_weight):
+    """(loss, dout [H][W][3], derr [H][W]) as sgi_kpcn_loss with scale 1:
+    loss = mean_{p,c} (out - t)^2 / (t^2 + 0.01) + err_weight mean_p (err - log(r_p + 1e-4))^2
+    with r_p = mean_c (out - t)^2 / (t^2 + 0.01) held constant."""
+    H, W = out.shape[:2]
     den = target * target + 0.01
     e = out - target
-    r = (e * e / den)
-    rp = r.mean(-1)
-    err_t = np.log(rp + 1e-4)
-    de = logits[..., TAPS] - err_t
+    r = e * e / den
+    err_t = np.log(r.mean(-1) + 1e-4)
+    de = err_logit - err_t
     N, P = 3.0 * H * W, float(H * W)
     L = r.sum() / N + err_weight * (de * de).sum() / P
-    d_out = 2.0 * e / den / N
-    dw = np.einsum("hwc,jhwc->hwj", d_out, taps)
-    dl = np.zeros(logits.shape)
-    dl[..., :TAPS] = w * (dw - (w * dw).sum(-1, keepdims=True))
-    dl[..., TAPS] = 2.0 * err_weight * de / P
-    return L, dl
+    return L, 2.0 * e / den / N, 2.0 * err_weight * de / P
+
+
+def pyramid(radiance):
+    """The radiance at levels 0..4 (each level the 2x2 mean of the finer one)."""
+    out = [radiance]
+    for _ in range(1, LEVELS):
+        out.append(pool(out[-1]))
+    return out
+
+
+def reconstruct(logits, radiance):
+    """Coarse-to-fine reconstruction: outputs of levels 0..4 (o[0] is the
+    denoised radiance) and the level-0 error logit, from the per-level logits."""
+    rad = pyramid(radiance)
+    o = [None] * LEVELS
+    o[LEVELS - 1], _ = apply(logits[LEVELS - 1], rad[LEVELS - 1])
+    for l in range(LEVELS - 2, -1, -1):
+        o[l], _ = apply(logits[l], rad[l], o[l + 1])
+    return o, logits[0][..., ERR]
+
+
+def reconstruct_backward(logits, radiance, o, dout, derr):
+    """Per-level dlogits of reconstruct for the gradient of o[0] (and of the error logit)."""
+    rad = pyramid(radiance)
+    dl = [None] * LEVELS
+    d = dout
+    for l in range(LEVELS):
+        coarse = o[l + 1] if l + 1 < LEVELS else None
+        dl[l], dup = apply_backward(logits[l], rad[l], coarse, d, derr if l == 0 else None)
+        if dup is not None:
+            d = upsample_bilinear_backward(dup)
+    return dl
 
 
 class Net:
@@ -188,14 +287,20 @@ class Net:
             a[f"u{l}"] = self._r(e + upsample(low, *e.shape[:2]))
             a[f"d{l}"] = self._conv(layer, a[f"u{l}"])
             low = a[f"d{l}"]
-        a["logits"] = conv3x3(a["d0"], self.W[14], self.b[14])
-        if self.bf16:
-            a["logits"] = a["logits"].astype(np.float32).astype(np.float64)
+        # the filter heads: level l reads d_l (e4 at the coarsest level)
+        a["logits"] = []
+        for l in range(LEVELS):
+            src = a[f"d{l}"] if l < LEVELS - 1 else a[f"e{LEVELS - 1}"]
+            lg = conv3x3(src, self.W[HEADS[l]], self.b[HEADS[l]])
+            a["logits"].append(lg.astype(np.float32).astype(np.float64) if self.bf16 else lg)
         self.a = a
         return a["logits"]
 
+    def head_input(self, l):
+        return self.a[f"d{l}"] if l < LEVELS - 1 else self.a[f"e{LEVELS - 1}"]
+
     def backward(self, dlogits):
-        """Gradient of every layer's parameters [15][9248] for the logits' gradient."""
+        """Gradient of every layer's parameters [19][9248] for the per-level logits' gradients."""
         a, r = self.a, self._r
         grads = np.zeros((LAYERS, LAYER_PARAMS))
 
@@ -206,13 +311,16 @@ class Net:
             grads[l, LAYER_WEIGHTS:] = db
             return r(dx) if need_dx else None
 
-        g = layer_back(14, a["d0"], None, r(dlogits), relu=False)
+        # the heads' input gradients (layer outputs, stored BF16)
+        dh = [layer_back(HEADS[l], self.head_input(l), None, r(dlogits[l]), relu=False) for l in range(LEVELS)]
+        g = dh[0]
         de = {}
         for l, layer in zip(range(0, LEVELS - 1), range(13, 9, -1)):
             du = layer_back(layer, a[f"u{l}"], a[f"d{l}"], g)
             de[l] = du
-            g = r(upsample_backward(du))
-        # g is now the gradient of e4 from the decoder
+            # d_{l+1} (e4 at the last step) feeds the decoder and its own head
+            g = r(dh[l + 1] + upsample_backward(du))
+        # g is now the gradient of e4 from the decoder and its head
         for l in range(LEVELS - 1, 0, -1):
             gc = layer_back(2 * l + 1, a[f"c{l}"], a[f"e{l}"], g)
             gp = layer_back(2 * l, a[f"p{l}"], a[f"c{l}"], gc)

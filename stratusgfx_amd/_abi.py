@@ -36,11 +36,13 @@ SCENE_CITY = 1
 VPL_FLOATS = 16  # sgi_vpl is 16 floats (64 B)
 KPCN_CHANNELS = 32
 KPCN_LEVELS = 5
-KPCN_LAYERS = 15
+KPCN_LAYERS = 19
 KPCN_LAYER_WEIGHTS = 9 * 32 * 32
 KPCN_LAYER_PARAMS = KPCN_LAYER_WEIGHTS + 32
 KPCN_PARAMS = KPCN_LAYERS * KPCN_LAYER_PARAMS
 KPCN_TAPS = 25
+KPCN_ERR_CHANNEL = 25
+KPCN_BLEND_CHANNEL = 26
 KPCN_FEATURES = 10
 
 
@@ -166,10 +168,14 @@ SGI_SIGNATURES = {
     "sgi_kpcn_pool": (c_int32, [_P, _I, _I, _P, _P]),
     "sgi_kpcn_pool_backward": (c_int32, [_P, _I, _I, _P, _I, _P]),
     "sgi_kpcn_upsample_add": (c_int32, [_P, _P, _I, _I, _P, _P]),
-    "sgi_kpcn_upsample_backward": (c_int32, [_P, _I, _I, _P, _P]),
+    "sgi_kpcn_upsample_backward": (c_int32, [_P, _I, _I, _P, _I, _P]),
+    "sgi_kpcn_pyramid_floats": (ctypes.c_size_t, [_I, _I]),
+    "sgi_kpcn_radiance_pyramid": (c_int32, [_P, _I, _I, _P, _P]),
+    "sgi_kpcn_apply_backward": (c_int32, [_P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
+    "sgi_kpcn_upsample_bilinear_backward": (c_int32, [_P, _I, _I, _P, _P]),
     "sgi_kpcn_features": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, _P]),
-    "sgi_kpcn_apply": (c_int32, [_P, _P, _I, _I, _P, _P, _P]),
-    "sgi_kpcn_loss": (c_int32, [_P, _P, _P, _I, _I, c_float, c_float, _P, _P, _P]),
+    "sgi_kpcn_apply": (c_int32, [_P, _P, _P, _I, _I, _P, _P, _P]),
+    "sgi_kpcn_loss": (c_int32, [_P, _P, _P, _I, _I, c_float, c_float, _P, _P, _P, _P]),
     "sgi_kpcn_adam": (c_int32, [_P, _P, _P, _P, c_float, c_float, c_float, c_float, _I, _P, _P, _P, _P]),
     "sgi_kpcn_pack_weights": (c_int32, [_P, _P, _P, _P, _P]),
     "sgi_kpcn_workspace_bytes": (ctypes.c_size_t, [_I, _I]),

@@ -373,13 +373,50 @@ int sgi_kpcn_upsample_add(const uint16_t* skip, const uint16_t* low, int32_t hei
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_upsample_add");
 }
 
-int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* dlow, void* stream) {
+int sgi_kpcn_upsample_backward(const uint16_t* dout, int32_t height, int32_t width, uint16_t* dlow, int32_t accumulate,
+                               void* stream) {
     int r = dims(height, width, "sgi_kpcn_upsample_backward");
     if (r != SGI_OK) return r;
     if (!dout || !dlow || !aligned16(dout) || !aligned16(dlow))
         return fail(SGI_E_INVALID, "sgi_kpcn_upsample_backward: NULL or unaligned buffer");
-    const hipError_t e = sgi_kpcn::upBackward(dout, height, width, dlow, (hipStream_t)stream);
+    const hipError_t e =
+        sgi_kpcn::upBackward(dout, height, width, dlow, accumulate ? 1 : 0, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_upsample_backward");
+}
+
+size_t sgi_kpcn_pyramid_floats(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0 || height > 16384 || width > 16384) return 0;
+    return sgi_kpcn::pyramidFloats(height, width);
+}
+
+int sgi_kpcn_radiance_pyramid(const float* radiance, int32_t height, int32_t width, float* pyramid, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_radiance_pyramid");
+    if (r != SGI_OK) return r;
+    if (!radiance || !pyramid) return fail(SGI_E_INVALID, "sgi_kpcn_radiance_pyramid: NULL buffer");
+    const hipError_t e = sgi_kpcn::pyramid(radiance, height, width, pyramid, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_radiance_pyramid");
+}
+
+int sgi_kpcn_apply_backward(const float* logits, const float* radiance, const float* coarse, const float* dout,
+                            const float* derr, int32_t height, int32_t width, uint16_t* dlogits, float* dup,
+                            void* stream) {
+    int r = dims(height, width, "sgi_kpcn_apply_backward");
+    if (r != SGI_OK) return r;
+    if (!logits || !radiance || !dout || !dlogits || !aligned16(logits) || !aligned16(dlogits))
+        return fail(SGI_E_INVALID, "sgi_kpcn_apply_backward: bad buffer");
+    if (coarse && !dup) return fail(SGI_E_INVALID, "sgi_kpcn_apply_backward: a coarser level needs dup");
+    const hipError_t e = sgi_kpcn::applyBackward(logits, radiance, coarse, dout, derr, height, width, dlogits, dup,
+                                                 (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_apply_backward");
+}
+
+int sgi_kpcn_upsample_bilinear_backward(const float* dup, int32_t height, int32_t width, float* dcoarse,
+                                        void* stream) {
+    int r = dims(height, width, "sgi_kpcn_upsample_bilinear_backward");
+    if (r != SGI_OK) return r;
+    if (!dup || !dcoarse) return fail(SGI_E_INVALID, "sgi_kpcn_upsample_bilinear_backward: NULL buffer");
+    const hipError_t e = sgi_kpcn::upBilinearBackward(dup, height, width, dcoarse, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_upsample_bilinear_backward");
 }
 
 int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* g, const float* reservoir, uint16_t* features, float* radiance,
@@ -395,23 +432,23 @@ int sgi_kpcn_features(sgi_ctx* ctx, const sgi_gbuffer* g, const float* reservoir
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_features");
 }
 
-int sgi_kpcn_apply(const float* logits, const float* radiance, int32_t height, int32_t width, float* out, float* err,
-                   void* stream) {
+int sgi_kpcn_apply(const float* logits, const float* radiance, const float* coarse, int32_t height, int32_t width,
+                   float* out, float* err, void* stream) {
     int r = dims(height, width, "sgi_kpcn_apply");
     if (r != SGI_OK) return r;
     if (!logits || !radiance || !out || !aligned16(logits)) return fail(SGI_E_INVALID, "sgi_kpcn_apply: bad buffer");
-    const hipError_t e = sgi_kpcn::apply(logits, radiance, height, width, out, err, (hipStream_t)stream);
+    const hipError_t e = sgi_kpcn::apply(logits, radiance, coarse, height, width, out, err, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_apply");
 }
 
-int sgi_kpcn_loss(const float* logits, const float* radiance, const float* target, int32_t height, int32_t width,
-                  float err_weight, float scale, uint16_t* dlogits, float* loss_sums, void* stream) {
+int sgi_kpcn_loss(const float* out, const float* logits, const float* target, int32_t height, int32_t width,
+                  float err_weight, float scale, float* dout, float* derr, float* loss_sums, void* stream) {
     int r = dims(height, width, "sgi_kpcn_loss");
     if (r != SGI_OK) return r;
-    if (!logits || !radiance || !target || !dlogits || !loss_sums || !aligned16(logits) || !aligned16(dlogits))
-        return fail(SGI_E_INVALID, "sgi_kpcn_loss: bad buffer");
-    const hipError_t e = sgi_kpcn::loss(logits, radiance, target, height, width, err_weight, scale, dlogits,
-                                        loss_sums, (hipStream_t)stream);
+    if (!out || !logits || !target || !dout || !derr || !loss_sums)
+        return fail(SGI_E_INVALID, "sgi_kpcn_loss: NULL buffer");
+    const hipError_t e = sgi_kpcn::loss(out, logits, target, height, width, err_weight, scale, dout, derr, loss_sums,
+                                        (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_loss");
 }
 
@@ -435,12 +472,14 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 namespace {
 
 // The inference workspace: per level l two activation planes (e_l and a
-// temporary for c_l / d_l), then the radiance, the filtered output, the error
-// logit and the tile flags.
+// temporary for c_l / d_l), then the features, the radiance pyramid, the
+// reconstruction of every level, the error logit and the tile flags.
 struct KpcnWs {
     uint16_t* act[SGI_KPCN_LEVELS][2];
     uint16_t* feat;
-    float *radiance, *out, *err;
+    float* rad[SGI_KPCN_LEVELS];  // the radiance pyramid (level 0 = the noisy radiance)
+    float* o[SGI_KPCN_LEVELS];    // the reconstruction per level (level 0 = the output)
+    float* err;
     uint8_t *flags, *dilated;
     int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
 };
@@ -465,15 +504,28 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     const size_t P = (size_t)H * W;
     const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
     char* feat = take(P * SGI_KPCN_CHANNELS * 2);
-    char* rad = take(P * 3 * 4);
-    char* out = take(P * 3 * 4);
+    char* rad0 = take(P * 3 * 4);
+    // levels 1..4 of the pyramid contiguous (the layout sgi_kpcn_radiance_pyramid writes)
+    char* pyr = take(sgi_kpcn::pyramidFloats(H, W) * 4);
+    char* outs[SGI_KPCN_LEVELS];
+    h = H, w = W;
+    for (int l = 0; l < SGI_KPCN_LEVELS; ++l) {
+        outs[l] = take((size_t)h * w * 3 * 4);
+        h = (h + 1) / 2;
+        w = (w + 1) / 2;
+    }
     char* err = take(P * 4);
     char* flags = take(tiles);
     char* dil = take(tiles);
     if (ws) {
         ws->feat = (uint16_t*)feat;
-        ws->radiance = (float*)rad;
-        ws->out = (float*)out;
+        ws->rad[0] = (float*)rad0;
+        size_t off = 0;
+        for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
+            ws->rad[l] = (float*)pyr + off;
+            off += (size_t)ws->h[l] * ws->w[l] * 3;
+        }
+        for (int l = 0; l < SGI_KPCN_LEVELS; ++l) ws->o[l] = (float*)outs[l];
         ws->err = (float*)err;
         ws->flags = (uint8_t*)flags;
         ws->dilated = (uint8_t*)dil;
@@ -521,7 +573,8 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
     // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.radiance, s));
+    KPCN_STEP(sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], s));
+    KPCN_STEP(sgi_kpcn::pyramid(ws.rad[0], H, W, ws.rad[1], s));
     KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
     KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][1], nullptr, Wl(1), Bl(1), 1, ws.h[0], ws.w[0], ws.act[0][0], 0, s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
@@ -530,20 +583,24 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
         KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
                                     ws.act[l][0], 0, s));
     }
-    // decoder: d_l = conv(e_l + up(d_{l+1})) with d_4 = e_4; d_l lands in act[l][1]
-    const uint16_t* low = ws.act[SGI_KPCN_LEVELS - 1][0];
-    for (int l = SGI_KPCN_LEVELS - 2, layer = 10; l >= 0; --l, ++layer) {
+    // decoder: d_l = conv(e_l + up(d_{l+1})) with d_4 = e_4; d_l lands in act[l][1]; each
+    // level's head runs the level's reconstruction step in its epilogue, coarse to fine
+    constexpr int L4 = SGI_KPCN_LEVELS - 1;
+    KPCN_STEP(sgi_kpcn::convApply(ws.act[L4][0], ws.h[L4], ws.w[L4], Wl(14 + L4), Bl(14 + L4), ws.rad[L4], nullptr,
+                                  ws.o[L4], nullptr, s));
+    const uint16_t* low = ws.act[L4][0];
+    for (int l = L4 - 1, layer = 10; l >= 0; --l, ++layer) {
         KPCN_STEP(sgi_kpcn::convUpAdd(ws.act[l][0], low, ws.h[l], ws.w[l], Wl(layer), Bl(layer), ws.act[l][1], s));
         low = ws.act[l][1];
+        KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wl(14 + l), Bl(14 + l), ws.rad[l], ws.o[l + 1],
+                                      ws.o[l], l == 0 ? ws.err : nullptr, s));
     }
-    // the last convolution and the 5x5 filter of the radiance in one kernel
-    KPCN_STEP(sgi_kpcn::convApply(ws.act[0][1], H, W, Wl(14), Bl(14), ws.radiance, ws.out, ws.err, s));
     KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
     // a-trous over the flagged tiles (and their margin), the network elsewhere; without a
     // threshold no tile can be flagged, and only the merge pass runs (for its normals)
     KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
                                          ws.dilated, std::isinf(threshold) && threshold > 0.0f, s));
-    KPCN_STEP(sgi_kpcn::select(ws.out, ws.flags, H, W, reservoirOut, s));
+    KPCN_STEP(sgi_kpcn::select(ws.o[0], ws.flags, H, W, reservoirOut, s));
     KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
                                           combined, giOut, historyOut, ctx->ws, s));
 #undef KPCN_STEP

@@ -159,15 +159,57 @@ __device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int 
     return o;
 }
 
+// One axis of 2x bilinear upsampling: fine pixel x samples the coarser grid
+// (n pixels) at (x + 0.5) / 2 - 0.5, i.e. coarse pixels (x - 1) / 2 and
+// (x + 1) / 2 with weights 1/4, 3/4 (x even) or 3/4, 1/4 (x odd), clamped.
+__device__ __forceinline__ void bilinearAxis(int x, int n, int& i0, int& i1, float& w0, float& w1) {
+    const int k = x >> 1;
+    const bool odd = (x & 1) != 0;
+    const int a = odd ? k : k - 1;
+    w0 = odd ? 0.75f : 0.25f;
+    w1 = 1.0f - w0;
+    i0 = min(max(a, 0), n - 1);
+    i1 = min(max(a + 1, 0), n - 1);
+}
+
+// up(coarse) at fine pixel (x, y); coarse is [h][w][3] with h, w the coarser level's size.
+__device__ __forceinline__ f3 upBilinear(const float* __restrict__ coarse, int h, int w, int x, int y) {
+    int y0, y1, x0, x1;
+    float wy0, wy1, wx0, wx1;
+    bilinearAxis(y, h, y0, y1, wy0, wy1);
+    bilinearAxis(x, w, x0, x1, wx0, wx1);
+    const f3 top = ld3(coarse + 3 * ((size_t)y0 * w + x0)) * wx0 + ld3(coarse + 3 * ((size_t)y0 * w + x1)) * wx1;
+    const f3 bot = ld3(coarse + 3 * ((size_t)y1 * w + x0)) * wx0 + ld3(coarse + 3 * ((size_t)y1 * w + x1)) * wx1;
+    return top * wy0 + bot * wy1;
+}
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// One level of the reconstruction at pixel (x, y): the 5x5 softmax filter of
+// this level's radiance, blended with the upsampled coarser output when given.
+__device__ __forceinline__ f3 composePixel(const float* lg, const float* __restrict__ radiance,
+                                           const float* __restrict__ coarse, int W, int H, int x, int y) {
+    float wgt[SGI_KPCN_TAPS];
+    softmax25(lg, wgt);
+    f3 o = filtered(radiance, W, H, x, y, wgt);
+    if (coarse) {
+        const float b = sigmoidf(lg[SGI_KPCN_BLEND_CHANNEL]);
+        o = o + (upBilinear(coarse, (H + 1) / 2, (W + 1) / 2, x, y) - o) * b;
+    }
+    return o;
+}
+
 // Where the convolution's input comes from (staged into the LDS halo):
 //   IN_PLAIN     a [h][w][32] (gated by mask when given);
 //   IN_UPADD     a + up(b): the skip tensor plus the coarser level (nearest);
 //   IN_POOL      2x2 mean of a at the finer level (fineH x fineW).
 // and where the result goes:
 //   OUT_BF16 / OUT_F32  [h][w][32] with bias (and ReLU for BF16 when relu != 0);
-//   OUT_APPLY           the 26 logits stay in LDS and the epilogue applies the
-//                       5x5 softmax filter to the radiance (out [h][w][3]) and
-//                       writes the error logit (err [h][w]).
+//   OUT_APPLY           the 27 used logits stay in LDS and the epilogue runs
+//                       one level of the reconstruction (composePixel: the
+//                       5x5 softmax filter of the level's radiance, blended
+//                       with the coarser output when given; out [h][w][3])
+//                       and writes the error logit (err [h][w]).
 // Each fused form computes exactly what the separate kernels compute.
 enum { IN_PLAIN = 0, IN_UPADD = 1, IN_POOL = 2 };
 enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2 };
@@ -177,7 +219,8 @@ struct ConvSrc {
     const uint16_t* b;
     const uint16_t* mask;
     int fineH, fineW;
-    const float* radiance;  // OUT_APPLY
+    const float* radiance;  // OUT_APPLY: this level's radiance
+    const float* coarse;    // OUT_APPLY: the coarser level's output, or NULL
 };
 
 __device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
@@ -239,7 +282,7 @@ __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int t
     }
 }
 
-constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (26 used)
+constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (channels 0..26)
 constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
                                                                              : kTS * kTS * kLogitStride / 4 + 1;
 
@@ -307,23 +350,22 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int co = mb * 16 + 4 * lg + r;
-                        if (co <= SGI_KPCN_TAPS)
+                        if (co < kLogitStride)
                             logits[((wave * 4 + nb) * kTS + l16) * kLogitStride + co] = acc[mb][nb][r] + bv[mb][r];
                     }
             __syncthreads();
             const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
             if (x < W && y < H) {
-                float lgv[SGI_KPCN_TAPS + 1], wgt[SGI_KPCN_TAPS];
+                float lgv[kLogitStride];
 #pragma unroll
-                for (int j = 0; j <= SGI_KPCN_TAPS; ++j) lgv[j] = logits[threadIdx.x * kLogitStride + j];
-                softmax25(lgv, wgt);
-                const f3 o = filtered(src.radiance, W, H, x, y, wgt);
+                for (int j = 0; j < kLogitStride; ++j) lgv[j] = logits[threadIdx.x * kLogitStride + j];
+                const f3 o = composePixel(lgv, src.radiance, src.coarse, W, H, x, y);
                 const size_t p = (size_t)y * W + x;
                 float* dst = static_cast<float*>(out) + 3 * p;
                 dst[0] = o.x;
                 dst[1] = o.y;
                 dst[2] = o.z;
-                if (err) err[p] = lgv[SGI_KPCN_TAPS];
+                if (err) err[p] = lgv[SGI_KPCN_ERR_CHANNEL];
             }
         } else {
             const int x = tx * kTS + l16;
@@ -539,8 +581,9 @@ __global__ void upAddKernel(const uint16_t* __restrict__ skip, const uint16_t* _
     *reinterpret_cast<uint4*>(out + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// dlow[Y][X] = sum of the in-range dout children
-__global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W, uint16_t* __restrict__ dlow) {
+// dlow[Y][X] (+)= sum of the in-range dout children
+__global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W, uint16_t* __restrict__ dlow,
+                                 int accumulate) {
     const int h = (H + 1) / 2, w = (W + 1) / 2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h * w * 4) return;
@@ -559,75 +602,201 @@ __global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W
                 s[2 * j + 1] += bf2f((uint16_t)(vv[j] >> 16));
             }
         }
+    uint16_t* dst = dlow + (size_t)p * kC + c * 8;
+    if (accumulate) {
+        const uint4 prev = *reinterpret_cast<const uint4*>(dst);
+        const uint32_t pp[4] = {prev.x, prev.y, prev.z, prev.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s[2 * j] = bf2f((uint16_t)(pp[j] & 0xffffu)) + s[2 * j];
+            s[2 * j + 1] = bf2f((uint16_t)(pp[j] >> 16)) + s[2 * j + 1];
+        }
+    }
     uint32_t o[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(s[2 * j]) | ((uint32_t)f2bf(s[2 * j + 1]) << 16);
-    *reinterpret_cast<uint4*>(dlow + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<uint4*>(dst) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-__global__ void applyKernel(const float* __restrict__ logits, const float* __restrict__ radiance, int H, int W,
-                            float* __restrict__ out, float* __restrict__ err) {
+__global__ void applyKernel(const float* __restrict__ logits, const float* __restrict__ radiance,
+                            const float* __restrict__ coarse, int H, int W, float* __restrict__ out,
+                            float* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int x = i % W, y = i / W;
+    float lg[28];
+    loadLogits(logits, i, lg);
+    const f3 o = composePixel(lg, radiance, coarse, W, H, x, y);
+    out[3 * i + 0] = o.x;
+    out[3 * i + 1] = o.y;
+    out[3 * i + 2] = o.z;
+    if (err) err[i] = lg[SGI_KPCN_ERR_CHANNEL];
+}
+
+// Backward of one reconstruction level at pixel i: the logits' gradient
+// (filter taps through the softmax, the blend through the sigmoid, the error
+// channel from derr) and the gradient of up(coarse).
+__global__ void applyBackwardKernel(const float* __restrict__ logits, const float* __restrict__ radiance,
+                                    const float* __restrict__ coarse, const float* __restrict__ dout,
+                                    const float* __restrict__ derr, int H, int W, uint16_t* __restrict__ dlogits,
+                                    float* __restrict__ dup) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
     const int x = i % W, y = i / W;
     float lg[28], wgt[SGI_KPCN_TAPS];
     loadLogits(logits, i, lg);
     softmax25(lg, wgt);
-    const f3 o = filtered(radiance, W, H, x, y, wgt);
-    out[3 * i + 0] = o.x;
-    out[3 * i + 1] = o.y;
-    out[3 * i + 2] = o.z;
-    if (err) err[i] = lg[SGI_KPCN_TAPS];
+    const f3 g = ld3(dout + 3 * (size_t)i);
+    f3 df = g;
+    float dl[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) dl[j] = 0.0f;
+    if (coarse) {
+        const f3 f = filtered(radiance, W, H, x, y, wgt);
+        const f3 u = upBilinear(coarse, (H + 1) / 2, (W + 1) / 2, x, y);
+        const float b = sigmoidf(lg[SGI_KPCN_BLEND_CHANNEL]);
+        df = g * (1.0f - b);
+        const f3 du = g * b;
+        dup[3 * (size_t)i + 0] = du.x;
+        dup[3 * (size_t)i + 1] = du.y;
+        dup[3 * (size_t)i + 2] = du.z;
+        dl[SGI_KPCN_BLEND_CHANNEL] = dot(g, u - f) * b * (1.0f - b);
+    }
+    float dw[SGI_KPCN_TAPS];
+    float sdw = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
+        dw[j] = dot(df, ld3(radiance + 3 * ((size_t)yy * W + xx)));
+        sdw += wgt[j] * dw[j];
+    }
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) dl[j] = wgt[j] * (dw[j] - sdw);
+    if (derr) dl[SGI_KPCN_ERR_CHANNEL] = derr[i];
+    uint4* dst = reinterpret_cast<uint4*>(dlogits + (size_t)i * kC);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t o4[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o4[j] = (uint32_t)f2bf(dl[q * 8 + 2 * j]) | ((uint32_t)f2bf(dl[q * 8 + 2 * j + 1]) << 16);
+        dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
+    }
 }
 
-__global__ __launch_bounds__(256) void lossKernel(const float* __restrict__ logits, const float* __restrict__ radiance,
+// Transpose of up(): coarse pixel (X, Y) collects the fine pixels whose
+// bilinear taps land on it (fine rows / columns 2Y - 2 .. 2Y + 3 cover them
+// all, clamping included), each with its tap weight.
+__global__ void upBilinearBackwardKernel(const float* __restrict__ dup, int H, int W, float* __restrict__ dcoarse) {
+    const int h = (H + 1) / 2, w = (W + 1) / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h * w) return;
+    const int X = i % w, Y = i / w;
+    f3 acc = splat(0.0f);
+    for (int y = max(2 * Y - 2, 0); y <= min(2 * Y + 3, H - 1); ++y) {
+        int y0, y1;
+        float wy0, wy1;
+        bilinearAxis(y, h, y0, y1, wy0, wy1);
+        const float wy = (y0 == Y ? wy0 : 0.0f) + (y1 == Y ? wy1 : 0.0f);
+        if (wy == 0.0f) continue;
+        for (int x = max(2 * X - 2, 0); x <= min(2 * X + 3, W - 1); ++x) {
+            int x0, x1;
+            float wx0, wx1;
+            bilinearAxis(x, w, x0, x1, wx0, wx1);
+            const float wx = (x0 == X ? wx0 : 0.0f) + (x1 == X ? wx1 : 0.0f);
+            if (wx == 0.0f) continue;
+            acc = acc + ld3(dup + 3 * ((size_t)y * W + x)) * (wy * wx);
+        }
+    }
+    dcoarse[3 * (size_t)i + 0] = acc.x;
+    dcoarse[3 * (size_t)i + 1] = acc.y;
+    dcoarse[3 * (size_t)i + 2] = acc.z;
+}
+
+// The radiance pyramid, one 16x16 level-0 tile per block: the tile's 8x8,
+// 4x4, 2x2 and 1x1 pixels of levels 1..4 (a level-l pixel's children all lie
+// in the same tile), each the mean of its in-range children in the order
+// (0,0), (0,1), (1,0), (1,1) as poolKernel sums.
+__global__ __launch_bounds__(256) void pyramidKernel(const float* __restrict__ r0, int H, int W, int tilesX,
+                                                     float* __restrict__ pyr) {
+    __shared__ float buf[2][kTS * kTS * 3];
+    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    {
+        const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
+        if (x < W && y < H) {
+            const f3 v = ld3(r0 + 3 * ((size_t)y * W + x));
+            buf[0][3 * threadIdx.x + 0] = v.x;
+            buf[0][3 * threadIdx.x + 1] = v.y;
+            buf[0][3 * threadIdx.x + 2] = v.z;
+        }
+    }
+    int hp = H, wp = W;  // the finer level's size
+    size_t off = 0;      // this level's offset in pyr (floats)
+    int side = kTS;      // the finer level's tile edge
+    for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
+        __syncthreads();
+        const int h = (hp + 1) / 2, w = (wp + 1) / 2, s = side / 2;
+        const float* src = buf[(l - 1) & 1];
+        float* dst = buf[l & 1];
+        if ((int)threadIdx.x < s * s) {
+            const int lx = (int)threadIdx.x % s, ly = (int)threadIdx.x / s;
+            const int X = tx * s + lx, Y = ty * s + ly;
+            if (X < w && Y < h) {
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+                int n = 0;
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        if (2 * Y + dy >= hp || 2 * X + dx >= wp) continue;
+                        const int c = (2 * ly + dy) * side + 2 * lx + dx;
+                        a0 += src[3 * c + 0];
+                        a1 += src[3 * c + 1];
+                        a2 += src[3 * c + 2];
+                        ++n;
+                    }
+                const float inv = 1.0f / (float)n;
+                const int d = ly * s + lx;
+                dst[3 * d + 0] = a0 * inv;
+                dst[3 * d + 1] = a1 * inv;
+                dst[3 * d + 2] = a2 * inv;
+                float* o = pyr + off + 3 * ((size_t)Y * w + X);
+                o[0] = a0 * inv;
+                o[1] = a1 * inv;
+                o[2] = a2 * inv;
+            }
+        }
+        off += (size_t)h * w * 3;
+        hp = h;
+        wp = w;
+        side = s;
+    }
+}
+
+// Loss on the reconstruction: the relative squared error of out and the error
+// head's regression onto log(r_p + 1e-4); gradients of out and of the error
+// logit, times scale.
+__global__ __launch_bounds__(256) void lossKernel(const float* __restrict__ out, const float* __restrict__ logits,
                                                   const float* __restrict__ target, int H, int W, float errWeight,
-                                                  float scale, uint16_t* __restrict__ dlogits,
+                                                  float scale, float* __restrict__ dout, float* __restrict__ derr,
                                                   float* __restrict__ lossSums) {
     __shared__ float red[2][256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float l0 = 0.0f, l1 = 0.0f;
     if (i < W * H) {
-        const int x = i % W, y = i / W;
         const float invN = 1.0f / (3.0f * (float)W * (float)H), invP = 1.0f / ((float)W * (float)H);
-        float lg[28], wgt[SGI_KPCN_TAPS];
-        loadLogits(logits, i, lg);
-        softmax25(lg, wgt);
-        const f3 o = filtered(radiance, W, H, x, y, wgt);
-        const f3 t = ld3(target + 3 * i);
+        const f3 o = ld3(out + 3 * (size_t)i);
+        const f3 t = ld3(target + 3 * (size_t)i);
         const f3 den = f3{t.x * t.x + 0.01f, t.y * t.y + 0.01f, t.z * t.z + 0.01f};
         const f3 e = o - t;
         const float rx = e.x * e.x / den.x, ry = e.y * e.y / den.y, rz = e.z * e.z / den.z;
         l0 = rx + ry + rz;
         const float rp = l0 * (1.0f / 3.0f);
-        const float errT = logf(rp + 1e-4f);
-        const float de = lg[SGI_KPCN_TAPS] - errT;
+        const float de = logits[(size_t)i * kC + SGI_KPCN_ERR_CHANNEL] - logf(rp + 1e-4f);
         l1 = de * de;
-        // d loss / d out_c, then through the weights and the softmax
-        const f3 dOut = f3{2.0f * e.x / den.x, 2.0f * e.y / den.y, 2.0f * e.z / den.z} * (invN * scale);
-        float dw[SGI_KPCN_TAPS];
-        float sdw = 0.0f;
-#pragma unroll
-        for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
-            const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
-            dw[j] = dot(dOut, ld3(radiance + 3 * ((size_t)yy * W + xx)));
-            sdw += wgt[j] * dw[j];
-        }
-        float dl[32];
-#pragma unroll
-        for (int j = 0; j < 32; ++j) dl[j] = 0.0f;
-#pragma unroll
-        for (int j = 0; j < SGI_KPCN_TAPS; ++j) dl[j] = wgt[j] * (dw[j] - sdw);
-        dl[SGI_KPCN_TAPS] = 2.0f * errWeight * de * invP * scale;
-        uint4* dst = reinterpret_cast<uint4*>(dlogits + (size_t)i * kC);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t o4[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                o4[j] = (uint32_t)f2bf(dl[q * 8 + 2 * j]) | ((uint32_t)f2bf(dl[q * 8 + 2 * j + 1]) << 16);
-            dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
-        }
+        const f3 d = f3{2.0f * e.x / den.x, 2.0f * e.y / den.y, 2.0f * e.z / den.z} * (invN * scale);
+        dout[3 * (size_t)i + 0] = d.x;
+        dout[3 * (size_t)i + 1] = d.y;
+        dout[3 * (size_t)i + 2] = d.z;
+        derr[i] = 2.0f * errWeight * de * invP * scale;
     }
     red[0][threadIdx.x] = l0;
     red[1][threadIdx.x] = l1;
@@ -777,10 +946,11 @@ hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, co
 }
 
 hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
-                     float* out, float* err, hipStream_t s) {
+                     const float* coarse, float* out, float* err, hipStream_t s) {
     ConvSrc src{};
     src.a = in;
     src.radiance = radiance;
+    src.coarse = coarse;
     return launchConv<IN_PLAIN, OUT_APPLY>(src, w, bias, 0, H, W, out, err, s);
 }
 
@@ -818,9 +988,38 @@ hipError_t upAdd(const uint16_t* skip, const uint16_t* low, int H, int W, uint16
     return hipGetLastError();
 }
 
-hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, hipStream_t s) {
+hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, int accumulate, hipStream_t s) {
     const int n = ((H + 1) / 2) * ((W + 1) / 2) * 4;
-    hipLaunchKernelGGL(upBackwardKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, dout, H, W, dlow);
+    hipLaunchKernelGGL(upBackwardKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, dout, H, W, dlow, accumulate);
+    return hipGetLastError();
+}
+
+size_t pyramidFloats(int H, int W) {
+    size_t n = 0;
+    for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
+        H = (H + 1) / 2;
+        W = (W + 1) / 2;
+        n += (size_t)H * W * 3;
+    }
+    return n;
+}
+
+hipError_t pyramid(const float* radiance, int H, int W, float* pyr, hipStream_t s) {
+    const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
+    hipLaunchKernelGGL(pyramidKernel, dim3(nTiles), dim3(256), 0, s, radiance, H, W, tilesX, pyr);
+    return hipGetLastError();
+}
+
+hipError_t applyBackward(const float* logits, const float* radiance, const float* coarse, const float* dout,
+                         const float* derr, int H, int W, uint16_t* dlogits, float* dup, hipStream_t s) {
+    hipLaunchKernelGGL(applyBackwardKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, coarse,
+                       dout, derr, H, W, dlogits, dup);
+    return hipGetLastError();
+}
+
+hipError_t upBilinearBackward(const float* dup, int H, int W, float* dcoarse, hipStream_t s) {
+    const int n = ((H + 1) / 2) * ((W + 1) / 2);
+    hipLaunchKernelGGL(upBilinearBackwardKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, dup, H, W, dcoarse);
     return hipGetLastError();
 }
 
@@ -831,15 +1030,17 @@ hipError_t features(const GbufArgs& g, const FrameArgs& fr, const float* reservo
     return hipGetLastError();
 }
 
-hipError_t apply(const float* logits, const float* radiance, int H, int W, float* out, float* err, hipStream_t s) {
-    hipLaunchKernelGGL(applyKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, H, W, out, err);
+hipError_t apply(const float* logits, const float* radiance, const float* coarse, int H, int W, float* out,
+                 float* err, hipStream_t s) {
+    hipLaunchKernelGGL(applyKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, coarse, H, W, out,
+                       err);
     return hipGetLastError();
 }
 
-hipError_t loss(const float* logits, const float* radiance, const float* target, int H, int W, float errWeight,
-                float scale, uint16_t* dlogits, float* lossSums, hipStream_t s) {
-    hipLaunchKernelGGL(lossKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, logits, radiance, target, H, W,
-                       errWeight, scale, dlogits, lossSums);
+hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
+                float scale, float* dout, float* derr, float* lossSums, hipStream_t s) {
+    hipLaunchKernelGGL(lossKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, out, logits, target, H, W, errWeight,
+                       scale, dout, derr, lossSums);
     return hipGetLastError();
 }
 

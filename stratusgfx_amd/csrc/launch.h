@@ -80,26 +80,33 @@ namespace sgi_kpcn {
 hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
                    int W, void* out, int outF32, hipStream_t s);
 // Inference forms of the convolution with the neighbouring pass fused into the
-// input staging (pooling, upsample + skip) or the epilogue (the 5x5 filter);
-// same results as the separate kernels.
+// input staging (pooling, upsample + skip) or the epilogue (one reconstruction
+// level: the 5x5 filter and the blend with the coarser output); same results
+// as the separate kernels.
 hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
                     hipStream_t s);
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
                      uint16_t* out, hipStream_t s);
 hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
-                     float* out, float* err, hipStream_t s);
+                     const float* coarse, float* out, float* err, hipStream_t s);
 int wgradBlocks(int H, int W);
 hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,
                  float* grad, hipStream_t s);
 hipError_t pool(const uint16_t* in, int H, int W, uint16_t* out, hipStream_t s);
 hipError_t poolBackward(const uint16_t* dout, int H, int W, uint16_t* din, int accumulate, hipStream_t s);
 hipError_t upAdd(const uint16_t* skip, const uint16_t* low, int H, int W, uint16_t* out, hipStream_t s);
-hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, hipStream_t s);
+hipError_t upBackward(const uint16_t* dout, int H, int W, uint16_t* dlow, int accumulate, hipStream_t s);
+size_t pyramidFloats(int H, int W);
+hipError_t pyramid(const float* radiance, int H, int W, float* pyr, hipStream_t s);
+hipError_t applyBackward(const float* logits, const float* radiance, const float* coarse, const float* dout,
+                         const float* derr, int H, int W, uint16_t* dlogits, float* dup, hipStream_t s);
+hipError_t upBilinearBackward(const float* dup, int H, int W, float* dcoarse, hipStream_t s);
 hipError_t features(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir, uint16_t* feat,
                     float* radiance, hipStream_t s);
-hipError_t apply(const float* logits, const float* radiance, int H, int W, float* out, float* err, hipStream_t s);
-hipError_t loss(const float* logits, const float* radiance, const float* target, int H, int W, float errWeight,
-                float scale, uint16_t* dlogits, float* lossSums, hipStream_t s);
+hipError_t apply(const float* logits, const float* radiance, const float* coarse, int H, int W, float* out,
+                 float* err, hipStream_t s);
+hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
+                float scale, float* dout, float* derr, float* lossSums, hipStream_t s);
 hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
                 int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
 hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);

@@ -20,18 +20,20 @@ LAYERS = _abi.KPCN_LAYERS
 LAYER_WEIGHTS = _abi.KPCN_LAYER_WEIGHTS
 LAYER_PARAMS = _abi.KPCN_LAYER_PARAMS
 PARAMS = _abi.KPCN_PARAMS
+HEADS = tuple(14 + l for l in range(LEVELS))  # the filter head of level l reads d_l (e_4 at level 4)
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "kpcn_city.npy")
 
 
 def init_params(seed: int) -> np.ndarray:
-    """He-normal weights (fan-in 9 x 32), zero biases; the last layer starts at
-    1/10 of that scale so the first filters are close to a 5x5 box."""
+    """He-normal weights (fan-in 9 x 32), zero biases; the filter heads start at
+    1/10 of that scale so the first filters are close to a 5x5 box and every
+    blend close to 1/2."""
     rng = np.random.default_rng(seed)
     p = np.zeros(PARAMS, np.float32)
     for l in range(LAYERS):
-        std = np.sqrt(2.0 / (9 * C)) * (0.1 if l == LAYERS - 1 else 1.0)
+        std = np.sqrt(2.0 / (9 * C)) * (0.1 if l in HEADS else 1.0)
         p[l * LAYER_PARAMS:l * LAYER_PARAMS + LAYER_WEIGHTS] = rng.normal(0.0, std, LAYER_WEIGHTS)
     return p
 
@@ -102,6 +104,7 @@ class Trainer:
         self.H, self.W = height, width
         self.sizes = level_sizes(height, width)
         bf = dict(dtype=torch.bfloat16, device=device)
+        f32 = dict(dtype=torch.float32, device=device)
 
         def act(l):
             h, w = self.sizes[l]
@@ -116,16 +119,30 @@ class Trainer:
         # gradient buffers: two per level (the gradient of a tensor and of its producer's input)
         self.g = [[act(l), act(l)] for l in range(LEVELS)]
         self.gd = [act(l) for l in range(LEVELS)]  # decoder-side gradients kept for the skip sums
-        self.logits = torch.empty((height, width, C), dtype=torch.float32, device=device)
-        self.dlogits = torch.empty((height, width, C), **bf)
-        self.partials = torch.empty(int(self.lib.sgi_kpcn_wgrad_partials(height, width)), dtype=torch.float32,
-                                    device=device)
-        self.layer_grad = torch.empty(LAYER_PARAMS, dtype=torch.float32, device=device)
-        self.grad = torch.zeros(PARAMS, dtype=torch.float32, device=device)
-        self.m = torch.zeros(PARAMS, dtype=torch.float32, device=device)
-        self.v = torch.zeros(PARAMS, dtype=torch.float32, device=device)
-        self.loss_sums = torch.zeros(2, dtype=torch.float32, device=device)
+        # per level: head logits (FP32), their gradient (BF16), the radiance pyramid, the
+        # reconstruction and its gradient, and the gradient of the upsampled coarser output
+        self.logits = [torch.empty((h, w, C), **f32) for h, w in self.sizes]
+        self.dlogits = [torch.empty((h, w, C), **bf) for h, w in self.sizes]
+        self.pyr = torch.empty(int(self.lib.sgi_kpcn_pyramid_floats(height, width)), **f32)
+        self.rad = [None] + self._split(self.pyr)
+        self.o = [torch.empty((h, w, 3), **f32) for h, w in self.sizes]
+        self.do = [torch.empty((h, w, 3), **f32) for h, w in self.sizes]
+        self.dup = [torch.empty((h, w, 3), **f32) for h, w in self.sizes]
+        self.derr = torch.empty((height, width), **f32)
+        self.partials = torch.empty(int(self.lib.sgi_kpcn_wgrad_partials(height, width)), **f32)
+        self.layer_grad = torch.empty(LAYER_PARAMS, **f32)
+        self.grad = torch.zeros(PARAMS, **f32)
+        self.m = torch.zeros(PARAMS, **f32)
+        self.v = torch.zeros(PARAMS, **f32)
+        self.loss_sums = torch.zeros(2, **f32)
         self.step_count = 0
+
+    def _split(self, pyr):
+        out, off = [], 0
+        for h, w in self.sizes[1:]:
+            out.append(pyr[off:off + h * w * 3].view(h, w, 3))
+            off += h * w * 3
+        return out
 
     # -- forward -----------------------------------------------------------
     def _conv(self, l, x, out, level, relu=True, out_f32=False, stream=0):
@@ -133,9 +150,14 @@ class Trainer:
         _abi.call(self.lib, "sgi_kpcn_conv3x3", _p(x), 0, _p(self.wts.layer_w(l)), _p(self.wts.layer_b(l)),
                   1 if relu else 0, h, w, _p(out), 1 if out_f32 else 0, stream)
 
-    def forward(self, feat, stream=0):
+    def head_input(self, l):
+        return self.a[f"d{l}"] if l < LEVELS - 1 else self.a[f"e{LEVELS - 1}"]
+
+    def forward(self, feat, radiance, stream=0):
+        """The network and the multi-scale reconstruction; returns the denoised radiance o_0."""
         a = self.a
         self.feat = feat
+        self.rad[0] = radiance
         self._conv(0, feat, a["c0"], 0, stream=stream)
         self._conv(1, a["c0"], a["e0"], 0, stream=stream)
         for l in range(1, LEVELS):
@@ -149,8 +171,15 @@ class Trainer:
             _abi.call(self.lib, "sgi_kpcn_upsample_add", _p(a[f"e{l}"]), _p(low), h, w, _p(a[f"u{l}"]), stream)
             self._conv(layer, a[f"u{l}"], a[f"d{l}"], l, stream=stream)
             low = a[f"d{l}"]
-        self._conv(14, a["d0"], self.logits, 0, relu=False, out_f32=True, stream=stream)
-        return self.logits
+        for l in range(LEVELS):
+            self._conv(HEADS[l], self.head_input(l), self.logits[l], l, relu=False, out_f32=True, stream=stream)
+        _abi.call(self.lib, "sgi_kpcn_radiance_pyramid", _p(radiance), self.H, self.W, _p(self.pyr), stream)
+        for l in range(LEVELS - 1, -1, -1):
+            h, w = self.sizes[l]
+            coarse = self.o[l + 1] if l + 1 < LEVELS else None
+            _abi.call(self.lib, "sgi_kpcn_apply", _p(self.logits[l]), _p(self.rad[l]), _p(coarse), h, w,
+                      _p(self.o[l]), 0, stream)
+        return self.o[0]
 
     # -- backward ------------------------------------------------------------
     def _layer_back(self, l, x, y, dy, dx_out, level, need_dx=True, stream=0):
@@ -163,20 +192,32 @@ class Trainer:
             _abi.call(self.lib, "sgi_kpcn_conv3x3", _p(dy), _p(y), _p(self.wts.layer_wt(l)), 0, 0, h, w,
                       _p(dx_out), 0, stream)
 
-    def loss_backward(self, radiance, target, err_weight=0.05, scale=1.0, stream=0):
+    def loss_backward(self, target, err_weight=0.05, scale=1.0, stream=0):
         """Loss of the last forward against target and the gradient of every layer (added to grad)."""
         a, g = self.a, self.g
-        _abi.call(self.lib, "sgi_kpcn_loss", _p(self.logits), _p(radiance), _p(target), self.H, self.W,
-                  ctypes.c_float(err_weight), ctypes.c_float(scale), _p(self.dlogits), _p(self.loss_sums), stream)
-        # layer 14 (no ReLU): mask = NULL
-        self._layer_back(14, a["d0"], None, self.dlogits, g[0][0], 0, stream=stream)
+        _abi.call(self.lib, "sgi_kpcn_loss", _p(self.o[0]), _p(self.logits[0]), _p(target), self.H, self.W,
+                  ctypes.c_float(err_weight), ctypes.c_float(scale), _p(self.do[0]), _p(self.derr),
+                  _p(self.loss_sums), stream)
+        # the reconstruction, fine to coarse: each level's logits and the coarser output's gradient
+        for l in range(LEVELS):
+            h, w = self.sizes[l]
+            coarse = self.o[l + 1] if l + 1 < LEVELS else None
+            _abi.call(self.lib, "sgi_kpcn_apply_backward", _p(self.logits[l]), _p(self.rad[l]), _p(coarse),
+                      _p(self.do[l]), _p(self.derr) if l == 0 else 0, h, w, _p(self.dlogits[l]),
+                      _p(self.dup[l]) if coarse is not None else 0, stream)
+            if coarse is not None:
+                _abi.call(self.lib, "sgi_kpcn_upsample_bilinear_backward", _p(self.dup[l]), h, w,
+                          _p(self.do[l + 1]), stream)
+        # the heads (no ReLU): d_l's gradient from its head lands in g[l][0]
+        for l in range(LEVELS):
+            self._layer_back(HEADS[l], self.head_input(l), None, self.dlogits[l], g[l][0], l, stream=stream)
         cur = g[0][0]  # gradient of d0
         for l, layer in zip(range(0, LEVELS - 1), range(13, 9, -1)):
             # gradient of u_l, which is also the skip gradient of e_l
             self._layer_back(layer, a[f"u{l}"], a[f"d{l}"], cur, self.gd[l], l, stream=stream)
             h, w = self.sizes[l]
-            nxt = g[l + 1][0]
-            _abi.call(self.lib, "sgi_kpcn_upsample_backward", _p(self.gd[l]), h, w, _p(nxt), stream)
+            nxt = g[l + 1][0]  # holds d_{l+1}'s (e4's) head gradient; add the decoder's
+            _abi.call(self.lib, "sgi_kpcn_upsample_backward", _p(self.gd[l]), h, w, _p(nxt), 1, stream)
             cur = nxt  # gradient of d_{l+1} (or e4 when l + 1 == 4)
         for l in range(LEVELS - 1, 0, -1):
             # cur = total gradient of e_l

@@ -1,8 +1,8 @@
 """Seeded synthetic inputs for the GI pass (wraps include/sgi_scene.h).
 
 Two scenes, as the benchmark configurations describe them:
-  * the Cornell box (C0: 640x360, 64 VPLs, 6x64x64 FP32 cubes, 1024^2 cascades);
-  * the procedural city block (C1: 1920x1080, ground + 2000 boxes, 1024 VPLs,
+  * the Cornell box (640x360, 64 VPLs, 6x64x64 FP32 cubes, 1024^2 cascades);
+  * the procedural city block (1920x1080, ground + 2000 boxes, 1024 VPLs,
     6x256x256 FP16 cubes, up to 64 VPLs per 16x16 tile).
 Inputs are produced either on the host (numpy; the host ray caster) or on the
 device (torch tensors; the HIP ray caster) from the same per-texel code.

@@ -190,8 +190,8 @@ def main(argv=None) -> None:
         tr.zero_grad()
         for j in rng.choice(len(train), a.batch, replace=False):
             s = train[j]
-            tr.forward(s.feat)
-            tr.loss_backward(s.rad, s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch)
+            tr.forward(s.feat, s.rad)
+            tr.loss_backward(s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch)
         tr.step(lr)
         if step % 50 == 0 or step == 1:
             ls = tr.loss_sums.cpu().numpy() / (a.batch * a.size * a.size)

@@ -39,11 +39,21 @@ STRUCTS = {"sgi_vpl": None, "sgi_atlas_entry": None, "sgi_tile_stage1": None, "s
            "sgi_views_config": _abi.ViewsConfig, "sgi_view_io": _abi.ViewIO}
 
 
+CONSTANTS = {"SGI_KPCN_CHANNELS": _abi.KPCN_CHANNELS, "SGI_KPCN_LEVELS": _abi.KPCN_LEVELS,
+             "SGI_KPCN_LAYERS": _abi.KPCN_LAYERS, "SGI_KPCN_LAYER_PARAMS": _abi.KPCN_LAYER_PARAMS,
+             "SGI_KPCN_PARAMS": _abi.KPCN_PARAMS, "SGI_KPCN_TAPS": _abi.KPCN_TAPS,
+             "SGI_KPCN_ERR_CHANNEL": _abi.KPCN_ERR_CHANNEL, "SGI_KPCN_BLEND_CHANNEL": _abi.KPCN_BLEND_CHANNEL,
+             "SGI_KPCN_FEATURES": _abi.KPCN_FEATURES}
+
+
 def test_struct_layouts_match_c():
-    """sizeof of every public struct as gcc lays it out vs the ctypes mirror and the reference sizes."""
+    """sizeof of every public struct as gcc lays it out vs the ctypes mirror and the reference sizes;
+    the learned denoiser's constants vs the mirror's."""
     src = '#include <stdio.h>\n#include "sgi_scene.h"\n#include "sgi_kpcn.h"\nint main(void){\n'
     for name in STRUCTS:
         src += f'printf("{name} %zu\\n", sizeof({name}));\n'
+    for name in CONSTANTS:
+        src += f'printf("{name} %d\\n", (int)({name}));\n'
     src += "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "s.c")
@@ -59,6 +69,8 @@ def test_struct_layouts_match_c():
     for name, cls in STRUCTS.items():
         if cls is not None:
             assert ctypes.sizeof(cls) == int(sizes[name]), name
+    for name, value in CONSTANTS.items():
+        assert int(sizes[name]) == value, name
 
 
 def test_create_validates_config_before_touching_the_device():

@@ -1,11 +1,12 @@
 """The learned denoiser's HIP kernels (MFMA convolutions, weight gradients,
-pooling, filter, loss, Adam) and the whole network's gradients against the
-CPU oracle (oracle/kpcn_oracle.py, pinned by finite differences).
+pooling, radiance pyramid, multi-scale filter reconstruction and its
+backward, loss, Adam) and the whole network's gradients against the CPU
+oracle (oracle/kpcn_oracle.py, pinned by finite differences).
 
 Bounds: BF16-stored outputs within one BF16 rounding of the oracle evaluated
 on the same BF16 operands (|d| <= 2^-7 |ref| + 1e-6); FP32 weight gradients
 within rel-L2 1e-5 (exact BF16 products, FP32 sums in another order); the
-whole network's gradients, which pass through 15 layers of BF16 activations
+whole network's gradients, which pass through 19 layers of BF16 activations
 and gradients (a rounding that lands on the other side of a tie in one layer
 moves every later layer's gradient), closer to the BF16-emulating oracle than
 BF16 rounding itself moves them (the oracle's BF16 vs FP64 distance, 2e-3 to
@@ -124,7 +125,9 @@ def test_pool_and_upsample(lib):
     up = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
     _abi.call(lib, "sgi_kpcn_upsample_add", xd.data_ptr(), lowd.data_ptr(), H, W, up.data_ptr(), 0)
     dlow = torch.empty((h, w, 32), dtype=torch.bfloat16, device="cuda")
-    _abi.call(lib, "sgi_kpcn_upsample_backward", xd.data_ptr(), H, W, dlow.data_ptr(), 0)
+    _abi.call(lib, "sgi_kpcn_upsample_backward", xd.data_ptr(), H, W, dlow.data_ptr(), 0, 0)
+    dlow_acc = lowd.clone()
+    _abi.call(lib, "sgi_kpcn_upsample_backward", xd.data_ptr(), H, W, dlow_acc.data_ptr(), 1, 0)
     din = _bf(torch, prev)
     _abi.call(lib, "sgi_kpcn_pool_backward", lowd.data_ptr(), H, W, din.data_ptr(), 1, 0)
     din0 = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
@@ -133,34 +136,94 @@ def test_pool_and_upsample(lib):
     _close_bf16(_np(out), ko.to_bf16(ko.pool(x)), "pool")
     _close_bf16(_np(up), ko.to_bf16(x + ko.upsample(low, H, W)), "upsample_add")
     _close_bf16(_np(dlow), ko.to_bf16(ko.upsample_backward(x)), "upsample_backward")
+    _close_bf16(_np(dlow_acc), ko.to_bf16(low + ko.upsample_backward(x)), "upsample_backward accumulate")
     _close_bf16(_np(din), ko.to_bf16(prev + ko.pool_backward(low, H, W)), "pool_backward accumulate")
     _close_bf16(_np(din0), ko.to_bf16(ko.pool_backward(low, H, W)), "pool_backward")
 
 
-def test_filter_apply_and_loss(lib):
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def test_radiance_pyramid(lib):
+    """Levels 1..4 of the 2x2-mean pyramid (partial tiles, odd level sizes)."""
+    torch = require_gpu()
+    rng = np.random.default_rng(10)
+    for hh, ww in ((H, W), (16, 16), (70, 33)):
+        rad = rng.uniform(0, 3, (hh, ww, 3)).astype(np.float32)
+        n = int(lib.sgi_kpcn_pyramid_floats(hh, ww))
+        want = ko.pyramid(rad.astype(np.float64))[1:]
+        assert n == sum(p.size for p in want)
+        rd = _dev(torch, rad)
+        pyr = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+        _abi.call(lib, "sgi_kpcn_radiance_pyramid", rd.data_ptr(), hh, ww, pyr.data_ptr(), 0)
+        torch.cuda.synchronize()
+        got = pyr.cpu().numpy().astype(np.float64)
+        off = 0
+        for l, p in enumerate(want, 1):
+            g = got[off:off + p.size].reshape(p.shape)
+            off += p.size
+            assert np.abs(g - p).max() <= 1e-6 * np.abs(p).max(), (hh, ww, l)
+
+
+def test_filter_apply_levels_and_backward(lib):
+    """One reconstruction level (filter alone, filter blended with the upsampled
+    coarser output), its backward and the bilinear transpose against the oracle."""
     torch = require_gpu()
     rng = np.random.default_rng(4)
+    h, w = (H + 1) // 2, (W + 1) // 2
     logits = rng.normal(0, 2, (H, W, 32)).astype(np.float32)
     rad = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    coarse = rng.uniform(0, 2, (h, w, 3)).astype(np.float32)
+    dout = rng.normal(0, 1, (H, W, 3)).astype(np.float32)
+    derr = rng.normal(0, 1, (H, W)).astype(np.float32)
+    ld, rd, cd, dd, ed = (_dev(torch, a) for a in (logits, rad, coarse, dout, derr))
+    f64 = lambda a: a.astype(np.float64)
+    for with_coarse in (False, True):
+        out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+        err = torch.empty((H, W), dtype=torch.float32, device="cuda")
+        c_ptr = cd.data_ptr() if with_coarse else 0
+        _abi.call(lib, "sgi_kpcn_apply", ld.data_ptr(), rd.data_ptr(), c_ptr, H, W, out.data_ptr(), err.data_ptr(), 0)
+        dl = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+        dup = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+        _abi.call(lib, "sgi_kpcn_apply_backward", ld.data_ptr(), rd.data_ptr(), c_ptr, dd.data_ptr(),
+                  ed.data_ptr() if with_coarse else 0, H, W, dl.data_ptr(), dup.data_ptr() if with_coarse else 0, 0)
+        dc = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
+        if with_coarse:
+            _abi.call(lib, "sgi_kpcn_upsample_bilinear_backward", dup.data_ptr(), H, W, dc.data_ptr(), 0)
+        torch.cuda.synchronize()
+        c64 = f64(coarse) if with_coarse else None
+        o, e = ko.apply(f64(logits), f64(rad), c64)
+        assert np.abs(out.cpu().numpy() - o).max() <= 1e-5 * np.abs(o).max(), with_coarse
+        assert np.array_equal(err.cpu().numpy(), logits[..., ko.ERR])
+        dlo, dupo = ko.apply_backward(f64(logits), f64(rad), c64, f64(dout), f64(derr) if with_coarse else None)
+        _close_bf16(_np(dl), ko.to_bf16(dlo), f"dlogits coarse={with_coarse}")
+        if with_coarse:
+            assert np.abs(dup.cpu().numpy() - dupo).max() <= 1e-6 * np.abs(dupo).max()
+            dco = ko.upsample_bilinear_backward(dup.cpu().numpy().astype(np.float64))
+            assert np.abs(dc.cpu().numpy() - dco).max() <= 1e-5 * np.abs(dco).max()
+
+
+def test_loss(lib):
+    torch = require_gpu()
+    rng = np.random.default_rng(11)
+    out = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
+    logits = rng.normal(0, 2, (H, W, 32)).astype(np.float32)
     target = rng.uniform(0.1, 1.5, (H, W, 3)).astype(np.float32)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    ld, rd, td = t(logits), t(rad), t(target)
-    out = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
-    err = torch.empty((H, W), dtype=torch.float32, device="cuda")
-    _abi.call(lib, "sgi_kpcn_apply", ld.data_ptr(), rd.data_ptr(), H, W, out.data_ptr(), err.data_ptr(), 0)
-    dl = torch.empty((H, W, 32), dtype=torch.bfloat16, device="cuda")
+    od, ld, td = (_dev(torch, a) for a in (out, logits, target))
+    dout = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    derr = torch.empty((H, W), dtype=torch.float32, device="cuda")
     sums = torch.zeros(2, dtype=torch.float32, device="cuda")
-    _abi.call(lib, "sgi_kpcn_loss", ld.data_ptr(), rd.data_ptr(), td.data_ptr(), H, W, ctypes.c_float(0.25),
-              ctypes.c_float(0.5), dl.data_ptr(), sums.data_ptr(), 0)
+    _abi.call(lib, "sgi_kpcn_loss", od.data_ptr(), ld.data_ptr(), td.data_ptr(), H, W, ctypes.c_float(0.25),
+              ctypes.c_float(0.5), dout.data_ptr(), derr.data_ptr(), sums.data_ptr(), 0)
     torch.cuda.synchronize()
-    o, e = ko.apply(logits.astype(np.float64), rad.astype(np.float64))
-    assert np.abs(out.cpu().numpy() - o).max() <= 1e-5 * np.abs(o).max()
-    assert np.array_equal(err.cpu().numpy(), logits[..., ko.TAPS])
-    L, dlo = ko.loss(logits.astype(np.float64), rad.astype(np.float64), target.astype(np.float64), 0.25)
+    L, do, de = ko.loss(out.astype(np.float64), logits[..., ko.ERR].astype(np.float64), target.astype(np.float64),
+                        0.25)
     s = sums.cpu().numpy().astype(np.float64)
     P = H * W
     assert abs(s[0] / (3 * P) + 0.25 * s[1] / P - L) <= 1e-4 * abs(L)
-    _close_bf16(_np(dl), ko.to_bf16(0.5 * dlo), "dlogits")
+    assert np.abs(dout.cpu().numpy() - 0.5 * do).max() <= 1e-5 * np.abs(0.5 * do).max()
+    assert np.abs(derr.cpu().numpy() - 0.5 * de).max() <= 1e-4 * np.abs(0.5 * de).max()
 
 
 def test_adam_step(lib):
@@ -208,23 +271,32 @@ def test_network_gradients_match_oracle(lib):
     wts = kpcn.Weights(params)
     tr = kpcn.Trainer(wts, H, W)
     tr.zero_grad()
-    logits = tr.forward(_bf(torch, feat))
     rd = torch.from_numpy(np.ascontiguousarray(rad)).cuda()
     td = torch.from_numpy(np.ascontiguousarray(target)).cuda()
-    tr.loss_backward(rd, td, err_weight=0.1)
+    fd = _bf(torch, feat)
+    out = tr.forward(fd, rd)
+    tr.loss_backward(td, err_weight=0.1)
     torch.cuda.synchronize()
+    rad64, t64 = rad.astype(np.float64), target.astype(np.float64)
+
+    def oracle_grads(net):
+        logits = net.forward(feat)
+        o, err = ko.reconstruct(logits, rad64)
+        _, dout, derr = ko.loss(o[0], err, t64, 0.1)
+        return logits, o, net.backward(ko.reconstruct_backward(logits, rad64, o, dout, derr))
+
     net = ko.Net(params.astype(np.float64), bf16=True)
-    want_logits = net.forward(feat)
-    got_logits = logits.cpu().numpy().astype(np.float64)
-    e_log = np.linalg.norm(got_logits - want_logits) / np.linalg.norm(want_logits)
-    print(f"logits rel-L2 {e_log:.2e}")
-    assert e_log <= 1e-2
-    _, dl = ko.loss(want_logits, rad.astype(np.float64), target.astype(np.float64), 0.1)
-    want = net.backward(dl)
+    want_logits, want_o, want = oracle_grads(net)
+    for l in range(ko.LEVELS):
+        got_l = tr.logits[l].cpu().numpy().astype(np.float64)
+        e_log = np.linalg.norm(got_l - want_logits[l]) / np.linalg.norm(want_logits[l])
+        print(f"level {l} logits rel-L2 {e_log:.2e}")
+        assert e_log <= 1e-2, l
+    e_out = np.linalg.norm(out.cpu().numpy() - want_o[0]) / np.linalg.norm(want_o[0])
+    print(f"reconstruction rel-L2 {e_out:.2e}")
+    assert e_out <= 1e-3
     # the size of BF16 rounding itself: the same oracle in FP64
-    net64 = ko.Net(params.astype(np.float64), bf16=False)
-    _, dl64 = ko.loss(net64.forward(feat), rad.astype(np.float64), target.astype(np.float64), 0.1)
-    want64 = net64.backward(dl64)
+    _, _, want64 = oracle_grads(ko.Net(params.astype(np.float64), bf16=False))
     got = tr.grad.cpu().numpy().astype(np.float64).reshape(ko.LAYERS, ko.LAYER_PARAMS)
     bad = []
     for l in range(ko.LAYERS):
@@ -240,7 +312,8 @@ def test_network_gradients_match_oracle(lib):
 
 def test_denoise_learned_threshold_semantics(lib):
     """threshold <= 0: every tile falls back, so the outputs are sgi_denoise's bit for bit;
-    threshold = inf: no tile falls back and gi is the network's filtered radiance."""
+    threshold = inf: no tile falls back and gi is the network's reconstruction, bit for bit the one
+    the training path's separate kernels compute."""
     from gpu_util import Case
     from stratusgfx_amd import scene as sc
     from stratusgfx_amd.gi import GiPass
@@ -290,9 +363,7 @@ def test_denoise_learned_threshold_semantics(lib):
     rad = torch.empty((Hh, Ww, 3), **f32)
     gi.kpcn_features(g, res, feat, rad)
     tr = kpcn.Trainer(wts, Hh, Ww)
-    logits = tr.forward(feat)
-    out = torch.empty((Hh, Ww, 3), **f32)
-    _abi.call(lib, "sgi_kpcn_apply", logits.data_ptr(), rad.data_ptr(), Hh, Ww, out.data_ptr(), 0, 0)
+    out = tr.forward(feat, rad)  # separate kernels: convolutions, pyramid, one apply per level
     torch.cuda.synchronize()
     assert int(fb[0]) == 0
     assert torch.equal(outs[1], out)

@@ -21,19 +21,18 @@ def _inputs(seed=0):
 
 def _total_loss(params, feat, rad, target, err_weight, err_target):
     net = ko.Net(params)
-    logits = net.forward(feat)
-    L, _ = _loss_frozen(logits, rad, target, err_weight, err_target)
+    o, err = ko.reconstruct(net.forward(feat), rad)
+    L, _, _ = _loss_frozen(o[0], err, target, err_weight, err_target)
     return L
 
 
-def _loss_frozen(logits, rad, target, err_weight, err_target):
+def _loss_frozen(out, err, target, err_weight, err_target):
     """The oracle's loss with the error head's target held at err_target (as the gradient assumes)."""
-    L, dl = ko.loss(logits, rad, target, 0.0)
-    de = logits[..., ko.TAPS] - err_target
+    L, dout, _ = ko.loss(out, err, target, 0.0)
+    de = err - err_target
     P = float(H * W)
     L += err_weight * (de * de).sum() / P
-    dl[..., ko.TAPS] = 2.0 * err_weight * de / P
-    return L, dl
+    return L, dout, 2.0 * err_weight * de / P
 
 
 def test_backward_matches_finite_differences():
@@ -47,14 +46,14 @@ def test_backward_matches_finite_differences():
     err_weight = 0.3
     net = ko.Net(params)
     logits = net.forward(feat)
-    out, _ = ko.apply(logits, rad)
-    rp = ((out - target) ** 2 / (target ** 2 + 0.01)).mean(-1)
+    o, err = ko.reconstruct(logits, rad)
+    rp = ((o[0] - target) ** 2 / (target ** 2 + 0.01)).mean(-1)
     err_target = np.log(rp + 1e-4)
-    _, dl = _loss_frozen(logits, rad, target, err_weight, err_target)
-    grads = net.backward(dl).reshape(-1)
+    _, dout, derr = _loss_frozen(o[0], err, target, err_weight, err_target)
+    grads = net.backward(ko.reconstruct_backward(logits, rad, o, dout, derr)).reshape(-1)
     # the error head's own gradient against the oracle's loss with err_target computed inside
-    L1, dl1 = ko.loss(logits, rad, target, err_weight)
-    assert np.allclose(dl1, dl, rtol=1e-12, atol=1e-15)
+    _, dout1, derr1 = ko.loss(o[0], err, target, err_weight)
+    assert np.allclose(dout1, dout, rtol=1e-12, atol=1e-15) and np.allclose(derr1, derr, rtol=1e-12, atol=1e-15)
     eps = 1e-6
     checked = 0
     for l in range(ko.LAYERS):
@@ -88,7 +87,58 @@ def test_linear_pieces_are_adjoint():
     assert np.isclose((p * g).sum(), (x * ko.pool_backward(g, H, W)).sum())
     u = ko.upsample(g, H, W)
     assert np.isclose((u * x).sum(), (g * ko.upsample_backward(x)).sum())
+    ub = ko.upsample_bilinear(g, H, W)
+    assert np.isclose((ub * x).sum(), (g * ko.upsample_bilinear_backward(x)).sum())
     assert np.allclose(ko.pool(np.full((H, W, 32), 3.25)), 3.25)
+
+
+@pytest.mark.parametrize("hw", [(13, 10), (16, 16), (1, 1), (2, 3)])
+def test_bilinear_upsampling_closed_forms(hw):
+    """Constants stay constant; an interior linear ramp of the coarse grid is
+    sampled at (x + 0.5) / 2 - 0.5 exactly; weights per fine pixel sum to 1."""
+    Hf, Wf = hw
+    h, w = (Hf + 1) // 2, (Wf + 1) // 2
+    assert np.allclose(ko.upsample_bilinear(np.full((h, w, 3), -1.5), Hf, Wf), -1.5)
+    X = np.arange(w, dtype=np.float64)
+    ramp = np.broadcast_to((2.0 * X + 1.0)[None, :, None], (h, w, 1))
+    up = ko.upsample_bilinear(ramp, Hf, Wf)[..., 0]
+    xs = np.arange(Wf)
+    s = (xs + 0.5) / 2.0 - 0.5
+    inside = (s >= 0) & (s <= w - 1)
+    assert np.allclose(up[:, inside], (2.0 * s[inside] + 1.0)[None, :])
+    # the transpose spreads a unit fine gradient with total weight 1
+    d = np.zeros((Hf, Wf, 1))
+    d[Hf // 2, Wf // 2] = 1.0
+    assert np.isclose(ko.upsample_bilinear_backward(d).sum(), 1.0)
+
+
+def test_blend_limits_and_backward():
+    """blend logit -> -inf: the level's own filter; -> +inf: the upsampled coarser
+    output; the blend / filter / coarse gradients match central differences."""
+    rng = np.random.default_rng(9)
+    rad = rng.uniform(0, 2, (H, W, 3))
+    coarse = rng.uniform(0, 2, ((H + 1) // 2, (W + 1) // 2, 3))
+    lg = rng.normal(0, 1, (H, W, 32))
+    lo, hi = lg.copy(), lg.copy()
+    lo[..., ko.BLEND], hi[..., ko.BLEND] = -80.0, 80.0
+    assert np.allclose(ko.apply(lo, rad, coarse)[0], ko.filter5(lo, rad))
+    assert np.allclose(ko.apply(hi, rad, coarse)[0], ko.upsample_bilinear(coarse, H, W))
+    g = rng.normal(0, 1, (H, W, 3))
+    dl, dup = ko.apply_backward(lg, rad, coarse, g)
+    dc = ko.upsample_bilinear_backward(dup)
+    f = lambda l_, c_: (ko.apply(l_, rad, c_)[0] * g).sum()
+    eps = 1e-6
+    for (y, x, c) in [(0, 0, 0), (6, 4, 7), (12, 9, 24), (5, 5, ko.BLEND), (3, 8, ko.BLEND)]:
+        p_, m_ = lg.copy(), lg.copy()
+        p_[y, x, c] += eps
+        m_[y, x, c] -= eps
+        assert abs((f(p_, coarse) - f(m_, coarse)) / (2 * eps) - dl[y, x, c]) <= 1e-7
+    for (y, x, c) in [(0, 0, 1), (3, 2, 2), (6, 4, 0)]:
+        p_, m_ = coarse.copy(), coarse.copy()
+        p_[y, x, c] += eps
+        m_[y, x, c] -= eps
+        assert abs((f(lg, p_) - f(lg, m_)) / (2 * eps) - dc[y, x, c]) <= 1e-7
+    assert np.all(dl[..., ko.ERR] == 0) and np.all(dl[..., ko.BLEND + 1:] == 0)
 
 
 def test_conv_tap_convention():
@@ -110,7 +160,7 @@ def test_filter_closed_forms():
         logits = np.full((H, W, 32), -60.0)
         logits[..., j] = 60.0
         logits[..., ko.TAPS] = 1.5
-        out, err = ko.apply(logits, rad)
+        out, err = ko.apply(logits, rad)  # no coarser level: the filter alone
         ox, oy = j % 5 - 2, j // 5 - 2
         ys = np.clip(np.arange(H) + oy, 0, H - 1)
         xs = np.clip(np.arange(W) + ox, 0, W - 1)
