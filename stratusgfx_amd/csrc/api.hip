@@ -472,10 +472,11 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 namespace {
 
 // The inference workspace: per level l two activation planes (e_l and a
-// temporary for c_l / d_l), then the features, the radiance pyramid, the
-// reconstruction of every level, the error logit and the tile flags.
+// temporary for c_l / d_l) and, below level 0, the pooled input p_l; then the
+// features, the radiance pyramid, the reconstruction of every level, the
+// error logit and the tile flags.
 struct KpcnWs {
-    uint16_t* act[SGI_KPCN_LEVELS][2];
+    uint16_t* act[SGI_KPCN_LEVELS][3];  // [l][2]: the pooled input p_l of level l >= 1
     uint16_t* feat;
     float* rad[SGI_KPCN_LEVELS];  // the radiance pyramid (level 0 = the noisy radiance)
     float* o[SGI_KPCN_LEVELS];    // the reconstruction per level (level 0 = the output)
@@ -494,8 +495,8 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     int h = H, w = W;
     for (int l = 0; l < SGI_KPCN_LEVELS; ++l) {
         if (ws) ws->h[l] = h, ws->w[l] = w;
-        for (int k = 0; k < 2; ++k) {
-            char* p = take((size_t)h * w * SGI_KPCN_CHANNELS * 2);
+        for (int k = 0; k < 3; ++k) {
+            char* p = (k < 2 || l > 0) ? take((size_t)h * w * SGI_KPCN_CHANNELS * 2) : nullptr;
             if (ws) ws->act[l][k] = (uint16_t*)p;
         }
         h = (h + 1) / 2;
@@ -576,12 +577,18 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     KPCN_STEP(sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], s));
     KPCN_STEP(sgi_kpcn::pyramid(ws.rad[0], H, W, ws.rad[1], s));
     KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
-    KPCN_STEP(sgi_kpcn::conv3x3(ws.act[0][1], nullptr, Wl(1), Bl(1), 1, ws.h[0], ws.w[0], ws.act[0][0], 0, s));
+    // e_l's convolution also writes p_(l+1), the 2x2 mean of e_l
+    KPCN_STEP(sgi_kpcn::convAndPool(ws.act[0][1], ws.h[0], ws.w[0], Wl(1), Bl(1), ws.act[0][0], ws.act[1][2], s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
-        KPCN_STEP(sgi_kpcn::convPool(ws.act[l - 1][0], ws.h[l - 1], ws.w[l - 1], Wl(2 * l), Bl(2 * l), ws.act[l][1],
-                                     s));
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
-                                    ws.act[l][0], 0, s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][1], 0,
+                                    s));
+        if (l + 1 < SGI_KPCN_LEVELS) {
+            KPCN_STEP(sgi_kpcn::convAndPool(ws.act[l][1], ws.h[l], ws.w[l], Wl(2 * l + 1), Bl(2 * l + 1), ws.act[l][0],
+                                            ws.act[l + 1][2], s));
+        } else {
+            KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
+                                        ws.act[l][0], 0, s));
+        }
     }
     // decoder: d_l = conv(e_l + up(d_{l+1})) with d_4 = e_4; d_l lands in act[l][1]; each
     // level's head runs the level's reconstruction step in its epilogue, coarse to fine

@@ -172,14 +172,16 @@ __device__ __forceinline__ void bilinearAxis(int x, int n, int& i0, int& i1, flo
     i1 = min(max(a + 1, 0), n - 1);
 }
 
-// up(coarse) at fine pixel (x, y); coarse is [h][w][3] with h, w the coarser level's size.
-__device__ __forceinline__ f3 upBilinear(const float* __restrict__ coarse, int h, int w, int x, int y) {
+// up(coarse) at fine pixel (x, y); T.coarse(X, Y) reads the coarser level
+// ((h, w) its size) at in-range coordinates.
+template <class T>
+__device__ __forceinline__ f3 upBilinearT(const T& src, int h, int w, int x, int y) {
     int y0, y1, x0, x1;
     float wy0, wy1, wx0, wx1;
     bilinearAxis(y, h, y0, y1, wy0, wy1);
     bilinearAxis(x, w, x0, x1, wx0, wx1);
-    const f3 top = ld3(coarse + 3 * ((size_t)y0 * w + x0)) * wx0 + ld3(coarse + 3 * ((size_t)y0 * w + x1)) * wx1;
-    const f3 bot = ld3(coarse + 3 * ((size_t)y1 * w + x0)) * wx0 + ld3(coarse + 3 * ((size_t)y1 * w + x1)) * wx1;
+    const f3 top = src.coarse(x0, y0) * wx0 + src.coarse(x1, y0) * wx1;
+    const f3 bot = src.coarse(x0, y1) * wx0 + src.coarse(x1, y1) * wx1;
     return top * wy0 + bot * wy1;
 }
 
@@ -187,38 +189,71 @@ __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(
 
 // One level of the reconstruction at pixel (x, y): the 5x5 softmax filter of
 // this level's radiance, blended with the upsampled coarser output when given.
-__device__ __forceinline__ f3 composePixel(const float* lg, const float* __restrict__ radiance,
-                                           const float* __restrict__ coarse, int W, int H, int x, int y) {
+// T.radiance(xx, yy) reads the level's radiance at in-image coordinates.
+template <class T>
+__device__ __forceinline__ f3 composePixelT(const float* lg, const T& src, bool hasCoarse, int W, int H, int x,
+                                            int y) {
     float wgt[SGI_KPCN_TAPS];
     softmax25(lg, wgt);
-    f3 o = filtered(radiance, W, H, x, y, wgt);
-    if (coarse) {
+    f3 o = splat(0.0f);
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
+        o = o + src.radiance(xx, yy) * wgt[j];
+    }
+    if (hasCoarse) {
         const float b = sigmoidf(lg[SGI_KPCN_BLEND_CHANNEL]);
-        o = o + (upBilinear(coarse, (H + 1) / 2, (W + 1) / 2, x, y) - o) * b;
+        o = o + (upBilinearT(src, (H + 1) / 2, (W + 1) / 2, x, y) - o) * b;
     }
     return o;
+}
+
+// The same taps read from global memory.
+struct GlobalTaps {
+    const float* rad;
+    const float* co;
+    int W, cw;
+    __device__ __forceinline__ f3 radiance(int xx, int yy) const { return ld3(rad + 3 * ((size_t)yy * W + xx)); }
+    __device__ __forceinline__ f3 coarse(int X, int Y) const { return ld3(co + 3 * ((size_t)Y * cw + X)); }
+};
+
+__device__ __forceinline__ f3 upBilinear(const float* __restrict__ coarse, int h, int w, int x, int y) {
+    const GlobalTaps t{nullptr, coarse, 0, w};
+    return upBilinearT(t, h, w, x, y);
+}
+
+__device__ __forceinline__ f3 composePixel(const float* lg, const float* __restrict__ radiance,
+                                           const float* __restrict__ coarse, int W, int H, int x, int y) {
+    const GlobalTaps t{radiance, coarse, W, (W + 1) / 2};
+    return composePixelT(lg, t, coarse != nullptr, W, H, x, y);
 }
 
 // Where the convolution's input comes from (staged into the LDS halo):
 //   IN_PLAIN     a [h][w][32] (gated by mask when given);
 //   IN_UPADD     a + up(b): the skip tensor plus the coarser level (nearest);
-//   IN_POOL      2x2 mean of a at the finer level (fineH x fineW).
+//                both chunks are prefetched raw and added when stored to LDS,
+//                so the loads stay in flight across the previous tile's MFMAs.
 // and where the result goes:
 //   OUT_BF16 / OUT_F32  [h][w][32] with bias (and ReLU for BF16 when relu != 0);
+//   OUT_BF16_POOL       OUT_BF16 plus the 2x2 mean of the stored BF16 values
+//                       ([ceil(h/2)][ceil(w/2)][32], the next level's input),
+//                       formed in the epilogue: a wave's rows 4w .. 4w + 3
+//                       hold two row pairs, and the column pair sits in
+//                       neighbouring lanes;
 //   OUT_APPLY           the 27 used logits stay in LDS and the epilogue runs
 //                       one level of the reconstruction (composePixel: the
 //                       5x5 softmax filter of the level's radiance, blended
 //                       with the coarser output when given; out [h][w][3])
 //                       and writes the error logit (err [h][w]).
 // Each fused form computes exactly what the separate kernels compute.
-enum { IN_PLAIN = 0, IN_UPADD = 1, IN_POOL = 2 };
-enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2 };
+enum { IN_PLAIN = 0, IN_UPADD = 1 };
+enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2, OUT_BF16_POOL = 3 };
 
 struct ConvSrc {
     const uint16_t* a;
     const uint16_t* b;
     const uint16_t* mask;
-    int fineH, fineW;
+    uint16_t* pooled;       // OUT_BF16_POOL
     const float* radiance;  // OUT_APPLY: this level's radiance
     const float* coarse;    // OUT_APPLY: the coarser level's output, or NULL
 };
@@ -229,48 +264,17 @@ __device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
     return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
 }
 
-template <int IN>
-__device__ __forceinline__ uint4 stageChunk(const ConvSrc& src, int H, int W, int gx, int gy, int c) {
-    if constexpr (IN == IN_PLAIN) {
-        const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
-        uint4 v = *reinterpret_cast<const uint4*>(src.a + g);
-        if (src.mask) v = gateBf16x8(v, *reinterpret_cast<const uint4*>(src.mask + g));
-        return v;
-    } else if constexpr (IN == IN_UPADD) {
-        const int w2 = (W + 1) / 2;
-        const uint4 a = *reinterpret_cast<const uint4*>(src.a + ((size_t)gy * W + gx) * kC + c * 8);
-        const uint4 b = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
-        return make_uint4(addBf16x2(a.x, b.x), addBf16x2(a.y, b.y), addBf16x2(a.z, b.z), addBf16x2(a.w, b.w));
-    } else {  // IN_POOL
-        float s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int n = 0;
-        for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) {
-                const int yy = 2 * gy + dy, xx = 2 * gx + dx;
-                if (yy >= src.fineH || xx >= src.fineW) continue;
-                ++n;
-                const uint4 v = *reinterpret_cast<const uint4*>(src.a + ((size_t)yy * src.fineW + xx) * kC + c * 8);
-                const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    s8[2 * j] += bf2f((uint16_t)(vv[j] & 0xffffu));
-                    s8[2 * j + 1] += bf2f((uint16_t)(vv[j] >> 16));
-                }
-            }
-        const float inv = 1.0f / (float)n;
-        uint32_t o[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            o[j] = (uint32_t)f2bf(s8[2 * j] * inv) | ((uint32_t)f2bf(s8[2 * j + 1] * inv) << 16);
-        return make_uint4(o[0], o[1], o[2], o[3]);
-    }
+__device__ __forceinline__ uint4 addChunk(uint4 a, uint4 b) {
+    return make_uint4(addBf16x2(a.x, b.x), addBf16x2(a.y, b.y), addBf16x2(a.z, b.z), addBf16x2(a.w, b.w));
 }
 
-// The halo chunks one thread stages (every load issued before any is used).
+// The halo chunks one thread stages (every load issued before any is used):
+// chunk i = 8 channels (c = i % 4) of halo pixel i / 4.
 constexpr int kStageSlots = (kHS * kHS * 4 + 255) / 256;
 
 template <int IN>
-__device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v) {
+__device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v,
+                                         uint4* vb) {
     const int x0 = (tile % tilesX) * kTS - 1, y0 = (tile / tilesX) * kTS - 1;
 #pragma unroll
     for (int k = 0; k < kStageSlots; ++k) {
@@ -278,13 +282,42 @@ __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int t
         const int px = i >> 2, c = i & 3;
         const int gy = y0 + px / kHS, gx = x0 + px % kHS;
         v[k] = make_uint4(0u, 0u, 0u, 0u);
-        if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) v[k] = stageChunk<IN>(src, H, W, gx, gy, c);
+        if constexpr (IN == IN_UPADD) vb[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
+            v[k] = *reinterpret_cast<const uint4*>(src.a + g);
+            if constexpr (IN == IN_UPADD) {
+                const int w2 = (W + 1) / 2;
+                vb[k] = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+            } else if (src.mask) {
+                v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(src.mask + g));
+            }
+        }
     }
 }
 
 constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (channels 0..26)
 constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
                                                                              : kTS * kTS * kLogitStride / 4 + 1;
+constexpr int kRadHS = kTS + 4;     // radiance staged for the 5x5 taps (OUT_APPLY)
+constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_APPLY)
+
+// Taps of the reconstruction from the LDS-staged radiance and coarser output;
+// (ox, oy) maps image coordinates to the staged radiance, (cx0, cy0) is the
+// staged coarser block's first pixel.
+struct LdsTaps {
+    const float* rad;
+    const float* co;
+    int ox, oy, cx0, cy0;
+    __device__ __forceinline__ f3 radiance(int xx, int yy) const {
+        const int i = (yy + oy) * kRadHS + xx + ox;
+        return f3{rad[3 * i], rad[3 * i + 1], rad[3 * i + 2]};
+    }
+    __device__ __forceinline__ f3 coarse(int X, int Y) const {
+        const int i = (Y - cy0) * kCoHS + X - cx0;
+        return f3{co[3 * i], co[3 * i + 1], co[3 * i + 2]};
+    }
+};
 
 template <int IN, int OUT>
 __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint16_t* __restrict__ w,
@@ -292,6 +325,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                                                      int tilesX, int nTiles, void* __restrict__ out,
                                                      float* __restrict__ err) {
     __shared__ uint4 smem[kSmemU4];  // the input halo, then (OUT_APPLY) the tile's logits
+    __shared__ float radS[OUT == OUT_APPLY ? kRadHS * kRadHS * 3 : 1];
+    __shared__ float coS[OUT == OUT_APPLY ? kCoHS * kCoHS * 3 : 1];
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
@@ -310,8 +345,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 
     int t0, t1;
     blockTiles(nTiles, t0, t1);
-    uint4 pre[kStageSlots];
-    if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre);
+    uint4 pre[kStageSlots], preB[IN == IN_UPADD ? kStageSlots : 1];
+    if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre, preB);
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
         __syncthreads();
@@ -319,10 +354,15 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 #pragma unroll
         for (int k = 0; k < kStageSlots; ++k) {
             const int i = (int)threadIdx.x + k * 256;
-            if (i < kHS * kHS * 4) halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
+            if (i < kHS * kHS * 4) {
+                if constexpr (IN == IN_UPADD)
+                    halo[(i >> 2) * kPxU4 + (i & 3)] = addChunk(pre[k], preB[k]);
+                else
+                    halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
+            }
         }
         __syncthreads();
-        if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre);
+        if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre, preB);
         f32x4 acc[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -342,7 +382,28 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
         if constexpr (OUT == OUT_APPLY) {
             float* logits = reinterpret_cast<float*>(smem);
+            // this level's radiance around the tile (2-pixel margin) and the coarser
+            // output under it (1-pixel margin), clamped as the global taps clamp
+            const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
+            const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
+            const int ch = (H + 1) / 2, cw = (W + 1) / 2;
             __syncthreads();  // every wave has read the halo the logits overwrite
+            for (int i = threadIdx.x; i < kRadHS * kRadHS; i += 256) {
+                const int gy = min(max(ry0 + i / kRadHS, 0), H - 1), gx = min(max(rx0 + i % kRadHS, 0), W - 1);
+                const f3 v = ld3(src.radiance + 3 * ((size_t)gy * W + gx));
+                radS[3 * i + 0] = v.x;
+                radS[3 * i + 1] = v.y;
+                radS[3 * i + 2] = v.z;
+            }
+            if (src.coarse) {
+                for (int i = threadIdx.x; i < kCoHS * kCoHS; i += 256) {
+                    const int gy = min(max(cy0 + i / kCoHS, 0), ch - 1), gx = min(max(cx0 + i % kCoHS, 0), cw - 1);
+                    const f3 v = ld3(src.coarse + 3 * ((size_t)gy * cw + gx));
+                    coS[3 * i + 0] = v.x;
+                    coS[3 * i + 1] = v.y;
+                    coS[3 * i + 2] = v.z;
+                }
+            }
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
@@ -359,7 +420,9 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                 float lgv[kLogitStride];
 #pragma unroll
                 for (int j = 0; j < kLogitStride; ++j) lgv[j] = logits[threadIdx.x * kLogitStride + j];
-                const f3 o = composePixel(lgv, src.radiance, src.coarse, W, H, x, y);
+                const int lx = (int)threadIdx.x % kTS, ly = (int)threadIdx.x / kTS;
+                const LdsTaps taps{radS, coS, lx + 2 - x, ly + 2 - y, cx0, cy0};
+                const f3 o = composePixelT(lgv, taps, src.coarse != nullptr, W, H, x, y);
                 const size_t p = (size_t)y * W + x;
                 float* dst = static_cast<float*>(out) + 3 * p;
                 dst[0] = o.x;
@@ -368,11 +431,11 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                 if (err) err[p] = lgv[SGI_KPCN_ERR_CHANNEL];
             }
         } else {
+            constexpr bool kBf16 = OUT == OUT_BF16 || OUT == OUT_BF16_POOL;
             const int x = tx * kTS + l16;
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 const int y = ty * kTS + wave * 4 + nb;
-                if (x >= W || y >= H) continue;
                 const size_t p = (size_t)y * W + x;
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb) {
@@ -380,16 +443,57 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         v[r] = acc[mb][nb][r] + bv[mb][r];
-                        if (OUT == OUT_BF16 && relu) v[r] = gmax(v[r], 0.0f);
+                        if (kBf16 && relu) v[r] = gmax(v[r], 0.0f);
                     }
                     const int co = mb * 16 + 4 * lg;
                     if constexpr (OUT == OUT_F32) {
-                        *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
-                            make_float4(v[0], v[1], v[2], v[3]);
+                        if (x < W && y < H)
+                            *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
+                                make_float4(v[0], v[1], v[2], v[3]);
                     } else {
-                        const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
-                                                   (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
-                        *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) = o;
+                        const uint32_t b0 = f2bf(v[0]), b1 = f2bf(v[1]), b2 = f2bf(v[2]), b3 = f2bf(v[3]);
+                        if (x < W && y < H)
+                            *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) =
+                                make_uint2(b0 | (b1 << 16), b2 | (b3 << 16));
+                        if constexpr (OUT == OUT_BF16_POOL) {  // keep the stored values for the pooling
+                            acc[mb][nb][0] = bf2f((uint16_t)b0);
+                            acc[mb][nb][1] = bf2f((uint16_t)b1);
+                            acc[mb][nb][2] = bf2f((uint16_t)b2);
+                            acc[mb][nb][3] = bf2f((uint16_t)b3);
+                        }
+                    }
+                }
+            }
+            if constexpr (OUT == OUT_BF16_POOL) {
+                // rows 4 wave + {0, 1} and {2, 3} pool into two coarse rows; the even lane
+                // of each column pair sums its children in poolKernel's order
+                const int cw = (W + 1) / 2;
+                const bool xIn = x < W, x1In = x + 1 < W, even = (l16 & 1) == 0;
+#pragma unroll
+                for (int pr = 0; pr < 2; ++pr) {
+                    const int y0 = ty * kTS + wave * 4 + 2 * pr;
+                    const bool y1In = y0 + 1 < H;
+                    const int n = (1 + (x1In ? 1 : 0)) * (1 + (y1In ? 1 : 0));
+                    const float inv = 1.0f / (float)n;
+#pragma unroll
+                    for (int mb = 0; mb < 2; ++mb) {
+                        uint32_t o[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const float a00 = acc[mb][2 * pr][r], a10 = acc[mb][2 * pr + 1][r];
+                            const float a01 = __shfl_xor(a00, 1), a11 = __shfl_xor(a10, 1);
+                            float sum = 0.0f;
+                            sum += a00;
+                            if (x1In) sum += a01;
+                            if (y1In) {
+                                sum += a10;
+                                if (x1In) sum += a11;
+                            }
+                            o[r] = f2bf(sum * inv);
+                        }
+                        if (even && xIn && y0 < H)
+                            *reinterpret_cast<uint2*>(src.pooled + ((size_t)(y0 / 2) * cw + x / 2) * kC + mb * 16 +
+                                                      4 * lg) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
                     }
                 }
             }
@@ -928,13 +1032,12 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
     return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
 }
 
-hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
-                    hipStream_t s) {
+hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
+                       uint16_t* pooled, hipStream_t s) {
     ConvSrc src{};
-    src.a = fine;
-    src.fineH = fineH;
-    src.fineW = fineW;
-    return launchConv<IN_POOL, OUT_BF16>(src, w, bias, 1, (fineH + 1) / 2, (fineW + 1) / 2, out, nullptr, s);
+    src.a = in;
+    src.pooled = pooled;
+    return launchConv<IN_PLAIN, OUT_BF16_POOL>(src, w, bias, 1, H, W, out, nullptr, s);
 }
 
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,

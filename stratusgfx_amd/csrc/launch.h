@@ -80,11 +80,11 @@ namespace sgi_kpcn {
 hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
                    int W, void* out, int outF32, hipStream_t s);
 // Inference forms of the convolution with the neighbouring pass fused into the
-// input staging (pooling, upsample + skip) or the epilogue (one reconstruction
-// level: the 5x5 filter and the blend with the coarser output); same results
-// as the separate kernels.
-hipError_t convPool(const uint16_t* fine, int fineH, int fineW, const uint16_t* w, const float* bias, uint16_t* out,
-                    hipStream_t s);
+// input staging (upsample + skip) or the epilogue (2x2 pooling of the output;
+// one reconstruction level: the 5x5 filter and the blend with the coarser
+// output); same results as the separate kernels.
+hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
+                       uint16_t* pooled, hipStream_t s);
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
                      uint16_t* out, hipStream_t s);
 hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
