@@ -8,23 +8,29 @@ A step is one GI frame on resident inputs, in RenderScene's order
 
   learned   global VPL sun-visibility cull and compaction, the visible slots'
             atlas entries, deferred direct sun light with cascaded PCF, an
-            8-sample-per-pixel gather over the visible VPLs (the reference's
+            S-sample-per-pixel gather over the visible VPLs (the reference's
             hashed estimator, vpl_pbr_gi.fs:145-200, at a fixed draw count
-            with failing draws skipped) and the learned denoiser (network + 5x5
-            predicted filter, a-trous on the tiles whose predicted error
-            exceeds the trained threshold, the temporal pass and combine);
+            with failing draws skipped) and the learned denoiser (network +
+            multi-scale predicted filters, a-trous on the tiles whose
+            predicted error exceeds the trained threshold, the temporal pass
+            and combine), S swept over --learned-spp;
   a-trous   the same steps with S samples per pixel and the 4-pass a-trous /
-            temporal denoiser, S swept over --atrous-spp; the equal-relMSE
-            point is the smallest S whose relMSE is at most the learned form's;
+            temporal denoiser, S swept over --atrous-spp;
   tile list per-tile VPL culling (16x16 tiles, up to 64 VPLs) and the
             tile-list gather + a-trous: the parity-tested plain pipeline, timed
             for its own throughput and roofline.
 
-The learned form is also run on the 1024-VPL scene (same geometry), whose own
-converged render lies `converged_1024_vs_4096_relmse` from the target.
-relMSE = mean of (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after
-one frame, against the converged render (every visible VPL of the 4096-VPL
-scene, FP64 sums).  `value` is the learned form's frames/s.  Inputs are
+For each learned sample count, the a-trous form at equal relMSE is the
+smallest swept S whose relMSE is at most the learned form's (and, beside it,
+the frames/s interpolated log-log between the two bracketing a-trous points).
+The operating point is the learned sample count with the highest ratio of
+frames/s at equal relMSE; it is then timed over exactly --steps frames, and
+`value` is its frames/s.  The learned form is also run on the 1024-VPL scene
+(same geometry), whose own converged render lies
+`converged_1024_vs_4096_relmse` from the target.  relMSE = mean of
+(x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after one frame, against
+the converged render (every visible VPL of the 4096-VPL scene, FP64 sums).
+Inputs are
 generated on the GPU before the timed regions; the denoiser history alternates
 between two buffer sets as the reference swaps its FBOs.
 
@@ -33,6 +39,7 @@ a bounded sample of the tile-list frame as the CPU baseline.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -50,8 +57,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
-    ap.add_argument("--spp", type=int, default=8, help="gather samples per pixel of the learned form")
-    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[2, 4, 8, 16, 32, 64, 128],
+    ap.add_argument("--learned-spp", type=int, nargs="+", default=[4, 8, 16, 32],
+                    help="gather samples per pixel of the learned form (the operating point is chosen among them)")
+    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
     ap.add_argument("--rejection-limit", type=int, default=-1,
@@ -161,6 +169,28 @@ def rel_mse(x, ref) -> float:
     """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2), in FP64."""
     x, r = x.double(), ref.double()
     return float(((x - r) ** 2 / (r * r + 1e-2)).mean())
+
+
+def atrous_at_equal(sweep, learned):
+    """The a-trous form at the learned form's relMSE: the cheapest swept point at or below it
+    (and the frames/s interpolated log-log between the bracketing points)."""
+    if not sweep:
+        return {}
+    rel = learned["relmse"]
+    ok = [r for r in sweep if r["relmse"] <= rel]
+    eq = min(ok, key=lambda r: r["spp"]) if ok else max(sweep, key=lambda r: r["spp"])
+    out = {"atrous_at_equal_relmse": {**eq, "reached": bool(ok)},
+           "speedup_vs_atrous_at_equal_relmse": round(learned["frames_per_s"] / eq["frames_per_s"], 3)}
+    pts = sorted(sweep, key=lambda r: r["spp"])
+    for a, b in zip(pts, pts[1:]):
+        if a["relmse"] >= rel >= b["relmse"] and a["relmse"] > b["relmse"]:
+            t = (math.log(a["relmse"]) - math.log(rel)) / (math.log(a["relmse"]) - math.log(b["relmse"]))
+            fps = math.exp(math.log(a["frames_per_s"]) + t * (math.log(b["frames_per_s"]) - math.log(a["frames_per_s"])))
+            spp = math.exp(math.log(a["spp"]) + t * (math.log(b["spp"]) - math.log(a["spp"])))
+            out["atrous_interpolated"] = {"spp": round(spp, 2), "frames_per_s": round(fps, 2)}
+            out["speedup_vs_atrous_interpolated"] = round(learned["frames_per_s"] / fps, 3)
+            break
+    return out
 
 
 def timed_loop(step, steps, warmup, nstages):
@@ -387,6 +417,7 @@ def main():
     # `converged_1024_vs_4096_relmse` from the target)
     weights_path = kpcn.DEFAULT_WEIGHTS
     learned_res = learned_1024 = None
+    learned_sweep = []
     el_learned = None
     wts = None
     if os.path.exists(weights_path) and args.forms in ("all", "learned"):
@@ -400,7 +431,7 @@ def main():
         n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
         fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
 
-        def learned_form(state, label):
+        def learned_form(state, label, S, steps, warmup):
             ws = state.gi.kpcn_workspace()
 
             def learned_step(k, ev, hist=True, thr=threshold):
@@ -414,7 +445,7 @@ def main():
                 p.gi.direct_light(p.g, fb.direct)
                 rec(2)
                 p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
-                               fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=args.spp,
+                               fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S,
                                rejection_limit=args.rejection_limit)
                 rec(3)
                 p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net,
@@ -437,24 +468,37 @@ def main():
             torch.cuda.synchronize()
             atrous_same = rel_mse(tmp[1], target)
             del tmp
-            print(f"[bench] timing {args.steps} learned frames ({label})", file=sys.stderr, flush=True)
-            el, per = timed_loop(learned_step, args.steps, args.warmup, 4)
+            el, per = timed_loop(learned_step, steps, warmup, 4)
             res = {
-                "frames_per_s": round(args.steps / el, 2), "ms_per_step": round(el * 1e3 / args.steps, 4),
+                "frames_per_s": round(steps / el, 2), "ms_per_step": round(el * 1e3 / steps, 4),
                 "relmse": round(rel, 6), "relmse_without_fallback": round(rel_nf, 6),
                 "relmse_noisy_input": round(noisy, 6), "relmse_atrous_on_same_input": round(atrous_same, 6),
                 "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
                 "kernels_ms": {k: round(v, 4) for k, v in
                                zip(["cull_global", "direct_light", "gather", "denoise_learned"], per)},
                 "vpls": state.w.num_vpls, "visible_vpls": int(fb.visible[0].item()),
-                "samples_per_pixel": args.spp, "weights": os.path.relpath(weights_path, HERE),
+                "samples_per_pixel": S, "weights": os.path.relpath(weights_path, HERE),
             }
-            print(f"[bench] learned ({label}): relMSE {rel:.5f}, {res['frames_per_s']} frames/s", file=sys.stderr,
-                  flush=True)
+            print(f"[bench] learned ({label}, {S} spp, {steps} frames): relMSE {rel:.5f}, {res['frames_per_s']} "
+                  f"frames/s", file=sys.stderr, flush=True)
             return res, el
 
-        learned_res, el_learned = learned_form(plain, "4096 VPLs")
-        learned_1024, _ = learned_form(learned, "1024 VPLs")
+        # the learned form's sample counts, each against the a-trous form at equal relMSE
+        learned_sweep = []
+        for S in args.learned_spp:
+            r, _ = learned_form(plain, "4096 VPLs", S, args.sweep_steps, 2)
+            r.update(atrous_at_equal(sweep, r))
+            learned_sweep.append(r)
+        if learned_sweep and learned_sweep[0].get("speedup_vs_atrous_at_equal_relmse") is not None:
+            best = max(learned_sweep, key=lambda r: r["speedup_vs_atrous_at_equal_relmse"])
+        else:
+            best = learned_sweep[len(learned_sweep) // 2]
+        S_op = best["samples_per_pixel"]
+        print(f"[bench] operating point {S_op} spp; timing {args.steps} learned frames", file=sys.stderr,
+              flush=True)
+        learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup)
+        learned_res.update(atrous_at_equal(sweep, learned_res))
+        learned_1024, _ = learned_form(learned, "1024 VPLs", S_op, args.sweep_steps, 2)
 
     texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
     gather_roof = None
@@ -479,12 +523,6 @@ def main():
         }
     if learned_res is not None:
         value, el, form = learned_res["frames_per_s"], el_learned, "learned"
-        if sweep:
-            ok = [r for r in sweep if r["relmse"] <= learned_res["relmse"]]
-            eq = min(ok, key=lambda r: r["spp"]) if ok else max(sweep, key=lambda r: r["spp"])
-            learned_res["atrous_at_equal_relmse"] = {**eq, "reached": bool(ok)}
-            learned_res["speedup_vs_atrous_at_equal_relmse"] = round(
-                learned_res["frames_per_s"] / eq["frames_per_s"], 3)
         if plain_res is not None:
             learned_res["speedup_vs_tile_list_plain"] = round(
                 learned_res["frames_per_s"] / plain_res["frames_per_s"], 3)
@@ -507,12 +545,14 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 shading, bf16 network (fp32 accumulation)" if learned_res else "fp32",
         "data": "synthetic: seeded procedural city block, ray-cast G-buffer / VPL cubes / cascades",
-        "config": {"workload": "city-1920x1080, 4096 VPLs: learned (8 spp + network) vs a-trous (S spp) at "
+        "config": {"workload": "city-1920x1080, 4096 VPLs: learned (S spp + network) vs a-trous (S' spp) at "
                                "equal relMSE",
+                   "learned_spp": learned_res["samples_per_pixel"] if learned_res else None,
                    "width": W, "height": H, "cube": f"6x{w_plain.cube_res}x{w_plain.cube_res} fp16",
                    "form": form, "converged": "all visible VPLs of the 4096-VPL scene, FP64 sums",
                    "converged_1024_vs_4096_relmse": round(floor_rel, 6)},
         "learned": learned_res,
+        "learned_sweep": learned_sweep if learned_res is not None else [],
         "learned_1024_vpls": learned_1024,
         "atrous_sweep": sweep,
         "plain_tile_lists": plain_res,

@@ -121,7 +121,10 @@ __global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restric
     dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
 }
 
-// Softmax weights of the 25 filter logits (max-subtracted).
+constexpr float kLog2e = 1.44269504088896f;
+
+// Softmax weights of the 25 filter logits (max-subtracted; exp through the
+// hardware exp2 of the log2(e)-scaled difference, all arguments <= 0).
 __device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
     float m = lg[0];
 #pragma unroll
@@ -129,7 +132,7 @@ __device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
     float s = 0.0f;
 #pragma unroll
     for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
-        wgt[j] = expf(lg[j] - m);
+        wgt[j] = __builtin_amdgcn_exp2f((lg[j] - m) * kLog2e);
         s += wgt[j];
     }
     const float inv = 1.0f / s;
@@ -185,7 +188,9 @@ __device__ __forceinline__ f3 upBilinearT(const T& src, int h, int w, int x, int
     return top * wy0 + bot * wy1;
 }
 
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+__device__ __forceinline__ float sigmoidf(float x) {
+    return 1.0f / (1.0f + __builtin_amdgcn_exp2f(gmin(-x * kLog2e, 126.0f)));
+}
 
 // One level of the reconstruction at pixel (x, y): the 5x5 softmax filter of
 // this level's radiance, blended with the upsampled coarser output when given.
@@ -301,6 +306,8 @@ constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS
                                                                              : kTS * kTS * kLogitStride / 4 + 1;
 constexpr int kRadHS = kTS + 4;     // radiance staged for the 5x5 taps (OUT_APPLY)
 constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_APPLY)
+constexpr int kRadSlots = (kRadHS * kRadHS * 3 + 255) / 256;  // staged floats per thread
+constexpr int kCoSlots = (kCoHS * kCoHS * 3 + 255) / 256;
 
 // Taps of the reconstruction from the LDS-staged radiance and coarser output;
 // (ox, oy) maps image coordinates to the staged radiance, (cx0, cy0) is the
@@ -325,8 +332,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                                                      int tilesX, int nTiles, void* __restrict__ out,
                                                      float* __restrict__ err) {
     __shared__ uint4 smem[kSmemU4];  // the input halo, then (OUT_APPLY) the tile's logits
-    __shared__ float radS[OUT == OUT_APPLY ? kRadHS * kRadHS * 3 : 1];
-    __shared__ float coS[OUT == OUT_APPLY ? kCoHS * kCoHS * 3 : 1];
+    __shared__ float radS[OUT == OUT_APPLY ? kRadSlots * 256 : 1];
+    __shared__ float coS[OUT == OUT_APPLY ? kCoSlots * 256 : 1];
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
@@ -363,6 +370,41 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         }
         __syncthreads();
         if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre, preB);
+        // OUT_APPLY: this tile's radiance (2-pixel margin) and coarser output (1-pixel margin),
+        // clamped to the image as the global taps clamp, loaded straight into LDS
+        // (global_load_lds: lane l of a wave writes dword l of the wave's 64-dword run)
+        // while the MFMAs run; float i of a staged block sits at index i, the slots
+        // past the block's end receive a valid but unused element
+        if constexpr (OUT == OUT_APPLY) {
+            const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
+#pragma unroll
+            for (int k = 0; k < kRadSlots; ++k) {
+                const int i = (int)threadIdx.x + k * 256;
+                size_t g = 0;
+                if (i < kRadHS * kRadHS * 3) {
+                    const int px = i / 3, c = i - 3 * px;
+                    const int gy = min(max(ry0 + px / kRadHS, 0), H - 1), gx = min(max(rx0 + px % kRadHS, 0), W - 1);
+                    g = 3 * ((size_t)gy * W + gx) + c;
+                }
+                __builtin_amdgcn_global_load_lds(src.radiance + g, radS + k * 256 + wave * 64, 4, 0, 0);
+            }
+            if (src.coarse) {
+                const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
+                const int ch = (H + 1) / 2, cw = (W + 1) / 2;
+#pragma unroll
+                for (int k = 0; k < kCoSlots; ++k) {
+                    const int i = (int)threadIdx.x + k * 256;
+                    size_t g = 0;
+                    if (i < kCoHS * kCoHS * 3) {
+                        const int px = i / 3, c = i - 3 * px;
+                        const int gy = min(max(cy0 + px / kCoHS, 0), ch - 1),
+                                  gx = min(max(cx0 + px % kCoHS, 0), cw - 1);
+                        g = 3 * ((size_t)gy * cw + gx) + c;
+                    }
+                    __builtin_amdgcn_global_load_lds(src.coarse + g, coS + k * 256 + wave * 64, 4, 0, 0);
+                }
+            }
+        }
         f32x4 acc[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
@@ -382,28 +424,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
         if constexpr (OUT == OUT_APPLY) {
             float* logits = reinterpret_cast<float*>(smem);
-            // this level's radiance around the tile (2-pixel margin) and the coarser
-            // output under it (1-pixel margin), clamped as the global taps clamp
-            const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
             const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
-            const int ch = (H + 1) / 2, cw = (W + 1) / 2;
             __syncthreads();  // every wave has read the halo the logits overwrite
-            for (int i = threadIdx.x; i < kRadHS * kRadHS; i += 256) {
-                const int gy = min(max(ry0 + i / kRadHS, 0), H - 1), gx = min(max(rx0 + i % kRadHS, 0), W - 1);
-                const f3 v = ld3(src.radiance + 3 * ((size_t)gy * W + gx));
-                radS[3 * i + 0] = v.x;
-                radS[3 * i + 1] = v.y;
-                radS[3 * i + 2] = v.z;
-            }
-            if (src.coarse) {
-                for (int i = threadIdx.x; i < kCoHS * kCoHS; i += 256) {
-                    const int gy = min(max(cy0 + i / kCoHS, 0), ch - 1), gx = min(max(cx0 + i % kCoHS, 0), cw - 1);
-                    const f3 v = ld3(src.coarse + 3 * ((size_t)gy * cw + gx));
-                    coS[3 * i + 0] = v.x;
-                    coS[3 * i + 1] = v.y;
-                    coS[3 * i + 2] = v.z;
-                }
-            }
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
@@ -414,6 +436,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                         if (co < kLogitStride)
                             logits[((wave * 4 + nb) * kTS + l16) * kLogitStride + co] = acc[mb][nb][r] + bv[mb][r];
                     }
+            // this thread's LDS loads have landed before any thread reads the staged blocks
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
             if (x < W && y < H) {

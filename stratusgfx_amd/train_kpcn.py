@@ -3,14 +3,17 @@
     python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --size 512 --steps 4000
 
 Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
-the block; input = the 8-sample-per-pixel gather (sgi_gather_ex SAMPLES, failing
-draws skipped, so that more samples converge to the target) of 4096 VPLs, target = the converged gather over every visible VPL with FP64 sums
-(sgi_gather_ex ALL), both normalised as the denoiser's pass 0 does.
+the block; input = a sampled gather of 4096 VPLs (sgi_gather_ex SAMPLES,
+failing draws skipped, so that more samples converge to the target) at a
+sample count drawn per view from --spp, target = the converged gather over
+every visible VPL with FP64 sums (sgi_gather_ex ALL), both normalised as the
+denoiser's pass 0 does.  Mixing sample counts lets one network serve every
+operating point: it reads the noise level from the input itself.
 Training: Adam on the relative-MSE loss plus the error head's regression
 (sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
-held-out seeds reports the relative MSE of the noisy input, the a-trous
-filter (sgi_denoise), the network alone, and the network with its a-trous
-fallback, and picks the fallback threshold.  Writes the FP32 parameters
+held-out seeds, at every sample count of --spp, reports the relative MSE of
+the noisy input, the a-trous filter (sgi_denoise), the network alone, and the
+network with its a-trous fallback, and picks the fallback threshold.  Writes the FP32 parameters
 (.npy) and a JSON summary beside them.
 """
 from __future__ import annotations
@@ -64,12 +67,15 @@ def random_view(scene: sc.Scene, rng, width: int, height: int | None = None) -> 
 class Sample:
     """One view's network inputs and targets on the device."""
 
-    def __init__(self, feat, rad, target, extra=None):
-        self.feat, self.rad, self.target, self.extra = feat, rad, target, extra
+    def __init__(self, feat, rad, target, extra=None, spp=0):
+        self.feat, self.rad, self.target, self.extra, self.spp = feat, rad, target, extra, spp
 
 
-def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: int, rng, keep_frame=False,
+def make_samples(seed: int, views: int, size: int, num_vpls: int, spp_choices, rng, keep_frame=False,
                  rejection_limit=-1):
+    """`views` random views of one seeded block; each view's gather takes a sample
+    count drawn from spp_choices (one view per entry when keep_frame, all at the
+    same pose, for validation at every count)."""
     import torch
 
     w = sc.Workload(f"city-train-{seed}", _abi.SCENE_CITY, seed, size, size, num_vpls, 64, _abi.TEXEL_F16, 1024, 16,
@@ -77,8 +83,15 @@ def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: in
     scene = sc.Scene(w)
     out = []
     inp = None
+    poses = []
     for _ in range(views):
-        scene.set_view(random_view(scene, rng, size))
+        v = random_view(scene, rng, size)
+        if keep_frame:
+            poses += [(v, s) for s in spp_choices]
+        else:
+            poses.append((v, int(rng.choice(spp_choices))))
+    for v, samples_pp in poses:
+        scene.set_view(v)
         inp = sc.generate_device(scene)
         gi = GiPass(size, size, 16, 64)
         gi.set_frame(inp.frame)
@@ -105,18 +118,21 @@ def make_samples(seed: int, views: int, size: int, num_vpls: int, samples_pp: in
             lit = torch.zeros((size, size, 3), **f32)
             gi.direct_light(g, lit)
             extra = dict(gi=gi, inp=inp, vis=vis, col=col, res=res, lit=lit)
-        out.append(Sample(feat, rad, target.contiguous(), extra))
+        out.append(Sample(feat, rad, target.contiguous(), extra, samples_pp))
     torch.cuda.synchronize()
     return out
 
 
 def evaluate(wts: kpcn.Weights, val, thresholds):
-    """relMSE of noisy / a-trous / network / network + fallback at each threshold, averaged over views."""
+    """relMSE of noisy / a-trous / network / network + fallback at each threshold, averaged over
+    views, and the noisy / a-trous / network relMSE per sample count."""
     import torch
 
     acc = {"noisy": 0.0, "atrous": 0.0, "network": 0.0}
+    per_spp = {}
     fb = {t: [0.0, 0.0] for t in thresholds}
     for s in val:
+        before = dict(acc)
         e = s.extra
         gi, inp = e["gi"], e["inp"]
         g = inp.gbuffer()
@@ -133,6 +149,10 @@ def evaluate(wts: kpcn.Weights, val, thresholds):
         gi.denoise_learned(g, None, e["vis"], e["lit"], e["col"], e["res"], None, None, wts.net(), float("inf"), ws,
                            outs[0], outs[1], outs[2], hist, fbt)
         acc["network"] += rel_mse(outs[1], s.target)
+        d = per_spp.setdefault(str(s.spp), {"views": 0, "noisy": 0.0, "atrous": 0.0, "network": 0.0})
+        d["views"] += 1
+        for k in ("noisy", "atrous", "network"):
+            d[k] += acc[k] - before[k]
         for t in thresholds:
             gi.denoise_learned(g, None, e["vis"], e["lit"], e["col"], e["res"], None, None, wts.net(), t, ws,
                                outs[0], outs[1], outs[2], hist, fbt)
@@ -140,6 +160,7 @@ def evaluate(wts: kpcn.Weights, val, thresholds):
             fb[t][1] += int(fbt[0]) / n_tiles
     n = len(val)
     res = {k: v / n for k, v in acc.items()}
+    res["per_spp"] = {k: {m: (v[m] / v["views"] if m != "views" else v[m]) for m in v} for k, v in per_spp.items()}
     res["fallback"] = {f"{t:g}": {"relmse": v[0] / n, "tiles": v[1] / n} for t, v in fb.items()}
     return res
 
@@ -153,7 +174,8 @@ def main(argv=None) -> None:
     ap.add_argument("--val-scenes", type=int, default=4)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--vpls", type=int, default=4096)
-    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--spp", type=int, nargs="+", default=[4, 8, 16, 32],
+                    help="sample counts of the input gathers (one drawn per training view)")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
