@@ -504,7 +504,7 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     }
     const size_t P = (size_t)H * W;
     const size_t tiles = (size_t)((W + 15) / 16) * ((H + 15) / 16);
-    char* feat = take(P * SGI_KPCN_CHANNELS * 2);
+    char* feat = take(P * 16 * 2);  // the inference features, [h][w][16]
     char* rad0 = take(P * 3 * 4);
     // levels 1..4 of the pyramid contiguous (the layout sgi_kpcn_radiance_pyramid writes)
     char* pyr = take(sgi_kpcn::pyramidFloats(H, W) * 4);
@@ -574,9 +574,9 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
     // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::features(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], s));
+    KPCN_STEP(sgi_kpcn::features16(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], s));
     KPCN_STEP(sgi_kpcn::pyramid(ws.rad[0], H, W, ws.rad[1], s));
-    KPCN_STEP(sgi_kpcn::conv3x3(ws.feat, nullptr, Wl(0), Bl(0), 1, ws.h[0], ws.w[0], ws.act[0][1], 0, s));
+    KPCN_STEP(sgi_kpcn::conv3x3In16(ws.feat, ws.h[0], ws.w[0], Wl(0), Bl(0), ws.act[0][1], s));
     // e_l's convolution also writes p_(l+1), the 2x2 mean of e_l
     KPCN_STEP(sgi_kpcn::convAndPool(ws.act[0][1], ws.h[0], ws.w[0], Wl(1), Bl(1), ws.act[0][0], ws.act[1][2], s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
@@ -602,14 +602,20 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
         KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wl(14 + l), Bl(14 + l), ws.rad[l], ws.o[l + 1],
                                       ws.o[l], l == 0 ? ws.err : nullptr, s));
     }
-    KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
-    // a-trous over the flagged tiles (and their margin), the network elsewhere; without a
-    // threshold no tile can be flagged, and only the merge pass runs (for its normals)
-    KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
-                                         ws.dilated, std::isinf(threshold) && threshold > 0.0f, s));
-    KPCN_STEP(sgi_kpcn::select(ws.o[0], ws.flags, H, W, reservoirOut, s));
-    KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
-                                          combined, giOut, historyOut, ctx->ws, s));
+    if (std::isinf(threshold) && threshold > 0.0f) {
+        // no threshold: no tile can fall back; the temporal pass reads the network's output
+        if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));
+        KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, ctx->frameArgs, screen, color, ws.o[0], prevGi, prevHistory,
+                                                 combined, giOut, reservoirOut, historyOut, s));
+    } else {
+        KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
+        // a-trous over the flagged tiles (and their margin), the network elsewhere
+        KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
+                                             ws.dilated, false, s));
+        KPCN_STEP(sgi_kpcn::select(ws.o[0], ws.flags, H, W, reservoirOut, s));
+        KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
+                                              combined, giOut, historyOut, ctx->ws, s));
+    }
 #undef KPCN_STEP
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise_learned");
 }

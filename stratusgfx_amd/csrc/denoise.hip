@@ -180,21 +180,36 @@ __global__ __launch_bounds__(256) void denoiseFilterViewsKernel(const sgi_batch:
 }
 
 // The final pass's temporal blend and combine (vpl_pbr_gi_denoise.fs:336-434) on
-// the filtered reservoir the last filter pass wrote.
+// the filtered reservoir the last filter pass wrote.  FROM_NET: the filtered
+// value is the learned denoiser's output (net, FP32 [h][w][3]), written back
+// as the output reservoir (net, 1), and the centre's decoded normal and depth
+// come from the G-buffer (the same values the merge pass stores).
+template <bool FROM_NET = false>
 __device__ __forceinline__ void temporalBody(const GbufArgs& g, const GbufArgs& prev, const FrameArgs& fr,
                                              const float4* __restrict__ filtered, const float4* __restrict__ normals,
                                              const float* __restrict__ screen, const float* __restrict__ color,
                                              const float* __restrict__ prevGi, const float* __restrict__ prevHistory,
                                              float* __restrict__ combined, float* __restrict__ giOut,
-                                             float* __restrict__ historyOut, int x, int y) {
+                                             float* __restrict__ historyOut, int x, int y,
+                                             const float* __restrict__ net = nullptr,
+                                             float4* __restrict__ reservoirOut = nullptr) {
     if (x >= g.W || y >= g.H) return;
     const int W = g.W, H = g.H;
     const size_t p = (size_t)y * W + x;
-    const float4 cn4 = normals[p];
-    const f3 centerNormal = f3{cn4.x, cn4.y, cn4.z};
-    const float centerDepth = cn4.w;
-    const float4 sf4 = filtered[p];
-    const f3 shadowFactor = f3{sf4.x, sf4.y, sf4.z};
+    f3 centerNormal, shadowFactor;
+    float centerDepth;
+    if constexpr (FROM_NET) {
+        centerNormal = decodeNormal(g.normal + 3 * p);
+        centerDepth = g.depth[p];
+        shadowFactor = ld3(net + 3 * p);
+        reservoirOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
+    } else {
+        const float4 cn4 = normals[p];
+        centerNormal = f3{cn4.x, cn4.y, cn4.z};
+        centerDepth = cn4.w;
+        const float4 sf4 = filtered[p];
+        shadowFactor = f3{sf4.x, sf4.y, sf4.z};
+    }
     const f3 gi = ld3(color + 3 * p);
     float historyAccum = prevHistory ? prevHistory[p] : 0.0f;
     float accumMultiplier = 1.0f;
@@ -245,6 +260,23 @@ __global__ __launch_bounds__(256) void denoiseTemporalKernel(GbufArgs g, GbufArg
     temporalBody(g, prev, fr, filtered, normals, screen, color, prevGi, prevHistory, combined, giOut, historyOut, x, y);
 }
 
+// The learned denoiser's temporal pass when no tile falls back to the a-trous filter.
+__global__ __launch_bounds__(256) void denoiseTemporalNetKernel(GbufArgs g, GbufArgs prev, FrameArgs fr,
+                                                                const float* __restrict__ net,
+                                                                const float* __restrict__ screen,
+                                                                const float* __restrict__ color,
+                                                                const float* __restrict__ prevGi,
+                                                                const float* __restrict__ prevHistory,
+                                                                float* __restrict__ combined,
+                                                                float* __restrict__ giOut,
+                                                                float4* __restrict__ reservoirOut,
+                                                                float* __restrict__ historyOut) {
+    int x, y;
+    blockPixel(g.W, g.H, x, y);
+    temporalBody<true>(g, prev, fr, nullptr, nullptr, screen, color, prevGi, prevHistory, combined, giOut, historyOut,
+                       x, y, net, reservoirOut);
+}
+
 // Batched views: the previous frame's G-buffer is the current one (a single frame per view).
 __global__ __launch_bounds__(256) void denoiseTemporalViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
                                                                   int numViews, int total) {
@@ -281,6 +313,15 @@ hipError_t denoiseTemporal(const GbufArgs& cur, const GbufArgs& prev, const Fram
     const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
     hipLaunchKernelGGL(denoiseTemporalKernel, grid, block, 0, s, cur, prev, fr, (const float4*)filtered, ws.normals,
                        screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    return hipGetLastError();
+}
+
+hipError_t denoiseTemporalNet(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const float* screen,
+                              const float* color, const float* net, const float* prevGi, const float* prevHistory,
+                              float* combined, float* giOut, float* reservoirOut, float* historyOut, hipStream_t s) {
+    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
+    hipLaunchKernelGGL(denoiseTemporalNetKernel, grid, block, 0, s, cur, prev, fr, net, screen, color, prevGi,
+                       prevHistory, combined, giOut, (float4*)reservoirOut, historyOut);
     return hipGetLastError();
 }
 

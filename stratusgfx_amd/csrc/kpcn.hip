@@ -103,9 +103,14 @@ __device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs
     }
 }
 
-// Network input features and the noisy radiance (the training path's separate pass).
+// Network input features and the noisy radiance.  CH = 32: the [h][w][32]
+// layout of every activation (training, sgi_kpcn_features); CH = 16: the
+// inference form [h][w][16] (channels 16..31 are zero, so the first
+// convolution stages zeros for them instead of reading them).
+template <int CH>
 __global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restrict__ reservoir,
                                uint16_t* __restrict__ feat, float* __restrict__ radiance) {
+    static_assert(CH == 16 || CH == 32, "feature planes hold 16 or 32 channels");
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.W * g.H) return;
     uint32_t o[16];
@@ -114,11 +119,13 @@ __global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restric
     radiance[3 * i + 0] = L[0];
     radiance[3 * i + 1] = L[1];
     radiance[3 * i + 2] = L[2];
-    uint4* dst = reinterpret_cast<uint4*>(feat + (size_t)i * kC);
+    uint4* dst = reinterpret_cast<uint4*>(feat + (size_t)i * CH);
     dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
     dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
-    dst[2] = make_uint4(o[8], o[9], o[10], o[11]);
-    dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
+    if constexpr (CH == 32) {
+        dst[2] = make_uint4(o[8], o[9], o[10], o[11]);
+        dst[3] = make_uint4(o[12], o[13], o[14], o[15]);
+    }
 }
 
 constexpr float kLog2e = 1.44269504088896f;
@@ -235,6 +242,7 @@ __device__ __forceinline__ f3 composePixel(const float* lg, const float* __restr
 
 // Where the convolution's input comes from (staged into the LDS halo):
 //   IN_PLAIN     a [h][w][32] (gated by mask when given);
+//   IN_PLAIN16   a [h][w][16], channels 16..31 staged as zeros;
 //   IN_UPADD     a + up(b): the skip tensor plus the coarser level (nearest);
 //                both chunks are prefetched raw and added when stored to LDS,
 //                so the loads stay in flight across the previous tile's MFMAs.
@@ -251,7 +259,7 @@ __device__ __forceinline__ f3 composePixel(const float* lg, const float* __restr
 //                       with the coarser output when given; out [h][w][3])
 //                       and writes the error logit (err [h][w]).
 // Each fused form computes exactly what the separate kernels compute.
-enum { IN_PLAIN = 0, IN_UPADD = 1 };
+enum { IN_PLAIN = 0, IN_UPADD = 1, IN_PLAIN16 = 2 };
 enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_APPLY = 2, OUT_BF16_POOL = 3 };
 
 struct ConvSrc {
@@ -288,13 +296,14 @@ __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int t
         const int gy = y0 + px / kHS, gx = x0 + px % kHS;
         v[k] = make_uint4(0u, 0u, 0u, 0u);
         if constexpr (IN == IN_UPADD) vb[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (IN == IN_PLAIN16 && c >= 2) continue;
         if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t g = ((size_t)gy * W + gx) * kC + c * 8;
+            const size_t g = ((size_t)gy * W + gx) * (IN == IN_PLAIN16 ? 16 : kC) + c * 8;
             v[k] = *reinterpret_cast<const uint4*>(src.a + g);
             if constexpr (IN == IN_UPADD) {
                 const int w2 = (W + 1) / 2;
                 vb[k] = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
-            } else if (src.mask) {
+            } else if (IN == IN_PLAIN && src.mask) {
                 v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(src.mask + g));
             }
         }
@@ -1008,6 +1017,11 @@ __global__ __launch_bounds__(1024) void countFlagsKernel(const uint8_t* __restri
     if (threadIdx.x == 0) counts[0] = red[0];
 }
 
+__global__ void clearCountsKernel(int n, int32_t* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) counts[i] = 0;
+}
+
 __global__ void dilateKernel(const uint8_t* __restrict__ flags, int tilesX, int tilesY, uint8_t* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tilesX * tilesY) return;
@@ -1054,6 +1068,13 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
     src.mask = mask;
     if (outF32) return launchConv<IN_PLAIN, OUT_F32>(src, w, bias, relu, H, W, out, nullptr, s);
     return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
+}
+
+hipError_t conv3x3In16(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
+                       hipStream_t s) {
+    ConvSrc src{};
+    src.a = in16;
+    return launchConv<IN_PLAIN16, OUT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
 }
 
 hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
@@ -1152,8 +1173,15 @@ hipError_t upBilinearBackward(const float* dup, int H, int W, float* dcoarse, hi
 
 hipError_t features(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, uint16_t* feat, float* radiance,
                     hipStream_t s) {
-    hipLaunchKernelGGL(featuresKernel, dim3(gridFor(g.W * g.H, 256)), dim3(256), 0, s, g, fr,
+    hipLaunchKernelGGL(featuresKernel<32>, dim3(gridFor(g.W * g.H, 256)), dim3(256), 0, s, g, fr,
                        (const float4*)reservoir, feat, radiance);
+    return hipGetLastError();
+}
+
+hipError_t features16(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, uint16_t* feat16,
+                      float* radiance, hipStream_t s) {
+    hipLaunchKernelGGL(featuresKernel<16>, dim3(gridFor(g.W * g.H, 256)), dim3(256), 0, s, g, fr,
+                       (const float4*)reservoir, feat16, radiance);
     return hipGetLastError();
 }
 
@@ -1203,6 +1231,12 @@ hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* f
         e = hipGetLastError();
     }
     return e;
+}
+
+hipError_t clearCounts(int H, int W, int32_t* counts, hipStream_t s) {
+    const int n = ((W + 15) / 16) * ((H + 15) / 16);
+    hipLaunchKernelGGL(clearCountsKernel, dim3(gridFor(n + 1, 256)), dim3(256), 0, s, n, counts);
+    return hipGetLastError();
 }
 
 hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s) {

@@ -71,6 +71,13 @@ hipError_t denoiseTemporal(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs
                            const float* screen, const float* color, const float* filtered, const float* prevGi,
                            const float* prevHistory, float* combined, float* giOut, float* historyOut,
                            const DenoiseBuffers& ws, hipStream_t s);
+// The temporal pass on the learned denoiser's output (net, FP32 [h][w][3]) when
+// no tile falls back: also writes reservoirOut = (net, 1); decodes the centre
+// normals itself (no merge pass).
+hipError_t denoiseTemporalNet(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, const sgi_dev::FrameArgs& fr,
+                              const float* screen, const float* color, const float* net, const float* prevGi,
+                              const float* prevHistory, float* combined, float* giOut, float* reservoirOut,
+                              float* historyOut, hipStream_t s);
 
 }  // namespace sgi_launch
 
@@ -83,6 +90,9 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
 // input staging (upsample + skip) or the epilogue (2x2 pooling of the output;
 // one reconstruction level: the 5x5 filter and the blend with the coarser
 // output); same results as the separate kernels.
+// The first convolution on the inference features' [h][w][16] layout.
+hipError_t conv3x3In16(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
+                       hipStream_t s);
 hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
                        uint16_t* pooled, hipStream_t s);
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
@@ -103,6 +113,9 @@ hipError_t applyBackward(const float* logits, const float* radiance, const float
 hipError_t upBilinearBackward(const float* dup, int H, int W, float* dcoarse, hipStream_t s);
 hipError_t features(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir, uint16_t* feat,
                     float* radiance, hipStream_t s);
+// The same features in the inference layout [h][w][16].
+hipError_t features16(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir,
+                      uint16_t* feat16, float* radiance, hipStream_t s);
 hipError_t apply(const float* logits, const float* radiance, const float* coarse, int H, int W, float* out,
                  float* err, hipStream_t s);
 hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
@@ -112,6 +125,8 @@ hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, 
 hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
 hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, uint8_t* dilated,
                      int32_t* counts, hipStream_t s);
+// counts[0] = 0 and counts[1 + t] = 0 for every tile (no tile falls back).
+hipError_t clearCounts(int H, int W, int32_t* counts, hipStream_t s);
 hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s);
 
 }  // namespace sgi_kpcn

@@ -1,25 +1,27 @@
 """Trains the learned denoiser on pairs the gather itself generates, on the GPU.
 
-    python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --size 512 --steps 4000
+    python -m stratusgfx_amd.train_kpcn --scenes 64 --views 2 --steps 12000 --batch 2
 
-Data: seed-generated city blocks (seeds 1000 + i), random camera poses over
-the block; input = a sampled gather of 4096 VPLs (sgi_gather_ex SAMPLES,
-failing draws skipped, so that more samples converge to the target) at a
-sample count drawn per view from --spp, target = the converged gather over
-every visible VPL with FP64 sums (sgi_gather_ex ALL), both normalised as the
+Data: seed-generated city blocks (seeds 1000 + i) at the benchmark's frame
+size and cube resolution, each block's default pose and random poses over
+it; input = a sampled gather of 4096 VPLs (sgi_gather_ex SAMPLES, failing
+draws skipped, so that more samples converge to the target) at a sample
+count drawn per view from --spp, target = the converged gather over every
+visible VPL with FP64 sums (sgi_gather_ex ALL), both normalised as the
 denoiser's pass 0 does.  Mixing sample counts lets one network serve every
 operating point: it reads the noise level from the input itself.
 Training: Adam on the relative-MSE loss plus the error head's regression
 (sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
 held-out seeds, at every sample count of --spp, reports the relative MSE of
-the noisy input, the a-trous filter (sgi_denoise), the network alone, and the
-network with its a-trous fallback, and picks the fallback threshold.  Writes the FP32 parameters
-(.npy) and a JSON summary beside them.
+the noisy input, the a-trous filter (sgi_denoise), the network alone, and
+the network with its a-trous fallback, and picks the fallback threshold.
+Writes the FP32 parameters (.npy) and a JSON summary beside them.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
+import dataclasses
 import json
 import math
 import os
@@ -71,56 +73,67 @@ class Sample:
         self.feat, self.rad, self.target, self.extra, self.spp = feat, rad, target, extra, spp
 
 
-def make_samples(seed: int, views: int, size: int, num_vpls: int, spp_choices, rng, keep_frame=False,
-                 rejection_limit=-1):
-    """`views` random views of one seeded block; each view's gather takes a sample
-    count drawn from spp_choices (one view per entry when keep_frame, all at the
-    same pose, for validation at every count)."""
+def make_samples(seed: int, views: int, width: int, height: int, num_vpls: int, spp_choices, rng, keep_frame=False,
+                 rejection_limit=-1, cube_res=256, default_view=True):
+    """`views` views of one seeded block (the block's default pose first when
+    default_view, then random poses); each view's gather takes a sample count
+    drawn from spp_choices (every count when keep_frame, for validation)."""
     import torch
 
-    w = sc.Workload(f"city-train-{seed}", _abi.SCENE_CITY, seed, size, size, num_vpls, 64, _abi.TEXEL_F16, 1024, 16,
-                    64)
+    w = sc.Workload(f"city-train-{seed}", _abi.SCENE_CITY, seed, width, height, num_vpls, cube_res, _abi.TEXEL_F16,
+                    1024, 16, 64)
     scene = sc.Scene(w)
     out = []
     inp = None
     poses = []
-    for _ in range(views):
-        v = random_view(scene, rng, size)
-        if keep_frame:
-            poses += [(v, s) for s in spp_choices]
+    for k in range(views):
+        if default_view and k == 0:
+            v = _abi.View()
+            _abi.call(scene.lib, "sgi_scene_default_view", scene.handle, width, height, ctypes.byref(v))
         else:
-            poses.append((v, int(rng.choice(spp_choices))))
-    for v, samples_pp in poses:
+            v = random_view(scene, rng, width, height)
+        poses.append((v, list(spp_choices) if keep_frame else [int(rng.choice(spp_choices))]))
+    for v, counts in poses:
         scene.set_view(v)
         inp = sc.generate_device(scene)
-        gi = GiPass(size, size, 16, 64)
-        gi.set_frame(inp.frame)
-        gi.set_csm(inp.csm_struct())
-        f32 = dict(dtype=torch.float32, device="cuda")
-        n = num_vpls
-        upd = torch.zeros((n, 16), **f32)
-        vis = torch.zeros((1 + n,), dtype=torch.int32, device="cuda")
-        gi.cull_global(inp.vpls, n, upd, vis)
-        ent = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
-        gi.visible_atlas_entries(inp.entries, vis, n, ent)
-        g = inp.gbuffer()
-        col, res = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
-        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=samples_pp,
-                     rejection_limit=rejection_limit)
-        colT, resT = torch.zeros((size, size, 3), **f32), torch.zeros((size, size, 4), **f32)
-        gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, colT, resT, mode=_abi.GATHER_ALL, accumulate_fp64=True)
-        feat = torch.empty((size, size, 32), dtype=torch.bfloat16, device="cuda")
-        rad = torch.empty((size, size, 3), **f32)
-        gi.kpcn_features(g, res, feat, rad)
-        target = resT[..., :3] / resT[..., 3:4].clamp_min(1e-5)
-        extra = None
-        if keep_frame:
-            lit = torch.zeros((size, size, 3), **f32)
-            gi.direct_light(g, lit)
-            extra = dict(gi=gi, inp=inp, vis=vis, col=col, res=res, lit=lit)
-        out.append(Sample(feat, rad, target.contiguous(), extra, samples_pp))
+        for samples_pp in counts:
+            out.append(_sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame))
+        del inp
     torch.cuda.synchronize()
     return out
+
+
+def _sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame):
+    """Network inputs and the converged target of one view at one sample count."""
+    import torch
+
+    gi = GiPass(width, height, 16, 64)
+    gi.set_frame(inp.frame)
+    gi.set_csm(inp.csm_struct())
+    f32 = dict(dtype=torch.float32, device="cuda")
+    n = num_vpls
+    upd = torch.zeros((n, 16), **f32)
+    vis = torch.zeros((1 + n,), dtype=torch.int32, device="cuda")
+    gi.cull_global(inp.vpls, n, upd, vis)
+    ent = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    gi.visible_atlas_entries(inp.entries, vis, n, ent)
+    g = inp.gbuffer()
+    col, res = torch.zeros((height, width, 3), **f32), torch.zeros((height, width, 4), **f32)
+    gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES, samples=samples_pp,
+                 rejection_limit=rejection_limit)
+    colT, resT = torch.zeros((height, width, 3), **f32), torch.zeros((height, width, 4), **f32)
+    gi.gather_ex(g, upd, vis, ent, inp.cubes(), None, colT, resT, mode=_abi.GATHER_ALL, accumulate_fp64=True)
+    feat = torch.empty((height, width, 32), dtype=torch.bfloat16, device="cuda")
+    rad = torch.empty((height, width, 3), **f32)
+    gi.kpcn_features(g, res, feat, rad)
+    target = resT[..., :3] / resT[..., 3:4].clamp_min(1e-5)
+    extra = None
+    if keep_frame:
+        lit = torch.zeros((height, width, 3), **f32)
+        gi.direct_light(g, lit)
+        # the G-buffer and the gather's outputs stay; the cubes are not needed again
+        extra = dict(gi=gi, inp=dataclasses.replace(inp, atlases=[]), vis=vis, col=col, res=res, lit=lit)
+    return Sample(feat, rad, target.contiguous(), extra, samples_pp)
 
 
 def evaluate(wts: kpcn.Weights, val, thresholds):
@@ -136,7 +149,7 @@ def evaluate(wts: kpcn.Weights, val, thresholds):
         e = s.extra
         gi, inp = e["gi"], e["inp"]
         g = inp.gbuffer()
-        H = W = s.rad.shape[0]
+        H, W = s.rad.shape[:2]
         f32 = dict(dtype=torch.float32, device="cuda")
         outs = [torch.zeros((H, W, c), **f32) for c in (3, 3, 4)]
         hist = torch.zeros((H, W), **f32)
@@ -172,7 +185,9 @@ def main(argv=None) -> None:
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--views", type=int, default=2)
     ap.add_argument("--val-scenes", type=int, default=4)
-    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cube-res", type=int, default=256, help="VPL distance-cube resolution of the training scenes")
     ap.add_argument("--vpls", type=int, default=4096)
     ap.add_argument("--spp", type=int, nargs="+", default=[4, 8, 16, 32],
                     help="sample counts of the input gathers (one drawn per training view)")
@@ -191,18 +206,19 @@ def main(argv=None) -> None:
     t0 = time.time()
     train = []
     for i in range(a.scenes):
-        train += make_samples(1000 + i, a.views, a.size, a.vpls, a.spp, rng)
+        train += make_samples(1000 + i, a.views, a.width, a.height, a.vpls, a.spp, rng, cube_res=a.cube_res)
         if i % 8 == 7:
             print(f"[data] {len(train)} training views, {time.time() - t0:.0f} s", flush=True)
     val = []
     for i in range(a.val_scenes):
-        val += make_samples(5000 + i, 1, a.size, a.vpls, a.spp, rng, keep_frame=True)
+        val += make_samples(5000 + i, 1, a.width, a.height, a.vpls, a.spp, rng, keep_frame=True, cube_res=a.cube_res,
+                            default_view=False)
     tmed = float(torch.stack([s.target.median() for s in train]).median())
     print(f"[data] {len(train)} train / {len(val)} val views in {time.time() - t0:.0f} s; median target {tmed:.4g}",
           flush=True)
 
     wts = kpcn.Weights(kpcn.init_params(a.seed))
-    tr = kpcn.Trainer(wts, a.size, a.size)
+    tr = kpcn.Trainer(wts, a.height, a.width)
     base = evaluate(wts, val, [])
     print(f"[val] before training {json.dumps(base)}", flush=True)
     hist = []
@@ -216,7 +232,7 @@ def main(argv=None) -> None:
             tr.loss_backward(s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch)
         tr.step(lr)
         if step % 50 == 0 or step == 1:
-            ls = tr.loss_sums.cpu().numpy() / (a.batch * a.size * a.size)
+            ls = tr.loss_sums.cpu().numpy() / (a.batch * a.width * a.height)
             hist.append({"step": step, "relmse": float(ls[0] / 3.0), "err_loss": float(ls[1]), "lr": lr})
             print(f"[train] step {step} relMSE {ls[0] / 3.0:.5f} err-head {ls[1]:.4f} lr {lr:.2e} "
                   f"{time.time() - t1:.0f} s", flush=True)
@@ -233,7 +249,7 @@ def main(argv=None) -> None:
         if v < best:
             best_t, best = t, v
     kpcn.save_params(wts.params.cpu().numpy(), a.out)
-    summary = {"train_views": len(train), "val_views": len(val), "size": a.size, "vpls": a.vpls, "spp": a.spp,
+    summary = {"train_views": len(train), "val_views": len(val), "width": a.width, "height": a.height, "cube_res": a.cube_res, "vpls": a.vpls, "spp": a.spp,
                "steps": a.steps, "batch": a.batch, "lr": a.lr, "err_weight": a.err_weight,
                "loss_scale": a.loss_scale,
                "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
