@@ -574,8 +574,7 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
     // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::features16(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], s));
-    KPCN_STEP(sgi_kpcn::pyramid(ws.rad[0], H, W, ws.rad[1], s));
+    KPCN_STEP(sgi_kpcn::features16Pyramid(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], ws.rad[1], s));
     KPCN_STEP(sgi_kpcn::conv3x3In16(ws.feat, ws.h[0], ws.w[0], Wl(0), Bl(0), ws.act[0][1], s));
     // e_l's convolution also writes p_(l+1), the 2x2 mean of e_l
     KPCN_STEP(sgi_kpcn::convAndPool(ws.act[0][1], ws.h[0], ws.w[0], Wl(1), Bl(1), ws.act[0][0], ws.act[1][2], s));

@@ -854,19 +854,9 @@ __global__ void upBilinearBackwardKernel(const float* __restrict__ dup, int H, i
 // 4x4, 2x2 and 1x1 pixels of levels 1..4 (a level-l pixel's children all lie
 // in the same tile), each the mean of its in-range children in the order
 // (0,0), (0,1), (1,0), (1,1) as poolKernel sums.
-__global__ __launch_bounds__(256) void pyramidKernel(const float* __restrict__ r0, int H, int W, int tilesX,
-                                                     float* __restrict__ pyr) {
-    __shared__ float buf[2][kTS * kTS * 3];
-    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    {
-        const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
-        if (x < W && y < H) {
-            const f3 v = ld3(r0 + 3 * ((size_t)y * W + x));
-            buf[0][3 * threadIdx.x + 0] = v.x;
-            buf[0][3 * threadIdx.x + 1] = v.y;
-            buf[0][3 * threadIdx.x + 2] = v.z;
-        }
-    }
+// Levels 1..4 of one 16x16 tile's pyramid from its level-0 radiance in buf[0].
+__device__ __forceinline__ void tilePyramid(float (*buf)[kTS * kTS * 3], int tx, int ty, int H, int W,
+                                            float* __restrict__ pyr) {
     int hp = H, wp = W;  // the finer level's size
     size_t off = 0;      // this level's offset in pyr (floats)
     int side = kTS;      // the finer level's tile edge
@@ -906,6 +896,47 @@ __global__ __launch_bounds__(256) void pyramidKernel(const float* __restrict__ r
         wp = w;
         side = s;
     }
+}
+
+__global__ __launch_bounds__(256) void pyramidKernel(const float* __restrict__ r0, int H, int W, int tilesX,
+                                                     float* __restrict__ pyr) {
+    __shared__ float buf[2][kTS * kTS * 3];
+    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
+    if (x < W && y < H) {
+        const f3 v = ld3(r0 + 3 * ((size_t)y * W + x));
+        buf[0][3 * threadIdx.x + 0] = v.x;
+        buf[0][3 * threadIdx.x + 1] = v.y;
+        buf[0][3 * threadIdx.x + 2] = v.z;
+    }
+    tilePyramid(buf, tx, ty, H, W, pyr);
+}
+
+// The inference features ([h][w][16]) and the radiance pyramid in one pass: one
+// 16x16 tile per block, the tile's radiance kept in LDS for its pyramid.
+__global__ __launch_bounds__(256) void featuresPyramidKernel(GbufArgs g, FrameArgs fr,
+                                                             const float4* __restrict__ reservoir, int tilesX,
+                                                             uint16_t* __restrict__ feat16,
+                                                             float* __restrict__ radiance, float* __restrict__ pyr) {
+    __shared__ float buf[2][kTS * kTS * 3];
+    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
+    if (x < g.W && y < g.H) {
+        uint32_t o[16];
+        float L[3];
+        pixelFeatures(g, fr, reservoir, x, y, o, L);
+        const size_t i = (size_t)y * g.W + x;
+        radiance[3 * i + 0] = L[0];
+        radiance[3 * i + 1] = L[1];
+        radiance[3 * i + 2] = L[2];
+        uint4* dst = reinterpret_cast<uint4*>(feat16 + i * 16);
+        dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+        buf[0][3 * threadIdx.x + 0] = L[0];
+        buf[0][3 * threadIdx.x + 1] = L[1];
+        buf[0][3 * threadIdx.x + 2] = L[2];
+    }
+    tilePyramid(buf, tx, ty, g.H, g.W, pyr);
 }
 
 // Loss on the reconstruction: the relative squared error of out and the error
@@ -1055,7 +1086,9 @@ template <int IN, int OUT>
 hipError_t launchConv(const ConvSrc& src, const uint16_t* w, const float* bias, int relu, int H, int W, void* out,
                       float* err, hipStream_t s) {
     const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
-    const int grid = nTiles < 2048 ? nTiles : 2048;
+    // two blocks per CU stay resident (LDS and VGPRs), so 512 blocks cover the 256 CUs once and
+    // each walks a contiguous run of tiles with its next halo in flight
+    const int grid = nTiles < 512 ? nTiles : 512;
     hipLaunchKernelGGL((conv3x3Kernel<IN, OUT>), dim3(grid), dim3(256), 0, s, src, w, bias, relu, H, W, tilesX,
                        nTiles, out, err);
     return hipGetLastError();
@@ -1178,10 +1211,11 @@ hipError_t features(const GbufArgs& g, const FrameArgs& fr, const float* reservo
     return hipGetLastError();
 }
 
-hipError_t features16(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, uint16_t* feat16,
-                      float* radiance, hipStream_t s) {
-    hipLaunchKernelGGL(featuresKernel<16>, dim3(gridFor(g.W * g.H, 256)), dim3(256), 0, s, g, fr,
-                       (const float4*)reservoir, feat16, radiance);
+hipError_t features16Pyramid(const GbufArgs& g, const FrameArgs& fr, const float* reservoir, uint16_t* feat16,
+                             float* radiance, float* pyr, hipStream_t s) {
+    const int tilesX = (g.W + kTS - 1) / kTS, nTiles = tilesX * ((g.H + kTS - 1) / kTS);
+    hipLaunchKernelGGL(featuresPyramidKernel, dim3(nTiles), dim3(256), 0, s, g, fr, (const float4*)reservoir, tilesX,
+                       feat16, radiance, pyr);
     return hipGetLastError();
 }
 

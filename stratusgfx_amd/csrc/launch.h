@@ -113,9 +113,10 @@ hipError_t applyBackward(const float* logits, const float* radiance, const float
 hipError_t upBilinearBackward(const float* dup, int H, int W, float* dcoarse, hipStream_t s);
 hipError_t features(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir, uint16_t* feat,
                     float* radiance, hipStream_t s);
-// The same features in the inference layout [h][w][16].
-hipError_t features16(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir,
-                      uint16_t* feat16, float* radiance, hipStream_t s);
+// The same features in the inference layout [h][w][16], and the radiance
+// pyramid (levels 1..4, the layout of pyramid()) from the same pass.
+hipError_t features16Pyramid(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const float* reservoir,
+                             uint16_t* feat16, float* radiance, float* pyr, hipStream_t s);
 hipError_t apply(const float* logits, const float* radiance, const float* coarse, int H, int W, float* out,
                  float* err, hipStream_t s);
 hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
