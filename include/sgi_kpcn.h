@@ -17,15 +17,15 @@
  * filter (softmax-normalised) of that level's radiance, and one blend logit;
  * the level-0 head also predicts the log relative squared error.
  *
- *   layer  0: features -> c0          (level 0)      layer 1: c0 -> e0
- *   layer 2: pool(e0) -> c2           (level 1)      layer 3: c2 -> e1
- *   layer 4: pool(e1) -> c4           (level 2)      layer 5: c4 -> e2
- *   layer 6: pool(e2) -> c6           (level 3)      layer 7: c6 -> e3
- *   layer 8: pool(e3) -> c8           (level 4)      layer 9: c8 -> e4
- *   layer 10: e3 + up(e4) -> d3       layer 11: e2 + up(d3) -> d2
- *   layer 12: e1 + up(d2) -> d1       layer 13: e0 + up(d1) -> d0
- *   heads (no ReLU, FP32 logits): layer 14: d0 (level 0), 15: d1, 16: d2,
- *   17: d3, 18: e4 (level 4)
+ *   layer 0: features -> e0           (level 0, one convolution at full resolution)
+ *   layer 1: pool(e0) -> c1           (level 1)      layer 2: c1 -> e1
+ *   layer 3: pool(e1) -> c3           (level 2)      layer 4: c3 -> e2
+ *   layer 5: pool(e2) -> c5           (level 3)      layer 6: c5 -> e3
+ *   layer 7: pool(e3) -> c7           (level 4)      layer 8: c7 -> e4
+ *   layer 9: e3 + up(e4) -> d3        layer 10: e2 + up(d3) -> d2
+ *   layer 11: e1 + up(d2) -> d1
+ *   heads (no ReLU, FP32 logits): layer 12: u0 = e0 + up(d1) (level 0),
+ *   13: d1, 14: d2, 15: d3, 16: e4 (level 4)
  *
  * Logit channels: 0..24 filter taps (tap j at offset (j % 5 - 2, j / 5 - 2),
  * clamped to the image), SGI_KPCN_ERR_CHANNEL the error estimate (level 0),
@@ -59,7 +59,7 @@ extern "C" {
 
 #define SGI_KPCN_CHANNELS 32
 #define SGI_KPCN_LEVELS 5
-#define SGI_KPCN_LAYERS 19
+#define SGI_KPCN_LAYERS 17
 #define SGI_KPCN_LAYER_WEIGHTS (9 * 32 * 32)
 #define SGI_KPCN_LAYER_PARAMS (SGI_KPCN_LAYER_WEIGHTS + 32)
 #define SGI_KPCN_PARAMS (SGI_KPCN_LAYERS * SGI_KPCN_LAYER_PARAMS)

@@ -20,11 +20,11 @@ import numpy as np
 
 C = 32
 LEVELS = 5
-LAYERS = 19
+LAYERS = 17
 TAPS = 25
 ERR = TAPS       # level-0 logit channel of the predicted log relative squared error
 BLEND = TAPS + 1  # logit channel of the blend with the coarser level's output
-HEADS = (14, 15, 16, 17, 18)  # the filter-predicting layer of levels 0..4
+HEADS = (12, 13, 14, 15, 16)  # the filter-predicting layer of levels 0..4
 LAYER_WEIGHTS = 9 * C * C
 LAYER_PARAMS = LAYER_WEIGHTS + C
 PARAMS = LAYERS * LAYER_PARAMS
@@ -275,32 +275,37 @@ class Net:
 
     def forward(self, feat):
         a = {"feat": feat}
-        a["c0"] = self._conv(0, feat)
-        a["e0"] = self._conv(1, a["c0"])
+        a["e0"] = self._conv(0, feat)
         for l in range(1, LEVELS):
             a[f"p{l}"] = self._r(pool(a[f"e{l - 1}"]))
-            a[f"c{l}"] = self._conv(2 * l, a[f"p{l}"])
-            a[f"e{l}"] = self._conv(2 * l + 1, a[f"c{l}"])
+            a[f"c{l}"] = self._conv(2 * l - 1, a[f"p{l}"])
+            a[f"e{l}"] = self._conv(2 * l, a[f"c{l}"])
         low = a[f"e{LEVELS - 1}"]
-        for l, layer in zip(range(LEVELS - 2, -1, -1), range(10, 14)):
+        for l, layer in zip(range(LEVELS - 2, 0, -1), range(9, 12)):
             e = a[f"e{l}"]
             a[f"u{l}"] = self._r(e + upsample(low, *e.shape[:2]))
             a[f"d{l}"] = self._conv(layer, a[f"u{l}"])
             low = a[f"d{l}"]
-        # the filter heads: level l reads d_l (e4 at the coarsest level)
+        a["u0"] = self._r(a["e0"] + upsample(low, *a["e0"].shape[:2]))
+        # the filter heads: level 0 reads u0, levels 1..3 d_l, level 4 e4
         a["logits"] = []
         for l in range(LEVELS):
-            src = a[f"d{l}"] if l < LEVELS - 1 else a[f"e{LEVELS - 1}"]
-            lg = conv3x3(src, self.W[HEADS[l]], self.b[HEADS[l]])
+            lg = conv3x3(self.head_input_of(a, l), self.W[HEADS[l]], self.b[HEADS[l]])
             a["logits"].append(lg.astype(np.float32).astype(np.float64) if self.bf16 else lg)
         self.a = a
         return a["logits"]
 
+    @staticmethod
+    def head_input_of(a, l):
+        if l == 0:
+            return a["u0"]
+        return a[f"d{l}"] if l < LEVELS - 1 else a[f"e{LEVELS - 1}"]
+
     def head_input(self, l):
-        return self.a[f"d{l}"] if l < LEVELS - 1 else self.a[f"e{LEVELS - 1}"]
+        return self.head_input_of(self.a, l)
 
     def backward(self, dlogits):
-        """Gradient of every layer's parameters [19][9248] for the per-level logits' gradients."""
+        """Gradient of every layer's parameters [17][9248] for the per-level logits' gradients."""
         a, r = self.a, self._r
         grads = np.zeros((LAYERS, LAYER_PARAMS))
 
@@ -311,22 +316,21 @@ class Net:
             grads[l, LAYER_WEIGHTS:] = db
             return r(dx) if need_dx else None
 
-        # the heads' input gradients (layer outputs, stored BF16)
+        # the heads' input gradients (layer outputs, stored BF16); u0's is also e0's skip gradient
         dh = [layer_back(HEADS[l], self.head_input(l), None, r(dlogits[l]), relu=False) for l in range(LEVELS)]
-        g = dh[0]
-        de = {}
-        for l, layer in zip(range(0, LEVELS - 1), range(13, 9, -1)):
+        de = {0: dh[0]}
+        g = r(dh[1] + upsample_backward(dh[0]))  # the gradient of d1
+        for l, layer in zip(range(1, LEVELS - 1), range(11, 8, -1)):
             du = layer_back(layer, a[f"u{l}"], a[f"d{l}"], g)
             de[l] = du
             # d_{l+1} (e4 at the last step) feeds the decoder and its own head
             g = r(dh[l + 1] + upsample_backward(du))
         # g is now the gradient of e4 from the decoder and its head
         for l in range(LEVELS - 1, 0, -1):
-            gc = layer_back(2 * l + 1, a[f"c{l}"], a[f"e{l}"], g)
-            gp = layer_back(2 * l, a[f"p{l}"], a[f"c{l}"], gc)
+            gc = layer_back(2 * l, a[f"c{l}"], a[f"e{l}"], g)
+            gp = layer_back(2 * l - 1, a[f"p{l}"], a[f"c{l}"], gc)
             H, W = a[f"e{l - 1}"].shape[:2]
             g = r(de[l - 1] + pool_backward(gp, H, W))
-        gc = layer_back(1, a["c0"], a["e0"], g)
-        layer_back(0, a["feat"], a["c0"], gc, need_dx=False)
+        layer_back(0, a["feat"], a["e0"], g, need_dx=False)
         return grads
 

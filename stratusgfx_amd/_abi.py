@@ -36,7 +36,7 @@ SCENE_CITY = 1
 VPL_FLOATS = 16  # sgi_vpl is 16 floats (64 B)
 KPCN_CHANNELS = 32
 KPCN_LEVELS = 5
-KPCN_LAYERS = 19
+KPCN_LAYERS = 17
 KPCN_LAYER_WEIGHTS = 9 * 32 * 32
 KPCN_LAYER_PARAMS = KPCN_LAYER_WEIGHTS + 32
 KPCN_PARAMS = KPCN_LAYERS * KPCN_LAYER_PARAMS

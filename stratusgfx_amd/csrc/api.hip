@@ -471,12 +471,12 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 
 namespace {
 
-// The inference workspace: per level l two activation planes (e_l and a
-// temporary for c_l / d_l) and, below level 0, the pooled input p_l; then the
+// The inference workspace: per level l the activation plane e_l and, below
+// level 0, a temporary for c_l / d_l and the pooled input p_l; then the
 // features, the radiance pyramid, the reconstruction of every level, the
 // error logit and the tile flags.
 struct KpcnWs {
-    uint16_t* act[SGI_KPCN_LEVELS][3];  // [l][2]: the pooled input p_l of level l >= 1
+    uint16_t* act[SGI_KPCN_LEVELS][3];  // [l][0] e_l, [l][1] c_l / d_l, [l][2] p_l (l >= 1)
     uint16_t* feat;
     float* rad[SGI_KPCN_LEVELS];  // the radiance pyramid (level 0 = the noisy radiance)
     float* o[SGI_KPCN_LEVELS];    // the reconstruction per level (level 0 = the output)
@@ -496,7 +496,7 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     for (int l = 0; l < SGI_KPCN_LEVELS; ++l) {
         if (ws) ws->h[l] = h, ws->w[l] = w;
         for (int k = 0; k < 3; ++k) {
-            char* p = (k < 2 || l > 0) ? take((size_t)h * w * SGI_KPCN_CHANNELS * 2) : nullptr;
+            char* p = (k == 0 || l > 0) ? take((size_t)h * w * SGI_KPCN_CHANNELS * 2) : nullptr;
             if (ws) ws->act[l][k] = (uint16_t*)p;
         }
         h = (h + 1) / 2;
@@ -575,32 +575,36 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     if (e == hipSuccess) e = (call)
     // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
     KPCN_STEP(sgi_kpcn::features16Pyramid(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], ws.rad[1], s));
-    KPCN_STEP(sgi_kpcn::conv3x3In16(ws.feat, ws.h[0], ws.w[0], Wl(0), Bl(0), ws.act[0][1], s));
-    // e_l's convolution also writes p_(l+1), the 2x2 mean of e_l
-    KPCN_STEP(sgi_kpcn::convAndPool(ws.act[0][1], ws.h[0], ws.w[0], Wl(1), Bl(1), ws.act[0][0], ws.act[1][2], s));
+    // layer 0 (features -> e0) also writes p1, the 2x2 mean of e0; likewise each level's second
+    // convolution writes the next level's input
+    KPCN_STEP(sgi_kpcn::conv3x3In16AndPool(ws.feat, ws.h[0], ws.w[0], Wl(0), Bl(0), ws.act[0][0], ws.act[1][2], s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][1], 0,
-                                    s));
+        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l - 1), Bl(2 * l - 1), 1, ws.h[l], ws.w[l],
+                                    ws.act[l][1], 0, s));
         if (l + 1 < SGI_KPCN_LEVELS) {
-            KPCN_STEP(sgi_kpcn::convAndPool(ws.act[l][1], ws.h[l], ws.w[l], Wl(2 * l + 1), Bl(2 * l + 1), ws.act[l][0],
+            KPCN_STEP(sgi_kpcn::convAndPool(ws.act[l][1], ws.h[l], ws.w[l], Wl(2 * l), Bl(2 * l), ws.act[l][0],
                                             ws.act[l + 1][2], s));
         } else {
-            KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l + 1), Bl(2 * l + 1), 1, ws.h[l], ws.w[l],
-                                        ws.act[l][0], 0, s));
+            KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][0],
+                                        0, s));
         }
     }
-    // decoder: d_l = conv(e_l + up(d_{l+1})) with d_4 = e_4; d_l lands in act[l][1]; each
-    // level's head runs the level's reconstruction step in its epilogue, coarse to fine
+    // decoder: d_l = conv(e_l + up(d_{l+1})) for l = 3..1 with d_4 = e_4 (d_l lands in act[l][1]);
+    // each level's head runs the level's reconstruction step in its epilogue, coarse to fine;
+    // the level-0 head reads e0 + up(d1) directly
     constexpr int L4 = SGI_KPCN_LEVELS - 1;
-    KPCN_STEP(sgi_kpcn::convApply(ws.act[L4][0], ws.h[L4], ws.w[L4], Wl(14 + L4), Bl(14 + L4), ws.rad[L4], nullptr,
+    auto head = [](int l) { return 12 + l; };
+    KPCN_STEP(sgi_kpcn::convApply(ws.act[L4][0], ws.h[L4], ws.w[L4], Wl(head(L4)), Bl(head(L4)), ws.rad[L4], nullptr,
                                   ws.o[L4], nullptr, s));
     const uint16_t* low = ws.act[L4][0];
-    for (int l = L4 - 1, layer = 10; l >= 0; --l, ++layer) {
+    for (int l = L4 - 1, layer = 9; l >= 1; --l, ++layer) {
         KPCN_STEP(sgi_kpcn::convUpAdd(ws.act[l][0], low, ws.h[l], ws.w[l], Wl(layer), Bl(layer), ws.act[l][1], s));
         low = ws.act[l][1];
-        KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wl(14 + l), Bl(14 + l), ws.rad[l], ws.o[l + 1],
-                                      ws.o[l], l == 0 ? ws.err : nullptr, s));
+        KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wl(head(l)), Bl(head(l)), ws.rad[l],
+                                      ws.o[l + 1], ws.o[l], nullptr, s));
     }
+    KPCN_STEP(sgi_kpcn::convUpAddApply(ws.act[0][0], low, H, W, Wl(head(0)), Bl(head(0)), ws.rad[0], ws.o[1], ws.o[0],
+                                       ws.err, s));
     if (std::isinf(threshold) && threshold > 0.0f) {
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));

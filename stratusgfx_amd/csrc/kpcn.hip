@@ -285,7 +285,9 @@ __device__ __forceinline__ uint4 addChunk(uint4 a, uint4 b) {
 // chunk i = 8 channels (c = i % 4) of halo pixel i / 4.
 constexpr int kStageSlots = (kHS * kHS * 4 + 255) / 256;
 
-template <int IN>
+// DEFER (IN_UPADD only): keep the coarser operand raw in vb, added at the LDS
+// store; otherwise it is added here (fewer registers live across the MFMAs).
+template <int IN, bool DEFER>
 __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v,
                                          uint4* vb) {
     const int x0 = (tile % tilesX) * kTS - 1, y0 = (tile / tilesX) * kTS - 1;
@@ -295,14 +297,18 @@ __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int t
         const int px = i >> 2, c = i & 3;
         const int gy = y0 + px / kHS, gx = x0 + px % kHS;
         v[k] = make_uint4(0u, 0u, 0u, 0u);
-        if constexpr (IN == IN_UPADD) vb[k] = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (IN == IN_UPADD && DEFER) vb[k] = make_uint4(0u, 0u, 0u, 0u);
         if (IN == IN_PLAIN16 && c >= 2) continue;
         if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const size_t g = ((size_t)gy * W + gx) * (IN == IN_PLAIN16 ? 16 : kC) + c * 8;
             v[k] = *reinterpret_cast<const uint4*>(src.a + g);
             if constexpr (IN == IN_UPADD) {
                 const int w2 = (W + 1) / 2;
-                vb[k] = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+                const uint4 b = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+                if constexpr (DEFER)
+                    vb[k] = b;
+                else
+                    v[k] = addChunk(v[k], b);
             } else if (IN == IN_PLAIN && src.mask) {
                 v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(src.mask + g));
             }
@@ -361,8 +367,10 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 
     int t0, t1;
     blockTiles(nTiles, t0, t1);
-    uint4 pre[kStageSlots], preB[IN == IN_UPADD ? kStageSlots : 1];
-    if (t0 < t1) loadHalo<IN>(src, H, W, t0, tilesX, pre, preB);
+    // the reconstruction epilogue needs the registers a deferred add would hold
+    constexpr bool kDefer = IN == IN_UPADD && OUT != OUT_APPLY;
+    uint4 pre[kStageSlots], preB[kDefer ? kStageSlots : 1];
+    if (t0 < t1) loadHalo<IN, kDefer>(src, H, W, t0, tilesX, pre, preB);
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
         __syncthreads();
@@ -371,14 +379,14 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         for (int k = 0; k < kStageSlots; ++k) {
             const int i = (int)threadIdx.x + k * 256;
             if (i < kHS * kHS * 4) {
-                if constexpr (IN == IN_UPADD)
+                if constexpr (kDefer)
                     halo[(i >> 2) * kPxU4 + (i & 3)] = addChunk(pre[k], preB[k]);
                 else
                     halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
             }
         }
         __syncthreads();
-        if (tile + 1 < t1) loadHalo<IN>(src, H, W, tile + 1, tilesX, pre, preB);
+        if (tile + 1 < t1) loadHalo<IN, kDefer>(src, H, W, tile + 1, tilesX, pre, preB);
         // OUT_APPLY: this tile's radiance (2-pixel margin) and coarser output (1-pixel margin),
         // clamped to the image as the global taps clamp, loaded straight into LDS
         // (global_load_lds: lane l of a wave writes dword l of the wave's 64-dword run)
@@ -1103,11 +1111,23 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
     return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
 }
 
-hipError_t conv3x3In16(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
-                       hipStream_t s) {
+hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias,
+                              uint16_t* out, uint16_t* pooled, hipStream_t s) {
     ConvSrc src{};
     src.a = in16;
-    return launchConv<IN_PLAIN16, OUT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
+    src.pooled = pooled;
+    return launchConv<IN_PLAIN16, OUT_BF16_POOL>(src, w, bias, 1, H, W, out, nullptr, s);
+}
+
+hipError_t convUpAddApply(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w,
+                          const float* bias, const float* radiance, const float* coarse, float* out, float* err,
+                          hipStream_t s) {
+    ConvSrc src{};
+    src.a = skip;
+    src.b = low;
+    src.radiance = radiance;
+    src.coarse = coarse;
+    return launchConv<IN_UPADD, OUT_APPLY>(src, w, bias, 0, H, W, out, err, s);
 }
 
 hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,

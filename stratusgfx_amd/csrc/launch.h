@@ -90,9 +90,15 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
 // input staging (upsample + skip) or the epilogue (2x2 pooling of the output;
 // one reconstruction level: the 5x5 filter and the blend with the coarser
 // output); same results as the separate kernels.
-// The first convolution on the inference features' [h][w][16] layout.
-hipError_t conv3x3In16(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
-                       hipStream_t s);
+// The first convolution on the inference features' [h][w][16] layout, with the
+// 2x2 mean of its output for the next level.
+hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias,
+                              uint16_t* out, uint16_t* pooled, hipStream_t s);
+// The level-0 head: its input e0 + up(d1) staged as convUpAdd stages it, its
+// epilogue the reconstruction step as convApply's.
+hipError_t convUpAddApply(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w,
+                          const float* bias, const float* radiance, const float* coarse, float* out, float* err,
+                          hipStream_t s);
 hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
                        uint16_t* pooled, hipStream_t s);
 hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,

@@ -20,7 +20,8 @@ LAYERS = _abi.KPCN_LAYERS
 LAYER_WEIGHTS = _abi.KPCN_LAYER_WEIGHTS
 LAYER_PARAMS = _abi.KPCN_LAYER_PARAMS
 PARAMS = _abi.KPCN_PARAMS
-HEADS = tuple(14 + l for l in range(LEVELS))  # the filter head of level l reads d_l (e_4 at level 4)
+# the filter head of level l reads u_0 = e_0 + up(d_1) at level 0, d_l at levels 1..3 and e_4 at level 4
+HEADS = tuple(12 + l for l in range(LEVELS))
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "kpcn_city.npy")
@@ -111,10 +112,10 @@ class Trainer:
             return torch.empty((h, w, C), **bf)
 
         self.a = {}
-        self.a["c0"], self.a["e0"] = act(0), act(0)
+        self.a["e0"], self.a["u0"] = act(0), act(0)
         for l in range(1, LEVELS):
             self.a[f"p{l}"], self.a[f"c{l}"], self.a[f"e{l}"] = act(l), act(l), act(l)
-        for l in range(LEVELS - 1):
+        for l in range(1, LEVELS - 1):
             self.a[f"u{l}"], self.a[f"d{l}"] = act(l), act(l)
         # gradient buffers: two per level (the gradient of a tensor and of its producer's input)
         self.g = [[act(l), act(l)] for l in range(LEVELS)]
@@ -151,6 +152,8 @@ class Trainer:
                   1 if relu else 0, h, w, _p(out), 1 if out_f32 else 0, stream)
 
     def head_input(self, l):
+        if l == 0:
+            return self.a["u0"]
         return self.a[f"d{l}"] if l < LEVELS - 1 else self.a[f"e{LEVELS - 1}"]
 
     def forward(self, feat, radiance, stream=0):
@@ -158,19 +161,20 @@ class Trainer:
         a = self.a
         self.feat = feat
         self.rad[0] = radiance
-        self._conv(0, feat, a["c0"], 0, stream=stream)
-        self._conv(1, a["c0"], a["e0"], 0, stream=stream)
+        self._conv(0, feat, a["e0"], 0, stream=stream)
         for l in range(1, LEVELS):
             h, w = self.sizes[l - 1]
             _abi.call(self.lib, "sgi_kpcn_pool", _p(a[f"e{l - 1}"]), h, w, _p(a[f"p{l}"]), stream)
-            self._conv(2 * l, a[f"p{l}"], a[f"c{l}"], l, stream=stream)
-            self._conv(2 * l + 1, a[f"c{l}"], a[f"e{l}"], l, stream=stream)
+            self._conv(2 * l - 1, a[f"p{l}"], a[f"c{l}"], l, stream=stream)
+            self._conv(2 * l, a[f"c{l}"], a[f"e{l}"], l, stream=stream)
         low = a[f"e{LEVELS - 1}"]
-        for l, layer in zip(range(LEVELS - 2, -1, -1), range(10, 14)):
+        for l, layer in zip(range(LEVELS - 2, 0, -1), range(9, 12)):
             h, w = self.sizes[l]
             _abi.call(self.lib, "sgi_kpcn_upsample_add", _p(a[f"e{l}"]), _p(low), h, w, _p(a[f"u{l}"]), stream)
             self._conv(layer, a[f"u{l}"], a[f"d{l}"], l, stream=stream)
             low = a[f"d{l}"]
+        h, w = self.sizes[0]
+        _abi.call(self.lib, "sgi_kpcn_upsample_add", _p(a["e0"]), _p(low), h, w, _p(a["u0"]), stream)
         for l in range(LEVELS):
             self._conv(HEADS[l], self.head_input(l), self.logits[l], l, relu=False, out_f32=True, stream=stream)
         _abi.call(self.lib, "sgi_kpcn_radiance_pyramid", _p(radiance), self.H, self.W, _p(self.pyr), stream)
@@ -208,11 +212,15 @@ class Trainer:
             if coarse is not None:
                 _abi.call(self.lib, "sgi_kpcn_upsample_bilinear_backward", _p(self.dup[l]), h, w,
                           _p(self.do[l + 1]), stream)
-        # the heads (no ReLU): d_l's gradient from its head lands in g[l][0]
-        for l in range(LEVELS):
+        # the heads (no ReLU): u0's gradient (also e0's skip gradient) lands in gd[0], d_l's
+        # (e4's) from its head in g[l][0]
+        self._layer_back(HEADS[0], self.head_input(0), None, self.dlogits[0], self.gd[0], 0, stream=stream)
+        for l in range(1, LEVELS):
             self._layer_back(HEADS[l], self.head_input(l), None, self.dlogits[l], g[l][0], l, stream=stream)
-        cur = g[0][0]  # gradient of d0
-        for l, layer in zip(range(0, LEVELS - 1), range(13, 9, -1)):
+        h, w = self.sizes[0]
+        _abi.call(self.lib, "sgi_kpcn_upsample_backward", _p(self.gd[0]), h, w, _p(g[1][0]), 1, stream)
+        cur = g[1][0]  # gradient of d1
+        for l, layer in zip(range(1, LEVELS - 1), range(11, 8, -1)):
             # gradient of u_l, which is also the skip gradient of e_l
             self._layer_back(layer, a[f"u{l}"], a[f"d{l}"], cur, self.gd[l], l, stream=stream)
             h, w = self.sizes[l]
@@ -221,14 +229,13 @@ class Trainer:
             cur = nxt  # gradient of d_{l+1} (or e4 when l + 1 == 4)
         for l in range(LEVELS - 1, 0, -1):
             # cur = total gradient of e_l
-            self._layer_back(2 * l + 1, a[f"c{l}"], a[f"e{l}"], cur, g[l][1], l, stream=stream)
-            self._layer_back(2 * l, a[f"p{l}"], a[f"c{l}"], g[l][1], g[l][0], l, stream=stream)
+            self._layer_back(2 * l, a[f"c{l}"], a[f"e{l}"], cur, g[l][1], l, stream=stream)
+            self._layer_back(2 * l - 1, a[f"p{l}"], a[f"c{l}"], g[l][1], g[l][0], l, stream=stream)
             h, w = self.sizes[l - 1]
             # e_{l-1}'s gradient: the decoder's skip gradient plus the pooled path
             _abi.call(self.lib, "sgi_kpcn_pool_backward", _p(g[l][0]), h, w, _p(self.gd[l - 1]), 1, stream)
             cur = self.gd[l - 1]
-        self._layer_back(1, a["c0"], a["e0"], cur, g[0][1], 0, stream=stream)
-        self._layer_back(0, self.feat, a["c0"], g[0][1], None, 0, need_dx=False, stream=stream)
+        self._layer_back(0, self.feat, a["e0"], cur, None, 0, need_dx=False, stream=stream)
 
     def zero_grad(self):
         self.grad.zero_()

@@ -6,11 +6,15 @@ oracle (oracle/kpcn_oracle.py, pinned by finite differences).
 Bounds: BF16-stored outputs within one BF16 rounding of the oracle evaluated
 on the same BF16 operands (|d| <= 2^-7 |ref| + 1e-6); FP32 weight gradients
 within rel-L2 1e-5 (exact BF16 products, FP32 sums in another order); the
-whole network's gradients, which pass through 19 layers of BF16 activations
+whole network's gradients, which pass through 17 layers of BF16 activations
 and gradients (a rounding that lands on the other side of a tie in one layer
-moves every later layer's gradient), closer to the BF16-emulating oracle than
-BF16 rounding itself moves them (the oracle's BF16 vs FP64 distance, 2e-3 to
-1.3e-1 per layer here) and at cosine >= 0.995."""
+moves every later layer's gradient), within 1.5x of the distance BF16
+rounding itself puts between the BF16-emulating oracle and its FP64 form
+(2e-3 to 9e-2 per layer here), at cosine >= 0.995 per layer and 0.999 over
+all parameters.  The 1.5x: FP32 instead of FP64 accumulation alone, run
+through the oracle on these inputs, moves the logits by 6e-4 .. 2.6e-3 and
+flips up to a quarter of the coarsest level's BF16 activations, as large as
+the GPU's own distance from the oracle."""
 import ctypes
 
 import numpy as np
@@ -305,9 +309,12 @@ def test_network_gradients_match_oracle(lib):
         noise = np.linalg.norm(b - want64[l]) / np.linalg.norm(want64[l])
         cos = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
         print(f"layer {l:2d} grad rel-L2 {e:.2e} (BF16 vs FP64 oracle {noise:.2e}) cos {cos:.6f}")
-        if not (e <= max(noise, 2e-3) and cos >= 0.995):
+        if not (e <= 1.5 * max(noise, 2e-3) and cos >= 0.995):
             bad.append(l)
     assert not bad, f"layers {bad}"
+    cos_all = (got.ravel() @ want.ravel()) / (np.linalg.norm(got) * np.linalg.norm(want))
+    print(f"all layers cos {cos_all:.6f}")
+    assert cos_all >= 0.999
 
 
 def test_denoise_learned_threshold_semantics(lib):
