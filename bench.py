@@ -109,6 +109,13 @@ def gather_algorithmic_bytes(tiles_np, tile_size, W, H, K, texel_bytes):
     return int(total), int((counts * pix).sum())
 
 
+def samples_algorithmic_bytes(W, H, S, visible, texel_bytes):
+    """Algorithmic HBM bytes of one sampled-gather launch (sgi_gather_ex SAMPLES, S draws per pixel):
+    per pixel the 68 B of the tile-list gather (G-buffer in, colour and reservoir out), per draw one
+    cube texel; the visible VPLs' 48 B packed records once."""
+    return int(W * H * (68 + S * texel_bytes) + visible * 48)
+
+
 def load_traffic(name):
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
@@ -240,7 +247,9 @@ class FormState:
 
 
 def conv_roofline(net_w, H, W, reps=20):
-    """HIP-event timing of the level-0 32 -> 32 convolution of the learned denoiser (layer 1) at the frame size."""
+    """HIP-event timing of the learned denoiser's convolution building block (sgi_kpcn_conv3x3, 32 -> 32
+    channels, BF16) at the full frame size: the kernel the network's full-resolution layers run with
+    their fused staging and epilogues."""
     import torch
 
     from stratusgfx_amd import _abi, kpcn
@@ -262,7 +271,7 @@ def conv_roofline(net_w, H, W, reps=20):
     alg_bytes = P * 32 * 2 * 2 + kpcn.LAYER_WEIGHTS * 2 + 32 * 4
     flops = 2 * 9 * 32 * 32 * P
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    return {"kernel": "conv3x3Kernel<bf16 out> (level 0, 32->32, 1920x1080)", "bound": "hbm",
+    return {"kernel": "conv3x3Kernel<plain in, bf16 out> (32->32, 1920x1080)", "bound": "hbm",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("conv_city1080p_traffic.json"),
             "algorithmic_bytes": alg_bytes, "flops": flops, "avg_launch_ms": round(ms, 4),
@@ -325,6 +334,7 @@ def main():
     w_plain = dataclasses.replace(sc.CITY_1080P, name="city-1920x1080-4096vpl", num_vpls=4096)
     plain = FormState(w_plain)
     learned = FormState(w_learned)
+    texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
     H, W = w_plain.height, w_plain.width
     f32 = dict(dtype=torch.float32, device="cuda")
 
@@ -500,7 +510,6 @@ def main():
         learned_res.update(atrous_at_equal(sweep, learned_res))
         learned_1024, _ = learned_form(learned, "1024 VPLs", S_op, args.sweep_steps, 2)
 
-    texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
     gather_roof = None
     if use_tiles and per_plain is not None:
         alg_bytes, pairs = gather_algorithmic_bytes(tiles_np, w_plain.tile_size, W, H, w_plain.max_vpls_per_tile,
@@ -521,12 +530,24 @@ def main():
                            zip(["cull_global", "direct_light", "cull_tiles", "gather", "denoise"], per_plain)},
             "vpls": w_plain.num_vpls, "visible_vpls": visible_plain, "gather": args.gather,
         }
+    conv_roof = None
     if learned_res is not None:
         value, el, form = learned_res["frames_per_s"], el_learned, "learned"
         if plain_res is not None:
             learned_res["speedup_vs_tile_list_plain"] = round(
                 learned_res["frames_per_s"] / plain_res["frames_per_s"], 3)
-        roofline = conv_roofline(wts, H, W)
+        # the learned frame's dominant kernel: the sampled gather at the operating point
+        S = learned_res["samples_per_pixel"]
+        alg = samples_algorithmic_bytes(W, H, S, learned_res["visible_vpls"], texel_bytes)
+        g_ms = learned_res["kernels_ms"]["gather"]
+        achieved = alg / (g_ms * 1e-3) / 1e9
+        roofline = {"kernel": f"gatherSamplesKernel<F16> ({S} draws per pixel)", "bound": "hbm",
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": load_traffic(f"gather_samples{S}_city1080p_traffic.json"),
+                    "algorithmic_bytes": alg, "draws": W * H * S, "avg_launch_ms": g_ms,
+                    "note": "VALU-bound (the BRDF and shadow test per draw), HBM fraction low by construction"}
+        conv_roof = conv_roofline(wts, H, W)
     elif plain_res is not None:
         value, el, form = plain_res["frames_per_s"], el_plain, "tile-list plain form only"
         roofline = gather_roof
@@ -557,6 +578,7 @@ def main():
         "atrous_sweep": sweep,
         "plain_tile_lists": plain_res,
         "roofline": roofline,
+        "roofline_network_conv": conv_roof,
         "roofline_plain_gather": gather_roof,
         "batched_views": None,
         "cpu_baseline": None,

@@ -230,7 +230,8 @@ enum {
 typedef struct sgi_gather_opts {
     int32_t mode;             /* SGI_GATHER_* */
     int32_t samples;          /* SGI_GATHER_SAMPLES: draws per pixel (1..4096) */
-    int32_t precision;        /* SGI_PRECISION_* of the shading */
+    int32_t precision;        /* SGI_PRECISION_* of the shading (FP16: SGI_GATHER_SAMPLES with
+                                 rejection_limit < 0 only, draws shaded in pairs) */
     int32_t accumulate_fp64;  /* SGI_GATHER_ALL: sum C and w in FP64 */
     int32_t rejection_limit;  /* SGI_GATHER_SAMPLES, >= 0: a draw failing the side / radius
                                  test is drawn again, and after this many rejections draws are

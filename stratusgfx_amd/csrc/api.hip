@@ -232,12 +232,17 @@ int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const
         return fail(SGI_E_INVALID, "sgi_gather_ex: unknown mode");
     if (opts->mode == SGI_GATHER_SAMPLES && (opts->samples < 1 || opts->samples > 4096))
         return fail(SGI_E_INVALID, "sgi_gather_ex: samples must be within 1..4096");
-    if (opts->precision != SGI_PRECISION_FP32)
-        return fail(SGI_E_INVALID, "sgi_gather_ex: these modes shade in FP32");
+    const bool half = opts->precision == SGI_PRECISION_FP16;
+    if (opts->precision != SGI_PRECISION_FP32 && !half)
+        return fail(SGI_E_INVALID, "sgi_gather_ex: unknown precision");
+    if (half && !(opts->mode == SGI_GATHER_SAMPLES && opts->rejection_limit < 0))
+        return fail(SGI_E_INVALID, "sgi_gather_ex: FP16 shading is for the fixed-count SAMPLES mode "
+                                   "(rejection_limit < 0); SGI_GATHER_ALL shades in FP32");
     sgi_dev::CubeArgs ca;
     if ((r = cubeArgs(cubes, "sgi_gather_ex", ca)) != SGI_OK) return r;
     const hipError_t e = sgi_launch::gatherEx(sgi_dev::gbufArgs(*g), ctx->frameArgs, ca, cubes->format, opts->mode,
-                                              opts->samples, opts->rejection_limit, opts->accumulate_fp64, vpls,
+                                              opts->samples, opts->rejection_limit, opts->accumulate_fp64,
+                                              half ? 1 : 0, vpls,
                                               visibleIndices,
                                               shadowIndices, ctx->vplPrep, color, reservoir, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gather_ex");

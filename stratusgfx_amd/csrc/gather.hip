@@ -578,7 +578,10 @@ __global__ __launch_bounds__(256) void gatherAllKernel(GbufArgs g, FrameArgs fr,
 
 // `samples` hashed-random slots per pixel over all visible VPLs, with the
 // reference's hash and seed stepping; the rejection rule as sgi_gather_opts.
-template <int FMT>
+// HALF (fixed draw count only, rejectionLimit < 0): the draws are shaded two
+// at a time in packed FP16 (visibility and acceptance stay FP32), in the same
+// draw order.
+template <int FMT, bool HALF>
 __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
                                                           int nTiles, int samples, int rejectionLimit,
                                                           const PackedVpl* __restrict__ prep,
@@ -597,6 +600,30 @@ __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs
         const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
         float seedZ = fr.time;
         PackedVpl next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+        if constexpr (HALF) {
+            const PixelH q = toHalf(px);
+            PackedVpl nextB = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+            int i = 0;
+            for (; i + 1 < samples; i += 2) {
+                const Vpl La = unpackVpl(next), Lb = unpackVpl(nextB);
+                next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+                nextB = prep[drawSlotAll(seedX, seedY, seedZ, count)];
+                LightH a = noLightH(), b = noLightH();
+                Hit h;
+                if (visibility<FMT>(px, fr, cubes, La, true, validSamples, h)) a = prepHalf(px, La, h);
+                if (visibility<FMT>(px, fr, cubes, Lb, true, validSamples, h)) b = prepHalf(px, Lb, h);
+                shadePairH(q, a, b, vplColor);
+            }
+            if (i < samples) {  // an odd count: the last draw alone
+                const Vpl La = unpackVpl(next);
+                LightH a = noLightH();
+                Hit h;
+                if (visibility<FMT>(px, fr, cubes, La, true, validSamples, h)) a = prepHalf(px, La, h);
+                shadePairH(q, a, noLightH(), vplColor);
+            }
+            writeOut(g, p, vplColor, validSamples, color, reservoir);
+            return;
+        }
         for (int i = 0, resamples = 0; i < samples; i += 1) {
             const Vpl L = unpackVpl(next);
             next = prep[drawSlotAll(seedX, seedY, seedZ, count)];
@@ -624,7 +651,7 @@ hipError_t prepare(const CubeArgs& cubes, const sgi_vpl* vpls, const int32_t* vi
 
 template <int FMT>
 hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int mode, int samples,
-                    int rejectionLimit, int accumulate64, const sgi_vpl* vpls, const int32_t* visible,
+                    int rejectionLimit, int accumulate64, int half, const sgi_vpl* vpls, const int32_t* visible,
                     const sgi_atlas_entry* entries, PackedVpl* prep, float* color, float* reservoir, hipStream_t s) {
     const int rx = (g.W + 15) / 16, ry = (g.H + 15) / 16;
     if (mode == SGI_GATHER_ALL) {
@@ -637,8 +664,12 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
     } else {
         const hipError_t e = prepare<FMT>(cubes, vpls, visible, entries, prep, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(gatherSamplesKernel<FMT>, dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx, rx * ry, samples,
-                           rejectionLimit, prep, visible, color, reservoir);
+        if (half)
+            hipLaunchKernelGGL((gatherSamplesKernel<FMT, true>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
+                               rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
+        else
+            hipLaunchKernelGGL((gatherSamplesKernel<FMT, false>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
+                               rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
     }
     return hipGetLastError();
 }
@@ -706,14 +737,14 @@ hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes,
 }
 
 hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int mode, int samples,
-                    int rejectionLimit, int accumulate64, const sgi_vpl* vpls, const int32_t* visible,
+                    int rejectionLimit, int accumulate64, int half, const sgi_vpl* vpls, const int32_t* visible,
                     const sgi_atlas_entry* entries, void* prep, float* color, float* reservoir, hipStream_t s) {
     PackedVpl* pv = static_cast<PackedVpl*>(prep);
     if (format == SGI_TEXEL_F16)
-        return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, vpls, visible,
+        return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, half, vpls, visible,
                                        entries, pv, color, reservoir, s);
-    return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, vpls, visible, entries,
-                                   pv, color, reservoir, s);
+    return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, half, vpls, visible,
+                                   entries, pv, color, reservoir, s);
 }
 
 hipError_t gatherViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles, int tileSize, int K,

@@ -30,7 +30,7 @@ hipError_t gather(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, cons
 
 // modes / samples / fp64 as sgi_gather_opts
 hipError_t gatherEx(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const sgi_dev::CubeArgs& cubes,
-                    int format, int mode, int samples, int rejectionLimit, int accumulate64, const sgi_vpl* vpls,
+                    int format, int mode, int samples, int rejectionLimit, int accumulate64, int half, const sgi_vpl* vpls,
                     const int32_t* visible, const sgi_atlas_entry* entries, void* prep, float* color,
                     float* reservoir, hipStream_t s);
 

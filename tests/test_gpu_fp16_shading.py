@@ -1,5 +1,7 @@
-"""The packed-FP16 shading form of the gather (SGI_PRECISION_FP16) against the
-FP64 twin of the oracle on the same inputs and tile lists.
+"""The packed-FP16 shading form of the gather (SGI_PRECISION_FP16: the tile-list
+and reference estimators of sgi_gather, and the fixed-count sampled estimator
+of sgi_gather_ex) against the FP64 twin of the oracle on the same inputs and
+tile lists.
 
 The FP16 form keeps the FP32 accept / shadow decisions, so the colour output
 and the sample weights (reservoir.w) must equal the FP32 GPU path bit for bit;
@@ -38,12 +40,19 @@ def _per_channel(got, want, ref32):
     return out
 
 
-def _run(gi, case, upd, vis, ent, tiles, precision):
+SAMPLES = 8
+
+
+def _run(gi, case, upd, vis, ent, tiles, precision, mode):
     torch = require_gpu()
     H, W = case.w.height, case.w.width
     col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     res = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
-    gi.gather(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), tiles, col, res, precision=precision)
+    if mode == "samples":  # the fixed-count estimator (the learned form's input), pairs of draws in FP16
+        gi.gather_ex(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES,
+                     samples=SAMPLES, rejection_limit=-1, precision=precision)
+    else:
+        gi.gather(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), tiles, col, res, precision=precision)
     torch.cuda.synchronize()
     return col.cpu().numpy(), res.cpu().numpy()
 
@@ -67,8 +76,8 @@ def _check(case, gi, mode, bands):
     w, h = case.w, case.host
     upd, vis, ent, tiles = _device_lists(case, gi)
     t = tiles if mode == "tiles" else None
-    c32, r32 = _run(gi, case, upd, vis, ent, t, _abi.PRECISION_FP32)
-    c16, r16 = _run(gi, case, upd, vis, ent, t, _abi.PRECISION_FP16)
+    c32, r32 = _run(gi, case, upd, vis, ent, t, _abi.PRECISION_FP32, mode)
+    c16, r16 = _run(gi, case, upd, vis, ent, t, _abi.PRECISION_FP16, mode)
     m = r32[..., :3] > 1e-3 * r32[..., :3].max()
     q = np.quantile(r16[..., :3][m] / r32[..., :3][m], [0.0, 0.01, 0.5, 0.99, 1.0])
     print(f"[{w.name} {mode}] FP16 / FP32 ratio quantiles 0, .01, .5, .99, 1: {np.array2string(q, precision=4)}")
@@ -79,10 +88,10 @@ def _check(case, gi, mode, bands):
     o_tiles = tiles.cpu().numpy() if mode == "tiles" else None
     got, want, ref = [], [], []
     for y0, y1 in bands:
+        o_mode = {"tiles": orc.GATHER_TILES, "random": orc.GATHER_RANDOM, "samples": orc.GATHER_SAMPLES}[mode]
         _, orr = orc.gather(h.frame, case.hg, o_upd, int(o_vis[0]), o_ent, case.hcubes, tiles=o_tiles,
-                            tile_size=w.tile_size, K=w.max_vpls_per_tile,
-                            mode=orc.GATHER_TILES if mode == "tiles" else orc.GATHER_RANDOM,
-                            precision=1, y0=y0, y1=y1)
+                            tile_size=w.tile_size, K=w.max_vpls_per_tile, mode=o_mode, precision=1, y0=y0, y1=y1,
+                            samples=SAMPLES, rejection_limit=-1)
         got.append(r16[y0:y1, :, :3])
         want.append(orr[..., :3])
         ref.append(r32[y0:y1, :, :3])
@@ -114,10 +123,28 @@ def _gi(case):
     return p
 
 
-@pytest.mark.parametrize("mode", ["tiles", "random"])
+@pytest.mark.parametrize("mode", ["tiles", "random", "samples"])
 def test_fp16_cornell(cornell, mode):
     _check(cornell, _gi(cornell), mode, [(0, cornell.w.height)])
 
 
-def test_fp16_city_bands(city):
-    _check(city, _gi(city), "tiles", CITY_BANDS)
+@pytest.mark.parametrize("mode", ["tiles", "samples"])
+def test_fp16_city_bands(city, mode):
+    _check(city, _gi(city), mode, CITY_BANDS)
+
+
+def test_fp16_refused_outside_the_fixed_count_mode(cornell):
+    require_gpu()
+    gi = _gi(cornell)
+    upd, vis, ent, _ = _device_lists(cornell, gi)
+    with pytest.raises(_abi.SgiError):
+        _run_ex_all_fp16(gi, cornell, upd, vis, ent)
+
+
+def _run_ex_all_fp16(gi, case, upd, vis, ent):
+    torch = require_gpu()
+    H, W = case.w.height, case.w.width
+    col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    res = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    gi.gather_ex(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), None, col, res, mode=_abi.GATHER_ALL,
+                 precision=_abi.PRECISION_FP16)
