@@ -68,6 +68,10 @@ __device__ __forceinline__ void blockTiles(int nTiles, int& t0, int& t1) {
 }
 
 __device__ __forceinline__ bf16x8 asBf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+// A 16-byte LDS read as one vector load (a uint4 read is four scalar loads that
+// the backend re-merges without the alias scopes its wait insertion uses to
+// tell the halo from the blocks being filled by LDS DMA).
+__device__ __forceinline__ bf16x8 ldsBf16x8(const uint4* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
 // Network input features of pixel i (the 16 packed BF16 pairs of its 32
 // channels) and its noisy radiance reservoir.rgb / max(reservoir.a, 1e-5).
@@ -285,9 +289,11 @@ __device__ __forceinline__ uint4 addChunk(uint4 a, uint4 b) {
 // chunk i = 8 channels (c = i % 4) of halo pixel i / 4.
 constexpr int kStageSlots = (kHS * kHS * 4 + 255) / 256;
 
-// DEFER (IN_UPADD only): keep the coarser operand raw in vb, added at the LDS
-// store; otherwise it is added here (fewer registers live across the MFMAs).
-template <int IN, bool DEFER>
+// IN_UPADD: the coarser operand is kept raw in vb and added at the LDS store
+// (adding it here would wait for both loads before the MFMAs; the OUT_APPLY
+// form loads only the skip operand here and stages the coarser one through
+// LDS: stageUp). MASKED: gate IN_PLAIN by src.mask when it is given.
+template <int IN, bool DEFER, bool MASKED = IN == IN_PLAIN>
 __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v,
                                          uint4* vb) {
     const int x0 = (tile % tilesX) * kTS - 1, y0 = (tile / tilesX) * kTS - 1;
@@ -297,22 +303,64 @@ __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int t
         const int px = i >> 2, c = i & 3;
         const int gy = y0 + px / kHS, gx = x0 + px % kHS;
         v[k] = make_uint4(0u, 0u, 0u, 0u);
-        if constexpr (IN == IN_UPADD && DEFER) vb[k] = make_uint4(0u, 0u, 0u, 0u);
+        static_assert(IN != IN_UPADD || DEFER, "the up-add operand is added at the LDS store");
+        if constexpr (IN == IN_UPADD) vb[k] = make_uint4(0u, 0u, 0u, 0u);
         if (IN == IN_PLAIN16 && c >= 2) continue;
         if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const size_t g = ((size_t)gy * W + gx) * (IN == IN_PLAIN16 ? 16 : kC) + c * 8;
             v[k] = *reinterpret_cast<const uint4*>(src.a + g);
             if constexpr (IN == IN_UPADD) {
                 const int w2 = (W + 1) / 2;
-                const uint4 b = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
-                if constexpr (DEFER)
-                    vb[k] = b;
-                else
-                    v[k] = addChunk(v[k], b);
-            } else if (IN == IN_PLAIN && src.mask) {
+                vb[k] = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+            } else if (MASKED && src.mask) {
                 v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(src.mask + g));
             }
         }
+    }
+}
+
+// LDS DMA (global_load_lds) of 12 or 16 bytes per lane: lane l of the wave
+// writes bytes 16 l .. 16 l + 11 (12-byte form) or .. 16 l + 15 (16-byte
+// form) past lds (checked on gfx950). These forms exist on gfx950 only, so
+// the builtin is checked and emitted in the device pass alone (the host pass
+// would reject it and drop the kernel). lds must be a pointer expression into
+// a __shared__ array.
+__device__ __forceinline__ void dmaToLds12(const void* g, void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(g, lds, 12, 0, 0);
+#endif
+}
+
+__device__ __forceinline__ void dmaToLds16(const void* g, void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+#endif
+}
+
+// IN_UPADD's coarser operand for one tile, staged in LDS by DMA so that no
+// registers hold it across the MFMAs: halo rows ty*kTS - 1 .. ty*kTS + kTS of
+// the finer level read coarser rows ty*kTS/2 - 1 .. ty*kTS/2 + kTS/2, a kUpHS
+// x kUpHS block at (tx*kTS/2 - 1, ty*kTS/2 - 1) whose 16-byte chunk d (pixel
+// d / 4, channels 8 (d % 4) .. 8 (d % 4) + 7) sits at index d, read clamped to
+// the coarser image (every entry an in-image halo pixel reads is in range).
+constexpr int kUpHS = kTS / 2 + 2;
+constexpr int kUpChunks = kUpHS * kUpHS * 4;  // 16-byte chunks (8 channels)
+constexpr int kUpSlots = (kUpChunks + 255) / 256;
+
+__device__ __forceinline__ void stageUp(const ConvSrc& src, int H, int W, int tile, int tilesX, uint32_t* upS) {
+    const int ch = (H + 1) / 2, cw = (W + 1) / 2;
+    const int cx0 = (tile % tilesX) * (kTS / 2) - 1, cy0 = (tile / tilesX) * (kTS / 2) - 1;
+    const int wave = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kUpSlots; ++k) {
+        const int d = (int)threadIdx.x + k * 256;
+        size_t g = 0;
+        if (d < kUpChunks) {
+            const int px = d >> 2, c = d & 3;
+            const int gy = min(max(cy0 + px / kUpHS, 0), ch - 1), gx = min(max(cx0 + px % kUpHS, 0), cw - 1);
+            g = ((size_t)gy * cw + gx) * kC + 8 * c;
+        }
+        dmaToLds16(src.b + g, upS + (k * 256 + wave * 64) * 4);
     }
 }
 
@@ -321,23 +369,24 @@ constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS
                                                                              : kTS * kTS * kLogitStride / 4 + 1;
 constexpr int kRadHS = kTS + 4;     // radiance staged for the 5x5 taps (OUT_APPLY)
 constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_APPLY)
-constexpr int kRadSlots = (kRadHS * kRadHS * 3 + 255) / 256;  // staged floats per thread
-constexpr int kCoSlots = (kCoHS * kCoHS * 3 + 255) / 256;
+constexpr int kRadSlots = (kRadHS * kRadHS + 255) / 256;  // staged pixels per thread
+constexpr int kCoSlots = (kCoHS * kCoHS + 255) / 256;
 
-// Taps of the reconstruction from the LDS-staged radiance and coarser output;
+// Taps of the reconstruction from the LDS-staged radiance and coarser output
+// (one pixel per 16 bytes, rgb in the first three floats, one LDS read each);
 // (ox, oy) maps image coordinates to the staged radiance, (cx0, cy0) is the
 // staged coarser block's first pixel.
 struct LdsTaps {
-    const float* rad;
-    const float* co;
+    const f32x4* rad;
+    const f32x4* co;
     int ox, oy, cx0, cy0;
     __device__ __forceinline__ f3 radiance(int xx, int yy) const {
-        const int i = (yy + oy) * kRadHS + xx + ox;
-        return f3{rad[3 * i], rad[3 * i + 1], rad[3 * i + 2]};
+        const f32x4 v = rad[(yy + oy) * kRadHS + xx + ox];
+        return f3{v.x, v.y, v.z};
     }
     __device__ __forceinline__ f3 coarse(int X, int Y) const {
-        const int i = (Y - cy0) * kCoHS + X - cx0;
-        return f3{co[3 * i], co[3 * i + 1], co[3 * i + 2]};
+        const f32x4 v = co[(Y - cy0) * kCoHS + X - cx0];
+        return f3{v.x, v.y, v.z};
     }
 };
 
@@ -347,8 +396,15 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                                                      int tilesX, int nTiles, void* __restrict__ out,
                                                      float* __restrict__ err) {
     __shared__ uint4 smem[kSmemU4];  // the input halo, then (OUT_APPLY) the tile's logits
-    __shared__ float radS[OUT == OUT_APPLY ? kRadSlots * 256 : 1];
-    __shared__ float coS[OUT == OUT_APPLY ? kCoSlots * 256 : 1];
+    __shared__ f32x4 radS[OUT == OUT_APPLY ? kRadSlots * 256 : 1];
+    __shared__ f32x4 coS[OUT == OUT_APPLY ? kCoSlots * 256 : 1];
+    // IN_UPADD with OUT_APPLY: the coarser operand staged by stageUp (the
+    // epilogue needs the registers a deferred add would hold)
+    constexpr bool kLdsUp = IN == IN_UPADD && OUT == OUT_APPLY;
+    constexpr bool kDefer = IN == IN_UPADD && !kLdsUp;
+    constexpr int kHaloIn = kLdsUp ? IN_PLAIN : IN;  // the skip operand loads as a plain input
+    __shared__ uint4 upS4[IN == IN_UPADD && OUT == OUT_APPLY ? kUpSlots * 256 : 1];
+    uint32_t* upS = reinterpret_cast<uint32_t*>(upS4);
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
@@ -367,43 +423,55 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 
     int t0, t1;
     blockTiles(nTiles, t0, t1);
-    // the reconstruction epilogue needs the registers a deferred add would hold
-    constexpr bool kDefer = IN == IN_UPADD && OUT != OUT_APPLY;
     uint4 pre[kStageSlots], preB[kDefer ? kStageSlots : 1];
-    if (t0 < t1) loadHalo<IN, kDefer>(src, H, W, t0, tilesX, pre, preB);
+    if (t0 < t1) {
+        loadHalo<kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN>(src, H, W, t0, tilesX, pre, preB);
+        if constexpr (kLdsUp) stageUp(src, H, W, t0, tilesX, upS);
+    }
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
+        // kLdsUp: this thread's staging DMA has landed before the barrier
+        if constexpr (kLdsUp) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         // this tile's chunks were loaded into `pre` before the previous tile's MFMAs
 #pragma unroll
         for (int k = 0; k < kStageSlots; ++k) {
             const int i = (int)threadIdx.x + k * 256;
             if (i < kHS * kHS * 4) {
-                if constexpr (kDefer)
+                if constexpr (kDefer) {
                     halo[(i >> 2) * kPxU4 + (i & 3)] = addChunk(pre[k], preB[k]);
-                else
+                } else if constexpr (kLdsUp) {
+                    // in-image halo pixels add their coarser pixel; the zero padding stays zero
+                    const int px = i >> 2, gy = ty * kTS - 1 + px / kHS, gx = tx * kTS - 1 + px % kHS;
+                    uint4 v = pre[k];
+                    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                        const int q = ((gy >> 1) - (ty * (kTS / 2) - 1)) * kUpHS + (gx >> 1) - (tx * (kTS / 2) - 1);
+                        v = addChunk(v, upS4[q * 4 + (i & 3)]);
+                    }
+                    halo[px * kPxU4 + (i & 3)] = v;
+                } else {
                     halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
+                }
             }
         }
         __syncthreads();
-        if (tile + 1 < t1) loadHalo<IN, kDefer>(src, H, W, tile + 1, tilesX, pre, preB);
         // OUT_APPLY: this tile's radiance (2-pixel margin) and coarser output (1-pixel margin),
-        // clamped to the image as the global taps clamp, loaded straight into LDS
-        // (global_load_lds: lane l of a wave writes dword l of the wave's 64-dword run)
-        // while the MFMAs run; float i of a staged block sits at index i, the slots
-        // past the block's end receive a valid but unused element
+        // clamped to the image as the global taps clamp, loaded straight into LDS while the
+        // MFMAs run (12-byte global_load_lds: lane l of a wave writes its pixel's rgb to the
+        // first 12 bytes of the 16 at 16 l); pixel i of a staged block sits at slot i, the
+        // slots past the block's end receive a valid but unused pixel. Issued before the
+        // halo prefetch, so that nothing this stage waits for queues behind that prefetch.
         if constexpr (OUT == OUT_APPLY) {
             const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
 #pragma unroll
             for (int k = 0; k < kRadSlots; ++k) {
                 const int i = (int)threadIdx.x + k * 256;
                 size_t g = 0;
-                if (i < kRadHS * kRadHS * 3) {
-                    const int px = i / 3, c = i - 3 * px;
-                    const int gy = min(max(ry0 + px / kRadHS, 0), H - 1), gx = min(max(rx0 + px % kRadHS, 0), W - 1);
-                    g = 3 * ((size_t)gy * W + gx) + c;
+                if (i < kRadHS * kRadHS) {
+                    const int gy = min(max(ry0 + i / kRadHS, 0), H - 1), gx = min(max(rx0 + i % kRadHS, 0), W - 1);
+                    g = 3 * ((size_t)gy * W + gx);
                 }
-                __builtin_amdgcn_global_load_lds(src.radiance + g, radS + k * 256 + wave * 64, 4, 0, 0);
+                dmaToLds12(src.radiance + g, radS + k * 256 + wave * 64);
             }
             if (src.coarse) {
                 const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
@@ -412,15 +480,17 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                 for (int k = 0; k < kCoSlots; ++k) {
                     const int i = (int)threadIdx.x + k * 256;
                     size_t g = 0;
-                    if (i < kCoHS * kCoHS * 3) {
-                        const int px = i / 3, c = i - 3 * px;
-                        const int gy = min(max(cy0 + px / kCoHS, 0), ch - 1),
-                                  gx = min(max(cx0 + px % kCoHS, 0), cw - 1);
-                        g = 3 * ((size_t)gy * cw + gx) + c;
+                    if (i < kCoHS * kCoHS) {
+                        const int gy = min(max(cy0 + i / kCoHS, 0), ch - 1), gx = min(max(cx0 + i % kCoHS, 0), cw - 1);
+                        g = 3 * ((size_t)gy * cw + gx);
                     }
-                    __builtin_amdgcn_global_load_lds(src.coarse + g, coS + k * 256 + wave * 64, 4, 0, 0);
+                    dmaToLds12(src.coarse + g, coS + k * 256 + wave * 64);
                 }
             }
+        }
+        if (tile + 1 < t1) {
+            loadHalo<kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN>(src, H, W, tile + 1, tilesX, pre, preB);
+            if constexpr (kLdsUp) stageUp(src, H, W, tile + 1, tilesX, upS);
         }
         f32x4 acc[2][4];
 #pragma unroll
@@ -433,7 +503,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 const int row = wave * 4 + nb;
-                const bf16x8 b = asBf16x8(halo[((row + dy) * kHS + l16 + dx) * kPxU4 + lg]);
+                const bf16x8 b = ldsBf16x8(halo + ((row + dy) * kHS + l16 + dx) * kPxU4 + lg);
                 acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], b, acc[0][nb], 0, 0, 0);
                 acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], b, acc[1][nb], 0, 0, 0);
             }
