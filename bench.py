@@ -201,20 +201,56 @@ def atrous_at_equal(sweep, learned):
 
 
 def timed_loop(step, steps, warmup, nstages):
-    """Runs warmup + steps frames; returns (elapsed seconds of the timed steps, mean ms per stage)."""
+    """Runs warmup + steps frames; returns (elapsed seconds of the timed steps, mean ms per stage of
+    the frame's stream, mean ms of the direct-light pass on its concurrent stream). step(k, ev, evd)
+    records ev[0..nstages] on the frame's stream and evd[0..1] around the concurrent pass."""
     import torch
     for k in range(warmup):
-        step(k, None)
+        step(k, None, None)
     torch.cuda.synchronize()
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(nstages + 1)] for _ in range(steps)]
+    side = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        step(warmup + k, events[k])
+        step(warmup + k, events[k], side[k])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     per = [sum(events[k][i].elapsed_time(events[k][i + 1]) for k in range(steps)) / steps for i in range(nstages)]
-    return elapsed, per
+    direct = sum(side[k][0].elapsed_time(side[k][1]) for k in range(steps)) / steps
+    return elapsed, per, direct
+
+
+def stage_ms(names, per, direct):
+    """Per-stage mean ms of the frame's stream, plus the concurrent direct-light pass."""
+    out = {k: round(v, 4) for k, v in zip(names, per)}
+    out["direct_light_concurrent"] = round(direct, 4)
+    return out
+
+
+_SIDE_STREAM = []
+
+
+def fork_direct(p, fb, evd):
+    """Issues the direct-light pass on a second stream of default priority: it reads only the
+    G-buffer and the shadow cascades, so it runs beside global culling and the VPL gather; the
+    returned event is what the denoiser (the only consumer of the direct light) waits on. The
+    fork is recorded after the previous frame's denoiser, which read the same buffer."""
+    import torch
+    if not _SIDE_STREAM:
+        _SIDE_STREAM.append(torch.cuda.Stream())
+    side = _SIDE_STREAM[0]
+    fork = torch.cuda.Event()
+    fork.record(torch.cuda.current_stream())
+    side.wait_event(fork)
+    if evd is not None:
+        evd[0].record(side)
+    p.gi.direct_light(p.g, fb.direct, stream=side.cuda_stream)
+    if evd is not None:
+        evd[1].record(side)
+    join = torch.cuda.Event()
+    join.record(side)
+    return join
 
 
 class FormState:
@@ -355,69 +391,68 @@ def main():
     # (1) the pipeline of the reference with tile lists (the parity-tested path): timed, roofline
     use_tiles = args.gather == "tiles"
 
-    def plain_step(k, ev, hist=True):
+    def plain_step(k, ev, evd, hist=True):
         p, fb = plain, plain.fb
         prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
         out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
+        direct_done = fork_direct(p, fb, evd)
         p.cull()
         rec(1)
-        p.gi.direct_light(p.g, fb.direct)
-        rec(2)
         if use_tiles:
             p.gi.cull_tiles(p.g, fb.vpl_updated, fb.visible, fb.tiles)
-        rec(3)
+        rec(2)
         p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, fb.tiles if use_tiles else None,
                     fb.color, fb.reservoir)
-        rec(4)
+        rec(3)
+        torch.cuda.current_stream().wait_event(direct_done)
         p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined, out_gi,
                      fb.reservoir_out, out_h)
-        rec(5)
+        rec(4)
         return out_gi
 
-    gi_plain = plain_step(0, None, hist=False)
+    gi_plain = plain_step(0, None, None, hist=False)
     torch.cuda.synchronize()
     plain_rel = rel_mse(gi_plain, target)
     el_plain, per_plain = None, None
     if args.forms in ("all", "tiles"):
         print(f"[bench] timing {args.steps} plain frames", file=sys.stderr, flush=True)
-        el_plain, per_plain = timed_loop(plain_step, args.steps, args.warmup, 5)
+        el_plain, per_plain, direct_plain = timed_loop(plain_step, args.steps, args.warmup, 4)
     tiles_np = plain.fb.tiles.cpu().numpy()
     visible_plain = int(plain.fb.visible[0].item())
 
     # (2) the a-trous form at equal relMSE: the reference's hashed estimator over every
     # visible VPL at S samples per pixel + the a-trous denoiser
     def atrous_step(S):
-        def step(k, ev, hist=True):
+        def step(k, ev, evd, hist=True):
             p, fb = plain, plain.fb
             prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
             out_gi, out_h = p.hist[(k + 1) % 2]
             rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
             rec(0)
+            direct_done = fork_direct(p, fb, evd)
             p.cull()
             rec(1)
-            p.gi.direct_light(p.g, fb.direct)
-            rec(2)
             p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                            fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S, rejection_limit=args.rejection_limit)
-            rec(3)
+            rec(2)
+            torch.cuda.current_stream().wait_event(direct_done)
             p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                          out_gi, fb.reservoir_out, out_h)
-            rec(4)
+            rec(3)
             return out_gi
         return step
 
     sweep = []
     for S in (args.atrous_spp if args.forms == "all" else []):
         st = atrous_step(S)
-        g_s = st(0, None, hist=False)
+        g_s = st(0, None, None, hist=False)
         torch.cuda.synchronize()
         rm = rel_mse(g_s, target)
-        el_s, per_s = timed_loop(st, args.sweep_steps, 2, 4)
+        el_s, per_s, direct_s = timed_loop(st, args.sweep_steps, 2, 3)
         sweep.append({"spp": S, "relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_s, 2),
-                      "kernels_ms": {k: round(v, 4) for k, v in
-                                     zip(["cull_global", "direct_light", "gather", "denoise"], per_s)}})
+                      "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_s, direct_s)})
         print(f"[bench] a-trous at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
               file=sys.stderr, flush=True)
 
@@ -444,32 +479,32 @@ def main():
         def learned_form(state, label, S, steps, warmup):
             ws = state.gi.kpcn_workspace()
 
-            def learned_step(k, ev, hist=True, thr=threshold):
+            def learned_step(k, ev, evd, hist=True, thr=threshold):
                 p, fb = state, state.fb
                 prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
                 out_gi, out_h = p.hist[(k + 1) % 2]
                 rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
                 rec(0)
+                direct_done = fork_direct(p, fb, evd)
                 p.cull()
                 rec(1)
-                p.gi.direct_light(p.g, fb.direct)
-                rec(2)
                 p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                                fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S,
                                rejection_limit=args.rejection_limit)
-                rec(3)
+                rec(2)
+                torch.cuda.current_stream().wait_event(direct_done)
                 p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net,
                                      thr, ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
-                rec(4)
+                rec(3)
                 return out_gi
 
-            gi_l = learned_step(0, None, hist=False)
+            gi_l = learned_step(0, None, None, hist=False)
             torch.cuda.synchronize()
             rel = rel_mse(gi_l, target)
             fallback_tiles = int(fbt[0].item())
             fb = state.fb
             noisy = rel_mse(fb.reservoir[..., :3] / fb.reservoir[..., 3:4].clamp_min(1e-5), target)
-            gi_nf = learned_step(0, None, hist=False, thr=float("inf"))
+            gi_nf = learned_step(0, None, None, hist=False, thr=float("inf"))
             torch.cuda.synchronize()
             rel_nf = rel_mse(gi_nf, target)
             # the a-trous filter alone on the same input
@@ -478,14 +513,13 @@ def main():
             torch.cuda.synchronize()
             atrous_same = rel_mse(tmp[1], target)
             del tmp
-            el, per = timed_loop(learned_step, steps, warmup, 4)
+            el, per, direct_ms = timed_loop(learned_step, steps, warmup, 3)
             res = {
                 "frames_per_s": round(steps / el, 2), "ms_per_step": round(el * 1e3 / steps, 4),
                 "relmse": round(rel, 6), "relmse_without_fallback": round(rel_nf, 6),
                 "relmse_noisy_input": round(noisy, 6), "relmse_atrous_on_same_input": round(atrous_same, 6),
                 "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
-                "kernels_ms": {k: round(v, 4) for k, v in
-                               zip(["cull_global", "direct_light", "gather", "denoise_learned"], per)},
+                "kernels_ms": stage_ms(["cull_global", "gather", "denoise_learned"], per, direct_ms),
                 "vpls": state.w.num_vpls, "visible_vpls": int(fb.visible[0].item()),
                 "samples_per_pixel": S, "weights": os.path.relpath(weights_path, HERE),
             }
@@ -514,20 +548,19 @@ def main():
     if use_tiles and per_plain is not None:
         alg_bytes, pairs = gather_algorithmic_bytes(tiles_np, w_plain.tile_size, W, H, w_plain.max_vpls_per_tile,
                                                     texel_bytes)
-        achieved = alg_bytes / (per_plain[3] * 1e-3) / 1e9
+        achieved = alg_bytes / (per_plain[2] * 1e-3) / 1e9
         gather_roof = {"kernel": "gatherTilesKernel<F16,16> (plain form)", "bound": "hbm",
                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4),
                        "traffic": load_traffic("gather_city1080p_traffic.json"),
                        "algorithmic_bytes": alg_bytes, "pixel_vpl_pairs": pairs,
-                       "avg_launch_ms": round(per_plain[3], 4)}
+                       "avg_launch_ms": round(per_plain[2], 4)}
     plain_res = None
     if per_plain is not None:
         plain_res = {
             "frames_per_s": round(args.steps / el_plain, 2), "ms_per_step": round(el_plain * 1e3 / args.steps, 4),
             "relmse": round(plain_rel, 6),
-            "kernels_ms": {k: round(v, 4) for k, v in
-                           zip(["cull_global", "direct_light", "cull_tiles", "gather", "denoise"], per_plain)},
+            "kernels_ms": stage_ms(["cull_global", "cull_tiles", "gather", "denoise"], per_plain, direct_plain),
             "vpls": w_plain.num_vpls, "visible_vpls": visible_plain, "gather": args.gather,
         }
     conv_roof = None

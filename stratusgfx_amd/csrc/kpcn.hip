@@ -364,9 +364,12 @@ __device__ __forceinline__ void stageUp(const ConvSrc& src, int H, int W, int ti
     }
 }
 
-constexpr int kLogitStride = 27;  // floats per pixel of the staged logits (channels 0..26)
+// Floats per pixel of the staged logits: channels 0..26 used, 27 padding, so a
+// lane's 4 channels move as one 16-byte LDS access (28 dwords per pixel puts the
+// 16 pixels of a 16-lane group on disjoint banks).
+constexpr int kLogitStride = 28;
 constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
-                                                                             : kTS * kTS * kLogitStride / 4 + 1;
+                                                                             : kTS * kTS * kLogitStride / 4;
 constexpr int kRadHS = kTS + 4;     // radiance staged for the 5x5 taps (OUT_APPLY)
 constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_APPLY)
 constexpr int kRadSlots = (kRadHS * kRadHS + 255) / 256;  // staged pixels per thread
@@ -512,17 +515,21 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         if constexpr (OUT == OUT_APPLY) {
             float* logits = reinterpret_cast<float*>(smem);
             const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
-            __syncthreads();  // every wave has read the halo the logits overwrite
+            // every wave has read the halo the logits overwrite: its LDS reads have returned
+            // (lgkmcnt) and no memory access moves across (the clobber). Not __syncthreads(),
+            // whose fence would also wait for the LDS DMA and the next tile's prefetch in flight.
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-                for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int co = mb * 16 + 4 * lg + r;
-                        if (co < kLogitStride)
-                            logits[((wave * 4 + nb) * kTS + l16) * kLogitStride + co] = acc[mb][nb][r] + bv[mb][r];
+                for (int mb = 0; mb < 2; ++mb) {
+                    const int co = mb * 16 + 4 * lg;  // this lane's 4 channels co .. co + 3
+                    if (co < kLogitStride) {
+                        const f32x4 v = {acc[mb][nb][0] + bv[mb][0], acc[mb][nb][1] + bv[mb][1],
+                                         acc[mb][nb][2] + bv[mb][2], acc[mb][nb][3] + bv[mb][3]};
+                        *reinterpret_cast<f32x4*>(logits + ((wave * 4 + nb) * kTS + l16) * kLogitStride + co) = v;
                     }
+                }
             // this thread's LDS loads have landed before any thread reads the staged blocks
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
@@ -530,7 +537,13 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
             if (x < W && y < H) {
                 float lgv[kLogitStride];
 #pragma unroll
-                for (int j = 0; j < kLogitStride; ++j) lgv[j] = logits[threadIdx.x * kLogitStride + j];
+                for (int j = 0; j < kLogitStride / 4; ++j) {
+                    const f32x4 v = *reinterpret_cast<const f32x4*>(logits + threadIdx.x * kLogitStride + 4 * j);
+                    lgv[4 * j] = v.x;
+                    lgv[4 * j + 1] = v.y;
+                    lgv[4 * j + 2] = v.z;
+                    lgv[4 * j + 3] = v.w;
+                }
                 const int lx = (int)threadIdx.x % kTS, ly = (int)threadIdx.x / kTS;
                 const LdsTaps taps{radS, coS, lx + 2 - x, ly + 2 - y, cx0, cy0};
                 const f3 o = composePixelT(lgv, taps, src.coarse != nullptr, W, H, x, y);
