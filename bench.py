@@ -20,12 +20,13 @@ A step is one GI frame on resident inputs, in RenderScene's order
             tile-list gather + a-trous: the parity-tested plain pipeline, timed
             for its own throughput and roofline.
 
-For each learned sample count, the a-trous form at equal relMSE is the
-smallest swept S whose relMSE is at most the learned form's (and, beside it,
-the frames/s interpolated log-log between the two bracketing a-trous points).
-The operating point is the learned sample count with the highest ratio of
-frames/s at equal relMSE; it is then timed over exactly --steps frames, and
-`value` is its frames/s.  The learned form is also run on the 1024-VPL scene
+For each learned sample count, the a-trous form at equal relMSE is
+interpolated log-log (sample count and frames/s against relMSE) between the two
+swept a-trous points that bracket the learned relMSE; both bracketing points are
+reported beside it.  The learned form runs with the BF16 network and with its
+FP8 (E4M3) inference form when that file exists; the operating point is the
+(precision, sample count) with the highest ratio of frames/s at equal relMSE,
+then timed over exactly --steps frames, and `value` is its frames/s.  The learned form is also run on the 1024-VPL scene
 (same geometry), whose own converged render lies
 `converged_1024_vs_4096_relmse` from the target.  relMSE = mean of
 (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after one frame, against
@@ -57,9 +58,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
-    ap.add_argument("--learned-spp", type=int, nargs="+", default=[4, 8, 16, 32],
+    ap.add_argument("--learned-spp", type=int, nargs="+", default=[8, 12, 16, 24],
                     help="gather samples per pixel of the learned form (the operating point is chosen among them)")
-    ap.add_argument("--atrous-spp", type=int, nargs="+", default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128],
+    ap.add_argument("--atrous-spp", type=int, nargs="+",
+                    default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
     ap.add_argument("--rejection-limit", type=int, default=-1,
@@ -68,6 +70,8 @@ def parse():
                          "every-VPL render")
     ap.add_argument("--views", type=int, default=32,
                     help="batched views of the city block at uneven resolutions (0 = skip)")
+    ap.add_argument("--net-precision", choices=["all", "bf16", "fp8"], default="all",
+                    help="the learned form's network: BF16, its FP8 inference form (when present), or both")
     ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
                     help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
@@ -179,24 +183,39 @@ def rel_mse(x, ref) -> float:
 
 
 def atrous_at_equal(sweep, learned):
-    """The a-trous form at the learned form's relMSE: the cheapest swept point at or below it
-    (and the frames/s interpolated log-log between the bracketing points)."""
+    """The a-trous form at exactly the learned form's relMSE: sample count and frames/s interpolated
+    log-log against relMSE between the two swept points that bracket it. The bracketing points are
+    reported beside it: the cheapest one at or below the learned relMSE (an upper bound of the
+    a-trous cost, so its speedup overstates) and the dearest one above it (a lower bound, so its
+    speedup understates). Outside the swept range the nearest point gives a bound only."""
     if not sweep:
         return {}
-    rel = learned["relmse"]
-    ok = [r for r in sweep if r["relmse"] <= rel]
-    eq = min(ok, key=lambda r: r["spp"]) if ok else max(sweep, key=lambda r: r["spp"])
-    out = {"atrous_at_equal_relmse": {**eq, "reached": bool(ok)},
-           "speedup_vs_atrous_at_equal_relmse": round(learned["frames_per_s"] / eq["frames_per_s"], 3)}
+    rel, fps_l = learned["relmse"], learned["frames_per_s"]
     pts = sorted(sweep, key=lambda r: r["spp"])
+    below = [r for r in pts if r["relmse"] <= rel]
+    above = [r for r in pts if r["relmse"] > rel]
+    out = {}
+    if below:
+        b = below[0]
+        out["atrous_at_or_below_relmse"] = {**b, "speedup": round(fps_l / b["frames_per_s"], 3)}
+    if above:
+        a = above[-1]
+        out["atrous_above_relmse"] = {**a, "speedup": round(fps_l / a["frames_per_s"], 3)}
     for a, b in zip(pts, pts[1:]):
         if a["relmse"] >= rel >= b["relmse"] and a["relmse"] > b["relmse"]:
             t = (math.log(a["relmse"]) - math.log(rel)) / (math.log(a["relmse"]) - math.log(b["relmse"]))
             fps = math.exp(math.log(a["frames_per_s"]) + t * (math.log(b["frames_per_s"]) - math.log(a["frames_per_s"])))
             spp = math.exp(math.log(a["spp"]) + t * (math.log(b["spp"]) - math.log(a["spp"])))
-            out["atrous_interpolated"] = {"spp": round(spp, 2), "frames_per_s": round(fps, 2)}
-            out["speedup_vs_atrous_interpolated"] = round(learned["frames_per_s"] / fps, 3)
-            break
+            out["atrous_at_equal_relmse"] = {"spp": round(spp, 2), "frames_per_s": round(fps, 2), "relmse": rel,
+                                             "interpolated": True}
+            out["speedup_vs_atrous_at_equal_relmse"] = round(fps_l / fps, 3)
+            return out
+    # not bracketed: below every swept relMSE (the a-trous form would need more samples than swept,
+    # the ratio to the last point is a lower bound) or above every one (an upper bound)
+    edge = pts[-1] if rel < pts[-1]["relmse"] else pts[0]
+    out["atrous_at_equal_relmse"] = {**edge, "interpolated": False,
+                                     "bound": "lower" if edge is pts[-1] else "upper"}
+    out["speedup_vs_atrous_at_equal_relmse"] = round(fps_l / edge["frames_per_s"], 3)
     return out
 
 
@@ -363,7 +382,7 @@ def main():
         copy_probe(args.copy_probe_mb)
         return
 
-    from stratusgfx_amd import _abi, kpcn
+    from stratusgfx_amd import _abi, kpcn, kpcn_fp8
     from stratusgfx_amd import scene as sc
 
     w_learned = sc.CITY_1080P
@@ -472,11 +491,19 @@ def main():
             with open(meta_path) as f:
                 threshold = float(json.load(f).get("threshold", float("inf")))
         wts = kpcn.Weights(kpcn.load_params(weights_path))
-        net = wts.net()
+        nets = {"bf16": wts.net()}
+        fp8_path = os.path.splitext(weights_path)[0] + "_fp8.npz"
+        if os.path.exists(fp8_path) and args.net_precision in ("all", "fp8"):
+            _, _, w8, epi = kpcn_fp8.load(fp8_path)
+            f8net = kpcn_fp8.Fp8Net(wts, w8, epi)
+            nets["fp8"] = f8net.net()
+        if args.net_precision == "fp8" and "fp8" in nets:
+            del nets["bf16"]
         n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
         fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
 
-        def learned_form(state, label, S, steps, warmup):
+        def learned_form(state, label, S, steps, warmup, precision):
+            net = nets[precision]
             ws = state.gi.kpcn_workspace()
 
             def learned_step(k, ev, evd, hist=True, thr=threshold):
@@ -521,28 +548,32 @@ def main():
                 "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
                 "kernels_ms": stage_ms(["cull_global", "gather", "denoise_learned"], per, direct_ms),
                 "vpls": state.w.num_vpls, "visible_vpls": int(fb.visible[0].item()),
-                "samples_per_pixel": S, "weights": os.path.relpath(weights_path, HERE),
+                "samples_per_pixel": S, "network_precision": precision,
+                "weights": os.path.relpath(fp8_path if precision == "fp8" else weights_path, HERE),
             }
-            print(f"[bench] learned ({label}, {S} spp, {steps} frames): relMSE {rel:.5f}, {res['frames_per_s']} "
-                  f"frames/s", file=sys.stderr, flush=True)
+            print(f"[bench] learned ({label}, {precision}, {S} spp, {steps} frames): relMSE {rel:.5f}, "
+                  f"{res['frames_per_s']} frames/s", file=sys.stderr, flush=True)
             return res, el
 
         # the learned form's sample counts, each against the a-trous form at equal relMSE
         learned_sweep = []
-        for S in args.learned_spp:
-            r, _ = learned_form(plain, "4096 VPLs", S, args.sweep_steps, 2)
-            r.update(atrous_at_equal(sweep, r))
-            learned_sweep.append(r)
-        if learned_sweep and learned_sweep[0].get("speedup_vs_atrous_at_equal_relmse") is not None:
-            best = max(learned_sweep, key=lambda r: r["speedup_vs_atrous_at_equal_relmse"])
+        for precision in nets:
+            for S in args.learned_spp:
+                r, _ = learned_form(plain, "4096 VPLs", S, args.sweep_steps, 2, precision)
+                r.update(atrous_at_equal(sweep, r))
+                learned_sweep.append(r)
+        # the operating point: the best ratio at equal relMSE among bracketed (interpolated) points
+        ranked = [r for r in learned_sweep if r.get("atrous_at_equal_relmse", {}).get("interpolated")]
+        if ranked:
+            best = max(ranked, key=lambda r: r["speedup_vs_atrous_at_equal_relmse"])
         else:
             best = learned_sweep[len(learned_sweep) // 2]
-        S_op = best["samples_per_pixel"]
-        print(f"[bench] operating point {S_op} spp; timing {args.steps} learned frames", file=sys.stderr,
-              flush=True)
-        learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup)
+        S_op, P_op = best["samples_per_pixel"], best["network_precision"]
+        print(f"[bench] operating point {P_op} network, {S_op} spp; timing {args.steps} learned frames",
+              file=sys.stderr, flush=True)
+        learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup, P_op)
         learned_res.update(atrous_at_equal(sweep, learned_res))
-        learned_1024, _ = learned_form(learned, "1024 VPLs", S_op, args.sweep_steps, 2)
+        learned_1024, _ = learned_form(learned, "1024 VPLs", S_op, args.sweep_steps, 2, P_op)
 
     gather_roof = None
     if use_tiles and per_plain is not None:
@@ -597,7 +628,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 shading, bf16 network (fp32 accumulation)" if learned_res else "fp32",
+        "dtype": (f"fp32 shading, {learned_res['network_precision']} network (fp32 accumulation)"
+                  if learned_res else "fp32"),
         "data": "synthetic: seeded procedural city block, ray-cast G-buffer / VPL cubes / cascades",
         "config": {"workload": "city-1920x1080, 4096 VPLs: learned (S spp + network) vs a-trous (S' spp) at "
                                "equal relMSE",

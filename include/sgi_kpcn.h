@@ -70,10 +70,25 @@ extern "C" {
 #define SGI_KPCN_TILE 16          /* fallback decision granularity (pixels) */
 #define SGI_KPCN_RADIANCE_SCALE 64.0f  /* radiance feature = log(1 + 64 radiance) */
 
-/* A trained network in inference form. */
+/* A trained network in inference form.
+ *
+ * FP8 inference (weights_fp8 != NULL): layers 1..16 run on OCP FP8 E4M3
+ * activations (stored 32 B per pixel) and weights with FP32 accumulation;
+ * layer 0 reads the BF16 features with weights[0] and stores E4M3. Each
+ * activation tensor a carries per-channel power-of-two scales s (stored
+ * q = e4m3(min(a s, 448))); a level's skip tensor e_l, the decoder output d_{l+1}
+ * added to it (e_4 for level 3) and their sum share one scale, and a pooled
+ * tensor keeps its source's. Layer l's FP8 weights are
+ * e4m3(W[t][o][i] / s_in[i] * w_o) with a power-of-two w_o per output channel,
+ * and its epilogue row [k1[32], k2[32]] gives the output channel o as
+ * fma(acc_o, k1_o, k2_o): an activation stores e4m3(min(max(., 0), 448))
+ * (k1 = s_out / w_o, k2 = b s_out; layer 0: k1 = s_out), a filter head keeps
+ * the FP32 logit (k1 = 1 / w_o, k2 = b). */
 typedef struct sgi_kpcn_net {
-    const uint16_t* weights;  /* device BF16 [SGI_KPCN_LAYERS][9][32][32] */
-    const float* biases;      /* device FP32 [SGI_KPCN_LAYERS][32] */
+    const uint16_t* weights;     /* device BF16 [SGI_KPCN_LAYERS][9][32][32] */
+    const float* biases;         /* device FP32 [SGI_KPCN_LAYERS][32] */
+    const uint8_t* weights_fp8;  /* NULL (BF16 inference) or device E4M3 [SGI_KPCN_LAYERS][9][32][32] */
+    const float* epilogue;       /* with weights_fp8: device FP32 [SGI_KPCN_LAYERS][2][32] */
 } sgi_kpcn_net;
 
 /* ---- building blocks (training and tests) -------------------------------- */
@@ -84,6 +99,17 @@ typedef struct sgi_kpcn_net {
  * [h][w][32] (out_f32 = 0) or FP32 [h][w][32] (out_f32 = 1). */
 int sgi_kpcn_conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* weights, const float* bias,
                      int32_t relu, int32_t height, int32_t width, void* out, int32_t out_f32, void* stream);
+
+/* FP8 inference layer: out = e4m3(min(max(fma(acc, k1, k2), 0), 448)) per
+ * output channel, acc = sum over taps and input channels of E4M3 weights [9][32][32]
+ * times E4M3 inputs (FP32 accumulation); in, out E4M3 [h][w][32] (zero outside
+ * the image); epilogue FP32 [2][32] = (k1, k2). */
+int sgi_kpcn_conv3x3_fp8(const uint8_t* in, const uint8_t* weights, const float* epilogue, int32_t height,
+                         int32_t width, uint8_t* out, void* stream);
+
+/* E4M3 bytes of n floats: clamped to +-448, rounded to the nearest even E4M3
+ * value (OCP FP8; the conversion the FP8 layers store with). */
+int sgi_kpcn_fp8_encode(const float* in, int64_t n, uint8_t* out, void* stream);
 
 /* Weight gradient of one layer: grad[t][o][i] = sum_p dz[p][o] * x[p + off(t)][i]
  * and grad[9216 + o] = sum_p dz[p][o], with dz = dy (x) [mask > 0] (mask NULL:

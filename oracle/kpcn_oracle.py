@@ -38,6 +38,20 @@ def to_bf16(x: np.ndarray) -> np.ndarray:
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+E4M3_MAX = 448.0  # largest finite OCP E4M3 value, 1.75 * 2^8
+
+
+def to_e4m3(x: np.ndarray) -> np.ndarray:
+    """Clamp to +-448, then round to the nearest OCP FP8 E4M3 value (ties to even), as float64.
+
+    E4M3 has a sign, 4 exponent bits (bias 7) and 3 fraction bits: normals 2^-6 .. 448 in steps of
+    2^(e - 3) within binade [2^e, 2^(e+1)), subnormals k * 2^-9 (k = 1..7), no infinities."""
+    a = np.clip(np.asarray(x, np.float64), -E4M3_MAX, E4M3_MAX)
+    _, e = np.frexp(np.abs(a))  # |a| in [2^(e-1), 2^e)
+    q = np.ldexp(1.0, np.maximum(e - 4, -9))
+    return np.copysign(np.round(np.abs(a) / q) * q, a)  # np.round rounds half to even
+
+
 def unpack(params: np.ndarray, layer: int):
     """(W [9][out][in], b [32]) of one layer from the flat FP32 parameters."""
     p = params[layer * LAYER_PARAMS:(layer + 1) * LAYER_PARAMS]
@@ -334,3 +348,68 @@ class Net:
         layer_back(0, a["feat"], a["e0"], g, need_dx=False)
         return grads
 
+
+
+def _pow2_floor(x):
+    """Largest power of two <= x (x > 0)."""
+    return 2.0 ** np.floor(np.log2(np.asarray(x, np.float64)))
+
+
+class NetF8:
+    """FP8 (E4M3) inference restatement of the network, following the layout that
+    include/sgi_kpcn.h's sgi_kpcn_net documents.
+
+    S [4][32]: per-channel power-of-two scales of levels 0..3 (shared by e_l, the
+    decoder output added to it and their sum u_l; e_4 uses S_3; a pooled tensor
+    keeps its source's scale); Cs [5][32]: those of c_1..c_4 (rows 1..4). Layer 0
+    convolves the BF16 features with BF16 weights; every other layer convolves
+    E4M3 activations (stored scaled: e4m3(a s)) with E4M3 weights
+    e4m3(W / s_in * w_o), w_o the largest power of two keeping the output
+    channel's weights within 448. Products and sums are exact here (float64); a
+    channel's value is rounded once to FP32 as acc * k1 + k2 (the GPU's fma) and
+    an activation is then stored as e4m3(max(., 0)). Pooling and the skip sums
+    are exact before their single E4M3 rounding."""
+
+    def __init__(self, params, S, Cs):
+        S, Cs = np.asarray(S, np.float64), np.asarray(Cs, np.float64)
+        io = [(None, S[0])]
+        for l in range(1, LEVELS):
+            io += [(S[l - 1], Cs[l]), (Cs[l], S[min(l, LEVELS - 2)])]
+        io += [(S[l], S[l - 1]) for l in range(LEVELS - 2, 0, -1)]
+        io += [(s, None) for s in (S[0], S[0], S[1], S[2], S[3])]
+        self.W, self.k1, self.k2 = [], [], []
+        for l, (s_in, s_out) in enumerate(io):
+            Wt, b = unpack(params, l)
+            if s_in is None:
+                self.W.append(to_bf16(Wt))
+                self.k1.append(s_out)
+                self.k2.append(b * s_out)
+                continue
+            Ws = Wt / s_in[None, None, :]
+            peak = np.abs(Ws).max(axis=(0, 2))
+            w_o = np.where(peak > 0, _pow2_floor(448.0 / np.where(peak > 0, peak, 1.0)), 1.0)
+            self.W.append(to_e4m3(Ws * w_o[None, :, None]))
+            self.k1.append(1.0 / w_o if s_out is None else s_out / w_o)
+            self.k2.append(b if s_out is None else b * s_out)
+
+    def _layer(self, l, x, act=True):
+        acc = conv3x3(x, self.W[l], np.zeros(C))
+        y = (acc * self.k1[l] + self.k2[l]).astype(np.float32).astype(np.float64)
+        return to_e4m3(np.maximum(y, 0.0)) if act else y
+
+    def forward(self, feat):
+        """Per-level FP32 logits from the BF16 features [H][W][32] (channels >= 10 zero)."""
+        a = {"e0": self._layer(0, to_bf16(feat))}
+        for l in range(1, LEVELS):
+            a[f"p{l}"] = to_e4m3(pool(a[f"e{l - 1}"]))
+            a[f"c{l}"] = self._layer(2 * l - 1, a[f"p{l}"])
+            a[f"e{l}"] = self._layer(2 * l, a[f"c{l}"])
+        low = a[f"e{LEVELS - 1}"]
+        for l, layer in zip(range(LEVELS - 2, 0, -1), range(9, 12)):
+            e = a[f"e{l}"]
+            a[f"u{l}"] = to_e4m3(e + upsample(low, *e.shape[:2]))
+            a[f"d{l}"] = self._layer(layer, a[f"u{l}"])
+            low = a[f"d{l}"]
+        a["u0"] = to_e4m3(a["e0"] + upsample(low, *a["e0"].shape[:2]))
+        self.a = a
+        return [self._layer(HEADS[l], Net.head_input_of(a, l), act=False) for l in range(LEVELS)]

@@ -119,7 +119,7 @@ GATHER_REFERENCE, GATHER_TILES, GATHER_ALL, GATHER_SAMPLES = 0, 1, 2, 3
 
 
 class KpcnNet(ctypes.Structure):
-    _fields_ = [("weights", c_void_p), ("biases", c_void_p)]
+    _fields_ = [("weights", c_void_p), ("biases", c_void_p), ("weights_fp8", c_void_p), ("epilogue", c_void_p)]
 
 
 class SceneDesc(ctypes.Structure):
@@ -163,6 +163,8 @@ SGI_SIGNATURES = {
     "sgi_views_replay": (c_int32, [_P, _P]),
     # include/sgi_kpcn.h
     "sgi_kpcn_conv3x3": (c_int32, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
+    "sgi_kpcn_conv3x3_fp8": (c_int32, [_P, _P, _P, _I, _I, _P, _P]),
+    "sgi_kpcn_fp8_encode": (c_int32, [_P, ctypes.c_int64, _P, _P]),
     "sgi_kpcn_wgrad_partials": (ctypes.c_size_t, [_I, _I]),
     "sgi_kpcn_conv3x3_wgrad": (c_int32, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "sgi_kpcn_pool": (c_int32, [_P, _I, _I, _P, _P]),

@@ -333,6 +333,24 @@ int sgi_kpcn_conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_conv3x3");
 }
 
+int sgi_kpcn_conv3x3_fp8(const uint8_t* in, const uint8_t* weights, const float* epilogue, int32_t height,
+                         int32_t width, uint8_t* out, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_conv3x3_fp8");
+    if (r != SGI_OK) return r;
+    if (!in || !weights || !epilogue || !out) return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3_fp8: NULL buffer");
+    if (!aligned16(in) || !aligned16(weights) || !aligned16(out))
+        return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3_fp8: buffers must be 16-byte aligned");
+    const hipError_t e = sgi_kpcn::convF8(in, height, width, weights, epilogue, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_conv3x3_fp8");
+}
+
+int sgi_kpcn_fp8_encode(const float* in, int64_t n, uint8_t* out, void* stream) {
+    if (n < 0 || (n > 0 && (!in || !out))) return fail(SGI_E_INVALID, "sgi_kpcn_fp8_encode: bad buffer or count");
+    if (n == 0) return SGI_OK;
+    const hipError_t e = sgi_kpcn::encodeF8(in, n, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_fp8_encode");
+}
+
 size_t sgi_kpcn_wgrad_partials(int32_t height, int32_t width) {
     if (height <= 0 || width <= 0) return 0;
     return (size_t)sgi_kpcn::wgradBlocks(height, width) * SGI_KPCN_LAYER_PARAMS;
@@ -562,6 +580,8 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
         return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL buffer");
     if (!net || !net->weights || !net->biases || !aligned16(net->weights))
         return fail(SGI_E_INVALID, "sgi_denoise_learned: network weights missing or unaligned");
+    if (net->weights_fp8 && (!net->epilogue || !aligned16(net->weights_fp8)))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: FP8 weights without their epilogue, or unaligned");
     if (!aligned16(reservoir) || !aligned16(reservoirOut))
         return fail(SGI_E_INVALID, "sgi_denoise_learned: reservoirs must be 16-byte aligned");
     const int H = cur->height, W = cur->width;
@@ -575,6 +595,18 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
     auto Wl = [&](int l) { return net->weights + (size_t)l * SGI_KPCN_LAYER_WEIGHTS; };
     auto Bl = [&](int l) { return net->biases + (size_t)l * SGI_KPCN_CHANNELS; };
+    // FP8 inference: layers 1..16 on E4M3 activations and weights, each with its epilogue
+    // (k1, k2); layer 0 reads the BF16 features with the BF16 weights and stores E4M3
+    const int f8 = net->weights_fp8 != nullptr;
+    auto Wt = [&](int l) -> const void* {
+        return f8 ? (const void*)(net->weights_fp8 + (size_t)l * SGI_KPCN_LAYER_WEIGHTS) : (const void*)Wl(l);
+    };
+    auto Kt = [&](int l) { return f8 ? net->epilogue + (size_t)l * 2 * SGI_KPCN_CHANNELS : Bl(l); };
+    auto conv = [&](const void* in, int l, int lev, void* out) {
+        return f8 ? sgi_kpcn::convF8(in, ws.h[lev], ws.w[lev], Wt(l), Kt(l), out, s)
+                  : sgi_kpcn::conv3x3(static_cast<const uint16_t*>(in), nullptr, Wl(l), Bl(l), 1, ws.h[lev], ws.w[lev],
+                                      out, 0, s);
+    };
     hipError_t e = hipSuccess;
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
@@ -582,16 +614,15 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     KPCN_STEP(sgi_kpcn::features16Pyramid(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], ws.rad[1], s));
     // layer 0 (features -> e0) also writes p1, the 2x2 mean of e0; likewise each level's second
     // convolution writes the next level's input
-    KPCN_STEP(sgi_kpcn::conv3x3In16AndPool(ws.feat, ws.h[0], ws.w[0], Wl(0), Bl(0), ws.act[0][0], ws.act[1][2], s));
+    KPCN_STEP(sgi_kpcn::conv3x3In16AndPool(ws.feat, ws.h[0], ws.w[0], Wl(0), Kt(0), ws.act[0][0], ws.act[1][2], f8,
+                                           s));
     for (int l = 1; l < SGI_KPCN_LEVELS; ++l) {
-        KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][2], nullptr, Wl(2 * l - 1), Bl(2 * l - 1), 1, ws.h[l], ws.w[l],
-                                    ws.act[l][1], 0, s));
+        KPCN_STEP(conv(ws.act[l][2], 2 * l - 1, l, ws.act[l][1]));
         if (l + 1 < SGI_KPCN_LEVELS) {
-            KPCN_STEP(sgi_kpcn::convAndPool(ws.act[l][1], ws.h[l], ws.w[l], Wl(2 * l), Bl(2 * l), ws.act[l][0],
-                                            ws.act[l + 1][2], s));
+            KPCN_STEP(sgi_kpcn::convAndPool(ws.act[l][1], ws.h[l], ws.w[l], Wt(2 * l), Kt(2 * l), ws.act[l][0],
+                                            ws.act[l + 1][2], f8, s));
         } else {
-            KPCN_STEP(sgi_kpcn::conv3x3(ws.act[l][1], nullptr, Wl(2 * l), Bl(2 * l), 1, ws.h[l], ws.w[l], ws.act[l][0],
-                                        0, s));
+            KPCN_STEP(conv(ws.act[l][1], 2 * l, l, ws.act[l][0]));
         }
     }
     // decoder: d_l = conv(e_l + up(d_{l+1})) for l = 3..1 with d_4 = e_4 (d_l lands in act[l][1]);
@@ -599,17 +630,17 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     // the level-0 head reads e0 + up(d1) directly
     constexpr int L4 = SGI_KPCN_LEVELS - 1;
     auto head = [](int l) { return 12 + l; };
-    KPCN_STEP(sgi_kpcn::convApply(ws.act[L4][0], ws.h[L4], ws.w[L4], Wl(head(L4)), Bl(head(L4)), ws.rad[L4], nullptr,
-                                  ws.o[L4], nullptr, s));
-    const uint16_t* low = ws.act[L4][0];
+    KPCN_STEP(sgi_kpcn::convApply(ws.act[L4][0], ws.h[L4], ws.w[L4], Wt(head(L4)), Kt(head(L4)), ws.rad[L4], nullptr,
+                                  ws.o[L4], nullptr, f8, s));
+    const void* low = ws.act[L4][0];
     for (int l = L4 - 1, layer = 9; l >= 1; --l, ++layer) {
-        KPCN_STEP(sgi_kpcn::convUpAdd(ws.act[l][0], low, ws.h[l], ws.w[l], Wl(layer), Bl(layer), ws.act[l][1], s));
+        KPCN_STEP(sgi_kpcn::convUpAdd(ws.act[l][0], low, ws.h[l], ws.w[l], Wt(layer), Kt(layer), ws.act[l][1], f8, s));
         low = ws.act[l][1];
-        KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wl(head(l)), Bl(head(l)), ws.rad[l],
-                                      ws.o[l + 1], ws.o[l], nullptr, s));
+        KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wt(head(l)), Kt(head(l)), ws.rad[l],
+                                      ws.o[l + 1], ws.o[l], nullptr, f8, s));
     }
-    KPCN_STEP(sgi_kpcn::convUpAddApply(ws.act[0][0], low, H, W, Wl(head(0)), Bl(head(0)), ws.rad[0], ws.o[1], ws.o[0],
-                                       ws.err, s));
+    KPCN_STEP(sgi_kpcn::convUpAddApply(ws.act[0][0], low, H, W, Wt(head(0)), Kt(head(0)), ws.rad[0], ws.o[1], ws.o[0],
+                                       ws.err, f8, s));
     if (std::isinf(threshold) && threshold > 0.0f) {
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));

@@ -35,7 +35,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kC = SGI_KPCN_CHANNELS;
 constexpr int kTS = 16;       // output tile edge
 constexpr int kHS = kTS + 2;  // halo tile edge
-constexpr int kPxU4 = 5;      // LDS uint4 slots per staged pixel: 4 used, 1 pad (bank spread)
 constexpr int kW = SGI_KPCN_LAYER_WEIGHTS;
 constexpr int kP = SGI_KPCN_LAYER_PARAMS;
 constexpr float kRadianceScale = SGI_KPCN_RADIANCE_SCALE;
@@ -285,35 +284,88 @@ __device__ __forceinline__ uint4 addChunk(uint4 a, uint4 b) {
     return make_uint4(addBf16x2(a.x, b.x), addBf16x2(a.y, b.y), addBf16x2(a.z, b.z), addBf16x2(a.w, b.w));
 }
 
-// The halo chunks one thread stages (every load issued before any is used):
-// chunk i = 8 channels (c = i % 4) of halo pixel i / 4.
-constexpr int kStageSlots = (kHS * kHS * 4 + 255) / 256;
+// Activation formats of the convolution: BF16 (64 B per 32-channel pixel; the
+// training and default inference format) and OCP FP8 E4M3 (32 B per pixel;
+// per-channel power-of-two scales folded into the weights and into the
+// epilogue's per-output-channel out = fma(acc, k1, k2)).
+enum { ACT_BF16 = 0, ACT_F8 = 1 };
+
+template <int ACT>
+struct Act {
+    static constexpr int kPixBytes = ACT == ACT_F8 ? 32 : 64;  // a 32-channel pixel
+    static constexpr int kChunks = kPixBytes / 16;              // its 16-byte chunks
+    static constexpr int kPxU4 = ACT == ACT_F8 ? 3 : 5;         // LDS uint4 per staged pixel, one pad (bank spread)
+    // the halo chunks one thread stages (every load issued before any is used):
+    // chunk i = 16 bytes (c = i % kChunks) of halo pixel i / kChunks
+    static constexpr int kStageSlots = (kHS * kHS * kChunks + 255) / 256;
+};
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// FP8 E4M3 <-> FP32 with gfx950's conversions (OCP E4M3, round to nearest even);
+// values are clamped to the finite range +-448 first, so the conversion's
+// out-of-range behaviour is never reached.
+__device__ __forceinline__ uint32_t packF8x4(float a, float b, float c, float d) {
+    constexpr float m = 448.0f;
+    const uint32_t lo = __builtin_amdgcn_cvt_pk_fp8_f32(clampv(a, -m, m), clampv(b, -m, m), 0, false);
+    return __builtin_amdgcn_cvt_pk_fp8_f32(clampv(c, -m, m), clampv(d, -m, m), lo, true);
+}
+
+__device__ __forceinline__ void unpackF8x4(uint32_t w, float* o) {
+    const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(w, false), hi = __builtin_amdgcn_cvt_pk_f32_fp8(w, true);
+    o[0] = lo.x;
+    o[1] = lo.y;
+    o[2] = hi.x;
+    o[3] = hi.y;
+}
+
+// e + up(d) of FP8 activations sharing one scale: the sum of two E4M3 values is
+// exact in FP32, so the only rounding is the store's.
+__device__ __forceinline__ uint32_t addF8x4(uint32_t a, uint32_t b) {
+    float x[4], y[4];
+    unpackF8x4(a, x);
+    unpackF8x4(b, y);
+    return packF8x4(x[0] + y[0], x[1] + y[1], x[2] + y[2], x[3] + y[3]);
+}
+
+template <int ACT>
+__device__ __forceinline__ uint4 addChunkT(uint4 a, uint4 b) {
+    if constexpr (ACT == ACT_F8)
+        return make_uint4(addF8x4(a.x, b.x), addF8x4(a.y, b.y), addF8x4(a.z, b.z), addF8x4(a.w, b.w));
+    else
+        return addChunk(a, b);
+}
 
 // IN_UPADD: the coarser operand is kept raw in vb and added at the LDS store
 // (adding it here would wait for both loads before the MFMAs; the OUT_APPLY
 // form loads only the skip operand here and stages the coarser one through
 // LDS: stageUp). MASKED: gate IN_PLAIN by src.mask when it is given.
-template <int IN, bool DEFER, bool MASKED = IN == IN_PLAIN>
+template <int ACT, int IN, bool DEFER, bool MASKED = IN == IN_PLAIN>
 __device__ __forceinline__ void loadHalo(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* v,
                                          uint4* vb) {
+    using A = Act<ACT>;
+    static_assert(IN != IN_UPADD || DEFER, "the up-add operand is added at the LDS store");
+    static_assert(!MASKED || ACT == ACT_BF16, "the gradient gate is a BF16 (training) form");
+    constexpr int kInBytes = IN == IN_PLAIN16 ? A::kPixBytes / 2 : A::kPixBytes;
+    const char* a = reinterpret_cast<const char*>(src.a);
+    const char* b = reinterpret_cast<const char*>(src.b);
     const int x0 = (tile % tilesX) * kTS - 1, y0 = (tile / tilesX) * kTS - 1;
 #pragma unroll
-    for (int k = 0; k < kStageSlots; ++k) {
+    for (int k = 0; k < A::kStageSlots; ++k) {
         const int i = (int)threadIdx.x + k * 256;
-        const int px = i >> 2, c = i & 3;
+        const int px = i / A::kChunks, c = i % A::kChunks;
         const int gy = y0 + px / kHS, gx = x0 + px % kHS;
         v[k] = make_uint4(0u, 0u, 0u, 0u);
-        static_assert(IN != IN_UPADD || DEFER, "the up-add operand is added at the LDS store");
         if constexpr (IN == IN_UPADD) vb[k] = make_uint4(0u, 0u, 0u, 0u);
-        if (IN == IN_PLAIN16 && c >= 2) continue;
-        if (i < kHS * kHS * 4 && gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const size_t g = ((size_t)gy * W + gx) * (IN == IN_PLAIN16 ? 16 : kC) + c * 8;
-            v[k] = *reinterpret_cast<const uint4*>(src.a + g);
+        if (IN == IN_PLAIN16 && c >= A::kChunks / 2) continue;
+        if (i < kHS * kHS * A::kChunks && gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const size_t g = ((size_t)gy * W + gx) * kInBytes + 16 * c;
+            v[k] = *reinterpret_cast<const uint4*>(a + g);
             if constexpr (IN == IN_UPADD) {
                 const int w2 = (W + 1) / 2;
-                vb[k] = *reinterpret_cast<const uint4*>(src.b + ((size_t)(gy / 2) * w2 + gx / 2) * kC + c * 8);
+                vb[k] = *reinterpret_cast<const uint4*>(b + ((size_t)(gy / 2) * w2 + gx / 2) * A::kPixBytes + 16 * c);
             } else if (MASKED && src.mask) {
-                v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(src.mask + g));
+                v[k] = gateBf16x8(v[k], *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(src.mask) + g));
             }
         }
     }
@@ -341,26 +393,33 @@ __device__ __forceinline__ void dmaToLds16(const void* g, void* lds) {
 // registers hold it across the MFMAs: halo rows ty*kTS - 1 .. ty*kTS + kTS of
 // the finer level read coarser rows ty*kTS/2 - 1 .. ty*kTS/2 + kTS/2, a kUpHS
 // x kUpHS block at (tx*kTS/2 - 1, ty*kTS/2 - 1) whose 16-byte chunk d (pixel
-// d / 4, channels 8 (d % 4) .. 8 (d % 4) + 7) sits at index d, read clamped to
-// the coarser image (every entry an in-image halo pixel reads is in range).
+// d / kChunks, chunk d % kChunks) sits at index d, read clamped to the coarser
+// image (every entry an in-image halo pixel reads is in range).
 constexpr int kUpHS = kTS / 2 + 2;
-constexpr int kUpChunks = kUpHS * kUpHS * 4;  // 16-byte chunks (8 channels)
-constexpr int kUpSlots = (kUpChunks + 255) / 256;
 
-__device__ __forceinline__ void stageUp(const ConvSrc& src, int H, int W, int tile, int tilesX, uint32_t* upS) {
+template <int ACT>
+struct UpStage {
+    static constexpr int kChunks = kUpHS * kUpHS * Act<ACT>::kChunks;
+    static constexpr int kSlots = (kChunks + 255) / 256;
+};
+
+template <int ACT>
+__device__ __forceinline__ void stageUp(const ConvSrc& src, int H, int W, int tile, int tilesX, uint4* upS) {
+    using A = Act<ACT>;
+    const char* b = reinterpret_cast<const char*>(src.b);
     const int ch = (H + 1) / 2, cw = (W + 1) / 2;
     const int cx0 = (tile % tilesX) * (kTS / 2) - 1, cy0 = (tile / tilesX) * (kTS / 2) - 1;
     const int wave = (int)threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < kUpSlots; ++k) {
+    for (int k = 0; k < UpStage<ACT>::kSlots; ++k) {
         const int d = (int)threadIdx.x + k * 256;
         size_t g = 0;
-        if (d < kUpChunks) {
-            const int px = d >> 2, c = d & 3;
+        if (d < UpStage<ACT>::kChunks) {
+            const int px = d / A::kChunks, c = d % A::kChunks;
             const int gy = min(max(cy0 + px / kUpHS, 0), ch - 1), gx = min(max(cx0 + px % kUpHS, 0), cw - 1);
-            g = ((size_t)gy * cw + gx) * kC + 8 * c;
+            g = ((size_t)gy * cw + gx) * A::kPixBytes + 16 * c;
         }
-        dmaToLds16(src.b + g, upS + (k * 256 + wave * 64) * 4);
+        dmaToLds16(b + g, upS + k * 256 + wave * 64);
     }
 }
 
@@ -368,8 +427,6 @@ __device__ __forceinline__ void stageUp(const ConvSrc& src, int H, int W, int ti
 // lane's 4 channels move as one 16-byte LDS access (28 dwords per pixel puts the
 // 16 pixels of a 16-lane group on disjoint banks).
 constexpr int kLogitStride = 28;
-constexpr int kSmemU4 = (kHS * kHS * kPxU4 > kTS * kTS * kLogitStride / 4) ? kHS * kHS * kPxU4
-                                                                             : kTS * kTS * kLogitStride / 4;
 constexpr int kRadHS = kTS + 4;     // radiance staged for the 5x5 taps (OUT_APPLY)
 constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_APPLY)
 constexpr int kRadSlots = (kRadHS * kRadHS + 255) / 256;  // staged pixels per thread
@@ -393,11 +450,30 @@ struct LdsTaps {
     }
 };
 
-template <int IN, int OUT>
-__global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint16_t* __restrict__ w,
+// The MFMA operand type of an activation format: 8 BF16 or 8 FP8 values.
+template <int ACT>
+struct MfmaOp {
+    typedef bf16x8 T;
+};
+template <>
+struct MfmaOp<ACT_F8> {
+    typedef long T;
+};
+
+// AIN: the input's format (halo, weights and MFMA); AOUT: the format an activation
+// output (OUT_BF16 / OUT_BF16_POOL) is stored in. When either is FP8 the epilogue
+// is scaled: bias holds [k1[32], k2[32]] and a channel's value is fma(acc, k1, k2)
+// (otherwise acc + bias).
+template <int IN, int OUT, int AIN, int AOUT>
+__global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const void* __restrict__ w,
                                                      const float* __restrict__ bias, int relu, int H, int W,
                                                      int tilesX, int nTiles, void* __restrict__ out,
                                                      float* __restrict__ err) {
+    using A = Act<AIN>;
+    constexpr bool kScaled = AIN == ACT_F8 || AOUT == ACT_F8;
+    constexpr int kHaloU4 = kHS * kHS * A::kPxU4;
+    constexpr int kLogitU4 = kTS * kTS * kLogitStride / 4;
+    constexpr int kSmemU4 = OUT == OUT_APPLY && kLogitU4 > kHaloU4 ? kLogitU4 : kHaloU4;
     __shared__ uint4 smem[kSmemU4];  // the input halo, then (OUT_APPLY) the tile's logits
     __shared__ f32x4 radS[OUT == OUT_APPLY ? kRadSlots * 256 : 1];
     __shared__ f32x4 coS[OUT == OUT_APPLY ? kCoSlots * 256 : 1];
@@ -406,30 +482,39 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
     constexpr bool kLdsUp = IN == IN_UPADD && OUT == OUT_APPLY;
     constexpr bool kDefer = IN == IN_UPADD && !kLdsUp;
     constexpr int kHaloIn = kLdsUp ? IN_PLAIN : IN;  // the skip operand loads as a plain input
-    __shared__ uint4 upS4[IN == IN_UPADD && OUT == OUT_APPLY ? kUpSlots * 256 : 1];
-    uint32_t* upS = reinterpret_cast<uint32_t*>(upS4);
+    __shared__ uint4 upS[IN == IN_UPADD && OUT == OUT_APPLY ? UpStage<AIN>::kSlots * 256 : 1];
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
     // A operands: W[t][mb*16 + l16][8 lg .. 8 lg + 7]
-    bf16x8 wa[9][2];
+    typename MfmaOp<AIN>::T wa[9][2];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
-            wa[t][mb] = asBf16x8(*reinterpret_cast<const uint4*>(w + (t * kC + mb * 16 + l16) * kC + 8 * lg));
-    float bv[2][4];
+        for (int mb = 0; mb < 2; ++mb) {
+            const size_t e = (size_t)(t * kC + mb * 16 + l16) * kC + 8 * lg;
+            if constexpr (AIN == ACT_F8)
+                wa[t][mb] = *reinterpret_cast<const long*>(static_cast<const uint8_t*>(w) + e);
+            else
+                wa[t][mb] = asBf16x8(*reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(w) + e));
+        }
+    float k1v[2][4], bv[2][4];  // scaled: (k1, k2); otherwise bv = bias
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bv[mb][r] = bias ? bias[mb * 16 + 4 * lg + r] : 0.0f;
+        for (int r = 0; r < 4; ++r) {
+            const int o = mb * 16 + 4 * lg + r;
+            k1v[mb][r] = kScaled ? bias[o] : 1.0f;
+            bv[mb][r] = kScaled ? bias[kC + o] : (bias ? bias[o] : 0.0f);
+        }
 
     int t0, t1;
     blockTiles(nTiles, t0, t1);
-    uint4 pre[kStageSlots], preB[kDefer ? kStageSlots : 1];
+    uint4 pre[A::kStageSlots], preB[kDefer ? A::kStageSlots : 1];
     if (t0 < t1) {
-        loadHalo<kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN>(src, H, W, t0, tilesX, pre, preB);
-        if constexpr (kLdsUp) stageUp(src, H, W, t0, tilesX, upS);
+        loadHalo<AIN, kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN && AIN == ACT_BF16>(src, H, W, t0, tilesX, pre,
+                                                                                     preB);
+        if constexpr (kLdsUp) stageUp<AIN>(src, H, W, t0, tilesX, upS);
     }
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
@@ -438,22 +523,23 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
         __syncthreads();
         // this tile's chunks were loaded into `pre` before the previous tile's MFMAs
 #pragma unroll
-        for (int k = 0; k < kStageSlots; ++k) {
+        for (int k = 0; k < A::kStageSlots; ++k) {
             const int i = (int)threadIdx.x + k * 256;
-            if (i < kHS * kHS * 4) {
+            if (i < kHS * kHS * A::kChunks) {
+                const int px = i / A::kChunks, c = i % A::kChunks;
                 if constexpr (kDefer) {
-                    halo[(i >> 2) * kPxU4 + (i & 3)] = addChunk(pre[k], preB[k]);
+                    halo[px * A::kPxU4 + c] = addChunkT<AIN>(pre[k], preB[k]);
                 } else if constexpr (kLdsUp) {
                     // in-image halo pixels add their coarser pixel; the zero padding stays zero
-                    const int px = i >> 2, gy = ty * kTS - 1 + px / kHS, gx = tx * kTS - 1 + px % kHS;
+                    const int gy = ty * kTS - 1 + px / kHS, gx = tx * kTS - 1 + px % kHS;
                     uint4 v = pre[k];
                     if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
                         const int q = ((gy >> 1) - (ty * (kTS / 2) - 1)) * kUpHS + (gx >> 1) - (tx * (kTS / 2) - 1);
-                        v = addChunk(v, upS4[q * 4 + (i & 3)]);
+                        v = addChunkT<AIN>(v, upS[q * A::kChunks + c]);
                     }
-                    halo[px * kPxU4 + (i & 3)] = v;
+                    halo[px * A::kPxU4 + c] = v;
                 } else {
-                    halo[(i >> 2) * kPxU4 + (i & 3)] = pre[k];
+                    halo[px * A::kPxU4 + c] = pre[k];
                 }
             }
         }
@@ -492,8 +578,9 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
             }
         }
         if (tile + 1 < t1) {
-            loadHalo<kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN>(src, H, W, tile + 1, tilesX, pre, preB);
-            if constexpr (kLdsUp) stageUp(src, H, W, tile + 1, tilesX, upS);
+            loadHalo<AIN, kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN && AIN == ACT_BF16>(src, H, W, tile + 1, tilesX,
+                                                                                         pre, preB);
+            if constexpr (kLdsUp) stageUp<AIN>(src, H, W, tile + 1, tilesX, upS);
         }
         f32x4 acc[2][4];
 #pragma unroll
@@ -506,9 +593,18 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 const int row = wave * 4 + nb;
-                const bf16x8 b = ldsBf16x8(halo + ((row + dy) * kHS + l16 + dx) * kPxU4 + lg);
-                acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], b, acc[0][nb], 0, 0, 0);
-                acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], b, acc[1][nb], 0, 0, 0);
+                const int hp = (row + dy) * kHS + l16 + dx;  // the staged halo pixel
+                if constexpr (AIN == ACT_F8) {
+                    // 8 FP8 channels 8 lg .. 8 lg + 7: one 8-byte LDS read
+                    const long b = *reinterpret_cast<const long*>(reinterpret_cast<const char*>(halo + hp * A::kPxU4) +
+                                                                  8 * lg);
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(wa[t][0], b, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(wa[t][1], b, acc[1][nb], 0, 0, 0);
+                } else {
+                    const bf16x8 b = ldsBf16x8(halo + hp * A::kPxU4 + lg);
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], b, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], b, acc[1][nb], 0, 0, 0);
+                }
             }
         }
         // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
@@ -525,8 +621,11 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                 for (int mb = 0; mb < 2; ++mb) {
                     const int co = mb * 16 + 4 * lg;  // this lane's 4 channels co .. co + 3
                     if (co < kLogitStride) {
-                        const f32x4 v = {acc[mb][nb][0] + bv[mb][0], acc[mb][nb][1] + bv[mb][1],
-                                         acc[mb][nb][2] + bv[mb][2], acc[mb][nb][3] + bv[mb][3]};
+                        f32x4 v;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            v[r] = kScaled ? __builtin_fmaf(acc[mb][nb][r], k1v[mb][r], bv[mb][r])
+                                           : acc[mb][nb][r] + bv[mb][r];
                         *reinterpret_cast<f32x4*>(logits + ((wave * 4 + nb) * kTS + l16) * kLogitStride + co) = v;
                     }
                 }
@@ -566,14 +665,25 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                     float v[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        v[r] = acc[mb][nb][r] + bv[mb][r];
+                        v[r] = kScaled ? __builtin_fmaf(acc[mb][nb][r], k1v[mb][r], bv[mb][r])
+                                       : acc[mb][nb][r] + bv[mb][r];
                         if (kBf16 && relu) v[r] = gmax(v[r], 0.0f);
                     }
                     const int co = mb * 16 + 4 * lg;
                     if constexpr (OUT == OUT_F32) {
+                        static_assert(!kScaled, "FP32 activations are a BF16 (training) form");
                         if (x < W && y < H)
                             *reinterpret_cast<float4*>(static_cast<float*>(out) + p * kC + co) =
                                 make_float4(v[0], v[1], v[2], v[3]);
+                    } else if constexpr (AOUT == ACT_F8) {
+                        const uint32_t q = packF8x4(v[0], v[1], v[2], v[3]);
+                        if (x < W && y < H) *reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(out) + p * kC + co) = q;
+                        if constexpr (OUT == OUT_BF16_POOL) {  // keep the stored values for the pooling
+                            float d[4];
+                            unpackF8x4(q, d);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) acc[mb][nb][r] = d[r];
+                        }
                     } else {
                         const uint32_t b0 = f2bf(v[0]), b1 = f2bf(v[1]), b2 = f2bf(v[2]), b3 = f2bf(v[3]);
                         if (x < W && y < H)
@@ -601,7 +711,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                     const float inv = 1.0f / (float)n;
 #pragma unroll
                     for (int mb = 0; mb < 2; ++mb) {
-                        uint32_t o[4];
+                        float m[4];
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float a00 = acc[mb][2 * pr][r], a10 = acc[mb][2 * pr + 1][r];
@@ -613,16 +723,36 @@ __global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const uint1
                                 sum += a10;
                                 if (x1In) sum += a11;
                             }
-                            o[r] = f2bf(sum * inv);
+                            m[r] = sum * inv;
                         }
-                        if (even && xIn && y0 < H)
-                            *reinterpret_cast<uint2*>(src.pooled + ((size_t)(y0 / 2) * cw + x / 2) * kC + mb * 16 +
-                                                      4 * lg) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+                        const size_t q = ((size_t)(y0 / 2) * cw + x / 2) * kC + mb * 16 + 4 * lg;
+                        if constexpr (AOUT == ACT_F8) {
+                            const uint32_t o = packF8x4(m[0], m[1], m[2], m[3]);
+                            if (even && xIn && y0 < H) *reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(src.pooled) + q) = o;
+                        } else {
+                            const uint32_t o0 = f2bf(m[0]), o1 = f2bf(m[1]), o2 = f2bf(m[2]), o3 = f2bf(m[3]);
+                            if (even && xIn && y0 < H)
+                                *reinterpret_cast<uint2*>(src.pooled + q) = make_uint2(o0 | (o1 << 16), o2 | (o3 << 16));
+                        }
                     }
                 }
             }
         }
     }
+}
+
+// E4M3 bytes of n floats (clamped to +-448, nearest even): thread i converts
+// values 4i .. 4i + 3 (those past n are left alone).
+__global__ __launch_bounds__(256) void encodeF8Kernel(const float* __restrict__ in, int64_t n, uint8_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = 4 * i;
+    if (j >= n) return;
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = j + k < n ? in[j + k] : 0.0f;
+    const uint32_t q = packF8x4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (j + k < n) out[j + k] = (uint8_t)(q >> (8 * k));
 }
 
 // Weight gradient.  LDS: xs[s][ci][row][chunk] = 8 pixels x[y0 - 1 + row][x0 + 8 chunk + s - 1 .. +7][ci]
@@ -1173,69 +1303,87 @@ int gridFor(int n, int b) { return (n + b - 1) / b; }
 
 namespace sgi_kpcn {
 
-template <int IN, int OUT>
-hipError_t launchConv(const ConvSrc& src, const uint16_t* w, const float* bias, int relu, int H, int W, void* out,
+template <int IN, int OUT, int AIN, int AOUT>
+hipError_t launchConv(const ConvSrc& src, const void* w, const float* bias, int relu, int H, int W, void* out,
                       float* err, hipStream_t s) {
     const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
     // two blocks per CU stay resident (LDS and VGPRs), so 512 blocks cover the 256 CUs once and
     // each walks a contiguous run of tiles with its next halo in flight
     const int grid = nTiles < 512 ? nTiles : 512;
-    hipLaunchKernelGGL((conv3x3Kernel<IN, OUT>), dim3(grid), dim3(256), 0, s, src, w, bias, relu, H, W, tilesX,
-                       nTiles, out, err);
+    hipLaunchKernelGGL((conv3x3Kernel<IN, OUT, AIN, AOUT>), dim3(grid), dim3(256), 0, s, src, w, bias, relu, H, W,
+                       tilesX, nTiles, out, err);
     return hipGetLastError();
+}
+
+// The same form with FP8 (f8 != 0) or BF16 activations in and out.
+template <int IN, int OUT>
+hipError_t launchConvAct(int f8, const ConvSrc& src, const void* w, const float* bias, int relu, int H, int W,
+                         void* out, float* err, hipStream_t s) {
+    if (f8) return launchConv<IN, OUT, ACT_F8, ACT_F8>(src, w, bias, relu, H, W, out, err, s);
+    return launchConv<IN, OUT, ACT_BF16, ACT_BF16>(src, w, bias, relu, H, W, out, err, s);
+}
+
+ConvSrc convSrc(const void* a, const void* b = nullptr) {
+    ConvSrc src{};
+    src.a = static_cast<const uint16_t*>(a);
+    src.b = static_cast<const uint16_t*>(b);
+    return src;
 }
 
 hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* bias, int relu, int H,
                    int W, void* out, int outF32, hipStream_t s) {
-    ConvSrc src{};
-    src.a = in;
+    ConvSrc src = convSrc(in);
     src.mask = mask;
-    if (outF32) return launchConv<IN_PLAIN, OUT_F32>(src, w, bias, relu, H, W, out, nullptr, s);
-    return launchConv<IN_PLAIN, OUT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
+    if (outF32) return launchConv<IN_PLAIN, OUT_F32, ACT_BF16, ACT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
+    return launchConv<IN_PLAIN, OUT_BF16, ACT_BF16, ACT_BF16>(src, w, bias, relu, H, W, out, nullptr, s);
 }
 
-hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias,
-                              uint16_t* out, uint16_t* pooled, hipStream_t s) {
-    ConvSrc src{};
-    src.a = in16;
-    src.pooled = pooled;
-    return launchConv<IN_PLAIN16, OUT_BF16_POOL>(src, w, bias, 1, H, W, out, nullptr, s);
+hipError_t convF8(const void* in, int H, int W, const void* w, const float* scales, void* out, hipStream_t s) {
+    return launchConv<IN_PLAIN, OUT_BF16, ACT_F8, ACT_F8>(convSrc(in), w, scales, 1, H, W, out, nullptr, s);
 }
 
-hipError_t convUpAddApply(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w,
-                          const float* bias, const float* radiance, const float* coarse, float* out, float* err,
+hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, void* out,
+                              void* pooled, int f8out, hipStream_t s) {
+    ConvSrc src = convSrc(in16);
+    src.pooled = static_cast<uint16_t*>(pooled);
+    if (f8out)
+        return launchConv<IN_PLAIN16, OUT_BF16_POOL, ACT_BF16, ACT_F8>(src, w, bias, 1, H, W, out, nullptr, s);
+    return launchConv<IN_PLAIN16, OUT_BF16_POOL, ACT_BF16, ACT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
+}
+
+hipError_t convUpAddApply(const void* skip, const void* low, int H, int W, const void* w, const float* bias,
+                          const float* radiance, const float* coarse, float* out, float* err, int f8,
                           hipStream_t s) {
-    ConvSrc src{};
-    src.a = skip;
-    src.b = low;
+    ConvSrc src = convSrc(skip, low);
     src.radiance = radiance;
     src.coarse = coarse;
-    return launchConv<IN_UPADD, OUT_APPLY>(src, w, bias, 0, H, W, out, err, s);
+    return launchConvAct<IN_UPADD, OUT_APPLY>(f8, src, w, bias, 0, H, W, out, err, s);
 }
 
-hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
-                       uint16_t* pooled, hipStream_t s) {
-    ConvSrc src{};
-    src.a = in;
-    src.pooled = pooled;
-    return launchConv<IN_PLAIN, OUT_BF16_POOL>(src, w, bias, 1, H, W, out, nullptr, s);
+hipError_t convAndPool(const void* in, int H, int W, const void* w, const float* bias, void* out, void* pooled,
+                       int f8, hipStream_t s) {
+    ConvSrc src = convSrc(in);
+    src.pooled = static_cast<uint16_t*>(pooled);
+    return launchConvAct<IN_PLAIN, OUT_BF16_POOL>(f8, src, w, bias, 1, H, W, out, nullptr, s);
 }
 
-hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
-                     uint16_t* out, hipStream_t s) {
-    ConvSrc src{};
-    src.a = skip;
-    src.b = low;
-    return launchConv<IN_UPADD, OUT_BF16>(src, w, bias, 1, H, W, out, nullptr, s);
+hipError_t convUpAdd(const void* skip, const void* low, int H, int W, const void* w, const float* bias, void* out,
+                     int f8, hipStream_t s) {
+    return launchConvAct<IN_UPADD, OUT_BF16>(f8, convSrc(skip, low), w, bias, 1, H, W, out, nullptr, s);
 }
 
-hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
-                     const float* coarse, float* out, float* err, hipStream_t s) {
-    ConvSrc src{};
-    src.a = in;
+hipError_t convApply(const void* in, int H, int W, const void* w, const float* bias, const float* radiance,
+                     const float* coarse, float* out, float* err, int f8, hipStream_t s) {
+    ConvSrc src = convSrc(in);
     src.radiance = radiance;
     src.coarse = coarse;
-    return launchConv<IN_PLAIN, OUT_APPLY>(src, w, bias, 0, H, W, out, err, s);
+    return launchConvAct<IN_PLAIN, OUT_APPLY>(f8, src, w, bias, 0, H, W, out, err, s);
+}
+
+hipError_t encodeF8(const float* in, int64_t n, uint8_t* out, hipStream_t s) {
+    const int64_t words = (n + 3) / 4;
+    hipLaunchKernelGGL(encodeF8Kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
 }
 
 int wgradBlocks(int H, int W) {

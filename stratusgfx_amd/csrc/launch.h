@@ -90,21 +90,27 @@ hipError_t conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w, 
 // input staging (upsample + skip) or the epilogue (2x2 pooling of the output;
 // one reconstruction level: the 5x5 filter and the blend with the coarser
 // output); same results as the separate kernels.
-// The first convolution on the inference features' [h][w][16] layout, with the
-// 2x2 mean of its output for the next level.
-hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias,
-                              uint16_t* out, uint16_t* pooled, hipStream_t s);
+// The first convolution on the inference features' [h][w][16] layout (BF16),
+// with the 2x2 mean of its output for the next level; f8out: both outputs FP8
+// E4M3 (bias then holds the epilogue's [k1[32], k2[32]]).
+hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t* w, const float* bias, void* out,
+                              void* pooled, int f8out, hipStream_t s);
+// The remaining forms take f8: 0 for BF16 activations and weights with a bias,
+// 1 for FP8 E4M3 activations and weights with the epilogue [k1[32], k2[32]].
 // The level-0 head: its input e0 + up(d1) staged as convUpAdd stages it, its
 // epilogue the reconstruction step as convApply's.
-hipError_t convUpAddApply(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w,
-                          const float* bias, const float* radiance, const float* coarse, float* out, float* err,
+hipError_t convUpAddApply(const void* skip, const void* low, int H, int W, const void* w, const float* bias,
+                          const float* radiance, const float* coarse, float* out, float* err, int f8,
                           hipStream_t s);
-hipError_t convAndPool(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, uint16_t* out,
-                       uint16_t* pooled, hipStream_t s);
-hipError_t convUpAdd(const uint16_t* skip, const uint16_t* low, int H, int W, const uint16_t* w, const float* bias,
-                     uint16_t* out, hipStream_t s);
-hipError_t convApply(const uint16_t* in, int H, int W, const uint16_t* w, const float* bias, const float* radiance,
-                     const float* coarse, float* out, float* err, hipStream_t s);
+hipError_t convAndPool(const void* in, int H, int W, const void* w, const float* bias, void* out, void* pooled,
+                       int f8, hipStream_t s);
+hipError_t convUpAdd(const void* skip, const void* low, int H, int W, const void* w, const float* bias, void* out,
+                     int f8, hipStream_t s);
+hipError_t convApply(const void* in, int H, int W, const void* w, const float* bias, const float* radiance,
+                     const float* coarse, float* out, float* err, int f8, hipStream_t s);
+// FP8 in, FP8 out (ReLU): out = e4m3(max(fma(acc, k1, k2), 0)), scales = [k1[32], k2[32]].
+hipError_t convF8(const void* in, int H, int W, const void* w, const float* scales, void* out, hipStream_t s);
+hipError_t encodeF8(const float* in, int64_t n, uint8_t* out, hipStream_t s);
 int wgradBlocks(int H, int W);
 hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,
                  float* grad, hipStream_t s);
