@@ -464,11 +464,18 @@ struct MfmaOp<ACT_F8> {
 // output (OUT_BF16 / OUT_BF16_POOL) is stored in. When either is FP8 the epilogue
 // is scaled: bias holds [k1[32], k2[32]] and a channel's value is fma(acc, k1, k2)
 // (otherwise acc + bias).
+// Blocks resident per CU: two (VGPRs near 256, and for OUT_APPLY the staged blocks in LDS),
+// three for FP8 inputs on the plain path (FP8 weights take 36 VGPRs instead of 72 and the
+// FP8 halo 15.5 KB of LDS; the deferred up-add's raw chunks would not fit the 168 VGPRs).
+template <int IN, int OUT, int AIN>
+constexpr int convBlocksPerCu() {
+    return AIN == ACT_F8 && OUT != OUT_APPLY && IN == IN_PLAIN ? 3 : 2;
+}
+
 template <int IN, int OUT, int AIN, int AOUT>
-__global__ __launch_bounds__(256, 2) void conv3x3Kernel(ConvSrc src, const void* __restrict__ w,
-                                                     const float* __restrict__ bias, int relu, int H, int W,
-                                                     int tilesX, int nTiles, void* __restrict__ out,
-                                                     float* __restrict__ err) {
+__global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x3Kernel(
+    ConvSrc src, const void* __restrict__ w, const float* __restrict__ bias, int relu, int H, int W, int tilesX,
+    int nTiles, void* __restrict__ out, float* __restrict__ err) {
     using A = Act<AIN>;
     constexpr bool kScaled = AIN == ACT_F8 || AOUT == ACT_F8;
     constexpr int kHaloU4 = kHS * kHS * A::kPxU4;
@@ -1307,9 +1314,10 @@ template <int IN, int OUT, int AIN, int AOUT>
 hipError_t launchConv(const ConvSrc& src, const void* w, const float* bias, int relu, int H, int W, void* out,
                       float* err, hipStream_t s) {
     const int tilesX = (W + kTS - 1) / kTS, nTiles = tilesX * ((H + kTS - 1) / kTS);
-    // two blocks per CU stay resident (LDS and VGPRs), so 512 blocks cover the 256 CUs once and
-    // each walks a contiguous run of tiles with its next halo in flight
-    const int grid = nTiles < 512 ? nTiles : 512;
+    // convBlocksPerCu blocks per CU stay resident (LDS and VGPRs), so 256 times that many cover
+    // the 256 CUs once and each walks a contiguous run of tiles with its next halo in flight
+    const int resident = 256 * convBlocksPerCu<IN, OUT, AIN>();
+    const int grid = nTiles < resident ? nTiles : resident;
     hipLaunchKernelGGL((conv3x3Kernel<IN, OUT, AIN, AOUT>), dim3(grid), dim3(256), 0, s, src, w, bias, relu, H, W,
                        tilesX, nTiles, out, err);
     return hipGetLastError();
