@@ -610,7 +610,9 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": load_traffic(f"gather_samples{S}_city1080p_traffic.json"),
                     "algorithmic_bytes": alg, "draws": W * H * S, "avg_launch_ms": g_ms,
-                    "note": "VALU-bound (the BRDF and shadow test per draw), HBM fraction low by construction"}
+                    "note": ("VALU-issue-bound, not HBM-bound: 250 wave VALU instructions per draw, whose issue "
+                             "takes 56 % of the launch at 16 draws (profiles/valu_counters_gather_samples16_"
+                             "city1080p.md); the HBM fraction is low by construction")}
         conv_roof = conv_roofline(wts, H, W)
     elif plain_res is not None:
         value, el, form = plain_res["frames_per_s"], el_plain, "tile-list plain form only"

@@ -432,23 +432,41 @@ constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_A
 constexpr int kRadSlots = (kRadHS * kRadHS + 255) / 256;  // staged pixels per thread
 constexpr int kCoSlots = (kCoHS * kCoHS + 255) / 256;
 
-// Taps of the reconstruction from the LDS-staged radiance and coarser output
-// (one pixel per 16 bytes, rgb in the first three floats, one LDS read each);
-// (ox, oy) maps image coordinates to the staged radiance, (cx0, cy0) is the
-// staged coarser block's first pixel.
-struct LdsTaps {
-    const f32x4* rad;
-    const f32x4* co;
-    int ox, oy, cx0, cy0;
-    __device__ __forceinline__ f3 radiance(int xx, int yy) const {
-        const f32x4 v = rad[(yy + oy) * kRadHS + xx + ox];
-        return f3{v.x, v.y, v.z};
+// One level of the reconstruction at tile pixel (lx, ly) from the LDS-staged
+// blocks (one pixel per 16 bytes, rgb in the first three floats): radiance
+// staged with a 2-pixel margin and the coarser output with a 1-pixel margin,
+// both read clamped to their images when staged, so every tap the clamped
+// formula names sits at a constant offset from the pixel's base: the same
+// values, summed in composePixelT's order, with one immediate-offset LDS read
+// per tap and no per-tap index arithmetic.
+__device__ __forceinline__ f3 composePixelStaged(const float* lg, const f32x4* rad, const f32x4* co, bool hasCoarse,
+                                                 int lx, int ly) {
+    float wgt[SGI_KPCN_TAPS];
+    softmax25(lg, wgt);
+    const f32x4* r = rad + ly * kRadHS + lx;  // tap (dx, dy) of the pixel: r[(dy + 2) kRadHS + dx + 2]
+    f3 o = splat(0.0f);
+#pragma unroll
+    for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
+        const f32x4 v = r[(j / 5) * kRadHS + j % 5];
+        o = o + f3{v.x, v.y, v.z} * wgt[j];
     }
-    __device__ __forceinline__ f3 coarse(int X, int Y) const {
-        const f32x4 v = co[(Y - cy0) * kCoHS + X - cx0];
-        return f3{v.x, v.y, v.z};
+    if (hasCoarse) {
+        // bilinearAxis: coarse pixels a, a + 1 (a = x / 2 for odd x, x / 2 - 1 for even), clamped;
+        // in the staged block (first pixel tx*kTS/2 - 1) a sits at (lx + 1) / 2
+        const int ax = (lx + 1) >> 1, ay = (ly + 1) >> 1;
+        const float wx0 = (lx & 1) ? 0.75f : 0.25f, wy0 = (ly & 1) ? 0.75f : 0.25f;
+        const float wx1 = 1.0f - wx0, wy1 = 1.0f - wy0;
+        auto c = [&](int X, int Y) {
+            const f32x4 v = co[Y * kCoHS + X];
+            return f3{v.x, v.y, v.z};
+        };
+        const f3 top = c(ax, ay) * wx0 + c(ax + 1, ay) * wx1;
+        const f3 bot = c(ax, ay + 1) * wx0 + c(ax + 1, ay + 1) * wx1;
+        const float b = sigmoidf(lg[SGI_KPCN_BLEND_CHANNEL]);
+        o = o + (top * wy0 + bot * wy1 - o) * b;
     }
-};
+    return o;
+}
 
 // The MFMA operand type of an activation format: 8 BF16 or 8 FP8 values.
 template <int ACT>
@@ -617,7 +635,6 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
         // lane holds out channels mb*16 + 4 lg + r of pixel (row, l16)
         if constexpr (OUT == OUT_APPLY) {
             float* logits = reinterpret_cast<float*>(smem);
-            const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
             // every wave has read the halo the logits overwrite: its LDS reads have returned
             // (lgkmcnt) and no memory access moves across (the clobber). Not __syncthreads(),
             // whose fence would also wait for the LDS DMA and the next tile's prefetch in flight.
@@ -651,8 +668,7 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                     lgv[4 * j + 3] = v.w;
                 }
                 const int lx = (int)threadIdx.x % kTS, ly = (int)threadIdx.x / kTS;
-                const LdsTaps taps{radS, coS, lx + 2 - x, ly + 2 - y, cx0, cy0};
-                const f3 o = composePixelT(lgv, taps, src.coarse != nullptr, W, H, x, y);
+                const f3 o = composePixelStaged(lgv, radS, coS, src.coarse != nullptr, lx, ly);
                 const size_t p = (size_t)y * W + x;
                 float* dst = static_cast<float*>(out) + 3 * p;
                 dst[0] = o.x;

@@ -199,6 +199,8 @@ def main(argv=None) -> None:
     ap.add_argument("--loss-scale", type=float, default=1000.0,
                     help="gradient scale (the relative-MSE gradients are ~1e-9 per logit at these radiances)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None, help="start from these parameters (a weights .npy) instead of a fresh network")
+    ap.add_argument("--first-scene", type=int, default=1000, help="seed of the first training block")
     ap.add_argument("--out", default=kpcn.DEFAULT_WEIGHTS)
     ap.add_argument("--log", default=None)
     a = ap.parse_args(argv)
@@ -206,7 +208,7 @@ def main(argv=None) -> None:
     t0 = time.time()
     train = []
     for i in range(a.scenes):
-        train += make_samples(1000 + i, a.views, a.width, a.height, a.vpls, a.spp, rng, cube_res=a.cube_res)
+        train += make_samples(a.first_scene + i, a.views, a.width, a.height, a.vpls, a.spp, rng, cube_res=a.cube_res)
         if i % 8 == 7:
             print(f"[data] {len(train)} training views, {time.time() - t0:.0f} s", flush=True)
     val = []
@@ -217,7 +219,7 @@ def main(argv=None) -> None:
     print(f"[data] {len(train)} train / {len(val)} val views in {time.time() - t0:.0f} s; median target {tmed:.4g}",
           flush=True)
 
-    wts = kpcn.Weights(kpcn.init_params(a.seed))
+    wts = kpcn.Weights(kpcn.load_params(a.init) if a.init else kpcn.init_params(a.seed))
     tr = kpcn.Trainer(wts, a.height, a.width)
     base = evaluate(wts, val, [])
     print(f"[val] before training {json.dumps(base)}", flush=True)
@@ -251,6 +253,7 @@ def main(argv=None) -> None:
     kpcn.save_params(wts.params.cpu().numpy(), a.out)
     summary = {"train_views": len(train), "val_views": len(val), "width": a.width, "height": a.height, "cube_res": a.cube_res, "vpls": a.vpls, "spp": a.spp,
                "steps": a.steps, "batch": a.batch, "lr": a.lr, "err_weight": a.err_weight,
+               "init": os.path.basename(a.init) if a.init else None, "first_scene": a.first_scene,
                "loss_scale": a.loss_scale,
                "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
                "before": base, "after": after, "threshold": best_t, "history": hist}
