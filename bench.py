@@ -333,6 +333,50 @@ def conv_roofline(net_w, H, W, reps=20):
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
 
 
+def vpl_colouring(state, reps=50):
+    """HIP-event timing of sgi_color_vpls (vpl_light_color.cs: colour and specular position of every
+    visible VPL from its diffuse and distance cubes) on the frame's visible list, writing a scratch copy
+    of the records (the timed frames keep the generator's colours).  A queued matmul keeps the launches
+    back to back behind it, and one event pair brackets all of them (an event between every two
+    launches would add its own packet's time to each)."""
+    import torch
+
+    from stratusgfx_amd import _abi
+    from stratusgfx_amd import scene as sc
+    dif, cd = sc.diffuse_cubes(state.scene, state.dev)
+    state.cull()
+    fb = state.fb
+    recs = fb.vpl_updated.clone()
+    cap = min(state.n, _abi.MAX_TOTAL_VPLS_PER_FRAME)
+
+    def run():
+        state.gi.color_vpls(fb.visible, fb.visible_entries, cd, state.cubes, cap, recs)
+
+    for _ in range(3):
+        run()
+    a = torch.randn((4096, 4096), device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    for _ in range(3):
+        a = a @ a
+        a = a / a.abs().amax()
+    ev[0].record()
+    for _ in range(reps):
+        run()
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    visible = int(fb.visible[0].item())
+    atlas_gb = sum(t.numel() for t in dif) / 1e9
+    del dif, a
+    torch.cuda.empty_cache()
+    return {"kernel": "colorVplsKernel<F16>", "avg_launch_ms": round(ms, 4), "visible_vpls": visible,
+            "algorithmic_bytes": visible * (64 + 8 + 4 + 2 + 32),
+            "diffuse_cubes": f"6x{state.w.cube_res}x{state.w.cube_res} rgba8, {atlas_gb:.1f} GB",
+            "note": "one thread per visible VPL (two texels, one record); outside the timed GI frame, whose "
+                    "records arrive coloured"}
+
+
 def batched_views(args):
     """Many views of the city block at uneven resolutions, one launch per stage over all views
     in a captured graph, launch shape from the autotuner."""
@@ -648,8 +692,12 @@ def main():
         "roofline_network_conv": conv_roof,
         "roofline_plain_gather": gather_roof,
         "batched_views": None,
+        "vpl_colouring": None,
         "cpu_baseline": None,
     }
+    if plain_res is not None:
+        print("[bench] timing the VPL colouring pass", file=sys.stderr, flush=True)
+        result["vpl_colouring"] = vpl_colouring(plain)
     if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)

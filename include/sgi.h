@@ -18,7 +18,10 @@
  *  - Cube map atlases follow GL_TEXTURE_CUBE_MAP_ARRAY: texel (i, j) of face f
  *    of layer l is at ((l * 6 + f) * res + j) * res + i; faces are
  *    +X, -X, +Y, -Y, +Z, -Z and (s, t) come from the GL major-axis table.
- *    Texels hold distance / farPlane (Source/Shaders/shadowVpl.fs:17-35).
+ *    Distance texels hold distance / farPlane (Source/Shaders/shadowVpl.fs:17-35);
+ *    diffuse texels hold the nearest surface's base colour as UNORM8 RGB
+ *    (shadowVpl.fs:37, the RGB8 colour attachment of
+ *    StratusRendererBackend.cpp:2419), read as k / 255.
  *  - Matrices are 4x4 column-major (glm / GL uniform layout).
  *
  * Errors: every entry point returns SGI_OK (0) or a negative SGI_E_* code and
@@ -96,9 +99,12 @@ typedef struct sgi_gbuffer {
     const float* ssao;                     /* [H][W]    SSAO; NULL = 1.0                */
 } sgi_gbuffer;
 
-enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1 };
+/* Texel formats of the cube atlases: distance cubes are F32 or F16; the diffuse
+ * (colour) cubes are RGBA8, 4 bytes per texel (r, g, b UNORM8 at bytes 0..2,
+ * byte 3 unused), the 4-byte texel GL keeps for an RGB8 attachment. */
+enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1, SGI_TEXEL_RGBA8 = 2 };
 
-/* VPL distance cube-map atlases (StratusRendererBackend.cpp:2397-2449). */
+/* VPL cube-map atlases, distance or diffuse (StratusRendererBackend.cpp:2397-2449). */
 typedef struct sgi_cube_atlases {
     int32_t resolution;       /* face edge in texels */
     int32_t format;           /* SGI_TEXEL_F32 or SGI_TEXEL_F16 */
@@ -187,6 +193,24 @@ int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n,
 int sgi_visible_atlas_entries(sgi_ctx* ctx, const sgi_atlas_entry* shadowIndices,
                               const int32_t* visibleIndices, int32_t maxCount,
                               sgi_atlas_entry* out, void* stream);
+
+/* vpl_light_color.cs:41-76, dispatched by PerformVirtualPointLightCullingStage2_
+ * (StratusRendererBackend.cpp:1703-1731) over the compacted visible VPLs.  For
+ * each slot s < min(visibleIndices[0], maxCount), with e = visibleEntries[s]
+ * (sgi_visible_atlas_entries' output) and d = -infiniteLightDirection, the
+ * NEAREST texels of the diffuse and distance cubes of atlas e.index, layer
+ * e.layer, in direction d give
+ *   vplUpdatedData[s].color = (rgb * infiniteLightColor / 1 * intensity * colorMultiplier, 1)
+ *   vplUpdatedData[s].specularPosition = (position - 1.1 * (radius * distance) * infiniteLightDirection, 1)
+ * with every product and difference rounded on its own, in the reference's
+ * order; the other fields are left as they are.  The reference's multiplier is
+ * 50000 (vpl_light_color.cs:44).  diffuse: SGI_TEXEL_RGBA8 atlases; distance:
+ * F32 or F16; both of one atlas geometry (the reference keeps them as the two
+ * attachments of one framebuffer).  Needs sgi_set_frame.  vplUpdatedData:
+ * device [maxCount]; visibleEntries: device [maxCount]; 0 <= maxCount <= 4480. */
+int sgi_color_vpls(sgi_ctx* ctx, const int32_t* visibleIndices, const sgi_atlas_entry* visibleEntries,
+                   const sgi_cube_atlases* diffuse, const sgi_cube_atlases* distance, float colorMultiplier,
+                   int32_t maxCount, sgi_vpl* vplUpdatedData, void* stream);
 
 /* vpl_tiled_deferred_culling_stage1.cs:50-90 + _stage2.cs:61-129 (the dispatches
  * commented out at StratusRendererBackend.cpp:1733-1783).
@@ -300,11 +324,15 @@ typedef struct sgi_frame_io {
     float* giOut;                          /* [H][W][3] history out */
     float* reservoirOut;                   /* [H][W][4] */
     float* historyOut;                     /* [H][W]    */
+    /* VPL colouring (vpl_light_color.cs): NULL = the records arrive coloured */
+    const sgi_cube_atlases* diffuseCubes;  /* RGBA8, the geometry of cubes */
+    float vplColorMultiplier;              /* 50000 in the reference */
 } sgi_frame_io;
 
 /* RenderScene's GI steps (StratusRendererBackend.cpp:1982, 2010-2045) without
  * host round trips: cull_global -> gather the visible atlas entries ->
- * direct_light -> (cull_tiles) -> gather -> denoise. */
+ * (color_vpls, when diffuseCubes is set) -> direct_light -> (cull_tiles) ->
+ * gather -> denoise. */
 int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream);
 
 /* ---- batched views ---------------------------------------------------------

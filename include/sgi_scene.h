@@ -76,9 +76,12 @@ int sgi_scene_gbuffer_device(sgi_scene* scene, float* depth, float* normal, floa
                              float* baseReflectivity, float* roughnessMetallicAmbient,
                              float* velocity, float* id, void* stream);
 
-/* Distance cube maps of VPLs [first, first + count) into the atlases of `cubes`
+/* Cube maps of VPLs [first, first + count) into the atlases of `cubes`
  * (resolution, format, layers_per_atlas, num_atlases and atlas[] must be set;
- * for _host the atlas pointers are host memory). */
+ * for _host the atlas pointers are host memory): distance / farPlane for the
+ * F32 and F16 formats (the depth attachment shadowVpl.fs:29-32 writes), the
+ * nearest surface's albedo as UNORM8 RGB for SGI_TEXEL_RGBA8 (its colour
+ * attachment, shadowVpl.fs:37; 0 where nothing lies within farPlane). */
 int sgi_scene_cubes_host(const sgi_scene* scene, const sgi_cube_atlases* cubes,
                          int32_t first, int32_t count);
 int sgi_scene_cubes_device(sgi_scene* scene, const sgi_cube_atlases* cubes,

@@ -141,9 +141,10 @@ inline float halfToFloat(uint16_t h) {
 }
 
 // samplerCubeArray NEAREST lookup (GL 4.6 spec table 8.19, major axis ties
-// resolved x before y before z), CLAMP_TO_EDGE within the face.
+// resolved x before y before z), CLAMP_TO_EDGE within the face: the texel
+// offset inside the clamped atlas, returned with that atlas.
 template <class R>
-R cubeFetch(const sgi_cube_atlases& cubes, int atlas, R layerCoord, V3<R> d) {
+size_t cubeTexel(const sgi_cube_atlases& cubes, int& atlas, R layerCoord, V3<R> d) {
     atlas = clampi(atlas, 0, cubes.num_atlases - 1);  // out-of-range entries read atlas 0
     const R ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
     int face;
@@ -163,7 +164,12 @@ R cubeFetch(const sgi_cube_atlases& cubes, int atlas, R layerCoord, V3<R> d) {
     const int i = texelIndex(std::floor(s * R(res)), res);
     const int j = texelIndex(std::floor(t * R(res)), res);
     const int layer = clampi((int)std::floor(layerCoord + R(0.5)), 0, cubes.layers_per_atlas - 1);
-    const size_t off = (((size_t)layer * 6 + face) * res + j) * res + i;
+    return (((size_t)layer * 6 + face) * res + j) * res + i;
+}
+
+template <class R>
+R cubeFetch(const sgi_cube_atlases& cubes, int atlas, R layerCoord, V3<R> d) {
+    const size_t off = cubeTexel(cubes, atlas, layerCoord, d);
     if (cubes.format == SGI_TEXEL_F16) return R(halfToFloat(((const uint16_t*)cubes.atlas[atlas])[off]));
     return R(((const float*)cubes.atlas[atlas])[off]);
 }
@@ -841,6 +847,38 @@ void denoise(const sgi_frame& fr, const GB& cur, const GB& prev, int numVisible,
 }  // namespace
 
 // ============================== C ABI ==========================================
+// vpl_light_color.cs:41-76 (dispatched by PerformVirtualPointLightCullingStage2_,
+// StratusRendererBackend.cpp:1703-1731), FP32, one VPL slot at a time:
+//   color    = textureLod(diffuseCubeMaps[e.index], vec4(-L, e.layer), 0).rgb * sunColor   (:52)
+//   magnitude = radius * textureLod(shadowCubeMaps[e.index], vec4(-L, e.layer), 0).r   (:53)
+//   specularPos = vec4(position.xyz - 1.1 * magnitude * L, 1)                          (:71)
+//   lightData[i].color = vec4(color / totalColors * intensity * colorMultiplier, 1)    (:73)
+// with totalColors = 1 (:56; the 8-sample loop at :57-66 is commented out).  The
+// RGB8 attachment reads as k / 255 (GL UNORM conversion).
+void colorVpls(const sgi_frame& fr, int count, const sgi_atlas_entry* entries, const sgi_cube_atlases& diffuse,
+               const sgi_cube_atlases& dist, float colorMultiplier, sgi_vpl* vpls) {
+    const V3<float> L = ld3<float>(fr.infiniteLightDirection);
+    const V3<float> d = v3(-L.x, -L.y, -L.z);
+    const float sun[3] = {fr.infiniteLightColor[0], fr.infiniteLightColor[1], fr.infiniteLightColor[2]};
+    const float Lv[3] = {L.x, L.y, L.z};
+    for (int i = 0; i < count; ++i) {
+        const sgi_atlas_entry e = entries[i];
+        int a = e.index;
+        const size_t off = cubeTexel<float>(diffuse, a, float(e.layer), d);
+        const uint8_t* rgb = (const uint8_t*)diffuse.atlas[a] + off * 4;
+        sgi_vpl& v = vpls[i];
+        const float magnitude = v.radius * cubeFetch<float>(dist, e.index, float(e.layer), d);
+        const float totalColors = 1.0f;
+        for (int k = 0; k < 3; ++k) {
+            const float color = (float)rgb[k] / 255.0f * sun[k];
+            v.color[k] = color / totalColors * v.intensity * colorMultiplier;
+            v.specularPosition[k] = v.position[k] - 1.1f * magnitude * Lv[k];
+        }
+        v.color[3] = 1.0f;
+        v.specularPosition[3] = 1.0f;
+    }
+}
+
 extern "C" {
 
 // -- unit functions (for closed-form pins) --
@@ -902,6 +940,11 @@ void orc_world_pos(float u, float v, float depth, const float invPV[16], float o
 }
 
 // -- pass-level functions --
+void orc_color_vpls(const sgi_frame* fr, int count, const sgi_atlas_entry* entries, const sgi_cube_atlases* diffuse,
+                    const sgi_cube_atlases* dist, float colorMultiplier, sgi_vpl* vpls) {
+    colorVpls(*fr, count, entries, *diffuse, *dist, colorMultiplier, vpls);
+}
+
 int orc_cull_global(const sgi_frame* fr, const sgi_csm* csm, const sgi_vpl* in, int n, sgi_vpl* out, int32_t* idx) {
     return cullGlobal<float>(*fr, *csm, in, n, out, idx);
 }

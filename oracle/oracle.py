@@ -53,6 +53,8 @@ def lib() -> ctypes.CDLL:
             "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
             "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
             "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+            "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
+                                      POINTER(_abi.CubeAtlases), c_float, c_void_p]),
             "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
                                         c_void_p]),
             "orc_cull_tiles": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_int, c_int, c_void_p, c_void_p,
@@ -90,6 +92,16 @@ def cull_global(frame: _abi.Frame, csm: _abi.Csm, vpls: np.ndarray):
     idx = np.zeros(1 + max(cap, 1), np.int32)
     count = lib().orc_cull_global(ctypes.byref(frame), ctypes.byref(csm), _p(vpls), n, _p(out), _p(idx))
     return out[:count], idx[:1 + count]
+
+
+def color_vpls(frame, vpls, entries, diffuse, distance, multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER):
+    """vpl_light_color.cs over the compacted records vpls [count,16] with their per-slot atlas entries:
+    returns a coloured copy."""
+    out = _f32(vpls).copy()
+    entries = np.ascontiguousarray(entries, np.int32)
+    lib().orc_color_vpls(ctypes.byref(frame), out.shape[0], _p(entries), ctypes.byref(diffuse), ctypes.byref(distance),
+                         multiplier, _p(out))
+    return out
 
 
 def cull_tiles(frame, gbuf, tile_size, K, vpls, visible, remap=0, t0=0, t1=None, tiles_x=None, tiles_y=None):

@@ -28,6 +28,8 @@ MAX_VPLS_PER_TILE_LIMIT = 64
 NUM_CASCADES = 4
 TEXEL_F32 = 0
 TEXEL_F16 = 1
+TEXEL_RGBA8 = 2
+REFERENCE_VPL_COLOR_MULTIPLIER = 50000.0  # vpl_light_color.cs:44
 PRECISION_FP32 = 0
 PRECISION_FP16 = 1
 SCENE_CORNELL = 0
@@ -96,6 +98,7 @@ class FrameIO(ctypes.Structure):
         ("tiles", c_void_p), ("direct", c_void_p), ("color", c_void_p), ("reservoir", c_void_p),
         ("prevGi", c_void_p), ("prevHistory", c_void_p), ("combined", c_void_p), ("giOut", c_void_p),
         ("reservoirOut", c_void_p), ("historyOut", c_void_p),
+        ("diffuseCubes", POINTER(CubeAtlases)), ("vplColorMultiplier", ctypes.c_float),
     ]
 
 
@@ -147,6 +150,7 @@ SGI_SIGNATURES = {
     "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
     "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
+    "sgi_color_vpls": (c_int32, [_P, _P, _P, POINTER(CubeAtlases), POINTER(CubeAtlases), c_float, _I, _P, _P]),
     "sgi_cull_tiles": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _I, _P, _P, _P]),
     "sgi_gather": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, _I, _P, _P, _P]),
     "sgi_gather_ex": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _P, POINTER(GatherOpts), _P,

@@ -200,6 +200,33 @@ int checkGatherArgs(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, con
 
 }  // namespace
 
+int sgi_color_vpls(sgi_ctx* ctx, const int32_t* visibleIndices, const sgi_atlas_entry* visibleEntries,
+                   const sgi_cube_atlases* diffuse, const sgi_cube_atlases* distance, float colorMultiplier,
+                   int32_t maxCount, sgi_vpl* vplUpdatedData, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_color_vpls: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_color_vpls: sgi_set_frame first");
+    if (maxCount < 0 || maxCount > SGI_MAX_TOTAL_VPLS_PER_FRAME)
+        return fail(SGI_E_INVALID, "sgi_color_vpls: maxCount must be within 0..4480");
+    if (!visibleIndices || !diffuse || !distance || (maxCount > 0 && (!visibleEntries || !vplUpdatedData)))
+        return fail(SGI_E_INVALID, "sgi_color_vpls: NULL buffer");
+    if (diffuse->format != SGI_TEXEL_RGBA8)
+        return fail(SGI_E_INVALID, "sgi_color_vpls: the diffuse atlases must be SGI_TEXEL_RGBA8");
+    if (diffuse->resolution != distance->resolution || diffuse->layers_per_atlas != distance->layers_per_atlas ||
+        diffuse->num_atlases != distance->num_atlases)
+        return fail(SGI_E_INVALID, "sgi_color_vpls: the diffuse and distance atlases must share one geometry");
+    if (!std::isfinite(colorMultiplier)) return fail(SGI_E_INVALID, "sgi_color_vpls: colorMultiplier must be finite");
+    sgi_dev::CubeArgs za, da;
+    int r = cubeArgs(distance, "sgi_color_vpls", za);
+    if (r != SGI_OK) return r;
+    sgi_cube_atlases geometry = *diffuse;  // the geometry checks of a distance atlas
+    geometry.format = SGI_TEXEL_F32;
+    if ((r = cubeArgs(&geometry, "sgi_color_vpls", da)) != SGI_OK) return r;
+    const hipError_t e = sgi_launch::colorVpls(da, za, distance->format, ctx->frameArgs, colorMultiplier,
+                                               visibleIndices, maxCount, visibleEntries, vplUpdatedData,
+                                               (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_color_vpls");
+}
+
 int sgi_gather(sgi_ctx* ctx, const sgi_gbuffer* g, const sgi_vpl* vpls, const int32_t* visibleIndices,
                const sgi_atlas_entry* shadowIndices, const sgi_cube_atlases* cubes, const int32_t* tiles,
                int32_t precision, float* color, float* reservoir, void* stream) {
@@ -297,6 +324,11 @@ int sgi_render_frame(sgi_ctx* ctx, const sgi_frame_io* io, void* stream) {
     const int maxCount = io->numVpls < SGI_MAX_TOTAL_VPLS_PER_FRAME ? io->numVpls : SGI_MAX_TOTAL_VPLS_PER_FRAME;
     if ((r = sgi_visible_atlas_entries(ctx, io->shadowIndices, io->visibleIndices, maxCount,
                                        io->visibleShadowIndices, stream)) != SGI_OK)
+        return r;
+    // PerformVirtualPointLightCullingStage2_: the vplColoring dispatch (vpl_light_color.cs)
+    if (io->diffuseCubes &&
+        (r = sgi_color_vpls(ctx, io->visibleIndices, io->visibleShadowIndices, io->diffuseCubes, io->cubes,
+                            io->vplColorMultiplier, maxCount, io->vplUpdatedData, stream)) != SGI_OK)
         return r;
     // deferred lighting pass (pbr.fs)
     if ((r = sgi_direct_light(ctx, io->gbuffer, io->direct, stream)) != SGI_OK) return r;

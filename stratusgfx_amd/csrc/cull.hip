@@ -13,6 +13,12 @@
 // registers (lane k holds entry k); candidates below the current K-th ratio
 // are inserted one at a time in slot order with a ballot for the position and
 // a lane shift, which reproduces the reference's strict-< insertion sort.
+//
+// colorVplsKernel — vpl_light_color.cs:41-76: one thread per visible slot
+// reads the diffuse and distance texels in direction -sun and writes the
+// slot's colour and specular position.
+#include <hip/hip_fp16.h>
+
 #include "launch.h"
 #include "views.h"
 
@@ -98,6 +104,39 @@ __global__ void gatherEntriesKernel(const sgi_atlas_entry* __restrict__ all, con
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int count = visible[0];
     if (s < count && s < maxCount) out[s] = all[visible[1 + s]];
+}
+
+// vpl_light_color.cs:49-74.  The diffuse and distance atlases have one
+// geometry (one framebuffer's two attachments), so both lookups of direction
+// -sun share one texel offset.  Built with -ffp-contract=off: every product and
+// difference is rounded on its own, in the shader's order.
+template <int ZFMT>
+__global__ __launch_bounds__(256) void colorVplsKernel(CubeArgs diffuse, CubeArgs dist, f3 sun, f3 sunColor,
+                                                        float mult, const int32_t* __restrict__ visible,
+                                                        int maxCount, const sgi_atlas_entry* __restrict__ entries,
+                                                        sgi_vpl* __restrict__ vpls) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int count = visible[0];
+    if (s >= count || s >= maxCount) return;
+    const sgi_atlas_entry e = entries[s];
+    const f3 d{-sun.x, -sun.y, -sun.z};
+    const uint32_t off = cubeTexelOffset(d, diffuse.res);
+    const uint32_t rgb = ((const uint32_t*)cubeLayerBase(diffuse, e.index, e.layer, 4))[off];
+    const void* zb = cubeLayerBase(dist, e.index, e.layer, ZFMT == SGI_TEXEL_F16 ? 2 : 4);
+    const float z = ZFMT == SGI_TEXEL_F16 ? __half2float(__ushort_as_half(((const unsigned short*)zb)[off]))
+                                          : ((const float*)zb)[off];
+    sgi_vpl& v = vpls[s];
+    const float intensity = v.intensity;
+    const float scaled = 1.1f * (v.radius * z);  // 1.1 * magnitude (:53, :71)
+    const float c[3] = {(float)(rgb & 0xFFu) / 255.0f, (float)((rgb >> 8) & 0xFFu) / 255.0f,
+                        (float)((rgb >> 16) & 0xFFu) / 255.0f};
+    const float sc[3] = {sunColor.x, sunColor.y, sunColor.z}, sd[3] = {sun.x, sun.y, sun.z};
+    for (int k = 0; k < 3; ++k) {
+        v.color[k] = c[k] * sc[k] * intensity * mult;  // color / totalColors (1) * intensity * colorMultiplier (:73)
+        v.specularPosition[k] = v.position[k] - scaled * sd[k];
+    }
+    v.color[3] = 1.0f;
+    v.specularPosition[3] = 1.0f;
 }
 
 // Batched: blocks [v * bpv, (v + 1) * bpv) gather view v's visible slots.
@@ -278,6 +317,22 @@ hipError_t cullGlobalViews(const sgi_batch::ViewDesc* views, int numViews, const
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(cullCompactViewsKernel, dim3(numViews), dim3(kCullThreads), 0, s, views, in, n);
+    return hipGetLastError();
+}
+
+hipError_t colorVpls(const sgi_dev::CubeArgs& diffuse, const sgi_dev::CubeArgs& dist, int distFormat,
+                     const sgi_dev::FrameArgs& fr, float mult, const int32_t* visible, int maxCount,
+                     const sgi_atlas_entry* entries, sgi_vpl* vpls, hipStream_t s) {
+    if (maxCount <= 0) return hipSuccess;
+    const f3 sun{fr.sunDir[0], fr.sunDir[1], fr.sunDir[2]};
+    const f3 col{fr.sunColor[0], fr.sunColor[1], fr.sunColor[2]};
+    const dim3 grid((unsigned)((maxCount + 255) / 256));
+    if (distFormat == SGI_TEXEL_F16)
+        hipLaunchKernelGGL(colorVplsKernel<SGI_TEXEL_F16>, grid, dim3(256), 0, s, diffuse, dist, sun, col, mult,
+                           visible, maxCount, entries, vpls);
+    else
+        hipLaunchKernelGGL(colorVplsKernel<SGI_TEXEL_F32>, grid, dim3(256), 0, s, diffuse, dist, sun, col, mult,
+                           visible, maxCount, entries, vpls);
     return hipGetLastError();
 }
 

@@ -15,6 +15,11 @@ hipError_t cullGlobal(const sgi_dev::CsmArgs& csm, const sgi_dev::FrameArgs& fr,
 hipError_t gatherVisibleEntries(const sgi_atlas_entry* all, const int32_t* visible, int maxCount,
                                 sgi_atlas_entry* out, hipStream_t s);
 
+// vplUpdatedData[s] colour and specular position from the diffuse / distance cubes
+hipError_t colorVpls(const sgi_dev::CubeArgs& diffuse, const sgi_dev::CubeArgs& dist, int distFormat,
+                     const sgi_dev::FrameArgs& fr, float mult, const int32_t* visible, int maxCount,
+                     const sgi_atlas_entry* entries, sgi_vpl* vpls, hipStream_t s);
+
 hipError_t cullTiles(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, int tileSize, int tilesX, int tilesY,
                      int K, const sgi_vpl* vpls, const int32_t* visible, int remap, sgi_tile_stage1* stage1,
                      int32_t* tiles, hipStream_t s);

@@ -456,7 +456,8 @@ uint16_t floatToHalf(float f) {
 
 bool checkCubes(const sgi_scene* s, const sgi_cube_atlases* c, int first, int count) {
     if (!s || !c) { sgi_set_error("sgi_scene_cubes: NULL argument"); return false; }
-    if (c->resolution <= 0 || (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16) ||
+    if (c->resolution <= 0 ||
+        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8) ||
         c->layers_per_atlas <= 0 || c->num_atlases <= 0 || c->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES) {
         sgi_set_error("sgi_scene_cubes: bad atlas description");
         return false;
@@ -634,9 +635,13 @@ int sgi_scene_cubes_host(const sgi_scene* s, const sgi_cube_atlases* c, int32_t 
         const int face = (int)(r / ((long)res * res));
         const int t = (int)((r / res) % res), i = (int)(r % res);
         const sgi_vpl& v = s->vpls[j];
-        const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, res);
         const int atlas = j / c->layers_per_atlas, layer = j % c->layers_per_atlas;
         const size_t off = (((size_t)layer * 6 + face) * res + t) * res + i;
+        if (c->format == SGI_TEXEL_RGBA8) {
+            ((uint32_t*)c->atlas[atlas])[off] = sgi_rt::cubeTexelColor(sv, v.position, v.farPlane, face, i, t, res);
+            return;
+        }
+        const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, res);
         if (c->format == SGI_TEXEL_F32) ((float*)c->atlas[atlas])[off] = val;
         else ((uint16_t*)c->atlas[atlas])[off] = floatToHalf(val);
     });

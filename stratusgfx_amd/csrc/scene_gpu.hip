@@ -85,9 +85,13 @@ __global__ void cubeKernel(sgi_rt::SceneView sv, CubeJob job) {
     const int face = (int)(r / ((long)job.res * job.res));
     const int t = (int)((r / job.res) % job.res), i = (int)(r % job.res);
     const sgi_vpl v = job.vpls[j];
-    const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, job.res);
     const int atlas = j / job.layersPerAtlas, layer = j % job.layersPerAtlas;
     const size_t off = (((size_t)layer * 6 + face) * job.res + t) * job.res + i;
+    if (job.format == SGI_TEXEL_RGBA8) {
+        ((uint32_t*)job.atlas[atlas])[off] = sgi_rt::cubeTexelColor(sv, v.position, v.farPlane, face, i, t, job.res);
+        return;
+    }
+    const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, job.res);
     if (job.format == SGI_TEXEL_F32) ((float*)job.atlas[atlas])[off] = val;
     else ((__half*)job.atlas[atlas])[off] = __float2half_rn(val);
 }
@@ -118,7 +122,8 @@ int sgi_scene_gbuffer_device(sgi_scene* s, float* depth, float* normal, float* a
 
 int sgi_scene_cubes_device(sgi_scene* s, const sgi_cube_atlases* c, int32_t first, int32_t count, void* stream) {
     if (!s || !c) { sgi_set_error("sgi_scene_cubes_device: NULL argument"); return SGI_E_INVALID; }
-    if (c->resolution <= 0 || (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16) ||
+    if (c->resolution <= 0 ||
+        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8) ||
         c->layers_per_atlas <= 0 || c->num_atlases <= 0 || c->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES ||
         first < 0 || count < 0 || first + count > (int)s->vpls.size() ||
         (long)(first + count) > (long)c->layers_per_atlas * c->num_atlases) {

@@ -253,6 +253,26 @@ SGI_HD inline float cubeTexelValue(const SceneView& s, const float pos[3], float
     return v < 1.0f ? v : 1.0f;
 }
 
+// Colour texel of a VPL's diffuse cube (shadowVpl.fs:37 writes the base colour
+// of the nearest surface into the RGB8 attachment, StratusRendererBackend.cpp:
+// 2419): UNORM8 r, g, b in bytes 0..2 (round to nearest), byte 3 = 255.  Where
+// nothing lies within farPlane the layer keeps its clear value 0 (:1453-1454);
+// the reference draws its skybox there, and these scenes have none.
+SGI_HD inline uint32_t cubeTexelColor(const SceneView& s, const float pos[3], float farPlane,
+                                      int face, int i, int j, int res) {
+    float d[3];
+    cubeTexelDir(face, i, j, res, d);
+    const Hit h = traceScene(s, pos, d, 0.0f, farPlane);
+    if (h.box < 0) return 0u;
+    uint32_t rgba = 0xFF000000u;
+    for (int k = 0; k < 3; ++k) {
+        float a = s.boxes[h.box].albedo[k];
+        a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+        rgba |= (uint32_t)(a * 255.0f + 0.5f) << (8 * k);
+    }
+    return rgba;
+}
+
 struct CsmRays {
     float right[3], up[3], fwd[3];  // light basis, fwd points away from the sun
     float cx[4], cy[4], dk[4];      // cascade centre (light x, y) and width

@@ -65,6 +65,12 @@ class GiPass:
         _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
                   _stream(stream))
 
+    def color_vpls(self, visible, visible_entries, diffuse: _abi.CubeAtlases, distance: _abi.CubeAtlases,
+                   max_count, vpls, color_multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER, stream=None) -> None:
+        """vpl_light_color.cs: colour and specular position of the visible VPL records, in place."""
+        _abi.call(self.lib, "sgi_color_vpls", self.ctx, _ptr(visible), _ptr(visible_entries), ctypes.byref(diffuse),
+                  ctypes.byref(distance), color_multiplier, max_count, _ptr(vpls), _stream(stream))
+
     def cull_tiles(self, gbuf: _abi.Gbuffer, vpls, visible, tiles, remap=0, stage1=None, stream=None) -> None:
         _abi.call(self.lib, "sgi_cull_tiles", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), remap,
                   _ptr(stage1), _ptr(tiles), _stream(stream))
@@ -149,8 +155,11 @@ class FrameBuffers:
         )
 
     def frame_io(self, gbuf_ref, prev_gbuf_ref, vpls, num_vpls, entries, cubes_ref, use_tiles: bool,
-                 precision=_abi.PRECISION_FP32, prev_gi=None, prev_history=None) -> _abi.FrameIO:
+                 precision=_abi.PRECISION_FP32, prev_gi=None, prev_history=None, diffuse_ref=None,
+                 color_multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER) -> _abi.FrameIO:
         io = _abi.FrameIO()
+        io.diffuseCubes = ctypes.pointer(diffuse_ref) if diffuse_ref is not None else None
+        io.vplColorMultiplier = color_multiplier
         io.gbuffer = ctypes.pointer(gbuf_ref)
         io.prevGbuffer = ctypes.pointer(prev_gbuf_ref) if prev_gbuf_ref is not None else None
         io.vplData = _ptr(vpls)

@@ -210,6 +210,31 @@ def generate_device(scene: Scene, device="cuda", stream=None) -> Inputs:
     return inp
 
 
+def diffuse_cubes(scene: Scene, inp: Inputs, stream=None) -> tuple[list, _abi.CubeAtlases]:
+    """The VPLs' diffuse (colour) cube atlases, RGBA8 [layers][6][res][res][4], with the geometry of
+    inp's distance atlases: numpy arrays ray-cast on the host when inp holds host data, torch uint8
+    tensors ray-cast on the device otherwise."""
+    shape = (inp.layers, 6, inp.cube_res, inp.cube_res, 4)
+    host = isinstance(inp.atlases[0], np.ndarray)
+    if host:
+        atlases = [np.zeros(shape, np.uint8) for _ in inp.atlases]
+    else:
+        import torch
+        atlases = [torch.empty(shape, dtype=torch.uint8, device=inp.atlases[0].device) for _ in inp.atlases]
+    c = _abi.CubeAtlases()
+    c.resolution, c.format, c.layers_per_atlas, c.num_atlases = inp.cube_res, _abi.TEXEL_RGBA8, inp.layers, len(atlases)
+    for i, a in enumerate(atlases):
+        c.atlas[i] = inp.ptr(a)
+    if host:
+        _abi.call(scene.lib, "sgi_scene_cubes_host", scene.handle, ctypes.byref(c), 0, scene.w.num_vpls)
+    else:
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _abi.call(scene.lib, "sgi_scene_cubes_device", scene.handle, ctypes.byref(c), 0, scene.w.num_vpls, s)
+        torch.cuda.synchronize()
+    return atlases, c
+
+
 def to_host(inp: Inputs) -> Inputs:
     """Copies device inputs to numpy (for the oracle)."""
     def h(a):
