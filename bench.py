@@ -377,6 +377,37 @@ def vpl_colouring(state, reps=50):
                     "records arrive coloured"}
 
 
+def gbuffer_unpack_roofline(state, reps=20):
+    """HIP-event timing of sgi_gbuffer_unpack at the frame size: the reference's RGB8 / RGBA8 / RG8 / RGB8
+    attachments and RG16F velocity (18 B per pixel) into the FP32 planes (56 B per pixel)."""
+    import torch
+
+    from stratusgfx_amd import _abi
+    W, H = state.w.width, state.w.height
+    i32 = dict(dtype=torch.int32, device="cuda")
+    src = [torch.randint(-2**31, 2**31 - 1, (H, W), **i32) for _ in range(4)]
+    refl = torch.randint(-2**15, 2**15 - 1, (H, W), dtype=torch.int16, device="cuda")
+    packed = _abi.GbufferPacked(W, H, src[0].data_ptr(), src[1].data_ptr(), refl.data_ptr(), src[2].data_ptr(),
+                                src[3].data_ptr())
+    out = [torch.empty((H, W, c), dtype=torch.float32, device="cuda") for c in (3, 4, 2, 3, 2)]
+    for _ in range(3):
+        state.gi.gbuffer_unpack(packed, *out)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        state.gi.gbuffer_unpack(packed, *out)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    alg = W * H * (18 + 56)
+    achieved = alg / (ms * 1e-3) / 1e9
+    del src, refl, out
+    return {"kernel": "unpackKernel (packed G-buffer attachments -> FP32 planes)", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg, "avg_launch_ms": round(ms, 4)}
+
+
 def batched_views(args):
     """Many views of the city block at uneven resolutions, one launch per stage over all views
     in a captured graph, launch shape from the autotuner."""
@@ -693,11 +724,13 @@ def main():
         "roofline_plain_gather": gather_roof,
         "batched_views": None,
         "vpl_colouring": None,
+        "roofline_gbuffer_unpack": None,
         "cpu_baseline": None,
     }
     if plain_res is not None:
         print("[bench] timing the VPL colouring pass", file=sys.stderr, flush=True)
         result["vpl_colouring"] = vpl_colouring(plain)
+        result["roofline_gbuffer_unpack"] = gbuffer_unpack_roofline(plain)
     if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)

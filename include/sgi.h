@@ -104,6 +104,23 @@ typedef struct sgi_gbuffer {
  * byte 3 unused), the 4-byte texel GL keeps for an RGB8 attachment. */
 enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1, SGI_TEXEL_RGBA8 = 2 };
 
+/* The G-buffer attachments as the reference allocates them
+ * (StratusRendererBackend.cpp:384-424), row-major [H][W] like the FP32 planes:
+ * 8-bit UNORM channels read as k / 255, RGB8 attachments in 4-byte texels
+ * (r, g, b in bytes 0..2, byte 3 unused; the texel GL keeps for RGB8), RG8 in
+ * 2-byte texels, velocity as two IEEE binary16 values.  Depth and ids are read
+ * as FP32 values and need no conversion: the FP32 sgi_gbuffer takes them as
+ * they are. */
+typedef struct sgi_gbuffer_packed {
+    int32_t width;
+    int32_t height;
+    const uint32_t* normals;                  /* RGB8:  n * 0.5 + 0.5            */
+    const uint32_t* albedo;                   /* RGBA8: rgb albedo, a = emissive.r */
+    const uint16_t* baseReflectivity;         /* RG8:   reflectance, emissive.g   */
+    const uint32_t* roughnessMetallicAmbient; /* RGB8:  roughness, metallic, emissive.b */
+    const uint16_t* velocity;                 /* RG16F; NULL = no velocity plane */
+} sgi_gbuffer_packed;
+
 /* VPL cube-map atlases, distance or diffuse (StratusRendererBackend.cpp:2397-2449). */
 typedef struct sgi_cube_atlases {
     int32_t resolution;       /* face edge in texels */
@@ -171,6 +188,16 @@ int sgi_set_frame(sgi_ctx* ctx, const sgi_frame* frame);
 int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* csm);
 
 /* ---- kernels ------------------------------------------------------------- */
+
+/* Samples the reference's packed G-buffer attachments into the FP32 planes
+ * that every pass reads (the texel-centre, NEAREST-equivalent reads the GLSL
+ * passes make of these attachments: UNORM8 -> k / 255, binary16 -> FP32, both
+ * exact in FP32).  Outputs: device normal [H][W][3], albedo [H][W][4],
+ * baseReflectivity [H][W][2], roughnessMetallicAmbient [H][W][3], velocity
+ * [H][W][2] (may be NULL when packed->velocity is NULL).  The packed size must
+ * be the context's viewport. */
+int sgi_gbuffer_unpack(sgi_ctx* ctx, const sgi_gbuffer_packed* packed, float* normal, float* albedo,
+                       float* baseReflectivity, float* roughnessMetallicAmbient, float* velocity, void* stream);
 
 /* viscull_vpls.cs:47-120 driven by PerformVirtualPointLightCullingStage1_
  * (StratusRendererBackend.cpp:1578-1661).  For i < n, VPL i is sun-visible when

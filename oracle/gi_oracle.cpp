@@ -940,6 +940,26 @@ void orc_world_pos(float u, float v, float depth, const float invPV[16], float o
 }
 
 // -- pass-level functions --
+// The reads the GLSL passes make of the reference's G-buffer attachments
+// (StratusRendererBackend.cpp:384-424): an RGB8 / RGBA8 / RG8 UNORM code k is
+// k / 255 (GL 4.6 section 2.3.5.1, c / (2^8 - 1)); RG16F velocity widens to
+// FP32.  Texel layouts as sgi_gbuffer_packed states them.
+void orc_gbuffer_unpack(const sgi_gbuffer_packed* g, float* normal, float* albedo, float* refl, float* rma,
+                        float* vel) {
+    const long n = (long)g->width * g->height;
+    const auto code = [](uint32_t v, int b) { return (float)((v >> (8 * b)) & 0xFFu) / 255.0f; };
+    for (long p = 0; p < n; ++p) {
+        for (int k = 0; k < 3; ++k) {
+            normal[3 * p + k] = code(g->normals[p], k);
+            rma[3 * p + k] = code(g->roughnessMetallicAmbient[p], k);
+        }
+        for (int k = 0; k < 4; ++k) albedo[4 * p + k] = code(g->albedo[p], k);
+        for (int k = 0; k < 2; ++k) refl[2 * p + k] = code(g->baseReflectivity[p], k);
+        if (vel && g->velocity)
+            for (int k = 0; k < 2; ++k) vel[2 * p + k] = halfToFloat(g->velocity[2 * p + k]);
+    }
+}
+
 void orc_color_vpls(const sgi_frame* fr, int count, const sgi_atlas_entry* entries, const sgi_cube_atlases* diffuse,
                     const sgi_cube_atlases* dist, float colorMultiplier, sgi_vpl* vpls) {
     colorVpls(*fr, count, entries, *diffuse, *dist, colorMultiplier, vpls);

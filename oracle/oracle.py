@@ -53,6 +53,7 @@ def lib() -> ctypes.CDLL:
             "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
             "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
             "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+            "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
             "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
                                       POINTER(_abi.CubeAtlases), c_float, c_void_p]),
             "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
@@ -92,6 +93,14 @@ def cull_global(frame: _abi.Frame, csm: _abi.Csm, vpls: np.ndarray):
     idx = np.zeros(1 + max(cap, 1), np.int32)
     count = lib().orc_cull_global(ctypes.byref(frame), ctypes.byref(csm), _p(vpls), n, _p(out), _p(idx))
     return out[:count], idx[:1 + count]
+
+
+def gbuffer_unpack(packed: _abi.GbufferPacked, with_velocity=True):
+    """The FP32 planes (normal, albedo, reflectivity, rma, velocity) the passes read from packed attachments."""
+    H, W = packed.height, packed.width
+    out = [np.zeros((H, W, c), np.float32) for c in (3, 4, 2, 3, 2)]
+    lib().orc_gbuffer_unpack(ctypes.byref(packed), *[_p(a) for a in out[:4]], _p(out[4]) if with_velocity else None)
+    return out
 
 
 def color_vpls(frame, vpls, entries, diffuse, distance, multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER):

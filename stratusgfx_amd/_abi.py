@@ -89,6 +89,11 @@ class Config(ctypes.Structure):
                 ("max_vpls_per_tile", c_int32)]
 
 
+class GbufferPacked(ctypes.Structure):
+    _fields_ = [("width", c_int32), ("height", c_int32), ("normals", c_void_p), ("albedo", c_void_p),
+                ("baseReflectivity", c_void_p), ("roughnessMetallicAmbient", c_void_p), ("velocity", c_void_p)]
+
+
 class FrameIO(ctypes.Structure):
     _fields_ = [
         ("gbuffer", POINTER(Gbuffer)), ("prevGbuffer", POINTER(Gbuffer)),
@@ -148,6 +153,7 @@ SGI_SIGNATURES = {
     "sgi_tile_counts": (c_int32, [_P, POINTER(c_int32), POINTER(c_int32)]),
     "sgi_set_frame": (c_int32, [_P, POINTER(Frame)]),
     "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
+    "sgi_gbuffer_unpack": (c_int32, [_P, POINTER(GbufferPacked), _P, _P, _P, _P, _P, _P]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
     "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
     "sgi_color_vpls": (c_int32, [_P, _P, _P, POINTER(CubeAtlases), POINTER(CubeAtlases), c_float, _I, _P, _P]),

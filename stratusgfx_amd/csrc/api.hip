@@ -126,6 +126,24 @@ int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* c) {
     return SGI_OK;
 }
 
+int sgi_gbuffer_unpack(sgi_ctx* ctx, const sgi_gbuffer_packed* g, float* normal, float* albedo,
+                       float* baseReflectivity, float* roughnessMetallicAmbient, float* velocity, void* stream) {
+    if (!ctx || !g) return fail(SGI_E_INVALID, "sgi_gbuffer_unpack: NULL argument");
+    if (g->width != ctx->cfg.width || g->height != ctx->cfg.height)
+        return fail(SGI_E_INVALID, "sgi_gbuffer_unpack: size differs from the context's viewport");
+    if (!g->normals || !g->albedo || !g->baseReflectivity || !g->roughnessMetallicAmbient || !normal || !albedo ||
+        !baseReflectivity || !roughnessMetallicAmbient || (g->velocity && !velocity))
+        return fail(SGI_E_INVALID, "sgi_gbuffer_unpack: NULL plane");
+    const auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+    if (!al(g->normals, 4) || !al(g->albedo, 4) || !al(g->roughnessMetallicAmbient, 4) ||
+        !al(g->baseReflectivity, 2) || (g->velocity && !al(g->velocity, 4)) || !al(normal, 4) || !al(albedo, 4) ||
+        !al(baseReflectivity, 4) || !al(roughnessMetallicAmbient, 4) || (velocity && !al(velocity, 4)))
+        return fail(SGI_E_INVALID, "sgi_gbuffer_unpack: misaligned plane (texel alignment)");
+    const hipError_t e = sgi_launch::gbufferUnpack(*g, normal, albedo, baseReflectivity, roughnessMetallicAmbient,
+                                                   g->velocity ? velocity : nullptr, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gbuffer_unpack");
+}
+
 int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n, sgi_vpl* vplUpdatedData, int32_t* visibleIndices,
                     void* stream) {
     if (!ctx) return fail(SGI_E_INVALID, "sgi_cull_global: NULL context");

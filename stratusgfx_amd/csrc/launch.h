@@ -15,6 +15,10 @@ hipError_t cullGlobal(const sgi_dev::CsmArgs& csm, const sgi_dev::FrameArgs& fr,
 hipError_t gatherVisibleEntries(const sgi_atlas_entry* all, const int32_t* visible, int maxCount,
                                 sgi_atlas_entry* out, hipStream_t s);
 
+// packed G-buffer attachments -> FP32 planes (velocity may be NULL)
+hipError_t gbufferUnpack(const sgi_gbuffer_packed& g, float* normal, float* albedo, float* refl, float* rma,
+                         float* velocity, hipStream_t s);
+
 // vplUpdatedData[s] colour and specular position from the diffuse / distance cubes
 hipError_t colorVpls(const sgi_dev::CubeArgs& diffuse, const sgi_dev::CubeArgs& dist, int distFormat,
                      const sgi_dev::FrameArgs& fr, float mult, const int32_t* visible, int maxCount,

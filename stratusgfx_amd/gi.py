@@ -65,6 +65,12 @@ class GiPass:
         _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
                   _stream(stream))
 
+    def gbuffer_unpack(self, packed: _abi.GbufferPacked, normal, albedo, refl, rma, velocity=None,
+                       stream=None) -> None:
+        """The reference's packed G-buffer attachments into the FP32 planes the passes read."""
+        _abi.call(self.lib, "sgi_gbuffer_unpack", self.ctx, ctypes.byref(packed), _ptr(normal), _ptr(albedo),
+                  _ptr(refl), _ptr(rma), _ptr(velocity), _stream(stream))
+
     def color_vpls(self, visible, visible_entries, diffuse: _abi.CubeAtlases, distance: _abi.CubeAtlases,
                    max_count, vpls, color_multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER, stream=None) -> None:
         """vpl_light_color.cs: colour and specular position of the visible VPL records, in place."""
