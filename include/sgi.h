@@ -325,6 +325,15 @@ int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
                 float* combined, float* giOut, float* reservoirOut, float* historyOut,
                 void* stream);
 
+/* gammaTonemap.fs:125-142 (PerformGammaTonemapPostFx_, StratusRendererBackend.cpp:
+ * 2356-2366), the consumer of the lit buffer: per channel the ACES film fit
+ * saturate(c (2.51 c + 0.03) / (c (2.43 c + 0.59) + 0.14)), then pow(., 1 / gamma),
+ * stored to the RGBA8 UNORM target as floor(saturate(v) * 255 + 0.5), alpha 255.
+ * lit: device [H][W][3] FP32 (the viewport); out: device [H][W] RGBA8 texels
+ * (r in byte 0).  gamma: 2.2 in the reference.  (The reference's TAA, bloom and
+ * atmospheric passes run before it and FXAA after it; they are not built.) */
+int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void* stream);
+
 /* ---- one frame ------------------------------------------------------------ */
 
 /* Caller-owned buffers of one GI frame, in the order RenderScene uses them. */

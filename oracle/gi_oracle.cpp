@@ -960,6 +960,25 @@ void orc_gbuffer_unpack(const sgi_gbuffer_packed* g, float* normal, float* albed
     }
 }
 
+// gammaTonemap.fs:20-29, 112-114, 125-142: per channel the ACES film fit in
+// the shader's order, saturate, pow(x, 1.0 / gamma), the RGBA8 UNORM store
+// floor(x * 255 + 0.5).  film (may be NULL) receives the value before the store.
+void orc_tonemap(const float* lit, long n, float gamma, uint32_t* out, float* film) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    const float invGamma = 1.0f / gamma;
+    const auto sat = [](float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); };
+    for (long p = 0; p < n; ++p) {
+        uint32_t t = 0xFF000000u;
+        for (int k = 0; k < 3; ++k) {
+            const float x = lit[3 * p + k];
+            const float v = sat(std::pow(sat((x * (a * x + b)) / (x * (c * x + d) + e)), invGamma));
+            if (film) film[3 * p + k] = v;
+            t |= (uint32_t)(v * 255.0f + 0.5f) << (8 * k);
+        }
+        out[p] = t;
+    }
+}
+
 void orc_color_vpls(const sgi_frame* fr, int count, const sgi_atlas_entry* entries, const sgi_cube_atlases* diffuse,
                     const sgi_cube_atlases* dist, float colorMultiplier, sgi_vpl* vpls) {
     colorVpls(*fr, count, entries, *diffuse, *dist, colorMultiplier, vpls);

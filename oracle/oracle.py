@@ -53,6 +53,7 @@ def lib() -> ctypes.CDLL:
             "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
             "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
             "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+            "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
             "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
             "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
                                       POINTER(_abi.CubeAtlases), c_float, c_void_p]),
@@ -93,6 +94,16 @@ def cull_global(frame: _abi.Frame, csm: _abi.Csm, vpls: np.ndarray):
     idx = np.zeros(1 + max(cap, 1), np.int32)
     count = lib().orc_cull_global(ctypes.byref(frame), ctypes.byref(csm), _p(vpls), n, _p(out), _p(idx))
     return out[:count], idx[:1 + count]
+
+
+def tonemap(lit, gamma=2.2):
+    """gammaTonemap.fs: returns (RGBA8 texels [H][W] uint32, the values before the store [H][W][3])."""
+    lit = _f32(lit)
+    H, W = lit.shape[0], lit.shape[1]
+    out = np.zeros((H, W), np.uint32)
+    film = np.zeros((H, W, 3), np.float32)
+    lib().orc_tonemap(_p(lit), H * W, gamma, _p(out), _p(film))
+    return out, film
 
 
 def gbuffer_unpack(packed: _abi.GbufferPacked, with_velocity=True):

@@ -126,6 +126,16 @@ int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* c) {
     return SGI_OK;
 }
 
+int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_tonemap: NULL context");
+    if (!lit || !out) return fail(SGI_E_INVALID, "sgi_tonemap: NULL buffer");
+    if (!(gamma > 0.0f) || !std::isfinite(gamma)) return fail(SGI_E_INVALID, "sgi_tonemap: gamma must be > 0");
+    if (((uintptr_t)lit & 3) || ((uintptr_t)out & 3)) return fail(SGI_E_INVALID, "sgi_tonemap: misaligned buffer");
+    const hipError_t e =
+        sgi_launch::tonemap(lit, (long)ctx->cfg.width * ctx->cfg.height, gamma, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_tonemap");
+}
+
 int sgi_gbuffer_unpack(sgi_ctx* ctx, const sgi_gbuffer_packed* g, float* normal, float* albedo,
                        float* baseReflectivity, float* roughnessMetallicAmbient, float* velocity, void* stream) {
     if (!ctx || !g) return fail(SGI_E_INVALID, "sgi_gbuffer_unpack: NULL argument");

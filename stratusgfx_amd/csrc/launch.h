@@ -15,6 +15,9 @@ hipError_t cullGlobal(const sgi_dev::CsmArgs& csm, const sgi_dev::FrameArgs& fr,
 hipError_t gatherVisibleEntries(const sgi_atlas_entry* all, const int32_t* visible, int maxCount,
                                 sgi_atlas_entry* out, hipStream_t s);
 
+// lit [n][3] FP32 -> ACES film + gamma -> RGBA8 texels
+hipError_t tonemap(const float* lit, long n, float gamma, uint32_t* out, hipStream_t s);
+
 // packed G-buffer attachments -> FP32 planes (velocity may be NULL)
 hipError_t gbufferUnpack(const sgi_gbuffer_packed& g, float* normal, float* albedo, float* refl, float* rma,
                          float* velocity, hipStream_t s);

@@ -65,6 +65,10 @@ class GiPass:
         _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
                   _stream(stream))
 
+    def tonemap(self, lit, out, gamma=2.2, stream=None) -> None:
+        """gammaTonemap.fs: ACES film + gamma of the lit buffer into RGBA8 texels."""
+        _abi.call(self.lib, "sgi_tonemap", self.ctx, _ptr(lit), gamma, _ptr(out), _stream(stream))
+
     def gbuffer_unpack(self, packed: _abi.GbufferPacked, normal, albedo, refl, rma, velocity=None,
                        stream=None) -> None:
         """The reference's packed G-buffer attachments into the FP32 planes the passes read."""
