@@ -325,6 +325,20 @@ int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
                 float* combined, float* giOut, float* reservoirOut, float* historyOut,
                 void* stream);
 
+/* taa.fs:12-100 (PerformTaaPostFx_, StratusRendererBackend.cpp:2326-2352), the
+ * first consumer of the lit buffer: per pixel, with uv the texel centre and
+ * vel the velocity texel, the previous frame sampled bilinearly (LINEAR,
+ * CLAMP_TO_EDGE) at uv - vel is clamped, in the tonemapped space
+ * c / (c + 1), to the range of the current texel and its 4 neighbours
+ * (edges clamped); current and clamped history are mapped back with
+ * c / (1 - c) and mixed: out = cur * (1 - 0.9) + hist * 0.9.  screen, prevScreen,
+ * out: device [H][W][3] FP32 of the viewport (out aliases neither input;
+ * the reference's history is its RGB8 previousFrameBuffer, :451-453, whose
+ * values k / 255 the caller passes as FP32); velocity: the G-buffer's [H][W][2]
+ * plane, NULL = 0.  The caller keeps out as the next frame's prevScreen. */
+int sgi_taa(sgi_ctx* ctx, const float* screen, const float* prevScreen, const float* velocity, float* out,
+            void* stream);
+
 /* gammaTonemap.fs:125-142 (PerformGammaTonemapPostFx_, StratusRendererBackend.cpp:
  * 2356-2366), the consumer of the lit buffer: per channel the ACES film fit
  * saturate(c (2.51 c + 0.03) / (c (2.43 c + 0.59) + 0.14)), then pow(., 1 / gamma),

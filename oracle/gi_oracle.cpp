@@ -960,6 +960,33 @@ void orc_gbuffer_unpack(const sgi_gbuffer_packed* g, float* normal, float* albed
     }
 }
 
+// taa.fs:12-100 with aa_common.glsl's tonemap c / (c + 1) and inverse
+// c / (1 - c): velocity texel (NEAREST), screen at texel centres (the 4
+// textureOffset taps clamp to the edge), prevScreen LINEAR at uv - velocity,
+// clamp(prev, tonemap(min), tonemap(max)), mix(cur, prev, 0.9) = cur (1 - 0.9)
+// + prev 0.9.
+void orc_taa(const float* screen, const float* prev, const float* vel, int W, int H, float* out) {
+    const Plane S{screen, W, H, 3}, P{prev, W, H, 3};
+    const auto tm = [](float c) { return c / (c + 1.0f); };
+    const auto itm = [](float c) { return c / (1.0f - c); };
+    const float a = 0.9f;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t p = (size_t)y * W + x;
+            const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+            const float vu = vel ? vel[2 * p] : 0.0f, vv = vel ? vel[2 * p + 1] : 0.0f;
+            for (int c = 0; c < 3; ++c) {
+                const float cur = S.at(x, y, c);
+                const float c1 = S.at(x, y + 1, c), c2 = S.at(x, y - 1, c), c3 = S.at(x + 1, y, c), c4 = S.at(x - 1, y, c);
+                const float prevColor = tm(bilinear<float>(P, u - vu, v - vv, c));
+                const float mn = tm(std::fmin(cur, std::fmin(c1, std::fmin(c2, std::fmin(c3, c4)))));
+                const float mx = tm(std::fmax(cur, std::fmax(c1, std::fmax(c2, std::fmax(c3, c4)))));
+                const float pc = std::fmin(std::fmax(prevColor, mn), mx);
+                out[p * 3 + c] = itm(tm(cur)) * (1.0f - a) + itm(pc) * a;
+            }
+        }
+}
+
 // gammaTonemap.fs:20-29, 112-114, 125-142: per channel the ACES film fit in
 // the shader's order, saturate, pow(x, 1.0 / gamma), the RGBA8 UNORM store
 // floor(x * 255 + 0.5).  film (may be NULL) receives the value before the store.

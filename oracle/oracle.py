@@ -53,6 +53,7 @@ def lib() -> ctypes.CDLL:
             "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
             "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
             "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+            "orc_taa": (None, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
             "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
             "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
             "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
@@ -94,6 +95,16 @@ def cull_global(frame: _abi.Frame, csm: _abi.Csm, vpls: np.ndarray):
     idx = np.zeros(1 + max(cap, 1), np.int32)
     count = lib().orc_cull_global(ctypes.byref(frame), ctypes.byref(csm), _p(vpls), n, _p(out), _p(idx))
     return out[:count], idx[:1 + count]
+
+
+def taa(screen, prev, velocity=None):
+    """taa.fs: the anti-aliased frame [H][W][3]."""
+    screen, prev = _f32(screen), _f32(prev)
+    H, W = screen.shape[0], screen.shape[1]
+    out = np.zeros((H, W, 3), np.float32)
+    vel = None if velocity is None else _f32(velocity)
+    lib().orc_taa(_p(screen), _p(prev), _p(vel), W, H, _p(out))
+    return out
 
 
 def tonemap(lit, gamma=2.2):

@@ -153,6 +153,7 @@ SGI_SIGNATURES = {
     "sgi_tile_counts": (c_int32, [_P, POINTER(c_int32), POINTER(c_int32)]),
     "sgi_set_frame": (c_int32, [_P, POINTER(Frame)]),
     "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
+    "sgi_taa": (c_int32, [_P, _P, _P, _P, _P, _P]),
     "sgi_tonemap": (c_int32, [_P, _P, c_float, _P, _P]),
     "sgi_gbuffer_unpack": (c_int32, [_P, POINTER(GbufferPacked), _P, _P, _P, _P, _P, _P]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),

@@ -126,6 +126,16 @@ int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* c) {
     return SGI_OK;
 }
 
+int sgi_taa(sgi_ctx* ctx, const float* screen, const float* prevScreen, const float* velocity, float* out,
+            void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_taa: NULL context");
+    if (!screen || !prevScreen || !out) return fail(SGI_E_INVALID, "sgi_taa: NULL buffer");
+    if (out == screen || out == prevScreen) return fail(SGI_E_INVALID, "sgi_taa: out must not alias an input");
+    const hipError_t e =
+        sgi_launch::taa(screen, prevScreen, velocity, ctx->cfg.width, ctx->cfg.height, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_taa");
+}
+
 int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void* stream) {
     if (!ctx) return fail(SGI_E_INVALID, "sgi_tonemap: NULL context");
     if (!lit || !out) return fail(SGI_E_INVALID, "sgi_tonemap: NULL buffer");

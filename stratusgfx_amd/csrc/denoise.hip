@@ -286,9 +286,51 @@ __global__ __launch_bounds__(256) void denoiseTemporalViewsKernel(const sgi_batc
                  d.prevHistory, d.combined, d.giOut, d.historyOut, x, y);
 }
 
+// taa.fs:12-100 (PerformTaaPostFx_, StratusRendererBackend.cpp:2326-2352): the
+// velocity texel, the current colour and its 4 neighbours at texel centres,
+// the previous frame bilinearly at uv - velocity, the history clamped to the
+// neighbourhood's range in tonemapped space (aa_common.glsl: c / (c + 1)),
+// both mapped back (c / (1 - c)) and mixed 0.1 / 0.9.  Per pixel: 5 current
+// taps, 4 history taps, one velocity texel; FP32 in, FP32 out.
+__device__ __forceinline__ float aaTonemap(float c) { return c / (c + 1.0f); }
+__device__ __forceinline__ float aaInverse(float c) { return c / (1.0f - c); }
+
+__global__ __launch_bounds__(256) void taaKernel(const float* __restrict__ screen, const float* __restrict__ prev,
+                                                  const float* __restrict__ velocity, int W, int H,
+                                                  float* __restrict__ out) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)W * H) return;
+    const int x = (int)(p % W), y = (int)(p / W);
+    const float u = texelCentre(x, W), v = texelCentre(y, H);
+    const float vu = velocity ? velocity[2 * p] : 0.0f, vv = velocity ? velocity[2 * p + 1] : 0.0f;
+    const float pu = u - vu, pv = v - vv;
+    const int xs[4] = {x, x, min(x + 1, W - 1), max(x - 1, 0)};
+    const int ys[4] = {min(y + 1, H - 1), max(y - 1, 0), y, y};
+    const float mixA = 0.9f;
+    for (int c = 0; c < 3; ++c) {
+        const float cur = screen[3 * p + c];
+        const float n1 = screen[((size_t)ys[0] * W + xs[0]) * 3 + c], n2 = screen[((size_t)ys[1] * W + xs[1]) * 3 + c];
+        const float n3 = screen[((size_t)ys[2] * W + xs[2]) * 3 + c], n4 = screen[((size_t)ys[3] * W + xs[3]) * 3 + c];
+        const float prevColor = aaTonemap(bilinear(prev, W, H, 3, pu, pv, c));
+        const float mn = aaTonemap(fminf(cur, fminf(n1, fminf(n2, fminf(n3, n4)))));
+        const float mx = aaTonemap(fmaxf(cur, fmaxf(n1, fmaxf(n2, fmaxf(n3, n4)))));
+        const float curT = aaTonemap(cur);
+        const float prevC = fminf(fmaxf(prevColor, mn), mx);
+        const float a = aaInverse(curT), b = aaInverse(prevC);
+        out[3 * p + c] = a * (1.0f - mixA) + b * mixA;
+    }
+}
+
 }  // namespace
 
 namespace sgi_launch {
+
+hipError_t taa(const float* screen, const float* prev, const float* velocity, int W, int H, float* out, hipStream_t s) {
+    const long n = (long)W * H;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(taaKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, screen, prev, velocity, W, H, out);
+    return hipGetLastError();
+}
 
 hipError_t denoiseSpatial(const GbufArgs& cur, const FrameArgs& fr, const int32_t* visible, const float* reservoir,
                           float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask, bool mergeOnly,

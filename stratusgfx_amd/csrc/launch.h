@@ -15,6 +15,9 @@ hipError_t cullGlobal(const sgi_dev::CsmArgs& csm, const sgi_dev::FrameArgs& fr,
 hipError_t gatherVisibleEntries(const sgi_atlas_entry* all, const int32_t* visible, int maxCount,
                                 sgi_atlas_entry* out, hipStream_t s);
 
+// taa.fs: current frame + history -> anti-aliased frame (velocity may be NULL)
+hipError_t taa(const float* screen, const float* prev, const float* velocity, int W, int H, float* out, hipStream_t s);
+
 // lit [n][3] FP32 -> ACES film + gamma -> RGBA8 texels
 hipError_t tonemap(const float* lit, long n, float gamma, uint32_t* out, hipStream_t s);
 

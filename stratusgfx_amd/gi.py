@@ -65,6 +65,11 @@ class GiPass:
         _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
                   _stream(stream))
 
+    def taa(self, screen, prev_screen, velocity, out, stream=None) -> None:
+        """taa.fs: the current frame blended with the neighbourhood-clamped history."""
+        _abi.call(self.lib, "sgi_taa", self.ctx, _ptr(screen), _ptr(prev_screen), _ptr(velocity), _ptr(out),
+                  _stream(stream))
+
     def tonemap(self, lit, out, gamma=2.2, stream=None) -> None:
         """gammaTonemap.fs: ACES film + gamma of the lit buffer into RGBA8 texels."""
         _abi.call(self.lib, "sgi_tonemap", self.ctx, _ptr(lit), gamma, _ptr(out), _stream(stream))
