@@ -408,6 +408,38 @@ def gbuffer_unpack_roofline(state, reps=20):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg, "avg_launch_ms": round(ms, 4)}
 
 
+def post_fx_roofline(state, reps=20):
+    """HIP-event timing of the lit buffer's consumers at the frame size on the plain form's last frame:
+    sgi_taa (current frame, history, velocity -> FP32; 44 B per pixel) and sgi_tonemap (FP32 -> RGBA8;
+    16 B per pixel)."""
+    import torch
+    W, H = state.w.width, state.w.height
+    lit = state.fb.combined
+    hist = lit.clone()
+    vel = torch.zeros((H, W, 2), dtype=torch.float32, device="cuda")
+    taa_out = torch.empty_like(lit)
+    rgba = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    out = {}
+    for name, run, per_px in (("taa", lambda: state.gi.taa(lit, hist, vel, taa_out), 44),
+                              ("tonemap", lambda: state.gi.tonemap(lit, rgba), 16)):
+        for _ in range(3):
+            run()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            run()
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        achieved = W * H * per_px / (ms * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": W * H * per_px,
+                     "avg_launch_ms": round(ms, 4)}
+    del hist, vel, taa_out, rgba
+    return out
+
+
 def batched_views(args):
     """Many views of the city block at uneven resolutions, one launch per stage over all views
     in a captured graph, launch shape from the autotuner."""
@@ -725,12 +757,14 @@ def main():
         "batched_views": None,
         "vpl_colouring": None,
         "roofline_gbuffer_unpack": None,
+        "roofline_post_fx": None,
         "cpu_baseline": None,
     }
     if plain_res is not None:
         print("[bench] timing the VPL colouring pass", file=sys.stderr, flush=True)
         result["vpl_colouring"] = vpl_colouring(plain)
         result["roofline_gbuffer_unpack"] = gbuffer_unpack_roofline(plain)
+        result["roofline_post_fx"] = post_fx_roofline(plain)
     if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)
