@@ -295,6 +295,11 @@ __global__ __launch_bounds__(256) void denoiseTemporalViewsKernel(const sgi_batc
 __device__ __forceinline__ float aaTonemap(float c) { return c / (c + 1.0f); }
 __device__ __forceinline__ float aaInverse(float c) { return c / (1.0f - c); }
 
+__device__ __forceinline__ f3 texel3(const float* t, int W, int x, int y) {
+    const float* q = t + ((size_t)y * W + x) * 3;
+    return f3{q[0], q[1], q[2]};
+}
+
 __global__ __launch_bounds__(256) void taaKernel(const float* __restrict__ screen, const float* __restrict__ prev,
                                                   const float* __restrict__ velocity, int W, int H,
                                                   float* __restrict__ out) {
@@ -303,21 +308,32 @@ __global__ __launch_bounds__(256) void taaKernel(const float* __restrict__ scree
     const int x = (int)(p % W), y = (int)(p / W);
     const float u = texelCentre(x, W), v = texelCentre(y, H);
     const float vu = velocity ? velocity[2 * p] : 0.0f, vv = velocity ? velocity[2 * p + 1] : 0.0f;
-    const float pu = u - vu, pv = v - vv;
-    const int xs[4] = {x, x, min(x + 1, W - 1), max(x - 1, 0)};
-    const int ys[4] = {min(y + 1, H - 1), max(y - 1, 0), y, y};
+    // bilinear history coordinates (GL LINEAR, CLAMP_TO_EDGE), shared by the three channels
+    const float fu = (u - vu) * (float)W - 0.5f, fv = (v - vv) * (float)H - 0.5f;
+    const float i0f = floorf(fu), j0f = floorf(fv);
+    const float a = fu - i0f, b = fv - j0f;
+    const int ia = texelIndex(i0f, W), ib = texelIndex(i0f + 1.0f, W);
+    const int ja = texelIndex(j0f, H), jb = texelIndex(j0f + 1.0f, H);
+    const f3 h00 = texel3(prev, W, ia, ja), h10 = texel3(prev, W, ib, ja);
+    const f3 h01 = texel3(prev, W, ia, jb), h11 = texel3(prev, W, ib, jb);
+    const f3 c0 = texel3(screen, W, x, y);
+    const f3 c1 = texel3(screen, W, x, min(y + 1, H - 1)), c2 = texel3(screen, W, x, max(y - 1, 0));
+    const f3 c3 = texel3(screen, W, min(x + 1, W - 1), y), c4 = texel3(screen, W, max(x - 1, 0), y);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
     const float mixA = 0.9f;
+    const float h00c[3] = {h00.x, h00.y, h00.z}, h10c[3] = {h10.x, h10.y, h10.z};
+    const float h01c[3] = {h01.x, h01.y, h01.z}, h11c[3] = {h11.x, h11.y, h11.z};
+    const float cc[5][3] = {{c0.x, c0.y, c0.z}, {c1.x, c1.y, c1.z}, {c2.x, c2.y, c2.z}, {c3.x, c3.y, c3.z},
+                            {c4.x, c4.y, c4.z}};
     for (int c = 0; c < 3; ++c) {
-        const float cur = screen[3 * p + c];
-        const float n1 = screen[((size_t)ys[0] * W + xs[0]) * 3 + c], n2 = screen[((size_t)ys[1] * W + xs[1]) * 3 + c];
-        const float n3 = screen[((size_t)ys[2] * W + xs[2]) * 3 + c], n4 = screen[((size_t)ys[3] * W + xs[3]) * 3 + c];
-        const float prevColor = aaTonemap(bilinear(prev, W, H, 3, pu, pv, c));
-        const float mn = aaTonemap(fminf(cur, fminf(n1, fminf(n2, fminf(n3, n4)))));
-        const float mx = aaTonemap(fmaxf(cur, fmaxf(n1, fmaxf(n2, fmaxf(n3, n4)))));
+        const float cur = cc[0][c];
+        const float hist = w00 * h00c[c] + w10 * h10c[c] + w01 * h01c[c] + w11 * h11c[c];
+        const float prevColor = aaTonemap(hist);
+        const float mn = aaTonemap(fminf(cur, fminf(cc[1][c], fminf(cc[2][c], fminf(cc[3][c], cc[4][c])))));
+        const float mx = aaTonemap(fmaxf(cur, fmaxf(cc[1][c], fmaxf(cc[2][c], fmaxf(cc[3][c], cc[4][c])))));
         const float curT = aaTonemap(cur);
         const float prevC = fminf(fmaxf(prevColor, mn), mx);
-        const float a = aaInverse(curT), b = aaInverse(prevC);
-        out[3 * p + c] = a * (1.0f - mixA) + b * mixA;
+        out[3 * p + c] = aaInverse(curT) * (1.0f - mixA) + aaInverse(prevC) * mixA;
     }
 }
 
