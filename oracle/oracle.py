@@ -29,52 +29,60 @@ def build(force: bool = False) -> None:
                        stdout=subprocess.DEVNULL)
 
 
+_SIGNATURES = {
+    "orc_hash1": (c_uint32, [c_uint32]),
+    "orc_hash3": (c_uint32, [c_uint32, c_uint32, c_uint32]),
+    "orc_random3": (c_float, [c_float, c_float, c_float]),
+    "orc_ndf_ggx": (c_double, [c_double, c_double]),
+    "orc_ndf_ggx_f": (c_float, [c_float, c_float]),
+    "orc_vis_smith": (c_double, [c_double, c_double, c_double]),
+    "orc_fresnel": (c_double, [c_double, c_double]),
+    "orc_fd_burley": (c_double, [c_double, c_double, c_double, c_double]),
+    "orc_vpl_atten": (c_double, [c_double, c_double]),
+    "orc_brdf_burley": (None, [c_void_p] * 4 + [c_double] * 4 + [c_int, c_void_p]),
+    "orc_vpl_lighting": (None, [c_void_p] * 7 + [c_double] * 5 + [c_int, c_void_p]),
+    "orc_cube_fetch": (c_double, [POINTER(_abi.CubeAtlases), c_int, c_int, c_double, c_double, c_double]),
+    "orc_shadow1": (c_float, [POINTER(_abi.CubeAtlases), c_int, c_int, c_float, c_void_p, c_void_p,
+                              c_float, c_float]),
+    "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
+    "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
+    "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
+    "orc_taa": (None, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
+    "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
+    "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
+                              POINTER(_abi.CubeAtlases), c_float, c_void_p]),
+    "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
+                                c_void_p]),
+    "orc_cull_tiles": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_int, c_int, c_void_p, c_void_p,
+                              c_int, c_int, c_int, c_void_p, c_void_p]),
+    "orc_gather": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_void_p, c_int, c_void_p,
+                          POINTER(_abi.CubeAtlases), c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                          c_int, c_int, c_void_p, c_void_p]),
+    "orc_direct_light": (None, [POINTER(_abi.Frame), POINTER(_abi.Csm), POINTER(_abi.Gbuffer), c_int, c_int,
+                                c_void_p]),
+    "orc_denoise": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), POINTER(_abi.Gbuffer), c_int]
+                    + [c_void_p] * 9),
+}
+
+
+def load(path: str) -> ctypes.CDLL:
+    """Loads a build of the oracle at `path` with the exported signatures set
+    (the tests also load deliberately mutated builds to show that a pin fails)."""
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        sig = {
-            "orc_hash1": (c_uint32, [c_uint32]),
-            "orc_hash3": (c_uint32, [c_uint32, c_uint32, c_uint32]),
-            "orc_random3": (c_float, [c_float, c_float, c_float]),
-            "orc_ndf_ggx": (c_double, [c_double, c_double]),
-            "orc_ndf_ggx_f": (c_float, [c_float, c_float]),
-            "orc_vis_smith": (c_double, [c_double, c_double, c_double]),
-            "orc_fresnel": (c_double, [c_double, c_double]),
-            "orc_fd_burley": (c_double, [c_double, c_double, c_double, c_double]),
-            "orc_vpl_atten": (c_double, [c_double, c_double]),
-            "orc_brdf_burley": (None, [c_void_p] * 4 + [c_double] * 4 + [c_int, c_void_p]),
-            "orc_vpl_lighting": (None, [c_void_p] * 7 + [c_double] * 5 + [c_int, c_void_p]),
-            "orc_cube_fetch": (c_double, [POINTER(_abi.CubeAtlases), c_int, c_int, c_double, c_double, c_double]),
-            "orc_shadow1": (c_float, [POINTER(_abi.CubeAtlases), c_int, c_int, c_float, c_void_p, c_void_p,
-                                      c_float, c_float]),
-            "orc_shadow_compare": (c_double, [c_void_p, c_int, c_double, c_double, c_int, c_double]),
-            "orc_infinite_shadow": (c_float, [POINTER(_abi.Csm), c_void_p, c_void_p, c_void_p, c_int]),
-            "orc_world_pos": (None, [c_float, c_float, c_float, c_void_p, c_void_p]),
-            "orc_taa": (None, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
-            "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
-            "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
-            "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
-                                      POINTER(_abi.CubeAtlases), c_float, c_void_p]),
-            "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
-                                        c_void_p]),
-            "orc_cull_tiles": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_int, c_int, c_void_p, c_void_p,
-                                      c_int, c_int, c_int, c_void_p, c_void_p]),
-            "orc_gather": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), c_void_p, c_int, c_void_p,
-                                  POINTER(_abi.CubeAtlases), c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
-                                  c_int, c_int, c_void_p, c_void_p]),
-            "orc_direct_light": (None, [POINTER(_abi.Frame), POINTER(_abi.Csm), POINTER(_abi.Gbuffer), c_int, c_int,
-                                        c_void_p]),
-            "orc_denoise": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), POINTER(_abi.Gbuffer), c_int]
-                            + [c_void_p] * 9),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = load(LIB)
     return _lib
 
 
@@ -152,8 +160,9 @@ def cull_tiles(frame, gbuf, tile_size, K, vpls, visible, remap=0, t0=0, t1=None,
 
 
 def gather(frame, gbuf, vpls, num_visible, entries, cubes, tiles=None, tile_size=16, K=12, mode=GATHER_RANDOM,
-           precision=0, y0=0, y1=None, samples=8, rejection_limit=5):
-    """vpl_pbr_gi.fs over rows [y0, y1): returns (color [rows,W,3], reservoir [rows,W,4])."""
+           precision=0, y0=0, y1=None, samples=8, rejection_limit=5, L=None):
+    """vpl_pbr_gi.fs over rows [y0, y1): returns (color [rows,W,3], reservoir [rows,W,4]).
+    L: a library from load() in place of the oracle's own build."""
     W, H = gbuf.width, gbuf.height
     y1 = H if y1 is None else y1
     dt = np.float32 if precision == 0 else np.float64
@@ -162,7 +171,8 @@ def gather(frame, gbuf, vpls, num_visible, entries, cubes, tiles=None, tile_size
     vpls = _f32(vpls)
     entries = np.ascontiguousarray(entries, np.int32)
     t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
-    lib().orc_gather(ctypes.byref(frame), ctypes.byref(gbuf), _p(vpls), num_visible, _p(entries), ctypes.byref(cubes),
+    (L or lib()).orc_gather(ctypes.byref(frame), ctypes.byref(gbuf), _p(vpls), num_visible, _p(entries),
+                            ctypes.byref(cubes),
                      _p(t), tile_size, K, mode, samples, rejection_limit, precision, y0, y1, _p(color), _p(res))
     return color, res
 
