@@ -256,12 +256,12 @@ def main(argv=None) -> None:
                "init": os.path.basename(a.init) if a.init else None, "first_scene": a.first_scene,
                "loss_scale": a.loss_scale,
                "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
-               "before": base, "after": after, "threshold": best_t, "history": hist}
+               "before": base, "after": after, "threshold": best_t, "history_points": len(hist)}
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
         json.dump(summary, f, indent=1)
-    if a.log:
+    if a.log:  # the loss curve stays out of the weights folder
         with open(a.log, "w") as f:
-            json.dump(summary, f, indent=1)
+            json.dump(dict(summary, history=hist), f, indent=1)
     print(f"[done] threshold {best_t} relMSE {best:.5f}; weights {a.out}", flush=True)
 
 
