@@ -496,7 +496,7 @@ def main():
     w_plain = dataclasses.replace(sc.CITY_1080P, name="city-1920x1080-4096vpl", num_vpls=4096)
     plain = FormState(w_plain)
     learned = FormState(w_learned)
-    texel_bytes = 2 if w_plain.cube_format == _abi.TEXEL_F16 else 4
+    texel_bytes = 4 if w_plain.cube_format == _abi.TEXEL_F32 else 2
     H, W = w_plain.height, w_plain.width
     f32 = dict(dtype=torch.float32, device="cuda")
 

@@ -99,10 +99,15 @@ typedef struct sgi_gbuffer {
     const float* ssao;                     /* [H][W]    SSAO; NULL = 1.0                */
 } sgi_gbuffer;
 
-/* Texel formats of the cube atlases: distance cubes are F32 or F16; the diffuse
+/* Texel formats of the cube atlases.  Distance cubes are F32, F16 (IEEE
+ * binary16) or UNORM16.  UNORM16 is the reference's own VPL cube format: the
+ * depth attachment of CreateShadowMap3DCache_ (DEPTH, BITS_16,
+ * StratusRendererBackend.cpp:296,2412) is GL_DEPTH_COMPONENT16
+ * (StratusTexture.cpp:542), whose code k reads as k / 65535 (correctly rounded
+ * to FP32), so an engine's atlases can be passed in unchanged.  The diffuse
  * (colour) cubes are RGBA8, 4 bytes per texel (r, g, b UNORM8 at bytes 0..2,
  * byte 3 unused), the 4-byte texel GL keeps for an RGB8 attachment. */
-enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1, SGI_TEXEL_RGBA8 = 2 };
+enum { SGI_TEXEL_F32 = 0, SGI_TEXEL_F16 = 1, SGI_TEXEL_RGBA8 = 2, SGI_TEXEL_UNORM16 = 3 };
 
 /* The G-buffer attachments as the reference allocates them
  * (StratusRendererBackend.cpp:384-424), row-major [H][W] like the FP32 planes:
@@ -124,7 +129,7 @@ typedef struct sgi_gbuffer_packed {
 /* VPL cube-map atlases, distance or diffuse (StratusRendererBackend.cpp:2397-2449). */
 typedef struct sgi_cube_atlases {
     int32_t resolution;       /* face edge in texels */
-    int32_t format;           /* SGI_TEXEL_F32 or SGI_TEXEL_F16 */
+    int32_t format;           /* distance: SGI_TEXEL_F32, _F16 or _UNORM16; diffuse: SGI_TEXEL_RGBA8 */
     int32_t layers_per_atlas; /* cube layers in each atlas */
     int32_t num_atlases;      /* <= SGI_MAX_TOTAL_SHADOW_ATLASES */
     const void* atlas[SGI_MAX_TOTAL_SHADOW_ATLASES]; /* device [layers][6][res][res] */
@@ -232,12 +237,51 @@ int sgi_visible_atlas_entries(sgi_ctx* ctx, const sgi_atlas_entry* shadowIndices
  * with every product and difference rounded on its own, in the reference's
  * order; the other fields are left as they are.  The reference's multiplier is
  * 50000 (vpl_light_color.cs:44).  diffuse: SGI_TEXEL_RGBA8 atlases; distance:
- * F32 or F16; both of one atlas geometry (the reference keeps them as the two
+ * F32, F16 or UNORM16; both of one atlas geometry (the reference keeps them as the two
  * attachments of one framebuffer).  Needs sgi_set_frame.  vplUpdatedData:
  * device [maxCount]; visibleEntries: device [maxCount]; 0 <= maxCount <= 4480. */
 int sgi_color_vpls(sgi_ctx* ctx, const int32_t* visibleIndices, const sgi_atlas_entry* visibleEntries,
                    const sgi_cube_atlases* diffuse, const sgi_cube_atlases* distance, float colorMultiplier,
                    int32_t maxCount, sgi_vpl* vplUpdatedData, void* stream);
+
+/* World-space triangles of the static meshes the VPL cube pass draws (the
+ * reference draws staticPbrMeshes with their model matrices applied in
+ * shadow.vs:31, StratusRendererBackend.cpp:1529-1536). */
+typedef struct sgi_triangles {
+    int32_t count;
+    const float* positions;  /* device [count][3][3]: world-space vertices */
+    const uint32_t* colors;  /* device [count] RGBA8 base colour (r in byte 0); needed for diffuse output */
+} sgi_triangles;
+
+/* PointLight::lightNearPlane (StratusLight.h:291), the near plane of every VPL cube face. */
+#define SGI_REFERENCE_VPL_NEAR_PLANE 0.1f
+
+/* The VPL shadow pre-pass (the cube update loop of
+ * StratusRendererBackend.cpp:1426-1575 with shadowVpl.fs:17-37 and
+ * shadow.vs:22-33): for each s < numUpdates, light v = vpls[updates[s]] is
+ * rendered into cube entries[updates[s]] (layer * 6 + face of that atlas) of
+ * the distance atlases and, when diffuse is not NULL, of the diffuse atlases.
+ * Per face f, the view is lookAt(pos, pos + axis_f, up_f) (:873-886) with
+ * perspective(90 deg, 1, nearPlane, v.farPlane) (:1450); the layer is
+ * cleared (distance 1, colour 0) and every triangle drawn with fragment value
+ * saturate(|P - pos| / farPlane) and its base colour, depth test LESS: per
+ * texel the smallest stored value (the distance format's F32 value, binary16
+ * or UNORM16 code) over the triangles the texel's pixel-centre ray meets with
+ * view depth in [nearPlane, farPlane], the first such triangle in list order
+ * on ties giving the colour (alpha 255).  The caller plays the reference's
+ * light-update queue: updates lists the lights popped this frame (the
+ * reference pops at most maxShadowUpdatesPerFrame = 5, StratusRendererBackend.h:
+ * 378) or every light for an initial fill.  The reference also draws its
+ * skybox into texels nothing covers (:1555-1570); there is no skybox here, so
+ * they keep the clear.  An entry outside the atlases is skipped.
+ * distance: F32, F16 or UNORM16 (the reference's GL_DEPTH_COMPONENT16);
+ * diffuse: RGBA8 atlases of the same geometry, or NULL.  nearPlane:
+ * SGI_REFERENCE_VPL_NEAR_PLANE in the reference.  The context keeps a list
+ * workspace (about 24 B per triangle per light of a batch, batches within
+ * 256 MB), allocated on first use. */
+int sgi_render_vpl_cubes(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_vpl* vpls, const int32_t* updates,
+                         int32_t numUpdates, const sgi_atlas_entry* entries, float nearPlane,
+                         const sgi_cube_atlases* distance, const sgi_cube_atlases* diffuse, void* stream);
 
 /* vpl_tiled_deferred_culling_stage1.cs:50-90 + _stage2.cs:61-129 (the dispatches
  * commented out at StratusRendererBackend.cpp:1733-1783).
@@ -335,18 +379,50 @@ int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
  * out: device [H][W][3] FP32 of the viewport (out aliases neither input;
  * the reference's history is its RGB8 previousFrameBuffer, :451-453, whose
  * values k / 255 the caller passes as FP32); velocity: the G-buffer's [H][W][2]
- * plane, NULL = 0.  The caller keeps out as the next frame's prevScreen. */
+ * plane, NULL = 0.  The next frame's prevScreen is sgi_taa_history(out). */
 int sgi_taa(sgi_ctx* ctx, const float* screen, const float* prevScreen, const float* velocity, float* out,
             void* stream);
+
+/* The history copy at the end of PerformTaaPostFx_ (StratusRendererBackend.cpp:
+ * 2342-2349): the TAA output, held by the reference in an RGBA16F target
+ * (:708-711), blitted NEAREST into the RGB8 UNORM previousFrameBuffer
+ * (:449-453).  Per channel: binary16 rounding (to nearest even), clamp to
+ * [0, 1], the UNORM8 store floor(v * 255 + 0.5), read back as k / 255 into
+ * prevScreen (FP32, what sgi_taa takes).  HDR values above 1 are clamped, as
+ * the reference's history clamps them.  taaOut, prevScreen: device [H][W][3]
+ * of the viewport; they may alias (the kernel is elementwise). */
+int sgi_taa_history(sgi_ctx* ctx, const float* taaOut, float* prevScreen, void* stream);
 
 /* gammaTonemap.fs:125-142 (PerformGammaTonemapPostFx_, StratusRendererBackend.cpp:
  * 2356-2366), the consumer of the lit buffer: per channel the ACES film fit
  * saturate(c (2.51 c + 0.03) / (c (2.43 c + 0.59) + 0.14)), then pow(., 1 / gamma),
  * stored to the RGBA8 UNORM target as floor(saturate(v) * 255 + 0.5), alpha 255.
  * lit: device [H][W][3] FP32 (the viewport); out: device [H][W] RGBA8 texels
- * (r in byte 0).  gamma: 2.2 in the reference.  (The reference's TAA, bloom and
- * atmospheric passes run before it and FXAA after it; they are not built.) */
+ * (r in byte 0).  gamma: 2.2 in the reference.  (The reference's TAA runs
+ * before it and FXAA after it, both built here; its bloom and atmospheric
+ * passes between TAA and the tonemap are not.) */
 int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void* stream);
+
+/* The uniforms of fxaa_smoothing.fs (:72-101), which the reference never sets
+ * (their defaults 0.0625, 0.125 and 1.0 apply; pass NULL for them). */
+typedef struct sgi_fxaa_params {
+    float contrastThreshold;
+    float relativeThreshold;
+    float subpixelBlending;
+} sgi_fxaa_params;
+
+/* PerformFxaaPostFx_ (StratusRendererBackend.cpp:2293-2321), the last pass of
+ * the chain: fxaa_luminance.fs (the tonemapped colour with the luminance
+ * 0.2126 r + 0.7152 g + 0.0722 b of its saturated rgb, aa_common.glsl:7-9,
+ * stored as an RGBA8 target) then fxaa_smoothing.fs over that target (LINEAR,
+ * CLAMP_TO_EDGE): edge detection on the plus-shaped neighbourhood's contrast,
+ * the sub-pixel blend factor, the walk along the edge (steps 1, 1.5, 2 x 7, 4,
+ * then a guess jump of 8 texels) and one bilinear tap offset across the edge,
+ * stored RGBA8 with alpha 255.  Both passes run in one kernel: each tap
+ * recomputes the luminance target's alpha from the tonemapped codes.
+ * tonemapped, out: device [H][W] RGBA8 texels of the viewport (sgi_tonemap's
+ * output; r in byte 0), not aliased.  params: NULL = the reference's values. */
+int sgi_fxaa(sgi_ctx* ctx, const uint32_t* tonemapped, const sgi_fxaa_params* params, uint32_t* out, void* stream);
 
 /* ---- one frame ------------------------------------------------------------ */
 

@@ -20,6 +20,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "../include/sgi.h"
@@ -140,6 +143,31 @@ inline float halfToFloat(uint16_t h) {
     return f;
 }
 
+// FP32 -> IEEE binary16, round to nearest, ties to even (the conversion a
+// 16F render target applies to a fragment's output); overflow to infinity.
+inline uint16_t floatToHalfRN(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) return sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u);  // NaN / infinity
+    const int e = (int)(ax >> 23) - 127 + 15;                                    // biased binary16 exponent
+    if (e >= 31) return sign | 0x7C00u;
+    if (e <= 0) {  // subnormal or zero: value = ax's significand >> (14 - e) in units of 2^-24
+        if (e < -10) return sign;
+        const uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+        const int shift = 14 - e;  // 14 .. 24
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        return sign | (uint16_t)q;
+    }
+    uint32_t q = ((uint32_t)e << 10) | ((ax >> 13) & 0x3FFu);
+    const uint32_t rem = ax & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;  // a carry into the exponent is still correct
+    return sign | (uint16_t)q;
+}
+
 // samplerCubeArray NEAREST lookup (GL 4.6 spec table 8.19, major axis ties
 // resolved x before y before z), CLAMP_TO_EDGE within the face: the texel
 // offset inside the clamped atlas, returned with that atlas.
@@ -171,6 +199,8 @@ template <class R>
 R cubeFetch(const sgi_cube_atlases& cubes, int atlas, R layerCoord, V3<R> d) {
     const size_t off = cubeTexel(cubes, atlas, layerCoord, d);
     if (cubes.format == SGI_TEXEL_F16) return R(halfToFloat(((const uint16_t*)cubes.atlas[atlas])[off]));
+    // GL_DEPTH_COMPONENT16: code k reads as k / (2^16 - 1), rounded to FP32
+    if (cubes.format == SGI_TEXEL_UNORM16) return R(float(((const uint16_t*)cubes.atlas[atlas])[off]) / 65535.0f);
     return R(((const float*)cubes.atlas[atlas])[off]);
 }
 
@@ -990,6 +1020,104 @@ void orc_taa(const float* screen, const float* prev, const float* vel, int W, in
 // gammaTonemap.fs:20-29, 112-114, 125-142: per channel the ACES film fit in
 // the shader's order, saturate, pow(x, 1.0 / gamma), the RGBA8 UNORM store
 // floor(x * 255 + 0.5).  film (may be NULL) receives the value before the store.
+uint16_t orc_half_rn(float f) { return floatToHalfRN(f); }
+
+// The VPL shadow pre-pass: the cube update loop of StratusRendererBackend.cpp:
+// 1426-1575, shadow.vs:22-33 and shadowVpl.fs:17-37, for the lights
+// vpls[updates[s]] into cubes entries[updates[s]].  Face f of a light is the
+// view lookAt(pos, pos + axis_f, up_f) (:873-886) under perspective(90 deg, 1,
+// near, farPlane) (:1450): the lookAt bases are signed permutations, so the
+// ray through the centre of window pixel (i, j) is the GL major-axis table's
+// direction d of cube texel (i, j) with major component +-1, and a point at
+// view depth t along it is pos + t d.  Each layer starts cleared (distance 1,
+// colour 0, :1453-1459); each triangle, in order, gives a fragment at texel
+// (i, j) when the ray meets it (the three signed volumes d . (b x c),
+// d . (c x a), d . (a x b) of the light-relative vertices of one sign, not all
+// zero) at view depth t = det(a, b, c) / (their sum) within [near, far] (GL
+// clipping; GL_DEPTH_CLAMP is off here), with value saturate(t |d| / far)
+// (gl_FragDepth = saturate(|fsPosition - lightPos| / farPlane)) stored in the
+// distance format (F32 value, binary16 or the UNORM16 code rint(v 65535)) and
+// kept when below the stored one (LESS); its colour goes to the RGBA8 layer
+// with alpha 255.  The exact pixel coverage of GL's rasteriser on shared
+// edges (top-left rule, sub-pixel snapping) is implementation-defined; here a
+// ray on an edge meets both triangles and the first in order wins ties.
+namespace {
+uint32_t depthKey(int fmt, float v) {
+    v = v < 1.0f ? v : 1.0f;
+    if (fmt == SGI_TEXEL_F32) { uint32_t b; std::memcpy(&b, &v, 4); return b; }
+    if (fmt == SGI_TEXEL_F16) return floatToHalfRN(v);
+    return (uint32_t)std::nearbyint(v * 65535.0f);
+}
+V3<float> faceRay(int f, int i, int j, int res) {
+    const float sc = 2.0f * ((float)i + 0.5f) / (float)res - 1.0f;
+    const float tc = 2.0f * ((float)j + 0.5f) / (float)res - 1.0f;
+    switch (f) {
+        case 0: return v3(1.0f, -tc, -sc);
+        case 1: return v3(-1.0f, -tc, sc);
+        case 2: return v3(sc, 1.0f, tc);
+        case 3: return v3(sc, -1.0f, -tc);
+        case 4: return v3(sc, -tc, 1.0f);
+        default: return v3(-sc, -tc, -1.0f);
+    }
+}
+V3<float> cross3(V3<float> a, V3<float> b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+float dot3(V3<float> a, V3<float> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+}  // namespace
+
+void orc_render_vpl_cubes(const float* pos, const uint32_t* colors, int count, const sgi_vpl* vpls,
+                          const int32_t* updates, int numUpdates, const sgi_atlas_entry* entries, float nearPlane,
+                          const sgi_cube_atlases* dist, const sgi_cube_atlases* diffuse) {
+    const int res = dist->resolution;
+    auto work = [&](int job) {
+        const int s = job / 6, f = job % 6;
+        const sgi_vpl& v = vpls[updates[s]];
+        const sgi_atlas_entry e = entries[updates[s]];
+        if (e.index < 0 || e.index >= dist->num_atlases || e.layer < 0 || e.layer >= dist->layers_per_atlas) return;
+        const V3<float> L = v3(v.position[0], v.position[1], v.position[2]);
+        const float farP = v.farPlane;
+        const size_t layerOff = ((size_t)e.layer * 6 + f) * res * res;
+        for (int j = 0; j < res; ++j)
+            for (int i = 0; i < res; ++i) {
+                const V3<float> d = faceRay(f, i, j, res);
+                const float dlen = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                uint32_t best = depthKey(dist->format, 1.0f), col = 0;
+                for (int k = 0; k < count; ++k) {
+                    const float* q = pos + (size_t)k * 9;
+                    const V3<float> a = v3(q[0] - L.x, q[1] - L.y, q[2] - L.z);
+                    const V3<float> b = v3(q[3] - L.x, q[4] - L.y, q[5] - L.z);
+                    const V3<float> c = v3(q[6] - L.x, q[7] - L.y, q[8] - L.z);
+                    const V3<float> C0 = cross3(b, c), C1 = cross3(c, a), C2 = cross3(a, b);
+                    const float e0 = dot3(d, C0), e1 = dot3(d, C1), e2 = dot3(d, C2);
+                    const bool in = (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) || (e0 <= 0.0f && e1 <= 0.0f && e2 <= 0.0f);
+                    const float den = e0 + e1 + e2;
+                    if (!in || den == 0.0f) continue;
+                    const float t = dot3(a, C0) / den;
+                    if (!(t >= nearPlane && t <= farP)) continue;
+                    const uint32_t key = depthKey(dist->format, (t * dlen) / farP);
+                    if (key < best) {
+                        best = key;
+                        col = colors ? colors[k] : 0u;
+                    }
+                }
+                const size_t off = layerOff + (size_t)j * res + i;
+                if (dist->format == SGI_TEXEL_F32) std::memcpy((float*)dist->atlas[e.index] + off, &best, 4);
+                else ((uint16_t*)dist->atlas[e.index])[off] = (uint16_t)best;
+                if (diffuse) ((uint32_t*)diffuse->atlas[e.index])[off] = col | 0xFF000000u;
+            }
+    };
+    const int jobs = numUpdates * 6;
+    const int nt = std::max(1, std::min<int>(jobs, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::atomic<int> next{0};
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (int job = next++; job < jobs; job = next++) work(job);
+        });
+    for (auto& t : th) t.join();
+}
+
 void orc_tonemap(const float* lit, long n, float gamma, uint32_t* out, float* film) {
     const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
     const float invGamma = 1.0f / gamma;
@@ -1004,6 +1132,159 @@ void orc_tonemap(const float* lit, long n, float gamma, uint32_t* out, float* fi
         }
         out[p] = t;
     }
+}
+
+// The history copy of PerformTaaPostFx_ (StratusRendererBackend.cpp:2342-2349):
+// the TAA output lives in an RGBA16F target (:708-711), so it is first rounded
+// to binary16; the NEAREST blit into the RGB8 UNORM previousFrameBuffer
+// (:449-453) clamps to [0, 1] and stores floor(v * 255 + 0.5); the next frame's
+// TAA reads that texel back as k / 255.  prev receives the values read back.
+void orc_taa_history(const float* taa, long n, float* prev) {
+    for (long q = 0; q < 3 * n; ++q) {
+        const float h = halfToFloat(floatToHalfRN(taa[q]));
+        const float s = std::fmin(std::fmax(h, 0.0f), 1.0f);
+        const uint32_t k = (uint32_t)(s * 255.0f + 0.5f);
+        prev[q] = (float)k / 255.0f;
+    }
+}
+
+// aa_common.glsl:7-9, in the shader's order.
+inline float fxaaLuminance(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+
+inline uint32_t unorm8Store(float v) {
+    v = std::fmin(std::fmax(v, 0.0f), 1.0f);
+    return (uint32_t)(v * 255.0f + 0.5f);
+}
+
+// fxaa_luminance.fs:16-20 (PerformFxaaPostFx_'s first pass,
+// StratusRendererBackend.cpp:2298-2303): the tonemapped RGBA8 colour (read as
+// k / 255) with the luminance of its saturated rgb in alpha, stored into the
+// RGBA8 target fxaaFbo1 (:683-688).
+void orc_fxaa_luminance(const uint32_t* in, long n, uint32_t* out) {
+    for (long p = 0; p < n; ++p) {
+        float c[3];
+        for (int k = 0; k < 3; ++k) c[k] = (float)((in[p] >> (8 * k)) & 0xFFu) / 255.0f;
+        const float sat[3] = {std::fmin(std::fmax(c[0], 0.0f), 1.0f), std::fmin(std::fmax(c[1], 0.0f), 1.0f),
+                              std::fmin(std::fmax(c[2], 0.0f), 1.0f)};
+        const float lum = fxaaLuminance(sat[0], sat[1], sat[2]);
+        out[p] = unorm8Store(c[0]) | (unorm8Store(c[1]) << 8) | (unorm8Store(c[2]) << 16) | (unorm8Store(lum) << 24);
+    }
+}
+
+// fxaa_smoothing.fs:72-277 (the second pass, :2308-2316) over fxaaFbo1 (RGBA8,
+// LINEAR, CLAMP_TO_EDGE, read as k / 255), stored into the RGBA8 target
+// fxaaFbo2 with alpha 1.  The centre and textureOffset taps sit on texel
+// centres and read the texel; the half-texel edge walk and the final offset
+// sample are GL LINEAR taps (bilinear, FP32 weights).  GLSL's order of
+// operations is kept; smoothstep(0, 1, x) = t t (3 - 2 t) with t = clamp(x, 0, 1).
+void orc_fxaa_smoothing(const uint32_t* lum, int W, int H, float contrastThreshold, float relativeThreshold,
+                        float subpixelBlending, uint32_t* out) {
+    std::vector<float> tex((size_t)W * H * 4);
+    for (size_t p = 0; p < (size_t)W * H; ++p)
+        for (int k = 0; k < 4; ++k) tex[4 * p + k] = (float)((lum[p] >> (8 * k)) & 0xFFu) / 255.0f;
+    const Plane T{tex.data(), W, H, 4};
+    const float tsx = 1.0f / (float)W, tsy = 1.0f / (float)H;  // computeTexelSize (common.glsl:111-115)
+    const float edgeStepOffsets[10] = {1.0f, 1.5f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 2.0f, 4.0f};
+    const float edgeGuessJumpSize = 8.0f;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t p = (size_t)y * W + x;
+            const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+            const float lumaCenter = T.at(x, y, 3);
+            const float lumaRight = T.at(x + 1, y, 3), lumaLeft = T.at(x - 1, y, 3);
+            const float lumaTop = T.at(x, y + 1, 3), lumaBot = T.at(x, y - 1, 3);
+            const float lumaMax = std::fmax(std::fmax(std::fmax(std::fmax(lumaCenter, lumaRight), lumaLeft), lumaTop),
+                                            lumaBot);
+            const float lumaMin = std::fmin(std::fmin(std::fmin(std::fmin(lumaCenter, lumaRight), lumaLeft), lumaTop),
+                                            lumaBot);
+            const float contrast = lumaMax - lumaMin;
+            const float threshold = std::fmax(contrastThreshold, relativeThreshold * lumaMax);
+            if (contrast < threshold) {
+                out[p] = unorm8Store(T.at(x, y, 0)) | (unorm8Store(T.at(x, y, 1)) << 8) |
+                         (unorm8Store(T.at(x, y, 2)) << 16) | 0xFF000000u;
+                continue;
+            }
+            const float lumaTopRight = T.at(x + 1, y + 1, 3), lumaBotRight = T.at(x + 1, y - 1, 3);
+            const float lumaTopLeft = T.at(x - 1, y + 1, 3), lumaBotLeft = T.at(x - 1, y - 1, 3);
+            float average = 2.0f * (lumaRight + lumaLeft + lumaTop + lumaBot) + lumaTopRight + lumaBotRight +
+                            lumaTopLeft + lumaBotLeft;
+            average = average / 12.0f;
+            const float centerAvgContrast = std::fabs(average - lumaCenter);
+            const float normalized = std::fmin(std::fmax(centerAvgContrast / contrast, 0.0f), 1.0f);
+            const float t = std::fmin(std::fmax(normalized, 0.0f), 1.0f);
+            float pixelBlendFactor = t * t * (3.0f - 2.0f * t);
+            pixelBlendFactor = pixelBlendFactor * pixelBlendFactor * subpixelBlending;
+            const float horizontal = std::fabs(lumaTop + lumaBot - 2.0f * lumaCenter) * 2.0f +
+                                     std::fabs(lumaTopRight + lumaBotRight - 2.0f * lumaRight) +
+                                     std::fabs(lumaTopLeft + lumaBotLeft - 2.0f * lumaLeft);
+            const float vertical = std::fabs(lumaRight + lumaLeft - 2.0f * lumaCenter) * 2.0f +
+                                   std::fabs(lumaTopRight + lumaTopLeft - 2.0f * lumaTop) +
+                                   std::fabs(lumaBotRight + lumaBotLeft - 2.0f * lumaBot);
+            const bool isHorizontal = horizontal >= vertical;
+            float pixelStep = isHorizontal ? tsy : tsx;
+            const float positiveLuma = isHorizontal ? lumaTop : lumaRight;
+            const float negativeLuma = isHorizontal ? lumaBot : lumaLeft;
+            const float positiveGradient = std::fabs(positiveLuma - lumaCenter);
+            const float negativeGradient = std::fabs(negativeLuma - lumaCenter);
+            float gradient, oppositeLuma;
+            if (positiveGradient < negativeGradient) {
+                pixelStep = -pixelStep;
+                oppositeLuma = negativeLuma;
+                gradient = negativeGradient;
+            } else {
+                oppositeLuma = positiveLuma;
+                gradient = positiveGradient;
+            }
+            float ue = u, ve = v, esx, esy;
+            if (isHorizontal) {
+                ve += pixelStep * 0.5f;
+                esx = tsx;
+                esy = 0.0f;
+            } else {
+                ue += pixelStep * 0.5f;
+                esx = 0.0f;
+                esy = tsy;
+            }
+            const float edgeLumaAvg = (lumaCenter + oppositeLuma) * 0.5f;
+            const float gradientThreshold = gradient * 0.25f;
+            float dist[2], deltas[2];
+            const float signs[2] = {1.0f, -1.0f};
+            for (int i = 0; i < 2; ++i) {
+                const float sx = esx * signs[i], sy = esy * signs[i];  // edgeSteps[i] = +-edgeStep
+                float pu = ue + sx * edgeStepOffsets[0], pv = ve + sy * edgeStepOffsets[0];
+                float delta = bilinear<float>(T, pu, pv, 3) - edgeLumaAvg;
+                bool atEnd = std::fabs(delta) >= gradientThreshold;
+                for (int j = 0; j < 9 && !atEnd; ++j) {
+                    pu += sx * edgeStepOffsets[j + 1];
+                    pv += sy * edgeStepOffsets[j + 1];
+                    delta = bilinear<float>(T, pu, pv, 3) - edgeLumaAvg;
+                    atEnd = std::fabs(delta) >= gradientThreshold;
+                }
+                if (!atEnd) {
+                    pu += sx * edgeGuessJumpSize;
+                    pv += sy * edgeGuessJumpSize;
+                }
+                deltas[i] = delta;
+                dist[i] = isHorizontal ? (signs[i] * pu) + (-signs[i] * u) : (signs[i] * pv) + (-signs[i] * v);
+            }
+            float shortest;
+            bool deltaSign;
+            if (dist[0] <= dist[1]) {
+                shortest = dist[0];
+                deltaSign = deltas[0] >= 0.0f;
+            } else {
+                shortest = dist[1];
+                deltaSign = deltas[1] >= 0.0f;
+            }
+            const float edgeBlendFactor =
+                deltaSign == (lumaCenter - edgeLumaAvg >= 0.0f) ? 0.0f : 0.5f - shortest / (dist[0] + dist[1]);
+            const float finalBlend = std::fmax(pixelBlendFactor, edgeBlendFactor);
+            float fu = u, fv = v;
+            if (isHorizontal) fv += pixelStep * finalBlend;
+            else fu += pixelStep * finalBlend;
+            out[p] = unorm8Store(bilinear<float>(T, fu, fv, 0)) | (unorm8Store(bilinear<float>(T, fu, fv, 1)) << 8) |
+                     (unorm8Store(bilinear<float>(T, fu, fv, 2)) << 16) | 0xFF000000u;
+        }
 }
 
 void orc_color_vpls(const sgi_frame* fr, int count, const sgi_atlas_entry* entries, const sgi_cube_atlases* diffuse,

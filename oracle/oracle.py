@@ -50,6 +50,12 @@ _SIGNATURES = {
     "orc_taa": (None, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
     "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
+    "orc_taa_history": (None, [c_void_p, ctypes.c_long, c_void_p]),
+    "orc_fxaa_luminance": (None, [c_void_p, ctypes.c_long, c_void_p]),
+    "orc_fxaa_smoothing": (None, [c_void_p, c_int, c_int, c_float, c_float, c_float, c_void_p]),
+    "orc_half_rn": (ctypes.c_uint16, [c_float]),
+    "orc_render_vpl_cubes": (None, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_float,
+                                    POINTER(_abi.CubeAtlases), POINTER(_abi.CubeAtlases)]),
     "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
                               POINTER(_abi.CubeAtlases), c_float, c_void_p]),
     "orc_cull_global": (c_int, [POINTER(_abi.Frame), POINTER(_abi.Csm), c_void_p, c_int, c_void_p,
@@ -123,6 +129,41 @@ def tonemap(lit, gamma=2.2):
     film = np.zeros((H, W, 3), np.float32)
     lib().orc_tonemap(_p(lit), H * W, gamma, _p(out), _p(film))
     return out, film
+
+
+def render_vpl_cubes(positions, colors, vpls, updates, entries, near, dist, diffuse=None):
+    """The VPL shadow pre-pass into host atlases `dist` / `diffuse` (CubeAtlases over numpy arrays)."""
+    pos = np.ascontiguousarray(positions, np.float32)
+    col = None if colors is None else np.ascontiguousarray(colors, np.uint32)
+    vp = _f32(vpls)
+    up = np.ascontiguousarray(updates, np.int32)
+    en = np.ascontiguousarray(entries, np.int32)
+    lib().orc_render_vpl_cubes(_p(pos), _p(col), pos.shape[0], _p(vp), _p(up), up.shape[0], _p(en), near,
+                               ctypes.byref(dist), ctypes.byref(diffuse) if diffuse is not None else None)
+
+
+def taa_history(taa):
+    """The TAA output as the next frame reads it back from the RGB8 history (RGBA16F, clamp, UNORM8)."""
+    taa = _f32(taa)
+    prev = np.zeros_like(taa)
+    lib().orc_taa_history(_p(taa), taa.size // 3, _p(prev))
+    return prev
+
+
+FXAA_CONTRAST_THRESHOLD, FXAA_RELATIVE_THRESHOLD, FXAA_SUBPIXEL_BLENDING = 0.0625, 0.125, 1.0
+
+
+def fxaa(tonemapped, contrast=FXAA_CONTRAST_THRESHOLD, relative=FXAA_RELATIVE_THRESHOLD,
+         subpixel=FXAA_SUBPIXEL_BLENDING):
+    """fxaa_luminance.fs then fxaa_smoothing.fs on RGBA8 texels [H][W] (uint32): returns
+    (the luminance target, the smoothed target), both RGBA8 texels."""
+    t = np.ascontiguousarray(tonemapped, np.uint32)
+    H, W = t.shape
+    lum = np.zeros_like(t)
+    out = np.zeros_like(t)
+    lib().orc_fxaa_luminance(_p(t), H * W, _p(lum))
+    lib().orc_fxaa_smoothing(_p(lum), W, H, contrast, relative, subpixel, _p(out))
+    return lum, out
 
 
 def gbuffer_unpack(packed: _abi.GbufferPacked, with_velocity=True):

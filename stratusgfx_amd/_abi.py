@@ -29,6 +29,7 @@ NUM_CASCADES = 4
 TEXEL_F32 = 0
 TEXEL_F16 = 1
 TEXEL_RGBA8 = 2
+TEXEL_UNORM16 = 3  # GL_DEPTH_COMPONENT16, the reference's VPL cube format
 REFERENCE_VPL_COLOR_MULTIPLIER = 50000.0  # vpl_light_color.cs:44
 PRECISION_FP32 = 0
 PRECISION_FP16 = 1
@@ -136,6 +137,17 @@ class SceneDesc(ctypes.Structure):
                 ("vpl_color_multiplier", c_float)]
 
 
+class Triangles(ctypes.Structure):
+    _fields_ = [("count", c_int32), ("positions", c_void_p), ("colors", c_void_p)]
+
+
+REFERENCE_VPL_NEAR_PLANE = 0.1  # StratusLight.h:291
+
+
+class FxaaParams(ctypes.Structure):
+    _fields_ = [("contrastThreshold", c_float), ("relativeThreshold", c_float), ("subpixelBlending", c_float)]
+
+
 class View(ctypes.Structure):
     _fields_ = [("width", c_int32), ("height", c_int32), ("eye", c_float * 3), ("yaw_deg", c_float),
                 ("pitch_deg", c_float), ("fovy_deg", c_float), ("znear", c_float), ("zfar", c_float),
@@ -155,6 +167,10 @@ SGI_SIGNATURES = {
     "sgi_set_csm": (c_int32, [_P, POINTER(Csm)]),
     "sgi_taa": (c_int32, [_P, _P, _P, _P, _P, _P]),
     "sgi_tonemap": (c_int32, [_P, _P, c_float, _P, _P]),
+    "sgi_taa_history": (c_int32, [_P, _P, _P, _P]),
+    "sgi_render_vpl_cubes": (c_int32, [_P, POINTER(Triangles), _P, _P, _I, _P, c_float, POINTER(CubeAtlases),
+                                       POINTER(CubeAtlases), _P]),
+    "sgi_fxaa": (c_int32, [_P, _P, _P, _P, _P]),
     "sgi_gbuffer_unpack": (c_int32, [_P, POINTER(GbufferPacked), _P, _P, _P, _P, _P, _P]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
     "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
@@ -226,6 +242,14 @@ class SgiError(RuntimeError):
     def __init__(self, fn: str, code: int, msg: str):
         super().__init__(f"{fn} failed with {code}: {msg}")
         self.code = code
+
+
+class SgiArgumentError(SgiError, ValueError):
+    """A buffer the Python layer refuses before the call (SGI_E_INVALID): the C
+    ABI takes plain pointers and cannot check their sizes itself."""
+
+    def __init__(self, msg: str):
+        super().__init__("argument check", SGI_E_INVALID, msg)
 
 
 _libs: dict[str, ctypes.CDLL] = {}

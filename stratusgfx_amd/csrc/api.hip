@@ -3,6 +3,7 @@
 // (StratusRendererBackend.cpp:1982, 2010-2045).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <string>
@@ -23,6 +24,8 @@ struct sgi_ctx {
     sgi_launch::DenoiseBuffers ws{};
     uint8_t* visFlags = nullptr;  // [SGI_MAX_TOTAL_VPLS_BEFORE_CULLING] per-VPL sun visibility
     void* vplPrep = nullptr;      // [SGI_MAX_TOTAL_VPLS_PER_FRAME] packed records of the sampled gathers
+    int32_t* cubeLists = nullptr; // per (light, face) triangle lists of sgi_render_vpl_cubes, grown on demand
+    size_t cubeListInts = 0;
     int device = 0;
 };
 
@@ -90,6 +93,7 @@ void sgi_destroy(sgi_ctx* ctx) {
     if (ctx->ws.normals) (void)hipFree(ctx->ws.normals);
     if (ctx->visFlags) (void)hipFree(ctx->visFlags);
     if (ctx->vplPrep) (void)hipFree(ctx->vplPrep);
+    if (ctx->cubeLists) (void)hipFree(ctx->cubeLists);
     delete ctx;
 }
 
@@ -134,6 +138,29 @@ int sgi_taa(sgi_ctx* ctx, const float* screen, const float* prevScreen, const fl
     const hipError_t e =
         sgi_launch::taa(screen, prevScreen, velocity, ctx->cfg.width, ctx->cfg.height, out, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_taa");
+}
+
+int sgi_taa_history(sgi_ctx* ctx, const float* taaOut, float* prevScreen, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_taa_history: NULL context");
+    if (!taaOut || !prevScreen) return fail(SGI_E_INVALID, "sgi_taa_history: NULL buffer");
+    const hipError_t e =
+        sgi_launch::taaHistory(taaOut, (long)ctx->cfg.width * ctx->cfg.height, prevScreen, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_taa_history");
+}
+
+int sgi_fxaa(sgi_ctx* ctx, const uint32_t* tonemapped, const sgi_fxaa_params* params, uint32_t* out, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_fxaa: NULL context");
+    if (!tonemapped || !out) return fail(SGI_E_INVALID, "sgi_fxaa: NULL buffer");
+    if (out == tonemapped) return fail(SGI_E_INVALID, "sgi_fxaa: out must not alias the input");
+    if (((uintptr_t)tonemapped & 3) || ((uintptr_t)out & 3)) return fail(SGI_E_INVALID, "sgi_fxaa: misaligned buffer");
+    sgi_fxaa_params p{0.0625f, 0.125f, 1.0f};  // fxaa_smoothing.fs:77, 90, 101
+    if (params) p = *params;
+    if (!std::isfinite(p.contrastThreshold) || !std::isfinite(p.relativeThreshold) ||
+        !std::isfinite(p.subpixelBlending))
+        return fail(SGI_E_INVALID, "sgi_fxaa: parameters must be finite");
+    const hipError_t e = sgi_launch::fxaa(tonemapped, ctx->cfg.width, ctx->cfg.height, p.contrastThreshold,
+                                          p.relativeThreshold, p.subpixelBlending, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_fxaa");
 }
 
 int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void* stream) {
@@ -206,7 +233,8 @@ namespace {
 
 int cubeArgs(const sgi_cube_atlases* cubes, const char* who, sgi_dev::CubeArgs& ca) {
     if (cubes->resolution <= 0 || cubes->resolution > 4096 ||
-        (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16) || cubes->layers_per_atlas <= 0 ||
+        (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16 && cubes->format != SGI_TEXEL_UNORM16) ||
+        cubes->layers_per_atlas <= 0 ||
         cubes->layers_per_atlas > SGI_MAX_TOTAL_SHADOWS_PER_ATLAS || cubes->num_atlases <= 0 ||
         cubes->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES)
         return fail(SGI_E_INVALID, std::string(who) + ": bad cube atlas description");
@@ -727,6 +755,53 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     }
 #undef KPCN_STEP
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise_learned");
+}
+
+int sgi_render_vpl_cubes(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_vpl* vpls, const int32_t* updates,
+                         int32_t numUpdates, const sgi_atlas_entry* entries, float nearPlane,
+                         const sgi_cube_atlases* distance, const sgi_cube_atlases* diffuse, void* stream) {
+    const char* who = "sgi_render_vpl_cubes";
+    if (!ctx || !mesh || !distance) return fail(SGI_E_INVALID, std::string(who) + ": NULL argument");
+    if (numUpdates < 0 || numUpdates > SGI_MAX_TOTAL_VPLS_PER_FRAME)
+        return fail(SGI_E_INVALID, std::string(who) + ": numUpdates must be within 0..4480");
+    if (mesh->count < 0 || (mesh->count > 0 && !mesh->positions))
+        return fail(SGI_E_INVALID, std::string(who) + ": bad triangle list");
+    if (!(nearPlane > 0.0f) || !std::isfinite(nearPlane))
+        return fail(SGI_E_INVALID, std::string(who) + ": nearPlane must be finite and > 0");
+    if (numUpdates > 0 && (!vpls || !updates || !entries))
+        return fail(SGI_E_INVALID, std::string(who) + ": NULL VPL / update / entry buffer");
+    sgi_dev::CubeArgs dz{}, dc{};
+    int r = cubeArgs(distance, who, dz);
+    if (r != SGI_OK) return r;
+    if (diffuse) {
+        if (diffuse->format != SGI_TEXEL_RGBA8 || diffuse->resolution != distance->resolution ||
+            diffuse->layers_per_atlas != distance->layers_per_atlas || diffuse->num_atlases != distance->num_atlases)
+            return fail(SGI_E_INVALID, std::string(who) + ": the diffuse atlases must be RGBA8 of the distance geometry");
+        sgi_cube_atlases g = *diffuse;
+        g.format = SGI_TEXEL_F32;  // cubeArgs checks the geometry; the texel size is the caller's
+        r = cubeArgs(&g, who, dc);
+        if (r != SGI_OK) return r;
+    }
+    if (numUpdates == 0) return SGI_OK;
+    if (diffuse && mesh->count > 0 && !mesh->colors)
+        return fail(SGI_E_INVALID, std::string(who) + ": diffuse output needs per-triangle colours");
+    // per-(light, face) lists of up to count triangles: batches of lights within a 256 MB budget
+    const size_t perLight = sgi_launch::cubeListInts(1, mesh->count);
+    const int batchMax = (int)std::max<size_t>(1, std::min<size_t>((size_t)numUpdates, (64u << 20) / perLight));
+    const size_t need = sgi_launch::cubeListInts(batchMax, mesh->count);
+    if (need > ctx->cubeListInts) {
+        if (ctx->cubeLists) (void)hipFree(ctx->cubeLists);
+        ctx->cubeLists = nullptr;
+        ctx->cubeListInts = 0;
+        const hipError_t e = hipMalloc((void**)&ctx->cubeLists, need * sizeof(int32_t));
+        if (e != hipSuccess) return hipFail(e, "sgi_render_vpl_cubes: list workspace");
+        ctx->cubeListInts = need;
+    }
+    const hipError_t e = sgi_launch::renderVplCubes(mesh->positions, diffuse ? mesh->colors : nullptr, mesh->count,
+                                                    vpls, updates, numUpdates, entries, nearPlane, dz,
+                                                    distance->format, dc, diffuse != nullptr, batchMax, ctx->cubeLists,
+                                                    (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, who);
 }
 
 }  // extern "C"

@@ -122,9 +122,10 @@ __global__ __launch_bounds__(256) void colorVplsKernel(CubeArgs diffuse, CubeArg
     const f3 d{-sun.x, -sun.y, -sun.z};
     const uint32_t off = cubeTexelOffset(d, diffuse.res);
     const uint32_t rgb = ((const uint32_t*)cubeLayerBase(diffuse, e.index, e.layer, 4))[off];
-    const void* zb = cubeLayerBase(dist, e.index, e.layer, ZFMT == SGI_TEXEL_F16 ? 2 : 4);
-    const float z = ZFMT == SGI_TEXEL_F16 ? __half2float(__ushort_as_half(((const unsigned short*)zb)[off]))
-                                          : ((const float*)zb)[off];
+    const void* zb = cubeLayerBase(dist, e.index, e.layer, texelBytes(ZFMT));
+    float z;
+    if constexpr (ZFMT == SGI_TEXEL_F32) z = ((const float*)zb)[off];
+    else z = decode16<ZFMT>(((const unsigned short*)zb)[off]);
     sgi_vpl& v = vpls[s];
     const float intensity = v.intensity;
     const float scaled = 1.1f * (v.radius * z);  // 1.1 * magnitude (:53, :71)
@@ -329,6 +330,9 @@ hipError_t colorVpls(const sgi_dev::CubeArgs& diffuse, const sgi_dev::CubeArgs& 
     const dim3 grid((unsigned)((maxCount + 255) / 256));
     if (distFormat == SGI_TEXEL_F16)
         hipLaunchKernelGGL(colorVplsKernel<SGI_TEXEL_F16>, grid, dim3(256), 0, s, diffuse, dist, sun, col, mult,
+                           visible, maxCount, entries, vpls);
+    else if (distFormat == SGI_TEXEL_UNORM16)
+        hipLaunchKernelGGL(colorVplsKernel<SGI_TEXEL_UNORM16>, grid, dim3(256), 0, s, diffuse, dist, sun, col, mult,
                            visible, maxCount, entries, vpls);
     else
         hipLaunchKernelGGL(colorVplsKernel<SGI_TEXEL_F32>, grid, dim3(256), 0, s, diffuse, dist, sun, col, mult,

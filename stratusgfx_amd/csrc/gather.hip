@@ -45,7 +45,7 @@ __device__ __forceinline__ Vpl loadVpl(const sgi_vpl* v, const sgi_atlas_entry* 
     o.invRadius = 1.0f / v->radius;
     o.farPlane = v->farPlane;
     const sgi_atlas_entry en = *e;
-    o.cube = cubeLayerBase(cubes, en.index, en.layer, FMT == SGI_TEXEL_F16 ? 2 : 4);
+    o.cube = cubeLayerBase(cubes, en.index, en.layer, texelBytes(FMT));
     return o;
 }
 
@@ -477,7 +477,9 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
     const Pixel px = setupPixel(g, fr, x, y, p);
     f3 vplColor = splat(0.0f);
     float validSamples = 0.0f;
-    const int numVisible = visible[0];
+    // prep holds min(visible[0], 4480) records (prepVplsKernel): a larger
+    // caller-supplied count must not index past them
+    const int numVisible = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
     if (numVisible > 0) {
         const float* rma = g.rma + 3 * p;
         const float roughness = gmax(0.5f, rma[0]);
@@ -732,6 +734,9 @@ hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes,
     if (format == SGI_TEXEL_F16)
         return launchPrecision<SGI_TEXEL_F16>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries,
                                               tiles, precision, pv, color, reservoir, s);
+    if (format == SGI_TEXEL_UNORM16)
+        return launchPrecision<SGI_TEXEL_UNORM16>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries,
+                                                  tiles, precision, pv, color, reservoir, s);
     return launchPrecision<SGI_TEXEL_F32>(g, fr, cubes, tileSize, tilesX, tilesY, K, vpls, visible, entries, tiles,
                                           precision, pv, color, reservoir, s);
 }
@@ -743,6 +748,9 @@ hipError_t gatherEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
     if (format == SGI_TEXEL_F16)
         return launchEx<SGI_TEXEL_F16>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, half, vpls, visible,
                                        entries, pv, color, reservoir, s);
+    if (format == SGI_TEXEL_UNORM16)
+        return launchEx<SGI_TEXEL_UNORM16>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, half, vpls,
+                                           visible, entries, pv, color, reservoir, s);
     return launchEx<SGI_TEXEL_F32>(g, fr, cubes, mode, samples, rejectionLimit, accumulate64, half, vpls, visible,
                                    entries, pv, color, reservoir, s);
 }
@@ -757,6 +765,11 @@ hipError_t gatherViews(const sgi_batch::ViewDesc* views, int numViews, int total
         if (tileSize == 8) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 8);
         else if (tileSize == 16) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 16);
         else if (tileSize == 32) SGI_GATHER_VIEWS(SGI_TEXEL_F16, 32);
+        else return hipErrorInvalidValue;
+    } else if (format == SGI_TEXEL_UNORM16) {
+        if (tileSize == 8) SGI_GATHER_VIEWS(SGI_TEXEL_UNORM16, 8);
+        else if (tileSize == 16) SGI_GATHER_VIEWS(SGI_TEXEL_UNORM16, 16);
+        else if (tileSize == 32) SGI_GATHER_VIEWS(SGI_TEXEL_UNORM16, 32);
         else return hipErrorInvalidValue;
     } else {
         if (tileSize == 8) SGI_GATHER_VIEWS(SGI_TEXEL_F32, 8);

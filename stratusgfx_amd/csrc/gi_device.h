@@ -201,13 +201,32 @@ __device__ __forceinline__ f3 fmaF(f3 a, float s, f3 c) {
     return f3{__builtin_fmaf(a.x, s, c.x), __builtin_fmaf(a.y, s, c.y), __builtin_fmaf(a.z, s, c.z)};
 }
 
+// Bytes per distance texel of a cube format.
+__host__ __device__ constexpr int texelBytes(int fmt) { return fmt == SGI_TEXEL_F32 ? 4 : 2; }
+
+// UNORM16 code k as k / 65535 correctly rounded: with y = k 2^-16 (exact),
+// k / 65535 = y (1 + 2^-16 + 2^-32 + ...), and y + y (2^-16 + 2^-32) rounded
+// once by the FMA equals the correctly rounded quotient for every k (checked
+// exhaustively against the exact rationals by the tests).
+__device__ __forceinline__ float unorm16ToFloat(uint32_t k) {
+    const float y = (float)k * 0x1p-16f;
+    return __builtin_fmaf(y, 0x1p-16f + 0x1p-32f, y);
+}
+
+// A 16-bit distance texel (F16 or UNORM16) as FP32.
+template <int FMT>
+__device__ __forceinline__ float decode16(unsigned short bits) {
+    if constexpr (FMT == SGI_TEXEL_UNORM16) return unorm16ToFloat(bits);
+    else return __half2float(__ushort_as_half(bits));
+}
+
 template <int FMT>
 __device__ __forceinline__ float cubeFetch(const void* layerBase, int res, f3 d) {
     typedef const __attribute__((address_space(1))) unsigned short* GHalf;
     typedef const __attribute__((address_space(1))) float* GFloat;
     const uint32_t off = cubeTexelOffsetFast(d, res);
-    if (FMT == SGI_TEXEL_F16) return __half2float(__ushort_as_half(((GHalf)layerBase)[off]));
-    return ((GFloat)layerBase)[off];
+    if constexpr (FMT == SGI_TEXEL_F32) return ((GFloat)layerBase)[off];
+    else return decode16<FMT>(((GHalf)layerBase)[off]);
 }
 
 // Kernel-argument view of the cascades and sun.

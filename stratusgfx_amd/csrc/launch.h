@@ -21,6 +21,22 @@ hipError_t taa(const float* screen, const float* prev, const float* velocity, in
 // lit [n][3] FP32 -> ACES film + gamma -> RGBA8 texels
 hipError_t tonemap(const float* lit, long n, float gamma, uint32_t* out, hipStream_t s);
 
+// VPL cube producer (shadowVpl.fs through the reference's cube update loop): the
+// cubes of vpls[updates[0 .. numUpdates)] rendered from `count` world-space
+// triangles, in batches of batchMax lights; scratch holds cubeListInts(batchMax, count) ints.
+size_t cubeListInts(int batch, int count);
+hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, const sgi_vpl* vpls,
+                          const int32_t* updates, int numUpdates, const sgi_atlas_entry* entries, float nearPlane,
+                          const sgi_dev::CubeArgs& dist, int format, const sgi_dev::CubeArgs& diffuse,
+                          bool withDiffuse, int batchMax, int32_t* scratch, hipStream_t s);
+
+// PerformTaaPostFx_'s history copy: TAA output -> the RGB8 history as read back (k / 255)
+hipError_t taaHistory(const float* taa, long n, float* prev, hipStream_t s);
+
+// fxaa_luminance.fs + fxaa_smoothing.fs fused: tonemapped RGBA8 -> anti-aliased RGBA8
+hipError_t fxaa(const uint32_t* in, int W, int H, float contrastThreshold, float relativeThreshold,
+                float subpixelBlending, uint32_t* out, hipStream_t s);
+
 // packed G-buffer attachments -> FP32 planes (velocity may be NULL)
 hipError_t gbufferUnpack(const sgi_gbuffer_packed& g, float* normal, float* albedo, float* refl, float* rma,
                          float* velocity, hipStream_t s);

@@ -457,7 +457,8 @@ uint16_t floatToHalf(float f) {
 bool checkCubes(const sgi_scene* s, const sgi_cube_atlases* c, int first, int count) {
     if (!s || !c) { sgi_set_error("sgi_scene_cubes: NULL argument"); return false; }
     if (c->resolution <= 0 ||
-        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8) ||
+        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8 &&
+         c->format != SGI_TEXEL_UNORM16) ||
         c->layers_per_atlas <= 0 || c->num_atlases <= 0 || c->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES) {
         sgi_set_error("sgi_scene_cubes: bad atlas description");
         return false;
@@ -643,6 +644,7 @@ int sgi_scene_cubes_host(const sgi_scene* s, const sgi_cube_atlases* c, int32_t 
         }
         const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, res);
         if (c->format == SGI_TEXEL_F32) ((float*)c->atlas[atlas])[off] = val;
+        else if (c->format == SGI_TEXEL_UNORM16) ((uint16_t*)c->atlas[atlas])[off] = sgi_rt::encodeUnorm16(val);
         else ((uint16_t*)c->atlas[atlas])[off] = floatToHalf(val);
     });
     return SGI_OK;

@@ -93,6 +93,7 @@ __global__ void cubeKernel(sgi_rt::SceneView sv, CubeJob job) {
     }
     const float val = sgi_rt::cubeTexelValue(sv, v.position, v.farPlane, face, i, t, job.res);
     if (job.format == SGI_TEXEL_F32) ((float*)job.atlas[atlas])[off] = val;
+    else if (job.format == SGI_TEXEL_UNORM16) ((uint16_t*)job.atlas[atlas])[off] = sgi_rt::encodeUnorm16(val);
     else ((__half*)job.atlas[atlas])[off] = __float2half_rn(val);
 }
 
@@ -123,7 +124,8 @@ int sgi_scene_gbuffer_device(sgi_scene* s, float* depth, float* normal, float* a
 int sgi_scene_cubes_device(sgi_scene* s, const sgi_cube_atlases* c, int32_t first, int32_t count, void* stream) {
     if (!s || !c) { sgi_set_error("sgi_scene_cubes_device: NULL argument"); return SGI_E_INVALID; }
     if (c->resolution <= 0 ||
-        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8) ||
+        (c->format != SGI_TEXEL_F32 && c->format != SGI_TEXEL_F16 && c->format != SGI_TEXEL_RGBA8 &&
+         c->format != SGI_TEXEL_UNORM16) ||
         c->layers_per_atlas <= 0 || c->num_atlases <= 0 || c->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES ||
         first < 0 || count < 0 || first + count > (int)s->vpls.size() ||
         (long)(first + count) > (long)c->layers_per_atlas * c->num_atlases) {

@@ -243,6 +243,14 @@ SGI_HD inline void shadeGbufferTexel(const SceneView& s, const CameraRays& cam, 
 
 // Distance (t) of the nearest hit from a VPL along a cube texel direction, as
 // the stored value dist / farPlane clamped to 1 (a cleared depth attachment).
+// A depth value as the GL_DEPTH_COMPONENT16 code the reference's VPL cubes hold:
+// clamped to [0, 1], times 65535, rounded to nearest (ties to even; GL leaves
+// ties to the implementation).
+SGI_HD inline uint16_t encodeUnorm16(float v) {
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    return (uint16_t)rintf(v * 65535.0f);
+}
+
 SGI_HD inline float cubeTexelValue(const SceneView& s, const float pos[3], float farPlane,
                                    int face, int i, int j, int res) {
     float d[3];

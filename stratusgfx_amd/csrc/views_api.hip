@@ -168,7 +168,8 @@ int sgi_views_set(sgi_views* v, int32_t numViews, const sgi_view_io* io, const s
         return fail(SGI_E_INVALID, "sgi_views_set: numVpls must be within 0..10000");
     if (numVpls > 0 && (!vpls || !shadowIndices)) return fail(SGI_E_INVALID, "sgi_views_set: NULL VPL buffer");
     if (cubes->resolution <= 0 || cubes->resolution > 4096 ||
-        (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16) || cubes->layers_per_atlas <= 0 ||
+        (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16 && cubes->format != SGI_TEXEL_UNORM16) ||
+        cubes->layers_per_atlas <= 0 ||
         cubes->layers_per_atlas > SGI_MAX_TOTAL_SHADOWS_PER_ATLAS || cubes->num_atlases <= 0 ||
         cubes->num_atlases > SGI_MAX_TOTAL_SHADOW_ATLASES)
         return fail(SGI_E_INVALID, "sgi_views_set: bad cube atlas description");

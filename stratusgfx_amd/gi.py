@@ -15,6 +15,29 @@ def _ptr(t) -> int:
     return 0 if t is None else t.data_ptr()
 
 
+_F32, _I32, _BF16, _U8 = "float32", "int32", "bfloat16", "uint8"
+
+
+def _dev(t, name: str, numel: int, dtypes=(_F32,), optional: bool = False) -> int:
+    """Device pointer of `t` after checking what the C ABI cannot: a contiguous
+    GPU tensor of an accepted dtype holding at least `numel` elements.  A wrong
+    buffer here would otherwise become an out-of-bounds device access."""
+    if t is None:
+        if optional:
+            return 0
+        raise _abi.SgiArgumentError(f"{name}: a device tensor is required")
+    if not getattr(t, "is_cuda", False):
+        raise _abi.SgiArgumentError(f"{name}: expected a GPU tensor, got {getattr(t, 'device', type(t))}")
+    dt = str(t.dtype).replace("torch.", "")
+    if dt not in dtypes:
+        raise _abi.SgiArgumentError(f"{name}: dtype {dt}, expected one of {dtypes}")
+    if not t.is_contiguous():
+        raise _abi.SgiArgumentError(f"{name}: tensor must be contiguous")
+    if t.numel() < numel:
+        raise _abi.SgiArgumentError(f"{name}: {t.numel()} elements, at least {numel} needed")
+    return t.data_ptr()
+
+
 def _stream(stream) -> int:
     if stream is not None:
         return stream
@@ -52,6 +75,11 @@ class GiPass:
         except Exception:
             pass
 
+    def _check_gbuf(self, gbuf) -> None:
+        if (gbuf.width, gbuf.height) != (self.width, self.height):
+            raise _abi.SgiArgumentError(f"G-buffer is {gbuf.width}x{gbuf.height}, the context's viewport "
+                             f"{self.width}x{self.height}")
+
     def set_frame(self, frame: _abi.Frame) -> None:
         _abi.call(self.lib, "sgi_set_frame", self.ctx, ctypes.byref(frame))
 
@@ -59,71 +87,144 @@ class GiPass:
         _abi.call(self.lib, "sgi_set_csm", self.ctx, ctypes.byref(csm))
 
     def cull_global(self, vpls, n, out, visible, stream=None) -> None:
-        _abi.call(self.lib, "sgi_cull_global", self.ctx, _ptr(vpls), n, _ptr(out), _ptr(visible), _stream(stream))
+        cap = min(max(n, 0), _abi.MAX_TOTAL_VPLS_PER_FRAME)
+        vf = _abi.VPL_FLOATS
+        _abi.call(self.lib, "sgi_cull_global", self.ctx, _dev(vpls, "vpls", n * vf), n,
+                  _dev(out, "out", cap * vf), _dev(visible, "visible", 1 + cap, (_I32,)), _stream(stream))
 
     def visible_atlas_entries(self, entries, visible, max_count, out, stream=None) -> None:
-        _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _ptr(entries), _ptr(visible), max_count, _ptr(out),
+        _abi.call(self.lib, "sgi_visible_atlas_entries", self.ctx, _dev(entries, "entries", 0, (_I32,)),
+                  _dev(visible, "visible", 1, (_I32,)), max_count, _dev(out, "out", 2 * max_count, (_I32,)),
                   _stream(stream))
 
     def taa(self, screen, prev_screen, velocity, out, stream=None) -> None:
         """taa.fs: the current frame blended with the neighbourhood-clamped history."""
-        _abi.call(self.lib, "sgi_taa", self.ctx, _ptr(screen), _ptr(prev_screen), _ptr(velocity), _ptr(out),
-                  _stream(stream))
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_taa", self.ctx, _dev(screen, "screen", 3 * P), _dev(prev_screen, "prev_screen", 3 * P),
+                  _dev(velocity, "velocity", 2 * P, optional=True), _dev(out, "out", 3 * P), _stream(stream))
 
     def tonemap(self, lit, out, gamma=2.2, stream=None) -> None:
         """gammaTonemap.fs: ACES film + gamma of the lit buffer into RGBA8 texels."""
-        _abi.call(self.lib, "sgi_tonemap", self.ctx, _ptr(lit), gamma, _ptr(out), _stream(stream))
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_tonemap", self.ctx, _dev(lit, "lit", 3 * P), gamma,
+                  _dev(out, "out", P * 4 // out.element_size() if out is not None else 0, (_I32, "uint32", _U8)),
+                  _stream(stream))
+
+    def render_vpl_cubes(self, positions, colors, vpls, updates, entries, distance: _abi.CubeAtlases,
+                         diffuse: _abi.CubeAtlases | None = None, near=_abi.REFERENCE_VPL_NEAR_PLANE,
+                         stream=None) -> None:
+        """The VPL shadow pre-pass: the cubes of vpls[updates] from world-space triangles."""
+        n = 0 if updates is None else updates.numel()
+        count = 0 if positions is None else positions.numel() // 9
+        if n > 0:  # the kernels index vpls and entries by these values
+            lo, hi = int(updates.min()), int(updates.max())
+            if lo < 0 or hi >= vpls.numel() // _abi.VPL_FLOATS or hi >= entries.numel() // 2:
+                raise _abi.SgiArgumentError(f"updates must index vpls and entries (got {lo}..{hi})")
+        tri = _abi.Triangles(count, _dev(positions, "positions", 9 * count, optional=count == 0),
+                             _dev(colors, "colors", count, (_I32, "uint32"), optional=True))
+        _abi.call(self.lib, "sgi_render_vpl_cubes", self.ctx, ctypes.byref(tri),
+                  _dev(vpls, "vpls", 0, optional=n == 0), _dev(updates, "updates", n, (_I32,), optional=n == 0), n,
+                  _dev(entries, "entries", 0, (_I32,), optional=n == 0), near, ctypes.byref(distance),
+                  ctypes.byref(diffuse) if diffuse is not None else None, _stream(stream))
+
+    def taa_history(self, taa_out, prev_screen, stream=None) -> None:
+        """The TAA output as the next frame's history (RGBA16F rounding, clamp, RGB8 store, k / 255)."""
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_taa_history", self.ctx, _dev(taa_out, "taa_out", 3 * P),
+                  _dev(prev_screen, "prev_screen", 3 * P), _stream(stream))
+
+    def fxaa(self, tonemapped, out, params: _abi.FxaaParams | None = None, stream=None) -> None:
+        """fxaa_luminance.fs + fxaa_smoothing.fs of the tonemapped RGBA8 target."""
+        P = self.width * self.height
+        n = lambda t: P * 4 // t.element_size() if t is not None else 0  # noqa: E731
+        _abi.call(self.lib, "sgi_fxaa", self.ctx, _dev(tonemapped, "tonemapped", n(tonemapped), (_I32, "uint32", _U8)),
+                  ctypes.byref(params) if params is not None else None,
+                  _dev(out, "out", n(out), (_I32, "uint32", _U8)), _stream(stream))
 
     def gbuffer_unpack(self, packed: _abi.GbufferPacked, normal, albedo, refl, rma, velocity=None,
                        stream=None) -> None:
         """The reference's packed G-buffer attachments into the FP32 planes the passes read."""
-        _abi.call(self.lib, "sgi_gbuffer_unpack", self.ctx, ctypes.byref(packed), _ptr(normal), _ptr(albedo),
-                  _ptr(refl), _ptr(rma), _ptr(velocity), _stream(stream))
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_gbuffer_unpack", self.ctx, ctypes.byref(packed), _dev(normal, "normal", 3 * P),
+                  _dev(albedo, "albedo", 4 * P), _dev(refl, "refl", 2 * P), _dev(rma, "rma", 3 * P),
+                  _dev(velocity, "velocity", 2 * P, optional=True), _stream(stream))
 
     def color_vpls(self, visible, visible_entries, diffuse: _abi.CubeAtlases, distance: _abi.CubeAtlases,
                    max_count, vpls, color_multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER, stream=None) -> None:
         """vpl_light_color.cs: colour and specular position of the visible VPL records, in place."""
-        _abi.call(self.lib, "sgi_color_vpls", self.ctx, _ptr(visible), _ptr(visible_entries), ctypes.byref(diffuse),
-                  ctypes.byref(distance), color_multiplier, max_count, _ptr(vpls), _stream(stream))
+        n = min(max(max_count, 0), _abi.MAX_TOTAL_VPLS_PER_FRAME)  # the C ABI refuses counts outside 0..4480
+        _abi.call(self.lib, "sgi_color_vpls", self.ctx, _dev(visible, "visible", 1, (_I32,)),
+                  _dev(visible_entries, "visible_entries", 2 * n, (_I32,), optional=n == 0),
+                  ctypes.byref(diffuse), ctypes.byref(distance), color_multiplier, max_count,
+                  _dev(vpls, "vpls", _abi.VPL_FLOATS * n, optional=n == 0), _stream(stream))
 
     def cull_tiles(self, gbuf: _abi.Gbuffer, vpls, visible, tiles, remap=0, stage1=None, stream=None) -> None:
-        _abi.call(self.lib, "sgi_cull_tiles", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), remap,
-                  _ptr(stage1), _ptr(tiles), _stream(stream))
+        self._check_gbuf(gbuf)
+        _abi.call(self.lib, "sgi_cull_tiles", self.ctx, ctypes.byref(gbuf), _dev(vpls, "vpls", 0),
+                  _dev(visible, "visible", 1, (_I32,)), remap, _dev(stage1, "stage1", 8 * self.num_tiles, optional=True),
+                  _dev(tiles, "tiles", self.num_tiles * (1 + self.K), (_I32,)), _stream(stream))
 
     def gather(self, gbuf: _abi.Gbuffer, vpls, visible, entries, cubes: _abi.CubeAtlases, tiles, color, reservoir,
                precision=_abi.PRECISION_FP32, stream=None) -> None:
-        _abi.call(self.lib, "sgi_gather", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
-                  ctypes.byref(cubes), _ptr(tiles), precision, _ptr(color), _ptr(reservoir), _stream(stream))
+        self._check_gbuf(gbuf)
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_gather", self.ctx, ctypes.byref(gbuf), _dev(vpls, "vpls", 0),
+                  _dev(visible, "visible", 1, (_I32,)), _dev(entries, "entries", 0, (_I32,)), ctypes.byref(cubes),
+                  _dev(tiles, "tiles", self.num_tiles * (1 + self.K), (_I32,), optional=True), precision,
+                  _dev(color, "color", 3 * P), _dev(reservoir, "reservoir", 4 * P), _stream(stream))
 
     def gather_ex(self, gbuf: _abi.Gbuffer, vpls, visible, entries, cubes: _abi.CubeAtlases, tiles, color, reservoir,
                   mode=_abi.GATHER_ALL, samples=8, accumulate_fp64=False, precision=_abi.PRECISION_FP32,
                   rejection_limit=5, stream=None) -> None:
         opts = _abi.GatherOpts(mode, samples, precision, 1 if accumulate_fp64 else 0, rejection_limit)
-        _abi.call(self.lib, "sgi_gather_ex", self.ctx, ctypes.byref(gbuf), _ptr(vpls), _ptr(visible), _ptr(entries),
-                  ctypes.byref(cubes), _ptr(tiles), ctypes.byref(opts), _ptr(color), _ptr(reservoir),
-                  _stream(stream))
+        self._check_gbuf(gbuf)
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_gather_ex", self.ctx, ctypes.byref(gbuf), _dev(vpls, "vpls", 0),
+                  _dev(visible, "visible", 1, (_I32,)), _dev(entries, "entries", 0, (_I32,)), ctypes.byref(cubes),
+                  _dev(tiles, "tiles", self.num_tiles * (1 + self.K), (_I32,), optional=True), ctypes.byref(opts),
+                  _dev(color, "color", 3 * P), _dev(reservoir, "reservoir", 4 * P), _stream(stream))
 
     def direct_light(self, gbuf: _abi.Gbuffer, lit, stream=None) -> None:
-        _abi.call(self.lib, "sgi_direct_light", self.ctx, ctypes.byref(gbuf), _ptr(lit), _stream(stream))
+        self._check_gbuf(gbuf)
+        _abi.call(self.lib, "sgi_direct_light", self.ctx, ctypes.byref(gbuf),
+                  _dev(lit, "lit", 3 * self.width * self.height), _stream(stream))
 
     def denoise(self, cur: _abi.Gbuffer, prev: _abi.Gbuffer | None, visible, screen, color, reservoir, prev_gi,
                 prev_history, combined, gi_out, reservoir_out, history_out, stream=None) -> None:
+        self._check_gbuf(cur)
+        if prev is not None:
+            self._check_gbuf(prev)
+        P = self.width * self.height
         _abi.call(self.lib, "sgi_denoise", self.ctx, ctypes.byref(cur), ctypes.byref(prev) if prev else None,
-                  _ptr(visible), _ptr(screen), _ptr(color), _ptr(reservoir), _ptr(prev_gi), _ptr(prev_history),
-                  _ptr(combined), _ptr(gi_out), _ptr(reservoir_out), _ptr(history_out), _stream(stream))
+                  _dev(visible, "visible", 1, (_I32,)), _dev(screen, "screen", 3 * P), _dev(color, "color", 3 * P),
+                  _dev(reservoir, "reservoir", 4 * P), _dev(prev_gi, "prev_gi", 3 * P, optional=True),
+                  _dev(prev_history, "prev_history", P, optional=True), _dev(combined, "combined", 3 * P),
+                  _dev(gi_out, "gi_out", 3 * P), _dev(reservoir_out, "reservoir_out", 4 * P),
+                  _dev(history_out, "history_out", P), _stream(stream))
 
     def kpcn_features(self, gbuf: _abi.Gbuffer, reservoir, features, radiance, stream=None) -> None:
-        _abi.call(self.lib, "sgi_kpcn_features", self.ctx, ctypes.byref(gbuf), _ptr(reservoir), _ptr(features),
-                  _ptr(radiance), _stream(stream))
+        self._check_gbuf(gbuf)
+        P = self.width * self.height
+        _abi.call(self.lib, "sgi_kpcn_features", self.ctx, ctypes.byref(gbuf), _dev(reservoir, "reservoir", 4 * P),
+                  _dev(features, "features", 32 * P, (_BF16,)), _dev(radiance, "radiance", 3 * P), _stream(stream))
 
     def denoise_learned(self, cur: _abi.Gbuffer, prev: _abi.Gbuffer | None, visible, screen, color, reservoir,
                         prev_gi, prev_history, net: _abi.KpcnNet, threshold: float, workspace, combined, gi_out,
                         reservoir_out, history_out, fallback_tiles=None, stream=None) -> None:
+        self._check_gbuf(cur)
+        if prev is not None:
+            self._check_gbuf(prev)
+        P = self.width * self.height
         _abi.call(self.lib, "sgi_denoise_learned", self.ctx, ctypes.byref(cur), ctypes.byref(prev) if prev else None,
-                  _ptr(visible), _ptr(screen), _ptr(color), _ptr(reservoir), _ptr(prev_gi), _ptr(prev_history),
-                  ctypes.byref(net), ctypes.c_float(threshold), _ptr(workspace),
-                  workspace.numel() * workspace.element_size(), _ptr(combined), _ptr(gi_out), _ptr(reservoir_out),
-                  _ptr(history_out), _ptr(fallback_tiles), _stream(stream))
+                  _dev(visible, "visible", 1, (_I32,)), _dev(screen, "screen", 3 * P), _dev(color, "color", 3 * P),
+                  _dev(reservoir, "reservoir", 4 * P), _dev(prev_gi, "prev_gi", 3 * P, optional=True),
+                  _dev(prev_history, "prev_history", P, optional=True), ctypes.byref(net), ctypes.c_float(threshold),
+                  _dev(workspace, "workspace", int(self.lib.sgi_kpcn_workspace_bytes(self.height, self.width)), (_U8,)),
+                  workspace.numel() * workspace.element_size(), _dev(combined, "combined", 3 * P),
+                  _dev(gi_out, "gi_out", 3 * P), _dev(reservoir_out, "reservoir_out", 4 * P),
+                  _dev(history_out, "history_out", P),
+                  _dev(fallback_tiles, "fallback_tiles", 1 + (-(-self.width // 16)) * (-(-self.height // 16)), (_I32,),
+                       optional=True), _stream(stream))
 
     def kpcn_workspace(self, device="cuda"):
         import torch

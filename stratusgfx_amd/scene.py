@@ -91,6 +91,18 @@ class Scene:
         except Exception:
             pass
 
+    def boxes(self) -> np.ndarray:
+        """The scene's boxes, [n][16] float32 (sgi_scene_boxes' records)."""
+        n = int(self.lib.sgi_scene_num_boxes(self.handle))
+        b = np.zeros((n, 16), np.float32)
+        _abi.call(self.lib, "sgi_scene_boxes", self.handle, b.ctypes.data)
+        return b
+
+    def triangles(self) -> tuple[np.ndarray, np.ndarray]:
+        """The boxes as a world-space triangle list (12 per box, in box order) with RGBA8 base colours:
+        the static meshes the VPL cube pass (sgi_render_vpl_cubes) draws."""
+        return box_triangles(self.boxes())
+
     def vpls(self) -> tuple[np.ndarray, np.ndarray]:
         n = self.w.num_vpls
         v = np.zeros((n, _abi.VPL_FLOATS), np.float32)
@@ -148,8 +160,42 @@ class Inputs:
         return c
 
 
+# corners of a box's faces (local signs), two triangles per face
+_FACES = [((1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)), ((-1, -1, 1), (-1, 1, 1), (-1, 1, -1), (-1, -1, -1)),
+          ((-1, 1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1)), ((-1, -1, 1), (-1, -1, -1), (1, -1, -1), (1, -1, 1)),
+          ((-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)), ((1, -1, -1), (-1, -1, -1), (-1, 1, -1), (1, 1, -1))]
+
+
+def box_triangles(boxes: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """World-space triangles [12 n][3][3] float32 of boxes rotated about +Y (records of sgi_scene_boxes:
+    centre, cos yaw, half extents, sin yaw, albedo ...), and their base colours as RGBA8 (UNORM8 albedo,
+    alpha 255)."""
+    b = np.asarray(boxes, np.float32)
+    n = b.shape[0]
+    c, cy, h, sy = b[:, 0:3], b[:, 3], b[:, 4:7], b[:, 7]
+    corners = np.zeros((n, 6, 4, 3), np.float32)
+    for f, quad in enumerate(_FACES):
+        for k, sgn in enumerate(quad):
+            lx, ly, lz = (np.float32(sgn[i]) * h[:, i] for i in range(3))
+            corners[:, f, k, 0] = c[:, 0] + (cy * lx + sy * lz)
+            corners[:, f, k, 1] = c[:, 1] + ly
+            corners[:, f, k, 2] = c[:, 2] + (-sy * lx + cy * lz)
+    tri = np.stack([corners[:, :, [0, 1, 2]], corners[:, :, [0, 2, 3]]], axis=2)  # [n][6][2][3][3]
+    pos = np.ascontiguousarray(tri.reshape(n * 12, 3, 3), np.float32)
+    alb = np.clip(b[:, 8:11], 0, 1)
+    code = np.floor(alb * np.float32(255) + np.float32(0.5)).astype(np.uint32)
+    rgba = code[:, 0] | (code[:, 1] << 8) | (code[:, 2] << 16) | np.uint32(0xFF000000)
+    return pos, np.repeat(rgba, 12).astype(np.uint32)
+
+
 def _cube_dtype(fmt: int):
-    return np.float32 if fmt == _abi.TEXEL_F32 else np.float16
+    return {_abi.TEXEL_F32: np.float32, _abi.TEXEL_F16: np.float16, _abi.TEXEL_UNORM16: np.uint16}[fmt]
+
+
+def _cube_dtype_torch(fmt: int):
+    import torch
+    # torch has no general uint16 tensor: the UNORM16 codes are held as int16 bits
+    return {_abi.TEXEL_F32: torch.float32, _abi.TEXEL_F16: torch.float16, _abi.TEXEL_UNORM16: torch.int16}[fmt]
 
 
 def generate_host(scene: Scene) -> Inputs:
@@ -198,7 +244,7 @@ def generate_device(scene: Scene, device="cuda", stream=None) -> Inputs:
     vpls = torch.from_numpy(vpls_np).to(device)
     entries = torch.from_numpy(entries_np).to(device)
     lpa, na = layers_per_atlas(w.num_vpls), num_atlases(w.num_vpls)
-    cdt = torch.float32 if w.cube_format == _abi.TEXEL_F32 else torch.float16
+    cdt = _cube_dtype_torch(w.cube_format)
     atlases = [torch.empty((lpa, 6, w.cube_res, w.cube_res), dtype=cdt, device=device) for _ in range(na)]
     inp = Inputs(W, H, scene.frame, scene.csm, depth, normal, albedo, refl, rma, vel, ids, vpls, entries,
                  atlases, w.cube_res, w.cube_format, lpa)

@@ -37,14 +37,16 @@ STRUCTS = {"sgi_vpl": None, "sgi_atlas_entry": None, "sgi_tile_stage1": None, "s
            "sgi_config": _abi.Config, "sgi_frame_io": _abi.FrameIO, "sgi_scene_desc": _abi.SceneDesc,
            "sgi_view": _abi.View, "sgi_kpcn_net": _abi.KpcnNet, "sgi_gather_opts": _abi.GatherOpts,
            "sgi_views_config": _abi.ViewsConfig, "sgi_view_io": _abi.ViewIO,
-           "sgi_gbuffer_packed": _abi.GbufferPacked}
+           "sgi_gbuffer_packed": _abi.GbufferPacked, "sgi_fxaa_params": _abi.FxaaParams}
 
 
 CONSTANTS = {"SGI_KPCN_CHANNELS": _abi.KPCN_CHANNELS, "SGI_KPCN_LEVELS": _abi.KPCN_LEVELS,
              "SGI_KPCN_LAYERS": _abi.KPCN_LAYERS, "SGI_KPCN_LAYER_PARAMS": _abi.KPCN_LAYER_PARAMS,
              "SGI_KPCN_PARAMS": _abi.KPCN_PARAMS, "SGI_KPCN_TAPS": _abi.KPCN_TAPS,
              "SGI_KPCN_ERR_CHANNEL": _abi.KPCN_ERR_CHANNEL, "SGI_KPCN_BLEND_CHANNEL": _abi.KPCN_BLEND_CHANNEL,
-             "SGI_KPCN_FEATURES": _abi.KPCN_FEATURES, "SGI_TEXEL_RGBA8": _abi.TEXEL_RGBA8}
+             "SGI_KPCN_FEATURES": _abi.KPCN_FEATURES, "SGI_TEXEL_RGBA8": _abi.TEXEL_RGBA8,
+             "SGI_TEXEL_UNORM16": _abi.TEXEL_UNORM16, "SGI_TEXEL_F16": _abi.TEXEL_F16,
+             "SGI_TEXEL_F32": _abi.TEXEL_F32}
 
 
 def test_struct_layouts_match_c():

@@ -1,0 +1,173 @@
+"""Pins of the oracle's VPL shadow pre-pass (orc_render_vpl_cubes: the cube
+update loop of StratusRendererBackend.cpp:1426-1575 with shadow.vs:22-33 and
+shadowVpl.fs:17-37) against what the geometry fixes:
+
+- a light at the centre of a closed box room (half size a, each wall its own
+  colour): every texel of face f holds a |d| / far with d the texel's face ray,
+  in the F32 and UNORM16 formats, and the colour of the wall on face f's axis
+  (which pins the six face orientations);
+- the lookups the gather makes (the GL major-axis table, NEAREST) of the cubes
+  the pass wrote return, for random directions, the distance to the wall the
+  direction meets: producer and consumer agree on every texel's place;
+- GL's near-plane clipping (0.1): a wall closer than near along the face axis
+  is not drawn; the clear (1, colour 0) stays where nothing is within far;
+- LESS keeps the first of two coincident triangles;
+- on the Cornell box, the cubes agree with the generator's independent
+  analytic ray caster wherever the near plane does not clip.
+"""
+import ctypes
+import dataclasses
+
+import numpy as np
+
+import oracle as orc
+from stratusgfx_amd import _abi
+from stratusgfx_amd import scene as sc
+
+L = orc.lib()
+FAR = 500.0
+
+
+def _atlases(res, fmt, layers=1, diffuse=True):
+    dt = {_abi.TEXEL_F32: np.float32, _abi.TEXEL_F16: np.uint16, _abi.TEXEL_UNORM16: np.uint16}[fmt]
+    z = np.full((layers, 6, res, res), 7, dt)  # a sentinel: every updated texel is overwritten
+    c = np.full((layers, 6, res, res), 0x12345678, np.uint32)
+    cz, cc = _abi.CubeAtlases(), _abi.CubeAtlases()
+    cz.resolution, cz.format, cz.layers_per_atlas, cz.num_atlases = res, fmt, layers, 1
+    cc.resolution, cc.format, cc.layers_per_atlas, cc.num_atlases = res, _abi.TEXEL_RGBA8, layers, 1
+    cz.atlas[0], cc.atlas[0] = z.ctypes.data, c.ctypes.data
+    return z, c, cz, (cc if diffuse else None)
+
+
+def _vpl(p, far=FAR):
+    v = np.zeros((1, 16), np.float32)
+    v[0, :3], v[0, 3] = p, 1
+    v[0, 12], v[0, 13], v[0, 14] = far, far, 1
+    return v
+
+
+def _face_rays(f, res):
+    i = np.arange(res, dtype=np.float32)
+    sc_ = np.float32(2) * (i + np.float32(0.5)) / np.float32(res) - np.float32(1)
+    S, T = np.meshgrid(sc_, sc_)  # [j][i]
+    one = np.ones_like(S)
+    return {0: (one, -T, -S), 1: (-one, -T, S), 2: (S, one, T), 3: (S, -one, -T), 4: (S, -T, one),
+            5: (-S, -T, -one)}[f]
+
+
+ROOM_COLORS = [0xFF0000FF, 0xFF00FF00, 0xFFFF0000, 0xFF00FFFF, 0xFFFF00FF, 0xFFFFFF00]
+
+
+def _room(a, centre=(0.0, 0.0, 0.0)):
+    """A closed cube room: wall k on axis k // 2 at sign + (k even) / - (odd), wall k coloured ROOM_COLORS[k]."""
+    boxes = []
+    for k in range(6):
+        ax, sgn = k // 2, 1 if k % 2 == 0 else -1
+        c = np.array(centre, np.float32)
+        c[ax] += sgn * (a + 0.5)
+        h = np.full(3, a + 1.0, np.float32)
+        h[ax] = 0.5
+        boxes.append(np.concatenate([c, [1.0], h, [0.0], [0.5, 0.5, 0.5, 1, 0, k + 1, 0, 0]]).astype(np.float32))
+    pos, _ = sc.box_triangles(np.array(boxes))
+    return pos, np.repeat(np.array(ROOM_COLORS, np.uint32), 12)
+
+
+def test_room_faces_hold_the_closed_form():
+    a, res = 2.0, 16
+    pos, col = _room(a)
+    for fmt in (_abi.TEXEL_F32, _abi.TEXEL_UNORM16):
+        z, c, cz, cc = _atlases(res, fmt)
+        orc.render_vpl_cubes(pos, col, _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc)
+        for f in range(6):
+            dx, dy, dz = [x.astype(np.float64) for x in _face_rays(f, res)]
+            want = a * np.sqrt(dx * dx + dy * dy + dz * dz) / FAR
+            if fmt == _abi.TEXEL_F32:
+                assert np.allclose(z[0, f], want, rtol=3e-6, atol=0), f
+            else:
+                assert np.all(np.abs(z[0, f].astype(np.float64) - want * 65535) <= 0.5 + 1e-3), f
+            assert np.all(c[0, f] == ROOM_COLORS[f]), f  # face f looks at wall f
+
+
+def test_gather_lookup_reads_the_texel_the_pass_wrote():
+    a, res = 3.0, 32
+    pos, col = _room(a, centre=(10.0, -4.0, 2.5))
+    L_ = np.array([10.0, -4.0, 2.5], np.float32)
+    z, c, cz, cc = _atlases(res, _abi.TEXEL_F32)
+    orc.render_vpl_cubes(pos, col, _vpl(L_), [0], [[0, 0]], 0.1, cz, cc)
+    rng = np.random.default_rng(2)
+    for _ in range(400):
+        d = rng.normal(size=3)
+        d /= np.abs(d).max()
+        # the texel centre nearest this direction: the GL table decides the texel, then its own ray
+        got = L.orc_cube_fetch(ctypes.byref(cz), 0, 0, *map(float, d))
+        ax = int(np.argmax(np.abs(d)))
+        f = 2 * ax + (0 if d[ax] > 0 else 1)
+        sc_, tc = {0: (-d[2], -d[1]), 1: (d[2], -d[1]), 2: (d[0], d[2]), 3: (d[0], -d[2]), 4: (d[0], -d[1]),
+                   5: (-d[0], -d[1])}[f]
+        i, j = min(int((sc_ + 1) / 2 * res), res - 1), min(int((tc + 1) / 2 * res), res - 1)
+        dx, dy, dz = [x[j, i] for x in _face_rays(f, res)]
+        assert abs(got - a * np.sqrt(dx * dx + dy * dy + dz * dz) / FAR) < 1e-6
+        assert c[0, f, j, i] == ROOM_COLORS[f]
+
+
+def test_near_plane_clips_and_clear_stays():
+    res = 8
+    # one wall of the +X face at x = 0.05 (inside the near plane) and one at x = 0.2
+    for x, drawn in ((0.05, False), (0.2, True)):
+        box = np.array([[x + 0.5, 0, 0, 1, 0.5, 50, 50, 0, 1, 1, 1, 1, 0, 1, 0, 0]], np.float32)
+        pos, col = sc.box_triangles(box)
+        z, c, cz, cc = _atlases(res, _abi.TEXEL_UNORM16)
+        orc.render_vpl_cubes(pos, col, _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc)
+        if drawn:
+            assert np.all(z[0, 0] < 65535) and np.all(c[0, 0] == 0xFFFFFFFF)
+        else:
+            # the near face is clipped; the slab's far side at x + 1 is drawn instead
+            dx, dy, dz = [t.astype(np.float64) for t in _face_rays(0, res)]
+            want = (x + 1) * np.sqrt(dx * dx + dy * dy + dz * dz) / FAR * 65535
+            assert np.all(np.abs(z[0, 0] - want) <= 0.5 + 1e-3)
+        # the -X face sees nothing within far: the clear (distance 1, colour 0, alpha 255)
+        assert np.all(z[0, 1] == 65535) and np.all(c[0, 1] == 0xFF000000)
+
+
+def test_less_keeps_the_first_of_coincident_triangles():
+    res = 4
+    tri = np.array([[[3, -5, -5], [3, 5, -5], [3, 0, 9]]], np.float32)
+    pos = np.concatenate([tri, tri])
+    for order in ((0xFF0000AA, 0xFF0000BB), (0xFF0000BB, 0xFF0000AA)):
+        z, c, cz, cc = _atlases(res, _abi.TEXEL_F32)
+        orc.render_vpl_cubes(pos, np.array(order, np.uint32), _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc)
+        covered = z[0, 0] < 1
+        assert covered.sum() > 4 and np.all(c[0, 0][covered] == order[0])
+
+
+def test_out_of_range_entry_writes_nothing():
+    pos, col = _room(2.0)
+    z, c, cz, cc = _atlases(8, _abi.TEXEL_F32)
+    orc.render_vpl_cubes(pos, col, _vpl((0, 0, 0)), [0], [[1, 0]], 0.1, cz, cc)  # atlas 1 does not exist
+    assert np.all(z == 7) and np.all(c == 0x12345678)
+
+
+def test_cornell_cubes_agree_with_the_analytic_ray_caster():
+    w = dataclasses.replace(sc.CORNELL_640x360, name="cornell-cubes-32", width=64, height=36, num_vpls=16,
+                            cube_res=32)
+    scene = sc.Scene(w)
+    inp = sc.generate_host(scene)  # analytic ray cast, F32
+    pos, col = scene.triangles()
+    lpa = inp.layers
+    z = np.zeros_like(inp.atlases[0])
+    cz = _abi.CubeAtlases()
+    cz.resolution, cz.format, cz.layers_per_atlas, cz.num_atlases = w.cube_res, _abi.TEXEL_F32, lpa, 1
+    cz.atlas[0] = z.ctypes.data
+    n = w.num_vpls
+    orc.render_vpl_cubes(pos, col, inp.vpls, np.arange(n), inp.entries, 0.1, cz)
+    ref = inp.atlases[0][:n]
+    got = z[:n]
+    # texels whose analytic hit lies closer than the near plane along the face axis are clipped by GL
+    major = np.stack([np.abs(np.stack(_face_rays(f, w.cube_res))).max(0) for f in range(6)])  # = 1
+    dl = np.stack([np.sqrt(sum(x.astype(np.float64) ** 2 for x in _face_rays(f, w.cube_res))) for f in range(6)])
+    t_axis = ref.astype(np.float64) * FAR / dl[None] * major[None]
+    ok = t_axis >= 0.1 * 1.01
+    close = np.abs(got.astype(np.float64) - ref) <= 1e-5 * np.maximum(ref, 1e-3) + 1e-6
+    frac = close[ok].mean()
+    print(f"[cubes vs ray caster] {frac:.5f} of {int(ok.sum())} unclipped texels agree")
+    assert ok.mean() > 0.6 and frac > 0.99  # VPLs sit 0.05 off a surface: the face toward it is clipped
