@@ -244,13 +244,20 @@ int sgi_color_vpls(sgi_ctx* ctx, const int32_t* visibleIndices, const sgi_atlas_
                    const sgi_cube_atlases* diffuse, const sgi_cube_atlases* distance, float colorMultiplier,
                    int32_t maxCount, sgi_vpl* vplUpdatedData, void* stream);
 
-/* World-space triangles of the static meshes the VPL cube pass draws (the
+/* Face culling of a triangle list, RenderFaceCulling (StratusRenderComponents.h:
+ * 20-24) as SetCullState applies it (StratusRendererBackend.cpp:901-918):
+ * NONE draws both sides; CW / CCW name the front faces' window-space winding
+ * and cull the back faces.  Meshes default to CCW (StratusRenderComponents.h:142). */
+enum { SGI_CULLING_NONE = 0, SGI_CULLING_CW = 1, SGI_CULLING_CCW = 2 };
+
+/* World-space triangles of the static meshes a shadow pass draws (the
  * reference draws staticPbrMeshes with their model matrices applied in
- * shadow.vs:31, StratusRendererBackend.cpp:1529-1536). */
+ * shadow.vs:31 and csm.vs:40, StratusRendererBackend.cpp:1529-1536, 1137). */
 typedef struct sgi_triangles {
     int32_t count;
-    const float* positions;  /* device [count][3][3]: world-space vertices */
+    const float* positions;  /* device [count][3][3]: world-space vertices, wound as the mesh */
     const uint32_t* colors;  /* device [count] RGBA8 base colour (r in byte 0); needed for diffuse output */
+    int32_t cull;            /* SGI_CULLING_* of the meshes */
 } sgi_triangles;
 
 /* PointLight::lightNearPlane (StratusLight.h:291), the near plane of every VPL cube face. */
@@ -263,8 +270,9 @@ typedef struct sgi_triangles {
  * the distance atlases and, when diffuse is not NULL, of the diffuse atlases.
  * Per face f, the view is lookAt(pos, pos + axis_f, up_f) (:873-886) with
  * perspective(90 deg, 1, nearPlane, v.farPlane) (:1450); the layer is
- * cleared (distance 1, colour 0) and every triangle drawn with fragment value
- * saturate(|P - pos| / farPlane) and its base colour, depth test LESS: per
+ * cleared (distance 1, colour 0) and every triangle drawn (culled by
+ * mesh->cull, as the pass draws with the meshes' own modes, :1536) with
+ * fragment value saturate(|P - pos| / farPlane) and its base colour, depth test LESS: per
  * texel the smallest stored value (the distance format's F32 value, binary16
  * or UNORM16 code) over the triangles the texel's pixel-centre ray meets with
  * view depth in [nearPlane, farPlane], the first such triangle in list order
@@ -282,6 +290,26 @@ typedef struct sgi_triangles {
 int sgi_render_vpl_cubes(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_vpl* vpls, const int32_t* updates,
                          int32_t numUpdates, const sgi_atlas_entry* entries, float nearPlane,
                          const sgi_cube_atlases* distance, const sgi_cube_atlases* diffuse, void* stream);
+
+/* glPolygonOffset's slope factor of the cascade render (StratusRendererBackend.cpp:1097). */
+#define SGI_REFERENCE_CSM_SLOPE_FACTOR 2.0f
+
+/* The sun cascades' depth render, RenderCSMDepth_ (StratusRendererBackend.cpp:
+ * 1081-1158) with csm.vs:29-41: layer c of depth (device [4][res][res] FP32,
+ * res = csm->resolution) receives every triangle transformed by
+ * csm->cascadeProjViews[c] (the cascade's projectionViewRender): the window
+ * depth z * 0.5 + 0.5 at each texel centre, plus glPolygonOffset(slopeFactor,
+ * 0)'s term slopeFactor * max(|dz_w/dx_w|, |dz_w/dy_w|) (GL's permitted form of
+ * the depth slope), clamped to [0, 1] (GL_DEPTH_CLAMP: no near / far clipping),
+ * depth test LESS against the clear 1.  Each mesh's culling is reversed as the
+ * pass draws it (:1029-1042, 1136-1137): CCW meshes keep their clockwise
+ * (light-averted) sides.  Coverage: the NDC edge functions of the texel centre
+ * all of one sign.  The shader's alpha test and the dynamic meshes do not
+ * apply (static, opaque triangles).  slopeFactor:
+ * SGI_REFERENCE_CSM_SLOPE_FACTOR in the reference.  The context keeps a
+ * workspace of 160 B per triangle, allocated on first use. */
+int sgi_render_csm(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_csm* csm, float slopeFactor, float* depth,
+                   void* stream);
 
 /* vpl_tiled_deferred_culling_stage1.cs:50-90 + _stage2.cs:61-129 (the dispatches
  * commented out at StratusRendererBackend.cpp:1733-1783).

@@ -1022,6 +1022,75 @@ void orc_taa(const float* screen, const float* prev, const float* vel, int W, in
 // floor(x * 255 + 0.5).  film (may be NULL) receives the value before the store.
 uint16_t orc_half_rn(float f) { return floatToHalfRN(f); }
 
+// The sun cascades' depth render, RenderCSMDepth_ (StratusRendererBackend.cpp:
+// 1081-1158) with csm.vs:29-41: per cascade, each triangle through
+// cascadeProjViews[c] (mat4 * vec4 as GLSL sums it, then the divide by w);
+// culling as the pass sets it, each mesh's mode reversed (RenderImmediate_(...,
+// true), :1029-1042): the NDC signed area ex1 ey2 - ex2 ey1 > 0 is
+// counter-clockwise; a texel centre (2 (i + 0.5) / res - 1, ...) is covered
+// when the three edge functions share a sign; z is interpolated by them;
+// glPolygonOffset(slope, 0) (:1097) adds slope * max(|dz_w/dx_w|, |dz_w/dy_w|)
+// = slope * max(|A|, |B|) / res for the NDC plane z = z0 + A dx + B dy; the
+// window depth z * 0.5 + 0.5 + offset is clamped to [0, 1] (GL_DEPTH_CLAMP,
+// :1094, which also keeps every caster unclipped) and kept when below the
+// stored one (LESS, clear 1).  GL's rasteriser's exact edge ownership
+// (top-left rule) is implementation-defined; here an edge texel belongs to both
+// triangles and the first wins ties.
+void orc_render_csm(const float* pos, int count, int meshCull, float slope, const sgi_csm* csm, float* depth) {
+    const int res = csm->resolution;
+    const int keepSign = meshCull == SGI_CULLING_CCW ? -1 : (meshCull == SGI_CULLING_CW ? 1 : 0);
+    auto edge = [](float ax, float ay, float bx, float by, float sx, float sy) {
+        return (bx - ax) * (sy - ay) - (by - ay) * (sx - ax);
+    };
+    auto work = [&](int c) {
+        std::vector<float> t;  // kept triangles: 9 NDC values + offset
+        for (int k = 0; k < count; ++k) {
+            float v[10];
+            for (int i = 0; i < 3; ++i) {
+                float cl[4];
+                mulMat4<float>(csm->cascadeProjViews[c], pos[9 * k + 3 * i], pos[9 * k + 3 * i + 1],
+                               pos[9 * k + 3 * i + 2], 1.0f, cl);
+                v[3 * i] = cl[0] / cl[3];
+                v[3 * i + 1] = cl[1] / cl[3];
+                v[3 * i + 2] = cl[2] / cl[3];
+            }
+            const float ex1 = v[3] - v[0], ey1 = v[4] - v[1], ez1 = v[5] - v[2];
+            const float ex2 = v[6] - v[0], ey2 = v[7] - v[1], ez2 = v[8] - v[2];
+            const float det = ex1 * ey2 - ex2 * ey1;
+            const bool facing = keepSign > 0 ? det > 0.0f : (keepSign < 0 ? det < 0.0f : det != 0.0f);
+            if (!facing) continue;
+            const float A = (ez1 * ey2 - ez2 * ey1) / det, B = (ex1 * ez2 - ex2 * ez1) / det;
+            v[9] = slope * (std::fmax(std::fabs(A), std::fabs(B)) / (float)res);
+            t.insert(t.end(), v, v + 10);
+        }
+        const size_t n = t.size() / 10;
+        float* layer = depth + (size_t)c * res * res;
+        for (int j = 0; j < res; ++j)
+            for (int i = 0; i < res; ++i) {
+                const float sx = 2.0f * ((float)i + 0.5f) / (float)res - 1.0f;
+                const float sy = 2.0f * ((float)j + 0.5f) / (float)res - 1.0f;
+                float best = 1.0f;
+                for (size_t q = 0; q < n; ++q) {
+                    const float* v = &t[10 * q];
+                    const float w0 = edge(v[3], v[4], v[6], v[7], sx, sy);
+                    const float w1 = edge(v[6], v[7], v[0], v[1], sx, sy);
+                    const float w2 = edge(v[0], v[1], v[3], v[4], sx, sy);
+                    const bool in = (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) || (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f);
+                    const float sum = w0 + w1 + w2;
+                    if (!in || sum == 0.0f) continue;
+                    const float z = (w0 * v[2] + w1 * v[5] + w2 * v[8]) / sum;
+                    float dw = z * 0.5f + 0.5f + v[9];
+                    dw = dw < 0.0f ? 0.0f : (dw > 1.0f ? 1.0f : dw);
+                    if (dw < best) best = dw;
+                }
+                layer[(size_t)j * res + i] = best;
+            }
+    };
+    std::vector<std::thread> th;
+    for (int c = 0; c < SGI_NUM_CASCADES; ++c) th.emplace_back(work, c);
+    for (auto& x : th) x.join();
+}
+
 // The VPL shadow pre-pass: the cube update loop of StratusRendererBackend.cpp:
 // 1426-1575, shadow.vs:22-33 and shadowVpl.fs:17-37, for the lights
 // vpls[updates[s]] into cubes entries[updates[s]].  Face f of a light is the
@@ -1066,7 +1135,7 @@ V3<float> cross3(V3<float> a, V3<float> b) {
 float dot3(V3<float> a, V3<float> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 }  // namespace
 
-void orc_render_vpl_cubes(const float* pos, const uint32_t* colors, int count, const sgi_vpl* vpls,
+void orc_render_vpl_cubes(const float* pos, const uint32_t* colors, int count, int cull, const sgi_vpl* vpls,
                           const int32_t* updates, int numUpdates, const sgi_atlas_entry* entries, float nearPlane,
                           const sgi_cube_atlases* dist, const sgi_cube_atlases* diffuse) {
     const int res = dist->resolution;
@@ -1089,11 +1158,15 @@ void orc_render_vpl_cubes(const float* pos, const uint32_t* colors, int count, c
                     const V3<float> b = v3(q[3] - L.x, q[4] - L.y, q[5] - L.z);
                     const V3<float> c = v3(q[6] - L.x, q[7] - L.y, q[8] - L.z);
                     const V3<float> C0 = cross3(b, c), C1 = cross3(c, a), C2 = cross3(a, b);
+                    // SetCullState (StratusRendererBackend.cpp:901-918): seen from the light the
+                    // triangle winds counter-clockwise (its front under GL_CCW) when a . (b x c) < 0
+                    const float V = dot3(a, C0);
+                    if ((cull == SGI_CULLING_CCW && !(V < 0.0f)) || (cull == SGI_CULLING_CW && !(V > 0.0f))) continue;
                     const float e0 = dot3(d, C0), e1 = dot3(d, C1), e2 = dot3(d, C2);
                     const bool in = (e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) || (e0 <= 0.0f && e1 <= 0.0f && e2 <= 0.0f);
                     const float den = e0 + e1 + e2;
                     if (!in || den == 0.0f) continue;
-                    const float t = dot3(a, C0) / den;
+                    const float t = V / den;
                     if (!(t >= nearPlane && t <= farP)) continue;
                     const uint32_t key = depthKey(dist->format, (t * dlen) / farP);
                     if (key < best) {

@@ -54,7 +54,8 @@ _SIGNATURES = {
     "orc_fxaa_luminance": (None, [c_void_p, ctypes.c_long, c_void_p]),
     "orc_fxaa_smoothing": (None, [c_void_p, c_int, c_int, c_float, c_float, c_float, c_void_p]),
     "orc_half_rn": (ctypes.c_uint16, [c_float]),
-    "orc_render_vpl_cubes": (None, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_float,
+    "orc_render_csm": (None, [c_void_p, c_int, c_int, c_float, POINTER(_abi.Csm), c_void_p]),
+    "orc_render_vpl_cubes": (None, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_float,
                                     POINTER(_abi.CubeAtlases), POINTER(_abi.CubeAtlases)]),
     "orc_color_vpls": (None, [POINTER(_abi.Frame), c_int, c_void_p, POINTER(_abi.CubeAtlases),
                               POINTER(_abi.CubeAtlases), c_float, c_void_p]),
@@ -131,15 +132,24 @@ def tonemap(lit, gamma=2.2):
     return out, film
 
 
-def render_vpl_cubes(positions, colors, vpls, updates, entries, near, dist, diffuse=None):
+def render_vpl_cubes(positions, colors, vpls, updates, entries, near, dist, diffuse=None, cull=_abi.CULLING_CCW):
     """The VPL shadow pre-pass into host atlases `dist` / `diffuse` (CubeAtlases over numpy arrays)."""
     pos = np.ascontiguousarray(positions, np.float32)
     col = None if colors is None else np.ascontiguousarray(colors, np.uint32)
     vp = _f32(vpls)
     up = np.ascontiguousarray(updates, np.int32)
     en = np.ascontiguousarray(entries, np.int32)
-    lib().orc_render_vpl_cubes(_p(pos), _p(col), pos.shape[0], _p(vp), _p(up), up.shape[0], _p(en), near,
+    lib().orc_render_vpl_cubes(_p(pos), _p(col), pos.shape[0], cull, _p(vp), _p(up), up.shape[0], _p(en), near,
                                ctypes.byref(dist), ctypes.byref(diffuse) if diffuse is not None else None)
+
+
+def render_csm(positions, csm, slope=2.0, cull=_abi.CULLING_CCW):
+    """The cascades' depth render: depth [4][res][res] from world-space triangles and csm's matrices."""
+    pos = np.ascontiguousarray(positions, np.float32)
+    res = csm.resolution
+    depth = np.zeros((4, res, res), np.float32)
+    lib().orc_render_csm(_p(pos), pos.shape[0], cull, slope, ctypes.byref(csm), _p(depth))
+    return depth
 
 
 def taa_history(taa):

@@ -138,10 +138,14 @@ class SceneDesc(ctypes.Structure):
 
 
 class Triangles(ctypes.Structure):
-    _fields_ = [("count", c_int32), ("positions", c_void_p), ("colors", c_void_p)]
+    _fields_ = [("count", c_int32), ("positions", c_void_p), ("colors", c_void_p), ("cull", c_int32)]
+
+
+CULLING_NONE, CULLING_CW, CULLING_CCW = 0, 1, 2  # RenderFaceCulling; meshes default to CCW
 
 
 REFERENCE_VPL_NEAR_PLANE = 0.1  # StratusLight.h:291
+REFERENCE_CSM_SLOPE_FACTOR = 2.0  # glPolygonOffset(2, 0), StratusRendererBackend.cpp:1097
 
 
 class FxaaParams(ctypes.Structure):
@@ -168,6 +172,7 @@ SGI_SIGNATURES = {
     "sgi_taa": (c_int32, [_P, _P, _P, _P, _P, _P]),
     "sgi_tonemap": (c_int32, [_P, _P, c_float, _P, _P]),
     "sgi_taa_history": (c_int32, [_P, _P, _P, _P]),
+    "sgi_render_csm": (c_int32, [_P, POINTER(Triangles), POINTER(Csm), c_float, _P, _P]),
     "sgi_render_vpl_cubes": (c_int32, [_P, POINTER(Triangles), _P, _P, _I, _P, c_float, POINTER(CubeAtlases),
                                        POINTER(CubeAtlases), _P]),
     "sgi_fxaa": (c_int32, [_P, _P, _P, _P, _P]),

@@ -26,6 +26,8 @@ struct sgi_ctx {
     void* vplPrep = nullptr;      // [SGI_MAX_TOTAL_VPLS_PER_FRAME] packed records of the sampled gathers
     int32_t* cubeLists = nullptr; // per (light, face) triangle lists of sgi_render_vpl_cubes, grown on demand
     size_t cubeListInts = 0;
+    float* csmTris = nullptr;     // per cascade transformed triangle lists of sgi_render_csm, grown on demand
+    size_t csmTrisFloats = 0;
     int device = 0;
 };
 
@@ -94,6 +96,7 @@ void sgi_destroy(sgi_ctx* ctx) {
     if (ctx->visFlags) (void)hipFree(ctx->visFlags);
     if (ctx->vplPrep) (void)hipFree(ctx->vplPrep);
     if (ctx->cubeLists) (void)hipFree(ctx->cubeLists);
+    if (ctx->csmTris) (void)hipFree(ctx->csmTris);
     delete ctx;
 }
 
@@ -764,7 +767,8 @@ int sgi_render_vpl_cubes(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_vpl*
     if (!ctx || !mesh || !distance) return fail(SGI_E_INVALID, std::string(who) + ": NULL argument");
     if (numUpdates < 0 || numUpdates > SGI_MAX_TOTAL_VPLS_PER_FRAME)
         return fail(SGI_E_INVALID, std::string(who) + ": numUpdates must be within 0..4480");
-    if (mesh->count < 0 || (mesh->count > 0 && !mesh->positions))
+    if (mesh->count < 0 || (mesh->count > 0 && !mesh->positions) || mesh->cull < SGI_CULLING_NONE ||
+        mesh->cull > SGI_CULLING_CCW)
         return fail(SGI_E_INVALID, std::string(who) + ": bad triangle list");
     if (!(nearPlane > 0.0f) || !std::isfinite(nearPlane))
         return fail(SGI_E_INVALID, std::string(who) + ": nearPlane must be finite and > 0");
@@ -798,9 +802,36 @@ int sgi_render_vpl_cubes(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_vpl*
         ctx->cubeListInts = need;
     }
     const hipError_t e = sgi_launch::renderVplCubes(mesh->positions, diffuse ? mesh->colors : nullptr, mesh->count,
-                                                    vpls, updates, numUpdates, entries, nearPlane, dz,
+                                                    mesh->cull, vpls, updates, numUpdates, entries, nearPlane, dz,
                                                     distance->format, dc, diffuse != nullptr, batchMax, ctx->cubeLists,
                                                     (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, who);
+}
+
+int sgi_render_csm(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_csm* csm, float slopeFactor, float* depth,
+                   void* stream) {
+    const char* who = "sgi_render_csm";
+    if (!ctx || !mesh || !csm || !depth) return fail(SGI_E_INVALID, std::string(who) + ": NULL argument");
+    if (mesh->count < 0 || (mesh->count > 0 && !mesh->positions) || mesh->cull < SGI_CULLING_NONE ||
+        mesh->cull > SGI_CULLING_CCW)
+        return fail(SGI_E_INVALID, std::string(who) + ": bad triangle list");
+    if (csm->resolution <= 0 || csm->resolution > 16384)
+        return fail(SGI_E_INVALID, std::string(who) + ": cascade resolution must be within 1..16384");
+    if (!std::isfinite(slopeFactor)) return fail(SGI_E_INVALID, std::string(who) + ": slopeFactor must be finite");
+    // the pass reverses each mesh's culling (RenderImmediate_(..., true)): a CCW mesh keeps its clockwise
+    // (light-averted) triangles, a CW mesh its counter-clockwise ones
+    const int keepSign = mesh->cull == SGI_CULLING_CCW ? -1 : (mesh->cull == SGI_CULLING_CW ? 1 : 0);
+    const size_t need = sgi_launch::csmScratchFloats(mesh->count);
+    if (need > ctx->csmTrisFloats) {
+        if (ctx->csmTris) (void)hipFree(ctx->csmTris);
+        ctx->csmTris = nullptr;
+        ctx->csmTrisFloats = 0;
+        const hipError_t e = hipMalloc((void**)&ctx->csmTris, need * sizeof(float));
+        if (e != hipSuccess) return hipFail(e, "sgi_render_csm: triangle workspace");
+        ctx->csmTrisFloats = need;
+    }
+    const hipError_t e = sgi_launch::renderCsm(mesh->positions, mesh->count, keepSign, slopeFactor, csm->resolution,
+                                               csm->cascadeProjViews, ctx->csmTris, depth, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, who);
 }
 

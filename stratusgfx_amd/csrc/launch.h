@@ -25,10 +25,17 @@ hipError_t tonemap(const float* lit, long n, float gamma, uint32_t* out, hipStre
 // cubes of vpls[updates[0 .. numUpdates)] rendered from `count` world-space
 // triangles, in batches of batchMax lights; scratch holds cubeListInts(batchMax, count) ints.
 size_t cubeListInts(int batch, int count);
-hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, const sgi_vpl* vpls,
+hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, int cull, const sgi_vpl* vpls,
                           const int32_t* updates, int numUpdates, const sgi_atlas_entry* entries, float nearPlane,
                           const sgi_dev::CubeArgs& dist, int format, const sgi_dev::CubeArgs& diffuse,
                           bool withDiffuse, int batchMax, int32_t* scratch, hipStream_t s);
+
+// Sun cascades' depth render (RenderCSMDepth_ with csm.vs): layers [4][res][res];
+// scratch holds csmScratchFloats(count) floats.  keepSign: +1 keep NDC-counter-clockwise
+// triangles, -1 clockwise, 0 both.
+size_t csmScratchFloats(int count);
+hipError_t renderCsm(const float* pos, int count, int keepSign, float slope, int res,
+                     const float (*projViews)[16], float* scratch, float* depth, hipStream_t s);
 
 // PerformTaaPostFx_'s history copy: TAA output -> the RGB8 history as read back (k / 255)
 hipError_t taaHistory(const float* taa, long n, float* prev, hipStream_t s);

@@ -320,12 +320,16 @@ void buildCsm(sgi_scene& s, const sgi_view& v, const ViewMats& m, sgi_csm* out) 
     normalize3d(fwd);
     double upRef[3] = {0.0, 1.0, 0.0};
     if (std::fabs(fwd[1]) > 0.99) { upRef[0] = 1.0; upRef[1] = 0.0; }
+    // The reference's light camera looks at the sun (infiniteLightDirection = its forward,
+    // StratusRendererBackend.cpp:2455-2457, Camera::GetDirection) and is right-handed, so its light
+    // space has x cross y = the direction away from the sun: right = upRef x fwd, up = fwd x right.
+    // That fixes the window winding its (reversed) face culling acts on.
     double right[3], up[3];
-    cross3(fwd, upRef, right);
+    cross3(upRef, fwd, right);
     normalize3d(right);
-    cross3(right, fwd, up);
+    cross3(fwd, right, up);
     for (int k = 0; k < 3; ++k) { r.right[k] = (float)right[k]; r.up[k] = (float)up[k]; r.fwd[k] = (float)fwd[k]; }
-    // depth range: the scene bounds along fwd
+    // traced range: the scene bounds along fwd (every caster; depths map through the slice range below)
     double dmin = 1e300, dmax = -1e300;
     for (int c = 0; c < 8; ++c) {
         const double p[3] = {(c & 1) ? s.boundsMax[0] : s.boundsMin[0], (c & 2) ? s.boundsMax[1] : s.boundsMin[1],
@@ -371,28 +375,34 @@ void buildCsm(sgi_scene& s, const sgi_view& v, const ViewMats& m, sgi_csm* out) 
                 maxLen = std::max(maxLen, std::sqrt(dx * dx + dy * dy + dz * dz));
             }
         const double dk = std::ceil(maxLen);
-        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300;
+        double mnx = 1e300, mxx = -1e300, mny = 1e300, mxy = -1e300, mne = 1e300, mxe = -1e300;
         for (int p = 0; p < 8; ++p) {
             const double x = corners[p][0] * right[0] + corners[p][1] * right[1] + corners[p][2] * right[2];
             const double y = corners[p][0] * up[0] + corners[p][1] * up[1] + corners[p][2] * up[2];
+            const double e = corners[p][0] * fwd[0] + corners[p][1] * fwd[1] + corners[p][2] * fwd[2];
             mnx = std::min(mnx, x); mxx = std::max(mxx, x);
             mny = std::min(mny, y); mxy = std::max(mxy, y);
+            mne = std::min(mne, e); mxe = std::max(mxe, e);
         }
         const double T = dk / res;  // texel-snapped centre keeps the cascade stable
         const double cx = std::floor((mxx + mnx) / (2.0 * T)) * T, cy = std::floor((mxy + mny) / (2.0 * T)) * T;
         r.cx[i] = (float)cx; r.cy[i] = (float)cy; r.dk[i] = (float)dk;
         r.dmin[i] = (float)dmin; r.dmax[i] = (float)dmax;
-        // clip = M p: x = (right.p - cx) 2/dk, y = (up.p - cy) 2/dk, z = 2 (fwd.p - dmin)/range - 1
+        r.emin[i] = (float)mne; r.erange[i] = (float)(mxe - mne);
+        // StratusRendererFrontend.cpp:767-784: the cascade camera sk = (cx, cy, minZ) in light space and
+        // the orthographic projection diag(2 / dk, 2 / dk, 1 / (maxZ - minZ)): clip = M p with
+        // x = (right.p - cx) 2 / dk, y = (up.p - cy) 2 / dk, z = (fwd.p - minZ) / (maxZ - minZ) in [0, 1]
+        // over the slice (window depth 0.5 .. 1; GL_DEPTH_CLAMP keeps casters outside it at 0 .. 1)
         double M[16] = {0};
-        const double range = (double)r.dmax[i] - (double)r.dmin[i];
+        const double erange = (double)r.erange[i];
         for (int k = 0; k < 3; ++k) {
             M[k * 4 + 0] = right[k] * 2.0 / dk;
             M[k * 4 + 1] = up[k] * 2.0 / dk;
-            M[k * 4 + 2] = fwd[k] * 2.0 / range;
+            M[k * 4 + 2] = fwd[k] / erange;
         }
         M[12] = -cx * 2.0 / dk;
         M[13] = -cy * 2.0 / dk;
-        M[14] = -2.0 * (double)r.dmin[i] / range - 1.0;
+        M[14] = -(double)r.emin[i] / erange;
         M[15] = 1.0;
         if (out) toFloat16(M, out->cascadeProjViews[i]);
         if (i > 0 && out) {

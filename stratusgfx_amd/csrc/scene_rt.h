@@ -284,10 +284,15 @@ SGI_HD inline uint32_t cubeTexelColor(const SceneView& s, const float pos[3], fl
 struct CsmRays {
     float right[3], up[3], fwd[3];  // light basis, fwd points away from the sun
     float cx[4], cy[4], dk[4];      // cascade centre (light x, y) and width
-    float dmin[4], dmax[4];         // depth range along fwd
+    float dmin[4], dmax[4];         // traced range along fwd (the scene's bounds)
+    float emin[4], erange[4];       // the cascade's depth mapping: its slice's corners along fwd
 };
 
-// Depth in [0,1] of the first surface seen by cascade c's texel (i, j).
+// Window depth of the first surface seen by cascade c's texel (i, j), as the
+// reference's cascade render stores it: NDC z = (e - emin) / erange with e the
+// surface's position along fwd (StratusRendererFrontend.cpp:769-773), window
+// z = 0.5 z + 0.5, clamped to [0, 1] by GL_DEPTH_CLAMP
+// (StratusRendererBackend.cpp:1094); 1 (the clear) where nothing is hit.
 SGI_HD inline float csmTexelValue(const SceneView& s, const CsmRays& r, int c, int i, int j, int res) {
     const float x = r.cx[c] + (((float)i + 0.5f) / (float)res * 2.0f - 1.0f) * 0.5f * r.dk[c];
     const float y = r.cy[c] + (((float)j + 0.5f) / (float)res * 2.0f - 1.0f) * 0.5f * r.dk[c];
@@ -296,7 +301,8 @@ SGI_HD inline float csmTexelValue(const SceneView& s, const CsmRays& r, int c, i
     const float range = r.dmax[c] - r.dmin[c];
     const Hit h = traceScene(s, o, r.fwd, 0.0f, range);
     if (h.box < 0) return 1.0f;
-    return h.t / range;
+    const float w = 0.5f * ((r.dmin[c] + h.t - r.emin[c]) / r.erange[c]) + 0.5f;
+    return w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
 }
 
 }  // namespace sgi_rt

@@ -1,3 +1,6 @@
+// The shadow-map producers of the pass's inputs: the VPL cube pre-pass and the
+// sun cascades' depth render, from world-space triangles.
+//
 // VPL distance / diffuse cube producer — the reference's shadow pre-pass for
 // virtual point lights (StratusRendererBackend.cpp:1426-1575): per updated
 // light, the six faces rendered with perspective(90 deg, 1, near, far) x
@@ -99,6 +102,7 @@ struct CubeJob {
     int batch;               // updates in this batch
     const sgi_atlas_entry* entries;
     float nearPlane;
+    int cull;                // SGI_CULLING_* of the meshes
     int32_t* lists;          // [batch][6][count]
     int32_t* listCounts;     // [batch][6]
 };
@@ -106,6 +110,18 @@ struct CubeJob {
 __device__ __forceinline__ f3 vertexRel(const float* pos, int k, int v, f3 L) {
     const float* q = pos + (size_t)k * 9 + v * 3;
     return f3{q[0] - L.x, q[1] - L.y, q[2] - L.z};
+}
+
+// The face culling of SetCullState (StratusRendererBackend.cpp:901-918) for a
+// triangle seen from the light: with a, b, c relative to the light,
+// V = a . (b x c) = (P - L) . n for n the winding's normal, so the triangle is
+// counter-clockwise in window space (its front under glFrontFace(GL_CCW)) when
+// V < 0.  The cube faces' views are rotations and their projections keep
+// orientation, so window winding is the world winding seen from the light.
+__device__ __forceinline__ bool culledFromLight(int cull, float V) {
+    if (cull == SGI_CULLING_CCW) return !(V < 0.0f);
+    if (cull == SGI_CULLING_CW) return !(V > 0.0f);
+    return false;
 }
 
 // (1) per (light, face): the triangles that may overlap the face, in order.
@@ -123,10 +139,13 @@ __global__ __launch_bounds__(kListBlock) void cubeFaceListKernel(CubeJob job) {
         const int k = c0 + (int)threadIdx.x;
         bool keep = false;
         if (k < job.count) {
+            const f3 a = vertexRel(job.pos, k, 0, L), b = vertexRel(job.pos, k, 1, L), c = vertexRel(job.pos, k, 2, L);
             float m[3], s[3], t[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) faceCoords(f, vertexRel(job.pos, k, q, L), m[q], s[q], t[q]);
-            keep = !outsideFrustum(m, s, t, -1.0f, 1.0f, -1.0f, 1.0f, job.nearPlane, farP);
+            faceCoords(f, a, m[0], s[0], t[0]);
+            faceCoords(f, b, m[1], s[1], t[1]);
+            faceCoords(f, c, m[2], s[2], t[2]);
+            keep = !outsideFrustum(m, s, t, -1.0f, 1.0f, -1.0f, 1.0f, job.nearPlane, farP) &&
+                   !culledFromLight(job.cull, dot(a, cross3(b, c)));
         }
         const uint64_t bal = __ballot(keep);
         if (lane == 0) waveCount[wave] = __popcll(bal);
@@ -263,13 +282,197 @@ __global__ __launch_bounds__(256) void cubeRasterKernel(CubeJob job, int res, in
     }
 }
 
+// ---- sun cascades: RenderCSMDepth_ (StratusRendererBackend.cpp:1081-1158) with csm.vs:29-41
+//
+// Per cascade c, every triangle is transformed by cascadeProjViews[c]
+// (projectionViewRender, an orthographic light projection) and rasterised
+// into layer c: the window depth z * 0.5 + 0.5 interpolated at the texel
+// centre, plus glPolygonOffset(factor, 0)'s slope term factor * m with
+// m = max(|dz_w/dx_w|, |dz_w/dy_w|) (the form GL allows in place of the
+// gradient's length), clamped to [0, 1] by GL_DEPTH_CLAMP (which also
+// disables near / far clipping), depth test LESS against the clear 1.  The
+// pass draws with each mesh's culling reversed (RenderImmediate_(..., true),
+// :1029-1042): CCW meshes cull their light-facing (counter-clockwise) sides.
+// Coverage: the 2-D edge functions of the NDC vertices at the texel centre all
+// of one sign (a texel centre on an edge belongs to both triangles, the first
+// in order winning LESS ties).
+struct CsmJob {
+    const float* pos;
+    int count;
+    int keepSign;        // +1: keep counter-clockwise (NDC area > 0), -1: clockwise, 0: both
+    float slope;         // glPolygonOffset factor
+    int res;
+    float m[SGI_NUM_CASCADES][16];
+    float* tris;         // [4][count][10]: NDC x, y, z of the 3 vertices and the polygon offset
+    int32_t* counts;     // [4]
+};
+
+__device__ __forceinline__ float edgeFn(float ax, float ay, float bx, float by, float sx, float sy) {
+    return (bx - ax) * (sy - ay) - (by - ay) * (sx - ax);
+}
+
+// (1) per cascade: the triangles that face the kept way and overlap the
+// cascade's window, in order, with their NDC vertices and polygon offset.
+__global__ __launch_bounds__(kListBlock) void csmListKernel(CsmJob job) {
+    const int c = blockIdx.x;
+    float* out = job.tris + (size_t)c * job.count * 10;
+    __shared__ int waveCount[kListBlock / 64];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < job.count; c0 += kListBlock) {
+        const int k = c0 + (int)threadIdx.x;
+        bool keep = false;
+        float v[10];
+        if (k < job.count) {
+            const float* q = job.pos + (size_t)k * 9;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                float cl[4];
+                mulMat4(job.m[c], q[3 * i], q[3 * i + 1], q[3 * i + 2], 1.0f, cl);
+                v[3 * i] = cl[0] / cl[3];
+                v[3 * i + 1] = cl[1] / cl[3];
+                v[3 * i + 2] = cl[2] / cl[3];
+            }
+            const float ex1 = v[3] - v[0], ey1 = v[4] - v[1], ez1 = v[5] - v[2];
+            const float ex2 = v[6] - v[0], ey2 = v[7] - v[1], ez2 = v[8] - v[2];
+            const float det = ex1 * ey2 - ex2 * ey1;  // twice the signed NDC area
+            const bool facing = job.keepSign > 0 ? det > 0.0f : (job.keepSign < 0 ? det < 0.0f : det != 0.0f);
+            const float mnx = fminf(v[0], fminf(v[3], v[6])), mxx = fmaxf(v[0], fmaxf(v[3], v[6]));
+            const float mny = fminf(v[1], fminf(v[4], v[7])), mxy = fmaxf(v[1], fmaxf(v[4], v[7]));
+            const bool inWindow = mxx >= -1.001f && mnx <= 1.001f && mxy >= -1.001f && mny <= 1.001f;
+            keep = facing && inWindow;
+            if (keep) {
+                const float A = (ez1 * ey2 - ez2 * ey1) / det;  // dz/dx of the NDC plane
+                const float B = (ex1 * ez2 - ex2 * ez1) / det;  // dz/dy
+                // window units: z_w = z / 2 + 1/2 and x_w = (x + 1) res / 2, so dz_w/dx_w = A / res
+                v[9] = job.slope * (fmaxf(fabsf(A), fabsf(B)) / (float)job.res);
+            }
+        }
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) waveCount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += waveCount[w];
+        if (keep) {
+            float* o = out + (size_t)(off + __popcll(bal & ((1ull << lane) - 1ull))) * 10;
+#pragma unroll
+            for (int i = 0; i < 10; ++i) o[i] = v[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int total = 0;
+            for (int w = 0; w < kListBlock / 64; ++w) total += waveCount[w];
+            base += total;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) job.counts[c] = base;
+}
+
+// (2) per (cascade, 32x32 region): rasterise the list's triangles that can
+// overlap the region (staged through LDS in order) at every texel centre.
+__global__ __launch_bounds__(256) void csmRasterKernel(CsmJob job, int regionsPerEdge, float* __restrict__ depth) {
+    const int perLayer = regionsPerEdge * regionsPerEdge;
+    const int c = blockIdx.x / perLayer, r = blockIdx.x % perLayer;
+    const int x0 = (r % regionsPerEdge) * kRegion, y0 = (r / regionsPerEdge) * kRegion;
+    const int res = job.res;
+    const float lox = 2.0f * (float)x0 / (float)res - 1.0f, hix = 2.0f * (float)min(x0 + kRegion, res) / (float)res - 1.0f;
+    const float loy = 2.0f * (float)y0 / (float)res - 1.0f, hiy = 2.0f * (float)min(y0 + kRegion, res) / (float)res - 1.0f;
+    const float* list = job.tris + (size_t)c * job.count * 10;
+    const int n = job.counts[c];
+    const int lx = x0 + ((int)threadIdx.x % 16) * 2, ly = y0 + ((int)threadIdx.x / 16) * 2;
+    float sx[4], sy[4], best[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        sx[p] = 2.0f * ((float)(lx + (p & 1)) + 0.5f) / (float)res - 1.0f;
+        sy[p] = 2.0f * ((float)(ly + (p >> 1)) + 0.5f) / (float)res - 1.0f;
+        best[p] = 1.0f;  // the depth clear
+    }
+    __shared__ float stage[kStage][10];
+    __shared__ int waveCount[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float mar = 1e-3f;
+    for (int c0 = 0; c0 < n; c0 += kStage) {
+        const int idx = c0 + (int)threadIdx.x;
+        bool keep = false;
+        float v[10];
+        if (idx < n) {
+#pragma unroll
+            for (int i = 0; i < 10; ++i) v[i] = list[(size_t)idx * 10 + i];
+            const float mnx = fminf(v[0], fminf(v[3], v[6])), mxx = fmaxf(v[0], fmaxf(v[3], v[6]));
+            const float mny = fminf(v[1], fminf(v[4], v[7])), mxy = fmaxf(v[1], fmaxf(v[4], v[7]));
+            keep = mxx >= lox - mar && mnx <= hix + mar && mxy >= loy - mar && mny <= hiy + mar;
+        }
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) waveCount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            off += w < wave ? waveCount[w] : 0;
+            total += waveCount[w];
+        }
+        if (keep) {
+            float* o = stage[off + __popcll(bal & ((1ull << lane) - 1ull))];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) o[i] = v[i];
+        }
+        __syncthreads();
+        for (int q = 0; q < total; ++q) {
+            const float* t = stage[q];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const float w0 = edgeFn(t[3], t[4], t[6], t[7], sx[p], sy[p]);
+                const float w1 = edgeFn(t[6], t[7], t[0], t[1], sx[p], sy[p]);
+                const float w2 = edgeFn(t[0], t[1], t[3], t[4], sx[p], sy[p]);
+                const bool in = (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) || (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f);
+                const float sum = w0 + w1 + w2;
+                if (!in || sum == 0.0f) continue;
+                const float z = (w0 * t[2] + w1 * t[5] + w2 * t[8]) / sum;
+                float dw = z * 0.5f + 0.5f + t[9];
+                dw = dw < 0.0f ? 0.0f : (dw > 1.0f ? 1.0f : dw);  // GL_DEPTH_CLAMP
+                if (dw < best[p]) best[p] = dw;
+            }
+        }
+        __syncthreads();
+    }
+    float* layer = depth + (size_t)c * res * res;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int i = lx + (p & 1), j = ly + (p >> 1);
+        if (i < res && j < res) layer[(size_t)j * res + i] = best[p];
+    }
+}
+
 }  // namespace
 
 namespace sgi_launch {
 
+size_t csmScratchFloats(int count) { return (size_t)SGI_NUM_CASCADES * count * 10 + SGI_NUM_CASCADES; }
+
+hipError_t renderCsm(const float* pos, int count, int keepSign, float slope, int res,
+                     const float (*projViews)[16], float* scratch, float* depth, hipStream_t s) {
+    CsmJob job{};
+    job.pos = pos;
+    job.count = count;
+    job.keepSign = keepSign;
+    job.slope = slope;
+    job.res = res;
+    for (int c = 0; c < SGI_NUM_CASCADES; ++c)
+        for (int k = 0; k < 16; ++k) job.m[c][k] = projViews[c][k];
+    job.tris = scratch;
+    job.counts = (int32_t*)(scratch + (size_t)SGI_NUM_CASCADES * count * 10);
+    hipLaunchKernelGGL(csmListKernel, dim3(SGI_NUM_CASCADES), dim3(kListBlock), 0, s, job);
+    const int rpe = (res + kRegion - 1) / kRegion;
+    hipLaunchKernelGGL(csmRasterKernel, dim3((unsigned)(SGI_NUM_CASCADES * rpe * rpe)), dim3(256), 0, s, job, rpe,
+                       depth);
+    return hipGetLastError();
+}
+
 size_t cubeListInts(int batch, int count) { return (size_t)batch * 6 * (size_t)count + (size_t)batch * 6; }
 
-hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, const sgi_vpl* vpls,
+hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, int cull, const sgi_vpl* vpls,
                           const int32_t* updates, int numUpdates, const sgi_atlas_entry* entries, float nearPlane,
                           const sgi_dev::CubeArgs& dist, int format, const sgi_dev::CubeArgs& diffuse,
                           bool withDiffuse, int batchMax, int32_t* scratch, hipStream_t s) {
@@ -277,7 +480,7 @@ hipError_t renderVplCubes(const float* pos, const uint32_t* colors, int count, c
     const int rpe = (res + kRegion - 1) / kRegion;
     for (int first = 0; first < numUpdates; first += batchMax) {
         const int batch = min(batchMax, numUpdates - first);
-        CubeJob job{pos, colors, count, vpls, updates, first, batch, entries, nearPlane, scratch,
+        CubeJob job{pos, colors, count, vpls, updates, first, batch, entries, nearPlane, cull, scratch,
                     scratch + (size_t)batch * 6 * (size_t)count};
         hipLaunchKernelGGL(cubeFaceListKernel, dim3((unsigned)(batch * 6)), dim3(kListBlock), 0, s, job);
         const dim3 grid((unsigned)(batch * 6 * rpe * rpe));

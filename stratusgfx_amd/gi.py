@@ -112,7 +112,7 @@ class GiPass:
 
     def render_vpl_cubes(self, positions, colors, vpls, updates, entries, distance: _abi.CubeAtlases,
                          diffuse: _abi.CubeAtlases | None = None, near=_abi.REFERENCE_VPL_NEAR_PLANE,
-                         stream=None) -> None:
+                         cull=_abi.CULLING_CCW, stream=None) -> None:
         """The VPL shadow pre-pass: the cubes of vpls[updates] from world-space triangles."""
         n = 0 if updates is None else updates.numel()
         count = 0 if positions is None else positions.numel() // 9
@@ -121,11 +121,20 @@ class GiPass:
             if lo < 0 or hi >= vpls.numel() // _abi.VPL_FLOATS or hi >= entries.numel() // 2:
                 raise _abi.SgiArgumentError(f"updates must index vpls and entries (got {lo}..{hi})")
         tri = _abi.Triangles(count, _dev(positions, "positions", 9 * count, optional=count == 0),
-                             _dev(colors, "colors", count, (_I32, "uint32"), optional=True))
+                             _dev(colors, "colors", count, (_I32, "uint32"), optional=True), cull)
         _abi.call(self.lib, "sgi_render_vpl_cubes", self.ctx, ctypes.byref(tri),
                   _dev(vpls, "vpls", 0, optional=n == 0), _dev(updates, "updates", n, (_I32,), optional=n == 0), n,
                   _dev(entries, "entries", 0, (_I32,), optional=n == 0), near, ctypes.byref(distance),
                   ctypes.byref(diffuse) if diffuse is not None else None, _stream(stream))
+
+    def render_csm(self, positions, csm: _abi.Csm, depth, slope=_abi.REFERENCE_CSM_SLOPE_FACTOR,
+                   cull=_abi.CULLING_CCW, stream=None) -> None:
+        """The sun cascades' depth render into depth [4][res][res] (csm's matrices and resolution)."""
+        count = 0 if positions is None else positions.numel() // 9
+        tri = _abi.Triangles(count, _dev(positions, "positions", 9 * count, optional=count == 0), 0, cull)
+        r = csm.resolution
+        _abi.call(self.lib, "sgi_render_csm", self.ctx, ctypes.byref(tri), ctypes.byref(csm), slope,
+                  _dev(depth, "depth", 4 * r * r), _stream(stream))
 
     def taa_history(self, taa_out, prev_screen, stream=None) -> None:
         """The TAA output as the next frame's history (RGBA16F rounding, clamp, RGB8 store, k / 255)."""

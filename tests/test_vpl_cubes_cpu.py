@@ -11,6 +11,9 @@ shadowVpl.fs:17-37) against what the geometry fixes:
   direction meets: producer and consumer agree on every texel's place;
 - GL's near-plane clipping (0.1): a wall closer than near along the face axis
   is not drawn; the clear (1, colour 0) stays where nothing is within far;
+- face culling as SetCullState sets it: under CCW (the meshes' default) a
+  triangle counter-clockwise as seen from the light is drawn and its reverse
+  is not, under CW the opposite, under NONE both;
 - LESS keeps the first of two coincident triangles;
 - on the Cornell box, the cubes agree with the generator's independent
   analytic ray caster wherever the near plane does not clip.
@@ -112,21 +115,39 @@ def test_gather_lookup_reads_the_texel_the_pass_wrote():
 
 def test_near_plane_clips_and_clear_stays():
     res = 8
-    # one wall of the +X face at x = 0.05 (inside the near plane) and one at x = 0.2
-    for x, drawn in ((0.05, False), (0.2, True)):
+    # a slab whose light-facing side lies at x = 0.05 (inside the near plane) or 0.2, 1 m thick
+    for x, cull in ((0.05, _abi.CULLING_NONE), (0.05, _abi.CULLING_CCW), (0.2, _abi.CULLING_CCW)):
         box = np.array([[x + 0.5, 0, 0, 1, 0.5, 50, 50, 0, 1, 1, 1, 1, 0, 1, 0, 0]], np.float32)
         pos, col = sc.box_triangles(box)
         z, c, cz, cc = _atlases(res, _abi.TEXEL_UNORM16)
-        orc.render_vpl_cubes(pos, col, _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc)
-        if drawn:
+        orc.render_vpl_cubes(pos, col, _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc, cull=cull)
+        if x > 0.1:
             assert np.all(z[0, 0] < 65535) and np.all(c[0, 0] == 0xFFFFFFFF)
-        else:
-            # the near face is clipped; the slab's far side at x + 1 is drawn instead
+        elif cull == _abi.CULLING_NONE:
+            # the near side is clipped; the slab's far side at x + 1 is drawn instead
             dx, dy, dz = [t.astype(np.float64) for t in _face_rays(0, res)]
             want = (x + 1) * np.sqrt(dx * dx + dy * dy + dz * dz) / FAR * 65535
             assert np.all(np.abs(z[0, 0] - want) <= 0.5 + 1e-3)
+        else:
+            # the far side is a back face: culled, the clear stays
+            assert np.all(z[0, 0] == 65535) and np.all(c[0, 0] == 0xFF000000)
         # the -X face sees nothing within far: the clear (distance 1, colour 0, alpha 255)
         assert np.all(z[0, 1] == 65535) and np.all(c[0, 1] == 0xFF000000)
+
+
+def test_face_culling_modes():
+    res = 4
+    ccw = np.array([[[3, -5, 9], [3, 5, -5], [3, -5, -5]]], np.float32)  # counter-clockwise seen from the origin
+    cw = ccw[:, ::-1].copy()
+    n = np.cross(ccw[0, 1] - ccw[0, 0], ccw[0, 2] - ccw[0, 0])
+    assert n[0] < 0  # its normal faces the light at the origin
+    for tri, cull, drawn in ((ccw, _abi.CULLING_CCW, True), (cw, _abi.CULLING_CCW, False),
+                             (ccw, _abi.CULLING_CW, False), (cw, _abi.CULLING_CW, True),
+                             (ccw, _abi.CULLING_NONE, True), (cw, _abi.CULLING_NONE, True)):
+        z, c, cz, cc = _atlases(res, _abi.TEXEL_F32)
+        orc.render_vpl_cubes(tri, np.array([0xFF112233], np.uint32), _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc,
+                             cull=cull)
+        assert bool((z[0, 0] < 1).any()) == drawn, (cull, drawn)
 
 
 def test_less_keeps_the_first_of_coincident_triangles():
@@ -135,7 +156,8 @@ def test_less_keeps_the_first_of_coincident_triangles():
     pos = np.concatenate([tri, tri])
     for order in ((0xFF0000AA, 0xFF0000BB), (0xFF0000BB, 0xFF0000AA)):
         z, c, cz, cc = _atlases(res, _abi.TEXEL_F32)
-        orc.render_vpl_cubes(pos, np.array(order, np.uint32), _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc)
+        orc.render_vpl_cubes(pos, np.array(order, np.uint32), _vpl((0, 0, 0)), [0], [[0, 0]], 0.1, cz, cc,
+                             cull=_abi.CULLING_NONE)
         covered = z[0, 0] < 1
         assert covered.sum() > 4 and np.all(c[0, 0][covered] == order[0])
 
