@@ -275,6 +275,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         return _libs[path]
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    if path != HOST_LIB_PATH:
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as the one this
+        # library links: loading torch first makes the process bind that one runtime for both, so
+        # the device memory and streams torch hands over belong to the runtime the kernels run on
+        import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     _bind(lib, SGI_SIGNATURES)
     _bind(lib, SCENE_SIGNATURES)

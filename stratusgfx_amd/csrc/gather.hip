@@ -159,6 +159,16 @@ struct Hit {
     float dist, nl, factor;
 };
 
+// The side and radius tests of vpl_pbr_gi.fs:163-170 (a VPL behind the surface
+// or beyond its radius is rejected), on the decisions of the restatement.
+// lmf = L - P, d2 = |lmf|^2, dist = the hardware sqrt of d2, nl = dot(N, lmf).
+__device__ __forceinline__ bool passesSideRadius(const Pixel& px, f3 lmf, float d2, float dist, float nl,
+                                                 float radius) {
+    if (__builtin_expect(signIsSafe(px.N, lmf, nl) && fabsf(dist - radius) > radius * 0x1p-18f, 1))
+        return !(nl < 0.0f || dist > radius);
+    return !rejectExact(px.N, lmf, d2, radius);
+}
+
 template <int FMT>
 __device__ __forceinline__ bool visibility(const Pixel& px, const FrameArgs& fr, const CubeArgs& cubes, const Vpl& L,
                                            bool allowReject, float& validSamples, Hit& h) {
@@ -166,14 +176,7 @@ __device__ __forceinline__ bool visibility(const Pixel& px, const FrameArgs& fr,
     const float d2 = dot(lmf, lmf);
     const float dist = fsqrt(d2);
     const float nl = dot(px.N, lmf);  // exact order: side test and shadow bias
-    if (allowReject) {
-        bool reject;
-        if (__builtin_expect(signIsSafe(px.N, lmf, nl) && fabsf(dist - L.radius) > L.radius * 0x1p-18f, 1))
-            reject = nl < 0.0f || dist > L.radius;
-        else
-            reject = rejectExact(px.N, lmf, d2, L.radius);
-        if (reject) return false;
-    }
+    if (allowReject && !passesSideRadius(px, lmf, d2, dist, nl, L.radius)) return false;
     const float distanceRatio = clampv((2.0f * dist) * L.invRadius, 0.0f, 1.0f);
     float shadowFactor = 0.0f;
     if (px.distToCamera < 500.0f) {  // calculateShadowValue1Sample, pbr.glsl:116-136
@@ -643,6 +646,128 @@ __global__ __launch_bounds__(256) void gatherSamplesKernel(GbufArgs g, FrameArgs
     writeOut(g, p, vplColor, validSamples, color, reservoir);
 }
 
+// The fixed-count sampler (rejectionLimit < 0) with the accepted draws
+// compacted across the wave.  Shading a draw costs several times its side /
+// radius test, and in the per-pixel loop a wave pays for the shadow test and
+// the BRDF of draw k whenever any of its 64 pixels accepts it (nearly always),
+// while a quarter to a half of the lanes do.  Here each batch of up to 8 draws
+// runs in three steps per wave:
+//   A. every lane hashes its pixel's draws and runs the side / radius test,
+//      recording the accepted slots (the seed sequence and the decisions are
+//      the per-pixel loop's);
+//   B. the accepted (pixel, slot) pairs, ordered pixel by pixel and draw by
+//      draw, are shaded 64 at a time, one pair per lane, with the owning
+//      pixel's shading inputs read from LDS: the shadow test, the reweighting
+//      and the lighting, exactly as visibility() and vplLighting() compute them;
+//   C. each pixel adds its own pairs' results in draw order with the per-pixel
+//      loop's operations, so the outputs are bit for bit that loop's.
+// Waves synchronise among their own lanes only (LDS ordered by a wavefront
+// fence), so the four waves of a block run their batches independently.
+constexpr int kDrawBatch = 8;
+
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void gatherSamplesCompactKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
+                                                                 int nTiles, int samples,
+                                                                 const PackedVpl* __restrict__ prep,
+                                                                 const int32_t* __restrict__ visible,
+                                                                 float* __restrict__ color,
+                                                                 float* __restrict__ reservoir) {
+    __shared__ Pixel pix[256];
+    __shared__ uint32_t pairs[4][64 * kDrawBatch];  // (owner lane << 16) | slot
+    __shared__ float4 lit[4][64];                    // a pair's lighting and its factor
+    __shared__ float rew[4][64];                     // a pair's reweighting factor
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const uint32_t t = xcdSwizzle(blockIdx.x, (uint32_t)nTiles);
+    const int tx = (int)t % tilesX, ty = (int)t / tilesX;
+    const int x = tx * 16 + (int)threadIdx.x % 16, y = ty * 16 + (int)threadIdx.x / 16;
+    const bool live = x < g.W && y < g.H;
+    const size_t p = live ? (size_t)y * g.W + x : 0;
+    Pixel px{};
+    if (live) px = setupPixel(g, fr, x, y, p);
+    pix[threadIdx.x] = px;
+    f3 vplColor = splat(0.0f);
+    float validSamples = 0.0f;
+    const int count = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
+    if (count > 0) {
+        const float seedX = (float)x + 0.5f, seedY = (float)y + 0.5f;
+        float seedZ = fr.time;
+        uint32_t* wp = pairs[wave];
+        for (int d0 = 0; d0 < samples; d0 += kDrawBatch) {
+            const int nd = min(kDrawBatch, samples - d0);
+            // A: the batch's slots hashed together and their positions fetched together (eight
+            // loads in flight), then the side / radius tests
+            uint32_t slots[kDrawBatch];
+            float4 pos[kDrawBatch];
+#pragma unroll
+            for (int k = 0; k < kDrawBatch; ++k) {
+                slots[k] = k < nd ? (uint32_t)drawSlotAll(seedX, seedY, seedZ, count) : 0u;
+                pos[k] = prep[slots[k]].posRadius;
+            }
+            uint32_t mask = 0;
+#pragma unroll
+            for (int k = 0; k < kDrawBatch; ++k) {
+                if (k < nd) {
+                    const f3 lmf = f3{pos[k].x, pos[k].y, pos[k].z} - px.P;
+                    const float d2 = dot(lmf, lmf);
+                    const float dist = fsqrt(d2);
+                    const float nl = dot(px.N, lmf);
+                    if (live && passesSideRadius(px, lmf, d2, dist, nl, pos[k].w)) mask |= 1u << k;
+                }
+            }
+
+            // the wave's exclusive prefix of accepted counts orders the pairs pixel by pixel
+            const int mine = __popc(mask);
+            int incl = mine;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            const int start = incl - mine;
+            const int total = __shfl(incl, 63, 64);
+#pragma unroll
+            for (int k = 0; k < kDrawBatch; ++k)
+                if (mask >> k & 1u) wp[start + __popc(mask & ((1u << k) - 1u))] = ((uint32_t)lane << 16) | slots[k];
+            waveSync();
+            // B + C: 64 pairs at a time, the next chunk's records fetched while this one is shaded
+            PackedVpl rec{};
+            if (lane < total) rec = prep[wp[lane] & 0xFFFFu];
+            for (int c0 = 0; c0 < total; c0 += 64) {
+                const int q = c0 + lane;
+                const PackedVpl here = rec;
+                if (q + 64 < total) rec = prep[wp[q + 64] & 0xFFFFu];
+                if (q < total) {
+                    const uint32_t e = wp[q];
+                    const Pixel& po = pix[wave * 64 + (int)(e >> 16)];
+                    const Vpl L = unpackVpl(here);
+                    float rw = 0.0f;
+                    Hit h;
+                    visibility<FMT>(po, fr, cubes, L, false, rw, h);
+                    const f3 l = vplLighting(po, L, h.dist, h.nl);
+                    lit[wave][lane] = make_float4(l.x, l.y, l.z, h.factor);
+                    rew[wave][lane] = rw;
+                }
+                waveSync();
+                // this lane's pairs within [c0, c0 + 64), in draw order
+                const int lo = max(start, c0), hi = min(start + mine, c0 + 64);
+                for (int r = lo; r < hi; ++r) {
+                    const float4 v = lit[wave][r - c0];
+                    vplColor = fmaF(f3{v.x, v.y, v.z}, v.w, vplColor);
+                    validSamples += rew[wave][r - c0];
+                }
+                waveSync();
+            }
+        }
+    }
+    if (live) writeOut(g, p, vplColor, validSamples, color, reservoir);
+}
+
 template <int FMT>
 hipError_t prepare(const CubeArgs& cubes, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,
                    PackedVpl* prep, hipStream_t s) {
@@ -669,6 +794,9 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
         if (half)
             hipLaunchKernelGGL((gatherSamplesKernel<FMT, true>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
                                rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
+        else if (rejectionLimit < 0)
+            hipLaunchKernelGGL((gatherSamplesCompactKernel<FMT>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
+                               rx * ry, samples, prep, visible, color, reservoir);
         else
             hipLaunchKernelGGL((gatherSamplesKernel<FMT, false>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
                                rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
