@@ -120,13 +120,56 @@ def samples_algorithmic_bytes(W, H, S, visible, texel_bytes):
     return int(W * H * (68 + S * texel_bytes) + visible * 48)
 
 
-def load_traffic(name):
+def load_profile(name):
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("traffic_bytes_per_launch")
+        return json.load(f)
+
+
+def load_traffic(name):
+    d = load_profile(name)
+    return None if d is None else d.get("traffic_bytes_per_launch")
+
+
+def valu_estimate(name, avg_launch_ms, draws_per_wave):
+    """VALU issue of a kernel from its SQ counter summary (profiles/*_valu.json): wave instructions
+    x 4 cycles / (SIMDs x clock), against the launch time measured here."""
+    d = load_profile(name)
+    if d is None:
+        return None
+    c = d["counters_per_launch"]
+    issue_ms = c["SQ_INSTS_VALU"] * d["cycles_per_wave_valu_instruction"] / (d["simds"] * d["clock_ghz"] * 1e9) * 1e3
+    return {"wave_instructions_per_launch": int(c["SQ_INSTS_VALU"]),
+            "per_wave": round(c["SQ_INSTS_VALU"] / c["SQ_WAVES"], 1),
+            "per_draw_and_wave": round(c["SQ_INSTS_VALU"] / c["SQ_WAVES"] / draws_per_wave, 1),
+            "issue_estimate_ms": round(issue_ms, 4), "issue_frac_estimate": round(issue_ms / avg_launch_ms, 3),
+            "source": d["source"]}
+
+
+def culling_roofline(per_plain, W, H, tiles, K, visible, num_vpls):
+    """Both culling stages of the plain frame against HBM: algorithmic bytes per launch
+    (tile culling: depth 4 + normal 12 B per pixel, the visible records 64 B each, the tile
+    records 4 + 4K B written; the global cull: 64 B per input record read, the compacted record
+    and its index, 68 B per visible VPL, written), HIP-event launch times, counter traffic."""
+    d = load_profile("cull_city1080p_traffic.json")
+    tr = (lambda k: d["kernels"][k]["traffic_bytes_per_launch"]) if d else (lambda k: None)
+    out = {}
+    for name, ms, alg, traffic in (
+            ("cull_global", per_plain[0], 64 * num_vpls + 68 * visible,
+             None if d is None else tr("cull_visibility") + tr("cull_compact")),
+            ("cull_tiles", per_plain[1], 16 * W * H + 64 * visible + tiles * (4 + 4 * K), tr("cull_tiles"))):
+        ach = alg / (ms * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": int(alg),
+                     "avg_launch_ms": round(ms, 4)}
+    out["cull_global"]["kernels"] = "cullVisibilityKernel + cullCompactKernel"
+    out["cull_tiles"]["kernels"] = "cullTilesKernel<16>"
+    out["note"] = ("neither stage is HBM-bound: the global cull is two launches of 64 and 16 waves over 4096 VPLs "
+                   "(latency), the tile cull one wave per tile against every visible VPL (VALU; "
+                   "profiles/valu_counters_gather_samples24_city1080p.md)")
+    return out
 
 
 def cpu_baseline(host, w, rows, visible_count):
@@ -695,6 +738,8 @@ def main():
                        "avg_launch_ms": round(per_plain[2], 4)}
     plain_res = None
     if per_plain is not None:
+        cull_roof = culling_roofline(per_plain, W, H, tiles_np.shape[0], w_plain.max_vpls_per_tile, visible_plain,
+                                     w_plain.num_vpls)
         plain_res = {
             "frames_per_s": round(args.steps / el_plain, 2), "ms_per_step": round(el_plain * 1e3 / args.steps, 4),
             "relmse": round(plain_rel, 6),
@@ -712,14 +757,15 @@ def main():
         alg = samples_algorithmic_bytes(W, H, S, learned_res["visible_vpls"], texel_bytes)
         g_ms = learned_res["kernels_ms"]["gather"]
         achieved = alg / (g_ms * 1e-3) / 1e9
-        roofline = {"kernel": f"gatherSamplesKernel<F16> ({S} draws per pixel)", "bound": "hbm",
+        roofline = {"kernel": f"gatherSamplesCompactKernel<F16> ({S} draws per pixel)", "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": load_traffic(f"gather_samples{S}_city1080p_traffic.json"),
                     "algorithmic_bytes": alg, "draws": W * H * S, "avg_launch_ms": g_ms,
-                    "note": ("VALU-issue-bound, not HBM-bound: 250 wave VALU instructions per draw, whose issue "
-                             "takes 56 % of the launch at 16 draws (profiles/valu_counters_gather_samples16_"
-                             "city1080p.md); the HBM fraction is low by construction")}
+                    "valu": valu_estimate(f"gather_samples{S}_city1080p_valu.json", g_ms, S),
+                    "note": ("not HBM-bound: the kernel is compute and memory-latency bound (per accepted draw a "
+                             "VPL record, then its cube texel, then the shadow test and the BRDF); the HBM "
+                             "fraction is low by construction, the VALU issue estimate is beside it")}
         conv_roof = conv_roofline(wts, H, W)
     elif plain_res is not None:
         value, el, form = plain_res["frames_per_s"], el_plain, "tile-list plain form only"
@@ -754,6 +800,7 @@ def main():
         "roofline": roofline,
         "roofline_network_conv": conv_roof,
         "roofline_plain_gather": gather_roof,
+        "roofline_culling": cull_roof if per_plain is not None else None,
         "batched_views": None,
         "vpl_colouring": None,
         "roofline_gbuffer_unpack": None,
