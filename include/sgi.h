@@ -374,6 +374,25 @@ int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
                   const sgi_cube_atlases* cubes, const int32_t* tiles,
                   const sgi_gather_opts* opts, float* color, float* reservoir, void* stream);
 
+/* No counterpart in the reference: sgi_gather_ex's fixed-count sampler
+ * (SGI_GATHER_SAMPLES, rejection_limit < 0, FP32 shading) for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It needs no context and calls no
+ * allocating routine: the per-frame frame block is passed in, and the packed
+ * visible-VPL records its kernels use live in a file-scope device array of
+ * 4480 x 64 B (280 KB per GPU, held until the process ends).  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads) race; each call rewrites the records it reads before reading
+ * them.  maxVisible: the capacity of the caller's visible list (the size of
+ * vpls and of visibleIndices minus one); the scratch holds 4480 records, so a
+ * capacity above 4480 is refused, and the kernels read at most
+ * min(visibleIndices[0], 4480) records as sgi_gather_ex does.  Other arguments
+ * and the outputs are those of sgi_gather_ex, bit for bit the same results. */
+int sgi_gather_samples_noalloc(const sgi_frame* frame, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
+                               const int32_t* visibleIndices, const sgi_atlas_entry* shadowIndices,
+                               const sgi_cube_atlases* cubes, int32_t maxVisible, int32_t samples, float* color,
+                               float* reservoir, void* stream);
+
 /* pbr.fs:82-145 with INFINITE_LIGHTING_ENABLED (StratusRendererBackend.cpp:2010-2038),
  * zero point lights: sun lighting (BRDF_Burley) times the 3x3 PCF cascade
  * shadow, plus emissive * emissionStrength, bounded to 65504.

@@ -172,6 +172,8 @@ SGI_SIGNATURES = {
     "sgi_taa": (c_int32, [_P, _P, _P, _P, _P, _P]),
     "sgi_tonemap": (c_int32, [_P, _P, c_float, _P, _P]),
     "sgi_taa_history": (c_int32, [_P, _P, _P, _P]),
+    "sgi_gather_samples_noalloc": (c_int32, [POINTER(Frame), POINTER(Gbuffer), _P, _P, _P, POINTER(CubeAtlases), _I,
+                                             _I, _P, _P, _P]),
     "sgi_render_csm": (c_int32, [_P, POINTER(Triangles), POINTER(Csm), c_float, _P, _P]),
     "sgi_render_vpl_cubes": (c_int32, [_P, POINTER(Triangles), _P, _P, _I, _P, c_float, POINTER(CubeAtlases),
                                        POINTER(CubeAtlases), _P]),

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -53,6 +54,36 @@ int checkGbuffer(const sgi_ctx* c, const sgi_gbuffer* g, const char* who, bool n
 }
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// ---- scratch of sgi_gather_samples_noalloc ---------------------------------
+// The packed visible-VPL records the sampled gather reads (sgi_launch::packedVplBytes() = 64 B
+// each), for up to 4480 visible VPLs: 280 KB of device memory per GPU, held until the process
+// ends.  In host code `scratch` names a host-side stand-in, so its device address is looked up
+// once per device (hipGetSymbolAddress, which allocates nothing) and only that address is used.
+constexpr int kNoAllocMaxVisible = SGI_MAX_TOTAL_VPLS_PER_FRAME;
+constexpr int kNoAllocRecordBytes = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocMaxVisible * kNoAllocRecordBytes];
+
+std::once_flag g_scratchOnce[kMaxDevices];
+void* g_scratchPtr[kMaxDevices];
+hipError_t g_scratchErr[kMaxDevices];
+
+int noAllocScratch(void** out) {
+    if (sgi_launch::packedVplBytes() != (size_t)kNoAllocRecordBytes)
+        return fail(SGI_E_STATE, "sgi_gather_samples_noalloc: packed record size differs from the scratch layout");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hipFail(e, "sgi_gather_samples_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(SGI_E_INVALID, "sgi_gather_samples_noalloc: device index >= 64");
+    std::call_once(g_scratchOnce[dev], [dev] {
+        g_scratchErr[dev] = hipGetSymbolAddress(&g_scratchPtr[dev], HIP_SYMBOL(scratch));
+    });
+    if (g_scratchErr[dev] != hipSuccess)
+        return hipFail(g_scratchErr[dev], "sgi_gather_samples_noalloc: hipGetSymbolAddress(scratch)");
+    *out = g_scratchPtr[dev];
+    return SGI_OK;
+}
 
 }  // namespace
 
@@ -832,6 +863,34 @@ int sgi_render_csm(sgi_ctx* ctx, const sgi_triangles* mesh, const sgi_csm* csm, 
     }
     const hipError_t e = sgi_launch::renderCsm(mesh->positions, mesh->count, keepSign, slopeFactor, csm->resolution,
                                                csm->cascadeProjViews, ctx->csmTris, depth, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, who);
+}
+
+int sgi_gather_samples_noalloc(const sgi_frame* frame, const sgi_gbuffer* g, const sgi_vpl* vpls,
+                                const int32_t* visibleIndices, const sgi_atlas_entry* shadowIndices,
+                                const sgi_cube_atlases* cubes, int32_t maxVisible, int32_t samples, float* color,
+                                float* reservoir, void* stream) {
+    const char* who = "sgi_gather_samples_noalloc";
+    if (!frame || !g || !vpls || !visibleIndices || !shadowIndices || !cubes || !color || !reservoir)
+        return fail(SGI_E_INVALID, std::string(who) + ": NULL argument");
+    if (maxVisible < 0 || maxVisible > kNoAllocMaxVisible)
+        return fail(SGI_E_INVALID, std::string(who) + ": maxVisible must be within 0..4480 (the scratch holds " +
+                                       std::to_string(kNoAllocMaxVisible) + " packed VPL records)");
+    if (samples < 1 || samples > 4096) return fail(SGI_E_INVALID, std::string(who) + ": samples must be within 1..4096");
+    if (g->width <= 0 || g->height <= 0 || g->width > 16384 || g->height > 16384 || !g->depth || !g->normal ||
+        !g->albedo || !g->baseReflectivity || !g->roughnessMetallicAmbient)
+        return fail(SGI_E_INVALID, std::string(who) + ": bad gbuffer");
+    if (!aligned16(reservoir)) return fail(SGI_E_INVALID, std::string(who) + ": reservoir must be 16-byte aligned");
+    if (!(frame->framesPerSecond > 0.0f)) return fail(SGI_E_INVALID, std::string(who) + ": framesPerSecond must be > 0");
+    sgi_dev::CubeArgs ca;
+    int r = cubeArgs(cubes, who, ca);
+    if (r != SGI_OK) return r;
+    void* prep = nullptr;
+    r = noAllocScratch(&prep);
+    if (r != SGI_OK) return r;
+    const hipError_t e = sgi_launch::gatherEx(sgi_dev::gbufArgs(*g), sgi_dev::frameArgs(*frame), ca, cubes->format,
+                                              SGI_GATHER_SAMPLES, samples, -1, 0, 0, vpls, visibleIndices,
+                                              shadowIndices, prep, color, reservoir, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, who);
 }
 
