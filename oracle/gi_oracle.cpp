@@ -29,6 +29,28 @@
 
 namespace {
 
+// The worker threads of the row-parallel passes (orc_set_threads; 1 by
+// default).  Every pixel of those passes depends only on the pass's inputs,
+// so the split of rows over threads changes no result.
+std::atomic<int> g_threads{1};
+
+template <class F>
+void forRows(int y0, int y1, F&& body) {
+    const int n = std::max(1, std::min(g_threads.load(), y1 - y0));
+    if (n == 1) {
+        for (int y = y0; y < y1; ++y) body(y);
+        return;
+    }
+    std::atomic<int> next{y0};
+    auto work = [&] {
+        for (int y = next.fetch_add(1); y < y1; y = next.fetch_add(1)) body(y);
+    };
+    std::vector<std::thread> th;
+    for (int i = 1; i < n; ++i) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
 template <class R>
 struct V3 {
     R x, y, z;
@@ -764,7 +786,7 @@ void denoise(const sgi_frame& fr, const GB& cur, const GB& prev, int numVisible,
     for (int pass = 0; pass < 4; ++pass) {
         const int multiplier = (1 << pass) - 1;
         const bool merge = pass == 0, fin = pass == 3;
-        for (int y = 0; y < H; ++y)
+        forRows(0, H, [&](int y) {
             for (int x = 0; x < W; ++x) {
                 const R u = R((float)((float)x + 0.5f) / (float)W), v = R((float)((float)y + 0.5f) / (float)H);
                 const V3<R> centerIllum = v3(R(illum.at(x, y, 0)), R(illum.at(x, y, 1)), R(illum.at(x, y, 2)));
@@ -870,6 +892,7 @@ void denoise(const sgi_frame& fr, const GB& cur, const GB& prev, int numVisible,
                 res2.at(x, y, 2) = shadowFactor.z;
                 res2.at(x, y, 3) = R(1);
             }
+        });
         std::swap(res.d, res2.d);
     }
 }
@@ -910,6 +933,14 @@ void colorVpls(const sgi_frame& fr, int count, const sgi_atlas_entry* entries, c
 }
 
 extern "C" {
+
+// Worker threads of the row-parallel passes (cull_tiles, gather, direct_light,
+// denoise); n <= 0 means every hardware thread.  Results do not depend on it.
+void orc_set_threads(int n) {
+    if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
+    g_threads.store(n);
+}
+int orc_get_threads() { return g_threads.load(); }
 
 // -- unit functions (for closed-form pins) --
 uint32_t orc_hash1(uint32_t x) { return hash1(x); }
@@ -1374,7 +1405,7 @@ void orc_cull_tiles(const sgi_frame* fr, const sgi_gbuffer* g, int ts, int K, co
                     const int32_t* visible, int remap, int t0, int t1, sgi_tile_stage1* stage1, int32_t* tiles) {
     const GB gb = wrap(*g);
     const int tx = (g->width + ts - 1) / ts;
-    for (int t = t0; t < t1; ++t) {
+    forRows(t0, t1, [&](int t) {
         float pos[3], nrm[3];
         tileStage1<float>(*fr, gb, ts, t % tx, t / tx, pos, nrm);
         if (stage1) {
@@ -1386,7 +1417,7 @@ void orc_cull_tiles(const sgi_frame* fr, const sgi_gbuffer* g, int ts, int K, co
             stage1[t].averageLocalNormal[3] = 1.0f;
         }
         tileStage2<float>(pos, nrm, vpls, visible, remap, K, tiles + (size_t)t * (1 + K));
-    }
+    });
 }
 
 // Gather over pixel rows [y0, y1).  mode: 0 random (reference), 1 tile lists, 2 every visible VPL,
@@ -1398,7 +1429,7 @@ void orc_gather(const sgi_frame* fr, const sgi_gbuffer* g, const sgi_vpl* vpls, 
                 void* reservoir) {
     const GB gb = wrap(*g);
     const int W = g->width, tx = (W + ts - 1) / ts;
-    for (int y = y0; y < y1; ++y)
+    forRows(y0, y1, [&](int y) {
         for (int x = 0; x < W; ++x) {
             const int32_t* list = nullptr;
             if (mode == TILES) list = tiles + (size_t)((y / ts) * tx + (x / ts)) * (1 + K);
@@ -1417,17 +1448,19 @@ void orc_gather(const sgi_frame* fr, const sgi_gbuffer* g, const sgi_vpl* vpls, 
                 for (int k = 0; k < 4; ++k) ((double*)reservoir)[p * 4 + k] = r[k];
             }
         }
+    });
 }
 
 void orc_direct_light(const sgi_frame* fr, const sgi_csm* csm, const sgi_gbuffer* g, int y0, int y1, float* lit) {
     const GB gb = wrap(*g);
     const int W = g->width;
-    for (int y = y0; y < y1; ++y)
+    forRows(y0, y1, [&](int y) {
         for (int x = 0; x < W; ++x) {
             float c[3];
             directPixel<float>(*fr, *csm, gb, x, y, c);
             for (int k = 0; k < 3; ++k) lit[((size_t)(y - y0) * W + x) * 3 + k] = c[k];
         }
+    });
 }
 
 void orc_denoise(const sgi_frame* fr, const sgi_gbuffer* cur, const sgi_gbuffer* prev, int numVisible,

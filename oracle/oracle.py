@@ -70,6 +70,8 @@ _SIGNATURES = {
                                 c_void_p]),
     "orc_denoise": (None, [POINTER(_abi.Frame), POINTER(_abi.Gbuffer), POINTER(_abi.Gbuffer), c_int]
                     + [c_void_p] * 9),
+    "orc_set_threads": (None, [c_int]),
+    "orc_get_threads": (c_int, []),
 }
 
 
@@ -91,6 +93,13 @@ def lib() -> ctypes.CDLL:
             build()
         _lib = load(LIB)
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """Worker threads of the row-parallel passes (cull_tiles, gather, direct_light, denoise);
+    n <= 0: every hardware thread.  Returns the count in effect.  Results do not depend on it."""
+    lib().orc_set_threads(int(n))
+    return int(lib().orc_get_threads())
 
 
 def _p(a) -> int:

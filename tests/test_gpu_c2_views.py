@@ -1,0 +1,165 @@
+"""The batched-views workload at its full size: 32 poses of the 2000-box city
+block (uniform in the block at 1.5-30 m, random yaw / pitch, the poses the
+bench draws), each at a resolution drawn from 1280x720, 1920x1080, 2560x1440
+and 3840x2160 (about 120 MP in all), sharing 1024 VPLs with 6x256x256 FP16
+cubes, rendered by sgi_views_* (one launch per stage over every view,
+replayed from a captured graph):
+
+- with 16x16 tiles and up to 64 VPLs per tile (the tuner's choice on this
+  batch) and with 8x8 tiles and 32: every view's tile lists, direct light,
+  gather, a-trous + combine and history equal the same view rendered alone by
+  sgi_render_frame, bit for bit, on the first replay and on a later one;
+- on one 16-row band per view (the band moves down the frame from view to
+  view, the last partial tile row of 1080-row views included): the view's
+  visible list and the band's tile lists bit-exact against the oracle, the
+  gather (colour and reservoir) and the direct light within 1e-4 rel-L2."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from gpu_util import assert_tiles_equal, require_gpu
+from stratusgfx_amd import _abi
+from stratusgfx_amd import scene as sc
+from stratusgfx_amd.gi import FrameBuffers, GiPass, Views, view_io
+from stratusgfx_amd.train_kpcn import random_view
+
+pytestmark = pytest.mark.gpu
+
+VIEWS = 32
+SIZES = [(1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+REL_L2 = 1e-4
+KEYS = ("tiles", "direct", "color", "reservoir", "combined", "gi", "reservoir_out", "history")
+
+
+@pytest.fixture(scope="module")
+def c2():
+    torch = require_gpu()
+    rng = np.random.default_rng(2)  # the bench's batch
+    scene = sc.Scene(sc.CITY_1080P)
+    assert scene.w.num_vpls == 1024 and scene.w.cube_res == 256 and scene.w.cube_format == _abi.TEXEL_F16
+    shared = sc.generate_device(scene)
+    inputs = []
+    for i in range(VIEWS):
+        W, H = SIZES[int(rng.integers(len(SIZES)))]
+        v = random_view(scene, rng, W, H)
+        v.csm_resolution = sc.CITY_1080P.csm_res
+        scene.set_view(v)
+        inputs.append(sc.generate_view_device(scene, shared))
+        print(f"[case] c2 view {i}: {W}x{H} eye ({v.eye[0]:.1f}, {v.eye[1]:.1f}, {v.eye[2]:.1f}) "
+              f"yaw {v.yaw_deg:.1f} pitch {v.pitch_deg:.1f}", flush=True)
+    torch.cuda.synchronize()
+    mp = sum(i.width * i.height for i in inputs) / 1e6
+    print(f"[c2] {VIEWS} views, {mp:.1f} MP, sizes {sorted({(i.width, i.height) for i in inputs})}", flush=True)
+    return dict(scene=scene, shared=shared, inputs=inputs, num_vpls=scene.w.num_vpls)
+
+
+def _bufs(torch, inp, tile, K):
+    H, W = inp.height, inp.width
+    f32 = dict(dtype=torch.float32, device="cuda")
+    nt = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    return dict(tiles=torch.full((nt, 1 + K), -5, dtype=torch.int32, device="cuda"),
+                direct=torch.zeros((H, W, 3), **f32), color=torch.zeros((H, W, 3), **f32),
+                reservoir=torch.zeros((H, W, 4), **f32), combined=torch.zeros((H, W, 3), **f32),
+                gi=torch.zeros((H, W, 3), **f32), reservoir_out=torch.zeros((H, W, 4), **f32),
+                history=torch.zeros((H, W), **f32))
+
+
+def _tiles_equal(torch, got, want):
+    """Tile lists equal: the counts, and the slots below each count (the rest are not written)."""
+    if not torch.equal(got[:, 0], want[:, 0]):
+        return False
+    K = want.shape[1] - 1
+    used = torch.arange(K, device=want.device)[None, :] < want[:, :1]
+    return bool(torch.equal(got[:, 1:][used], want[:, 1:][used]))
+
+
+def _single(torch, inp, shared, n, tile, K):
+    gi = GiPass(inp.width, inp.height, tile, K)
+    gi.set_frame(inp.frame)
+    gi.set_csm(inp.csm_struct())
+    fb = FrameBuffers.allocate(inp.width, inp.height, n, gi.num_tiles, K)
+    gi.render_frame(fb.frame_io(inp.gbuffer(), None, shared.vpls, n, shared.entries, shared.cubes(), True))
+    torch.cuda.synchronize()
+    return gi, fb
+
+
+@pytest.mark.parametrize("tile,K", [(16, 64), (8, 32)])
+def test_c2_batch_equals_single_views(c2, tile, K):
+    torch = require_gpu()
+    inputs, shared, n = c2["inputs"], c2["shared"], c2["num_vpls"]
+    bufs = [_bufs(torch, inp, tile, K) for inp in inputs]
+    ios = [view_io(inp, **b) for inp, b in zip(inputs, bufs)]
+    total = sum(inp.width * inp.height for inp in inputs)
+    vw = Views(len(inputs), total, tile, K)
+    s = torch.cuda.Stream()
+    vw.set(ios, shared.vpls, n, shared.entries, shared.cubes(), stream=s.cuda_stream)
+    vw.capture(s.cuda_stream)
+    bad = []
+    for rep in range(2):
+        if rep == 1:
+            for b in bufs:
+                for t in b.values():
+                    t.zero_()
+            torch.cuda.synchronize()
+            for _ in range(2):
+                vw.replay(s.cuda_stream)
+        else:
+            vw.replay(s.cuda_stream)
+        s.synchronize()
+        for i, (inp, b) in enumerate(zip(inputs, bufs)):
+            gi, fb = _single(torch, inp, shared, n, tile, K)
+            for k in KEYS:
+                got, want = b[k], getattr(fb, k)
+                if k == "tiles":
+                    if not _tiles_equal(torch, got, want):
+                        bad.append((rep, i, k))
+                elif not torch.equal(got, want):
+                    bad.append((rep, i, k))
+            if rep == 0 and i % 8 == 0:
+                print(f"[c2 {tile}x{tile} K={K}] view {i} {inp.width}x{inp.height}: "
+                      f"{int(fb.tiles[:, 0].sum().item())} tile entries, {int(fb.visible[0].item())} VPLs visible",
+                      flush=True)
+            del gi, fb
+    vw.close()
+    print(f"[c2 {tile}x{tile} K={K}] {len(inputs)} views x 2 replays compared, mismatches {bad[:6]}", flush=True)
+    assert not bad
+
+
+def test_c2_row_bands_against_the_oracle(c2):
+    torch = require_gpu()
+    inputs, shared, n = c2["inputs"], c2["shared"], c2["num_vpls"]
+    tile, K = 16, 64
+    hv = sc.to_host(dataclasses.replace(shared, depth=None, normal=None, albedo=None, refl=None, rma=None,
+                                        velocity=None, ids=None, csm_depth=None))
+    hvpls, hent = hv.vpls, hv.entries
+    worst, lit_bands = 0.0, 0
+    for i, inp in enumerate(inputs):
+        gi, fb = _single(torch, inp, shared, n, tile, K)
+        W, H = inp.width, inp.height
+        tx, ty = (W + tile - 1) // tile, (H + tile - 1) // tile
+        row = ty - 1 if i % 8 == 7 else (i * 7) % ty
+        y0, y1 = row * tile, min(row * tile + tile, H)
+        h = sc.to_host(dataclasses.replace(inp, atlases=[]))
+        hg, hcsm = h.gbuffer(), h.csm_struct()
+        upd, vis = orc.cull_global(h.frame, hcsm, hvpls)
+        c = int(vis[0])
+        assert np.array_equal(fb.visible.cpu().numpy()[:1 + c], vis), i
+        t0, t1 = row * tx, (row + 1) * tx
+        _, o_tiles = orc.cull_tiles(h.frame, hg, tile, K, upd, vis, t0=t0, t1=t1)
+        assert_tiles_equal(fb.tiles.cpu().numpy()[t0:t1], o_tiles[t0:t1])
+        ent = hent[vis[1:]]
+        oc, orr = orc.gather(h.frame, hg, upd, c, ent, hv.cubes(), tiles=o_tiles, tile_size=tile, K=K,
+                             mode=orc.GATHER_TILES, y0=y0, y1=y1)
+        e = max(orc.rel_l2(fb.color[y0:y1].cpu().numpy(), oc), orc.rel_l2(fb.reservoir[y0:y1].cpu().numpy(), orr))
+        od = orc.direct_light(h.frame, hcsm, hg, y0, y1)
+        ed = orc.rel_l2(fb.direct[y0:y1].cpu().numpy(), od)
+        worst = max(worst, e, ed)
+        print(f"[c2 oracle] view {i} {W}x{H} rows {y0}..{y1}: {c} visible, "
+              f"{int(o_tiles[t0:t1, 0].sum())} band tile entries, gather rel-L2 {e:.2e}, direct {ed:.2e}", flush=True)
+        assert e <= REL_L2 and ed <= REL_L2, i
+        lit_bands += int(float(np.abs(orr[..., :3]).sum()) > 0)
+        del gi, fb, h
+    print(f"[c2 oracle] worst rel-L2 {worst:.2e}; {lit_bands} of {len(inputs)} bands receive GI", flush=True)
+    assert lit_bands >= len(inputs) // 2

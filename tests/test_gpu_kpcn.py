@@ -20,6 +20,7 @@ import ctypes
 import numpy as np
 import pytest
 
+import kpcn_check
 import kpcn_oracle as ko
 from gpu_util import require_gpu
 from stratusgfx_amd import _abi, kpcn
@@ -278,43 +279,10 @@ def test_network_gradients_match_oracle(lib):
     rd = torch.from_numpy(np.ascontiguousarray(rad)).cuda()
     td = torch.from_numpy(np.ascontiguousarray(target)).cuda()
     fd = _bf(torch, feat)
-    out = tr.forward(fd, rd)
+    tr.forward(fd, rd)
     tr.loss_backward(td, err_weight=0.1)
     torch.cuda.synchronize()
-    rad64, t64 = rad.astype(np.float64), target.astype(np.float64)
-
-    def oracle_grads(net):
-        logits = net.forward(feat)
-        o, err = ko.reconstruct(logits, rad64)
-        _, dout, derr = ko.loss(o[0], err, t64, 0.1)
-        return logits, o, net.backward(ko.reconstruct_backward(logits, rad64, o, dout, derr))
-
-    net = ko.Net(params.astype(np.float64), bf16=True)
-    want_logits, want_o, want = oracle_grads(net)
-    for l in range(ko.LEVELS):
-        got_l = tr.logits[l].cpu().numpy().astype(np.float64)
-        e_log = np.linalg.norm(got_l - want_logits[l]) / np.linalg.norm(want_logits[l])
-        print(f"level {l} logits rel-L2 {e_log:.2e}")
-        assert e_log <= 1e-2, l
-    e_out = np.linalg.norm(out.cpu().numpy() - want_o[0]) / np.linalg.norm(want_o[0])
-    print(f"reconstruction rel-L2 {e_out:.2e}")
-    assert e_out <= 1e-3
-    # the size of BF16 rounding itself: the same oracle in FP64
-    _, _, want64 = oracle_grads(ko.Net(params.astype(np.float64), bf16=False))
-    got = tr.grad.cpu().numpy().astype(np.float64).reshape(ko.LAYERS, ko.LAYER_PARAMS)
-    bad = []
-    for l in range(ko.LAYERS):
-        a, b = got[l], want[l]
-        e = np.linalg.norm(a - b) / np.linalg.norm(b)
-        noise = np.linalg.norm(b - want64[l]) / np.linalg.norm(want64[l])
-        cos = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
-        print(f"layer {l:2d} grad rel-L2 {e:.2e} (BF16 vs FP64 oracle {noise:.2e}) cos {cos:.6f}")
-        if not (e <= 1.5 * max(noise, 2e-3) and cos >= 0.995):
-            bad.append(l)
-    assert not bad, f"layers {bad}"
-    cos_all = (got.ravel() @ want.ravel()) / (np.linalg.norm(got) * np.linalg.norm(want))
-    print(f"all layers cos {cos_all:.6f}")
-    assert cos_all >= 0.999
+    kpcn_check.compare(tr, params, feat, rad, target, 0.1)
 
 
 def test_denoise_learned_threshold_semantics(lib):

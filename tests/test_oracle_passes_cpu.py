@@ -346,3 +346,39 @@ def test_sampled_gather_converges_to_every_vpl_render(cornell):
     assert e1024 < 0.05 * e4
     # the reference's rule: capped rejections leave a bias that more samples do not remove
     assert err(1024, 5) > 5 * e1024
+
+
+def test_row_threads_change_no_result(cornell):
+    """The row-parallel passes (tile culling, every gather mode, direct light, denoiser) give the
+    same bits on 1 and on 7 worker threads (uneven row shares)."""
+    w, inp = cornell
+    g, cubes, csm = inp.gbuffer(), inp.cubes(), inp.csm_struct()
+    upd, vis = orc.cull_global(inp.frame, csm, inp.vpls)
+    ent = inp.entries[vis[1:]]
+    c = int(vis[0])
+
+    def run():
+        out = []
+        _, tiles = orc.cull_tiles(inp.frame, g, 16, 12, upd, vis)
+        out.append(tiles)
+        for mode in (orc.GATHER_RANDOM, orc.GATHER_TILES, orc.GATHER_CONVERGED, orc.GATHER_SAMPLES):
+            out += list(orc.gather(inp.frame, g, upd, c, ent, cubes, tiles=tiles, tile_size=16, K=12, mode=mode,
+                                   y0=3, y1=50))
+        lit = orc.direct_light(inp.frame, csm, g)
+        out.append(lit)
+        col, res = orc.gather(inp.frame, g, upd, c, ent, cubes, tiles=tiles, tile_size=16, K=12,
+                              mode=orc.GATHER_TILES)
+        out += list(orc.denoise(inp.frame, g, None, c, lit, col, res))
+        return out
+
+    try:
+        assert orc.set_threads(1) == 1
+        one = run()
+        assert orc.set_threads(7) == 7
+        many = run()
+    finally:
+        orc.set_threads(1)
+    assert len(one) == len(many)
+    for a, b in zip(one, many):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    assert np.abs(one[-3]).sum() > 0
