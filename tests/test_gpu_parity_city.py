@@ -1,10 +1,15 @@
 """Parity on the procedural city block at 1920x1080 (1024 VPLs, 6x256x256 FP16
-cubes, 16x16 tiles, K = 64) — the benchmark workload.
+cubes, 16x16 tiles, K = 64) and on the same block with 4096 VPLs (the
+benchmark's sampled gather at 24 draws per pixel).
 
-Lists are compared bit-exactly on every tile; the gather and direct light are
-compared with the oracle on bands of rows (the oracle takes minutes for the
-whole frame), the denoiser on the whole frame; size-independent invariants
+Lists are compared bit-exactly on every tile; the gathers (tile lists, the
+reference's random estimator, the fixed-count sampled gather), the direct
+light and the denoiser are compared with the oracle on the whole frame (the
+oracle runs its rows on up to 16 host threads); size-independent invariants
 are checked on the whole frame."""
+import dataclasses
+import os
+
 import numpy as np
 import pytest
 
@@ -16,7 +21,14 @@ from stratusgfx_amd.gi import GiPass
 pytestmark = pytest.mark.gpu
 
 REL_L2 = 1e-4
-BANDS = [(0, 8), (536, 552), (1064, 1080)]
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    print(f"[oracle] {orc.set_threads(THREADS)} threads", flush=True)
+    yield
+    orc.set_threads(1)
 
 
 @pytest.fixture(scope="module")
@@ -97,32 +109,29 @@ def test_tile_lists_invariants(case, run):
 
 
 @pytest.mark.parametrize("mode", ["tiles", "random"])
-def test_gather_bands(case, run, mode):
+def test_gather_whole_frame(case, run, mode):
     w, h = case.w, case.host
     col, res = (a.cpu().numpy() for a in run["out"][mode])
     tiles = run["o_tiles"] if mode == "tiles" else None
-    worst = 0.0
-    for y0, y1 in BANDS:
-        oc, orr = orc.gather(h.frame, case.hg, run["o_upd"], int(run["o_vis"][0]), run["o_ent"], case.hcubes,
-                             tiles=tiles, tile_size=w.tile_size, K=w.max_vpls_per_tile,
-                             mode=orc.GATHER_TILES if tiles is not None else orc.GATHER_RANDOM, y0=y0, y1=y1)
-        e = max(orc.rel_l2(col[y0:y1], oc), orc.rel_l2(res[y0:y1], orr))
-        worst = max(worst, e)
-        print(f"gather[{mode}] rows {y0}..{y1} rel-L2 {e:.3e}")
-    assert worst <= REL_L2
+    oc, orr = orc.gather(h.frame, case.hg, run["o_upd"], int(run["o_vis"][0]), run["o_ent"], case.hcubes,
+                         tiles=tiles, tile_size=w.tile_size, K=w.max_vpls_per_tile,
+                         mode=orc.GATHER_TILES if tiles is not None else orc.GATHER_RANDOM)
+    e = max(orc.rel_l2(col, oc), orc.rel_l2(res, orr))
+    exact = float((res.view(np.uint32) == orr.view(np.uint32)).all(-1).mean())
+    print(f"gather[{mode}] 1920x1080 rel-L2 {e:.3e}; {exact:.4f} of the pixels bit-exact")
+    assert e <= REL_L2
     # whole-frame invariants: colour = albedo + 1e-5, weights >= 1, finite
     alb = np.maximum(h.albedo[..., :3], 0.01) + np.float32(1e-5)
     assert np.array_equal(col, alb.astype(np.float32))
     assert np.all(res[..., 3] >= 1.0) and np.all(np.isfinite(res))
 
 
-def test_direct_light_bands(case, run):
+def test_direct_light_whole_frame(case, run):
     lit = run["lit"].cpu().numpy()
-    for y0, y1 in BANDS:
-        o = orc.direct_light(case.host.frame, case.hcsm, case.hg, y0, y1)
-        e = orc.rel_l2(lit[y0:y1], o)
-        print(f"direct rows {y0}..{y1} rel-L2 {e:.3e}")
-        assert e <= REL_L2
+    o = orc.direct_light(case.host.frame, case.hcsm, case.hg)
+    e = orc.rel_l2(lit, o)
+    print(f"direct 1920x1080 rel-L2 {e:.3e}; {float((lit == o).all(-1).mean()):.4f} of the pixels bit-exact")
+    assert e <= REL_L2
     assert np.all(np.isfinite(lit)) and lit.max() > 0
 
 
@@ -142,3 +151,37 @@ def test_denoise_whole_frame(case, run, gi):
     print(f"denoise 1080p rel-L2 combined {e:.3e} gi {e_gi:.3e}")
     assert e <= REL_L2 and e_gi <= REL_L2
     assert np.array_equal(hist.cpu().numpy(), histo)
+
+
+def test_sampled_gather_whole_frame_4096_vpls():
+    """The benchmark's gather: 4096 VPLs, 24 hashed draws per pixel over every visible VPL, failing
+    draws skipped (the compacting kernel), against the oracle on every pixel."""
+    from stratusgfx_amd import _abi
+    torch = require_gpu()
+    case = Case(dataclasses.replace(sc.CITY_1080P, name="city-1920x1080-4096vpl", num_vpls=4096))
+    w = case.w
+    n, H, W = w.num_vpls, w.height, w.width
+    gi = GiPass(W, H, w.tile_size, w.max_vpls_per_tile)
+    gi.set_frame(case.dev.frame)
+    gi.set_csm(case.dev.csm_struct())
+    upd = torch.zeros((n, 16), dtype=torch.float32, device="cuda")
+    vis = torch.zeros((1 + n,), dtype=torch.int32, device="cuda")
+    gi.cull_global(case.dev.vpls, n, upd, vis)
+    ent = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    gi.visible_atlas_entries(case.dev.entries, vis, min(n, _abi.MAX_TOTAL_VPLS_PER_FRAME), ent)
+    col = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    res = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    gi.gather_ex(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), None, col, res, mode=_abi.GATHER_SAMPLES,
+                 samples=24, rejection_limit=-1)
+    torch.cuda.synchronize()
+    o_upd, o_vis, o_ent = case.oracle_cull()
+    c = int(o_vis[0])
+    assert np.array_equal(vis.cpu().numpy()[:1 + c], o_vis)
+    oc, orr = orc.gather(case.host.frame, case.hg, o_upd, c, o_ent, case.hcubes, mode=orc.GATHER_SAMPLES,
+                         samples=24, rejection_limit=-1)
+    got = res.cpu().numpy()
+    e = max(orc.rel_l2(col.cpu().numpy(), oc), orc.rel_l2(got, orr))
+    exact = float((got.view(np.uint32) == orr.view(np.uint32)).all(-1).mean())
+    print(f"sampled gather 4096 VPLs ({c} visible), 24 draws, 1920x1080: rel-L2 {e:.3e}; "
+          f"{exact:.4f} of the pixels bit-exact", flush=True)
+    assert e <= REL_L2 and np.abs(orr[..., :3]).sum() > 0
