@@ -75,7 +75,9 @@ def parse():
     ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
                     help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
-    ap.add_argument("--cpu-rows", type=int, default=24, help="image rows of the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=120, help="image rows of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the CPU baseline (the oracle's rows split over them)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="instead of the benchmark, run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -172,18 +174,33 @@ def culling_roofline(per_plain, W, H, tiles, K, visible, num_vpls):
     return out
 
 
-def cpu_baseline(host, w, rows, visible_count):
-    """Times the oracle (single-threaded FP32 port) on a bounded sample of the frame."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(host, w, rows, visible_count, spp, rejection_limit, threads):
+    """Times the oracle (the FP32 restatement, rows split over `threads` host threads) on a bounded
+    sample of the frame: the a-trous form the learned form is compared with (global culling, the
+    sampled gather at `spp` draws per pixel over every visible VPL, direct light, the a-trous
+    denoiser), and beside it the reference's tile-list form (tile culling + tile-list gather)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
-    import ctypes
 
     import numpy as np
 
     import oracle as orc
 
+    used = orc.set_threads(threads)
     H, W = w.height, w.width
     y0 = max(0, H // 2 - rows // 2)
     y1 = min(H, y0 + rows)
+    scale = H / (y1 - y0)
     g, cubes, csm = host.gbuffer(), host.cubes(), host.csm_struct()
     t = {}
     t0 = time.perf_counter()
@@ -191,31 +208,45 @@ def cpu_baseline(host, w, rows, visible_count):
     t["cull_global"] = time.perf_counter() - t0
     ent = host.entries[vis[1:]]
     t0 = time.perf_counter()
-    _, tiles = orc.cull_tiles(host.frame, g, w.tile_size, w.max_vpls_per_tile, upd, vis)
-    t["cull_tiles"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
     lit = orc.direct_light(host.frame, csm, g, y0, y1)
-    t["direct_light"] = (time.perf_counter() - t0) * H / (y1 - y0)
+    t["direct_light"] = (time.perf_counter() - t0) * scale
     t0 = time.perf_counter()
-    col, res = orc.gather(host.frame, g, upd, int(vis[0]), ent, cubes, tiles=tiles, tile_size=w.tile_size,
-                          K=w.max_vpls_per_tile, mode=orc.GATHER_TILES, y0=y0, y1=y1)
-    t["gather"] = (time.perf_counter() - t0) * H / (y1 - y0)
+    col, res = orc.gather(host.frame, g, upd, int(vis[0]), ent, cubes, mode=orc.GATHER_SAMPLES, samples=spp,
+                          rejection_limit=rejection_limit, y0=y0, y1=y1)
+    t["gather"] = (time.perf_counter() - t0) * scale
     # the denoiser on the sampled rows as an image of their own (same per-pixel work)
-    crop = lambda a: np.ascontiguousarray(a[y0:y1])
+    crop = lambda a: np.ascontiguousarray(a[y0:y1])  # noqa: E731
     planes = [crop(host.depth), crop(host.normal), crop(host.albedo), crop(host.refl), crop(host.rma),
               crop(host.velocity), crop(host.ids)]
     from stratusgfx_amd import _abi
     cg = _abi.Gbuffer(W, y1 - y0, *[p.ctypes.data for p in planes], 0)
     t0 = time.perf_counter()
     orc.denoise(host.frame, cg, None, int(vis[0]), lit, col, res)
-    t["denoise"] = (time.perf_counter() - t0) * H / (y1 - y0)
+    t["denoise"] = (time.perf_counter() - t0) * scale
+    # the tile-list form's own stages
+    tl = {}
+    t0 = time.perf_counter()
+    _, tiles = orc.cull_tiles(host.frame, g, w.tile_size, w.max_vpls_per_tile, upd, vis)
+    tl["cull_tiles"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.gather(host.frame, g, upd, int(vis[0]), ent, cubes, tiles=tiles, tile_size=w.tile_size,
+               K=w.max_vpls_per_tile, mode=orc.GATHER_TILES, y0=y0, y1=y1)
+    tl["gather"] = (time.perf_counter() - t0) * scale
+    orc.set_threads(1)
     total = sum(t.values())
+    tl_total = t["cull_global"] + t["direct_light"] + t["denoise"] + sum(tl.values())
     return {
-        "value": 1.0 / total, "unit": "frames/s", "cores": 1, "kind": "port",
-        "sample": (f"FP32 oracle, 1 thread: cull_global ({visible_count} visible of {w.num_vpls}) and tile culling "
-                   f"on the whole frame; direct light, tile-list gather and denoiser on rows {y0}..{y1} of {H}, "
-                   f"scaled by {H / (y1 - y0):.1f} to a frame"),
+        "value": round(1.0 / total, 4), "unit": "frames/s", "cores": used, "kind": "port",
+        "cpu": cpu_model(), "host_threads_available": os.cpu_count(),
+        "sample": (f"FP32 oracle on {used} threads (rows split over threads): the a-trous form at {spp} draws "
+                   f"per pixel (the learned form's equal-relMSE point) - global culling ({visible_count} "
+                   f"visible of {w.num_vpls}) on the whole frame, direct light, sampled gather and denoiser on "
+                   f"rows {y0}..{y1} of {H}, scaled by {scale:.2f} to a frame"),
         "seconds_per_frame": {k: round(v, 3) for k, v in t.items()},
+        "tile_list_form": {"frames_per_s": round(1.0 / tl_total, 4),
+                           "seconds_per_frame": {k: round(v, 3) for k, v in tl.items()},
+                           "note": "cull_global, direct light and denoiser as above; tile culling on the whole "
+                                   "frame, the tile-list gather on the same rows"},
     }
 
 
@@ -453,8 +484,8 @@ def gbuffer_unpack_roofline(state, reps=20):
 
 def post_fx_roofline(state, reps=20):
     """HIP-event timing of the lit buffer's consumers at the frame size on the plain form's last frame:
-    sgi_taa (current frame, history, velocity -> FP32; 44 B per pixel) and sgi_tonemap (FP32 -> RGBA8;
-    16 B per pixel)."""
+    sgi_taa (current frame, history, velocity -> FP32; 44 B per pixel), sgi_tonemap (FP32 -> RGBA8;
+    16 B per pixel) and sgi_fxaa on the tonemapped target (RGBA8 in and out; 8 B per pixel)."""
     import torch
     W, H = state.w.width, state.w.height
     lit = state.fb.combined
@@ -463,8 +494,10 @@ def post_fx_roofline(state, reps=20):
     taa_out = torch.empty_like(lit)
     rgba = torch.empty((H, W), dtype=torch.int32, device="cuda")
     out = {}
+    aa = torch.empty_like(rgba)
     for name, run, per_px in (("taa", lambda: state.gi.taa(lit, hist, vel, taa_out), 44),
-                              ("tonemap", lambda: state.gi.tonemap(lit, rgba), 16)):
+                              ("tonemap", lambda: state.gi.tonemap(lit, rgba), 16),
+                              ("fxaa", lambda: state.gi.fxaa(rgba, aa), 8)):
         for _ in range(3):
             run()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -479,7 +512,68 @@ def post_fx_roofline(state, reps=20):
         out[name] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": W * H * per_px,
                      "avg_launch_ms": round(ms, 4)}
-    del hist, vel, taa_out, rgba
+    del hist, vel, taa_out, rgba, aa
+    return out
+
+
+def _event_ms(run, reps):
+    """Mean ms of `reps` back-to-back calls of run() between one pair of HIP events (3 calls first)."""
+    import torch
+    for _ in range(3):
+        run()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        run()
+    ev[1].record()
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+def shadow_passes(state, reps=10):
+    """HIP-event timing of the two shadow pre-passes on the block's meshes (12 triangles per box):
+    sgi_render_vpl_cubes for the reference's per-frame budget of 5 lights (maxShadowUpdatesPerFrame,
+    StratusRendererBackend.h:378) and for 256 lights, into scratch atlases (distance in the frame's
+    texel format and the RGBA8 colour layer, both written as the reference's pass does), and
+    sgi_render_csm (the frame's 4 cascades)."""
+    import numpy as np
+    import torch
+
+    from stratusgfx_amd import _abi
+    w = state.w
+    pos, col = state.scene.triangles()
+    pos_d = torch.from_numpy(np.ascontiguousarray(pos)).cuda()
+    col_d = torch.from_numpy(np.ascontiguousarray(col.view(np.int32))).cuda()
+    res = w.cube_res
+    out = {"triangles": int(pos.shape[0])}
+    vis = state.fb.visible[1:1 + int(state.fb.visible[0].item())]
+    for n in (5, 256):
+        layers = n
+        dt = torch.float32 if w.cube_format == _abi.TEXEL_F32 else torch.float16
+        z = torch.empty((layers, 6, res, res), dtype=dt, device="cuda")
+        c = torch.empty((layers, 6, res, res), dtype=torch.int32, device="cuda")
+        cz, cc = _abi.CubeAtlases(), _abi.CubeAtlases()
+        cz.resolution, cz.format, cz.layers_per_atlas, cz.num_atlases = res, w.cube_format, layers, 1
+        cc.resolution, cc.format, cc.layers_per_atlas, cc.num_atlases = res, _abi.TEXEL_RGBA8, layers, 1
+        cz.atlas[0], cc.atlas[0] = z.data_ptr(), c.data_ptr()
+        ups = vis[:n].contiguous()
+        ent = torch.zeros((state.n, 2), dtype=torch.int32, device="cuda")
+        ent[ups.long(), 1] = torch.arange(ups.numel(), dtype=torch.int32, device="cuda")
+        ms = _event_ms(lambda: state.gi.render_vpl_cubes(pos_d, col_d, state.dev.vpls, ups, ent, cz, cc),
+                       reps if n <= 5 else 2)
+        texels = ups.numel() * 6 * res * res
+        out[f"vpl_cubes_{n}"] = {"lights": int(ups.numel()), "avg_ms": round(ms, 4),
+                                 "texels_per_s": round(texels / (ms * 1e-3), 1),
+                                 "written_bytes": texels * (z.element_size() + 4)}
+        del z, c
+    csm = state.dev.csm_struct()
+    r = csm.resolution
+    depth = torch.empty((4, r, r), dtype=torch.float32, device="cuda")
+    ms = _event_ms(lambda: state.gi.render_csm(pos_d, csm, depth), reps)
+    out["csm"] = {"cascades": 4, "resolution": r, "avg_ms": round(ms, 4), "written_bytes": 4 * r * r * 4}
+    out["kernels"] = "cubeFaceListKernel + cubeRasterKernel; csmListKernel + csmRasterKernel"
+    del depth
     return out
 
 
@@ -553,7 +647,8 @@ def main():
     learned.cull()
     learned.gi.gather_ex(learned.g, learned.fb.vpl_updated, learned.fb.visible, learned.fb.visible_entries,
                          learned.cubes, None, conv_col, conv_res, mode=_abi.GATHER_ALL, accumulate_fp64=True)
-    floor_rel = rel_mse(conv_res[..., :3] / conv_res[..., 3:4].clamp_min(1e-5), target)
+    target_1024 = (conv_res[..., :3] / conv_res[..., 3:4].clamp_min(1e-5)).contiguous()
+    floor_rel = rel_mse(target_1024, target)
     del conv_col, conv_res
 
     # ---- plain forms ---------------------------------------------------------
@@ -613,6 +708,39 @@ def main():
             return out_gi
         return step
 
+    # (3) the reference's own live estimator (vpl_pbr_gi.fs:139-200: 1-6 hashed draws per pixel by
+    # roughness and distance, at most 5 rejections) over every visible VPL + the a-trous denoiser
+    def random_step(k, ev, evd, hist=True):
+        p, fb = plain, plain.fb
+        prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
+        out_gi, out_h = p.hist[(k + 1) % 2]
+        rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
+        rec(0)
+        direct_done = fork_direct(p, fb, evd)
+        p.cull()
+        rec(1)
+        p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color, fb.reservoir)
+        rec(2)
+        torch.cuda.current_stream().wait_event(direct_done)
+        p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
+                     out_gi, fb.reservoir_out, out_h)
+        rec(3)
+        return out_gi
+
+    random_res = None
+    if args.forms == "all":
+        g_r = random_step(0, None, None, hist=False)
+        torch.cuda.synchronize()
+        rm = rel_mse(g_r, target)
+        draws = float(plain.fb.reservoir[..., 3].double().mean())
+        el_r, per_r, direct_r = timed_loop(random_step, args.sweep_steps, 2, 3)
+        random_res = {"relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_r, 2),
+                      "mean_reservoir_weight": round(draws, 3),
+                      "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_r, direct_r),
+                      "note": "the reference's per-pixel estimator as shipped (gatherRandomKernel), 4096 VPLs"}
+        print(f"[bench] reference estimator: relMSE {rm:.5f}, {args.sweep_steps / el_r:.1f} frames/s",
+              file=sys.stderr, flush=True)
+
     sweep = []
     for S in (args.atrous_spp if args.forms == "all" else []):
         st = atrous_step(S)
@@ -632,6 +760,7 @@ def main():
     weights_path = kpcn.DEFAULT_WEIGHTS
     learned_res = learned_1024 = None
     learned_sweep = []
+    learned_1024_sweep = []
     el_learned = None
     wts = None
     if os.path.exists(weights_path) and args.forms in ("all", "learned"):
@@ -652,7 +781,7 @@ def main():
         n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
         fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
 
-        def learned_form(state, label, S, steps, warmup, precision):
+        def learned_form(state, label, S, steps, warmup, precision, own_target=None):
             net = nets[precision]
             ws = state.gi.kpcn_workspace()
 
@@ -678,6 +807,7 @@ def main():
             gi_l = learned_step(0, None, None, hist=False)
             torch.cuda.synchronize()
             rel = rel_mse(gi_l, target)
+            rel_own = rel_mse(gi_l, own_target) if own_target is not None else None
             fallback_tiles = int(fbt[0].item())
             fb = state.fb
             noisy = rel_mse(fb.reservoir[..., :3] / fb.reservoir[..., 3:4].clamp_min(1e-5), target)
@@ -701,6 +831,8 @@ def main():
                 "samples_per_pixel": S, "network_precision": precision,
                 "weights": os.path.relpath(fp8_path if precision == "fp8" else weights_path, HERE),
             }
+            if rel_own is not None:
+                res["relmse_vs_own_converged"] = round(rel_own, 6)
             print(f"[bench] learned ({label}, {precision}, {S} spp, {steps} frames): relMSE {rel:.5f}, "
                   f"{res['frames_per_s']} frames/s", file=sys.stderr, flush=True)
             return res, el
@@ -723,7 +855,18 @@ def main():
               file=sys.stderr, flush=True)
         learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup, P_op)
         learned_res.update(atrous_at_equal(sweep, learned_res))
-        learned_1024, _ = learned_form(learned, "1024 VPLs", S_op, args.sweep_steps, 2, P_op)
+        # the 1024-VPL scene: each sample count against the 4096-VPL a-trous sweep at equal relMSE to
+        # the 4096-VPL converged render (whose distance from the 1024-VPL converged render,
+        # converged_1024_vs_4096_relmse, floors the learned form's relMSE), and, each form against its
+        # own VPL set's converged render, at equal relMSE to that
+        for S in args.learned_spp:
+            r, _ = learned_form(learned, "1024 VPLs", S, args.sweep_steps, 2, P_op, own_target=target_1024)
+            r.update(atrous_at_equal(sweep, r))
+            own = atrous_at_equal(sweep, {"relmse": r["relmse_vs_own_converged"], "frames_per_s": r["frames_per_s"]})
+            r["each_vs_own_converged"] = {k: own[k] for k in ("atrous_at_equal_relmse",
+                                                              "speedup_vs_atrous_at_equal_relmse") if k in own}
+            learned_1024_sweep.append(r)
+        learned_1024 = max(learned_1024_sweep, key=lambda r: r.get("speedup_vs_atrous_at_equal_relmse", 0.0))
 
     gather_roof = None
     if use_tiles and per_plain is not None:
@@ -795,8 +938,10 @@ def main():
         "learned": learned_res,
         "learned_sweep": learned_sweep if learned_res is not None else [],
         "learned_1024_vpls": learned_1024,
+        "learned_1024_sweep": learned_1024_sweep,
         "atrous_sweep": sweep,
         "plain_tile_lists": plain_res,
+        "reference_estimator": random_res,
         "roofline": roofline,
         "roofline_network_conv": conv_roof,
         "roofline_plain_gather": gather_roof,
@@ -805,6 +950,7 @@ def main():
         "vpl_colouring": None,
         "roofline_gbuffer_unpack": None,
         "roofline_post_fx": None,
+        "shadow_passes": None,
         "cpu_baseline": None,
     }
     if plain_res is not None:
@@ -812,10 +958,15 @@ def main():
         result["vpl_colouring"] = vpl_colouring(plain)
         result["roofline_gbuffer_unpack"] = gbuffer_unpack_roofline(plain)
         result["roofline_post_fx"] = post_fx_roofline(plain)
+        print("[bench] timing the shadow pre-passes", file=sys.stderr, flush=True)
+        result["shadow_passes"] = shadow_passes(plain)
     if not args.no_cpu_baseline and plain_res is not None:
         print("[bench] timing the CPU oracle on a sample", file=sys.stderr, flush=True)
         host = sc.to_host(plain.dev)
-        result["cpu_baseline"] = cpu_baseline(host, w_plain, args.cpu_rows, visible_plain)
+        eq = (learned_res or {}).get("atrous_at_equal_relmse", {})
+        spp_cpu = int(round(eq["spp"])) if eq.get("spp") else max(args.atrous_spp)
+        result["cpu_baseline"] = cpu_baseline(host, w_plain, args.cpu_rows, visible_plain, spp_cpu,
+                                              args.rejection_limit, args.cpu_threads)
         del host
     if args.views > 0 and args.forms == "all":
         del plain, learned
