@@ -188,6 +188,19 @@ int sgi_kpcn_upsample_bilinear_backward(const float* dup, int32_t height, int32_
 int sgi_kpcn_loss(const float* out, const float* logits, const float* target, int32_t height, int32_t width,
                   float err_weight, float scale, float* dout, float* derr, float* loss_sums, void* stream);
 
+/* sgi_kpcn_loss with the error channel trained on the log ratio of the
+ * reconstruction's relative squared error to that of another output of the
+ * same input, alternative (FP32 [h][w][3]; the a-trous filter's output):
+ * the error term is err_weight * mean_p (err - log(r_p + 1e-4) + log(a_p +
+ * 1e-4))^2, a_p = mean_c (alternative - t)^2 / (t^2 + 0.01) (held constant),
+ * so that exp(err) estimates (r_p + 1e-4) / (a_p + 1e-4), the quantity whose
+ * tile mean sgi_denoise_learned compares with its threshold when the network
+ * was trained this way.  The reconstruction term and its gradient are
+ * sgi_kpcn_loss's. */
+int sgi_kpcn_loss_ratio(const float* out, const float* logits, const float* target, const float* alternative,
+                        int32_t height, int32_t width, float err_weight, float scale, float* dout, float* derr,
+                        float* loss_sums, void* stream);
+
 /* One Adam step over all SGI_KPCN_PARAMS parameters (FP32 params, grad, m, v)
  * followed by the BF16 copies: weights [layers][9][32][32], weights_t (the
  * transposed, tap-mirrored weights of the input gradient: weights_t[l][t][i][o]

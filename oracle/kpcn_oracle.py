@@ -222,15 +222,19 @@ def apply_backward(logits, radiance, coarse, dout, derr=None):
     return dl, dup
 
 
-def loss(out, err_logit, target, err_weight):
+def loss(out, err_logit, target, err_weight, alt=None):
     """(loss, dout [H][W][3], derr [H][W]) as sgi_kpcn_loss with scale 1:
     loss = mean_{p,c} (out - t)^2 / (t^2 + 0.01) + err_weight mean_p (err - log(r_p + 1e-4))^2
-    with r_p = mean_c (out - t)^2 / (t^2 + 0.01) held constant."""
+    with r_p = mean_c (out - t)^2 / (t^2 + 0.01) held constant; with alt (sgi_kpcn_loss_ratio) the
+    error target is log(r_p + 1e-4) - log(a_p + 1e-4), a_p the same mean for alt."""
     H, W = out.shape[:2]
     den = target * target + 0.01
     e = out - target
     r = e * e / den
     err_t = np.log(r.mean(-1) + 1e-4)
+    if alt is not None:
+        ea = alt - target
+        err_t = err_t - np.log((ea * ea / den).mean(-1) + 1e-4)
     de = err_logit - err_t
     N, P = 3.0 * H * W, float(H * W)
     L = r.sum() / N + err_weight * (de * de).sum() / P

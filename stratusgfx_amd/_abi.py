@@ -213,6 +213,7 @@ SGI_SIGNATURES = {
     "sgi_kpcn_features": (c_int32, [_P, POINTER(Gbuffer), _P, _P, _P, _P]),
     "sgi_kpcn_apply": (c_int32, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "sgi_kpcn_loss": (c_int32, [_P, _P, _P, _I, _I, c_float, c_float, _P, _P, _P, _P]),
+    "sgi_kpcn_loss_ratio": (c_int32, [_P, _P, _P, _P, _I, _I, c_float, c_float, _P, _P, _P, _P]),
     "sgi_kpcn_adam": (c_int32, [_P, _P, _P, _P, c_float, c_float, c_float, c_float, _I, _P, _P, _P, _P]),
     "sgi_kpcn_pack_weights": (c_int32, [_P, _P, _P, _P, _P]),
     "sgi_kpcn_workspace_bytes": (ctypes.c_size_t, [_I, _I]),

@@ -602,9 +602,21 @@ int sgi_kpcn_loss(const float* out, const float* logits, const float* target, in
     if (r != SGI_OK) return r;
     if (!out || !logits || !target || !dout || !derr || !loss_sums)
         return fail(SGI_E_INVALID, "sgi_kpcn_loss: NULL buffer");
-    const hipError_t e = sgi_kpcn::loss(out, logits, target, height, width, err_weight, scale, dout, derr, loss_sums,
-                                        (hipStream_t)stream);
+    const hipError_t e = sgi_kpcn::loss(out, logits, target, nullptr, height, width, err_weight, scale, dout, derr,
+                                        loss_sums, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_loss");
+}
+
+int sgi_kpcn_loss_ratio(const float* out, const float* logits, const float* target, const float* alternative,
+                        int32_t height, int32_t width, float err_weight, float scale, float* dout, float* derr,
+                        float* loss_sums, void* stream) {
+    int r = dims(height, width, "sgi_kpcn_loss_ratio");
+    if (r != SGI_OK) return r;
+    if (!out || !logits || !target || !alternative || !dout || !derr || !loss_sums)
+        return fail(SGI_E_INVALID, "sgi_kpcn_loss_ratio: NULL buffer");
+    const hipError_t e = sgi_kpcn::loss(out, logits, target, alternative, height, width, err_weight, scale, dout,
+                                        derr, loss_sums, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_loss_ratio");
 }
 
 int sgi_kpcn_adam(float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2, float eps,

@@ -1186,9 +1186,13 @@ __global__ __launch_bounds__(256) void featuresPyramidKernel(GbufArgs g, FrameAr
 // Loss on the reconstruction: the relative squared error of out and the error
 // head's regression onto log(r_p + 1e-4); gradients of out and of the error
 // logit, times scale.
+// alt: NULL (the error channel learns log(r_p + 1e-4)) or another output of the
+// same input (the error channel learns log(r_p + 1e-4) - log(a_p + 1e-4), a_p
+// its relative squared error, so exp(err) estimates the ratio of the two).
 __global__ __launch_bounds__(256) void lossKernel(const float* __restrict__ out, const float* __restrict__ logits,
-                                                  const float* __restrict__ target, int H, int W, float errWeight,
-                                                  float scale, float* __restrict__ dout, float* __restrict__ derr,
+                                                  const float* __restrict__ target, const float* __restrict__ alt,
+                                                  int H, int W, float errWeight, float scale,
+                                                  float* __restrict__ dout, float* __restrict__ derr,
                                                   float* __restrict__ lossSums) {
     __shared__ float red[2][256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1202,7 +1206,13 @@ __global__ __launch_bounds__(256) void lossKernel(const float* __restrict__ out,
         const float rx = e.x * e.x / den.x, ry = e.y * e.y / den.y, rz = e.z * e.z / den.z;
         l0 = rx + ry + rz;
         const float rp = l0 * (1.0f / 3.0f);
-        const float de = logits[(size_t)i * kC + SGI_KPCN_ERR_CHANNEL] - logf(rp + 1e-4f);
+        float errT = logf(rp + 1e-4f);
+        if (alt) {
+            const f3 ea = ld3(alt + 3 * (size_t)i) - t;
+            const float ra = (ea.x * ea.x / den.x + ea.y * ea.y / den.y + ea.z * ea.z / den.z) * (1.0f / 3.0f);
+            errT -= logf(ra + 1e-4f);
+        }
+        const float de = logits[(size_t)i * kC + SGI_KPCN_ERR_CHANNEL] - errT;
         l1 = de * de;
         const f3 d = f3{2.0f * e.x / den.x, 2.0f * e.y / den.y, 2.0f * e.z / den.z} * (invN * scale);
         dout[3 * (size_t)i + 0] = d.x;
@@ -1501,9 +1511,9 @@ hipError_t apply(const float* logits, const float* radiance, const float* coarse
     return hipGetLastError();
 }
 
-hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
-                float scale, float* dout, float* derr, float* lossSums, hipStream_t s) {
-    hipLaunchKernelGGL(lossKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, out, logits, target, H, W, errWeight,
+hipError_t loss(const float* out, const float* logits, const float* target, const float* alt, int H, int W,
+                float errWeight, float scale, float* dout, float* derr, float* lossSums, hipStream_t s) {
+    hipLaunchKernelGGL(lossKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, out, logits, target, alt, H, W, errWeight,
                        scale, dout, derr, lossSums);
     return hipGetLastError();
 }

@@ -169,8 +169,8 @@ hipError_t features16Pyramid(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArg
                              uint16_t* feat16, float* radiance, float* pyr, hipStream_t s);
 hipError_t apply(const float* logits, const float* radiance, const float* coarse, int H, int W, float* out,
                  float* err, hipStream_t s);
-hipError_t loss(const float* out, const float* logits, const float* target, int H, int W, float errWeight,
-                float scale, float* dout, float* derr, float* lossSums, hipStream_t s);
+hipError_t loss(const float* out, const float* logits, const float* target, const float* alt, int H, int W,
+                float errWeight, float scale, float* dout, float* derr, float* lossSums, hipStream_t s);
 hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
                 int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
 hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);

@@ -196,12 +196,19 @@ class Trainer:
             _abi.call(self.lib, "sgi_kpcn_conv3x3", _p(dy), _p(y), _p(self.wts.layer_wt(l)), 0, 0, h, w,
                       _p(dx_out), 0, stream)
 
-    def loss_backward(self, target, err_weight=0.05, scale=1.0, stream=0):
-        """Loss of the last forward against target and the gradient of every layer (added to grad)."""
+    def loss_backward(self, target, err_weight=0.05, scale=1.0, stream=0, alternative=None):
+        """Loss of the last forward against target and the gradient of every layer (added to grad).
+        alternative: another output of the same input (the a-trous filter's); the error channel then
+        learns the log ratio of the two outputs' relative squared errors (sgi_kpcn_loss_ratio)."""
         a, g = self.a, self.g
-        _abi.call(self.lib, "sgi_kpcn_loss", _p(self.o[0]), _p(self.logits[0]), _p(target), self.H, self.W,
-                  ctypes.c_float(err_weight), ctypes.c_float(scale), _p(self.do[0]), _p(self.derr),
-                  _p(self.loss_sums), stream)
+        if alternative is None:
+            _abi.call(self.lib, "sgi_kpcn_loss", _p(self.o[0]), _p(self.logits[0]), _p(target), self.H, self.W,
+                      ctypes.c_float(err_weight), ctypes.c_float(scale), _p(self.do[0]), _p(self.derr),
+                      _p(self.loss_sums), stream)
+        else:
+            _abi.call(self.lib, "sgi_kpcn_loss_ratio", _p(self.o[0]), _p(self.logits[0]), _p(target),
+                      _p(alternative), self.H, self.W, ctypes.c_float(err_weight), ctypes.c_float(scale),
+                      _p(self.do[0]), _p(self.derr), _p(self.loss_sums), stream)
         # the reconstruction, fine to coarse: each level's logits and the coarser output's gradient
         for l in range(LEVELS):
             h, w = self.sizes[l]

@@ -10,12 +10,22 @@ count drawn per view from --spp, target = the converged gather over every
 visible VPL with FP64 sums (sgi_gather_ex ALL), both normalised as the
 denoiser's pass 0 does.  Mixing sample counts lets one network serve every
 operating point: it reads the noise level from the input itself.
-Training: Adam on the relative-MSE loss plus the error head's regression
-(sgi_kpcn_loss), gradients from the project's MFMA kernels.  Validation on
-held-out seeds, at every sample count of --spp, reports the relative MSE of
-the noisy input, the a-trous filter (sgi_denoise), the network alone, and
-the network with its a-trous fallback, and picks the fallback threshold.
+Training: Adam on the relative-MSE loss plus the error head's regression,
+gradients from the project's MFMA kernels.  The error channel learns, per
+pixel, the log ratio of the network's relative squared error to that of the
+a-trous filter on the same input (sgi_kpcn_loss_ratio; --err-target error:
+the network's own log error, sgi_kpcn_loss), so that a tile's mean of
+exp(err) predicts whether the a-trous filter would do better there.
+--train error-head updates only that channel's taps and bias, which leaves
+the network's output unchanged: it calibrates the fallback of a trained
+network.  Validation on held-out seeds, at every sample count of --spp,
+reports the relative MSE of the noisy input, the a-trous filter
+(sgi_denoise), the network alone, and the network with its a-trous fallback
+at each threshold, and picks the threshold with the lowest relative MSE.
 Writes the FP32 parameters (.npy) and a JSON summary beside them.
+
+    python -m stratusgfx_amd.train_kpcn --init stratusgfx_amd/weights/kpcn_city.npy \
+        --train error-head --err-weight 1 --steps 3000 --scenes 256 --views 2
 """
 from __future__ import annotations
 
@@ -69,12 +79,13 @@ def random_view(scene: sc.Scene, rng, width: int, height: int | None = None) -> 
 class Sample:
     """One view's network inputs and targets on the device."""
 
-    def __init__(self, feat, rad, target, extra=None, spp=0):
+    def __init__(self, feat, rad, target, extra=None, spp=0, alt=None):
         self.feat, self.rad, self.target, self.extra, self.spp = feat, rad, target, extra, spp
+        self.alt = alt  # the a-trous filter's output on the same input (the error head's reference)
 
 
 def make_samples(seed: int, views: int, width: int, height: int, num_vpls: int, spp_choices, rng, keep_frame=False,
-                 rejection_limit=-1, cube_res=256, default_view=True):
+                 rejection_limit=-1, cube_res=256, default_view=True, with_alt=False):
     """`views` views of one seeded block (the block's default pose first when
     default_view, then random poses); each view's gather takes a sample count
     drawn from spp_choices (every count when keep_frame, for validation)."""
@@ -97,13 +108,13 @@ def make_samples(seed: int, views: int, width: int, height: int, num_vpls: int, 
         scene.set_view(v)
         inp = sc.generate_device(scene)
         for samples_pp in counts:
-            out.append(_sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame))
+            out.append(_sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame, with_alt))
         del inp
     torch.cuda.synchronize()
     return out
 
 
-def _sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame):
+def _sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_frame, with_alt=False):
     """Network inputs and the converged target of one view at one sample count."""
     import torch
 
@@ -127,13 +138,21 @@ def _sample(inp, width, height, num_vpls, samples_pp, rejection_limit, keep_fram
     rad = torch.empty((height, width, 3), **f32)
     gi.kpcn_features(g, res, feat, rad)
     target = resT[..., :3] / resT[..., 3:4].clamp_min(1e-5)
+    alt = None
+    if with_alt:  # the a-trous filter on the same input (its GI output does not read the direct light)
+        outs = [torch.zeros((height, width, c), **f32) for c in (3, 3, 4)]
+        hist = torch.zeros((height, width), **f32)
+        gi.denoise(g, None, vis, torch.zeros((height, width, 3), **f32), col, res, None, None, outs[0], outs[1],
+                   outs[2], hist)
+        alt = outs[1]
+        del outs, hist
     extra = None
     if keep_frame:
         lit = torch.zeros((height, width, 3), **f32)
         gi.direct_light(g, lit)
         # the G-buffer and the gather's outputs stay; the cubes are not needed again
         extra = dict(gi=gi, inp=dataclasses.replace(inp, atlases=[]), vis=vis, col=col, res=res, lit=lit)
-    return Sample(feat, rad, target.contiguous(), extra, samples_pp)
+    return Sample(feat, rad, target.contiguous(), extra, samples_pp, alt)
 
 
 def evaluate(wts: kpcn.Weights, val, thresholds):
@@ -178,6 +197,19 @@ def evaluate(wts: kpcn.Weights, val, thresholds):
     return res
 
 
+def pick_threshold(val: dict, thresholds, tol=1e-3):
+    """The fallback threshold from a validation result (evaluate()): the smallest threshold whose
+    relMSE is within `tol` (relative) of the best of the network alone and every threshold, so that
+    the fallback acts as early as held-out data allows without costing more than `tol`; +inf when
+    no threshold qualifies.  Returns (threshold, its validation relMSE)."""
+    rel = {float("inf"): val["network"]}
+    for t in thresholds:
+        rel[t] = val["fallback"][f"{t:g}"]["relmse"]
+    best = min(rel.values())
+    t = min(k for k, v in rel.items() if v <= best * (1.0 + tol))
+    return t, rel[t]
+
+
 def main(argv=None) -> None:
     import torch
 
@@ -203,12 +235,19 @@ def main(argv=None) -> None:
     ap.add_argument("--first-scene", type=int, default=1000, help="seed of the first training block")
     ap.add_argument("--out", default=kpcn.DEFAULT_WEIGHTS)
     ap.add_argument("--log", default=None)
+    ap.add_argument("--err-target", choices=["error", "ratio"], default="ratio",
+                    help="the error channel's target: the log relative squared error of the output (error), or "
+                         "its log ratio to the a-trous filter's on the same input (ratio, sgi_kpcn_loss_ratio)")
+    ap.add_argument("--train", choices=["all", "error-head"], default="all",
+                    help="update every parameter, or only the error channel of the level-0 head (the rest of "
+                         "the network, and so its output, unchanged: calibrating a trained network's fallback)")
     a = ap.parse_args(argv)
     rng = np.random.default_rng(a.seed)
     t0 = time.time()
     train = []
     for i in range(a.scenes):
-        train += make_samples(a.first_scene + i, a.views, a.width, a.height, a.vpls, a.spp, rng, cube_res=a.cube_res)
+        train += make_samples(a.first_scene + i, a.views, a.width, a.height, a.vpls, a.spp, rng, cube_res=a.cube_res,
+                              with_alt=a.err_target == "ratio")
         if i % 8 == 7:
             print(f"[data] {len(train)} training views, {time.time() - t0:.0f} s", flush=True)
     val = []
@@ -221,6 +260,15 @@ def main(argv=None) -> None:
 
     wts = kpcn.Weights(kpcn.load_params(a.init) if a.init else kpcn.init_params(a.seed))
     tr = kpcn.Trainer(wts, a.height, a.width)
+    mask = None
+    if a.train == "error-head":  # the error channel's taps and bias in the level-0 head
+        m = np.zeros(kpcn.PARAMS, np.float32)
+        base_l = kpcn.HEADS[0] * kpcn.LAYER_PARAMS
+        for t in range(9):
+            o = base_l + t * kpcn.C * kpcn.C + _abi.KPCN_ERR_CHANNEL * kpcn.C
+            m[o:o + kpcn.C] = 1.0
+        m[base_l + kpcn.LAYER_WEIGHTS + _abi.KPCN_ERR_CHANNEL] = 1.0
+        mask = torch.from_numpy(m).cuda()
     base = evaluate(wts, val, [])
     print(f"[val] before training {json.dumps(base)}", flush=True)
     hist = []
@@ -231,7 +279,9 @@ def main(argv=None) -> None:
         for j in rng.choice(len(train), a.batch, replace=False):
             s = train[j]
             tr.forward(s.feat, s.rad)
-            tr.loss_backward(s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch)
+            tr.loss_backward(s.target, err_weight=a.err_weight, scale=a.loss_scale / a.batch, alternative=s.alt)
+        if mask is not None:
+            tr.grad.mul_(mask)
         tr.step(lr)
         if step % 50 == 0 or step == 1:
             ls = tr.loss_sums.cpu().numpy() / (a.batch * a.width * a.height)
@@ -240,21 +290,18 @@ def main(argv=None) -> None:
                   f"{time.time() - t1:.0f} s", flush=True)
     torch.cuda.synchronize()
     train_s = time.time() - t1
-    # thresholds on the predicted mean relative squared error of a tile
-    thresholds = [0.01, 0.02, 0.05, 0.1, 0.2, 0.5]
+    # thresholds on a tile's mean of exp(err): the predicted relative squared error (error), or the
+    # predicted ratio of the network's to the a-trous filter's (ratio)
+    thresholds = ([0.5, 0.7, 0.85, 1.0, 1.2, 1.5, 2.0, 3.0] if a.err_target == "ratio"
+                  else [0.01, 0.02, 0.05, 0.1, 0.2, 0.5])
     after = evaluate(wts, val, thresholds)
     print(f"[val] after training {json.dumps(after)}", flush=True)
-    # the fallback threshold: the one with the lowest validation relMSE (inf = none)
-    best_t, best = float("inf"), after["network"]
-    for t in thresholds:
-        v = after["fallback"][f"{t:g}"]["relmse"]
-        if v < best:
-            best_t, best = t, v
+    best_t, best = pick_threshold(after, thresholds)
     kpcn.save_params(wts.params.cpu().numpy(), a.out)
     summary = {"train_views": len(train), "val_views": len(val), "width": a.width, "height": a.height, "cube_res": a.cube_res, "vpls": a.vpls, "spp": a.spp,
                "steps": a.steps, "batch": a.batch, "lr": a.lr, "err_weight": a.err_weight,
                "init": os.path.basename(a.init) if a.init else None, "first_scene": a.first_scene,
-               "loss_scale": a.loss_scale,
+               "loss_scale": a.loss_scale, "err_target": a.err_target, "trained": a.train,
                "train_seconds": train_s, "data_seconds": t1 - t0, "median_target": tmed,
                "before": base, "after": after, "threshold": best_t, "history_points": len(hist)}
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:

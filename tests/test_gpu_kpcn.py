@@ -16,6 +16,7 @@ through the oracle on these inputs, moves the logits by 6e-4 .. 2.6e-3 and
 flips up to a quarter of the coarsest level's BF16 activations, as large as
 the GPU's own distance from the oracle."""
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -209,21 +210,29 @@ def test_filter_apply_levels_and_backward(lib):
             assert np.abs(dc.cpu().numpy() - dco).max() <= 1e-5 * np.abs(dco).max()
 
 
-def test_loss(lib):
+@pytest.mark.parametrize("ratio", [False, True])
+def test_loss(lib, ratio):
+    """sgi_kpcn_loss, and sgi_kpcn_loss_ratio (the error channel against the log ratio of the output's
+    relative squared error to an alternative output's)."""
     torch = require_gpu()
     rng = np.random.default_rng(11)
     out = rng.uniform(0, 2, (H, W, 3)).astype(np.float32)
     logits = rng.normal(0, 2, (H, W, 32)).astype(np.float32)
     target = rng.uniform(0.1, 1.5, (H, W, 3)).astype(np.float32)
-    od, ld, td = (_dev(torch, a) for a in (out, logits, target))
+    alt = (target + rng.normal(0, 0.3, (H, W, 3))).astype(np.float32)
+    od, ld, td, ad = (_dev(torch, a) for a in (out, logits, target, alt))
     dout = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
     derr = torch.empty((H, W), dtype=torch.float32, device="cuda")
     sums = torch.zeros(2, dtype=torch.float32, device="cuda")
-    _abi.call(lib, "sgi_kpcn_loss", od.data_ptr(), ld.data_ptr(), td.data_ptr(), H, W, ctypes.c_float(0.25),
-              ctypes.c_float(0.5), dout.data_ptr(), derr.data_ptr(), sums.data_ptr(), 0)
+    if ratio:
+        _abi.call(lib, "sgi_kpcn_loss_ratio", od.data_ptr(), ld.data_ptr(), td.data_ptr(), ad.data_ptr(), H, W,
+                  ctypes.c_float(0.25), ctypes.c_float(0.5), dout.data_ptr(), derr.data_ptr(), sums.data_ptr(), 0)
+    else:
+        _abi.call(lib, "sgi_kpcn_loss", od.data_ptr(), ld.data_ptr(), td.data_ptr(), H, W, ctypes.c_float(0.25),
+                  ctypes.c_float(0.5), dout.data_ptr(), derr.data_ptr(), sums.data_ptr(), 0)
     torch.cuda.synchronize()
     L, do, de = ko.loss(out.astype(np.float64), logits[..., ko.ERR].astype(np.float64), target.astype(np.float64),
-                        0.25)
+                        0.25, alt.astype(np.float64) if ratio else None)
     s = sums.cpu().numpy().astype(np.float64)
     P = H * W
     assert abs(s[0] / (3 * P) + 0.25 * s[1] / P - L) <= 1e-4 * abs(L)
@@ -288,7 +297,8 @@ def test_network_gradients_match_oracle(lib):
 def test_denoise_learned_threshold_semantics(lib):
     """threshold <= 0: every tile falls back, so the outputs are sgi_denoise's bit for bit;
     threshold = inf: no tile falls back and gi is the network's reconstruction, bit for bit the one
-    the training path's separate kernels compute."""
+    the training path's separate kernels compute; a threshold that flags some tiles: the flagged
+    tiles' outputs are sgi_denoise's, the others the network's, bit for bit."""
     from gpu_util import Case
     from stratusgfx_amd import scene as sc
     from stratusgfx_amd.gi import GiPass
@@ -344,3 +354,26 @@ def test_denoise_learned_threshold_semantics(lib):
     assert torch.equal(outs[1], out)
     assert torch.all(hist == 1.0)
     assert torch.equal(outs[0], lit + col * out)
+
+    # a threshold between: each flagged tile takes sgi_denoise's outputs bit for bit (the a-trous
+    # passes run over the flagged tiles and a one-tile margin, wider than their 14-pixel reach),
+    # every other tile the network's
+    net_outs, net_hist = outs, hist
+    lo, hi = -30.0, 30.0  # log thresholds: every tile falls back at lo, none at hi; bisect
+    for _ in range(60):
+        thr = math.exp(0.5 * (lo + hi))
+        outs, hist, fb = run_learned(thr)
+        torch.cuda.synchronize()
+        c = int(fb[0])
+        if 0 < c < n_tiles:
+            break
+        lo, hi = (math.log(thr), hi) if c == n_tiles else (lo, math.log(thr))
+    count = int(fb[0])
+    assert 0 < count < n_tiles, "no threshold splits the tiles"
+    flags = fb[1:].view((Hh + 15) // 16, (Ww + 15) // 16).bool()
+    assert int(flags.sum()) == count
+    mask = flags.repeat_interleave(16, 0).repeat_interleave(16, 1)[:Hh, :Ww]
+    print(f"threshold {thr:.3g}: {count} of {n_tiles} tiles fall back", flush=True)
+    for a, at, nt in zip(outs + [hist], ref + [rh], net_outs + [net_hist]):
+        m = mask if a.dim() == 2 else mask[..., None]
+        assert torch.equal(a, torch.where(m, at, nt))

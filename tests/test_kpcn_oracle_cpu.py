@@ -187,3 +187,24 @@ def test_level_sizes(hw):
     for s in sizes:
         assert x.shape[:2] == s
         x = ko.pool(x)
+
+
+def test_ratio_error_target():
+    """The error channel's target with an alternative output (sgi_kpcn_loss_ratio): log(r + 1e-4) -
+    log(a + 1e-4) from the two relative squared errors; the reconstruction's gradient is unchanged."""
+    rng = np.random.default_rng(3)
+    H, W = 3, 4
+    t = rng.uniform(0.1, 2.0, (H, W, 3))
+    s = np.sqrt(t * t + 0.01)
+    out, alt = t + 0.1 * s, t - 0.2 * s  # relative squared errors 0.01 and 0.04 in every channel
+    err = rng.normal(0, 1, (H, W))
+    L0, do0, de0 = ko.loss(out, err, t, 0.5)
+    L1, do1, de1 = ko.loss(out, err, t, 0.5, alt)
+    want_t = np.log(0.01 + 1e-4) - np.log(0.04 + 1e-4)
+    assert np.allclose(de1, 2 * 0.5 * (err - want_t) / (H * W), rtol=1e-12, atol=1e-15)
+    assert np.allclose(de0, 2 * 0.5 * (err - np.log(0.01 + 1e-4)) / (H * W), rtol=1e-12, atol=1e-15)
+    assert np.array_equal(do0, do1)
+    assert abs(L1 - (0.01 + 0.5 * np.mean((err - want_t) ** 2))) < 1e-12
+    # an alternative equal to the output: target 0
+    _, _, de2 = ko.loss(out, err, t, 0.5, out)
+    assert np.allclose(de2, 2 * 0.5 * err / (H * W), rtol=1e-12, atol=1e-15)
