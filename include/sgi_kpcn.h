@@ -218,10 +218,14 @@ int sgi_kpcn_pack_weights(const float* params, uint16_t* weights, uint16_t* weig
 size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width);
 
 /* The learned denoise of one frame: features -> network -> multi-scale
- * reconstruction of the noisy radiance; then per 16x16 tile, if the mean predicted relative squared
- * error exp(err) of its pixels exceeds `threshold`, the tile takes the a-trous
- * result instead (the a-trous passes run only over the flagged tiles and a
- * 28-pixel margin, and not at all when no tile is flagged).  Outputs as
+ * reconstruction of the noisy radiance; then per 16x16 tile, if the mean of
+ * exp(err) over its pixels exceeds `threshold`, the tile takes the a-trous
+ * result instead.  exp(err) is the network's predicted relative squared error
+ * when its error channel was trained by sgi_kpcn_loss, and the predicted ratio
+ * of that error to the a-trous filter's when trained by sgi_kpcn_loss_ratio
+ * (the shipped weights; threshold 1.2).  The a-trous passes run only over
+ * the flagged tiles and a 28-pixel margin, and not at all when no tile is
+ * flagged.  Outputs as
  * sgi_denoise (single frame: history = 1, reservoirOut = (gi, 1)); prevGi /
  * prevHistory as sgi_denoise.  fallbackTiles (device int32 [1 + tiles], may be
  * NULL) receives the number of fallback tiles and one flag per tile.

@@ -356,8 +356,8 @@ def test_denoise_learned_threshold_semantics(lib):
     assert torch.equal(outs[0], lit + col * out)
 
     # a threshold between: each flagged tile takes sgi_denoise's outputs bit for bit (the a-trous
-    # passes run over the flagged tiles and a one-tile margin, wider than their 14-pixel reach),
-    # every other tile the network's
+    # passes run over the flagged tiles and a margin wider than their reach), every other tile the
+    # network's
     net_outs, net_hist = outs, hist
     lo, hi = -30.0, 30.0  # log thresholds: every tile falls back at lo, none at hi; bisect
     for _ in range(60):
