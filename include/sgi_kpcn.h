@@ -239,6 +239,22 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
                         float* combined, float* giOut, float* reservoirOut, float* historyOut,
                         int32_t* fallbackTiles, void* stream);
 
+/* The learned form of a batch of views (sgi_views_*): VPL culling, atlas
+ * entries and direct light stay one launch over every view; then per view
+ * the sampled gather (sgi_gather_ex SAMPLES: `samples` hashed draws per pixel
+ * over every visible VPL, rejectionLimit as sgi_gather_opts, FP32 shading)
+ * and sgi_denoise_learned with `net` and `threshold` (no previous G-buffer:
+ * prevGi / prevHistory of the view's sgi_view_io), each view's outputs equal
+ * to that chain run on the view alone.  Tile lists are not computed (the
+ * views' tiles buffers are left as they are).  samples = 0 restores the
+ * tile-list form.  Needs sgi_views_set first: the network's workspace is
+ * sized for the batch's largest view (a later sgi_views_set with larger
+ * views grows it); net and the weights it names must stay valid while the
+ * batch runs or replays.  A captured graph keeps the form it was captured
+ * with: sgi_views_replay refuses a batch whose form changed since. */
+int sgi_views_set_learned(sgi_views* views, int32_t samples, int32_t rejectionLimit, const sgi_kpcn_net* net,
+                          float threshold);
+
 #ifdef __cplusplus
 }
 #endif

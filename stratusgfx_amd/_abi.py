@@ -196,6 +196,7 @@ SGI_SIGNATURES = {
     "sgi_views_run": (c_int32, [_P, _P]),
     "sgi_views_capture": (c_int32, [_P, _P]),
     "sgi_views_replay": (c_int32, [_P, _P]),
+    "sgi_views_set_learned": (c_int32, [_P, _I, _I, POINTER(KpcnNet), c_float]),
     # include/sgi_kpcn.h
     "sgi_kpcn_conv3x3": (c_int32, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
     "sgi_kpcn_conv3x3_fp8": (c_int32, [_P, _P, _P, _I, _I, _P, _P]),

@@ -703,40 +703,21 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
 
 }  // namespace
 
-extern "C" {
+namespace sgi_launch {
 
-size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width) {
-    if (height <= 0 || width <= 0 || height > 16384 || width > 16384) return 0;
-    return kpcnLayout(height, width, nullptr, nullptr) + 256;
-}
-
-int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev, const int32_t* visibleIndices,
-                        const float* screen, const float* color, const float* reservoir, const float* prevGi,
-                        const float* prevHistory, const sgi_kpcn_net* net, float threshold, void* workspace,
-                        size_t workspaceBytes, float* combined, float* giOut, float* reservoirOut, float* historyOut,
-                        int32_t* fallbackTiles, void* stream) {
-    if (!ctx) return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL context");
-    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_denoise_learned: sgi_set_frame first");
-    int r = checkGbuffer(ctx, cur, "sgi_denoise_learned", true);
-    if (r != SGI_OK) return r;
-    if (prev && (r = checkGbuffer(ctx, prev, "sgi_denoise_learned(prev)", false)) != SGI_OK) return r;
-    if (!visibleIndices || !screen || !color || !reservoir || !combined || !giOut || !reservoirOut || !historyOut)
-        return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL buffer");
-    if (!net || !net->weights || !net->biases || !aligned16(net->weights))
-        return fail(SGI_E_INVALID, "sgi_denoise_learned: network weights missing or unaligned");
-    if (net->weights_fp8 && (!net->epilogue || !aligned16(net->weights_fp8)))
-        return fail(SGI_E_INVALID, "sgi_denoise_learned: FP8 weights without their epilogue, or unaligned");
-    if (!aligned16(reservoir) || !aligned16(reservoirOut))
-        return fail(SGI_E_INVALID, "sgi_denoise_learned: reservoirs must be 16-byte aligned");
-    const int H = cur->height, W = cur->width;
-    if (!workspace || workspaceBytes < sgi_kpcn_workspace_bytes(H, W))
-        return fail(SGI_E_INVALID, "sgi_denoise_learned: workspace smaller than sgi_kpcn_workspace_bytes()");
+// The body of sgi_denoise_learned (arguments checked by the caller; workspace of
+// sgi_kpcn_workspace_bytes(c.H, c.W) bytes): shared with the batched views.
+hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p, const sgi_dev::FrameArgs& fr,
+                          const int32_t* visibleIndices, const float* screen, const float* color,
+                          const float* reservoir, const float* prevGi, const float* prevHistory,
+                          const sgi_kpcn_net& netArgs, float threshold, void* workspace, float* combined,
+                          float* giOut, float* reservoirOut, float* historyOut, int32_t* fallbackTiles,
+                          const DenoiseBuffers& dws, hipStream_t s) {
+    const int H = c.H, W = c.W;
     char* base = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
     KpcnWs ws;
     kpcnLayout(H, W, base, &ws);
-    hipStream_t s = (hipStream_t)stream;
-    const sgi_dev::GbufArgs c = sgi_dev::gbufArgs(*cur);
-    const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
+    const sgi_kpcn_net* net = &netArgs;
     auto Wl = [&](int l) { return net->weights + (size_t)l * SGI_KPCN_LAYER_WEIGHTS; };
     auto Bl = [&](int l) { return net->biases + (size_t)l * SGI_KPCN_CHANNELS; };
     // FP8 inference: layers 1..16 on E4M3 activations and weights, each with its epilogue
@@ -755,7 +736,7 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
 #define KPCN_STEP(call) \
     if (e == hipSuccess) e = (call)
     // encoder: level 0 from the G-buffer features, then pool -> conv -> conv per level
-    KPCN_STEP(sgi_kpcn::features16Pyramid(c, ctx->frameArgs, reservoir, ws.feat, ws.rad[0], ws.rad[1], s));
+    KPCN_STEP(sgi_kpcn::features16Pyramid(c, fr, reservoir, ws.feat, ws.rad[0], ws.rad[1], s));
     // layer 0 (features -> e0) also writes p1, the 2x2 mean of e0; likewise each level's second
     // convolution writes the next level's input
     KPCN_STEP(sgi_kpcn::conv3x3In16AndPool(ws.feat, ws.h[0], ws.w[0], Wl(0), Kt(0), ws.act[0][0], ws.act[1][2], f8,
@@ -788,18 +769,56 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
     if (std::isinf(threshold) && threshold > 0.0f) {
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));
-        KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, ctx->frameArgs, screen, color, ws.o[0], prevGi, prevHistory,
+        KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
                                                  combined, giOut, reservoirOut, historyOut, s));
     } else {
         KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
         // a-trous over the flagged tiles (and their margin), the network elsewhere
-        KPCN_STEP(sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
+        KPCN_STEP(sgi_launch::denoiseSpatial(c, fr, visibleIndices, reservoir, reservoirOut, dws,
                                              ws.dilated, false, s));
         KPCN_STEP(sgi_kpcn::select(ws.o[0], ws.flags, H, W, reservoirOut, s));
-        KPCN_STEP(sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
-                                              combined, giOut, historyOut, ctx->ws, s));
+        KPCN_STEP(sgi_launch::denoiseTemporal(c, p, fr, screen, color, reservoirOut, prevGi, prevHistory,
+                                              combined, giOut, historyOut, dws, s));
     }
 #undef KPCN_STEP
+    return e;
+}
+
+}  // namespace sgi_launch
+
+extern "C" {
+
+size_t sgi_kpcn_workspace_bytes(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0 || height > 16384 || width > 16384) return 0;
+    return kpcnLayout(height, width, nullptr, nullptr) + 256;
+}
+
+int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev, const int32_t* visibleIndices,
+                        const float* screen, const float* color, const float* reservoir, const float* prevGi,
+                        const float* prevHistory, const sgi_kpcn_net* net, float threshold, void* workspace,
+                        size_t workspaceBytes, float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                        int32_t* fallbackTiles, void* stream) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL context");
+    if (!ctx->hasFrame) return fail(SGI_E_STATE, "sgi_denoise_learned: sgi_set_frame first");
+    int r = checkGbuffer(ctx, cur, "sgi_denoise_learned", true);
+    if (r != SGI_OK) return r;
+    if (prev && (r = checkGbuffer(ctx, prev, "sgi_denoise_learned(prev)", false)) != SGI_OK) return r;
+    if (!visibleIndices || !screen || !color || !reservoir || !combined || !giOut || !reservoirOut || !historyOut)
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: NULL buffer");
+    if (!net || !net->weights || !net->biases || !aligned16(net->weights))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: network weights missing or unaligned");
+    if (net->weights_fp8 && (!net->epilogue || !aligned16(net->weights_fp8)))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: FP8 weights without their epilogue, or unaligned");
+    if (!aligned16(reservoir) || !aligned16(reservoirOut))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: reservoirs must be 16-byte aligned");
+    const int H = cur->height, W = cur->width;
+    if (!workspace || workspaceBytes < sgi_kpcn_workspace_bytes(H, W))
+        return fail(SGI_E_INVALID, "sgi_denoise_learned: workspace smaller than sgi_kpcn_workspace_bytes()");
+    const hipError_t e = sgi_launch::denoiseLearned(sgi_dev::gbufArgs(*cur), sgi_dev::gbufArgs(prev ? *prev : *cur),
+                                                    ctx->frameArgs, visibleIndices, screen, color, reservoir, prevGi,
+                                                    prevHistory, *net, threshold, workspace, combined, giOut,
+                                                    reservoirOut, historyOut, fallbackTiles, ctx->ws,
+                                                    (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise_learned");
 }
 

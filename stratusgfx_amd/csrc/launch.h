@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/sgi_kpcn.h"
 #include "gi_device.h"
 #include "views.h"
 
@@ -116,6 +117,16 @@ hipError_t denoiseTemporalNet(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufA
                               const float* screen, const float* color, const float* net, const float* prevGi,
                               const float* prevHistory, float* combined, float* giOut, float* reservoirOut,
                               float* historyOut, hipStream_t s);
+
+// sgi_denoise_learned's work (api.hip) on checked arguments: workspace of
+// sgi_kpcn_workspace_bytes(cur.H, cur.W) bytes, dws the a-trous planes of the
+// fallback.  Shared with the batched views' learned form.
+hipError_t denoiseLearned(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, const sgi_dev::FrameArgs& fr,
+                          const int32_t* visibleIndices, const float* screen, const float* color,
+                          const float* reservoir, const float* prevGi, const float* prevHistory,
+                          const sgi_kpcn_net& net, float threshold, void* workspace, float* combined, float* giOut,
+                          float* reservoirOut, float* historyOut, int32_t* fallbackTiles, const DenoiseBuffers& dws,
+                          hipStream_t s);
 
 }  // namespace sgi_launch
 

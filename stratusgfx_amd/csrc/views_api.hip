@@ -3,11 +3,15 @@
 // optional captured graph of one frame.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/sgi.h"
+#include "../../include/sgi_kpcn.h"
 #include "launch.h"
 
 void sgi_set_error(const std::string& msg);
@@ -29,6 +33,14 @@ struct sgi_views {
     sgi_atlas_entry* entries = nullptr;        // [max_views][4480]
     uint8_t* visFlags = nullptr;               // [max_views][10000]
     float4* planes = nullptr;                  // [3][max_pixels]
+    // the learned form (samples > 0): per-view sampled gather + learned denoise
+    int samples = 0, rejectionLimit = -1;
+    sgi_kpcn_net net{};
+    float threshold = 0.0f;
+    void* prep = nullptr;        // the sampled gather's packed records (views run one after another)
+    void* kws = nullptr;         // the network's workspace, for the largest view
+    size_t kwsBytes = 0;
+    std::vector<sgi_batch::ViewDesc> host;  // the descriptors on the host (per-view launches)
     // the captured frame
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -38,6 +50,10 @@ struct sgi_views {
         const sgi_vpl* vpls;
         const sgi_atlas_entry* shadowIndices;
         sgi_dev::CubeArgs cubes;
+        int samples, rejectionLimit;
+        sgi_kpcn_net net;
+        float threshold;
+        const void* kws;
     } captured{};
     bool hasGraph = false;
 };
@@ -84,6 +100,11 @@ sgi_views::Shape shapeOf(const sgi_views* v) {
     s.vpls = v->vpls;
     s.shadowIndices = v->shadowIndices;
     s.cubes = v->cubes;
+    s.samples = v->samples;
+    s.rejectionLimit = v->rejectionLimit;
+    s.net = v->net;
+    s.threshold = v->threshold;
+    s.kws = v->kws;
     return s;
 }
 
@@ -93,7 +114,24 @@ bool sameShape(const sgi_views::Shape& a, const sgi_views::Shape& b) {
                 a.vpls == b.vpls && a.shadowIndices == b.shadowIndices && a.cubes.numAtlases == b.cubes.numAtlases &&
                 a.cubes.res == b.cubes.res && a.cubes.layers == b.cubes.layers;
     for (int i = 0; same && i < SGI_MAX_TOTAL_SHADOW_ATLASES; ++i) same = a.cubes.atlas[i] == b.cubes.atlas[i];
+    same = same && a.samples == b.samples && a.rejectionLimit == b.rejectionLimit && a.kws == b.kws &&
+           a.net.weights == b.net.weights && a.net.biases == b.net.biases &&
+           a.net.weights_fp8 == b.net.weights_fp8 && a.net.epilogue == b.net.epilogue &&
+           std::memcmp(&a.threshold, &b.threshold, sizeof(float)) == 0;
     return same;
+}
+
+// The network's workspace for the batch's largest view (outside any capture).
+hipError_t sizeLearnedWorkspace(sgi_views* v) {
+    size_t need = 0;
+    for (const sgi_batch::ViewDesc& d : v->host) need = std::max(need, sgi_kpcn_workspace_bytes(d.g.H, d.g.W));
+    if (need <= v->kwsBytes) return hipSuccess;
+    if (v->kws) (void)hipFree(v->kws);
+    v->kws = nullptr;
+    v->kwsBytes = 0;
+    const hipError_t e = hipMalloc(&v->kws, need);
+    if (e == hipSuccess) v->kwsBytes = need;
+    return e;
 }
 
 hipError_t runStages(sgi_views* v, hipStream_t s) {
@@ -102,6 +140,21 @@ hipError_t runStages(sgi_views* v, hipStream_t s) {
     if (e == hipSuccess)
         e = sgi_launch::gatherVisibleEntriesViews(v->descs, v->numViews, v->shadowIndices, v->maxEntries, s);
     if (e == hipSuccess) e = sgi_launch::directLightViews(v->descs, v->numViews, v->totalTiles16, s);
+    if (v->samples > 0) {
+        // the learned form: each view's sampled gather and network in turn (sizes differ per view)
+        for (int i = 0; i < v->numViews && e == hipSuccess; ++i) {
+            const sgi_batch::ViewDesc& d = v->host[(size_t)i];
+            e = sgi_launch::gatherEx(d.g, d.fr, v->cubes, v->cubeFormat, SGI_GATHER_SAMPLES, v->samples,
+                                     v->rejectionLimit, 0, 0, d.vplUpdated, d.visible, d.entries, v->prep, d.color,
+                                     d.reservoir, s);
+            if (e == hipSuccess)
+                e = sgi_launch::denoiseLearned(d.g, d.g, d.fr, d.visible, d.direct, d.color, d.reservoir, d.prevGi,
+                                               d.prevHistory, v->net, v->threshold, v->kws, d.combined, d.giOut,
+                                               d.reservoirOut, d.historyOut, nullptr,
+                                               sgi_launch::DenoiseBuffers{d.resA, d.resB, d.normals}, s);
+        }
+        return e;
+    }
     if (e == hipSuccess)
         e = sgi_launch::cullTilesViews(v->descs, v->numViews, v->totalTiles, c.tile_size, c.max_vpls_per_tile, s);
     if (e == hipSuccess)
@@ -156,6 +209,8 @@ void sgi_views_destroy(sgi_views* v) {
     if (v->entries) (void)hipFree(v->entries);
     if (v->visFlags) (void)hipFree(v->visFlags);
     if (v->planes) (void)hipFree(v->planes);
+    if (v->prep) (void)hipFree(v->prep);
+    if (v->kws) (void)hipFree(v->kws);
     delete v;
 }
 
@@ -239,6 +294,13 @@ int sgi_views_set(sgi_views* v, int32_t numViews, const sgi_view_io* io, const s
     // the staging buffer is rewritten by the next call, so the copy must have completed
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return hipFail(e, "sgi_views_set");
+    try {
+        v->host.assign(v->hostDescs, v->hostDescs + numViews);
+    } catch (const std::bad_alloc&) {
+        return fail(SGI_E_INVALID, "sgi_views_set: out of host memory");
+    }
+    if (v->samples > 0 && (e = sizeLearnedWorkspace(v)) != hipSuccess)
+        return hipFail(e, "sgi_views_set: learned workspace");
     v->numViews = numViews;
     v->totalTiles = tiles;
     v->totalTiles16 = tiles16;
@@ -291,6 +353,35 @@ int sgi_views_replay(sgi_views* v, void* stream) {
         return fail(SGI_E_STATE, "sgi_views_replay: the batch changed shape since the capture");
     const hipError_t e = hipGraphLaunch(v->exec, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_views_replay");
+}
+
+int sgi_views_set_learned(sgi_views* v, int32_t samples, int32_t rejectionLimit, const sgi_kpcn_net* net,
+                          float threshold) {
+    if (!v) return fail(SGI_E_INVALID, "sgi_views_set_learned: NULL context");
+    if (samples == 0) {
+        v->samples = 0;
+        return SGI_OK;
+    }
+    if (!v->hasBatch) return fail(SGI_E_STATE, "sgi_views_set_learned: sgi_views_set first");
+    if (samples < 1 || samples > 4096) return fail(SGI_E_INVALID, "sgi_views_set_learned: samples must be within 0..4096");
+    if (!net || !net->weights || !net->biases || ((uintptr_t)net->weights & 15u) != 0)
+        return fail(SGI_E_INVALID, "sgi_views_set_learned: network weights missing or unaligned");
+    if (net->weights_fp8 && (!net->epilogue || ((uintptr_t)net->weights_fp8 & 15u) != 0))
+        return fail(SGI_E_INVALID, "sgi_views_set_learned: FP8 weights without their epilogue, or unaligned");
+    if (std::isnan(threshold)) return fail(SGI_E_INVALID, "sgi_views_set_learned: threshold is NaN");
+    hipError_t e = hipSuccess;
+    if (!v->prep) e = hipMalloc(&v->prep, (size_t)SGI_MAX_TOTAL_VPLS_PER_FRAME * sgi_launch::packedVplBytes());
+    const int previous = v->samples;
+    v->samples = samples;
+    if (e == hipSuccess) e = sizeLearnedWorkspace(v);
+    if (e != hipSuccess) {
+        v->samples = previous;
+        return hipFail(e, "sgi_views_set_learned: workspace allocation");
+    }
+    v->rejectionLimit = rejectionLimit;
+    v->net = *net;
+    v->threshold = threshold;
+    return SGI_OK;
 }
 
 }  // extern "C"

@@ -349,6 +349,14 @@ class Views:
         _abi.call(self.lib, "sgi_views_set", self.handle, len(ios), arr, _ptr(vpls), num_vpls, _ptr(entries),
                   ctypes.byref(cubes), _stream(stream))
 
+    def set_learned(self, samples: int, net: _abi.KpcnNet | None = None, threshold: float = float("inf"),
+                    rejection_limit: int = -1) -> None:
+        """The batch's learned form (sgi_views_set_learned): per view the sampled gather at `samples`
+        draws per pixel and the learned denoiser; samples = 0 restores the tile-list form.  The
+        tensors behind `net` must outlive the batch."""
+        _abi.call(self.lib, "sgi_views_set_learned", self.handle, samples, rejection_limit,
+                  ctypes.byref(net) if net is not None else None, threshold)
+
     def run(self, stream=None) -> None:
         _abi.call(self.lib, "sgi_views_run", self.handle, _stream(stream))
 

@@ -9,6 +9,10 @@ replayed from a captured graph):
   batch) and with 8x8 tiles and 32: every view's tile lists, direct light,
   gather, a-trous + combine and history equal the same view rendered alone by
   sgi_render_frame, bit for bit, on the first replay and on a later one;
+- the batch's learned form (sgi_views_set_learned: the sampled gather at 24
+  draws per pixel and the FP8 network with its shipped fallback threshold):
+  every view's direct light, gather and denoised outputs equal the same view
+  run alone through that chain, bit for bit, from the captured graph;
 - on one 16-row band per view (the band moves down the frame from view to
   view, the last partial tile row of 1080-row views included): the view's
   visible list and the band's tile lists bit-exact against the oracle, the
@@ -19,7 +23,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from gpu_util import assert_tiles_equal, require_gpu
+from gpu_util import assert_tiles_equal, learned_single, require_gpu, shipped_nets
 from stratusgfx_amd import _abi
 from stratusgfx_amd import scene as sc
 from stratusgfx_amd.gi import FrameBuffers, GiPass, Views, view_io
@@ -163,3 +167,33 @@ def test_c2_row_bands_against_the_oracle(c2):
         del gi, fb, h
     print(f"[c2 oracle] worst rel-L2 {worst:.2e}; {lit_bands} of {len(inputs)} bands receive GI", flush=True)
     assert lit_bands >= len(inputs) // 2
+
+
+def test_c2_learned_batch_equals_single_views(c2):
+    torch = require_gpu()
+    inputs, shared, n = c2["inputs"], c2["shared"], c2["num_vpls"]
+    tile, K, S = 16, 64, 24
+    wts, f8, thr = shipped_nets()
+    net = f8.net()
+    bufs = [_bufs(torch, inp, tile, K) for inp in inputs]
+    ios = [view_io(inp, **b) for inp, b in zip(inputs, bufs)]
+    vw = Views(len(inputs), sum(inp.width * inp.height for inp in inputs), tile, K)
+    s = torch.cuda.Stream()
+    vw.set(ios, shared.vpls, n, shared.entries, shared.cubes(), stream=s.cuda_stream)
+    vw.set_learned(S, net, thr)
+    vw.capture(s.cuda_stream)
+    for _ in range(2):
+        vw.replay(s.cuda_stream)
+    s.synchronize()
+    bad = []
+    keys = ("direct", "color", "reservoir", "combined", "gi", "reservoir_out", "history")
+    for i, (inp, b) in enumerate(zip(inputs, bufs)):
+        ref = learned_single(inp, shared, n, net, thr, S, tile, K)
+        bad += [(i, k) for k in keys if not torch.equal(b[k], ref[k])]
+        if i % 8 == 0:
+            print(f"[c2 learned] view {i} {inp.width}x{inp.height}: mean GI {float(ref['gi'].mean()):.4g}", flush=True)
+        del ref
+    vw.close()
+    print(f"[c2 learned] {len(inputs)} views ({S} draws, FP8 network, threshold {thr:g}) compared, "
+          f"mismatches {bad[:6]}", flush=True)
+    assert not bad

@@ -7,7 +7,7 @@ import dataclasses
 import numpy as np
 import pytest
 
-from gpu_util import assert_tiles_equal, require_gpu
+from gpu_util import assert_tiles_equal, learned_single, require_gpu, shipped_nets
 from stratusgfx_amd import _abi
 from stratusgfx_amd import scene as sc
 from stratusgfx_amd.gi import FrameBuffers, GiPass, Views, view_io
@@ -110,3 +110,40 @@ def test_replay_refuses_a_changed_batch(batch):
         vw.replay()
     vw.set(ios, shared.vpls, 256, shared.entries, shared.cubes())
     vw.replay()
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_learned_batch_equals_single_views(batch, precision):
+    """sgi_views_set_learned: every view's direct light, sampled gather (12 draws) and learned
+    denoise (shipped weights and threshold) equal the view rendered alone, bit for bit, on a run and
+    on graph replays; a replay after the form changed is refused."""
+    torch = require_gpu()
+    wts, f8, thr = shipped_nets()
+    net = wts.net() if precision == "bf16" else f8.net()
+    vw, ios, shared = batch["vw"], batch["ios"], batch["shared"]
+    vw.set(ios, shared.vpls, 256, shared.entries, shared.cubes())
+    vw.set_learned(12, net, thr)
+    keys = ("direct", "color", "reservoir", "combined", "gi", "reservoir_out", "history")
+    refs = [learned_single(inp, shared, 256, net, thr, 12, TILE, K) for (_, _, inp) in batch["views"]]
+    s = torch.cuda.Stream()
+    for rep in range(3):
+        for b in batch["bufs"]:
+            for t in b.values():
+                t.zero_()
+        torch.cuda.synchronize()
+        if rep == 0:
+            vw.run()
+        else:
+            if rep == 1:
+                vw.capture(s.cuda_stream)
+            with torch.cuda.stream(s):
+                vw.replay(s.cuda_stream)
+            s.synchronize()
+        torch.cuda.synchronize()
+        for i, (ref, b) in enumerate(zip(refs, batch["bufs"])):
+            for k in keys:
+                assert torch.equal(b[k], ref[k]), f"{precision} pass {rep}: view {i} {k}"
+    assert any(float(r["gi"].abs().sum()) > 0 for r in refs)
+    vw.set_learned(0)  # back to the tile-list form: the graph captured with the learned form is refused
+    with pytest.raises(_abi.SgiError):
+        vw.replay(s.cuda_stream)
