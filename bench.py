@@ -577,7 +577,7 @@ def shadow_passes(state, reps=10):
     return out
 
 
-def batched_views(args):
+def batched_views(args, learned=None):
     """Many views of the city block at uneven resolutions, one launch per stage over all views
     in a captured graph, launch shape from the autotuner."""
     import numpy as np
@@ -604,11 +604,60 @@ def batched_views(args):
     c = tuned["choice"]
     ms, _ = autotune.measure(inputs, shared, sc.CITY_1080P.num_vpls, c["tile_size"], c["max_vpls_per_tile"],
                              reps=args.steps)
-    return {"views": len(inputs), "megapixels": round(mp, 2), "tile_size": c["tile_size"],
-            "max_vpls_per_tile": c["max_vpls_per_tile"], "ms_per_batch": round(ms, 3),
+    out = {"views": len(inputs), "megapixels": round(mp, 2), "tile_size": c["tile_size"],
+           "max_vpls_per_tile": c["max_vpls_per_tile"], "ms_per_batch": round(ms, 3),
+           "views_per_s": round(len(inputs) / (ms * 1e-3), 1), "megapixels_per_s": round(mp / (ms * 1e-3), 1),
+           "relmse": c["relmse"], "reference_shape": tuned["reference"], "autotune": tuned["table"],
+           "note": "tile-list form: one launch per stage over every view, replayed from a captured graph"}
+    if learned is not None:
+        out["learned"] = batched_learned(inputs, shared, sc.CITY_1080P.num_vpls, learned, args.steps)
+    return out
+
+
+def batched_learned(inputs, shared, num_vpls, learned, reps):
+    """The same batch in its learned form (sgi_views_set_learned): batched culling and direct light,
+    then each view's sampled gather and network with the fallback threshold, in one captured graph;
+    HIP-event time per batch and the mean relMSE of the views' GI against their converged renders."""
+    import torch
+
+    from stratusgfx_amd import autotune
+    from stratusgfx_amd.gi import Views, view_io
+    S, net, thr, label = learned
+    f32 = dict(dtype=torch.float32, device="cuda")
+    bufs, ios = [], []
+    for inp in inputs:
+        H, W = inp.height, inp.width
+        nt = ((W + 15) // 16) * ((H + 15) // 16)
+        b = dict(tiles=torch.zeros((nt, 65), dtype=torch.int32, device="cuda"), direct=torch.zeros((H, W, 3), **f32),
+                 color=torch.zeros((H, W, 3), **f32), reservoir=torch.zeros((H, W, 4), **f32),
+                 combined=torch.zeros((H, W, 3), **f32), gi=torch.zeros((H, W, 3), **f32),
+                 reservoir_out=torch.zeros((H, W, 4), **f32), history=torch.zeros((H, W), **f32))
+        bufs.append(b)
+        ios.append(view_io(inp, **b))
+    vw = Views(len(inputs), sum(i.width * i.height for i in inputs), 16, 64)
+    s = torch.cuda.Stream()
+    vw.set(ios, shared.vpls, num_vpls, shared.entries, shared.cubes(), stream=s.cuda_stream)
+    vw.set_learned(S, net, thr)
+    vw.capture(s.cuda_stream)
+    for _ in range(2):
+        vw.replay(s.cuda_stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    with torch.cuda.stream(s):
+        ev[0].record()
+        for i in range(reps):
+            vw.replay(s.cuda_stream)
+            ev[i + 1].record()
+    s.synchronize()
+    ms = sum(ev[i].elapsed_time(ev[i + 1]) for i in range(reps)) / reps
+    rel = sum(autotune.rel_mse(b["gi"], autotune.converged(inp, shared, num_vpls))
+              for b, inp in zip(bufs, inputs)) / len(inputs)
+    vw.close()
+    mp = sum(i.width * i.height for i in inputs) / 1e6
+    return {"samples_per_pixel": S, "network": label, "fallback_threshold": thr, "ms_per_batch": round(ms, 3),
             "views_per_s": round(len(inputs) / (ms * 1e-3), 1), "megapixels_per_s": round(mp / (ms * 1e-3), 1),
-            "reference_shape": tuned["reference"], "autotune": tuned["table"],
-            "note": "one launch per stage over every view, replayed from a captured graph"}
+            "relmse": round(rel, 6),
+            "note": "culling and direct light one launch over every view; per view the sampled gather and the "
+                    "network (their sizes differ), all in one captured graph"}
 
 
 def main():
@@ -971,7 +1020,11 @@ def main():
     if args.views > 0 and args.forms == "all":
         del plain, learned
         torch.cuda.empty_cache()
-        result["batched_views"] = batched_views(args)
+        bl = None
+        if learned_res is not None:
+            bl = (learned_res["samples_per_pixel"], (f8net.net() if learned_res["network_precision"] == "fp8"
+                                                      else wts.net()), threshold, learned_res["network_precision"])
+        result["batched_views"] = batched_views(args, bl)
     print(json.dumps(result), flush=True)
 
 

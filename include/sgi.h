@@ -85,7 +85,15 @@ typedef struct sgi_tile_stage1 {
  * with the cap K a run-time value: record t is int32[1 + K] at t * (1 + K):
  * { numVisible, indices[K] }.  Entries past numVisible are unspecified. */
 
-/* G-buffer planes (StratusRendererBackend.h:327-338), FP32. */
+struct sgi_gbuffer_packed;
+
+/* G-buffer planes (StratusRendererBackend.h:327-338), FP32.  With `packed`
+ * set, every pass reads the normals, albedo, base reflectivity, roughness /
+ * metallic / ambient and velocity from the reference's packed attachments
+ * instead (the FP32 pointers of those five are then ignored and may be NULL;
+ * depth, id and SSAO stay FP32): 22 B per pixel read instead of 60, and the
+ * same results bit for bit as the FP32 planes sgi_gbuffer_unpack makes from
+ * the same attachments.  packed->width / height must equal width / height. */
 typedef struct sgi_gbuffer {
     int32_t width;
     int32_t height;
@@ -97,6 +105,7 @@ typedef struct sgi_gbuffer {
     const float* velocity;                 /* [H][W][2] uv motion (cur - prev); NULL = 0 */
     const float* id;                       /* [H][W]    mesh id; NULL = 0               */
     const float* ssao;                     /* [H][W]    SSAO; NULL = 1.0                */
+    const struct sgi_gbuffer_packed* packed; /* NULL: the FP32 planes above            */
 } sgi_gbuffer;
 
 /* Texel formats of the cube atlases.  Distance cubes are F32, F16 (IEEE
@@ -203,6 +212,22 @@ int sgi_set_csm(sgi_ctx* ctx, const sgi_csm* csm);
  * be the context's viewport. */
 int sgi_gbuffer_unpack(sgi_ctx* ctx, const sgi_gbuffer_packed* packed, float* normal, float* albedo,
                        float* baseReflectivity, float* roughnessMetallicAmbient, float* velocity, void* stream);
+
+/* The geometry pass's attachment writes: its fragment outputs gNormal,
+ * gAlbedo, gReflectivityEmissive, gRoughnessMetallicAmbient and
+ * gVelocityBuffer (pbr_geometry_pass.fs:127-137), given as the FP32 planes of
+ * `planes`, stored into the attachments' formats
+ * (StratusRendererBackend.cpp:384-424) as GL converts a fragment output: a
+ * UNORM8 channel clamps to [0, 1] and stores the code floor(v * 255 + 0.5)
+ * (NaN stores 0), velocity rounds to the nearest binary16 (ties to even);
+ * RGB8 texels keep byte 3 zero.  Writes the attachments `out` names (its
+ * pointers are the destinations): normals, albedo, baseReflectivity,
+ * roughnessMetallicAmbient, and velocity when both planes->velocity and
+ * out->velocity are set.  planes->depth, id, ssao and packed are not read.
+ * Sizes: the context's viewport.  sgi_gbuffer_unpack(sgi_gbuffer_pack(x))
+ * is x rounded to the attachments' precision; sgi_gbuffer_pack of unpacked
+ * attachments gives the same codes back. */
+int sgi_gbuffer_pack(sgi_ctx* ctx, const sgi_gbuffer* planes, const sgi_gbuffer_packed* out, void* stream);
 
 /* viscull_vpls.cs:47-120 driven by PerformVirtualPointLightCullingStage1_
  * (StratusRendererBackend.cpp:1578-1661).  For i < n, VPL i is sun-visible when

@@ -1021,6 +1021,35 @@ void orc_gbuffer_unpack(const sgi_gbuffer_packed* g, float* normal, float* albed
     }
 }
 
+// The geometry pass's attachment writes (pbr_geometry_pass.fs:127-137 outputs
+// into the formats of StratusRendererBackend.cpp:384-424): GL's conversion of
+// a fragment output into a UNORM8 channel, clamp to [0, 1] then
+// floor(v * 255 + 0.5) in FP32 (NaN clamps to 0), and into RG16F, the
+// nearest binary16 with ties to even.
+void orc_gbuffer_pack(const sgi_gbuffer* planes, uint32_t* n8, uint32_t* a8, uint16_t* f8, uint32_t* m8,
+                      uint16_t* v16) {
+    const long n = (long)planes->width * planes->height;
+    const auto code = [](float v) -> uint32_t {
+        const float c = v >= 0.0f ? (v <= 1.0f ? v : 1.0f) : 0.0f;
+        const float s = c * 255.0f;
+        return (uint32_t)std::floor(s + 0.5f);
+    };
+    for (long p = 0; p < n; ++p) {
+        uint32_t nv = 0, mv = 0, av = 0;
+        for (int k = 0; k < 3; ++k) {
+            nv |= code(planes->normal[3 * p + k]) << (8 * k);
+            mv |= code(planes->roughnessMetallicAmbient[3 * p + k]) << (8 * k);
+        }
+        for (int k = 0; k < 4; ++k) av |= code(planes->albedo[4 * p + k]) << (8 * k);
+        n8[p] = nv;
+        a8[p] = av;
+        m8[p] = mv;
+        f8[p] = (uint16_t)(code(planes->baseReflectivity[2 * p]) | (code(planes->baseReflectivity[2 * p + 1]) << 8));
+        if (v16 && planes->velocity)
+            for (int k = 0; k < 2; ++k) v16[2 * p + k] = floatToHalfRN(planes->velocity[2 * p + k]);
+    }
+}
+
 // taa.fs:12-100 with aa_common.glsl's tonemap c / (c + 1) and inverse
 // c / (1 - c): velocity texel (NEAREST), screen at texel centres (the 4
 // textureOffset taps clamp to the edge), prevScreen LINEAR at uv - velocity,

@@ -50,6 +50,7 @@ _SIGNATURES = {
     "orc_taa": (None, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "orc_tonemap": (None, [c_void_p, ctypes.c_long, c_float, c_void_p, c_void_p]),
     "orc_gbuffer_unpack": (None, [POINTER(_abi.GbufferPacked)] + [c_void_p] * 5),
+    "orc_gbuffer_pack": (None, [POINTER(_abi.Gbuffer)] + [c_void_p] * 5),
     "orc_taa_history": (None, [c_void_p, ctypes.c_long, c_void_p]),
     "orc_fxaa_luminance": (None, [c_void_p, ctypes.c_long, c_void_p]),
     "orc_fxaa_smoothing": (None, [c_void_p, c_int, c_int, c_float, c_float, c_float, c_void_p]),
@@ -191,6 +192,19 @@ def gbuffer_unpack(packed: _abi.GbufferPacked, with_velocity=True):
     out = [np.zeros((H, W, c), np.float32) for c in (3, 4, 2, 3, 2)]
     lib().orc_gbuffer_unpack(ctypes.byref(packed), *[_p(a) for a in out[:4]], _p(out[4]) if with_velocity else None)
     return out
+
+
+def gbuffer_pack(normal, albedo, refl, rma, velocity=None):
+    """The geometry pass's attachment writes of FP32 planes [H][W][c]: (normals uint32, albedo uint32,
+    baseReflectivity uint16, roughnessMetallicAmbient uint32, velocity uint16 [H][W][2] or None)."""
+    planes = [np.ascontiguousarray(a, np.float32) for a in (normal, albedo, refl, rma)]
+    vel = None if velocity is None else np.ascontiguousarray(velocity, np.float32)
+    H, W = planes[0].shape[:2]
+    g = _abi.Gbuffer(W, H, 0, *[a.ctypes.data for a in planes], _p(vel), 0, 0, None)
+    out = [np.zeros((H, W), np.uint32), np.zeros((H, W), np.uint32), np.zeros((H, W), np.uint16),
+           np.zeros((H, W), np.uint32), np.zeros((H, W, 2), np.uint16) if vel is not None else None]
+    lib().orc_gbuffer_pack(ctypes.byref(g), *[_p(a) for a in out])
+    return tuple(out)
 
 
 def color_vpls(frame, vpls, entries, diffuse, distance, multiplier=_abi.REFERENCE_VPL_COLOR_MULTIPLIER):

@@ -55,6 +55,7 @@ class Gbuffer(ctypes.Structure):
         ("depth", c_void_p), ("normal", c_void_p), ("albedo", c_void_p),
         ("baseReflectivity", c_void_p), ("roughnessMetallicAmbient", c_void_p),
         ("velocity", c_void_p), ("id", c_void_p), ("ssao", c_void_p),
+        ("packed", c_void_p),  # const sgi_gbuffer_packed* (NULL: the FP32 planes)
     ]
 
 
@@ -179,6 +180,7 @@ SGI_SIGNATURES = {
                                        POINTER(CubeAtlases), _P]),
     "sgi_fxaa": (c_int32, [_P, _P, _P, _P, _P]),
     "sgi_gbuffer_unpack": (c_int32, [_P, POINTER(GbufferPacked), _P, _P, _P, _P, _P, _P]),
+    "sgi_gbuffer_pack": (c_int32, [_P, POINTER(Gbuffer), POINTER(GbufferPacked), _P]),
     "sgi_cull_global": (c_int32, [_P, _P, _I, _P, _P, _P]),
     "sgi_visible_atlas_entries": (c_int32, [_P, _P, _P, _I, _P, _P]),
     "sgi_color_vpls": (c_int32, [_P, _P, _P, POINTER(CubeAtlases), POINTER(CubeAtlases), c_float, _I, _P, _P]),

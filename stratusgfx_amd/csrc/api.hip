@@ -47,6 +47,15 @@ int checkGbuffer(const sgi_ctx* c, const sgi_gbuffer* g, const char* who, bool n
     if (!g) return fail(SGI_E_INVALID, std::string(who) + ": NULL gbuffer");
     if (g->width != c->cfg.width || g->height != c->cfg.height)
         return fail(SGI_E_INVALID, std::string(who) + ": gbuffer size differs from the context's viewport");
+    if (g->packed) {
+        const sgi_gbuffer_packed& k = *g->packed;
+        if (k.width != g->width || k.height != g->height)
+            return fail(SGI_E_INVALID, std::string(who) + ": packed attachments differ in size from the gbuffer");
+        if (!g->depth || !k.normals) return fail(SGI_E_INVALID, std::string(who) + ": gbuffer depth / normals are NULL");
+        if (needMaterial && (!k.albedo || !k.baseReflectivity || !k.roughnessMetallicAmbient))
+            return fail(SGI_E_INVALID, std::string(who) + ": packed albedo / reflectivity / roughness is NULL");
+        return SGI_OK;
+    }
     if (!g->depth || !g->normal) return fail(SGI_E_INVALID, std::string(who) + ": gbuffer depth / normal is NULL");
     if (needMaterial && (!g->albedo || !g->baseReflectivity || !g->roughnessMetallicAmbient))
         return fail(SGI_E_INVALID, std::string(who) + ": gbuffer albedo / reflectivity / roughness plane is NULL");
@@ -205,6 +214,24 @@ int sgi_tonemap(sgi_ctx* ctx, const float* lit, float gamma, uint32_t* out, void
     const hipError_t e =
         sgi_launch::tonemap(lit, (long)ctx->cfg.width * ctx->cfg.height, gamma, out, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_tonemap");
+}
+
+int sgi_gbuffer_pack(sgi_ctx* ctx, const sgi_gbuffer* planes, const sgi_gbuffer_packed* out, void* stream) {
+    if (!ctx || !planes || !out) return fail(SGI_E_INVALID, "sgi_gbuffer_pack: NULL argument");
+    if (planes->width != ctx->cfg.width || planes->height != ctx->cfg.height || out->width != planes->width ||
+        out->height != planes->height)
+        return fail(SGI_E_INVALID, "sgi_gbuffer_pack: size differs from the context's viewport");
+    if (!planes->normal || !planes->albedo || !planes->baseReflectivity || !planes->roughnessMetallicAmbient ||
+        !out->normals || !out->albedo || !out->baseReflectivity || !out->roughnessMetallicAmbient)
+        return fail(SGI_E_INVALID, "sgi_gbuffer_pack: NULL plane or attachment");
+    const auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+    if (!al(out->normals, 4) || !al(out->albedo, 4) || !al(out->roughnessMetallicAmbient, 4) ||
+        !al(out->baseReflectivity, 2) || (out->velocity && !al(out->velocity, 4)) || !al(planes->normal, 4) ||
+        !al(planes->albedo, 4) || !al(planes->baseReflectivity, 4) || !al(planes->roughnessMetallicAmbient, 4) ||
+        (planes->velocity && !al(planes->velocity, 4)))
+        return fail(SGI_E_INVALID, "sgi_gbuffer_pack: misaligned plane (texel alignment)");
+    const hipError_t e = sgi_launch::gbufferPack(*planes, *out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_gbuffer_pack");
 }
 
 int sgi_gbuffer_unpack(sgi_ctx* ctx, const sgi_gbuffer_packed* g, float* normal, float* albedo,

@@ -167,7 +167,7 @@ __device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs&
             const int x = x0 + lx, y = y0 + ly;
             const size_t p = (size_t)y * g.W + x;
             const f3 fp = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV);
-            const f3 nn = decodeNormal(g.normal + 3 * p);
+            const f3 nn = gNormal(g, p);
             comp[0][l] = fp.x; comp[1][l] = fp.y; comp[2][l] = fp.z;
             comp[3][l] = nn.x; comp[4][l] = nn.y; comp[5][l] = nn.z;
         }

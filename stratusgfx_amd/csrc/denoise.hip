@@ -50,15 +50,21 @@ __device__ __forceinline__ const sgi_batch::ViewDesc& viewBlock(const sgi_batch:
 }
 
 // GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv.
-__device__ __forceinline__ float bilinear(const float* t, int W, int H, int C, float u, float v, int c) {
+// Bilinear sample at uv of the texels fetch(q), q the row-major texel index.
+template <class F>
+__device__ __forceinline__ float bilinearAt(int W, int H, float u, float v, F fetch) {
     const float fu = u * (float)W - 0.5f, fv = v * (float)H - 0.5f;
     const float i0f = floorf(fu), j0f = floorf(fv);
     const float a = fu - i0f, b = fv - j0f;
     const int ia = texelIndex(i0f, W), ib = texelIndex(i0f + 1.0f, W);
     const int ja = texelIndex(j0f, H), jb = texelIndex(j0f + 1.0f, H);
-    const float t00 = t[((size_t)ja * W + ia) * C + c], t10 = t[((size_t)ja * W + ib) * C + c];
-    const float t01 = t[((size_t)jb * W + ia) * C + c], t11 = t[((size_t)jb * W + ib) * C + c];
+    const float t00 = fetch((size_t)ja * W + ia), t10 = fetch((size_t)ja * W + ib);
+    const float t01 = fetch((size_t)jb * W + ia), t11 = fetch((size_t)jb * W + ib);
     return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 + a * b * t11;
+}
+
+__device__ __forceinline__ float bilinear(const float* t, int W, int H, int C, float u, float v, int c) {
+    return bilinearAt(W, H, u, v, [&](size_t q) { return t[q * C + c]; });
 }
 
 __device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C, float u, float v, int c) {
@@ -78,7 +84,7 @@ __device__ __forceinline__ void mergeBody(const GbufArgs& g, const FrameArgs& fr
                                           int y) {
     if (x >= g.W || y >= g.H) return;
     const size_t p = (size_t)y * g.W + x;
-    const f3 centerNormal = decodeNormal(g.normal + 3 * p);
+    const f3 centerNormal = gNormal(g, p);
     const float centerDepth = g.depth[p];
     normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, centerDepth);
     if (maskedOut(mask, g.W, x, y)) return;
@@ -199,7 +205,7 @@ __device__ __forceinline__ void temporalBody(const GbufArgs& g, const GbufArgs& 
     f3 centerNormal, shadowFactor;
     float centerDepth;
     if constexpr (FROM_NET) {
-        centerNormal = decodeNormal(g.normal + 3 * p);
+        centerNormal = gNormal(g, p);
         centerDepth = g.depth[p];
         shadowFactor = ld3(net + 3 * p);
         reservoirOut[p] = make_float4(shadowFactor.x, shadowFactor.y, shadowFactor.z, 1.0f);
@@ -215,12 +221,14 @@ __device__ __forceinline__ void temporalBody(const GbufArgs& g, const GbufArgs& 
     float accumMultiplier = 1.0f;
     const f3 currGi = shadowFactor;
     const float u = texelCentre(x, W), v = texelCentre(y, H);
-    const float vx = g.velocity ? g.velocity[2 * p] : 0.0f, vy = g.velocity ? g.velocity[2 * p + 1] : 0.0f;
-    const float pu = u - vx, pv = v - vy;
+    const float2 vel = gVelocity(g, p);
+    const float pu = u - vel.x, pv = v - vel.y;
     const float prevCenterDepth = bilinear(prev.depth, W, H, 1, pu, pv, 0);
-    const f3 prevCenterNormal = normalize(f3{bilinear(prev.normal, W, H, 3, pu, pv, 0),
-                                             bilinear(prev.normal, W, H, 3, pu, pv, 1),
-                                             bilinear(prev.normal, W, H, 3, pu, pv, 2)} * 2.0f - splat(1.0f));
+    const f3 prevCenterNormal =
+        normalize(f3{bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 0); }),
+                     bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 1); }),
+                     bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 2); })} * 2.0f -
+                  splat(1.0f));
     f3 prevGiV = splat(0.0f);
     if (prevGi)
         prevGiV = f3{bilinear(prevGi, W, H, 3, pu, pv, 0), bilinear(prevGi, W, H, 3, pu, pv, 1),

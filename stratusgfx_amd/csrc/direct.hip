@@ -19,14 +19,15 @@ __device__ __forceinline__ void directBody(const GbufArgs& g, const FrameArgs& f
     const f3 viewMinusFrag = ld3(fr.viewPos) - fragPos;
     const f3 V = normalize(viewMinusFrag);
     const float viewDist = length(viewMinusFrag);
-    const float* al = g.albedo + 4 * p;
-    const f3 baseColor = f3{al[0], al[1], al[2]};
-    const f3 N = decodeNormal(g.normal + 3 * p);
-    const float* rma = g.rma + 3 * p;
-    const float roughness = rma[0], metallic = rma[1];
+    const float4 al = gAlbedo(g, p);
+    const f3 baseColor = f3{al.x, al.y, al.z};
+    const f3 N = gNormal(g, p);
+    const f3 rma = gRma(g, p);
+    const float roughness = rma.x, metallic = rma.y;
     const float ambient = g.ssao ? g.ssao[p] : 1.0f;
-    const float reflR = g.refl[2 * p], reflG = g.refl[2 * p + 1];
-    const f3 emissive = f3{al[3], reflG, rma[2]};
+    const float2 refl = gRefl(g, p);
+    const float reflR = refl.x, reflG = refl.y;
+    const f3 emissive = f3{al.w, reflG, rma.z};
 
     const f3 sun = ld3(fr.sunDir);
     const f3 blends = cascadeBlends(csm, fragPos);

@@ -346,14 +346,14 @@ __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& 
     const float vp2 = dot(vp, vp);
     px.distToCamera = __builtin_sqrtf(vp2);  // exact: shadow on / off and the sample count
     px.V = vp * frsq(vp2);
-    const float* al = g.albedo + 4 * p;
-    const f3 baseColor = f3{gmax(al[0], 0.01f), gmax(al[1], 0.01f), gmax(al[2], 0.01f)};
-    px.N = decodeNormal(g.normal + 3 * p);  // exact: side test
-    const float* rma = g.rma + 3 * p;
-    const float roughness = gmax(0.5f, rma[0]);
-    const float metallic = rma[1];
+    const float4 al = gAlbedo(g, p);
+    const f3 baseColor = f3{gmax(al.x, 0.01f), gmax(al.y, 0.01f), gmax(al.z, 0.01f)};
+    px.N = gNormal(g, p);  // exact: side test
+    const f3 rma = gRma(g, p);
+    const float roughness = gmax(0.5f, rma.x);
+    const float metallic = rma.y;
     px.ao = clampv(g.ssao ? g.ssao[p] : 1.0f, 0.35f, 1.0f);
-    const f3 baseRefl = splat(g.refl[2 * p]);
+    const f3 baseRefl = splat(gRefl(g, p).x);
     // BRDF_Base with diffuseDivisor = baseColor, specularMultiplier = 1 / (baseColor + eps)
     px.a = remapRoughness(roughness, fr);
     const f3 specMul = f3{frcp(baseColor.x + kEPS), frcp(baseColor.y + kEPS), frcp(baseColor.z + kEPS)};
@@ -370,10 +370,10 @@ __device__ __forceinline__ Pixel setupPixel(const GbufArgs& g, const FrameArgs& 
 
 __device__ __forceinline__ void writeOut(const GbufArgs& g, size_t p, f3 vplColor, float validSamples, float* color,
                                          float* reservoir) {
-    const float* al = g.albedo + 4 * p;
-    color[3 * p + 0] = gmax(al[0], 0.01f) + kEPS;
-    color[3 * p + 1] = gmax(al[1], 0.01f) + kEPS;
-    color[3 * p + 2] = gmax(al[2], 0.01f) + kEPS;
+    const float4 al = gAlbedo(g, p);
+    color[3 * p + 0] = gmax(al.x, 0.01f) + kEPS;
+    color[3 * p + 1] = gmax(al.y, 0.01f) + kEPS;
+    color[3 * p + 2] = gmax(al.z, 0.01f) + kEPS;
     float4 r;
     r.x = gmin(vplColor.x, 65504.0f);
     r.y = gmin(vplColor.y, 65504.0f);
@@ -484,8 +484,7 @@ __global__ __launch_bounds__(256) void gatherRandomKernel(GbufArgs g, FrameArgs 
     // caller-supplied count must not index past them
     const int numVisible = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
     if (numVisible > 0) {
-        const float* rma = g.rma + 3 * p;
-        const float roughness = gmax(0.5f, rma[0]);
+        const float roughness = gmax(0.5f, gRma(g, p).x);
         const float roughnessWeight = 1.0f - roughness;
         const float distRatioToCamera = gmin(1.0f - px.distToCamera / 1000.0f, 1.0f);
         const int maxSamplesPerPixel = (int)mixv(2.0f, 10.0f, roughnessWeight);

@@ -363,7 +363,12 @@ __device__ __forceinline__ float infiniteShadowValueBlock(const CsmArgs& c, f3 s
     return mixv(light2, light1, weight) * (1.0f / 9.0f);
 }
 
-// Kernel-argument view of a G-buffer.
+// Kernel-argument view of a G-buffer: the FP32 planes, or (n8 != nullptr)
+// the reference's packed attachments for the normals, albedo, reflectivity,
+// roughness / metallic / ambient and velocity (sgi_gbuffer.packed); depth, id
+// and SSAO are FP32 either way.  The passes read it through the accessors
+// below, which decode an attachment exactly as sgi_gbuffer_unpack does, so a
+// pass gives the same bits on either form.
 struct GbufArgs {
     int W, H;
     const float* depth;
@@ -374,21 +379,88 @@ struct GbufArgs {
     const float* velocity;
     const float* id;
     const float* ssao;
+    const uint32_t* n8;   // RGB8 normals (4-byte texels)
+    const uint32_t* a8;   // RGBA8 albedo
+    const uint16_t* f8;   // RG8 base reflectivity
+    const uint32_t* m8;   // RGB8 roughness, metallic, ambient
+    const uint32_t* v16;  // RG16F velocity (NULL: none)
 };
 
 __host__ inline GbufArgs gbufArgs(const sgi_gbuffer& g) {
-    GbufArgs a;
+    GbufArgs a{};
     a.W = g.width;
     a.H = g.height;
     a.depth = g.depth;
-    a.normal = g.normal;
-    a.albedo = g.albedo;
-    a.refl = g.baseReflectivity;
-    a.rma = g.roughnessMetallicAmbient;
-    a.velocity = g.velocity;
     a.id = g.id;
     a.ssao = g.ssao;
+    if (g.packed) {
+        a.n8 = g.packed->normals;
+        a.a8 = g.packed->albedo;
+        a.f8 = g.packed->baseReflectivity;
+        a.m8 = g.packed->roughnessMetallicAmbient;
+        a.v16 = reinterpret_cast<const uint32_t*>(g.packed->velocity);
+    } else {
+        a.normal = g.normal;
+        a.albedo = g.albedo;
+        a.refl = g.baseReflectivity;
+        a.rma = g.roughnessMetallicAmbient;
+        a.velocity = g.velocity;
+    }
     return a;
+}
+
+// A UNORM8 channel (byte `byte` of a texel) as GL reads it: k / 255, correctly rounded.
+__device__ __forceinline__ float unorm8(uint32_t v, int byte) { return (float)((v >> (8 * byte)) & 0xFFu) / 255.0f; }
+
+// The stored normal channel c (n * 0.5 + 0.5) of pixel p.
+__device__ __forceinline__ float gNormalC(const GbufArgs& g, size_t p, int c) {
+    return g.n8 ? unorm8(g.n8[p], c) : g.normal[3 * p + c];
+}
+// The decoded normal (common.glsl's normal * 2 - 1, normalised) of pixel p.
+__device__ __forceinline__ f3 gNormal(const GbufArgs& g, size_t p) {
+    if (g.n8) {
+        const uint32_t v = g.n8[p];
+        return normalize(f3{unorm8(v, 0), unorm8(v, 1), unorm8(v, 2)} * 2.0f - splat(1.0f));
+    }
+    return decodeNormal(g.normal + 3 * p);
+}
+// Albedo rgb and emissive.r.
+__device__ __forceinline__ float4 gAlbedo(const GbufArgs& g, size_t p) {
+    if (g.a8) {
+        const uint32_t v = g.a8[p];
+        return make_float4(unorm8(v, 0), unorm8(v, 1), unorm8(v, 2), unorm8(v, 3));
+    }
+    const float* a = g.albedo + 4 * p;
+    return make_float4(a[0], a[1], a[2], a[3]);
+}
+// Reflectance and emissive.g.
+__device__ __forceinline__ float2 gRefl(const GbufArgs& g, size_t p) {
+    if (g.f8) {
+        const uint32_t v = g.f8[p];
+        return make_float2(unorm8(v, 0), unorm8(v, 1));
+    }
+    return make_float2(g.refl[2 * p], g.refl[2 * p + 1]);
+}
+// Roughness, metallic and emissive.b.
+__device__ __forceinline__ f3 gRma(const GbufArgs& g, size_t p) {
+    if (g.m8) {
+        const uint32_t v = g.m8[p];
+        return f3{unorm8(v, 0), unorm8(v, 1), unorm8(v, 2)};
+    }
+    const float* m = g.rma + 3 * p;
+    return f3{m[0], m[1], m[2]};
+}
+__device__ __forceinline__ bool gHasVelocity(const GbufArgs& g) { return g.n8 ? g.v16 != nullptr : g.velocity != nullptr; }
+// The uv motion (cur - prev) of pixel p; 0 without a velocity plane.
+__device__ __forceinline__ float2 gVelocity(const GbufArgs& g, size_t p) {
+    if (g.n8) {
+        if (!g.v16) return make_float2(0.0f, 0.0f);
+        const uint32_t v = g.v16[p];
+        return make_float2(__half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))),
+                           __half2float(__ushort_as_half((unsigned short)(v >> 16))));
+    }
+    if (!g.velocity) return make_float2(0.0f, 0.0f);
+    return make_float2(g.velocity[2 * p], g.velocity[2 * p + 1]);
 }
 
 // Per-frame uniforms the kernels read.

@@ -85,14 +85,15 @@ __device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs
     L[2] = r.z * inv;
     const f3 P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[i], fr.invPV);
     const f3 d = ld3(fr.viewPos) - P;
-    const f3 n = decodeNormal(g.normal + 3 * i);
+    const f3 n = gNormal(g, i);
     float f[SGI_KPCN_FEATURES];
     f[0] = log1pf(gmax(L[0], 0.0f) * kRadianceScale);
     f[1] = log1pf(gmax(L[1], 0.0f) * kRadianceScale);
     f[2] = log1pf(gmax(L[2], 0.0f) * kRadianceScale);
-    f[3] = g.albedo[4 * i + 0];
-    f[4] = g.albedo[4 * i + 1];
-    f[5] = g.albedo[4 * i + 2];
+    const float4 al = gAlbedo(g, i);
+    f[3] = al.x;
+    f[4] = al.y;
+    f[5] = al.z;
     f[6] = n.x;
     f[7] = n.y;
     f[8] = n.z;

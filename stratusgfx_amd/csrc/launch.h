@@ -46,6 +46,7 @@ hipError_t fxaa(const uint32_t* in, int W, int H, float contrastThreshold, float
                 float subpixelBlending, uint32_t* out, hipStream_t s);
 
 // packed G-buffer attachments -> FP32 planes (velocity may be NULL)
+hipError_t gbufferPack(const sgi_gbuffer& planes, const sgi_gbuffer_packed& out, hipStream_t s);
 hipError_t gbufferUnpack(const sgi_gbuffer_packed& g, float* normal, float* albedo, float* refl, float* rma,
                          float* velocity, hipStream_t s);
 

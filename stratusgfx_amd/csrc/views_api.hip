@@ -247,8 +247,14 @@ int sgi_views_set(sgi_views* v, int32_t numViews, const sgi_view_io* io, const s
         const sgi_gbuffer& g = x.gbuffer;
         if (g.width <= 0 || g.height <= 0 || g.width > 16384 || g.height > 16384)
             return fail(SGI_E_INVALID, "sgi_views_set: view size must be within 1..16384");
-        if (!g.depth || !g.normal || !g.albedo || !g.baseReflectivity || !g.roughnessMetallicAmbient)
+        if (g.packed) {
+            const sgi_gbuffer_packed& k = *g.packed;
+            if (k.width != g.width || k.height != g.height || !g.depth || !k.normals || !k.albedo ||
+                !k.baseReflectivity || !k.roughnessMetallicAmbient)
+                return fail(SGI_E_INVALID, "sgi_views_set: a view's packed attachments are NULL or of another size");
+        } else if (!g.depth || !g.normal || !g.albedo || !g.baseReflectivity || !g.roughnessMetallicAmbient) {
             return fail(SGI_E_INVALID, "sgi_views_set: a view's G-buffer plane is NULL");
+        }
         if (!x.tiles || !x.direct || !x.color || !x.reservoir || !x.combined || !x.giOut || !x.reservoirOut ||
             !x.historyOut)
             return fail(SGI_E_INVALID, "sgi_views_set: a view's buffer is NULL");

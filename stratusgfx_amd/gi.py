@@ -150,6 +150,11 @@ class GiPass:
                   ctypes.byref(params) if params is not None else None,
                   _dev(out, "out", n(out), (_I32, "uint32", _U8)), _stream(stream))
 
+    def gbuffer_pack(self, planes: _abi.Gbuffer, out: _abi.GbufferPacked, stream=None) -> None:
+        """The geometry pass's attachment writes: planes' FP32 normal / albedo / reflectivity /
+        roughness / velocity into the packed attachments `out` names."""
+        _abi.call(self.lib, "sgi_gbuffer_pack", self.ctx, ctypes.byref(planes), ctypes.byref(out), _stream(stream))
+
     def gbuffer_unpack(self, packed: _abi.GbufferPacked, normal, albedo, refl, rma, velocity=None,
                        stream=None) -> None:
         """The reference's packed G-buffer attachments into the FP32 planes the passes read."""

@@ -141,7 +141,14 @@ class Inputs:
     def gbuffer(self, ssao=None) -> _abi.Gbuffer:
         p = self.ptr
         return _abi.Gbuffer(self.width, self.height, p(self.depth), p(self.normal), p(self.albedo),
-                            p(self.refl), p(self.rma), p(self.velocity), p(self.ids), p(ssao))
+                            p(self.refl), p(self.rma), p(self.velocity), p(self.ids), p(ssao), None)
+
+    def gbuffer_packed(self, packed: "PackedAttachments", ssao=None) -> _abi.Gbuffer:
+        """This view's G-buffer read from packed attachments (the FP32 planes of the normals,
+        albedo, reflectivity, roughness and velocity left out): depth, ids and SSAO stay FP32."""
+        p = self.ptr
+        return _abi.Gbuffer(self.width, self.height, p(self.depth), None, None, None, None, None, p(self.ids),
+                            p(ssao), ctypes.addressof(packed.struct))
 
     def cubes(self) -> _abi.CubeAtlases:
         c = _abi.CubeAtlases()
@@ -317,3 +324,31 @@ def generate_view_device(scene: Scene, shared: Inputs, device="cuda", stream=Non
     _abi.call(lib, "sgi_scene_csm_device", h, inp.csm_depth.data_ptr(), s)
     torch.cuda.synchronize()
     return inp
+
+
+class PackedAttachments:
+    """The reference's G-buffer attachments on the device (sgi_gbuffer_packed: RGB8 normals, RGBA8
+    albedo, RG8 reflectivity, RGB8 roughness / metallic / ambient, RG16F velocity), written from a
+    view's FP32 planes by sgi_gbuffer_pack (the geometry pass's attachment writes)."""
+
+    def __init__(self, width: int, height: int, velocity: bool = True, device="cuda"):
+        import torch
+
+        i32 = dict(dtype=torch.int32, device=device)
+        self.width, self.height = width, height
+        self.normals = torch.zeros((height, width), **i32)
+        self.albedo = torch.zeros((height, width), **i32)
+        self.refl = torch.zeros((height, width), dtype=torch.int16, device=device)
+        self.rma = torch.zeros((height, width), **i32)
+        self.velocity = torch.zeros((height, width), **i32) if velocity else None
+        self.struct = _abi.GbufferPacked(width, height, self.normals.data_ptr(), self.albedo.data_ptr(),
+                                         self.refl.data_ptr(), self.rma.data_ptr(),
+                                         self.velocity.data_ptr() if velocity else None)
+
+    @classmethod
+    def pack(cls, gi, inp: Inputs, stream=None) -> "PackedAttachments":
+        """Attachments holding inp's FP32 planes as the geometry pass stores them (gi: a GiPass of
+        the view's size)."""
+        a = cls(inp.width, inp.height, inp.velocity is not None)
+        gi.gbuffer_pack(inp.gbuffer(), a.struct, stream=stream)
+        return a

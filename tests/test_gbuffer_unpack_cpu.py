@@ -75,3 +75,55 @@ def test_null_velocity_writes_nothing():
     z16 = np.zeros((H, W), np.uint16)
     n, a, r, m, v = orc.gbuffer_unpack(_packed(H, W, z32, z32, z16, z32, None), with_velocity=False)
     assert np.all(v == 0.0) and np.all(n == 0.0)
+
+
+def test_pack_codes_and_clamps():
+    """The oracle's attachment writes (orc_gbuffer_pack): the UNORM8 code of v is floor(v * 255 + 0.5)
+    after clamping to [0, 1] (numpy's float32 arithmetic as an independent check; NaN and negatives
+    store 0, values above 1 store 255), RGB8 texels keep byte 3 zero, RG8 packs r in the low byte,
+    velocity is numpy's float32 -> float16 rounding (to nearest, ties to even)."""
+    rng = np.random.default_rng(2)
+    H, W = 3, 257
+    vals = rng.uniform(-0.2, 1.2, (H, W, 4)).astype(np.float32)
+    vals[0, :256, :] = (np.arange(256, dtype=np.float32) / np.float32(255))[:, None]  # every code k / 255
+    vals[1, :8, 0] = [np.nan, -1.0, 0.0, 1.0, 2.0, 0.5, 1.0 / 510, 3.0 / 510]
+    vel = rng.normal(0, 0.01, (H, W, 2)).astype(np.float32)
+    vel[2, :4, 0] = [np.float32(1.0 / 3.0), np.float32(65520.0), np.float32(-0.0), np.float32(6.0e-8)]
+    n8, a8, f8, m8, v16 = orc.gbuffer_pack(vals[..., :3], vals, vals[..., :2], vals[..., 1:4], vel)
+
+    def code(v):
+        c = np.where(np.isnan(v), np.float32(0), np.clip(v, np.float32(0), np.float32(1))).astype(np.float32)
+        return np.floor(c * np.float32(255) + np.float32(0.5)).astype(np.uint32)
+
+    byte = lambda t, b: (t >> np.uint32(8 * b)) & np.uint32(0xFF)  # noqa: E731
+    for c in range(3):
+        assert np.array_equal(byte(n8, c), code(vals[..., c]))
+        assert np.array_equal(byte(m8, c), code(vals[..., 1 + c]))
+    assert np.all(byte(n8, 3) == 0) and np.all(byte(m8, 3) == 0)
+    for c in range(4):
+        assert np.array_equal(byte(a8, c), code(vals[..., c]))
+    f8 = f8.astype(np.uint32)
+    assert np.array_equal(f8 & 0xFF, code(vals[..., 0])) and np.array_equal(f8 >> 8, code(vals[..., 1]))
+    assert np.array_equal(byte(n8[0, :256], 0), np.arange(256))  # k / 255 stores k
+    assert list(byte(n8[1, :8], 0)) == [0, 0, 0, 255, 255, 128, 1, 2]  # 0.5 -> 127.5 + 0.5; 1/510 -> 1
+    with np.errstate(over="ignore"):  # 65520 rounds to +inf in binary16
+        want16 = vel.astype(np.float16)
+    assert np.array_equal(v16.view(np.float16), want16) and np.isinf(want16[2, 1, 0])
+
+
+def test_pack_then_unpack_round_trip():
+    """unpack(pack(x)) is x to within half a code (1 / 510) after clamping; pack(unpack(codes)) gives
+    the codes back; velocity survives to binary16 precision."""
+    rng = np.random.default_rng(5)
+    H, W = 16, 33
+    planes = [rng.uniform(-0.1, 1.1, (H, W, c)).astype(np.float32) for c in (3, 4, 2, 3)]
+    vel = rng.normal(0, 0.05, (H, W, 2)).astype(np.float32)
+    n8, a8, f8, m8, v16 = orc.gbuffer_pack(*planes, vel)
+    p = _packed(H, W, n8, a8, f8, m8, v16)
+    back = orc.gbuffer_unpack(p)
+    for x, y in zip(planes, back[:4]):
+        assert np.abs(np.clip(x, 0, 1) - y).max() <= 1.0 / 510 + 1e-7
+    assert np.array_equal(back[4], vel.astype(np.float16).astype(np.float32))
+    again = orc.gbuffer_pack(*back[:4], back[4])
+    for a, b in zip((n8, a8, f8, m8, v16), again):
+        assert np.array_equal(a, b)
