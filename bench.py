@@ -807,7 +807,7 @@ def main():
     # denoiser differ), and on the 1024-VPL scene (whose own converged render sits
     # `converged_1024_vs_4096_relmse` from the target)
     weights_path = kpcn.DEFAULT_WEIGHTS
-    learned_res = learned_1024 = None
+    learned_res = learned_1024 = learned_packed = None
     learned_sweep = []
     learned_1024_sweep = []
     el_learned = None
@@ -904,6 +904,21 @@ def main():
               file=sys.stderr, flush=True)
         learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup, P_op)
         learned_res.update(atrous_at_equal(sweep, learned_res))
+        # the same form reading the reference's packed G-buffer attachments (sgi_gbuffer.packed, 22 B
+        # per pixel instead of the FP32 planes' 60), written once by sgi_gbuffer_pack outside the
+        # timed frames; the relMSE against the same converged render (the attachments quantise
+        # normals, albedo and material to 8 bits, as the reference's G-buffer does)
+        packed_att = sc.PackedAttachments.pack(plain.gi, plain.dev)
+        g_fp32 = plain.g
+        plain.g = plain.dev.gbuffer_packed(packed_att)
+        try:
+            packed_res, _ = learned_form(plain, "4096 VPLs, packed G-buffer", S_op, args.sweep_steps, 2, P_op)
+        finally:
+            plain.g = g_fp32
+        learned_packed = {k: packed_res[k] for k in ("frames_per_s", "ms_per_step", "relmse", "kernels_ms",
+                                                     "fallback_tiles")}
+        learned_packed["note"] = ("every pass reads RGB8 normals, RGBA8 albedo, RG8 reflectivity, RGB8 "
+                                  "roughness / metallic and RG16F velocity directly; depth FP32")
         # the 1024-VPL scene: each sample count against the 4096-VPL a-trous sweep at equal relMSE to
         # the 4096-VPL converged render (whose distance from the 1024-VPL converged render,
         # converged_1024_vs_4096_relmse, floors the learned form's relMSE), and, each form against its
@@ -986,6 +1001,7 @@ def main():
                    "converged_1024_vs_4096_relmse": round(floor_rel, 6)},
         "learned": learned_res,
         "learned_sweep": learned_sweep if learned_res is not None else [],
+        "learned_packed_gbuffer": learned_packed,
         "learned_1024_vpls": learned_1024,
         "learned_1024_sweep": learned_1024_sweep,
         "atrous_sweep": sweep,

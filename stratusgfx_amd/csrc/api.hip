@@ -675,7 +675,8 @@ struct KpcnWs {
     float* rad[SGI_KPCN_LEVELS];  // the radiance pyramid (level 0 = the noisy radiance)
     float* o[SGI_KPCN_LEVELS];    // the reconstruction per level (level 0 = the output)
     float* err;
-    uint8_t *flags, *dilated;
+    uint8_t* flags;
+    int32_t* dilatedList;  // [1 + tiles]: count, then tile indices
     int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
 };
 
@@ -711,7 +712,7 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
     }
     char* err = take(P * 4);
     char* flags = take(tiles);
-    char* dil = take(tiles);
+    char* dilList = take((1 + tiles) * 4);
     if (ws) {
         ws->feat = (uint16_t*)feat;
         ws->rad[0] = (float*)rad0;
@@ -723,7 +724,7 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
         for (int l = 0; l < SGI_KPCN_LEVELS; ++l) ws->o[l] = (float*)outs[l];
         ws->err = (float*)err;
         ws->flags = (uint8_t*)flags;
-        ws->dilated = (uint8_t*)dil;
+        ws->dilatedList = (int32_t*)dilList;
     }
     return off;
 }
@@ -797,15 +798,18 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));
         KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
-                                                 combined, giOut, reservoirOut, historyOut, s));
+                                                 combined, giOut, reservoirOut, historyOut, nullptr, nullptr, s));
     } else {
-        KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilated, fallbackTiles, s));
-        // a-trous over the flagged tiles (and their margin), the network elsewhere
+        KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilatedList, fallbackTiles, s));
+        // the a-trous passes over the flagged tiles and their margin (on a tile list: their cost
+        // follows the flagged tiles), then one temporal pass: the a-trous result in the flagged
+        // tiles, the network's output in every other tile (which also overwrites the margin's
+        // reservoir)
         KPCN_STEP(sgi_launch::denoiseSpatial(c, fr, visibleIndices, reservoir, reservoirOut, dws,
-                                             ws.dilated, false, s));
-        KPCN_STEP(sgi_kpcn::select(ws.o[0], ws.flags, H, W, reservoirOut, s));
-        KPCN_STEP(sgi_launch::denoiseTemporal(c, p, fr, screen, color, reservoirOut, prevGi, prevHistory,
-                                              combined, giOut, historyOut, dws, s));
+                                             ws.dilatedList, false, s));
+        KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
+                                                 combined, giOut, reservoirOut, historyOut, ws.flags, dws.normals,
+                                                 s));
     }
 #undef KPCN_STEP
     return e;

@@ -72,22 +72,32 @@ __device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C,
     return t[((size_t)j * W + i) * C + c];
 }
 
-// mask (per 16x16 tile, may be NULL): pixels of unmarked tiles are skipped
-// (the decoded normals are still written: the temporal pass reads them all).
-__device__ __forceinline__ bool maskedOut(const uint8_t* mask, int W, int x, int y) {
-    return mask && !mask[(y >> 4) * ((W + 15) >> 4) + (x >> 4)];
+// A tile list (int32 [1 + n]: the count, then 16x16 tile indices, row-major
+// over ceil(W / 16) tiles per row): the blocks of a fixed grid take its tiles
+// in turn, one pixel per thread; an empty list costs one load per block.
+constexpr int kListGrid = 1024;
+template <class F>
+__device__ __forceinline__ void forListedTiles(const int32_t* __restrict__ list, int W, F body) {
+    const int n = list[0], tilesX = (W + 15) >> 4;
+    for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
+        const int t = list[1 + i];
+        body((t % tilesX) * 16 + (int)threadIdx.x % 16, (t / tilesX) * 16 + (int)threadIdx.x / 16);
+    }
+}
+
+// flags (per 16x16 tile, may be NULL): whether pixel (x, y)'s tile is marked.
+__device__ __forceinline__ bool flagged(const uint8_t* flags, int W, int x, int y) {
+    return flags && flags[(y >> 4) * ((W + 15) >> 4) + (x >> 4)];
 }
 
 __device__ __forceinline__ void mergeBody(const GbufArgs& g, const FrameArgs& fr, const int32_t* __restrict__ visible,
                                           const float4* __restrict__ resIn, float4* __restrict__ resOut,
-                                          float4* __restrict__ normals, const uint8_t* __restrict__ mask, int x,
-                                          int y) {
+                                          float4* __restrict__ normals, int x, int y) {
     if (x >= g.W || y >= g.H) return;
     const size_t p = (size_t)y * g.W + x;
     const f3 centerNormal = gNormal(g, p);
     const float centerDepth = g.depth[p];
     normals[p] = make_float4(centerNormal.x, centerNormal.y, centerNormal.z, centerDepth);
-    if (maskedOut(mask, g.W, x, y)) return;
     const int numVisible = visible[0];
     float cr0 = 0.0f, cr1 = 0.0f, cr2 = 0.0f, cr3 = 0.0f;
     const float depthCutoff = 0.1f * centerDepth;
@@ -114,25 +124,31 @@ __device__ __forceinline__ void mergeBody(const GbufArgs& g, const FrameArgs& fr
 
 __global__ __launch_bounds__(256) void denoiseMergeKernel(GbufArgs g, FrameArgs fr, const int32_t* __restrict__ visible,
                                                           const float4* __restrict__ resIn, float4* __restrict__ resOut,
-                                                          float4* __restrict__ normals,
-                                                          const uint8_t* __restrict__ mask) {
+                                                          float4* __restrict__ normals) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
-    mergeBody(g, fr, visible, resIn, resOut, normals, mask, x, y);
+    mergeBody(g, fr, visible, resIn, resOut, normals, x, y);
+}
+
+__global__ __launch_bounds__(256) void denoiseMergeListKernel(GbufArgs g, FrameArgs fr,
+                                                              const int32_t* __restrict__ visible,
+                                                              const float4* __restrict__ resIn,
+                                                              float4* __restrict__ resOut, float4* __restrict__ normals,
+                                                              const int32_t* __restrict__ list) {
+    forListedTiles(list, g.W, [&](int x, int y) { mergeBody(g, fr, visible, resIn, resOut, normals, x, y); });
 }
 
 __global__ __launch_bounds__(256) void denoiseMergeViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
                                                                int numViews, int total) {
     int x, y;
     const sgi_batch::ViewDesc& d = viewBlock(views, numViews, total, x, y);
-    mergeBody(d.g, d.fr, d.visible, (const float4*)d.reservoir, d.resA, d.normals, nullptr, x, y);
+    mergeBody(d.g, d.fr, d.visible, (const float4*)d.reservoir, d.resA, d.normals, x, y);
 }
 
 __device__ __forceinline__ void filterBody(const GbufArgs& g, int stride, const float4* __restrict__ resIn,
-                                           const float4* __restrict__ normals, float4* __restrict__ resOut,
-                                           const uint8_t* __restrict__ mask, int x, int y) {
+                                           const float4* __restrict__ normals, float4* __restrict__ resOut, int x,
+                                           int y) {
     if (x >= g.W || y >= g.H) return;
-    if (maskedOut(mask, g.W, x, y)) return;
     const int W = g.W, H = g.H;
     const size_t p = (size_t)y * W + x;
     const float4 cn4 = normals[p];
@@ -169,10 +185,17 @@ __device__ __forceinline__ void filterBody(const GbufArgs& g, int stride, const 
 
 __global__ __launch_bounds__(256) void denoiseFilterKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
                                                            const float4* __restrict__ normals,
-                                                           float4* __restrict__ resOut, const uint8_t* __restrict__ mask) {
+                                                           float4* __restrict__ resOut) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
-    filterBody(g, stride, resIn, normals, resOut, mask, x, y);
+    filterBody(g, stride, resIn, normals, resOut, x, y);
+}
+
+__global__ __launch_bounds__(256) void denoiseFilterListKernel(GbufArgs g, int stride, const float4* __restrict__ resIn,
+                                                               const float4* __restrict__ normals,
+                                                               float4* __restrict__ resOut,
+                                                               const int32_t* __restrict__ list) {
+    forListedTiles(list, g.W, [&](int x, int y) { filterBody(g, stride, resIn, normals, resOut, x, y); });
 }
 
 // pass 1: resA -> resB, pass 2: resB -> resA, pass 3: resA -> reservoirOut
@@ -182,7 +205,7 @@ __global__ __launch_bounds__(256) void denoiseFilterViewsKernel(const sgi_batch:
     const sgi_batch::ViewDesc& d = viewBlock(views, numViews, total, x, y);
     const float4* in = pass == 2 ? d.resB : d.resA;
     float4* out = pass == 1 ? d.resB : (pass == 2 ? d.resA : (float4*)d.reservoirOut);
-    filterBody(d.g, 1 << pass, in, d.normals, out, nullptr, x, y);
+    filterBody(d.g, 1 << pass, in, d.normals, out, x, y);
 }
 
 // The final pass's temporal blend and combine (vpl_pbr_gi_denoise.fs:336-434) on
@@ -278,9 +301,16 @@ __global__ __launch_bounds__(256) void denoiseTemporalNetKernel(GbufArgs g, Gbuf
                                                                 float* __restrict__ combined,
                                                                 float* __restrict__ giOut,
                                                                 float4* __restrict__ reservoirOut,
-                                                                float* __restrict__ historyOut) {
+                                                                float* __restrict__ historyOut,
+                                                                const uint8_t* __restrict__ fallback,
+                                                                const float4* __restrict__ normals) {
     int x, y;
     blockPixel(g.W, g.H, x, y);
+    if (flagged(fallback, g.W, x, y)) {  // the a-trous result already in reservoirOut, its decoded normals
+        temporalBody(g, prev, fr, reservoirOut, normals, screen, color, prevGi, prevHistory, combined, giOut,
+                     historyOut, x, y);
+        return;
+    }
     temporalBody<true>(g, prev, fr, nullptr, nullptr, screen, color, prevGi, prevHistory, combined, giOut, historyOut,
                        x, y, net, reservoirOut);
 }
@@ -357,37 +387,47 @@ hipError_t taa(const float* screen, const float* prev, const float* velocity, in
 }
 
 hipError_t denoiseSpatial(const GbufArgs& cur, const FrameArgs& fr, const int32_t* visible, const float* reservoir,
-                          float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask, bool mergeOnly,
+                          float* reservoirOut, const DenoiseBuffers& ws, const int32_t* tiles, bool mergeOnly,
                           hipStream_t s) {
-    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
-    hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
-                       ws.normals, mask);
+    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256), lgrid(kListGrid);
+    if (tiles)
+        hipLaunchKernelGGL(denoiseMergeListKernel, lgrid, block, 0, s, cur, fr, visible, (const float4*)reservoir,
+                           ws.resA, ws.normals, tiles);
+    else
+        hipLaunchKernelGGL(denoiseMergeKernel, grid, block, 0, s, cur, fr, visible, (const float4*)reservoir, ws.resA,
+                           ws.normals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || mergeOnly) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2, ws.resA, ws.normals, ws.resB, mask);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 4, ws.resB, ws.normals, ws.resA, mask);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 8, ws.resA, ws.normals, (float4*)reservoirOut,
-                       mask);
-    return hipGetLastError();
+    float4* const in[3] = {ws.resA, ws.resB, ws.resA};
+    float4* const out[3] = {ws.resB, ws.resA, (float4*)reservoirOut};
+    for (int pass = 0; pass < 3 && e == hipSuccess; ++pass) {
+        if (tiles)
+            hipLaunchKernelGGL(denoiseFilterListKernel, lgrid, block, 0, s, cur, 2 << pass, in[pass], ws.normals,
+                               out[pass], tiles);
+        else
+            hipLaunchKernelGGL(denoiseFilterKernel, grid, block, 0, s, cur, 2 << pass, in[pass], ws.normals,
+                               out[pass]);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t denoiseTemporal(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const float* screen,
                            const float* color, const float* filtered, const float* prevGi, const float* prevHistory,
                            float* combined, float* giOut, float* historyOut, const DenoiseBuffers& ws, hipStream_t s) {
-    const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
-    hipLaunchKernelGGL(denoiseTemporalKernel, grid, block, 0, s, cur, prev, fr, (const float4*)filtered, ws.normals,
-                       screen, color, prevGi, prevHistory, combined, giOut, historyOut);
+    hipLaunchKernelGGL(denoiseTemporalKernel, dim3(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), dim3(256), 0, s, cur,
+                       prev, fr, (const float4*)filtered, ws.normals, screen, color, prevGi, prevHistory, combined,
+                       giOut, historyOut);
     return hipGetLastError();
 }
 
 hipError_t denoiseTemporalNet(const GbufArgs& cur, const GbufArgs& prev, const FrameArgs& fr, const float* screen,
                               const float* color, const float* net, const float* prevGi, const float* prevHistory,
-                              float* combined, float* giOut, float* reservoirOut, float* historyOut, hipStream_t s) {
+                              float* combined, float* giOut, float* reservoirOut, float* historyOut,
+                              const uint8_t* fallback, const float4* normals, hipStream_t s) {
     const dim3 grid(((cur.W + 63) / 64) * ((cur.H + 3) / 4)), block(256);
     hipLaunchKernelGGL(denoiseTemporalNetKernel, grid, block, 0, s, cur, prev, fr, net, screen, color, prevGi,
-                       prevHistory, combined, giOut, (float4*)reservoirOut, historyOut);
+                       prevHistory, combined, giOut, (float4*)reservoirOut, historyOut, fallback, normals);
     return hipGetLastError();
 }
 

@@ -1267,7 +1267,8 @@ __global__ void weightCopiesKernel(const float* __restrict__ params, uint16_t* _
 
 // Per 16x16 tile: flag = mean exp(err) > threshold (every tile when threshold <= 0).
 __global__ __launch_bounds__(256) void tileFlagKernel(const float* __restrict__ err, int H, int W, int tilesX,
-                                                      float threshold, uint8_t* __restrict__ flags) {
+                                                      float threshold, uint8_t* __restrict__ flags,
+                                                      int32_t* __restrict__ dilatedList, int32_t* __restrict__ counts) {
     __shared__ float red[256];
     const int t = blockIdx.x;
     const int tx = t % tilesX, ty = t / tilesX;
@@ -1281,54 +1282,48 @@ __global__ __launch_bounds__(256) void tileFlagKernel(const float* __restrict__ 
     }
     if (threadIdx.x == 0) {
         const int n = (min(W, tx * 16 + 16) - tx * 16) * (min(H, ty * 16 + 16) - ty * 16);
-        flags[t] = (!(threshold > 0.0f) || red[0] / (float)n > threshold) ? 1 : 0;
+        const uint8_t f = (!(threshold > 0.0f) || red[0] / (float)n > threshold) ? 1 : 0;
+        flags[t] = f;
+        if (counts) counts[1 + t] = f;
+        if (t == 0) {  // the counters dilateListKernel accumulates into
+            dilatedList[0] = 0;
+            if (counts) counts[0] = 0;
+        }
     }
 }
 
-// counts[0] = number of flagged tiles, counts[1 + t] = flag of tile t (one block).
-__global__ __launch_bounds__(1024) void countFlagsKernel(const uint8_t* __restrict__ flags, int n,
-                                                         int32_t* __restrict__ counts) {
-    __shared__ int red[1024];
-    int c = 0;
-    for (int t = threadIdx.x; t < n; t += 1024) {
-        c += flags[t];
-        counts[1 + t] = flags[t];
+// Per tile (one thread each): whether a flagged tile lies within two tiles,
+// and if so the tile's index appended to dilatedList (one atomic per wave;
+// the list's order does not matter, every listed tile is processed on its
+// own); counts[0] (may be NULL) gathers the flagged tiles.  Both counters
+// start at 0 (tileFlagKernel's block 0 clears them).
+__global__ __launch_bounds__(256) void dilateListKernel(const uint8_t* __restrict__ flags, int tilesX, int tilesY,
+                                                        int32_t* __restrict__ dilatedList,
+                                                        int32_t* __restrict__ counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = tilesX * tilesY;
+    bool f = false, d = false;
+    if (t < n) {
+        f = flags[t] != 0;
+        const int tx = t % tilesX, ty = t / tilesX;
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int xx = tx + dx, yy = ty + dy;
+                if (xx >= 0 && xx < tilesX && yy >= 0 && yy < tilesY && flags[yy * tilesX + xx]) d = true;
+            }
     }
-    red[threadIdx.x] = c;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) counts[0] = red[0];
+    const int lane = (int)threadIdx.x & 63;
+    const uint64_t bd = __ballot(d), bf = __ballot(f);
+    int base = 0;
+    if (lane == 0 && bd) base = atomicAdd(&dilatedList[0], __popcll(bd));
+    base = __shfl(base, 0);
+    if (d) dilatedList[1 + base + __popcll(bd & ((1ull << lane) - 1ull))] = t;
+    if (counts && lane == 0 && bf) atomicAdd(&counts[0], __popcll(bf));
 }
 
 __global__ void clearCountsKernel(int n, int32_t* __restrict__ counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= n) counts[i] = 0;
-}
-
-__global__ void dilateKernel(const uint8_t* __restrict__ flags, int tilesX, int tilesY, uint8_t* __restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tilesX * tilesY) return;
-    const int tx = t % tilesX, ty = t / tilesX;
-    uint8_t f = 0;
-    for (int dy = -2; dy <= 2; ++dy)
-        for (int dx = -2; dx <= 2; ++dx) {
-            const int xx = tx + dx, yy = ty + dy;
-            if (xx >= 0 && xx < tilesX && yy >= 0 && yy < tilesY) f |= flags[yy * tilesX + xx];
-        }
-    out[t] = f;
-}
-
-// reservoirOut = flagged tile ? a-trous result (already there) : (network output, 1)
-__global__ void selectKernel(const float* __restrict__ net, const uint8_t* __restrict__ flags, int H, int W,
-                             int tilesX, float4* __restrict__ res) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W * H) return;
-    const int x = i % W, y = i / W;
-    if (flags[(y / 16) * tilesX + x / 16]) return;
-    res[i] = make_float4(net[3 * i], net[3 * i + 1], net[3 * i + 2], 1.0f);
 }
 
 int gridFor(int n, int b) { return (n + b - 1) / b; }
@@ -1537,31 +1532,21 @@ hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* bi
     return hipGetLastError();
 }
 
-hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, uint8_t* dilated,
+hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, int32_t* dilatedList,
                      int32_t* counts, hipStream_t s) {
-    const int tilesX = (W + 15) / 16, tilesY = (H + 15) / 16;
-    hipLaunchKernelGGL(tileFlagKernel, dim3(tilesX * tilesY), dim3(256), 0, s, err, H, W, tilesX, threshold, flags);
-    hipError_t e = hipGetLastError();
+    const int tilesX = (W + 15) / 16, tilesY = (H + 15) / 16, n = tilesX * tilesY;
+    hipLaunchKernelGGL(tileFlagKernel, dim3(n), dim3(256), 0, s, err, H, W, tilesX, threshold, flags, dilatedList,
+                       counts);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(dilateKernel, dim3(gridFor(tilesX * tilesY, 256)), dim3(256), 0, s, flags, tilesX, tilesY,
-                       dilated);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (counts) {
-        hipLaunchKernelGGL(countFlagsKernel, dim3(1), dim3(1024), 0, s, flags, tilesX * tilesY, counts);
-        e = hipGetLastError();
-    }
-    return e;
+    hipLaunchKernelGGL(dilateListKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, flags, tilesX, tilesY, dilatedList,
+                       counts);
+    return hipGetLastError();
 }
 
 hipError_t clearCounts(int H, int W, int32_t* counts, hipStream_t s) {
     const int n = ((W + 15) / 16) * ((H + 15) / 16);
     hipLaunchKernelGGL(clearCountsKernel, dim3(gridFor(n + 1, 256)), dim3(256), 0, s, n, counts);
-    return hipGetLastError();
-}
-
-hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s) {
-    hipLaunchKernelGGL(selectKernel, dim3(gridFor(W * H, 256)), dim3(256), 0, s, net, flags, H, W, (W + 15) / 16,
-                       (float4*)res);
     return hipGetLastError();
 }
 

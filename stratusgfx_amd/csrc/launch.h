@@ -99,11 +99,13 @@ hipError_t gatherViews(const sgi_batch::ViewDesc* views, int numViews, int total
 hipError_t directLightViews(const sgi_batch::ViewDesc* views, int numViews, int totalTiles16, hipStream_t s);
 hipError_t denoiseViews(const sgi_batch::ViewDesc* views, int numViews, int totalBlocks, hipStream_t s);
 
-// The merge and the three filter passes into reservoirOut; mask (per 16x16
-// tile, NULL = all) limits them to the marked tiles; mergeOnly stops after the
-// merge (whose decoded normals the temporal pass reads).
+// The merge and the three filter passes into reservoirOut; tiles (a device
+// list of 16x16 tiles: int32 count, then tile indices; NULL = the whole
+// frame) limits them, and the merge's decoded normals, to the listed tiles;
+// mergeOnly stops after the merge (whose decoded normals the temporal pass
+// reads).
 hipError_t denoiseSpatial(const sgi_dev::GbufArgs& cur, const sgi_dev::FrameArgs& fr, const int32_t* visible,
-                          const float* reservoir, float* reservoirOut, const DenoiseBuffers& ws, const uint8_t* mask,
+                          const float* reservoir, float* reservoirOut, const DenoiseBuffers& ws, const int32_t* tiles,
                           bool mergeOnly, hipStream_t s);
 
 // The temporal blend and combine on a filtered reservoir (normals from denoiseSpatial).
@@ -111,13 +113,15 @@ hipError_t denoiseTemporal(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs
                            const float* screen, const float* color, const float* filtered, const float* prevGi,
                            const float* prevHistory, float* combined, float* giOut, float* historyOut,
                            const DenoiseBuffers& ws, hipStream_t s);
-// The temporal pass on the learned denoiser's output (net, FP32 [h][w][3]) when
-// no tile falls back: also writes reservoirOut = (net, 1); decodes the centre
-// normals itself (no merge pass).
+// The temporal pass on the learned denoiser's output (net, FP32 [h][w][3]):
+// also writes reservoirOut = (net, 1); decodes the centre normals itself (no
+// merge pass).  fallback (per 16x16 tile, NULL = none): the marked tiles take
+// the a-trous result already in reservoirOut instead, with the decoded
+// normals of denoiseSpatial (the temporal pass of sgi_denoise).
 hipError_t denoiseTemporalNet(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs& prev, const sgi_dev::FrameArgs& fr,
                               const float* screen, const float* color, const float* net, const float* prevGi,
                               const float* prevHistory, float* combined, float* giOut, float* reservoirOut,
-                              float* historyOut, hipStream_t s);
+                              float* historyOut, const uint8_t* fallback, const float4* normals, hipStream_t s);
 
 // sgi_denoise_learned's work (api.hip) on checked arguments: workspace of
 // sgi_kpcn_workspace_bytes(cur.H, cur.W) bytes, dws the a-trous planes of the
@@ -186,10 +190,14 @@ hipError_t loss(const float* out, const float* logits, const float* target, cons
 hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
                 int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
 hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
-hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, uint8_t* dilated,
+// Per 16x16 tile: flags (mean exp(err) > threshold, or every tile when
+// threshold <= 0), the list of the flagged tiles grown by two tiles (the
+// a-trous passes' 28-pixel reach; int32 [1 + tiles]: count, then the tile
+// indices in no particular order) and, when counts is set, the sgi_denoise_learned
+// fallbackTiles record (count, then one flag per tile).
+hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, int32_t* dilatedList,
                      int32_t* counts, hipStream_t s);
 // counts[0] = 0 and counts[1 + t] = 0 for every tile (no tile falls back).
 hipError_t clearCounts(int H, int W, int32_t* counts, hipStream_t s);
-hipError_t select(const float* net, const uint8_t* flags, int H, int W, float* res, hipStream_t s);
 
 }  // namespace sgi_kpcn
