@@ -58,10 +58,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gather", choices=["tiles", "random"], default="tiles", help="the plain form's gather")
-    ap.add_argument("--learned-spp", type=int, nargs="+", default=[8, 12, 16, 24],
+    ap.add_argument("--learned-spp", type=int, nargs="+", default=[8, 12, 16, 24, 32],
                     help="gather samples per pixel of the learned form (the operating point is chosen among them)")
     ap.add_argument("--atrous-spp", type=int, nargs="+",
-                    default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192],
+                    default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256, 320],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
     ap.add_argument("--rejection-limit", type=int, default=-1,
@@ -807,7 +807,7 @@ def main():
     # denoiser differ), and on the 1024-VPL scene (whose own converged render sits
     # `converged_1024_vs_4096_relmse` from the target)
     weights_path = kpcn.DEFAULT_WEIGHTS
-    learned_res = learned_1024 = learned_packed = None
+    learned_res = learned_1024 = learned_packed = learned_no_fallback = None
     learned_sweep = []
     learned_1024_sweep = []
     el_learned = None
@@ -830,11 +830,12 @@ def main():
         n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
         fbt = torch.zeros((1 + n_tiles,), dtype=torch.int32, device="cuda")
 
-        def learned_form(state, label, S, steps, warmup, precision, own_target=None):
+        def learned_form(state, label, S, steps, warmup, precision, own_target=None, form_threshold=None):
+            thr_form = threshold if form_threshold is None else form_threshold
             net = nets[precision]
             ws = state.gi.kpcn_workspace()
 
-            def learned_step(k, ev, evd, hist=True, thr=threshold):
+            def learned_step(k, ev, evd, hist=True, thr=thr_form):
                 p, fb = state, state.fb
                 prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
                 out_gi, out_h = p.hist[(k + 1) % 2]
@@ -874,7 +875,7 @@ def main():
                 "frames_per_s": round(steps / el, 2), "ms_per_step": round(el * 1e3 / steps, 4),
                 "relmse": round(rel, 6), "relmse_without_fallback": round(rel_nf, 6),
                 "relmse_noisy_input": round(noisy, 6), "relmse_atrous_on_same_input": round(atrous_same, 6),
-                "fallback_threshold": threshold, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
+                "fallback_threshold": thr_form, "fallback_tiles": fallback_tiles, "tiles": n_tiles,
                 "kernels_ms": stage_ms(["cull_global", "gather", "denoise_learned"], per, direct_ms),
                 "vpls": state.w.num_vpls, "visible_vpls": int(fb.visible[0].item()),
                 "samples_per_pixel": S, "network_precision": precision,
@@ -904,6 +905,13 @@ def main():
               file=sys.stderr, flush=True)
         learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup, P_op)
         learned_res.update(atrous_at_equal(sweep, learned_res))
+        # the operating point with the fallback switched off (threshold +inf): the fallback's own cost
+        nofb, _ = learned_form(plain, "4096 VPLs, no fallback", S_op, args.sweep_steps, 2, P_op,
+                               form_threshold=float("inf"))
+        nofb.update(atrous_at_equal(sweep, nofb))
+        learned_no_fallback = {k: nofb[k] for k in ("frames_per_s", "relmse", "kernels_ms", "fallback_threshold")
+                               if k in nofb}
+        learned_no_fallback["speedup_vs_atrous_at_equal_relmse"] = nofb.get("speedup_vs_atrous_at_equal_relmse")
         # the same form reading the reference's packed G-buffer attachments (sgi_gbuffer.packed, 22 B
         # per pixel instead of the FP32 planes' 60), written once by sgi_gbuffer_pack outside the
         # timed frames; the relMSE against the same converged render (the attachments quantise
@@ -1001,6 +1009,7 @@ def main():
                    "converged_1024_vs_4096_relmse": round(floor_rel, 6)},
         "learned": learned_res,
         "learned_sweep": learned_sweep if learned_res is not None else [],
+        "learned_without_fallback": learned_no_fallback,
         "learned_packed_gbuffer": learned_packed,
         "learned_1024_vpls": learned_1024,
         "learned_1024_sweep": learned_1024_sweep,
