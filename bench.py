@@ -26,17 +26,23 @@ swept a-trous points that bracket the learned relMSE; both bracketing points are
 reported beside it.  The learned form runs with the BF16 network and with its
 FP8 (E4M3) inference form when that file exists; the operating point is the
 (precision, sample count) with the highest ratio of frames/s at equal relMSE,
-then timed over exactly --steps frames, and `value` is its frames/s.  The learned form is also run on the 1024-VPL scene
-(same geometry), whose own converged render lies
-`converged_1024_vs_4096_relmse` from the target.  relMSE = mean of
+then timed over exactly --steps frames, and `value` is its frames/s.  Every
+form's frames/s comes from its frame captured in two hipGraphs (one per
+history buffer set) and replayed, checked bit-equal to the eager frame first;
+the per-stage `kernels_ms` come from an eager run with HIP events between the
+stages.  The learned form is also run on the 1024-VPL scene (same geometry),
+whose own converged render lies `converged_1024_vs_4096_relmse` from the
+target, and at the operating point without the fallback and on the packed
+G-buffer attachments.  relMSE = mean of
 (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after one frame, against
 the converged render (every visible VPL of the 4096-VPL scene, FP64 sums).
 Inputs are
 generated on the GPU before the timed regions; the denoiser history alternates
 between two buffer sets as the reference swaps its FBOs.
 
-Prints one JSON line.  With the default flags it also times the CPU oracle on
-a bounded sample of the tile-list frame as the CPU baseline.
+Prints one JSON line.  With the default flags it also times the CPU oracle
+(16 host threads) on a bounded sample of the a-trous frame at the learned
+operating point's equal-relMSE sample count as the CPU baseline.
 """
 import argparse
 import json
@@ -312,6 +318,46 @@ def timed_loop(step, steps, warmup, nstages):
     per = [sum(events[k][i].elapsed_time(events[k][i + 1]) for k in range(steps)) / steps for i in range(nstages)]
     direct = sum(side[k][0].elapsed_time(side[k][1]) for k in range(steps)) / steps
     return elapsed, per, direct
+
+
+def graph_loop(step, steps, warmup, state, outputs):
+    """The frame captured in two hipGraphs (torch.cuda.graph; frame parity 0 and 1 alternate the two
+    history buffer sets, as the eager loop does) and replayed: warmup replays, then exactly `steps`
+    replays between two synchronisations; returns the elapsed seconds.  One launch per frame, so the
+    host's per-call argument checks stay out of the frame time.  Before timing, frame 0 runs eagerly
+    and then from its graph on the same state (`state`: the tensors frame 0 reads and no frame
+    rewrites before it, restored in between) and `outputs` must come out bit-equal."""
+    import torch
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graphs = []
+    for par in range(2):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            step(par, None, None)
+        graphs.append(g)
+    torch.cuda.synchronize()
+    saved = [t.clone() for t in state]
+    step(0, None, None)
+    torch.cuda.synchronize()
+    ref = [t.clone() for t in outputs]
+    for t, v in zip(state, saved):
+        t.copy_(v)
+    torch.cuda.synchronize()
+    graphs[0].replay()
+    torch.cuda.synchronize()
+    if not all(torch.equal(a, b) for a, b in zip(outputs, ref)):
+        raise SystemExit("bench.py: a graph replay differs from the eager frame")
+    for k in range(warmup):
+        graphs[k % 2].replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        graphs[(warmup + k) % 2].replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    del graphs
+    return elapsed
 
 
 def stage_ms(names, per, direct):
@@ -731,7 +777,9 @@ def main():
     el_plain, per_plain = None, None
     if args.forms in ("all", "tiles"):
         print(f"[bench] timing {args.steps} plain frames", file=sys.stderr, flush=True)
-        el_plain, per_plain, direct_plain = timed_loop(plain_step, args.steps, args.warmup, 4)
+        _, per_plain, direct_plain = timed_loop(plain_step, args.steps, args.warmup, 4)
+        el_plain = graph_loop(plain_step, args.steps, args.warmup, list(plain.hist[0]),
+                              [plain.hist[1][0], plain.fb.combined])
     tiles_np = plain.fb.tiles.cpu().numpy()
     visible_plain = int(plain.fb.visible[0].item())
 
@@ -782,7 +830,8 @@ def main():
         torch.cuda.synchronize()
         rm = rel_mse(g_r, target)
         draws = float(plain.fb.reservoir[..., 3].double().mean())
-        el_r, per_r, direct_r = timed_loop(random_step, args.sweep_steps, 2, 3)
+        _, per_r, direct_r = timed_loop(random_step, args.sweep_steps, 2, 3)
+        el_r = graph_loop(random_step, args.sweep_steps, 2, list(plain.hist[0]), [plain.hist[1][0], plain.fb.combined])
         random_res = {"relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_r, 2),
                       "mean_reservoir_weight": round(draws, 3),
                       "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_r, direct_r),
@@ -796,7 +845,8 @@ def main():
         g_s = st(0, None, None, hist=False)
         torch.cuda.synchronize()
         rm = rel_mse(g_s, target)
-        el_s, per_s, direct_s = timed_loop(st, args.sweep_steps, 2, 3)
+        _, per_s, direct_s = timed_loop(st, args.sweep_steps, 2, 3)
+        el_s = graph_loop(st, args.sweep_steps, 2, list(plain.hist[0]), [plain.hist[1][0], plain.fb.combined])
         sweep.append({"spp": S, "relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_s, 2),
                       "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_s, direct_s)})
         print(f"[bench] a-trous at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
@@ -870,7 +920,8 @@ def main():
             torch.cuda.synchronize()
             atrous_same = rel_mse(tmp[1], target)
             del tmp
-            el, per, direct_ms = timed_loop(learned_step, steps, warmup, 3)
+            _, per, direct_ms = timed_loop(learned_step, steps, warmup, 3)
+            el = graph_loop(learned_step, steps, warmup, list(state.hist[0]), [state.hist[1][0], state.fb.combined])
             res = {
                 "frames_per_s": round(steps / el, 2), "ms_per_step": round(el * 1e3 / steps, 4),
                 "relmse": round(rel, 6), "relmse_without_fallback": round(rel_nf, 6),
