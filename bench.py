@@ -81,6 +81,9 @@ def parse():
     ap.add_argument("--forms", choices=["all", "learned", "tiles"], default="all",
                     help="run every form (default) or one of them alone (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
+    ap.add_argument("--no-learned-extras", action="store_true",
+                    help="time only the learned sweep and operating point (no 1024-VPL, packed G-buffer or "
+                         "no-fallback runs): counter runs then see only the 4096-VPL frame's kernels")
     ap.add_argument("--cpu-rows", type=int, default=120, help="image rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host threads of the CPU baseline (the oracle's rows split over them)")
@@ -176,7 +179,7 @@ def culling_roofline(per_plain, W, H, tiles, K, visible, num_vpls):
     out["cull_tiles"]["kernels"] = "cullTilesKernel<16>"
     out["note"] = ("neither stage is HBM-bound: the global cull is two launches of 64 and 16 waves over 4096 VPLs "
                    "(latency), the tile cull one wave per tile against every visible VPL (VALU; "
-                   "profiles/valu_counters_gather_samples24_city1080p.md)")
+                   "profiles/valu_counters_gather_samples_city1080p.md)")
     return out
 
 
@@ -956,6 +959,7 @@ def main():
               file=sys.stderr, flush=True)
         learned_res, el_learned = learned_form(plain, "4096 VPLs", S_op, args.steps, args.warmup, P_op)
         learned_res.update(atrous_at_equal(sweep, learned_res))
+    if learned_res is not None and not args.no_learned_extras:
         # the operating point with the fallback switched off (threshold +inf): the fallback's own cost
         nofb, _ = learned_form(plain, "4096 VPLs, no fallback", S_op, args.sweep_steps, 2, P_op,
                                form_threshold=float("inf"))
