@@ -375,9 +375,11 @@ _SIDE_STREAM = []
 
 def fork_direct(p, fb, evd):
     """Issues the direct-light pass on a second stream of default priority: it reads only the
-    G-buffer and the shadow cascades, so it runs beside global culling and the VPL gather; the
-    returned event is what the denoiser (the only consumer of the direct light) waits on. The
-    fork is recorded after the previous frame's denoiser, which read the same buffer."""
+    G-buffer and the shadow cascades, so it runs beside global culling, the VPL gather and the
+    denoiser's spatial passes. The pass context's screen event (sgi_set_screen_event) is recorded
+    after it: the denoiser waits for it just before its temporal pass, the only one that reads the
+    direct light. The fork is recorded after the previous frame's denoiser, which read the same
+    buffer."""
     import torch
     if not _SIDE_STREAM:
         _SIDE_STREAM.append(torch.cuda.Stream())
@@ -390,9 +392,10 @@ def fork_direct(p, fb, evd):
     p.gi.direct_light(p.g, fb.direct, stream=side.cuda_stream)
     if evd is not None:
         evd[1].record(side)
-    join = torch.cuda.Event()
-    join.record(side)
-    return join
+    if getattr(p, "screen_event", None) is None:
+        p.screen_event = torch.cuda.Event()
+    p.screen_event.record(side)
+    p.gi.set_screen_event(p.screen_event)
 
 
 class FormState:
@@ -759,7 +762,7 @@ def main():
         out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
-        direct_done = fork_direct(p, fb, evd)
+        fork_direct(p, fb, evd)
         p.cull()
         rec(1)
         if use_tiles:
@@ -768,7 +771,6 @@ def main():
         p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, fb.tiles if use_tiles else None,
                     fb.color, fb.reservoir)
         rec(3)
-        torch.cuda.current_stream().wait_event(direct_done)
         p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined, out_gi,
                      fb.reservoir_out, out_h)
         rec(4)
@@ -795,13 +797,12 @@ def main():
             out_gi, out_h = p.hist[(k + 1) % 2]
             rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
             rec(0)
-            direct_done = fork_direct(p, fb, evd)
+            fork_direct(p, fb, evd)
             p.cull()
             rec(1)
             p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                            fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S, rejection_limit=args.rejection_limit)
             rec(2)
-            torch.cuda.current_stream().wait_event(direct_done)
             p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                          out_gi, fb.reservoir_out, out_h)
             rec(3)
@@ -816,12 +817,11 @@ def main():
         out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
-        direct_done = fork_direct(p, fb, evd)
+        fork_direct(p, fb, evd)
         p.cull()
         rec(1)
         p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color, fb.reservoir)
         rec(2)
-        torch.cuda.current_stream().wait_event(direct_done)
         p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                      out_gi, fb.reservoir_out, out_h)
         rec(3)
@@ -894,14 +894,13 @@ def main():
                 out_gi, out_h = p.hist[(k + 1) % 2]
                 rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
                 rec(0)
-                direct_done = fork_direct(p, fb, evd)
+                fork_direct(p, fb, evd)
                 p.cull()
                 rec(1)
                 p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                                fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S,
                                rejection_limit=args.rejection_limit)
                 rec(2)
-                torch.cuda.current_stream().wait_event(direct_done)
                 p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net,
                                      thr, ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)
                 rec(3)

@@ -441,6 +441,19 @@ int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev,
                 float* combined, float* giOut, float* reservoirOut, float* historyOut,
                 void* stream);
 
+/* No reference counterpart (the reference renders the direct light and the GI
+ * passes in sequence, StratusRendererBackend.cpp:1982-2045).  event: a
+ * hipEvent_t that the caller records on another stream after the pass that
+ * writes the direct light (screen), or NULL (the default) for none.  With an
+ * event set, sgi_denoise, sgi_denoise_learned and sgi_render_frame wait for
+ * it (hipStreamWaitEvent on their own stream) just before their temporal pass,
+ * the only pass that reads screen, so the direct light may run beside culling,
+ * the gather and the spatial passes instead of being joined before the
+ * denoiser.  The context keeps the handle: the caller records the event before
+ * each such call and keeps it alive while it is set.  Under stream capture the
+ * wait becomes a graph edge from the event's record. */
+int sgi_set_screen_event(sgi_ctx* ctx, void* event);
+
 /* taa.fs:12-100 (PerformTaaPostFx_, StratusRendererBackend.cpp:2326-2352), the
  * first consumer of the lit buffer: per pixel, with uv the texel centre and
  * vel the velocity texel, the previous frame sampled bilinearly (LINEAR,

@@ -107,6 +107,10 @@ int sgi_kpcn_conv3x3(const uint16_t* in, const uint16_t* mask, const uint16_t* w
 int sgi_kpcn_conv3x3_fp8(const uint8_t* in, const uint8_t* weights, const float* epilogue, int32_t height,
                          int32_t width, uint8_t* out, void* stream);
 
+/* BF16 of n floats (n <= 2^38), rounded to the nearest even: the conversion
+ * every BF16 activation, gradient and weight store of the network uses. */
+int sgi_kpcn_bf16_encode(const float* in, int64_t n, uint16_t* out, void* stream);
+
 /* E4M3 bytes of n floats: clamped to +-448, rounded to the nearest even E4M3
  * value (OCP FP8; the conversion the FP8 layers store with). */
 int sgi_kpcn_fp8_encode(const float* in, int64_t n, uint8_t* out, void* stream);

@@ -191,6 +191,7 @@ SGI_SIGNATURES = {
     "sgi_direct_light": (c_int32, [_P, POINTER(Gbuffer), _P, _P]),
     "sgi_denoise": (c_int32, [_P, POINTER(Gbuffer), POINTER(Gbuffer), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               _P]),
+    "sgi_set_screen_event": (c_int32, [_P, _P]),
     "sgi_render_frame": (c_int32, [_P, POINTER(FrameIO), _P]),
     "sgi_views_create": (c_int32, [POINTER(ViewsConfig), POINTER(_P)]),
     "sgi_views_destroy": (None, [_P]),
@@ -202,6 +203,7 @@ SGI_SIGNATURES = {
     # include/sgi_kpcn.h
     "sgi_kpcn_conv3x3": (c_int32, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
     "sgi_kpcn_conv3x3_fp8": (c_int32, [_P, _P, _P, _I, _I, _P, _P]),
+    "sgi_kpcn_bf16_encode": (c_int32, [_P, ctypes.c_int64, _P, _P]),
     "sgi_kpcn_fp8_encode": (c_int32, [_P, ctypes.c_int64, _P, _P]),
     "sgi_kpcn_wgrad_partials": (ctypes.c_size_t, [_I, _I]),
     "sgi_kpcn_conv3x3_wgrad": (c_int32, [_P, _P, _P, _I, _I, _P, _P, _P]),

@@ -30,6 +30,7 @@ struct sgi_ctx {
     float* csmTris = nullptr;     // per cascade transformed triangle lists of sgi_render_csm, grown on demand
     size_t csmTrisFloats = 0;
     int device = 0;
+    hipEvent_t screenReady = nullptr;  // sgi_set_screen_event: waited for before the passes that read screen
 };
 
 namespace {
@@ -428,10 +429,21 @@ int sgi_denoise(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer* prev, c
         return fail(SGI_E_INVALID, "sgi_denoise: reservoirs must be 16-byte aligned");
     const sgi_dev::GbufArgs c = sgi_dev::gbufArgs(*cur);
     const sgi_dev::GbufArgs p = sgi_dev::gbufArgs(prev ? *prev : *cur);
-    const hipError_t e = sgi_launch::denoise(c, p, ctx->frameArgs, visibleIndices, screen, color, reservoir, prevGi,
-                                             prevHistory, combined, giOut, reservoirOut, historyOut, ctx->ws,
-                                             (hipStream_t)stream);
+    const hipStream_t s = (hipStream_t)stream;
+    // passes 0-3 read no direct light; the temporal pass (the combine) is the one that waits for it
+    hipError_t e = sgi_launch::denoiseSpatial(c, ctx->frameArgs, visibleIndices, reservoir, reservoirOut, ctx->ws,
+                                              nullptr, false, s);
+    if (e == hipSuccess && ctx->screenReady) e = hipStreamWaitEvent(s, ctx->screenReady, 0);
+    if (e == hipSuccess)
+        e = sgi_launch::denoiseTemporal(c, p, ctx->frameArgs, screen, color, reservoirOut, prevGi, prevHistory,
+                                        combined, giOut, historyOut, ctx->ws, s);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise");
+}
+
+int sgi_set_screen_event(sgi_ctx* ctx, void* event) {
+    if (!ctx) return fail(SGI_E_INVALID, "sgi_set_screen_event: NULL context");
+    ctx->screenReady = (hipEvent_t)event;
+    return SGI_OK;
 }
 
 // RenderScene's GI steps without the reference's host readbacks: the visible
@@ -501,6 +513,14 @@ int sgi_kpcn_conv3x3_fp8(const uint8_t* in, const uint8_t* weights, const float*
         return fail(SGI_E_INVALID, "sgi_kpcn_conv3x3_fp8: buffers must be 16-byte aligned");
     const hipError_t e = sgi_kpcn::convF8(in, height, width, weights, epilogue, out, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_conv3x3_fp8");
+}
+
+int sgi_kpcn_bf16_encode(const float* in, int64_t n, uint16_t* out, void* stream) {
+    if (n < 0 || n > ((int64_t)1 << 38) || (n > 0 && (!in || !out)))
+        return fail(SGI_E_INVALID, "sgi_kpcn_bf16_encode: bad buffer or count");
+    if (n == 0) return SGI_OK;
+    const hipError_t e = sgi_kpcn::encodeBf16(in, n, out, (hipStream_t)stream);
+    return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_kpcn_bf16_encode");
 }
 
 int sgi_kpcn_fp8_encode(const float* in, int64_t n, uint8_t* out, void* stream) {
@@ -740,7 +760,7 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p
                           const float* reservoir, const float* prevGi, const float* prevHistory,
                           const sgi_kpcn_net& netArgs, float threshold, void* workspace, float* combined,
                           float* giOut, float* reservoirOut, float* historyOut, int32_t* fallbackTiles,
-                          const DenoiseBuffers& dws, hipStream_t s) {
+                          const DenoiseBuffers& dws, hipEvent_t screenReady, hipStream_t s) {
     const int H = c.H, W = c.W;
     char* base = (char*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
     KpcnWs ws;
@@ -797,6 +817,7 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p
     if (std::isinf(threshold) && threshold > 0.0f) {
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));
+        if (screenReady) KPCN_STEP(hipStreamWaitEvent(s, screenReady, 0));
         KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
                                                  combined, giOut, reservoirOut, historyOut, nullptr, nullptr, s));
     } else {
@@ -807,6 +828,7 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p
         // reservoir)
         KPCN_STEP(sgi_launch::denoiseSpatial(c, fr, visibleIndices, reservoir, reservoirOut, dws,
                                              ws.dilatedList, false, s));
+        if (screenReady) KPCN_STEP(hipStreamWaitEvent(s, screenReady, 0));
         KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
                                                  combined, giOut, reservoirOut, historyOut, ws.flags, dws.normals,
                                                  s));
@@ -849,7 +871,7 @@ int sgi_denoise_learned(sgi_ctx* ctx, const sgi_gbuffer* cur, const sgi_gbuffer*
                                                     ctx->frameArgs, visibleIndices, screen, color, reservoir, prevGi,
                                                     prevHistory, *net, threshold, workspace, combined, giOut,
                                                     reservoirOut, historyOut, fallbackTiles, ctx->ws,
-                                                    (hipStream_t)stream);
+                                                    ctx->screenReady, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_denoise_learned");
 }
 

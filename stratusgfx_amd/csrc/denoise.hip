@@ -49,22 +49,30 @@ __device__ __forceinline__ const sgi_batch::ViewDesc& viewBlock(const sgi_batch:
     return d;
 }
 
-// GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv.
-// Bilinear sample at uv of the texels fetch(q), q the row-major texel index.
-template <class F>
-__device__ __forceinline__ float bilinearAt(int W, int H, float u, float v, F fetch) {
+// GL LINEAR + CLAMP_TO_EDGE at an arbitrary uv: the four texel indices and
+// weights, computed once for every channel read there; blend() is
+// ((w00 t00 + w10 t10) + w01 t01) + w11 t11 with w00 = (1 - a)(1 - b),
+// w10 = a (1 - b), w01 = (1 - a) b, w11 = a b.
+struct BilinTaps {
+    size_t q00, q10, q01, q11;
+    float w00, w10, w01, w11;
+};
+__device__ __forceinline__ BilinTaps bilinTaps(int W, int H, float u, float v) {
     const float fu = u * (float)W - 0.5f, fv = v * (float)H - 0.5f;
     const float i0f = floorf(fu), j0f = floorf(fv);
     const float a = fu - i0f, b = fv - j0f;
     const int ia = texelIndex(i0f, W), ib = texelIndex(i0f + 1.0f, W);
     const int ja = texelIndex(j0f, H), jb = texelIndex(j0f + 1.0f, H);
-    const float t00 = fetch((size_t)ja * W + ia), t10 = fetch((size_t)ja * W + ib);
-    const float t01 = fetch((size_t)jb * W + ia), t11 = fetch((size_t)jb * W + ib);
-    return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 + a * b * t11;
+    return BilinTaps{(size_t)ja * W + ia,       (size_t)ja * W + ib,  (size_t)jb * W + ia,
+                     (size_t)jb * W + ib,       (1.0f - a) * (1.0f - b), a * (1.0f - b),
+                     (1.0f - a) * b,            a * b};
 }
-
-__device__ __forceinline__ float bilinear(const float* t, int W, int H, int C, float u, float v, int c) {
-    return bilinearAt(W, H, u, v, [&](size_t q) { return t[q * C + c]; });
+__device__ __forceinline__ float blend(const BilinTaps& t, float v00, float v10, float v01, float v11) {
+    return t.w00 * v00 + t.w10 * v10 + t.w01 * v01 + t.w11 * v11;
+}
+__device__ __forceinline__ f3 blend3(const BilinTaps& t, f3 v00, f3 v10, f3 v01, f3 v11) {
+    return f3{blend(t, v00.x, v10.x, v01.x, v11.x), blend(t, v00.y, v10.y, v01.y, v11.y),
+              blend(t, v00.z, v10.z, v01.z, v11.z)};
 }
 
 __device__ __forceinline__ float nearestTap(const float* t, int W, int H, int C, float u, float v, int c) {
@@ -246,16 +254,18 @@ __device__ __forceinline__ void temporalBody(const GbufArgs& g, const GbufArgs& 
     const float u = texelCentre(x, W), v = texelCentre(y, H);
     const float2 vel = gVelocity(g, p);
     const float pu = u - vel.x, pv = v - vel.y;
-    const float prevCenterDepth = bilinear(prev.depth, W, H, 1, pu, pv, 0);
-    const f3 prevCenterNormal =
-        normalize(f3{bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 0); }),
-                     bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 1); }),
-                     bilinearAt(W, H, pu, pv, [&](size_t q) { return gNormalC(prev, q, 2); })} * 2.0f -
-                  splat(1.0f));
+    // the previous frame's depth, normal and GI at uv - velocity: one set of taps, each texel read once
+    const BilinTaps bt = bilinTaps(W, H, pu, pv);
+    const float prevCenterDepth =
+        blend(bt, prev.depth[bt.q00], prev.depth[bt.q10], prev.depth[bt.q01], prev.depth[bt.q11]);
+    const f3 prevCenterNormal = normalize(blend3(bt, gNormalRaw(prev, bt.q00), gNormalRaw(prev, bt.q10),
+                                                 gNormalRaw(prev, bt.q01), gNormalRaw(prev, bt.q11)) *
+                                              2.0f -
+                                          splat(1.0f));
     f3 prevGiV = splat(0.0f);
     if (prevGi)
-        prevGiV = f3{bilinear(prevGi, W, H, 3, pu, pv, 0), bilinear(prevGi, W, H, 3, pu, pv, 1),
-                     bilinear(prevGi, W, H, 3, pu, pv, 2)};
+        prevGiV = blend3(bt, ld3(prevGi + 3 * bt.q00), ld3(prevGi + 3 * bt.q10), ld3(prevGi + 3 * bt.q01),
+                         ld3(prevGi + 3 * bt.q11));
     const float currId = g.id ? g.id[p] : 0.0f;
     const float prevId = prev.id ? nearestTap(prev.id, W, H, 1, pu, pv, 0) : 0.0f;
     const float wn = gmax(0.0f, dot(centerNormal, prevCenterNormal));

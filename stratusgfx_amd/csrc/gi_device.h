@@ -412,9 +412,13 @@ __host__ inline GbufArgs gbufArgs(const sgi_gbuffer& g) {
 // A UNORM8 channel (byte `byte` of a texel) as GL reads it: k / 255, correctly rounded.
 __device__ __forceinline__ float unorm8(uint32_t v, int byte) { return (float)((v >> (8 * byte)) & 0xFFu) / 255.0f; }
 
-// The stored normal channel c (n * 0.5 + 0.5) of pixel p.
-__device__ __forceinline__ float gNormalC(const GbufArgs& g, size_t p, int c) {
-    return g.n8 ? unorm8(g.n8[p], c) : g.normal[3 * p + c];
+// The stored normal of pixel p (n * 0.5 + 0.5 per channel, not yet decoded).
+__device__ __forceinline__ f3 gNormalRaw(const GbufArgs& g, size_t p) {
+    if (g.n8) {
+        const uint32_t v = g.n8[p];
+        return f3{unorm8(v, 0), unorm8(v, 1), unorm8(v, 2)};
+    }
+    return ld3(g.normal + 3 * p);
 }
 // The decoded normal (common.glsl's normal * 2 - 1, normalised) of pixel p.
 __device__ __forceinline__ f3 gNormal(const GbufArgs& g, size_t p) {

@@ -40,9 +40,14 @@ constexpr int kP = SGI_KPCN_LAYER_PARAMS;
 constexpr float kRadianceScale = SGI_KPCN_RADIANCE_SCALE;
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-    const uint32_t u = __float_as_uint(f);
-    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// FP32 -> BF16 rounded to the nearest even (gfx950's v_cvt_pk_bf16_f32; FP32
+// denormals are kept, so every finite value rounds as the integer form
+// (u + 0x7fff + ((u >> 16) & 1)) >> 16 does); f2bf2 packs two, lo first.
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    const bf16x2v v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
 }
 
 // Zero the BF16 lanes of v whose mask lane is not > 0.
@@ -72,24 +77,40 @@ __device__ __forceinline__ bf16x8 asBf16x8(uint4 v) { return __builtin_bit_cast(
 // tell the halo from the blocks being filled by LDS DMA).
 __device__ __forceinline__ bf16x8 ldsBf16x8(const uint4* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
+// ln(1 + x), x >= 0, through the hardware log2 (v_log_f32, 1 ulp): 1 + x
+// rounds away at most 2^-24 of x, so the result is within 2^-24 absolute and
+// a few 2^-24 relative of log1p(x); the features it feeds are stored as BF16
+// (2^-9 relative).
+constexpr float kLn2 = 0.693147180559945f;
+__device__ __forceinline__ float log1pFeature(float x) { return __builtin_amdgcn_logf(1.0f + x) * kLn2; }
+
 // Network input features of pixel i (the 16 packed BF16 pairs of its 32
 // channels) and its noisy radiance reservoir.rgb / max(reservoir.a, 1e-5).
+// The features are rounded to BF16 right away, so they are formed with the
+// hardware reciprocal, square root and logarithm (each within a few FP32 ulp)
+// rather than the correctly rounded forms the GI passes decide with; the FP32
+// radiance the reconstruction filters is within 2 ulp of the quotient.
 __device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs& fr,
                                               const float4* __restrict__ reservoir, int x, int y, uint32_t* o,
                                               float* L) {
     const size_t i = (size_t)y * g.W + x;
     const float4 r = reservoir[i];
-    const float inv = 1.0f / gmax(kEPS, r.w);
+    const float inv = __builtin_amdgcn_rcpf(gmax(kEPS, r.w));
     L[0] = r.x * inv;
     L[1] = r.y * inv;
     L[2] = r.z * inv;
-    const f3 P = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[i], fr.invPV);
-    const f3 d = ld3(fr.viewPos) - P;
-    const f3 n = gNormal(g, i);
+    // world position: invPV (x_ndc, y_ndc, 2 depth - 1, 1), divided by w
+    float w4[4];
+    mulMat4(fr.invPV, ((float)x + 0.5f) * (2.0f / (float)g.W) - 1.0f, ((float)y + 0.5f) * (2.0f / (float)g.H) - 1.0f,
+            g.depth[i] * 2.0f - 1.0f, 1.0f, w4);
+    const float iw = __builtin_amdgcn_rcpf(w4[3]);
+    const f3 d = ld3(fr.viewPos) - f3{w4[0] * iw, w4[1] * iw, w4[2] * iw};
+    const f3 nr = gNormalRaw(g, i) * 2.0f - splat(1.0f);
+    const f3 n = nr * __builtin_amdgcn_rsqf(dot(nr, nr));
     float f[SGI_KPCN_FEATURES];
-    f[0] = log1pf(gmax(L[0], 0.0f) * kRadianceScale);
-    f[1] = log1pf(gmax(L[1], 0.0f) * kRadianceScale);
-    f[2] = log1pf(gmax(L[2], 0.0f) * kRadianceScale);
+    f[0] = log1pFeature(gmax(L[0], 0.0f) * kRadianceScale);
+    f[1] = log1pFeature(gmax(L[1], 0.0f) * kRadianceScale);
+    f[2] = log1pFeature(gmax(L[2], 0.0f) * kRadianceScale);
     const float4 al = gAlbedo(g, i);
     f[3] = al.x;
     f[4] = al.y;
@@ -97,7 +118,7 @@ __device__ __forceinline__ void pixelFeatures(const GbufArgs& g, const FrameArgs
     f[6] = n.x;
     f[7] = n.y;
     f[8] = n.z;
-    f[9] = log1pf(sqrtf(dot(d, d))) * 0.125f;
+    f[9] = log1pFeature(__builtin_amdgcn_sqrtf(dot(d, d))) * 0.125f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const int a = 2 * j, b = 2 * j + 1;
@@ -163,12 +184,17 @@ __device__ __forceinline__ void loadLogits(const float* logits, size_t p, float*
     }
 }
 
+// One filter tap: o + v w per channel, one rounding each (FMA).
+__device__ __forceinline__ f3 fmaTap(f3 v, float w, f3 o) {
+    return f3{__builtin_fmaf(v.x, w, o.x), __builtin_fmaf(v.y, w, o.y), __builtin_fmaf(v.z, w, o.z)};
+}
+
 __device__ __forceinline__ f3 filtered(const float* radiance, int W, int H, int x, int y, const float* wgt) {
     f3 o = splat(0.0f);
 #pragma unroll
     for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
         const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
-        o = o + ld3(radiance + 3 * ((size_t)yy * W + xx)) * wgt[j];
+        o = fmaTap(ld3(radiance + 3 * ((size_t)yy * W + xx)), wgt[j], o);
     }
     return o;
 }
@@ -215,7 +241,7 @@ __device__ __forceinline__ f3 composePixelT(const float* lg, const T& src, bool 
 #pragma unroll
     for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
         const int xx = min(max(x + j % 5 - 2, 0), W - 1), yy = min(max(y + j / 5 - 2, 0), H - 1);
-        o = o + src.radiance(xx, yy) * wgt[j];
+        o = fmaTap(src.radiance(xx, yy), wgt[j], o);
     }
     if (hasCoarse) {
         const float b = sigmoidf(lg[SGI_KPCN_BLEND_CHANNEL]);
@@ -278,7 +304,7 @@ struct ConvSrc {
 __device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
     const float lo = bf2f((uint16_t)(a & 0xffffu)) + bf2f((uint16_t)(b & 0xffffu));
     const float hi = bf2f((uint16_t)(a >> 16)) + bf2f((uint16_t)(b >> 16));
-    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+    return f2bf2(lo, hi);
 }
 
 __device__ __forceinline__ uint4 addChunk(uint4 a, uint4 b) {
@@ -433,6 +459,57 @@ constexpr int kCoHS = kTS / 2 + 2;  // the coarser output staged for up() (OUT_A
 constexpr int kRadSlots = (kRadHS * kRadHS + 255) / 256;  // staged pixels per thread
 constexpr int kCoSlots = (kCoHS * kCoHS + 255) / 256;
 
+// OUT_APPLY's blocks of one tile, loaded into registers one tile ahead and
+// stored to LDS at the tile's start: its radiance (2-pixel margin, kRadHS^2
+// pixels) and its coarser output (1-pixel margin, kCoHS^2 pixels), clamped to
+// the image as the global taps clamp. Two pixels per thread: thread t holds
+// radiance pixel t, and radiance pixel t + 256 or, past the radiance block's
+// end, coarser pixel t + 256 - kRadHS^2 (the coarser block fills threads the
+// radiance block leaves idle). Loads are never skipped: a thread past both
+// blocks, or a coarser pixel with no coarser level, loads an in-range pixel it
+// does not store.
+static_assert(kRadSlots == 2 && kRadHS * kRadHS + kCoHS * kCoHS <= 512, "two staged pixels per thread");
+struct ApplyRegs {
+    float v[2][3];
+};
+
+__device__ __forceinline__ void loadApply(const float* radiance, const float* coarse, int H, int W, int tile,
+                                          int tilesX, ApplyRegs& r) {
+    const int tx = tile % tilesX, ty = tile / tilesX;
+    const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
+    const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
+    const int ch = (H + 1) / 2, cw = (W + 1) / 2;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = (int)threadIdx.x + k * 256;
+        const float* q;
+        if (i < kRadHS * kRadHS || !coarse) {
+            const int j = min(i, kRadHS * kRadHS - 1);
+            const int gy = min(max(ry0 + j / kRadHS, 0), H - 1), gx = min(max(rx0 + j % kRadHS, 0), W - 1);
+            q = radiance + 3 * ((size_t)gy * W + gx);
+        } else {
+            const int j = min(i - kRadHS * kRadHS, kCoHS * kCoHS - 1);
+            const int gy = min(max(cy0 + j / kCoHS, 0), ch - 1), gx = min(max(cx0 + j % kCoHS, 0), cw - 1);
+            q = coarse + 3 * ((size_t)gy * cw + gx);
+        }
+        r.v[k][0] = q[0];
+        r.v[k][1] = q[1];
+        r.v[k][2] = q[2];
+    }
+}
+
+__device__ __forceinline__ void storeApply(const ApplyRegs& r, bool hasCoarse, f32x4* radS, f32x4* coS) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = (int)threadIdx.x + k * 256;
+        const f32x4 v = f32x4{r.v[k][0], r.v[k][1], r.v[k][2], 0.0f};
+        if (i < kRadHS * kRadHS)
+            radS[i] = v;
+        else if (hasCoarse && i < kRadHS * kRadHS + kCoHS * kCoHS)
+            coS[i - kRadHS * kRadHS] = v;
+    }
+}
+
 // One level of the reconstruction at tile pixel (lx, ly) from the LDS-staged
 // blocks (one pixel per 16 bytes, rgb in the first three floats): radiance
 // staged with a 2-pixel margin and the coarser output with a 1-pixel margin,
@@ -449,7 +526,7 @@ __device__ __forceinline__ f3 composePixelStaged(const float* lg, const f32x4* r
 #pragma unroll
     for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
         const f32x4 v = r[(j / 5) * kRadHS + j % 5];
-        o = o + f3{v.x, v.y, v.z} * wgt[j];
+        o = fmaTap(f3{v.x, v.y, v.z}, wgt[j], o);
     }
     if (hasCoarse) {
         // bilinearAxis: coarse pixels a, a + 1 (a = x / 2 for odd x, x / 2 - 1 for even), clamped;
@@ -537,10 +614,12 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
     int t0, t1;
     blockTiles(nTiles, t0, t1);
     uint4 pre[A::kStageSlots], preB[kDefer ? A::kStageSlots : 1];
+    ApplyRegs apl;  // OUT_APPLY: the staged blocks' values, one tile ahead
     if (t0 < t1) {
         loadHalo<AIN, kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN && AIN == ACT_BF16>(src, H, W, t0, tilesX, pre,
                                                                                      preB);
         if constexpr (kLdsUp) stageUp<AIN>(src, H, W, t0, tilesX, upS);
+        if constexpr (OUT == OUT_APPLY) loadApply(src.radiance, src.coarse, H, W, t0, tilesX, apl);
     }
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
@@ -569,44 +648,13 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                 }
             }
         }
+        if constexpr (OUT == OUT_APPLY) storeApply(apl, src.coarse != nullptr, radS, coS);
         __syncthreads();
-        // OUT_APPLY: this tile's radiance (2-pixel margin) and coarser output (1-pixel margin),
-        // clamped to the image as the global taps clamp, loaded straight into LDS while the
-        // MFMAs run (12-byte global_load_lds: lane l of a wave writes its pixel's rgb to the
-        // first 12 bytes of the 16 at 16 l); pixel i of a staged block sits at slot i, the
-        // slots past the block's end receive a valid but unused pixel. Issued before the
-        // halo prefetch, so that nothing this stage waits for queues behind that prefetch.
-        if constexpr (OUT == OUT_APPLY) {
-            const int rx0 = tx * kTS - 2, ry0 = ty * kTS - 2;
-#pragma unroll
-            for (int k = 0; k < kRadSlots; ++k) {
-                const int i = (int)threadIdx.x + k * 256;
-                size_t g = 0;
-                if (i < kRadHS * kRadHS) {
-                    const int gy = min(max(ry0 + i / kRadHS, 0), H - 1), gx = min(max(rx0 + i % kRadHS, 0), W - 1);
-                    g = 3 * ((size_t)gy * W + gx);
-                }
-                dmaToLds12(src.radiance + g, radS + k * 256 + wave * 64);
-            }
-            if (src.coarse) {
-                const int cx0 = tx * (kTS / 2) - 1, cy0 = ty * (kTS / 2) - 1;
-                const int ch = (H + 1) / 2, cw = (W + 1) / 2;
-#pragma unroll
-                for (int k = 0; k < kCoSlots; ++k) {
-                    const int i = (int)threadIdx.x + k * 256;
-                    size_t g = 0;
-                    if (i < kCoHS * kCoHS) {
-                        const int gy = min(max(cy0 + i / kCoHS, 0), ch - 1), gx = min(max(cx0 + i % kCoHS, 0), cw - 1);
-                        g = 3 * ((size_t)gy * cw + gx);
-                    }
-                    dmaToLds12(src.coarse + g, coS + k * 256 + wave * 64);
-                }
-            }
-        }
         if (tile + 1 < t1) {
             loadHalo<AIN, kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN && AIN == ACT_BF16>(src, H, W, tile + 1, tilesX,
                                                                                          pre, preB);
             if constexpr (kLdsUp) stageUp<AIN>(src, H, W, tile + 1, tilesX, upS);
+            if constexpr (OUT == OUT_APPLY) loadApply(src.radiance, src.coarse, H, W, tile + 1, tilesX, apl);
         }
         f32x4 acc[2][4];
 #pragma unroll
@@ -654,9 +702,9 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                         *reinterpret_cast<f32x4*>(logits + ((wave * 4 + nb) * kTS + l16) * kLogitStride + co) = v;
                     }
                 }
-            // this thread's LDS loads have landed before any thread reads the staged blocks
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
+            // the logits are in LDS before any thread reads them (the staged blocks were stored
+            // before the tile's second barrier)
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
             const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
             if (x < W && y < H) {
                 float lgv[kLogitStride];
@@ -709,15 +757,14 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                             for (int r = 0; r < 4; ++r) acc[mb][nb][r] = d[r];
                         }
                     } else {
-                        const uint32_t b0 = f2bf(v[0]), b1 = f2bf(v[1]), b2 = f2bf(v[2]), b3 = f2bf(v[3]);
+                        const uint32_t b01 = f2bf2(v[0], v[1]), b23 = f2bf2(v[2], v[3]);
                         if (x < W && y < H)
-                            *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) =
-                                make_uint2(b0 | (b1 << 16), b2 | (b3 << 16));
+                            *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + p * kC + co) = make_uint2(b01, b23);
                         if constexpr (OUT == OUT_BF16_POOL) {  // keep the stored values for the pooling
-                            acc[mb][nb][0] = bf2f((uint16_t)b0);
-                            acc[mb][nb][1] = bf2f((uint16_t)b1);
-                            acc[mb][nb][2] = bf2f((uint16_t)b2);
-                            acc[mb][nb][3] = bf2f((uint16_t)b3);
+                            acc[mb][nb][0] = bf2f((uint16_t)(b01 & 0xffffu));
+                            acc[mb][nb][1] = bf2f((uint16_t)(b01 >> 16));
+                            acc[mb][nb][2] = bf2f((uint16_t)(b23 & 0xffffu));
+                            acc[mb][nb][3] = bf2f((uint16_t)(b23 >> 16));
                         }
                     }
                 }
@@ -732,7 +779,7 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                     const int y0 = ty * kTS + wave * 4 + 2 * pr;
                     const bool y1In = y0 + 1 < H;
                     const int n = (1 + (x1In ? 1 : 0)) * (1 + (y1In ? 1 : 0));
-                    const float inv = 1.0f / (float)n;
+                    const float inv = n == 4 ? 0.25f : (n == 2 ? 0.5f : 1.0f);  // 1 / n, exact
 #pragma unroll
                     for (int mb = 0; mb < 2; ++mb) {
                         float m[4];
@@ -754,9 +801,8 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                             const uint32_t o = packF8x4(m[0], m[1], m[2], m[3]);
                             if (even && xIn && y0 < H) *reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(src.pooled) + q) = o;
                         } else {
-                            const uint32_t o0 = f2bf(m[0]), o1 = f2bf(m[1]), o2 = f2bf(m[2]), o3 = f2bf(m[3]);
-                            if (even && xIn && y0 < H)
-                                *reinterpret_cast<uint2*>(src.pooled + q) = make_uint2(o0 | (o1 << 16), o2 | (o3 << 16));
+                            const uint32_t o01 = f2bf2(m[0], m[1]), o23 = f2bf2(m[2], m[3]);
+                            if (even && xIn && y0 < H) *reinterpret_cast<uint2*>(src.pooled + q) = make_uint2(o01, o23);
                         }
                     }
                 }
@@ -767,6 +813,13 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
 
 // E4M3 bytes of n floats (clamped to +-448, nearest even): thread i converts
 // values 4i .. 4i + 3 (those past n are left alone).
+// BF16 of n floats (f2bf: nearest even).
+__global__ __launch_bounds__(256) void encodeBf16Kernel(const float* __restrict__ in, int64_t n,
+                                                        uint16_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = f2bf(in[i]);
+}
+
 __global__ __launch_bounds__(256) void encodeF8Kernel(const float* __restrict__ in, int64_t n, uint8_t* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = 4 * i;
     if (j >= n) return;
@@ -912,7 +965,7 @@ __global__ void poolKernel(const uint16_t* __restrict__ in, int H, int W, uint16
     const float inv = 1.0f / (float)n;
     uint32_t o[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(s[2 * j] * inv) | ((uint32_t)f2bf(s[2 * j + 1] * inv) << 16);
+    for (int j = 0; j < 4; ++j) o[j] = f2bf2(s[2 * j] * inv, s[2 * j + 1] * inv);
     *reinterpret_cast<uint4*>(out + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
@@ -937,7 +990,7 @@ __global__ void poolBackwardKernel(const uint16_t* __restrict__ dout, int H, int
     for (int j = 0; j < 4; ++j) {
         const float lo = bf2f((uint16_t)(pp[j] & 0xffffu)) + bf2f((uint16_t)(gg[j] & 0xffffu)) * inv;
         const float hi = bf2f((uint16_t)(pp[j] >> 16)) + bf2f((uint16_t)(gg[j] >> 16)) * inv;
-        o[j] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+        o[j] = f2bf2(lo, hi);
     }
     *reinterpret_cast<uint4*>(dst) = make_uint4(o[0], o[1], o[2], o[3]);
 }
@@ -958,7 +1011,7 @@ __global__ void upAddKernel(const uint16_t* __restrict__ skip, const uint16_t* _
     for (int j = 0; j < 4; ++j) {
         const float lo = bf2f((uint16_t)(aa[j] & 0xffffu)) + bf2f((uint16_t)(bb[j] & 0xffffu));
         const float hi = bf2f((uint16_t)(aa[j] >> 16)) + bf2f((uint16_t)(bb[j] >> 16));
-        o[j] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+        o[j] = f2bf2(lo, hi);
     }
     *reinterpret_cast<uint4*>(out + (size_t)p * kC + c * 8) = make_uint4(o[0], o[1], o[2], o[3]);
 }
@@ -996,7 +1049,7 @@ __global__ void upBackwardKernel(const uint16_t* __restrict__ dout, int H, int W
     }
     uint32_t o[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(s[2 * j]) | ((uint32_t)f2bf(s[2 * j + 1]) << 16);
+    for (int j = 0; j < 4; ++j) o[j] = f2bf2(s[2 * j], s[2 * j + 1]);
     *reinterpret_cast<uint4*>(dst) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
@@ -1061,7 +1114,7 @@ __global__ void applyBackwardKernel(const float* __restrict__ logits, const floa
         uint32_t o4[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            o4[j] = (uint32_t)f2bf(dl[q * 8 + 2 * j]) | ((uint32_t)f2bf(dl[q * 8 + 2 * j + 1]) << 16);
+            o4[j] = f2bf2(dl[q * 8 + 2 * j], dl[q * 8 + 2 * j + 1]);
         dst[q] = make_uint4(o4[0], o4[1], o4[2], o4[3]);
     }
 }
@@ -1125,7 +1178,7 @@ __device__ __forceinline__ void tilePyramid(float (*buf)[kTS * kTS * 3], int tx,
                         a2 += src[3 * c + 2];
                         ++n;
                     }
-                const float inv = 1.0f / (float)n;
+                const float inv = n == 4 ? 0.25f : (n == 2 ? 0.5f : 1.0f);  // 1 / n, exact
                 const int d = ly * s + lx;
                 dst[3 * d + 0] = a0 * inv;
                 dst[3 * d + 1] = a1 * inv;
@@ -1266,23 +1319,30 @@ __global__ void weightCopiesKernel(const float* __restrict__ params, uint16_t* _
 }
 
 // Per 16x16 tile: flag = mean exp(err) > threshold (every tile when threshold <= 0).
+// One wave per 16x16 tile: the mean of exp(min(err, 80)) over the tile's
+// in-image pixels against the threshold. The sum is the binary tree over the
+// tile's 256 pixels (pixel i + 128 into i, then i + 64 into i, ... 1): lane l
+// holds pixels l, l + 64, l + 128 and l + 192 and forms the first two levels
+// in registers, the last six through shuffles.
 __global__ __launch_bounds__(256) void tileFlagKernel(const float* __restrict__ err, int H, int W, int tilesX,
-                                                      float threshold, uint8_t* __restrict__ flags,
+                                                      int nTiles, float threshold, uint8_t* __restrict__ flags,
                                                       int32_t* __restrict__ dilatedList, int32_t* __restrict__ counts) {
-    __shared__ float red[256];
-    const int t = blockIdx.x;
+    const int t = blockIdx.x * 4 + ((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    if (t >= nTiles) return;
     const int tx = t % tilesX, ty = t / tilesX;
-    const int x = tx * 16 + (int)threadIdx.x % 16, y = ty * 16 + (int)threadIdx.x / 16;
-    const bool in = x < W && y < H;
-    red[threadIdx.x] = in ? expf(gmin(err[(size_t)y * W + x], 80.0f)) : 0.0f;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = lane + 64 * k;
+        const int x = tx * 16 + i % 16, y = ty * 16 + i / 16;
+        v[k] = x < W && y < H ? expf(gmin(err[(size_t)y * W + x], 80.0f)) : 0.0f;
     }
-    if (threadIdx.x == 0) {
+    float sum = (v[0] + v[2]) + (v[1] + v[3]);
+#pragma unroll
+    for (int sft = 32; sft > 0; sft >>= 1) sum = sum + __shfl_down(sum, sft);
+    if (lane == 0) {
         const int n = (min(W, tx * 16 + 16) - tx * 16) * (min(H, ty * 16 + 16) - ty * 16);
-        const uint8_t f = (!(threshold > 0.0f) || red[0] / (float)n > threshold) ? 1 : 0;
+        const uint8_t f = (!(threshold > 0.0f) || sum / (float)n > threshold) ? 1 : 0;
         flags[t] = f;
         if (counts) counts[1 + t] = f;
         if (t == 0) {  // the counters dilateListKernel accumulates into
@@ -1410,6 +1470,11 @@ hipError_t convApply(const void* in, int H, int W, const void* w, const float* b
     return launchConvAct<IN_PLAIN, OUT_APPLY>(f8, src, w, bias, 0, H, W, out, err, s);
 }
 
+hipError_t encodeBf16(const float* in, int64_t n, uint16_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(encodeBf16Kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
+}
+
 hipError_t encodeF8(const float* in, int64_t n, uint8_t* out, hipStream_t s) {
     const int64_t words = (n + 3) / 4;
     hipLaunchKernelGGL(encodeF8Kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, in, n, out);
@@ -1535,8 +1600,8 @@ hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* bi
 hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, int32_t* dilatedList,
                      int32_t* counts, hipStream_t s) {
     const int tilesX = (W + 15) / 16, tilesY = (H + 15) / 16, n = tilesX * tilesY;
-    hipLaunchKernelGGL(tileFlagKernel, dim3(n), dim3(256), 0, s, err, H, W, tilesX, threshold, flags, dilatedList,
-                       counts);
+    hipLaunchKernelGGL(tileFlagKernel, dim3(gridFor(n, 4)), dim3(256), 0, s, err, H, W, tilesX, n, threshold, flags,
+                       dilatedList, counts);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dilateListKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, flags, tilesX, tilesY, dilatedList,

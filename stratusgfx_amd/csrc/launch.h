@@ -131,7 +131,7 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& cur, const sgi_dev::GbufArgs&
                           const float* reservoir, const float* prevGi, const float* prevHistory,
                           const sgi_kpcn_net& net, float threshold, void* workspace, float* combined, float* giOut,
                           float* reservoirOut, float* historyOut, int32_t* fallbackTiles, const DenoiseBuffers& dws,
-                          hipStream_t s);
+                          hipEvent_t screenReady, hipStream_t s);
 
 }  // namespace sgi_launch
 
@@ -164,6 +164,7 @@ hipError_t convApply(const void* in, int H, int W, const void* w, const float* b
                      const float* coarse, float* out, float* err, int f8, hipStream_t s);
 // FP8 in, FP8 out (ReLU): out = e4m3(max(fma(acc, k1, k2), 0)), scales = [k1[32], k2[32]].
 hipError_t convF8(const void* in, int H, int W, const void* w, const float* scales, void* out, hipStream_t s);
+hipError_t encodeBf16(const float* in, int64_t n, uint16_t* out, hipStream_t s);
 hipError_t encodeF8(const float* in, int64_t n, uint8_t* out, hipStream_t s);
 int wgradBlocks(int H, int W);
 hipError_t wgrad(const uint16_t* x, const uint16_t* dy, const uint16_t* mask, int H, int W, float* partials,

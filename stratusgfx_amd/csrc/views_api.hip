@@ -151,7 +151,7 @@ hipError_t runStages(sgi_views* v, hipStream_t s) {
                 e = sgi_launch::denoiseLearned(d.g, d.g, d.fr, d.visible, d.direct, d.color, d.reservoir, d.prevGi,
                                                d.prevHistory, v->net, v->threshold, v->kws, d.combined, d.giOut,
                                                d.reservoirOut, d.historyOut, nullptr,
-                                               sgi_launch::DenoiseBuffers{d.resA, d.resB, d.normals}, s);
+                                               sgi_launch::DenoiseBuffers{d.resA, d.resB, d.normals}, nullptr, s);
         }
         return e;
     }

@@ -203,6 +203,15 @@ class GiPass:
         _abi.call(self.lib, "sgi_direct_light", self.ctx, ctypes.byref(gbuf),
                   _dev(lit, "lit", 3 * self.width * self.height), _stream(stream))
 
+    def set_screen_event(self, event) -> None:
+        """sgi_set_screen_event: a torch.cuda.Event (recorded at least once), a raw hipEvent_t, or None.
+        The denoisers then wait for it just before their temporal pass, the only one that reads screen."""
+        handle = getattr(event, "cuda_event", event) if event is not None else None
+        if event is not None and not handle:
+            raise ValueError("set_screen_event: the event has not been recorded yet")
+        self._screen_event = event  # kept alive while it is set
+        _abi.call(self.lib, "sgi_set_screen_event", self.ctx, ctypes.c_void_p(handle) if handle else None)
+
     def denoise(self, cur: _abi.Gbuffer, prev: _abi.Gbuffer | None, visible, screen, color, reservoir, prev_gi,
                 prev_history, combined, gi_out, reservoir_out, history_out, stream=None) -> None:
         self._check_gbuf(cur)

@@ -59,6 +59,30 @@ def _rand_layer(seed):
     return rng, Wt, b, x
 
 
+def test_bf16_encode_is_nearest_even_on_every_pattern(lib):
+    """sgi_kpcn_bf16_encode (the conversion every BF16 store of the network makes)
+    on all 2^32 FP32 bit patterns: each non-NaN pattern u gives the integer
+    round-to-nearest-even (u + 0x7fff + ((u >> 16) & 1)) >> 16 (denormals kept,
+    overflow to infinity, signed zeros), each NaN a NaN."""
+    torch = require_gpu()
+    n = 1 << 28
+    bad = nan_bad = 0
+    out = torch.empty(n, dtype=torch.int16, device="cuda")
+    for c in range(16):
+        u = torch.arange(c * n, (c + 1) * n, dtype=torch.int64, device="cuda")
+        f = u.to(torch.int32).view(torch.float32)  # the low 32 bits, as FP32
+        _abi.call(lib, "sgi_kpcn_bf16_encode", f.data_ptr(), n, out.data_ptr(), 0)
+        got = out.to(torch.int64) & 0xFFFF
+        want = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+        nan = (((u >> 23) & 0xFF) == 0xFF) & ((u & 0x7FFFFF) != 0)
+        bad += int(((got != want) & ~nan).sum())
+        nan_bad += int((nan & ~((((got >> 7) & 0xFF) == 0xFF) & ((got & 0x7F) != 0))).sum())
+        del u, f
+    torch.cuda.synchronize()
+    print(f"[bf16] 2^32 patterns: {bad} non-NaN mismatches, {nan_bad} NaNs not kept NaN", flush=True)
+    assert bad == 0 and nan_bad == 0
+
+
 def test_conv3x3_forward(lib):
     torch = require_gpu()
     _, Wt, b, x = _rand_layer(0)
