@@ -70,6 +70,9 @@ def parse():
                     default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256, 320],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
+    ap.add_argument("--direct-fork", choices=["start", "after-cull", "after-gather"], default="start",
+                    help="where the frame forks the direct-light pass onto the second stream (it must end before the "
+                         "denoiser's temporal pass, which waits for it)")
     ap.add_argument("--rejection-limit", type=int, default=-1,
                     help="sampled gathers: rejections before draws are taken unconditionally (5 = the reference's "
                          "rule); -1 = fixed draw count with failing draws skipped, which converges to the "
@@ -756,20 +759,27 @@ def main():
     # (1) the pipeline of the reference with tile lists (the parity-tested path): timed, roofline
     use_tiles = args.gather == "tiles"
 
+    def fork_at(where, p, fb, evd):
+        """Forks the direct-light pass at the frame stage --direct-fork names."""
+        if where == args.direct_fork:
+            fork_direct(p, fb, evd)
+
     def plain_step(k, ev, evd, hist=True):
         p, fb = plain, plain.fb
         prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
         out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
-        fork_direct(p, fb, evd)
+        fork_at("start", p, fb, evd)
         p.cull()
+        fork_at("after-cull", p, fb, evd)
         rec(1)
         if use_tiles:
             p.gi.cull_tiles(p.g, fb.vpl_updated, fb.visible, fb.tiles)
         rec(2)
         p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, fb.tiles if use_tiles else None,
                     fb.color, fb.reservoir)
+        fork_at("after-gather", p, fb, evd)
         rec(3)
         p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined, out_gi,
                      fb.reservoir_out, out_h)
@@ -797,11 +807,13 @@ def main():
             out_gi, out_h = p.hist[(k + 1) % 2]
             rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
             rec(0)
-            fork_direct(p, fb, evd)
+            fork_at("start", p, fb, evd)
             p.cull()
+            fork_at("after-cull", p, fb, evd)
             rec(1)
             p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                            fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S, rejection_limit=args.rejection_limit)
+            fork_at("after-gather", p, fb, evd)
             rec(2)
             p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                          out_gi, fb.reservoir_out, out_h)
@@ -817,10 +829,12 @@ def main():
         out_gi, out_h = p.hist[(k + 1) % 2]
         rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
         rec(0)
-        fork_direct(p, fb, evd)
+        fork_at("start", p, fb, evd)
         p.cull()
+        fork_at("after-cull", p, fb, evd)
         rec(1)
         p.gi.gather(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color, fb.reservoir)
+        fork_at("after-gather", p, fb, evd)
         rec(2)
         p.gi.denoise(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, fb.combined,
                      out_gi, fb.reservoir_out, out_h)
@@ -894,12 +908,14 @@ def main():
                 out_gi, out_h = p.hist[(k + 1) % 2]
                 rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
                 rec(0)
-                fork_direct(p, fb, evd)
+                fork_at("start", p, fb, evd)
                 p.cull()
+                fork_at("after-cull", p, fb, evd)
                 rec(1)
                 p.gi.gather_ex(p.g, fb.vpl_updated, fb.visible, fb.visible_entries, p.cubes, None, fb.color,
                                fb.reservoir, mode=_abi.GATHER_SAMPLES, samples=S,
                                rejection_limit=args.rejection_limit)
+                fork_at("after-gather", p, fb, evd)
                 rec(2)
                 p.gi.denoise_learned(p.g, None, fb.visible, fb.direct, fb.color, fb.reservoir, prev_gi, prev_h, net,
                                      thr, ws, fb.combined, out_gi, fb.reservoir_out, out_h, fbt)

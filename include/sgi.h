@@ -237,8 +237,8 @@ int sgi_gbuffer_pack(sgi_ctx* ctx, const sgi_gbuffer* planes, const sgi_gbuffer_
  * min(count, SGI_MAX_TOTAL_VPLS_PER_FRAME).  (The reference compacts with
  * atomics, so its order is nondeterministic; ascending order is one of the
  * orders it can produce.)  Needs sgi_set_frame and sgi_set_csm.
- * vplData: device [n]; vplUpdatedData: device [min(n, 4480)];
- * visibleIndices: device [1 + min(n, 4480)].  0 <= n <= 10000. */
+ * vplData: device [n]; vplUpdatedData: device [min(n, 4480)] (both 16-byte
+ * aligned); visibleIndices: device [1 + min(n, 4480)].  0 <= n <= 10000. */
 int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n,
                     sgi_vpl* vplUpdatedData, int32_t* visibleIndices, void* stream);
 

@@ -261,6 +261,8 @@ int sgi_cull_global(sgi_ctx* ctx, const sgi_vpl* vplData, int32_t n, sgi_vpl* vp
         return fail(SGI_E_INVALID, "sgi_cull_global: n must be within 0..10000");
     if (!visibleIndices || (n > 0 && (!vplData || !vplUpdatedData)))
         return fail(SGI_E_INVALID, "sgi_cull_global: NULL buffer");
+    if (n > 0 && (!aligned16(vplData) || !aligned16(vplUpdatedData)))
+        return fail(SGI_E_INVALID, "sgi_cull_global: VPL records must be 16-byte aligned");
     const hipError_t e = sgi_launch::cullGlobal(ctx->csmArgs, ctx->frameArgs, vplData, n, vplUpdatedData,
                                                 visibleIndices, ctx->visFlags, (hipStream_t)stream);
     return e == hipSuccess ? SGI_OK : hipFail(e, "sgi_cull_global");

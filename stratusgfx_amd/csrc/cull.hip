@@ -28,15 +28,16 @@ using namespace sgi_dev;
 
 constexpr int kCullThreads = 1024;
 
-// Per-VPL sun visibility, one thread per VPL over many blocks (each VPL's 36
-// cascade compares read scattered texels; spreading them over CUs keeps the
+// Per-VPL sun visibility, one thread per VPL over many blocks (each VPL's two
+// 4x4 cascade texel blocks are scattered; spreading them over CUs keeps the
 // loads in flight on many L1s).
 __device__ __forceinline__ void cullVisibilityBody(const CsmArgs& csm, f3 sun, const sgi_vpl* __restrict__ in, int n,
                                                    uint8_t* __restrict__ flags, int i) {
     if (i >= n) return;
     const f3 p = ld3(in[i].position);
     const f3 blends = cascadeBlends(csm, p);
-    const float shadowFactor = 1.0f - infiniteShadowValue(csm, sun, p, blends, sun, 0.0f);
+    // the 4x4-block form of the 36 cascade compares (one read per texel; the same value)
+    const float shadowFactor = 1.0f - infiniteShadowValueBlock(csm, sun, p, blends, sun, 0.0f);
     flags[i] = shadowFactor < 1.0f ? 1 : 0;
 }
 
@@ -52,32 +53,50 @@ __global__ __launch_bounds__(64) void cullVisibilityViewsKernel(const sgi_batch:
     cullVisibilityBody(d.csm, ld3(d.fr.sunDir), in, n, d.visFlags, (blockIdx.x % bpv) * 64 + threadIdx.x);
 }
 
-// Stable compaction of the flags: wave ballots and a block prefix, in input order.
+// Stable compaction of the flags: wave ballots and a block prefix, in input
+// order. The block takes kCompactRounds chunks of kCullThreads items at a
+// time: the chunks' flags are loaded together before the one barrier, then
+// positions follow chunk, wave and lane order (each chunk's items after all
+// earlier chunks') and the visible records are copied. Few registers, so the
+// block is placed beside a large kernel running on another stream.
+constexpr int kCompactRounds = 4;
 __device__ __forceinline__ void cullCompactBody(const sgi_vpl* __restrict__ in, int n, const uint8_t* __restrict__ flags,
                                                 sgi_vpl* __restrict__ out, int32_t* __restrict__ visible,
-                                                int* waveCount) {
+                                                int (*waveCount)[kCullThreads / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     int running = 0;
-    for (int c0 = 0; c0 < n; c0 += kCullThreads) {
-        const int i = c0 + threadIdx.x;
-        const bool vis = i < n && flags[i] != 0;
-        const uint64_t m = __ballot(vis);
-        const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-        const int lanePrefix = __popcll(below);
-        if (lane == 0) waveCount[wave] = __popcll(m);
+    for (int c0 = 0; c0 < n; c0 += kCullThreads * kCompactRounds) {
+        uint8_t f[kCompactRounds];
+#pragma unroll
+        for (int k = 0; k < kCompactRounds; ++k) {
+            const int i = c0 + k * kCullThreads + (int)threadIdx.x;
+            f[k] = i < n ? flags[i] : 0;
+        }
+        int lanePrefix[kCompactRounds];
+#pragma unroll
+        for (int k = 0; k < kCompactRounds; ++k) {
+            const uint64_t m = __ballot(f[k] != 0);
+            lanePrefix[k] = __popcll(m & below);
+            if (lane == 0) waveCount[k][wave] = __popcll(m);
+        }
         __syncthreads();
-        int wavePrefix = 0, total = 0;
-        for (int w = 0; w < kCullThreads / 64; ++w) {
-            const int c = waveCount[w];
-            if (w < wave) wavePrefix += c;
-            total += c;
+#pragma unroll
+        for (int k = 0; k < kCompactRounds; ++k) {
+            int wavePrefix = 0, total = 0;
+            for (int w = 0; w < kCullThreads / 64; ++w) {
+                const int c = waveCount[k][w];
+                if (w < wave) wavePrefix += c;
+                total += c;
+            }
+            const int pos = running + wavePrefix + lanePrefix[k];
+            if (f[k] != 0 && pos < SGI_MAX_TOTAL_VPLS_PER_FRAME) {
+                const int i = c0 + k * kCullThreads + (int)threadIdx.x;
+                out[pos] = in[i];
+                visible[1 + pos] = i;
+            }
+            running += total;
         }
-        const int pos = running + wavePrefix + lanePrefix;
-        if (vis && pos < SGI_MAX_TOTAL_VPLS_PER_FRAME) {
-            out[pos] = in[i];
-            visible[1 + pos] = i;
-        }
-        running += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) visible[0] = running < SGI_MAX_TOTAL_VPLS_PER_FRAME ? running : SGI_MAX_TOTAL_VPLS_PER_FRAME;
@@ -87,14 +106,14 @@ __global__ __launch_bounds__(kCullThreads) void cullCompactKernel(const sgi_vpl*
                                                                   const uint8_t* __restrict__ flags,
                                                                   sgi_vpl* __restrict__ out,
                                                                   int32_t* __restrict__ visible) {
-    __shared__ int waveCount[kCullThreads / 64];
+    __shared__ int waveCount[kCompactRounds][kCullThreads / 64];
     cullCompactBody(in, n, flags, out, visible, waveCount);
 }
 
 // Batched: one block per view.
 __global__ __launch_bounds__(kCullThreads) void cullCompactViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
                                                                        const sgi_vpl* __restrict__ in, int n) {
-    __shared__ int waveCount[kCullThreads / 64];
+    __shared__ int waveCount[kCompactRounds][kCullThreads / 64];
     const sgi_batch::ViewDesc& d = views[blockIdx.x];
     cullCompactBody(in, n, d.visFlags, d.vplUpdated, d.visible, waveCount);
 }

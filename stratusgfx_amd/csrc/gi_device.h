@@ -260,38 +260,6 @@ __device__ __forceinline__ f3 cascadeBlends(const CsmArgs& c, f3 p) {
     return f3{b[0], b[1], b[2]};
 }
 
-// pbr.glsl:158-227 calculateInfiniteShadowValue; `bias` already zero when unused.
-__device__ __forceinline__ float infiniteShadowValue(const CsmArgs& c, f3 sunDir, f3 fragPos, f3 blends, f3 normal,
-                                                     float bias) {
-    const f3 position = fragPos + normal * (1.0f - saturate(dot(normal, sunDir))) * 1.0f;
-    const bool beyond2 = blends.y >= 0.0f, beyond3 = blends.z >= 0.0f;
-    const int index1 = beyond2 ? 2 : 0, index2 = beyond3 ? 3 : 1;
-    float c1[4], c2[4];
-    mulMat4(c.projViews[index1], position.x, position.y, position.z, 1.0f, c1);
-    mulMat4(c.projViews[index2], position.x, position.y, position.z, 1.0f, c2);
-    const float s1 = (c1[0] / c1[3]) * 0.5f + 0.5f, t1 = (c1[1] / c1[3]) * 0.5f + 0.5f;
-    const float depth1 = (c1[2] / c1[3]) * 0.5f + 0.5f;
-    const float s2 = (c2[0] / c2[3]) * 0.5f + 0.5f, t2 = (c2[1] / c2[3]) * 0.5f + 0.5f;
-    float depth2 = (c2[2] / c2[3]) * 0.5f + 0.5f;
-    depth2 = beyond3 ? saturate(depth2) : depth2;
-    const float weight = beyond2 ? saturate(blends.y) - saturate(blends.z) : 1.0f - saturate(blends.x);
-    const float wh = 1.0f / (float)c.res;
-    float light1 = 0.0f, light2 = 0.0f, samples = 0.0f;
-    // the reference loops a float from -1 to 1; integer counters give the same
-    // offsets and let the 36 compares' loads issue together
-#pragma unroll
-    for (int iy = -1; iy <= 1; ++iy) {
-#pragma unroll
-        for (int ix = -1; ix <= 1; ++ix) {
-            const float x = (float)ix, y = (float)iy;
-            light1 += shadowCompare(c, s1 + x * wh, t1 + y * wh, index1, depth1 - bias);
-            light2 += shadowCompare(c, s2 + x * wh, t2 + y * wh, index2, depth2 - bias);
-            samples += 1.0f;
-        }
-    }
-    return mixv(light2, light1, weight) * (1.0f / samples);
-}
-
 // One cascade's 9 PCF taps (pbr.glsl:166-174), each the bilinear blend of 4
 // LEQUAL compares, summed in the reference's (y, x) order.  The taps sit one
 // texel apart, so together they cover a 4x4 texel block: the block is read and
@@ -343,7 +311,9 @@ __device__ __forceinline__ float pcf9Block(const CsmArgs& c, float s, float t, i
     return light;
 }
 
-// calculateInfiniteShadowValue with the block-read PCF (same result).
+// pbr.glsl:158-227 calculateInfiniteShadowValue with the block-read PCF (the
+// 9 taps of each of the two cascades from one 4x4 texel block: the same value
+// as 36 single compares); `bias` already zero when unused.
 __device__ __forceinline__ float infiniteShadowValueBlock(const CsmArgs& c, f3 sunDir, f3 fragPos, f3 blends, f3 normal,
                                                           float bias) {
     const f3 position = fragPos + normal * (1.0f - saturate(dot(normal, sunDir))) * 1.0f;

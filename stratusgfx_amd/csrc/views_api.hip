@@ -222,6 +222,8 @@ int sgi_views_set(sgi_views* v, int32_t numViews, const sgi_view_io* io, const s
     if (numVpls < 0 || numVpls > SGI_MAX_TOTAL_VPLS_BEFORE_CULLING)
         return fail(SGI_E_INVALID, "sgi_views_set: numVpls must be within 0..10000");
     if (numVpls > 0 && (!vpls || !shadowIndices)) return fail(SGI_E_INVALID, "sgi_views_set: NULL VPL buffer");
+    if (numVpls > 0 && ((uintptr_t)vpls & 15u) != 0)
+        return fail(SGI_E_INVALID, "sgi_views_set: VPL records must be 16-byte aligned");
     if (cubes->resolution <= 0 || cubes->resolution > 4096 ||
         (cubes->format != SGI_TEXEL_F32 && cubes->format != SGI_TEXEL_F16 && cubes->format != SGI_TEXEL_UNORM16) ||
         cubes->layers_per_atlas <= 0 ||

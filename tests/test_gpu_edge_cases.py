@@ -109,6 +109,20 @@ def test_visible_cap_4480(case):
     assert np.array_equal(d_out.cpu().numpy().view(np.uint32), o_upd.view(np.uint32))
 
 
+def test_cull_refuses_misaligned_records(case):
+    """The compaction moves records as 16-byte vectors: record buffers off 16-byte alignment are refused."""
+    torch = require_gpu()
+    gi = _ctx(case, 16, 12)
+    raw = torch.zeros(64 * 8 + 16, dtype=torch.uint8, device="cuda")
+    out = torch.zeros((8, 16), dtype=torch.float32, device="cuda")
+    vis = torch.zeros((9,), dtype=torch.int32, device="cuda")
+    lib = gi.lib
+    assert lib.sgi_cull_global(gi.ctx, raw.data_ptr() + 4, 8, out.data_ptr(), vis.data_ptr(), None) != 0
+    assert b"16-byte aligned" in lib.sgi_last_error()
+    assert lib.sgi_cull_global(gi.ctx, raw.data_ptr(), 8, out.data_ptr(), vis.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+
+
 def test_temporal_history_two_frames(case):
     """Frame 2 reuses frame 1's history (same view): history length 2, gi = mix(prev, cur, 1/2)."""
     torch = require_gpu()
