@@ -156,15 +156,17 @@ __global__ void featuresKernel(GbufArgs g, FrameArgs fr, const float4* __restric
 constexpr float kLog2e = 1.44269504088896f;
 
 // Softmax weights of the 25 filter logits (max-subtracted; exp through the
-// hardware exp2 of the log2(e)-scaled difference, all arguments <= 0).
+// hardware exp2 of lg log2(e) - m log2(e), one FMA per tap: every argument is
+// at most the rounding error of m log2(e), so no weight exceeds 1 + 2^-23).
 __device__ __forceinline__ void softmax25(const float* lg, float* wgt) {
     float m = lg[0];
 #pragma unroll
     for (int j = 1; j < SGI_KPCN_TAPS; ++j) m = gmax(m, lg[j]);
+    const float mScaled = m * kLog2e;
     float s = 0.0f;
 #pragma unroll
     for (int j = 0; j < SGI_KPCN_TAPS; ++j) {
-        wgt[j] = __builtin_amdgcn_exp2f((lg[j] - m) * kLog2e);
+        wgt[j] = __builtin_amdgcn_exp2f(__builtin_fmaf(lg[j], kLog2e, -mScaled));
         s += wgt[j];
     }
     const float inv = 1.0f / s;
@@ -656,6 +658,9 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
             if constexpr (kLdsUp) stageUp<AIN>(src, H, W, tile + 1, tilesX, upS);
             if constexpr (OUT == OUT_APPLY) loadApply(src.radiance, src.coarse, H, W, tile + 1, tilesX, apl);
         }
+        // the bias is added to the finished sums (sum + bias, the oracle's order: starting the
+        // sums from the bias moves FP32 results by an ulp, which decides BF16 rounding ties
+        // differently, and the coarse levels' pooled values sit on ties often)
         f32x4 acc[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
