@@ -564,10 +564,21 @@ struct MfmaOp<ACT_F8> {
 // (otherwise acc + bias).
 // Blocks resident per CU: two (VGPRs near 256, and for OUT_APPLY the staged blocks in LDS),
 // three for FP8 inputs on the plain path (FP8 weights take 36 VGPRs instead of 72 and the
-// FP8 halo 15.5 KB of LDS; the deferred up-add's raw chunks would not fit the 168 VGPRs).
+// FP8 halo 15.5 KB of LDS; the deferred up-add's raw chunks would not fit the 168 VGPRs)
+// and for the first convolution with its weights in LDS.
+// The first convolution (BF16, 16-channel features in) keeps its weight fragments in
+// LDS, one copy per block (9 taps x 2 channel blocks x 64 lanes x 16 B = 18 KB), each
+// wave reading a tap's two fragments before its MFMAs: without the 72 VGPRs they take in
+// registers it runs three blocks per CU (75 -> 72 us at 1080p). On the other layers the
+// extra LDS round trip before the first MFMA cost more than the third block gave.
+template <int IN, int AIN>
+constexpr bool weightsInLds() {
+    return AIN == ACT_BF16 && IN == IN_PLAIN16;
+}
+
 template <int IN, int OUT, int AIN>
 constexpr int convBlocksPerCu() {
-    return AIN == ACT_F8 && OUT != OUT_APPLY && IN == IN_PLAIN ? 3 : 2;
+    return OUT != OUT_APPLY && ((AIN == ACT_F8 && IN == IN_PLAIN) || weightsInLds<IN, AIN>()) ? 3 : 2;
 }
 
 template <int IN, int OUT, int AIN, int AOUT>
@@ -588,21 +599,32 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
     constexpr bool kDefer = IN == IN_UPADD && !kLdsUp;
     constexpr int kHaloIn = kLdsUp ? IN_PLAIN : IN;  // the skip operand loads as a plain input
     __shared__ uint4 upS[IN == IN_UPADD && OUT == OUT_APPLY ? UpStage<AIN>::kSlots * 256 : 1];
+    constexpr bool kWLds = weightsInLds<IN, AIN>();
+    __shared__ uint4 wS[kWLds ? 9 * 2 * 64 : 1];  // fragment (t, mb) of lane l at (2 t + mb) 64 + l
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
     // A operands: W[t][mb*16 + l16][8 lg .. 8 lg + 7]
-    typename MfmaOp<AIN>::T wa[9][2];
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-            const size_t e = (size_t)(t * kC + mb * 16 + l16) * kC + 8 * lg;
-            if constexpr (AIN == ACT_F8)
-                wa[t][mb] = *reinterpret_cast<const long*>(static_cast<const uint8_t*>(w) + e);
-            else
-                wa[t][mb] = asBf16x8(*reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(w) + e));
+    typename MfmaOp<AIN>::T wa[kWLds ? 1 : 9][2];
+    if constexpr (kWLds) {
+        // visible to every wave after the first tile's barriers, before its first MFMA
+        for (int i = (int)threadIdx.x; i < 9 * 2 * 64; i += 256) {
+            const int t = i >> 7, mb = (i >> 6) & 1, ln = i & 63;
+            const size_t e = (size_t)(t * kC + mb * 16 + (ln & 15)) * kC + 8 * (ln >> 4);
+            wS[i] = *reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(w) + e);
         }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                const size_t e = (size_t)(t * kC + mb * 16 + l16) * kC + 8 * lg;
+                if constexpr (AIN == ACT_F8)
+                    wa[t][mb] = *reinterpret_cast<const long*>(static_cast<const uint8_t*>(w) + e);
+                else
+                    wa[t][mb] = asBf16x8(*reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(w) + e));
+            }
+    }
     float k1v[2][4], bv[2][4];  // scaled: (k1, k2); otherwise bv = bias
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -669,6 +691,14 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int dy = t / 3, dx = t % 3;
+            typename MfmaOp<AIN>::T a0, a1;
+            if constexpr (kWLds) {
+                a0 = ldsBf16x8(wS + (2 * t) * 64 + lane);
+                a1 = ldsBf16x8(wS + (2 * t + 1) * 64 + lane);
+            } else {
+                a0 = wa[t][0];
+                a1 = wa[t][1];
+            }
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 const int row = wave * 4 + nb;
@@ -677,12 +707,12 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                     // 8 FP8 channels 8 lg .. 8 lg + 7: one 8-byte LDS read
                     const long b = *reinterpret_cast<const long*>(reinterpret_cast<const char*>(halo + hp * A::kPxU4) +
                                                                   8 * lg);
-                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(wa[t][0], b, acc[0][nb], 0, 0, 0);
-                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(wa[t][1], b, acc[1][nb], 0, 0, 0);
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a0, b, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a1, b, acc[1][nb], 0, 0, 0);
                 } else {
                     const bf16x8 b = ldsBf16x8(halo + hp * A::kPxU4 + lg);
-                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], b, acc[0][nb], 0, 0, 0);
-                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], b, acc[1][nb], 0, 0, 0);
+                    acc[0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][nb], 0, 0, 0);
+                    acc[1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][nb], 0, 0, 0);
                 }
             }
         }
