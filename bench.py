@@ -32,8 +32,10 @@ history buffer set) and replayed, checked bit-equal to the eager frame first;
 the per-stage `kernels_ms` come from an eager run with HIP events between the
 stages.  The learned form is also run on the 1024-VPL scene (same geometry),
 whose own converged render lies `converged_1024_vs_4096_relmse` from the
-target, and at the operating point without the fallback and on the packed
-G-buffer attachments.  relMSE = mean of
+target: each sample count against the 4096-VPL a-trous sweep, and
+(`same_scene`) against an a-trous sweep on the 1024-VPL scene, both forms
+scored against the 1024-VPL converged render.  The operating point is also run
+without the fallback and on the packed G-buffer attachments.  relMSE = mean of
 (x - r)^2 / (r^2 + 1e-2) in FP64 over the GI output after one frame, against
 the converged render (every visible VPL of the 4096-VPL scene, FP64 sums).
 Inputs are
@@ -67,7 +69,7 @@ def parse():
     ap.add_argument("--learned-spp", type=int, nargs="+", default=[8, 12, 16, 24, 32],
                     help="gather samples per pixel of the learned form (the operating point is chosen among them)")
     ap.add_argument("--atrous-spp", type=int, nargs="+",
-                    default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256, 320],
+                    default=[2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256, 320, 384, 448],
                     help="samples per pixel of the a-trous form's sweep")
     ap.add_argument("--sweep-steps", type=int, default=20, help="timed frames per a-trous sweep point")
     ap.add_argument("--direct-fork", choices=["start", "after-cull", "after-gather"], default="start",
@@ -800,9 +802,9 @@ def main():
 
     # (2) the a-trous form at equal relMSE: the reference's hashed estimator over every
     # visible VPL at S samples per pixel + the a-trous denoiser
-    def atrous_step(S):
+    def atrous_step(S, p=plain):
         def step(k, ev, evd, hist=True):
-            p, fb = plain, plain.fb
+            fb = p.fb
             prev_gi, prev_h = p.hist[k % 2] if hist else (None, None)
             out_gi, out_h = p.hist[(k + 1) % 2]
             rec = (lambda i: ev[i].record()) if ev is not None else (lambda i: None)
@@ -856,18 +858,24 @@ def main():
         print(f"[bench] reference estimator: relMSE {rm:.5f}, {args.sweep_steps / el_r:.1f} frames/s",
               file=sys.stderr, flush=True)
 
-    sweep = []
-    for S in (args.atrous_spp if args.forms == "all" else []):
-        st = atrous_step(S)
-        g_s = st(0, None, None, hist=False)
-        torch.cuda.synchronize()
-        rm = rel_mse(g_s, target)
-        _, per_s, direct_s = timed_loop(st, args.sweep_steps, 2, 3)
-        el_s = graph_loop(st, args.sweep_steps, 2, list(plain.hist[0]), [plain.hist[1][0], plain.fb.combined])
-        sweep.append({"spp": S, "relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_s, 2),
-                      "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_s, direct_s)})
-        print(f"[bench] a-trous at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
-              file=sys.stderr, flush=True)
+    def atrous_sweep(p, tgt, label):
+        out = []
+        for S in (args.atrous_spp if args.forms == "all" else []):
+            st = atrous_step(S, p)
+            g_s = st(0, None, None, hist=False)
+            torch.cuda.synchronize()
+            rm = rel_mse(g_s, tgt)
+            _, per_s, direct_s = timed_loop(st, args.sweep_steps, 2, 3)
+            el_s = graph_loop(st, args.sweep_steps, 2, list(p.hist[0]), [p.hist[1][0], p.fb.combined])
+            out.append({"spp": S, "relmse": round(rm, 6), "frames_per_s": round(args.sweep_steps / el_s, 2),
+                        "kernels_ms": stage_ms(["cull_global", "gather", "denoise"], per_s, direct_s)})
+            print(f"[bench] a-trous ({label}) at {S} spp: relMSE {rm:.5f}, {args.sweep_steps / el_s:.1f} frames/s",
+                  file=sys.stderr, flush=True)
+        return out
+
+    sweep = atrous_sweep(plain, target, "4096 VPLs")
+    # the same sweep on the 1024-VPL scene against its own converged render
+    sweep_1024 = atrous_sweep(learned, target_1024, "1024 VPLs")
 
     # ---- learned form ----------------------------------------------------------
     # on the a-trous form's 4096-VPL scene (same VPLs, same sampler: only the sample count and the
@@ -997,18 +1005,21 @@ def main():
                                                      "fallback_tiles")}
         learned_packed["note"] = ("every pass reads RGB8 normals, RGBA8 albedo, RG8 reflectivity, RGB8 "
                                   "roughness / metallic and RG16F velocity directly; depth FP32")
-        # the 1024-VPL scene: each sample count against the 4096-VPL a-trous sweep at equal relMSE to
-        # the 4096-VPL converged render (whose distance from the 1024-VPL converged render,
-        # converged_1024_vs_4096_relmse, floors the learned form's relMSE), and, each form against its
-        # own VPL set's converged render, at equal relMSE to that
+        # the 1024-VPL scene: each sample count (1) against the 1024-VPL a-trous sweep, both forms on
+        # the same scene at equal relMSE to its own converged render (same_scene), and (2) against the
+        # 4096-VPL a-trous sweep at equal relMSE to the 4096-VPL converged render (whose distance from
+        # the 1024-VPL one, converged_1024_vs_4096_relmse, floors the learned form's relMSE there)
         for S in args.learned_spp:
             r, _ = learned_form(learned, "1024 VPLs", S, args.sweep_steps, 2, P_op, own_target=target_1024)
             r.update(atrous_at_equal(sweep, r))
-            own = atrous_at_equal(sweep, {"relmse": r["relmse_vs_own_converged"], "frames_per_s": r["frames_per_s"]})
-            r["each_vs_own_converged"] = {k: own[k] for k in ("atrous_at_equal_relmse",
-                                                              "speedup_vs_atrous_at_equal_relmse") if k in own}
+            own = atrous_at_equal(sweep_1024, {"relmse": r["relmse_vs_own_converged"],
+                                               "frames_per_s": r["frames_per_s"]})
+            r["same_scene"] = {k: own[k] for k in ("atrous_at_or_below_relmse", "atrous_above_relmse",
+                                                   "atrous_at_equal_relmse", "speedup_vs_atrous_at_equal_relmse")
+                               if k in own}
             learned_1024_sweep.append(r)
-        learned_1024 = max(learned_1024_sweep, key=lambda r: r.get("speedup_vs_atrous_at_equal_relmse", 0.0))
+        learned_1024 = max(learned_1024_sweep,
+                           key=lambda r: r["same_scene"].get("speedup_vs_atrous_at_equal_relmse", 0.0))
 
     gather_roof = None
     if use_tiles and per_plain is not None:
@@ -1083,6 +1094,7 @@ def main():
         "learned_packed_gbuffer": learned_packed,
         "learned_1024_vpls": learned_1024,
         "learned_1024_sweep": learned_1024_sweep,
+        "atrous_1024_sweep": [{k: v for k, v in r.items() if k != "kernels_ms"} for r in sweep_1024],
         "atrous_sweep": sweep,
         "plain_tile_lists": plain_res,
         "reference_estimator": random_res,
