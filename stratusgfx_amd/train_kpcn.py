@@ -197,16 +197,21 @@ def evaluate(wts: kpcn.Weights, val, thresholds):
     return res
 
 
-def pick_threshold(val: dict, thresholds, tol=1e-3):
+def pick_threshold(val: dict, thresholds, tol=1e-3, max_tiles=0.01):
     """The fallback threshold from a validation result (evaluate()): the smallest threshold whose
-    relMSE is within `tol` (relative) of the best of the network alone and every threshold, so that
-    the fallback acts as early as held-out data allows without costing more than `tol`; +inf when
-    no threshold qualifies.  Returns (threshold, its validation relMSE)."""
+    relMSE is within `tol` (relative) of the best of the network alone and every threshold, and at
+    which at most `max_tiles` of the held-out tiles fall back, so that the fallback acts as early as
+    held-out data allows without costing more than `tol` in error, while on data like the training
+    data (where the network rarely fails) it stays rare and costs little time; +inf when no
+    threshold qualifies.  Returns (threshold, its validation relMSE)."""
     rel = {float("inf"): val["network"]}
     for t in thresholds:
-        rel[t] = val["fallback"][f"{t:g}"]["relmse"]
-    best = min(rel.values())
-    t = min(k for k, v in rel.items() if v <= best * (1.0 + tol))
+        f = val["fallback"][f"{t:g}"]
+        if f["tiles"] <= max_tiles:
+            rel[t] = f["relmse"]
+    best = min([val["network"]] + [val["fallback"][f"{t:g}"]["relmse"] for t in thresholds])
+    ok = [k for k, v in rel.items() if v <= best * (1.0 + tol)]
+    t = min(ok) if ok else float("inf")
     return t, rel[t]
 
 

@@ -20,10 +20,11 @@ and the a-trous form's on the same input (with its own temporal chain): the
 FP8 network's within 10 % of the BF16 network's, and both below the a-trous
 form's, on every frame.  With the shipped fallback threshold (the network's
 error channel predicting the ratio of its error to the a-trous filter's),
-part of every frame falls back to the a-trous filter, and each network's relMSE
-stays within 0.1 % of its own relMSE without the fallback on every frame (the
-tolerance the threshold was chosen with on held-out blocks; on this path the
-shipped network's fallback moves it by a few 1e-5 relative either way)."""
+tiles of the path fall back to the a-trous filter (the threshold keeps the
+fallback under 1 % of held-out tiles, so in distribution it is rare; this path
+is outside it), never every tile, and each network's relMSE stays within 0.1 %
+of its own relMSE without the fallback on every frame (the tolerance the
+threshold was chosen with on held-out blocks)."""
 import ctypes
 import dataclasses
 import json
@@ -240,9 +241,9 @@ def test_learned_sequence_fp8(seq):
     assert all(a < b for a, b in zip(rel["fp8"], rel["atrous"]))
     assert all(a < b for a, b in zip(rel["bf16"], rel["atrous"]))
     # the fallback: off at +inf; at the shipped threshold it takes the a-trous filter on part of
-    # every frame of this path (4K, temporal, outside the training views); the threshold is the
-    # smallest whose held-out relMSE was within 0.1 % of the best, and the fallback stays within that
+    # this path (4K, temporal, outside the training views); the threshold is the smallest whose
+    # held-out relMSE was within 0.1 % of the best, and the fallback stays within that
     for name in nets:
         assert all(n == 0 for n in fallback[name])
-        assert all(0 < n < n_tiles for n in fallback[name + "+fallback"])
+        assert sum(fallback[name + "+fallback"]) > 0 and all(n < n_tiles for n in fallback[name + "+fallback"])
         assert all(a <= b * (1.0 + 1e-3) for a, b in zip(rel[name + "+fallback"], rel[name])), name

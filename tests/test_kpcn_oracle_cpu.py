@@ -208,3 +208,26 @@ def test_ratio_error_target():
     # an alternative equal to the output: target 0
     _, _, de2 = ko.loss(out, err, t, 0.5, out)
     assert np.allclose(de2, 2 * 0.5 * err / (H * W), rtol=1e-12, atol=1e-15)
+
+
+def test_fallback_threshold_rule():
+    """train_kpcn.pick_threshold: the smallest candidate within 0.1 % of the best held-out relMSE
+    (the network alone included) at which at most 1 % of the held-out tiles fall back; +inf when
+    none qualifies."""
+    from stratusgfx_amd.train_kpcn import pick_threshold
+
+    def table(net, rows):
+        return {"network": net, "fallback": {f"{t:g}": {"relmse": r, "tiles": f} for t, r, f in rows}}
+
+    th = [0.5, 1.0, 1.2, 2.0]
+    # the network alone is best: the earliest threshold within the tolerance and the tile budget
+    v = table(1.0, [(0.5, 1.5, 0.7), (1.0, 1.0005, 0.05), (1.2, 1.0004, 0.005), (2.0, 1.0, 0.0)])
+    assert pick_threshold(v, th) == (1.2, 1.0004)
+    assert pick_threshold(v, th, max_tiles=1.0) == (1.0, 1.0005)
+    # a fallback that helps sets the bar, and a candidate beyond the tolerance is skipped
+    v = table(1.0, [(0.5, 0.9, 0.009), (1.0, 0.95, 0.005), (1.2, 0.9008, 0.001), (2.0, 1.0, 0.0)])
+    assert pick_threshold(v, th) == (0.5, 0.9)
+    assert pick_threshold(v, th, tol=0.0, max_tiles=0.005) == (float("inf"), 1.0)
+    # nothing within the tile budget: the network alone
+    v = table(1.0, [(t, 0.5, 0.5) for t in th])
+    assert pick_threshold(v, th) == (float("inf"), 1.0)
