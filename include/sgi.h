@@ -405,7 +405,7 @@ int sgi_gather_ex(sgi_ctx* ctx, const sgi_gbuffer* gbuffer, const sgi_vpl* vpls,
  * loops, harnesses that forbid hipMalloc).  It needs no context and calls no
  * allocating routine: the per-frame frame block is passed in, and the packed
  * visible-VPL records its kernels use live in a file-scope device array of
- * 4480 x 64 B (280 KB per GPU, held until the process ends).  Every call on a
+ * 4480 x 72 B (315 KB per GPU, held until the process ends).  Every call on a
  * GPU shares that GPU's array, so calls that overlap on one GPU (other streams,
  * other threads) race; each call rewrites the records it reads before reading
  * them.  maxVisible: the capacity of the caller's visible list (the size of

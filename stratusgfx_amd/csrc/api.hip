@@ -66,12 +66,12 @@ int checkGbuffer(const sgi_ctx* c, const sgi_gbuffer* g, const char* who, bool n
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 // ---- scratch of sgi_gather_samples_noalloc ---------------------------------
-// The packed visible-VPL records the sampled gather reads (sgi_launch::packedVplBytes() = 64 B
-// each), for up to 4480 visible VPLs: 280 KB of device memory per GPU, held until the process
-// ends.  In host code `scratch` names a host-side stand-in, so its device address is looked up
+// The packed visible-VPL records the sampled gather reads with their binary16 positions
+// (sgi_launch::packedVplBytes() = 64 + 8 B each), for up to 4480 visible VPLs: 315 KB of device
+// memory per GPU, held until the process ends.  In host code `scratch` names a host-side stand-in, so its device address is looked up
 // once per device (hipGetSymbolAddress, which allocates nothing) and only that address is used.
 constexpr int kNoAllocMaxVisible = SGI_MAX_TOTAL_VPLS_PER_FRAME;
-constexpr int kNoAllocRecordBytes = 64;
+constexpr int kNoAllocRecordBytes = 72;
 constexpr int kMaxDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kNoAllocMaxVisible * kNoAllocRecordBytes];
 

@@ -82,6 +82,18 @@ __device__ __forceinline__ Vpl unpackVpl(const PackedVpl& p) {
     return v;
 }
 
+// Beside the records the prep buffer holds each visible VPL's position and radius rounded to
+// binary16 (8 B, SGI_MAX_TOTAL_VPLS_PER_FRAME of them after the records): the compacting gather's
+// side / radius tests read this table, a quarter of the records' bytes, so that far more of its
+// random reads hit the vector L1.
+__host__ __device__ inline uint2* halfPositions(PackedVpl* prep) {
+    return reinterpret_cast<uint2*>(prep + SGI_MAX_TOTAL_VPLS_PER_FRAME);
+}
+
+__device__ __forceinline__ uint32_t halfBits(float v) { return __half_as_ushort(__float2half(v)); }
+
+__device__ __forceinline__ float halfValue(uint32_t bits) { return __half2float(__ushort_as_half((unsigned short)bits)); }
+
 template <int FMT>
 __global__ __launch_bounds__(256) void prepVplsKernel(const sgi_vpl* __restrict__ vpls,
                                                      const sgi_atlas_entry* __restrict__ entries,
@@ -89,7 +101,12 @@ __global__ __launch_bounds__(256) void prepVplsKernel(const sgi_vpl* __restrict_
                                                      PackedVpl* __restrict__ out) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     const int count = min(max(visible[0], 0), SGI_MAX_TOTAL_VPLS_PER_FRAME);
-    if (s < count) out[s] = packVpl(loadVpl<FMT>(vpls + s, entries + s, cubes));
+    if (s < count) {
+        const PackedVpl r = packVpl(loadVpl<FMT>(vpls + s, entries + s, cubes));
+        out[s] = r;
+        const float4 q = r.posRadius;  // round to nearest even
+        halfPositions(out)[s] = make_uint2(halfBits(q.x) | halfBits(q.y) << 16, halfBits(q.z) | halfBits(q.w) << 16);
+    }
 }
 
 // vplDiffuseAttenuation (pbr.glsl:260-267) with the light distance known:
@@ -167,6 +184,30 @@ __device__ __forceinline__ bool passesSideRadius(const Pixel& px, f3 lmf, float 
     if (__builtin_expect(signIsSafe(px.N, lmf, nl) && fabsf(dist - radius) > radius * 0x1p-18f, 1))
         return !(nl < 0.0f || dist > radius);
     return !rejectExact(px.N, lmf, d2, radius);
+}
+
+// passesSideRadius's decision from the VPL's binary16 position and radius h (halfPositions):
+// 1 accept, 0 reject, -1 when the rounding to binary16 could change it (the caller then runs
+// passesSideRadius on the FP32 record).  A finite binary16 value v lies within 2^-11 |v| + 2^-25
+// of the FP32 value it rounds, so over the three coordinates dot(N, L - P) and |L - P| move by at
+// most E, the sum of those bounds (|N_i| <= 1), and the radius by its own bound.  A decision is
+// taken only where it clears them by a further 2^-16 relative, far above the FP32 rounding of
+// these sums and above the 2^-20 and 2^-18 margins within which passesSideRadius defers to the
+// reference formula, so it is passesSideRadius's decision; an infinite or NaN value never clears.
+__device__ __forceinline__ int sideRadiusHalf(const Pixel& px, uint2 h) {
+    const float x = halfValue(h.x & 0xFFFFu), y = halfValue(h.x >> 16), z = halfValue(h.y & 0xFFFFu);
+    const float r = halfValue(h.y >> 16);
+    const f3 lmf = f3{x, y, z} - px.P;
+    const float nl = dot(px.N, lmf);
+    const float dist = fsqrt(dot(lmf, lmf));
+    const float E = 0x1.01p-11f * (fabsf(x) + fabsf(y) + fabsf(z)) + 0x1p-23f;
+    const float mag = fabsf(px.N.x * lmf.x) + fabsf(px.N.y * lmf.y) + fabsf(px.N.z * lmf.z) + E;
+    const bool sideSure = fabsf(nl) > E + mag * 0x1p-16f;
+    if (sideSure && nl < 0.0f) return 0;  // behind the surface, whatever the distance
+    const float Er = E + 0x1.01p-11f * fabsf(r) + 0x1p-23f;
+    const bool radiusSure = fabsf(dist - r) > Er + (dist + fabsf(r)) * 0x1p-16f;
+    if (sideSure && radiusSure) return dist > r ? 0 : 1;
+    return -1;
 }
 
 template <int FMT>
@@ -674,6 +715,7 @@ template <int FMT>
 __global__ __launch_bounds__(256) void gatherSamplesCompactKernel(GbufArgs g, FrameArgs fr, CubeArgs cubes, int tilesX,
                                                                  int nTiles, int samples,
                                                                  const PackedVpl* __restrict__ prep,
+                                                                 const uint2* __restrict__ halfPos,
                                                                  const int32_t* __restrict__ visible,
                                                                  float* __restrict__ color,
                                                                  float* __restrict__ reservoir) {
@@ -699,24 +741,34 @@ __global__ __launch_bounds__(256) void gatherSamplesCompactKernel(GbufArgs g, Fr
         uint32_t* wp = pairs[wave];
         for (int d0 = 0; d0 < samples; d0 += kDrawBatch) {
             const int nd = min(kDrawBatch, samples - d0);
-            // A: the batch's slots hashed together and their positions fetched together (eight
-            // loads in flight), then the side / radius tests
+            // A: the batch's slots hashed together and their binary16 positions fetched together
+            // (eight loads in flight), then the side / radius tests; the few that the rounding
+            // leaves open are decided on the FP32 record
             uint32_t slots[kDrawBatch];
-            float4 pos[kDrawBatch];
+            uint2 posH[kDrawBatch];
 #pragma unroll
             for (int k = 0; k < kDrawBatch; ++k) {
                 slots[k] = k < nd ? (uint32_t)drawSlotAll(seedX, seedY, seedZ, count) : 0u;
-                pos[k] = prep[slots[k]].posRadius;
+                posH[k] = halfPos[slots[k]];
             }
-            uint32_t mask = 0;
+            uint32_t mask = 0, open = 0;
 #pragma unroll
             for (int k = 0; k < kDrawBatch; ++k) {
-                if (k < nd) {
-                    const f3 lmf = f3{pos[k].x, pos[k].y, pos[k].z} - px.P;
-                    const float d2 = dot(lmf, lmf);
-                    const float dist = fsqrt(d2);
-                    const float nl = dot(px.N, lmf);
-                    if (live && passesSideRadius(px, lmf, d2, dist, nl, pos[k].w)) mask |= 1u << k;
+                if (k < nd && live) {
+                    const int dcs = sideRadiusHalf(px, posH[k]);
+                    if (dcs > 0) mask |= 1u << k;
+                    else if (dcs < 0) open |= 1u << k;
+                }
+            }
+            if (open) {
+#pragma unroll
+                for (int k = 0; k < kDrawBatch; ++k) {
+                    if (open >> k & 1u) {
+                        const float4 q = prep[slots[k]].posRadius;
+                        const f3 lmf = f3{q.x, q.y, q.z} - px.P;
+                        const float d2 = dot(lmf, lmf);
+                        if (passesSideRadius(px, lmf, d2, fsqrt(d2), dot(px.N, lmf), q.w)) mask |= 1u << k;
+                    }
                 }
             }
 
@@ -795,7 +847,7 @@ hipError_t launchEx(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cube
                                rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
         else if (rejectionLimit < 0)
             hipLaunchKernelGGL((gatherSamplesCompactKernel<FMT>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
-                               rx * ry, samples, prep, visible, color, reservoir);
+                               rx * ry, samples, prep, halfPositions(prep), visible, color, reservoir);
         else
             hipLaunchKernelGGL((gatherSamplesKernel<FMT, false>), dim3(rx * ry), dim3(256), 0, s, g, fr, cubes, rx,
                                rx * ry, samples, rejectionLimit, prep, visible, color, reservoir);
@@ -851,7 +903,7 @@ hipError_t launchPrecision(const GbufArgs& g, const FrameArgs& fr, const CubeArg
 
 namespace sgi_launch {
 
-size_t packedVplBytes() { return sizeof(PackedVpl); }
+size_t packedVplBytes() { return sizeof(PackedVpl) + sizeof(uint2); }  // a record and its binary16 position
 
 hipError_t gather(const GbufArgs& g, const FrameArgs& fr, const CubeArgs& cubes, int format, int tileSize, int tilesX,
                   int tilesY, int K, const sgi_vpl* vpls, const int32_t* visible, const sgi_atlas_entry* entries,

@@ -60,7 +60,7 @@ hipError_t cullTiles(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, i
                      int32_t* tiles, hipStream_t s);
 
 // prep: device scratch of SGI_MAX_TOTAL_VPLS_PER_FRAME * packedVplBytes() (the
-// sampled estimators' per-frame VPL records)
+// sampled estimators' per-frame VPL records, then their binary16 positions)
 size_t packedVplBytes();
 
 hipError_t gather(const sgi_dev::GbufArgs& g, const sgi_dev::FrameArgs& fr, const sgi_dev::CubeArgs& cubes,

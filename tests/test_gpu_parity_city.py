@@ -182,6 +182,10 @@ def test_sampled_gather_whole_frame_4096_vpls():
     got = res.cpu().numpy()
     e = max(orc.rel_l2(col.cpu().numpy(), oc), orc.rel_l2(got, orr))
     exact = float((got.view(np.uint32) == orr.view(np.uint32)).all(-1).mean())
+    # every draw's side / radius and shadow decision is the restatement's: one that differed would
+    # move its pixel's weight (a sum of reweighting factors 0.95 ... 1) far beyond rounding
+    dw = float(np.abs(got[..., 3].astype(np.float64) - orr[..., 3]).max())
     print(f"sampled gather 4096 VPLs ({c} visible), 24 draws, 1920x1080: rel-L2 {e:.3e}; "
-          f"{exact:.4f} of the pixels bit-exact", flush=True)
+          f"{exact:.4f} of the pixels bit-exact; max weight difference {dw:.2e}", flush=True)
     assert e <= REL_L2 and np.abs(orr[..., :3]).sum() > 0
+    assert dw < 1e-3

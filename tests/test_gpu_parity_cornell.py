@@ -182,6 +182,11 @@ def test_gather_ex_modes(case, ora, gi, mode, acc64, rej):
     oc, orr = orc.gather(h.frame, case.hg, ora["upd"], int(ora["vis"][0]), ora["ent"], case.hcubes, mode=omode,
                          precision=0, samples=8, rejection_limit=rej)
     e = orc.rel_l2(res.cpu().numpy(), orr)
-    print(f"gather_ex[{mode}, fp64={acc64}, rejection limit {rej}] rel-L2 {e:.3e}")
+    # the weight channel sums each accepted draw's reweighting factor (0.95 ... 1): one side / radius
+    # or shadow decision that differs from the restatement's moves a pixel's weight by far more than
+    # the rounding of the continuous factors
+    dw = float(np.abs(res[..., 3].cpu().numpy().astype(np.float64) - orr[..., 3]).max())
+    print(f"gather_ex[{mode}, fp64={acc64}, rejection limit {rej}] rel-L2 {e:.3e}, max weight difference {dw:.2e}")
     assert e <= REL_L2
     assert orc.rel_l2(color.cpu().numpy(), oc) <= REL_L2
+    assert dw < 1e-3
