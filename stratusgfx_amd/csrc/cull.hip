@@ -168,37 +168,49 @@ __global__ void gatherEntriesViewsKernel(const sgi_batch::ViewDesc* __restrict__
     if (s < count && s < maxCount) d.entries[s] = all[d.visible[1 + s]];
 }
 
+// Floats of LDS a tile-culling block uses: three per-pixel components at a time, later the
+// stage-2 histogram (256 counters) and compacted slot list (at least 256 entries). Three rather
+// than all six components at once keep 16x16 blocks at 3 KB, which lets a CU hold the eight
+// one-wave blocks per SIMD that cover every tile of a 1080p frame in one round.
+template <int TS>
+constexpr int cullLdsFloats() {
+    return 3 * TS * TS > 256 + 128 ? 3 * TS * TS : 256 + 128;
+}
+
 template <int TS>
 __device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs& fr, int t, int tilesX, int K,
                                              const sgi_vpl* __restrict__ vpls, const int32_t* __restrict__ visible,
                                              int remap, sgi_tile_stage1* __restrict__ stage1,
-                                             int32_t* __restrict__ tiles, float (*comp)[TS * TS], float* sums) {
+                                             int32_t* __restrict__ tiles, float* comp, float* sums) {
     constexpr int NPX = TS * TS;
     const int tx = t % tilesX, ty = t / tilesX;
     const int lane = threadIdx.x;
     const int x0 = tx * TS, y0 = ty * TS;
     const int vw = min(TS, g.W - x0), vh = min(TS, g.H - y0);
 
-    // Stage 1: per-pixel world position and decoded normal.
-    for (int l = lane; l < NPX; l += 64) {
-        const int lx = l % TS, ly = l / TS;
-        if (lx < vw && ly < vh) {
-            const int x = x0 + lx, y = y0 + ly;
-            const size_t p = (size_t)y * g.W + x;
-            const f3 fp = worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV);
-            const f3 nn = gNormal(g, p);
-            comp[0][l] = fp.x; comp[1][l] = fp.y; comp[2][l] = fp.z;
-            comp[3][l] = nn.x; comp[4][l] = nn.y; comp[5][l] = nn.z;
+    // Stage 1: per-pixel world positions, then decoded normals, each summed by the reference's
+    // serial loop (one component per lane).
+    for (int half = 0; half < 2; ++half) {
+        for (int l = lane; l < NPX; l += 64) {
+            const int lx = l % TS, ly = l / TS;
+            if (lx < vw && ly < vh) {
+                const int x = x0 + lx, y = y0 + ly;
+                const size_t p = (size_t)y * g.W + x;
+                const f3 v = half == 0
+                                 ? worldPositionFromDepth(texelCentre(x, g.W), texelCentre(y, g.H), g.depth[p], fr.invPV)
+                                 : gNormal(g, p);
+                comp[l] = v.x; comp[NPX + l] = v.y; comp[2 * NPX + l] = v.z;
+            }
         }
+        __syncthreads();
+        if (lane < 3) {
+            float s = 0.0f;
+            for (int ly = 0; ly < vh; ++ly)
+                for (int lx = 0; lx < vw; ++lx) s += comp[lane * NPX + ly * TS + lx];
+            sums[3 * half + lane] = s;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (lane < 6) {  // the reference's serial loop, one component per lane
-        float s = 0.0f;
-        for (int ly = 0; ly < vh; ++ly)
-            for (int lx = 0; lx < vw; ++lx) s += comp[lane][ly * TS + lx];
-        sums[lane] = s;
-    }
-    __syncthreads();
     const float cnt = (float)(vw * vh);
     const f3 P = f3{sums[0], sums[1], sums[2]} / cnt;
     const f3 Nrm = normalize(f3{sums[3], sums[4], sums[5]} / cnt);
@@ -211,16 +223,90 @@ __device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs&
         stage1[t] = o;
     }
 
-    // Stage 2: sorted top-K in lane registers.
-    float myDist = kFLOAT_MAX;
-    int myIdx = 0;
-    int count = 0;
     const int numVisible = visible[0];
+    // Stage 2a: an upper bound of the K-th smallest candidate ratio, so that stage 2 inserts
+    // little beyond the final list (the insertions, not the ratios, dominate its cost). A
+    // 256-bucket histogram over [0, 1) of estimated ratios (hardware sqrt and reciprocal: within
+    // 2^-21 relative of the correctly rounded ratio) counts only VPLs that are certainly
+    // candidates (the exact side test, estimate <= 1 - 2^-18). If K of them fall in buckets up to
+    // b, K candidates have ratios below T = (b + 1) / 256 times (1 + 2^-21), so the K-th smallest
+    // ratio lies below cap = T (1 + 2^-18): stage 2, started with cap in every slot instead of
+    // FLT_MAX, inserts in slot order only the candidates below cap, a subsequence that holds the
+    // whole stable top-K, and so ends with the same list. The histogram reuses comp (free once
+    // the sums are formed).
+    uint32_t* hist = reinterpret_cast<uint32_t*>(comp);
+    for (int i = lane; i < 256; i += 64) hist[i] = 0u;
+    __syncthreads();
     for (int b = 0; b < numVisible; b += 64) {
         const int s = b + lane;
+        if (s < numVisible) {
+            const f3 lmf = ld3(vpls[s].position) - P;
+            if (!(dot(Nrm, lmf) < 0.0f)) {
+                const float r = __builtin_amdgcn_sqrtf(dot(lmf, lmf)) * __builtin_amdgcn_rcpf(vpls[s].radius);
+                if (r >= 0.0f && r <= 1.0f - 0x1p-18f) atomicAdd(&hist[(int)(r * 256.0f)], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    float cap = kFLOAT_MAX;
+    {
+        const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+        const uint32_t mine = h0 + h1 + h2 + h3;
+        uint32_t incl = mine;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const uint64_t reach = __ballot(incl >= (uint32_t)K);
+        if (reach) {
+            const int L = __builtin_ctzll(reach);
+            const uint32_t e0 = incl - mine;
+            const int within = e0 + h0 >= (uint32_t)K ? 0 : (e0 + h0 + h1 >= (uint32_t)K ? 1 : (e0 + h0 + h1 + h2 >= (uint32_t)K ? 2 : 3));
+            const int bucket = 4 * L + __builtin_amdgcn_readlane(within, L);
+            cap = (float)(bucket + 1) * (1.0f / 256.0f) * (1.0f + 0x1p-18f);
+        }
+    }
+
+    // Stage 2b: the slots that may lie below cap, compacted in slot order into comp after the
+    // histogram (the exact side test, and an estimate within 2^-21 of the ratio kept unless it
+    // exceeds cap by more than 2^-19 relative: every candidate below cap is kept). When there is
+    // no cap, or more slots than fit, stage 2 scans every slot.
+    uint16_t* list = reinterpret_cast<uint16_t*>(hist + 256);
+    constexpr int kListCap = (cullLdsFloats<TS>() - 256) * 2;
+    int n = numVisible;
+    bool useList = false;
+    if (cap < kFLOAT_MAX) {
+        const float keepBelow = cap * (1.0f + 0x1p-19f);
+        int nList = 0;
+        for (int b = 0; b < numVisible; b += 64) {
+            const int s = b + lane;
+            bool keep = false;
+            if (s < numVisible) {
+                const f3 lmf = ld3(vpls[s].position) - P;
+                if (!(dot(Nrm, lmf) < 0.0f))
+                    keep = !(__builtin_amdgcn_sqrtf(dot(lmf, lmf)) * __builtin_amdgcn_rcpf(vpls[s].radius) > keepBelow);
+            }
+            const uint64_t m = __ballot(keep);
+            const int pos = nList + __popcll(m & ((1ull << lane) - 1ull));
+            if (keep && pos < kListCap) list[pos] = (uint16_t)s;
+            nList += __popcll(m);
+        }
+        useList = nList <= kListCap;
+        if (useList) n = nList;
+        __syncthreads();
+    }
+
+    // Stage 2: sorted top-K in lane registers, over the compacted slots or every slot.
+    float myDist = cap;
+    int myIdx = 0;
+    int count = 0;
+    for (int b = 0; b < n; b += 64) {
+        const int i = b + lane;
+        const int s = i < n ? (useList ? (int)list[i] : i) : 0;
         bool cand = false;
         float ratio = 0.0f;
-        if (s < numVisible) {
+        if (i < n) {
             const f3 lmf = ld3(vpls[s].position) - P;
             const float side = dot(Nrm, lmf);
             if (!(side < 0.0f)) {
@@ -239,6 +325,7 @@ __device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs&
             const float thrNow = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myDist), K - 1));
             if (!(r < thrNow)) continue;
             const int p = __popcll(__ballot(lane < K && myDist <= r));
+            const int sc = __builtin_amdgcn_readlane(s, c);
             // lane i takes lane i-1's entry: a DPP wave shift, no LDS round trip
             const float upD = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(myDist), __float_as_int(myDist),
                                                                          0x138, 0xF, 0xF, false));
@@ -249,7 +336,7 @@ __device__ __forceinline__ void cullTileBody(const GbufArgs& g, const FrameArgs&
                     myIdx = upI;
                 } else if (lane == p) {
                     myDist = r;
-                    myIdx = remap ? visible[1 + b + c] : b + c;
+                    myIdx = remap ? visible[1 + sc] : sc;
                 }
             }
             count = count + 1 < K ? count + 1 : K;
@@ -266,7 +353,7 @@ __global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, 
                                                       const int32_t* __restrict__ visible, int remap,
                                                       sgi_tile_stage1* __restrict__ stage1,
                                                       int32_t* __restrict__ tiles) {
-    __shared__ float comp[6][TS * TS];
+    __shared__ float comp[cullLdsFloats<TS>()];
     __shared__ float sums[6];
     cullTileBody<TS>(g, fr, (int)xcdSwizzle(blockIdx.x, (uint32_t)nTiles), tilesX, K, vpls, visible, remap, stage1,
                      tiles, comp, sums);
@@ -276,7 +363,7 @@ __global__ __launch_bounds__(64) void cullTilesKernel(GbufArgs g, FrameArgs fr, 
 template <int TS>
 __global__ __launch_bounds__(64) void cullTilesViewsKernel(const sgi_batch::ViewDesc* __restrict__ views,
                                                            int numViews, int totalTiles, int K) {
-    __shared__ float comp[6][TS * TS];
+    __shared__ float comp[cullLdsFloats<TS>()];
     __shared__ float sums[6];
     const int gt = (int)xcdSwizzle(blockIdx.x, (uint32_t)totalTiles);
     const sgi_batch::ViewDesc& d = views[sgi_batch::findView<sgi_batch::kTiles>(views, numViews, gt)];
