@@ -183,7 +183,8 @@ def culling_roofline(per_plain, W, H, tiles, K, visible, num_vpls):
     out["cull_global"]["kernels"] = "cullVisibilityKernel + cullCompactKernel"
     out["cull_tiles"]["kernels"] = "cullTilesKernel<16>"
     out["note"] = ("neither stage is HBM-bound: the global cull is two launches of 64 and 16 waves over 4096 VPLs "
-                   "(latency), the tile cull one wave per tile against every visible VPL (VALU; "
+                   "(latency), the tile cull one wave per tile against every visible VPL (VALU for the estimated "
+                   "ratios and the exact ones below the histogram bound, and the serial stage-1 sums; "
                    "profiles/valu_counters_gather_samples_city1080p.md)")
     return out
 
@@ -1059,9 +1060,10 @@ def main():
                     "traffic": load_traffic(f"gather_samples{S}_city1080p_traffic.json"),
                     "algorithmic_bytes": alg, "draws": W * H * S, "avg_launch_ms": g_ms,
                     "valu": valu_estimate(f"gather_samples{S}_city1080p_valu.json", g_ms, S),
-                    "note": ("not HBM-bound: the kernel is compute and memory-latency bound (per accepted draw a "
-                             "VPL record, then its cube texel, then the shadow test and the BRDF); the HBM "
-                             "fraction is low by construction, the VALU issue estimate is beside it")}
+                    "note": ("not HBM-bound: VALU issue (estimate beside it) and the vector memory pipeline "
+                             "filling L1 lines for random reads (a binary16 VPL position per draw, a VPL record "
+                             "and its cube texel per accepted draw) bound it; the HBM fraction is low by "
+                             "construction")}
         conv_roof = conv_roofline(wts, H, W)
     elif plain_res is not None:
         value, el, form = plain_res["frames_per_s"], el_plain, "tile-list plain form only"
