@@ -190,3 +190,40 @@ def test_gather_ex_modes(case, ora, gi, mode, acc64, rej):
     assert e <= REL_L2
     assert orc.rel_l2(color.cpu().numpy(), oc) <= REL_L2
     assert dw < 1e-3
+
+
+def test_sampled_gather_outside_binary16(case, ora, gi):
+    """The compacting gather decides each draw's side / radius test from the VPL's position and radius
+    rounded to binary16 wherever the rounding cannot change it, and otherwise from the FP32 record. Four
+    visible VPLs are moved to where binary16 fails: one coordinate at 7e4 and one at -7e4 (beyond its
+    largest finite value, 65504: the rounded position is infinite), a radius of 1e5 (infinite when
+    rounded) and a radius of -1 (a ratio test every draw fails). Against the oracle on the same records,
+    16 draws per pixel with failing draws skipped: within 1e-4 rel-L2, every draw's decision equal
+    (the weight channel)."""
+    from stratusgfx_amd import _abi
+    torch = require_gpu()
+    upd_h = ora["upd"].copy()
+    count = int(ora["vis"][0])
+    assert count >= 4
+    upd_h[0, 0] = 7.0e4
+    upd_h[1, 2] = -7.0e4
+    upd_h[2, 12] = 1.0e5
+    upd_h[3, 12] = -1.0
+    upd = torch.from_numpy(upd_h).cuda()
+    vis = torch.from_numpy(ora["vis"]).cuda()
+    ent = torch.from_numpy(np.ascontiguousarray(ora["ent"])).cuda()
+    W, H = case.w.width, case.w.height
+    color = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    res = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    gi.gather_ex(case.dev.gbuffer(), upd, vis, ent, case.dev.cubes(), None, color, res, mode=_abi.GATHER_SAMPLES,
+                 samples=16, rejection_limit=-1)
+    torch.cuda.synchronize()
+    h = case.host
+    oc, orr = orc.gather(h.frame, case.hg, upd_h, count, ora["ent"], case.hcubes, mode=orc.GATHER_SAMPLES,
+                         precision=0, samples=16, rejection_limit=-1)
+    got = res.cpu().numpy()
+    e = orc.rel_l2(got, orr)
+    dw = float(np.abs(got[..., 3].astype(np.float64) - orr[..., 3]).max())
+    print(f"sampled gather with VPLs outside binary16: rel-L2 {e:.3e}, max weight difference {dw:.2e}")
+    assert e <= REL_L2 and orc.rel_l2(color.cpu().numpy(), oc) <= REL_L2
+    assert dw < 1e-3
