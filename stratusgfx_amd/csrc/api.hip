@@ -690,13 +690,13 @@ namespace {
 // The inference workspace: per level l the activation plane e_l and, below
 // level 0, a temporary for c_l / d_l and the pooled input p_l; then the
 // features, the radiance pyramid, the reconstruction of every level, the
-// error logit and the tile flags.
+// tile flags and the list of the tiles that fall back (the level-0 head forms
+// the flags from its error logit directly, so no error plane is stored).
 struct KpcnWs {
     uint16_t* act[SGI_KPCN_LEVELS][3];  // [l][0] e_l, [l][1] c_l / d_l, [l][2] p_l (l >= 1)
     uint16_t* feat;
     float* rad[SGI_KPCN_LEVELS];  // the radiance pyramid (level 0 = the noisy radiance)
     float* o[SGI_KPCN_LEVELS];    // the reconstruction per level (level 0 = the output)
-    float* err;
     uint8_t* flags;
     int32_t* dilatedList;  // [1 + tiles]: count, then tile indices
     int h[SGI_KPCN_LEVELS], w[SGI_KPCN_LEVELS];
@@ -732,7 +732,6 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
         h = (h + 1) / 2;
         w = (w + 1) / 2;
     }
-    char* err = take(P * 4);
     char* flags = take(tiles);
     char* dilList = take((1 + tiles) * 4);
     if (ws) {
@@ -744,7 +743,6 @@ size_t kpcnLayout(int H, int W, char* base, KpcnWs* ws) {
             off += (size_t)ws->h[l] * ws->w[l] * 3;
         }
         for (int l = 0; l < SGI_KPCN_LEVELS; ++l) ws->o[l] = (float*)outs[l];
-        ws->err = (float*)err;
         ws->flags = (uint8_t*)flags;
         ws->dilatedList = (int32_t*)dilList;
     }
@@ -814,16 +812,19 @@ hipError_t denoiseLearned(const sgi_dev::GbufArgs& c, const sgi_dev::GbufArgs& p
         KPCN_STEP(sgi_kpcn::convApply(ws.act[l][1], ws.h[l], ws.w[l], Wt(head(l)), Kt(head(l)), ws.rad[l],
                                       ws.o[l + 1], ws.o[l], nullptr, f8, s));
     }
+    const bool noFallback = std::isinf(threshold) && threshold > 0.0f;
+    // with a threshold the head also forms the per-tile fallback flags
     KPCN_STEP(sgi_kpcn::convUpAddApply(ws.act[0][0], low, H, W, Wt(head(0)), Kt(head(0)), ws.rad[0], ws.o[1], ws.o[0],
-                                       ws.err, f8, s));
-    if (std::isinf(threshold) && threshold > 0.0f) {
+                                       nullptr, f8, s, threshold, noFallback ? nullptr : ws.flags,
+                                       ws.dilatedList, fallbackTiles));
+    if (noFallback) {
         // no threshold: no tile can fall back; the temporal pass reads the network's output
         if (fallbackTiles) KPCN_STEP(sgi_kpcn::clearCounts(H, W, fallbackTiles, s));
         if (screenReady) KPCN_STEP(hipStreamWaitEvent(s, screenReady, 0));
         KPCN_STEP(sgi_launch::denoiseTemporalNet(c, p, fr, screen, color, ws.o[0], prevGi, prevHistory,
                                                  combined, giOut, reservoirOut, historyOut, nullptr, nullptr, s));
     } else {
-        KPCN_STEP(sgi_kpcn::tileFlags(ws.err, H, W, threshold, ws.flags, ws.dilatedList, fallbackTiles, s));
+        KPCN_STEP(sgi_kpcn::dilateFlags(H, W, ws.flags, ws.dilatedList, fallbackTiles, s));
         // the a-trous passes over the flagged tiles and their margin (on a tile list: their cost
         // follows the flagged tiles), then one temporal pass: the a-trous result in the flagged
         // tiles, the network's output in every other tile (which also overwrites the margin's

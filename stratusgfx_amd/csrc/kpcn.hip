@@ -301,6 +301,13 @@ struct ConvSrc {
     uint16_t* pooled;       // OUT_BF16_POOL
     const float* radiance;  // OUT_APPLY: this level's radiance
     const float* coarse;    // OUT_APPLY: the coarser level's output, or NULL
+    // level-0 head (IN_UPADD, OUT_APPLY) with the error output: when flags is set, the epilogue
+    // also forms each tile's fallback flag (flags[tile]; flagCounts[1 + tile] when set; tile 0
+    // resets dilatedList[0] and flagCounts[0]) against flagThreshold
+    uint8_t* flags;
+    int32_t* dilatedList;
+    int32_t* flagCounts;
+    float flagThreshold;
 };
 
 __device__ __forceinline__ uint32_t addBf16x2(uint32_t a, uint32_t b) {
@@ -601,6 +608,8 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
     __shared__ uint4 upS[IN == IN_UPADD && OUT == OUT_APPLY ? UpStage<AIN>::kSlots * 256 : 1];
     constexpr bool kWLds = weightsInLds<IN, AIN>();
     __shared__ uint4 wS[kWLds ? 9 * 2 * 64 : 1];  // fragment (t, mb) of lane l at (2 t + mb) 64 + l
+    constexpr bool kFlags = IN == IN_UPADD && OUT == OUT_APPLY;
+    __shared__ float flagS[kFlags ? 256 : 1];  // the tile's exp(min(err, 80)), pixel by pixel
     uint4* halo = smem;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lg = lane >> 4;
@@ -645,6 +654,31 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
         if constexpr (kLdsUp) stageUp<AIN>(src, H, W, t0, tilesX, upS);
         if constexpr (OUT == OUT_APPLY) loadApply(src.radiance, src.coarse, H, W, t0, tilesX, apl);
     }
+    // the fallback flag of a finished tile: the mean of exp(min(err, 80)) over its in-image
+    // pixels (terms in flagS, 0 outside the image) against the threshold (every tile when the
+    // threshold is <= 0), summed as a fixed binary tree over the 256 pixels (pixel i + 128 into i,
+    // then i + 64 into i, ... 1): lane l of wave 0 adds pixels l, l + 64, l + 128, l + 192 in
+    // registers and the last six levels through shuffles; tile 0 also resets the counters
+    // dilateListKernel accumulates into
+    int prevTile = -1;
+    auto flagTile = [&](int ft) {
+        if (wave != 0) return;
+        float sum = (flagS[lane] + flagS[lane + 128]) + (flagS[lane + 64] + flagS[lane + 192]);
+#pragma unroll
+        for (int sft = 32; sft > 0; sft >>= 1) sum = sum + __shfl_down(sum, sft);
+        if (lane == 0) {
+            const int fx = ft % tilesX, fy = ft / tilesX;
+            const int n = (min(W, fx * kTS + kTS) - fx * kTS) * (min(H, fy * kTS + kTS) - fy * kTS);
+            const float thr = src.flagThreshold;
+            const uint8_t f = (!(thr > 0.0f) || sum / (float)n > thr) ? 1 : 0;
+            src.flags[ft] = f;
+            if (src.flagCounts) src.flagCounts[1 + ft] = f;
+            if (ft == 0) {
+                src.dilatedList[0] = 0;
+                if (src.flagCounts) src.flagCounts[0] = 0;
+            }
+        }
+    };
     for (int tile = t0; tile < t1; ++tile) {
         const int tx = tile % tilesX, ty = tile / tilesX;
         // kLdsUp: this thread's staging DMA has landed before the barrier
@@ -673,6 +707,10 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
             }
         }
         if constexpr (OUT == OUT_APPLY) storeApply(apl, src.coarse != nullptr, radS, coS);
+        // the previous tile's terms were stored before this tile's first barrier and are next
+        // written after its second
+        if constexpr (kFlags)
+            if (src.flags && prevTile >= 0) flagTile(prevTile);
         __syncthreads();
         if (tile + 1 < t1) {
             loadHalo<AIN, kHaloIn, kDefer, !kLdsUp && IN == IN_PLAIN && AIN == ACT_BF16>(src, H, W, tile + 1, tilesX,
@@ -741,6 +779,7 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
             // before the tile's second barrier)
             asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
             const int x = tx * kTS + (int)threadIdx.x % kTS, y = ty * kTS + (int)threadIdx.x / kTS;
+            float errV = 0.0f;
             if (x < W && y < H) {
                 float lgv[kLogitStride];
 #pragma unroll
@@ -759,6 +798,13 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                 dst[1] = o.y;
                 dst[2] = o.z;
                 if (err) err[p] = lgv[SGI_KPCN_ERR_CHANNEL];
+                errV = lgv[SGI_KPCN_ERR_CHANNEL];
+            }
+            if constexpr (kFlags) {
+                // this pixel's term of the tile's fallback flag, 0 outside the image (reduced by
+                // wave 0 at the next tile's start, or after the loop)
+                if (src.flags) flagS[threadIdx.x] = x < W && y < H ? expf(gmin(errV, 80.0f)) : 0.0f;
+                prevTile = tile;
             }
         } else {
             constexpr bool kBf16 = OUT == OUT_BF16 || OUT == OUT_BF16_POOL;
@@ -842,6 +888,12 @@ __global__ __launch_bounds__(256, (convBlocksPerCu<IN, OUT, AIN>())) void conv3x
                     }
                 }
             }
+        }
+    }
+    if constexpr (kFlags) {
+        if (src.flags && prevTile >= 0) {  // the block's last tile
+            __syncthreads();
+            flagTile(prevTile);
         }
     }
 }
@@ -1353,45 +1405,11 @@ __global__ void weightCopiesKernel(const float* __restrict__ params, uint16_t* _
     if (wt) wt[(size_t)l * kW + ((8 - t) * kC + in) * kC + o] = f2bf(val);
 }
 
-// Per 16x16 tile: flag = mean exp(err) > threshold (every tile when threshold <= 0).
-// One wave per 16x16 tile: the mean of exp(min(err, 80)) over the tile's
-// in-image pixels against the threshold. The sum is the binary tree over the
-// tile's 256 pixels (pixel i + 128 into i, then i + 64 into i, ... 1): lane l
-// holds pixels l, l + 64, l + 128 and l + 192 and forms the first two levels
-// in registers, the last six through shuffles.
-__global__ __launch_bounds__(256) void tileFlagKernel(const float* __restrict__ err, int H, int W, int tilesX,
-                                                      int nTiles, float threshold, uint8_t* __restrict__ flags,
-                                                      int32_t* __restrict__ dilatedList, int32_t* __restrict__ counts) {
-    const int t = blockIdx.x * 4 + ((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
-    if (t >= nTiles) return;
-    const int tx = t % tilesX, ty = t / tilesX;
-    float v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int i = lane + 64 * k;
-        const int x = tx * 16 + i % 16, y = ty * 16 + i / 16;
-        v[k] = x < W && y < H ? expf(gmin(err[(size_t)y * W + x], 80.0f)) : 0.0f;
-    }
-    float sum = (v[0] + v[2]) + (v[1] + v[3]);
-#pragma unroll
-    for (int sft = 32; sft > 0; sft >>= 1) sum = sum + __shfl_down(sum, sft);
-    if (lane == 0) {
-        const int n = (min(W, tx * 16 + 16) - tx * 16) * (min(H, ty * 16 + 16) - ty * 16);
-        const uint8_t f = (!(threshold > 0.0f) || sum / (float)n > threshold) ? 1 : 0;
-        flags[t] = f;
-        if (counts) counts[1 + t] = f;
-        if (t == 0) {  // the counters dilateListKernel accumulates into
-            dilatedList[0] = 0;
-            if (counts) counts[0] = 0;
-        }
-    }
-}
-
 // Per tile (one thread each): whether a flagged tile lies within two tiles,
 // and if so the tile's index appended to dilatedList (one atomic per wave;
 // the list's order does not matter, every listed tile is processed on its
 // own); counts[0] (may be NULL) gathers the flagged tiles.  Both counters
-// start at 0 (tileFlagKernel's block 0 clears them).
+// start at 0 (the level-0 head's tile 0 clears them).
 __global__ __launch_bounds__(256) void dilateListKernel(const uint8_t* __restrict__ flags, int tilesX, int tilesY,
                                                         int32_t* __restrict__ dilatedList,
                                                         int32_t* __restrict__ counts) {
@@ -1478,10 +1496,14 @@ hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t
 
 hipError_t convUpAddApply(const void* skip, const void* low, int H, int W, const void* w, const float* bias,
                           const float* radiance, const float* coarse, float* out, float* err, int f8,
-                          hipStream_t s) {
+                          hipStream_t s, float threshold, uint8_t* flags, int32_t* dilatedList, int32_t* counts) {
     ConvSrc src = convSrc(skip, low);
     src.radiance = radiance;
     src.coarse = coarse;
+    src.flags = flags;
+    src.dilatedList = dilatedList;
+    src.flagCounts = counts;
+    src.flagThreshold = threshold;
     return launchConvAct<IN_UPADD, OUT_APPLY>(f8, src, w, bias, 0, H, W, out, err, s);
 }
 
@@ -1632,13 +1654,8 @@ hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* bi
     return hipGetLastError();
 }
 
-hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, int32_t* dilatedList,
-                     int32_t* counts, hipStream_t s) {
+hipError_t dilateFlags(int H, int W, const uint8_t* flags, int32_t* dilatedList, int32_t* counts, hipStream_t s) {
     const int tilesX = (W + 15) / 16, tilesY = (H + 15) / 16, n = tilesX * tilesY;
-    hipLaunchKernelGGL(tileFlagKernel, dim3(gridFor(n, 4)), dim3(256), 0, s, err, H, W, tilesX, n, threshold, flags,
-                       dilatedList, counts);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(dilateListKernel, dim3(gridFor(n, 256)), dim3(256), 0, s, flags, tilesX, tilesY, dilatedList,
                        counts);
     return hipGetLastError();

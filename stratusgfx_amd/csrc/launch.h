@@ -152,10 +152,13 @@ hipError_t conv3x3In16AndPool(const uint16_t* in16, int H, int W, const uint16_t
 // The remaining forms take f8: 0 for BF16 activations and weights with a bias,
 // 1 for FP8 E4M3 activations and weights with the epilogue [k1[32], k2[32]].
 // The level-0 head: its input e0 + up(d1) staged as convUpAdd stages it, its
-// epilogue the reconstruction step as convApply's.
+// epilogue the reconstruction step as convApply's. With flags set (err too) the
+// epilogue also writes the per-tile fallback flags (flag = mean exp(min(err, 80))
+// over the tile > threshold), the counts record and the list counter's reset.
 hipError_t convUpAddApply(const void* skip, const void* low, int H, int W, const void* w, const float* bias,
                           const float* radiance, const float* coarse, float* out, float* err, int f8,
-                          hipStream_t s);
+                          hipStream_t s, float threshold = 0.0f, uint8_t* flags = nullptr,
+                          int32_t* dilatedList = nullptr, int32_t* counts = nullptr);
 hipError_t convAndPool(const void* in, int H, int W, const void* w, const float* bias, void* out, void* pooled,
                        int f8, hipStream_t s);
 hipError_t convUpAdd(const void* skip, const void* low, int H, int W, const void* w, const float* bias, void* out,
@@ -191,13 +194,12 @@ hipError_t loss(const float* out, const float* logits, const float* target, cons
 hipError_t adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
                 int step, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
 hipError_t packWeights(const float* params, uint16_t* w, uint16_t* wt, float* biases, hipStream_t s);
-// Per 16x16 tile: flags (mean exp(err) > threshold, or every tile when
-// threshold <= 0), the list of the flagged tiles grown by two tiles (the
-// a-trous passes' 28-pixel reach; int32 [1 + tiles]: count, then the tile
-// indices in no particular order) and, when counts is set, the sgi_denoise_learned
+// From the per-tile flags the level-0 head formed (convUpAddApply with flags): the
+// list of the flagged tiles grown by two tiles (the a-trous passes' 28-pixel reach;
+// int32 [1 + tiles]: count, then the tile indices in no particular order) and, when
+// counts is set, the flagged-tile count completing the sgi_denoise_learned
 // fallbackTiles record (count, then one flag per tile).
-hipError_t tileFlags(const float* err, int H, int W, float threshold, uint8_t* flags, int32_t* dilatedList,
-                     int32_t* counts, hipStream_t s);
+hipError_t dilateFlags(int H, int W, const uint8_t* flags, int32_t* dilatedList, int32_t* counts, hipStream_t s);
 // counts[0] = 0 and counts[1 + t] = 0 for every tile (no tile falls back).
 hipError_t clearCounts(int H, int W, int32_t* counts, hipStream_t s);
 
