@@ -1,5 +1,6 @@
 """Edge cases on the GPU: partial tiles, tile sizes 8 / 32, K = 1 and K = 64,
-no VPLs, no visible VPL, more visible VPLs than the per-frame cap, history."""
+tile culling over 4083 visible VPLs on every path of its stage 2, no VPLs, no
+visible VPL, more visible VPLs than the per-frame cap, history."""
 import numpy as np
 import pytest
 
@@ -50,6 +51,38 @@ def test_tiles_and_gather_tile_shapes(case, ts, K):
                          K=K, mode=orc.GATHER_TILES)
     assert orc.rel_l2(res.cpu().numpy(), orr) <= REL_L2
     assert orc.rel_l2(col.cpu().numpy(), oc) <= REL_L2
+
+
+@pytest.fixture(scope="module")
+def city():
+    return Case(small(sc.CITY_1080P, name="city-320x180-4096vpl", width=320, height=180, num_vpls=4096,
+                      cube_res=32, csm_res=512))
+
+
+@pytest.mark.parametrize("radius", [None, 1.0e6, 30.0, 10.0, 1.0])
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_tile_lists_many_vpls(city, ts, radius):
+    """Tile culling with 4083 visible VPLs and K = 64, against the oracle bit for bit, over the three
+    paths of its stage 2: the radii as generated (a histogram of estimated ratios bounds the 64th
+    smallest and only the slots below the bound are scanned exactly), every radius 1e6 (every ratio
+    in the histogram's first bucket: more slots below the bound than the compacted list holds for 8
+    and 16 pixel tiles, so every slot is scanned) and every radius 30, 10 or 1 (fewer than 64
+    candidates, down to none: no bound)."""
+    torch = require_gpu()
+    K = 64
+    gi = _ctx(city, ts, K)
+    upd, vis, ent = city.oracle_cull()
+    upd = upd.copy()
+    if radius is not None:
+        upd[:, 12] = radius
+    _, o_tiles = orc.cull_tiles(city.host.frame, city.hg, ts, K, upd, vis)
+    tiles = torch.zeros((gi.num_tiles, 1 + K), dtype=torch.int32, device="cuda")
+    gi.cull_tiles(city.dev.gbuffer(), torch.from_numpy(upd).cuda(), torch.from_numpy(vis).cuda(), tiles)
+    torch.cuda.synchronize()
+    got = tiles.cpu().numpy()
+    print(f"[tiles {ts}, radius {radius}] {int(vis[0])} visible, list lengths mean {got[:, 0].mean():.1f}, "
+          f"{int(((got[:, 0] > 0) & (got[:, 0] < K)).sum())} of {len(got)} tiles between 1 and {K - 1}", flush=True)
+    assert_tiles_equal(got, o_tiles)
 
 
 def test_no_vpls(case):
